@@ -1,0 +1,77 @@
+"""Writes the golden fixtures in tests/golden/ from the reference engine.
+
+The reference's dense CPU path is compiled from its own sources by
+oracle/ref.mk into oracle/_ref/ (never copied into this repository); the
+harness oracle/ref_harness.cpp drives it through the reference API and this
+script stores what it wrote.  Run from the repository root, in a container
+where the reference sources are mounted:
+
+    python tests/golden/make_golden.py [--ref /root/reference]
+
+Fixtures (all JSON; doubles are C99 hex floats, exact):
+  pcg32.json            pcg32 / pcg wrapper sequences for six seeds
+  gates_n5.json         every gate API case at n=5, with and without gc_mask
+  qms_*.json            per-step metro_step return codes, pcg32 state after
+                        each step, E/X measurements, reverse counters
+  qms_c1_out.txt        the stock reference `qms` output file (E A lines)
+                        for beta=1, 1000 steps, r=4, seed 42, E in [-1,3]
+  evo_n3.txt            the stock reference `evo` output, 20 steps, dt=0.1
+  circuit_*.json        the random-circuit family (gate list, state before
+                        the measurements, outcomes, final state)
+"""
+import argparse
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+# (name, beta, steps, reset_each, r, seed, ene_min, ene_max, pe, therm, max_rev, record_rev)
+QMS_CASES = [
+    ("qms_c1", 1.0, 1000, 1, 4, 42, -1.0, 3.0, 10, 100, 100, 1),
+    ("qms_r3_b05", 0.5, 300, 2, 3, 7, 0.0, 1.0, 10, 20, 100, 1),
+    ("qms_r3_maxrev1", 2.0, 300, 1, 3, 123, -1.0, 3.0, 10, 10, 1, 1),
+    ("qms_r5_b3", 3.0, 200, 1, 5, 99, -1.0, 3.0, 10, 10, 100, 0),
+    ("qms_r6_b01", 0.1, 60, 3, 6, 5, -1.0, 3.0, 10, 5, 100, 1),
+]
+
+CIRCUITS = [("circuit_n10", 10, 200, 7), ("circuit_n6", 6, 20, 3)]
+
+
+def run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, **kw)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    a = ap.parse_args()
+    run(["make", "-s", "-f", "oracle/ref.mk", "REF=" + a.ref, "-j8"], cwd=ROOT)
+    h = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    out = lambda n: os.path.join(HERE, n)
+    run([h, "pcg", out("pcg32.json")], stdout=subprocess.DEVNULL)
+    run([h, "gates", out("gates_n5.json")], stdout=subprocess.DEVNULL)
+    for (name, beta, steps, re_, r, seed, emin, emax, pe, th, mr, rr) in QMS_CASES:
+        run([h, "qms", repr(beta), str(steps), str(re_), str(r), str(seed), repr(emin),
+             repr(emax), str(pe), str(th), str(mr), str(rr), out(name + ".json")],
+            stdout=subprocess.DEVNULL)
+    for (name, n, layers, seed) in CIRCUITS:
+        run([h, "circuit", str(n), str(layers), str(seed), out(name + ".json")],
+            stdout=subprocess.DEVNULL)
+    # stock drivers: output files, byte for byte
+    tmp = out("qms_c1_out.txt")
+    if os.path.exists(tmp):
+        os.remove(tmp)
+    run([os.path.join(ROOT, "oracle", "_ref", "qms"), "1.0", "1000", "1", "4", tmp, "--seed", "42",
+         "--ene-min", "-1", "--ene-max", "3", "--PE-steps", "10"], stdout=subprocess.DEVNULL)
+    tmp = out("evo_n3.txt")
+    if os.path.exists(tmp):
+        os.remove(tmp)
+    run([os.path.join(ROOT, "oracle", "_ref", "evo"), "20", "0.1", tmp], stdout=subprocess.DEVNULL)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
