@@ -1,0 +1,153 @@
+"""Benchmark: plain FP64 gate sweeps of the 30-qubit random-circuit workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 30] [--no-cpu]
+
+Workload: 30 qubits, complex FP64 (16 GiB in HBM), |0> then an H layer, then
+layers of 30 random gates from {H, X, RX, RZ, U1, CX, MCX(3)} drawn with
+pcg32 seed 7 (suqa_amd/workloads.py).  One step = one layer (30 gates, one
+HIP sweep each).  W warmup layers are applied untimed, then exactly K layers
+are timed between stream synchronizations; the default W=2, K=198 applies
+all 200 layers.  The 8 final measure_qbit calls run after the timed region.
+
+value = algorithmic bytes of the timed layers / wall time (GB/s); a gate's
+algorithmic bytes are 32 B per amplitude it modifies (H/X/RX/RZ 32N,
+U1/CX 16N, MCX3 4N).  roofline: the dominant kernel family, its algorithmic
+bytes over the sum of its per-launch HIP-event durations, against the
+8 TB/s HBM3E peak.  cpu_baseline: the oracle (a single-threaded restatement
+of the reference CPU path) on the first layer at 28 qubits.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def hbm_copy_calibration(capi, eng, nbytes=4 << 30, reps=5):
+    """Measured copy bandwidth (read + write bytes / time) of a plain copy kernel."""
+    import ctypes as C
+    a, b = C.c_void_p(), C.c_void_p()
+    capi.check(capi.lib().suqa_hip_device_alloc(nbytes, C.byref(a)))
+    capi.check(capi.lib().suqa_hip_device_alloc(nbytes, C.byref(b)))
+    try:
+        eng.timing(True)
+        eng.timing_reset()
+        for _ in range(reps):
+            capi.check(capi.lib().suqa_hip_copy_kernel(eng.ctx, b, a, nbytes))
+        rep = eng.timing_report()["copy"]
+        eng.timing(False)
+        eng.timing_reset()
+        return rep["bytes"] / (rep["ms"] * 1e-3) / 1e9
+    finally:
+        capi.lib().suqa_hip_device_free(a)
+        capi.lib().suqa_hip_device_free(b)
+
+
+def cpu_baseline(n=28):
+    """Oracle (reference CPU path restated, 1 thread) on layer 0 of the same circuit at n qubits."""
+    from oracle import pyoracle as po
+    from suqa_amd import workloads as wl
+    gates, _, _ = wl.random_circuit(n, 1, 7)
+    og, ng, _, _ = po.circuit(n, 1, 7)
+    st = po.State(n)
+    for q in range(n):
+        st.h(q)
+    t0 = time.perf_counter()
+    for g in og[:ng]:
+        st.apply_gate(g)
+    dt = time.perf_counter() - t0
+    nbytes = sum(wl.gate_bytes(g, n) for g in gates)
+    del st
+    return dict(value=nbytes / dt / 1e9, unit="GB/s", cores=1, kind="port",
+                sample=f"first layer (30 gates) of the seed-7 circuit at {n} qubits, "
+                       f"{dt:.1f} s on 1 host core (oracle/suqa_oracle.c, gcc -O2)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=198)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU sharding is not part of "
+              f"this build", file=sys.stderr)
+        sys.exit(2)
+    from suqa_amd import capi, workloads as wl
+    n = a.n
+    layers = a.warmup + a.steps
+    gates, mq, mr = wl.random_circuit(n, max(layers, 1), 7)
+    arr = capi.gates_from(gates)
+    log(f"bench: n={n} warmup={a.warmup} steps={a.steps}")
+    eng = capi.Engine(n)
+    peak_copy = hbm_copy_calibration(capi, eng)
+    log(f"copy-kernel bandwidth {peak_copy:.0f} GB/s")
+    for q in range(n):
+        eng.h(q)
+    eng.synchronize()
+    per = n
+    for s in range(a.warmup):
+        capi.check(capi.lib().suqa_hip_run_gates(eng.ctx, capi.C.byref(arr, s * per * capi.C.sizeof(capi.Gate)), per))
+    eng.synchronize()
+    eng.timing(True)
+    eng.timing_reset()
+    t0 = time.perf_counter()
+    for s in range(a.warmup, layers):
+        capi.check(capi.lib().suqa_hip_run_gates(eng.ctx, capi.C.byref(arr, s * per * capi.C.sizeof(capi.Gate)), per))
+        if (s - a.warmup) % 50 == 49:
+            log(f"  layer {s + 1}/{layers}")
+    eng.synchronize()
+    t1 = time.perf_counter()
+    rep = eng.timing_report()
+    eng.timing(False)
+    wall = t1 - t0
+    alg = sum(wl.gate_bytes(g, n) for g in gates[a.warmup * per:layers * per])
+    value = alg / wall / 1e9
+    fam = max((k for k in rep if k not in ("norm_final",)), key=lambda k: rep[k]["ms"])
+    achieved = rep[fam]["bytes"] / (rep[fam]["ms"] * 1e-3) / 1e9
+    # the final measurements of the workload (not in the timed region)
+    tm = time.perf_counter()
+    outcomes = [eng.measure(q, r) for q, r in zip(mq, mr)]
+    tm = time.perf_counter() - tm
+    norm = eng.vnorm()
+    eng.close()
+    cpu = None if a.no_cpu else cpu_baseline(min(n, 28))
+    fams = {k: dict(launches=v["launches"], ms=round(v["ms"], 3),
+                    gbs=round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)) for k, v in rep.items()}
+    out = {
+        "metric": "QMS Metropolis steps/s at fixed FP64 error; gate-sweep GB/s vs HBM roof",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / max(a.steps, 1) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: pcg32 seed-7 random circuit",
+        "config": {"workload": "plain FP64 gate sweep, random circuit {H,X,RX,RZ,U1,CX,MCX3}",
+                   "qubits": n, "gates_per_step": per, "layers_total": layers, "seed": 7},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": fam},
+        "cpu_baseline": cpu,
+        "extra": {"copy_kernel_gbs": round(peak_copy, 1), "kernel_families": fams,
+                  "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,177 @@
+/*
+ * suqa_hip.h -- C-ABI of the MI355X (gfx950) state-vector engine.
+ *
+ * This is the drop-in boundary below the reference's C++ `suqa::` API
+ * (include/suqa.cuh:94-176 in the reference).  The C++ mirror in
+ * suqa_amd/host/suqa.hpp keeps the reference signatures and calls these
+ * entry points; any FFI (ctypes, cgo, JNI) can bind them directly.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (SUQA_OK == 0).  A non-zero
+ *     status leaves a message in suqa_hip_last_error().  The C++ mirror turns
+ *     it into std::runtime_error, the reference's exception type.
+ *   - Plain pointers and sizes only.  Qubit q is bit q of the basis index.
+ *   - gc_mask is the reference's global control mask (suqa.cu:36): every
+ *     gate taking it acts only on basis states whose gc_mask bits are all 1.
+ *   - State layout (the de-facto ABI of complex_defines.cuh:22-45):
+ *     one device allocation of 2*N doubles, re = data, im = data + N.
+ *   - One HIP stream per context; calls on one context are serialized on it.
+ *     Entry points that return a value (vnorm, measure, prob_filter)
+ *     synchronize that stream.  Contexts are not thread-safe.
+ *   - Reductions are deterministic: a fixed two-level tree for a given nq.
+ */
+#ifndef SUQA_HIP_H
+#define SUQA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SUQA_OK               0
+#define SUQA_ERR_ARG         -1  /* invalid argument (reference: std::runtime_error) */
+#define SUQA_ERR_UNSUPPORTED -2  /* Pauli pattern the reference rejects (suqa.cu:555,563,579) */
+#define SUQA_ERR_HIP         -3  /* HIP runtime failure */
+#define SUQA_ERR_NOMEM       -4  /* device allocation failed */
+
+#define SUQA_PAULI_ID 0u         /* suqa.cuh:60-63 */
+#define SUQA_PAULI_X  1u
+#define SUQA_PAULI_Y  2u
+#define SUQA_PAULI_Z  3u
+
+typedef struct suqa_hip_ctx suqa_hip_ctx;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* suqa_hip_last_error(void);
+
+/* ---- lifecycle ------------------------------------------------------- */
+/* suqa::setup (suqa.cu:792-844): allocate 2^nq amplitudes on the current
+ * device and set |0...0>.  1 <= nq <= 32. */
+int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out);
+/* suqa::clear (suqa.cu:846-866): free everything owned by the context. */
+int suqa_hip_clear(suqa_hip_ctx* ctx);
+/* suqa::allocate_state (suqa.cu:772-788): resize to 2^nq and set |0...0>. */
+int suqa_hip_allocate_state(suqa_hip_ctx* ctx, uint32_t nq);
+/* suqa::init_state (suqa.cu:70-76). */
+int suqa_hip_init_state(suqa_hip_ctx* ctx);
+/* suqa::vnorm (suqa.cu:78-92): sqrt(sum |psi_i|^2). */
+int suqa_hip_vnorm(suqa_hip_ctx* ctx, double* norm);
+/* suqa::vnormalize (suqa.cu:97-111): multiply by 1./vnorm(). */
+int suqa_hip_vnormalize(suqa_hip_ctx* ctx);
+
+/* ---- gates (each one pass over the touched amplitudes) ---------------- */
+/* suqa::apply_x(uint) (suqa.cu:115-127) */
+int suqa_hip_apply_x(suqa_hip_ctx* ctx, uint32_t q, uint32_t gc_mask);
+/* suqa::apply_y(uint) (suqa.cu:136-148) */
+int suqa_hip_apply_y(suqa_hip_ctx* ctx, uint32_t q, uint32_t gc_mask);
+/* suqa::apply_h(uint) (suqa.cu:207-218) */
+int suqa_hip_apply_h(suqa_hip_ctx* ctx, uint32_t q, uint32_t gc_mask);
+/* suqa::apply_u1(uint,uint,double) (suqa.cu:271-285); apply_z/s/t/tdg
+ * (suqa.cu:156-262) are u1 with pi, pi/2, pi/4, -pi/4.  q_mask selects the
+ * |1> (1) or |0> (0) component. */
+int suqa_hip_apply_u1(suqa_hip_ctx* ctx, uint32_t q, uint32_t q_mask, double phase,
+                      uint32_t gc_mask);
+/* suqa::apply_cx / apply_mcx (suqa.cu:289-344): X on target when control k
+ * reads ctrl_vals[k] (NULL: all 1). */
+int suqa_hip_apply_mcx(suqa_hip_ctx* ctx, const uint32_t* ctrls, const uint32_t* ctrl_vals,
+                       uint32_t nctrl, uint32_t target, uint32_t gc_mask);
+/* suqa::apply_cu1 / apply_mcu1 (suqa.cu:346-393). */
+int suqa_hip_apply_mcu1(suqa_hip_ctx* ctx, const uint32_t* ctrls, const uint32_t* ctrl_vals,
+                        uint32_t nctrl, uint32_t target, double phase, uint32_t gc_mask);
+/* suqa::apply_swap (suqa.cu:396-416). */
+int suqa_hip_apply_swap(suqa_hip_ctx* ctx, uint32_t q1, uint32_t q2, uint32_t gc_mask);
+/* suqa::apply_phase_list (suqa.cu:419-454, suqa.cuh:157): phases[k] on the
+ * states whose bits q0..q0+q_size-1 read k (documented intent: all gc_mask
+ * bits set; the reference GPU kernel instead tests any gc bit).
+ * 1 <= q_size <= 7, nphases == 2^q_size. */
+int suqa_hip_apply_phase_list(suqa_hip_ctx* ctx, uint32_t q0, uint32_t q_size,
+                              const double* phases, uint32_t nphases, uint32_t gc_mask);
+/* suqa::apply_pauli_TP_rotation (suqa.cu:467-604): the reference operator,
+ * bit for bit, for every pattern the reference accepts (weights 1, 2 and
+ * Z(XX|YY|ZZ)); SUQA_ERR_UNSUPPORTED where the reference throws. */
+int suqa_hip_apply_pauli_TP_rotation(suqa_hip_ctx* ctx, const uint32_t* qs, const uint32_t* types,
+                                     uint32_t n, double phase, uint32_t gc_mask);
+/* Extension (the reference's TODO at suqa.cu:459): exp(i*phase*P) for any
+ * Pauli string, P = prod_k sigma_{types[k]} on qs[k], in one pass. */
+int suqa_hip_apply_pauli_rotation_general(suqa_hip_ctx* ctx, const uint32_t* qs,
+                                          const uint32_t* types, uint32_t n, double phase,
+                                          uint32_t gc_mask);
+/* suqa::apply_qft / apply_qft_inverse (suqa.cu:635-661): no final swaps; the
+ * crm phase table is filled once per context by the first call
+ * (suqa.cu:621-625 does it once per process). */
+int suqa_hip_apply_qft(suqa_hip_ctx* ctx, const uint32_t* qs, uint32_t n, uint32_t gc_mask);
+int suqa_hip_apply_qft_inverse(suqa_hip_ctx* ctx, const uint32_t* qs, uint32_t n,
+                               uint32_t gc_mask);
+
+/* ---- measurement ------------------------------------------------------ */
+/* suqa::measure_qbit (suqa.cu:673-696): c = (rdoub < P(q=1)); collapse and
+ * renormalize.  One fused reduction + one fused collapse/scale pass. */
+int suqa_hip_measure_qbit(suqa_hip_ctx* ctx, uint32_t q, double rdoub, uint32_t* c);
+/* suqa::apply_reset(uint,double) (suqa.cu:730-738): measure, X if 1. */
+int suqa_hip_apply_reset(suqa_hip_ctx* ctx, uint32_t q, double rdoub, uint32_t gc_mask);
+/* suqa::prob_filter (suqa.cu:705-727): sum |psi_i|^2 over i whose bits qs[k]
+ * read q_mask[k]. */
+int suqa_hip_prob_filter(suqa_hip_ctx* ctx, const uint32_t* qs, const uint32_t* q_mask,
+                         uint32_t n, double* prob);
+
+/* ---- QMS -------------------------------------------------------------- */
+/* qms::apply_W (qms.cuh:220-329): acceptance rotation on ancilla q_acc (the
+ * most significant qubit) driven by the energy register read by
+ * (i & W_mask_Enew) >> bm_enes0.  fs1 = exp(-dE/2), fs2 = sqrt(1-fs1^2) are
+ * evaluated on the host with libm (as the reference CPU path does). */
+int suqa_hip_qms_apply_W(suqa_hip_ctx* ctx, uint32_t q_acc, uint32_t W_mask_Enew,
+                         uint32_t bm_enes0, double c_factor, uint32_t curr_E_old);
+
+/* ---- state access (ComplexVec view, suqa.cu:17) ----------------------- */
+int suqa_hip_state_info(suqa_hip_ctx* ctx, uint32_t* nq, uint64_t* N, double** dev_re,
+                        double** dev_im);
+int suqa_hip_download(suqa_hip_ctx* ctx, double* re, double* im);
+int suqa_hip_upload(suqa_hip_ctx* ctx, const double* re, const double* im);
+int suqa_hip_synchronize(suqa_hip_ctx* ctx);
+
+/* ---- workload runner and timing (extensions) -------------------------- */
+/* Gate kinds of the random-circuit workload. RX(a) = exp(-i a X/2),
+ * RZ(a) = exp(-i a Z/2), U1(a) = diag(1, e^{ia}). */
+#define SUQA_GATE_H   0
+#define SUQA_GATE_X   1
+#define SUQA_GATE_RX  2
+#define SUQA_GATE_RZ  3
+#define SUQA_GATE_U1  4
+#define SUQA_GATE_CX  5
+#define SUQA_GATE_MCX 6
+typedef struct {
+    int32_t  kind;
+    uint32_t target;
+    uint32_t nctrl;
+    uint32_t ctrl[3];
+    double   angle;
+} suqa_hip_gate;
+
+/* Applies gates[0..ngates) in order, asynchronously on the context stream. */
+int suqa_hip_run_gates(suqa_hip_ctx* ctx, const suqa_hip_gate* gates, uint32_t ngates);
+
+typedef struct {
+    char     name[32];     /* kernel family */
+    uint64_t launches;
+    double   total_ms;     /* sum of per-launch HIP-event durations */
+    double   alg_bytes;    /* sum of algorithmic HBM bytes */
+} suqa_hip_kstat;
+
+/* Per-launch HIP events around every kernel launch on the context stream. */
+int suqa_hip_timing_enable(suqa_hip_ctx* ctx, int on);
+int suqa_hip_timing_reset(suqa_hip_ctx* ctx);
+/* Synchronizes, then fills up to max entries; *n receives the count. */
+int suqa_hip_timing_report(suqa_hip_ctx* ctx, suqa_hip_kstat* out, uint32_t max, uint32_t* n);
+
+/* Device copy dst <- src of nbytes on the context stream (HBM calibration). */
+int suqa_hip_copy_kernel(suqa_hip_ctx* ctx, void* dst, const void* src, uint64_t nbytes);
+/* Device buffer helpers for the calibration above. */
+int suqa_hip_device_alloc(uint64_t nbytes, void** ptr);
+int suqa_hip_device_free(void* ptr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,294 @@
+"""ctypes binding of the C-ABI in include/suqa_hip.h (suqa_amd/lib/libsuqa_hip.so).
+
+`Engine` exposes the reference's suqa:: API names (suqa.cuh:94-176) over one
+GPU state; gc_mask is held on the Python side and passed to every gate, as
+the reference's global suqa::gc_mask is (suqa.cu:36, 45-61).  There is no
+CPU fallback: if the HIP library is missing or fails to load, importing
+this module raises.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsuqa_hip.so")
+
+U32 = C.c_uint32
+PU32 = C.POINTER(C.c_uint32)
+PD = C.POINTER(C.c_double)
+CTX = C.c_void_p
+
+SUQA_OK = 0
+SUQA_ERR_ARG = -1
+SUQA_ERR_UNSUPPORTED = -2
+
+GATE_H, GATE_X, GATE_RX, GATE_RZ, GATE_U1, GATE_CX, GATE_MCX = range(7)
+
+
+class Gate(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("target", U32), ("nctrl", U32), ("ctrl", U32 * 3), ("angle", C.c_double)]
+
+
+class KStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double),
+                ("alg_bytes", C.c_double)]
+
+
+# name -> (restype, argtypes); every entry point of include/suqa_hip.h
+SIGNATURES = {
+    "suqa_hip_last_error": (C.c_char_p, []),
+    "suqa_hip_setup": (C.c_int, [U32, C.POINTER(CTX)]),
+    "suqa_hip_clear": (C.c_int, [CTX]),
+    "suqa_hip_allocate_state": (C.c_int, [CTX, U32]),
+    "suqa_hip_init_state": (C.c_int, [CTX]),
+    "suqa_hip_vnorm": (C.c_int, [CTX, PD]),
+    "suqa_hip_vnormalize": (C.c_int, [CTX]),
+    "suqa_hip_apply_x": (C.c_int, [CTX, U32, U32]),
+    "suqa_hip_apply_y": (C.c_int, [CTX, U32, U32]),
+    "suqa_hip_apply_h": (C.c_int, [CTX, U32, U32]),
+    "suqa_hip_apply_u1": (C.c_int, [CTX, U32, U32, C.c_double, U32]),
+    "suqa_hip_apply_mcx": (C.c_int, [CTX, PU32, PU32, U32, U32, U32]),
+    "suqa_hip_apply_mcu1": (C.c_int, [CTX, PU32, PU32, U32, U32, C.c_double, U32]),
+    "suqa_hip_apply_swap": (C.c_int, [CTX, U32, U32, U32]),
+    "suqa_hip_apply_phase_list": (C.c_int, [CTX, U32, U32, PD, U32, U32]),
+    "suqa_hip_apply_pauli_TP_rotation": (C.c_int, [CTX, PU32, PU32, U32, C.c_double, U32]),
+    "suqa_hip_apply_pauli_rotation_general": (C.c_int, [CTX, PU32, PU32, U32, C.c_double, U32]),
+    "suqa_hip_apply_qft": (C.c_int, [CTX, PU32, U32, U32]),
+    "suqa_hip_apply_qft_inverse": (C.c_int, [CTX, PU32, U32, U32]),
+    "suqa_hip_measure_qbit": (C.c_int, [CTX, U32, C.c_double, PU32]),
+    "suqa_hip_apply_reset": (C.c_int, [CTX, U32, C.c_double, U32]),
+    "suqa_hip_prob_filter": (C.c_int, [CTX, PU32, PU32, U32, PD]),
+    "suqa_hip_qms_apply_W": (C.c_int, [CTX, U32, U32, U32, C.c_double, U32]),
+    "suqa_hip_state_info": (C.c_int, [CTX, PU32, C.POINTER(C.c_uint64), C.POINTER(PD), C.POINTER(PD)]),
+    "suqa_hip_download": (C.c_int, [CTX, PD, PD]),
+    "suqa_hip_upload": (C.c_int, [CTX, PD, PD]),
+    "suqa_hip_synchronize": (C.c_int, [CTX]),
+    "suqa_hip_run_gates": (C.c_int, [CTX, C.POINTER(Gate), U32]),
+    "suqa_hip_timing_enable": (C.c_int, [CTX, C.c_int]),
+    "suqa_hip_timing_reset": (C.c_int, [CTX]),
+    "suqa_hip_timing_report": (C.c_int, [CTX, C.POINTER(KStat), U32, PU32]),
+    "suqa_hip_copy_kernel": (C.c_int, [CTX, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "suqa_hip_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "suqa_hip_device_free": (C.c_int, [C.c_void_p]),
+}
+
+
+class SuqaError(RuntimeError):
+    """A non-zero status from the C-ABI (the reference raises std::runtime_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"suqa_hip error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise ImportError(f"HIP engine library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def _u32(xs):
+    return (U32 * max(1, len(xs)))(*[int(x) for x in xs])
+
+
+def check(rc):
+    if rc != SUQA_OK:
+        raise SuqaError(rc, lib().suqa_hip_last_error().decode())
+    return rc
+
+
+class Engine:
+    """One GPU state vector behind the reference's API names."""
+
+    def __init__(self, nq):
+        self.ctx = CTX()
+        check(lib().suqa_hip_setup(nq, C.byref(self.ctx)))
+        self.nq = nq
+        self.N = 1 << nq
+        self.gc_mask = 0
+
+    def close(self):
+        if self.ctx:
+            lib().suqa_hip_clear(self.ctx)
+            self.ctx = CTX()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # suqa.cu:45-61
+    def activate_gc_mask(self, qs):
+        for q in qs:
+            self.gc_mask |= 1 << q
+
+    def deactivate_gc_mask(self, qs):
+        for q in qs:
+            self.gc_mask &= ~(1 << q)
+
+    def allocate_state(self, nq):
+        check(lib().suqa_hip_allocate_state(self.ctx, nq))
+        self.nq, self.N = nq, 1 << nq
+
+    def init_state(self):
+        check(lib().suqa_hip_init_state(self.ctx))
+
+    def vnorm(self):
+        v = C.c_double()
+        check(lib().suqa_hip_vnorm(self.ctx, C.byref(v)))
+        return v.value
+
+    def vnormalize(self):
+        check(lib().suqa_hip_vnormalize(self.ctx))
+
+    def x(self, q):
+        check(lib().suqa_hip_apply_x(self.ctx, q, self.gc_mask))
+
+    def y(self, q):
+        check(lib().suqa_hip_apply_y(self.ctx, q, self.gc_mask))
+
+    def z(self, q):
+        self.u1(q, math.pi)
+
+    def s(self, q):
+        self.u1(q, math.pi / 2.0)
+
+    def t(self, q):
+        self.u1(q, math.pi / 4.0)
+
+    def tdg(self, q):
+        self.u1(q, -math.pi / 4.0)
+
+    def h(self, q):
+        check(lib().suqa_hip_apply_h(self.ctx, q, self.gc_mask))
+
+    def u1(self, q, phase, q_mask=1):
+        check(lib().suqa_hip_apply_u1(self.ctx, q, q_mask, phase, self.gc_mask))
+
+    def cx(self, qc, qt, q_mask=1):
+        check(lib().suqa_hip_apply_mcx(self.ctx, _u32([qc]), _u32([q_mask]), 1, qt, self.gc_mask))
+
+    def mcx(self, ctrls, qt, q_mask=None):
+        vals = None if q_mask is None else _u32(q_mask)
+        check(lib().suqa_hip_apply_mcx(self.ctx, _u32(ctrls), vals, len(ctrls), qt, self.gc_mask))
+
+    def cu1(self, qc, qt, phase, q_mask=1):
+        check(lib().suqa_hip_apply_mcu1(self.ctx, _u32([qc]), _u32([q_mask]), 1, qt, phase, self.gc_mask))
+
+    def mcu1(self, ctrls, qt, phase, q_mask=None):
+        vals = None if q_mask is None else _u32(q_mask)
+        check(lib().suqa_hip_apply_mcu1(self.ctx, _u32(ctrls), vals, len(ctrls), qt, phase, self.gc_mask))
+
+    def swap(self, a, b):
+        check(lib().suqa_hip_apply_swap(self.ctx, a, b, self.gc_mask))
+
+    def phase_list(self, q0, q_size, phases):
+        arr = (C.c_double * max(1, len(phases)))(*phases)
+        return lib().suqa_hip_apply_phase_list(self.ctx, q0, q_size, arr, len(phases), self.gc_mask)
+
+    def pauli(self, qs, types, phase):
+        """Returns 0, or SUQA_ERR_UNSUPPORTED where the reference throws."""
+        rc = lib().suqa_hip_apply_pauli_TP_rotation(self.ctx, _u32(qs), _u32(types), len(qs), phase,
+                                                    self.gc_mask)
+        if rc not in (SUQA_OK, SUQA_ERR_UNSUPPORTED):
+            check(rc)
+        return rc
+
+    def pauli_general(self, qs, types, phase):
+        return check(lib().suqa_hip_apply_pauli_rotation_general(self.ctx, _u32(qs), _u32(types), len(qs),
+                                                                 phase, self.gc_mask))
+
+    def qft(self, qs):
+        check(lib().suqa_hip_apply_qft(self.ctx, _u32(qs), len(qs), self.gc_mask))
+
+    def qft_inverse(self, qs):
+        check(lib().suqa_hip_apply_qft_inverse(self.ctx, _u32(qs), len(qs), self.gc_mask))
+
+    def measure(self, q, rdoub):
+        c = U32()
+        check(lib().suqa_hip_measure_qbit(self.ctx, q, rdoub, C.byref(c)))
+        return c.value
+
+    def reset(self, q, rdoub):
+        check(lib().suqa_hip_apply_reset(self.ctx, q, rdoub, self.gc_mask))
+
+    def prob_filter(self, qs, q_mask):
+        p = C.c_double()
+        check(lib().suqa_hip_prob_filter(self.ctx, _u32(qs), _u32(q_mask), len(qs), C.byref(p)))
+        return p.value
+
+    def qms_apply_W(self, q_acc, w_mask_enew, bm_enes0, c_factor, curr_E_old):
+        check(lib().suqa_hip_qms_apply_W(self.ctx, q_acc, w_mask_enew, bm_enes0, c_factor, curr_E_old))
+
+    # state access -----------------------------------------------------------
+    def get(self):
+        re = np.empty(self.N, dtype=np.float64)
+        im = np.empty(self.N, dtype=np.float64)
+        check(lib().suqa_hip_download(self.ctx, re.ctypes.data_as(PD), im.ctypes.data_as(PD)))
+        return re, im
+
+    def set(self, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64)
+        im = np.ascontiguousarray(im, dtype=np.float64)
+        check(lib().suqa_hip_upload(self.ctx, re.ctypes.data_as(PD), im.ctypes.data_as(PD)))
+
+    def re(self):
+        return self.get()[0]
+
+    def im(self):
+        return self.get()[1]
+
+    def synchronize(self):
+        check(lib().suqa_hip_synchronize(self.ctx))
+
+    def run_gates(self, gates, n=None):
+        n = len(gates) if n is None else n
+        check(lib().suqa_hip_run_gates(self.ctx, gates, n))
+
+    def apply_gate(self, g):
+        self.run_gates((Gate * 1)(g), 1)
+
+    def timing(self, on):
+        check(lib().suqa_hip_timing_enable(self.ctx, 1 if on else 0))
+
+    def timing_reset(self):
+        check(lib().suqa_hip_timing_reset(self.ctx))
+
+    def timing_report(self):
+        arr = (KStat * 64)()
+        n = U32()
+        check(lib().suqa_hip_timing_report(self.ctx, arr, 64, C.byref(n)))
+        return {arr[k].name.decode(): dict(launches=arr[k].launches, ms=arr[k].total_ms, bytes=arr[k].alg_bytes)
+                for k in range(min(n.value, 64))}
+
+
+def gates_from(seq):
+    """Converts a sequence of objects with kind/target/nctrl/ctrl/angle into a Gate array."""
+    seq = list(seq)
+    arr = (Gate * len(seq))()
+    for k, g in enumerate(seq):
+        arr[k].kind = g.kind
+        arr[k].target = g.target
+        arr[k].nctrl = g.nctrl
+        for j in range(3):
+            arr[k].ctrl[j] = g.ctrl[j]
+        arr[k].angle = g.angle
+    return arr

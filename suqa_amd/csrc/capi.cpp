@@ -1,0 +1,686 @@
+// capi.cpp -- the extern "C" entry points of include/suqa_hip.h.
+//
+// Host side of each reference API call: build the reference's bit masks
+// (suqa.cu), evaluate sincos/exp/sqrt with the host libm exactly where the
+// reference does, turn the predicate into a dense selection and launch one
+// sweep.  No CPU fallback: every amplitude update happens in a HIP kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+using sqe::PauliCoef;
+using sqe::Sel;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define SQE_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(SUQA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define SQE_CTX(c)                                                                             \
+    do {                                                                                       \
+        if (!(c) || !(c)->d_data) return fail(SUQA_ERR_ARG, "null or cleared context");        \
+    } while (0)
+
+uint32_t popc32(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+
+int read_result(suqa_hip_ctx* c, int n) {
+    SQE_HIP(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    SQE_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+int free_state(suqa_hip_ctx* c) {
+    if (c->d_data) SQE_HIP(hipFree(c->d_data));
+    c->d_data = c->d_re = c->d_im = nullptr;
+    c->nq = 0;
+    c->N = 0;
+    return SUQA_OK;
+}
+
+// exp(i phase P) with P given by X/Y/Z masks (qubits already validated).
+int pauli_masks(suqa_hip_ctx* c, uint32_t xm, uint32_t ym, uint32_t zm, double phase, uint32_t gc) {
+    double s, cs;
+    sincos(phase, &s, &cs);
+    const uint64_t xy = (uint64_t)(xm | ym), yz = (uint64_t)(ym | zm);
+    const uint32_t ny = popc32(ym);
+    Sel sel;
+    if (xy == 0) {
+        // diagonal: psi' = (c + i (-1)^{par(i&Z)} s) psi  (util_rotate2 form)
+        if (!sqe::make_sel(c->nq, gc, gc, &sel)) return SUQA_OK;
+        SQE_HIP(sqe::launch_phase(c, sel, cs, s, yz, "pauli_diag", 32.0 * (double)sel.count));
+        return SUQA_OK;
+    }
+    const uint64_t pivot = xy & (~xy + 1ull);
+    if (!sqe::make_sel(c->nq, (uint64_t)gc | pivot, gc, &sel)) return SUQA_OK;
+    PauliCoef pc;
+    pc.c = cs;
+    pc.s = s;
+    pc.yz = yz;
+    pc.ny_odd = ny & 1u;
+    pc.base_imag = (ny + 1u) & 1u;
+    pc.base_neg = ((ny + 1u) >> 1) & 1u;
+    SQE_HIP(sqe::launch_pair(c, sqe::OP_PAULI, sel, xy, pc, "pauli_pair", 64.0 * (double)sel.count));
+    return SUQA_OK;
+}
+
+int check_qubits(const suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_t gc) {
+    uint32_t seen = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+        const uint32_t b = 1u << qs[k];
+        if (seen & b) return fail(SUQA_ERR_ARG, "repeated qubit in a Pauli string");
+        if (gc & b) return fail(SUQA_ERR_ARG, "Pauli rotation on a qubit of the active gc_mask");
+        seen |= b;
+    }
+    return SUQA_OK;
+}
+
+// crm of the QFT (suqa.cu:620-633): the phase table is filled by the first call.
+int qft_crm(suqa_hip_ctx* c, uint32_t qc, uint32_t qt, int m, int nqubits, uint32_t gc) {
+    if (!c->rphase_filled) {
+        c->rphase.assign((size_t)nqubits + 1, 0.0);
+        uint32_t cc = 1;
+        for (int i = 0; i < nqubits + 1; ++i) {
+            c->rphase[i] = (2.0 * M_PI / (double)cc);
+            cc <<= 1;
+        }
+        c->rphase_filled = true;
+    }
+    const size_t idx = (size_t)(m > 0 ? m : -m);
+    if (idx >= c->rphase.size())
+        return fail(SUQA_ERR_ARG, "QFT register larger than the first one (phase table filled once)");
+    double rphase = c->rphase[idx];
+    if (m <= 0) rphase *= -1;
+    const uint32_t one = 1u;
+    return suqa_hip_apply_mcu1(c, &qc, &one, 1, qt, rphase, gc);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ timing
+namespace sqe {
+
+void timing_flush(suqa_hip_ctx* c) {
+    if (c->recs.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& r : c->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) c->stats[r.stat].total_ms += ms;
+        c->evpool.push_back(r.e0);
+        c->evpool.push_back(r.e1);
+    }
+    c->recs.clear();
+}
+
+static hipEvent_t get_event(suqa_hip_ctx* c) {
+    if (!c->evpool.empty()) {
+        hipEvent_t e = c->evpool.back();
+        c->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes) {
+    if (!c->timing) return;
+    if (c->recs.size() >= 4096) timing_flush(c);
+    int idx = -1;
+    for (size_t k = 0; k < c->stats.size(); ++k)
+        if (c->stats[k].name == tag) idx = (int)k;
+    if (idx < 0) {
+        c->stats.push_back(KStat{tag, 0, 0.0, 0.0});
+        idx = (int)c->stats.size() - 1;
+    }
+    c->stats[idx].launches++;
+    c->stats[idx].alg_bytes += alg_bytes;
+    suqa_hip_ctx::Rec r{idx, get_event(c), get_event(c)};
+    (void)hipEventRecord(r.e0, c->stream);
+    c->recs.push_back(r);
+}
+
+void timing_end(suqa_hip_ctx* c) {
+    if (!c->timing || c->recs.empty()) return;
+    (void)hipEventRecord(c->recs.back().e1, c->stream);
+}
+
+}  // namespace sqe
+
+// ------------------------------------------------------------ entry points
+extern "C" {
+
+const char* suqa_hip_last_error(void) { return g_err.c_str(); }
+
+int suqa_hip_allocate_state(suqa_hip_ctx* c, uint32_t nq) {
+    if (!c) return fail(SUQA_ERR_ARG, "null context");
+    if (nq < 1 || nq > 32) return fail(SUQA_ERR_ARG, "nq must be in [1, 32]");
+    if (c->d_data && c->nq == nq) return suqa_hip_init_state(c);
+    int rc = free_state(c);
+    if (rc) return rc;
+    const uint64_t N = 1ull << nq;
+    void* p = nullptr;
+    if (hipMalloc(&p, 2 * N * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SUQA_ERR_NOMEM, "hipMalloc of the state failed");
+    }
+    c->d_data = static_cast<double*>(p);
+    c->d_re = c->d_data;
+    c->d_im = c->d_data + N;
+    c->nq = nq;
+    c->N = N;
+    return suqa_hip_init_state(c);
+}
+
+int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out) {
+    if (!out) return fail(SUQA_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (nq < 1 || nq > 32) return fail(SUQA_ERR_ARG, "nq must be in [1, 32]");
+    auto* c = new suqa_hip_ctx();
+    SQE_HIP(hipGetDevice(&c->device));
+    int ncu = 0;
+    SQE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    c->num_cu = ncu > 0 ? ncu : 256;
+    c->max_blocks = (uint32_t)c->num_cu * 8u;
+    SQE_HIP(hipStreamCreate(&c->stream));
+    void* p = nullptr;
+    SQE_HIP(hipMalloc(&p, sizeof(double) * 2 * c->max_blocks));
+    c->d_partials = static_cast<double*>(p);
+    SQE_HIP(hipMalloc(&p, sizeof(double) * 4));
+    c->d_result = static_cast<double*>(p);
+    SQE_HIP(hipHostMalloc(&p, sizeof(double) * 4, hipHostMallocDefault));
+    c->h_result = static_cast<double*>(p);
+    int rc = suqa_hip_allocate_state(c, nq);
+    if (rc) {
+        std::string keep = g_err;
+        suqa_hip_clear(c);
+        return fail(rc, keep);
+    }
+    *out = c;
+    return SUQA_OK;
+}
+
+int suqa_hip_clear(suqa_hip_ctx* c) {
+    if (!c) return SUQA_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& r : c->recs) {
+        (void)hipEventDestroy(r.e0);
+        (void)hipEventDestroy(r.e1);
+    }
+    for (auto e : c->evpool) (void)hipEventDestroy(e);
+    free_state(c);
+    if (c->d_partials) (void)hipFree(c->d_partials);
+    if (c->d_result) (void)hipFree(c->d_result);
+    if (c->h_result) (void)hipHostFree(c->h_result);
+    if (c->d_wtab) (void)hipFree(c->d_wtab);
+    if (c->d_wrow) (void)hipFree(c->d_wrow);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SUQA_OK;
+}
+
+int suqa_hip_init_state(suqa_hip_ctx* c) {
+    SQE_CTX(c);
+    SQE_HIP(sqe::launch_init(c));
+    return SUQA_OK;
+}
+
+int suqa_hip_vnorm(suqa_hip_ctx* c, double* norm) {
+    SQE_CTX(c);
+    Sel sel;
+    sqe::make_sel(c->nq, 0, 0, &sel);
+    SQE_HIP(sqe::launch_norm(c, sel, 0, "norm", 16.0 * (double)c->N));
+    int rc = read_result(c, 1);
+    if (rc) return rc;
+    if (norm) *norm = std::sqrt(c->h_result[0]);
+    return SUQA_OK;
+}
+
+int suqa_hip_vnormalize(suqa_hip_ctx* c) {
+    double v = 0.0;
+    int rc = suqa_hip_vnorm(c, &v);
+    if (rc) return rc;
+    SQE_HIP(sqe::launch_scale(c, 1. / v));
+    return SUQA_OK;
+}
+
+// X/Y on q: pairs (j, j|T) with the reference predicate (i & (gc|T)) == gc|T on i = j|T.
+static int pair_1q(suqa_hip_ctx* c, int op, uint32_t q, uint32_t gc, const char* tag) {
+    SQE_CTX(c);
+    if (q >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    const uint64_t T = 1ull << q;
+    const uint64_t M = (uint64_t)gc | T;
+    Sel sel;
+    if (!sqe::make_sel(c->nq, M, M & ~T, &sel)) return SUQA_OK;
+    PauliCoef pc{};
+    SQE_HIP(sqe::launch_pair(c, op, sel, T, pc, tag, 64.0 * (double)sel.count));
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_x(suqa_hip_ctx* c, uint32_t q, uint32_t gc) { return pair_1q(c, sqe::OP_SWAP, q, gc, "swap_pair"); }
+int suqa_hip_apply_y(suqa_hip_ctx* c, uint32_t q, uint32_t gc) { return pair_1q(c, sqe::OP_Y, q, gc, "y_pair"); }
+
+int suqa_hip_apply_h(suqa_hip_ctx* c, uint32_t q, uint32_t gc) {
+    SQE_CTX(c);
+    if (q >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    if (gc & (1u << q)) return fail(SUQA_ERR_ARG, "apply_h on a qubit of the active gc_mask");
+    const uint64_t T = 1ull << q;
+    Sel sel;
+    if (!sqe::make_sel(c->nq, (uint64_t)gc | T, gc, &sel)) return SUQA_OK;
+    PauliCoef pc{};
+    SQE_HIP(sqe::launch_pair(c, sqe::OP_H, sel, T, pc, "h_pair", 64.0 * (double)sel.count));
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_u1(suqa_hip_ctx* c, uint32_t q, uint32_t q_mask, double phase, uint32_t gc) {
+    SQE_CTX(c);
+    if (q >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    const uint32_t qmask = gc | (q_mask << q);       // suqa.cu:272
+    const uint32_t glob = gc | (1u << q);            // suqa_cpu.hpp:211
+    double s, cs;
+    sincos(phase, &s, &cs);
+    Sel sel;
+    if (!sqe::make_sel(c->nq, glob, qmask, &sel)) return SUQA_OK;
+    SQE_HIP(sqe::launch_phase(c, sel, cs, s, 0, "phase", 32.0 * (double)sel.count));
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_mcx(suqa_hip_ctx* c, const uint32_t* ctrls, const uint32_t* vals, uint32_t nctrl,
+                       uint32_t target, uint32_t gc) {
+    SQE_CTX(c);
+    if (target >= c->nq) return fail(SUQA_ERR_ARG, "target index out of range");
+    if (nctrl && !ctrls) return fail(SUQA_ERR_ARG, "null control list");
+    uint32_t mask = gc, mqs = gc;                    // suqa.cu:290-292, 308-310, 326-332
+    for (uint32_t k = 0; k < nctrl; ++k) {
+        if (ctrls[k] >= c->nq) return fail(SUQA_ERR_ARG, "control index out of range");
+        mask |= 1u << ctrls[k];
+        if (!vals || vals[k]) mqs |= 1u << ctrls[k];
+    }
+    const uint64_t T = 1ull << target;               // suqa_cpu.hpp:239-240
+    const uint64_t M = (uint64_t)mask | T, Vup = (uint64_t)mqs | T;
+    Sel sel;
+    if (!sqe::make_sel(c->nq, M, Vup & ~T, &sel)) return SUQA_OK;
+    PauliCoef pc{};
+    SQE_HIP(sqe::launch_pair(c, sqe::OP_SWAP, sel, T, pc, "swap_pair", 64.0 * (double)sel.count));
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_mcu1(suqa_hip_ctx* c, const uint32_t* ctrls, const uint32_t* vals, uint32_t nctrl,
+                        uint32_t target, double phase, uint32_t gc) {
+    SQE_CTX(c);
+    if (target >= c->nq) return fail(SUQA_ERR_ARG, "target index out of range");
+    if (nctrl && !ctrls) return fail(SUQA_ERR_ARG, "null control list");
+    uint32_t mask = (1u << target) | gc, mqs = (1u << target) | gc;   // suqa.cu:347-349, 367-373
+    for (uint32_t k = 0; k < nctrl; ++k) {
+        if (ctrls[k] >= c->nq) return fail(SUQA_ERR_ARG, "control index out of range");
+        mask |= 1u << ctrls[k];
+        if (!vals || vals[k]) mqs |= 1u << ctrls[k];
+    }
+    double s, cs;
+    sincos(phase, &s, &cs);
+    Sel sel;
+    if (!sqe::make_sel(c->nq, mask, mqs, &sel)) return SUQA_OK;
+    SQE_HIP(sqe::launch_phase(c, sel, cs, s, 0, "phase", 32.0 * (double)sel.count));
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_swap(suqa_hip_ctx* c, uint32_t q1, uint32_t q2, uint32_t gc) {
+    SQE_CTX(c);
+    if (q1 >= c->nq || q2 >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    if (q1 == q2) return SUQA_OK;                    // the reference swaps each amplitude with itself
+    const uint32_t m1 = 1u << q1, m2 = 1u << q2;
+    if (gc & (m1 | m2)) return fail(SUQA_ERR_ARG, "apply_swap on a qubit of the active gc_mask");
+    // suqa_cpu.hpp:303-309: (i & (gc|m1|m2)) == gc, swap i|m1 <-> i|m2
+    Sel sel;
+    if (!sqe::make_sel(c->nq, (uint64_t)(gc | m1 | m2), (uint64_t)(gc | m1), &sel)) return SUQA_OK;
+    PauliCoef pc{};
+    SQE_HIP(sqe::launch_pair(c, sqe::OP_SWAP, sel, (uint64_t)(m1 | m2), pc, "swap_pair",
+                             64.0 * (double)sel.count));
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_phase_list(suqa_hip_ctx* c, uint32_t q0, uint32_t q_size, const double* phases,
+                              uint32_t nphases, uint32_t gc) {
+    SQE_CTX(c);
+    if (!(q_size > 0u && q_size <= 7u && nphases == (1u << q_size)) || !phases)
+        return fail(SUQA_ERR_ARG, "in suqa::apply_phase_list(): invalid q_size or phases.size()");
+    if (q0 >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    double cs[128], sn[128];
+    for (uint32_t k = 0; k < nphases; ++k) sincos(phases[k], &sn[k], &cs[k]);
+    Sel sel;
+    if (!sqe::make_sel(c->nq, gc, gc, &sel)) return SUQA_OK;
+    SQE_HIP(sqe::launch_phase_list(c, sel, q0, (1u << q_size) - 1u, cs, sn));
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_pauli_TP_rotation(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* types, uint32_t n,
+                                     double phase, uint32_t gc) {
+    SQE_CTX(c);
+    if (n == 0 || n > 3)
+        return fail(SUQA_ERR_UNSUPPORTED, "ERROR: unimplemented pauli TP rotation in the selected configuration");
+    if (!qs || !types) return fail(SUQA_ERR_ARG, "null argument");
+    int rc = check_qubits(c, qs, n, gc);
+    if (rc) return rc;
+    uint32_t t[3], q[3];
+    for (uint32_t k = 0; k < n; ++k) {
+        t[k] = types[k];
+        q[k] = qs[k];
+    }
+    auto sw = [&](int a, int b) {
+        std::swap(t[a], t[b]);
+        std::swap(q[a], q[b]);
+    };
+    uint32_t xm = 0, ym = 0, zm = 0;
+    auto add = [&](uint32_t type, uint32_t qq) {
+        if (type == SUQA_PAULI_X) xm |= 1u << qq;
+        else if (type == SUQA_PAULI_Y) ym |= 1u << qq;
+        else if (type == SUQA_PAULI_Z) zm |= 1u << qq;
+    };
+    if (n == 1) {
+        if (t[0] < 1 || t[0] > 3) return SUQA_OK;    // suqa_cpu.hpp:360-370: no matching case
+        add(t[0], q[0]);
+    } else if (n == 2) {
+        if (t[0] > t[1]) sw(0, 1);                   // suqa.cu:506-509
+        const bool known = t[0] >= 1 && t[1] <= 3 && t[0] <= t[1];
+        if (!known) return SUQA_OK;                  // suqa_cpu.hpp:422-440: no matching case
+        if (t[0] == SUQA_PAULI_X && t[1] == SUQA_PAULI_Y) {
+            // suqa_cpu.hpp:431-433: the reference's X(q1)Y(q2) block rotates
+            // about Y on q1 and X on q2.
+            add(SUQA_PAULI_Y, q[0]);
+            add(SUQA_PAULI_X, q[1]);
+        } else {
+            add(t[0], q[0]);
+            add(t[1], q[1]);
+        }
+    } else {
+        if (t[0] > t[1]) sw(0, 1);                   // suqa.cu:536-547
+        if (t[1] > t[2]) sw(1, 2);
+        if (t[0] > t[1]) sw(0, 1);
+        if (t[2] != SUQA_PAULI_Z || t[0] != t[1])
+            return fail(SUQA_ERR_UNSUPPORTED, "ERROR: unimplemented pauli TP rotation in the selected configuration");
+        if (t[0] < 1) return SUQA_OK;                // suqa_cpu.hpp:493-508: no matching case
+        add(t[0], q[0]);
+        add(t[1], q[1]);
+        add(SUQA_PAULI_Z, q[2]);
+    }
+    return pauli_masks(c, xm, ym, zm, phase, gc);
+}
+
+int suqa_hip_apply_pauli_rotation_general(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* types, uint32_t n,
+                                          double phase, uint32_t gc) {
+    SQE_CTX(c);
+    if (n > 32 || (n && (!qs || !types))) return fail(SUQA_ERR_ARG, "bad Pauli string");
+    int rc = check_qubits(c, qs, n, gc);
+    if (rc) return rc;
+    uint32_t xm = 0, ym = 0, zm = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (types[k] > 3u) return fail(SUQA_ERR_ARG, "Pauli type must be 0..3");
+        if (types[k] == SUQA_PAULI_X) xm |= 1u << qs[k];
+        else if (types[k] == SUQA_PAULI_Y) ym |= 1u << qs[k];
+        else if (types[k] == SUQA_PAULI_Z) zm |= 1u << qs[k];
+    }
+    return pauli_masks(c, xm, ym, zm, phase, gc);
+}
+
+int suqa_hip_apply_qft(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_t gc) {
+    SQE_CTX(c);
+    const int qsize = (int)n;                        // suqa.cu:635-647
+    for (int o = qsize - 1; o >= 0; o--) {
+        int rc = suqa_hip_apply_h(c, qs[o], gc);
+        if (rc) return rc;
+        for (int in = o - 1; in >= 0; in--) {
+            rc = qft_crm(c, qs[in], qs[o], +1 + (o - in), qsize, gc);
+            if (rc) return rc;
+        }
+    }
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_qft_inverse(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_t gc) {
+    SQE_CTX(c);
+    const int qsize = (int)n;                        // suqa.cu:649-661
+    for (int o = 0; o < qsize; o++) {
+        for (int in = 0; in < o; in++) {
+            int rc = qft_crm(c, qs[in], qs[o], -1 - (o - in), qsize, gc);
+            if (rc) return rc;
+        }
+        int rc = suqa_hip_apply_h(c, qs[o], gc);
+        if (rc) return rc;
+    }
+    return SUQA_OK;
+}
+
+int suqa_hip_measure_qbit(suqa_hip_ctx* c, uint32_t q, double rdoub, uint32_t* cout) {
+    SQE_CTX(c);
+    if (q >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    Sel sel;
+    sqe::make_sel(c->nq, 0, 0, &sel);
+    const uint64_t qb = 1ull << q;
+    SQE_HIP(sqe::launch_norm(c, sel, qb, "measure_norm", 16.0 * (double)c->N));
+    int rc = read_result(c, 2);
+    if (rc) return rc;
+    const double p0 = c->h_result[0], p1 = c->h_result[1];
+    const uint32_t cc = (uint32_t)(rdoub < p1);      // suqa.cu:690
+    // set_ampl_to_zero + vnormalize (suqa.cu:694-695): the norm after the
+    // collapse is the kept half's.
+    const double vec_norm = std::sqrt(cc ? p1 : p0);
+    SQE_HIP(sqe::launch_collapse(c, qb, cc, 1. / vec_norm));
+    if (cout) *cout = cc;
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_reset(suqa_hip_ctx* c, uint32_t q, double rdoub, uint32_t gc) {
+    uint32_t cc = 0;
+    int rc = suqa_hip_measure_qbit(c, q, rdoub, &cc);
+    if (rc) return rc;
+    if (cc) return suqa_hip_apply_x(c, q, gc);
+    return SUQA_OK;
+}
+
+int suqa_hip_prob_filter(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* q_mask, uint32_t n, double* prob) {
+    SQE_CTX(c);
+    if (n && (!qs || !q_mask)) return fail(SUQA_ERR_ARG, "null argument");
+    uint32_t mqs = 0u, mask = 0u;                    // suqa.cu:706-713
+    for (uint32_t k = 0; k < n; ++k) {
+        if (qs[k] >= 32) return fail(SUQA_ERR_ARG, "qubit index out of range");
+        mqs |= 1u << qs[k];
+    }
+    for (uint32_t k = 0; k < n; ++k)
+        if (q_mask[k]) mask |= q_mask[k] << qs[k];
+    Sel sel;
+    if (!sqe::make_sel(c->nq, mqs, mask, &sel)) {
+        if (prob) *prob = 0.0;
+        return SUQA_OK;
+    }
+    SQE_HIP(sqe::launch_norm(c, sel, 0, "prob_filter", 16.0 * (double)sel.count));
+    int rc = read_result(c, 1);
+    if (rc) return rc;
+    if (prob) *prob = c->h_result[0];
+    return SUQA_OK;
+}
+
+int suqa_hip_qms_apply_W(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmask, uint32_t b0, double cf,
+                         uint32_t Eold) {
+    SQE_CTX(c);
+    if (q_acc != c->nq - 1) return fail(SUQA_ERR_ARG, "apply_W needs the acceptance qubit to be the most significant");
+    if (b0 >= 32) return fail(SUQA_ERR_ARG, "bad energy register offset");
+    const uint64_t L = (uint64_t)(wmask >> b0) + 1ull;
+    if (L > (1ull << 20)) return fail(SUQA_ERR_ARG, "energy register too wide for apply_W");
+    // fs1/fs2 exactly as qms.cuh:291-300 evaluates them (host libm).
+    auto coef = [&](uint64_t En, uint64_t Eo, double* out) {
+        const double dE = ((double)(uint32_t)En * cf - (double)(uint32_t)Eo * cf);
+        if (dE > 0) {
+            out[0] = std::exp(-dE / 2.0);
+            out[1] = std::sqrt(1.0 - out[0] * out[0]);
+        } else {
+            out[0] = 1.0;
+            out[1] = 0.0;
+        }
+    };
+    const double* row = nullptr;
+    if (L <= 256 && Eold < L) {
+        if (!(c->wtab_valid && c->wtab_c == cf && c->wtab_mask == wmask && c->wtab_b0 == b0)) {
+            std::vector<double> tab(2 * L * L);
+            for (uint64_t eo = 0; eo < L; ++eo)
+                for (uint64_t en = 0; en < L; ++en) coef(en, eo, &tab[2 * (eo * L + en)]);
+            if (c->wtab_L < L) {
+                if (c->d_wtab) SQE_HIP(hipFree(c->d_wtab));
+                c->d_wtab = nullptr;
+                void* p = nullptr;
+                SQE_HIP(hipMalloc(&p, sizeof(double) * 2 * 256 * 256));
+                c->d_wtab = static_cast<double*>(p);
+                c->wtab_L = 256;
+            }
+            SQE_HIP(hipMemcpyAsync(c->d_wtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice,
+                                   c->stream));
+            SQE_HIP(hipStreamSynchronize(c->stream));
+            c->wtab_valid = true;
+            c->wtab_c = cf;
+            c->wtab_mask = wmask;
+            c->wtab_b0 = b0;
+        }
+        row = c->d_wtab + 2 * (uint64_t)Eold * L;
+    } else {
+        std::vector<double> r(2 * L);
+        for (uint64_t en = 0; en < L; ++en) coef(en, Eold, &r[2 * en]);
+        if (c->wrow_cap < L) {
+            if (c->d_wrow) SQE_HIP(hipFree(c->d_wrow));
+            c->d_wrow = nullptr;
+            void* p = nullptr;
+            SQE_HIP(hipMalloc(&p, sizeof(double) * 2 * L));
+            c->d_wrow = static_cast<double*>(p);
+            c->wrow_cap = L;
+        }
+        SQE_HIP(hipMemcpyAsync(c->d_wrow, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
+        SQE_HIP(hipStreamSynchronize(c->stream));
+        row = c->d_wrow;
+    }
+    SQE_HIP(sqe::launch_qms_W(c, row, wmask, b0));
+    return SUQA_OK;
+}
+
+int suqa_hip_state_info(suqa_hip_ctx* c, uint32_t* nq, uint64_t* N, double** re, double** im) {
+    SQE_CTX(c);
+    if (nq) *nq = c->nq;
+    if (N) *N = c->N;
+    if (re) *re = c->d_re;
+    if (im) *im = c->d_im;
+    return SUQA_OK;
+}
+
+int suqa_hip_download(suqa_hip_ctx* c, double* re, double* im) {
+    SQE_CTX(c);
+    if (re) SQE_HIP(hipMemcpyAsync(re, c->d_re, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (im) SQE_HIP(hipMemcpyAsync(im, c->d_im, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SQE_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+int suqa_hip_upload(suqa_hip_ctx* c, const double* re, const double* im) {
+    SQE_CTX(c);
+    if (re) SQE_HIP(hipMemcpyAsync(c->d_re, re, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (im) SQE_HIP(hipMemcpyAsync(c->d_im, im, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SQE_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+int suqa_hip_synchronize(suqa_hip_ctx* c) {
+    SQE_CTX(c);
+    SQE_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+int suqa_hip_run_gates(suqa_hip_ctx* c, const suqa_hip_gate* gates, uint32_t ngates) {
+    SQE_CTX(c);
+    const uint32_t tx = SUQA_PAULI_X, tz = SUQA_PAULI_Z;
+    for (uint32_t k = 0; k < ngates; ++k) {
+        const suqa_hip_gate& g = gates[k];
+        int rc = SUQA_OK;
+        switch (g.kind) {
+        case SUQA_GATE_H: rc = suqa_hip_apply_h(c, g.target, 0); break;
+        case SUQA_GATE_X: rc = suqa_hip_apply_x(c, g.target, 0); break;
+        case SUQA_GATE_RX: rc = suqa_hip_apply_pauli_TP_rotation(c, &g.target, &tx, 1, -g.angle / 2.0, 0); break;
+        case SUQA_GATE_RZ: rc = suqa_hip_apply_pauli_TP_rotation(c, &g.target, &tz, 1, -g.angle / 2.0, 0); break;
+        case SUQA_GATE_U1: rc = suqa_hip_apply_u1(c, g.target, 1u, g.angle, 0); break;
+        case SUQA_GATE_CX:
+        case SUQA_GATE_MCX:
+            if (g.nctrl > 3) return fail(SUQA_ERR_ARG, "at most 3 controls per workload gate");
+            rc = suqa_hip_apply_mcx(c, g.ctrl, nullptr, g.nctrl, g.target, 0);
+            break;
+        default: return fail(SUQA_ERR_ARG, "unknown gate kind");
+        }
+        if (rc) return rc;
+    }
+    return SUQA_OK;
+}
+
+int suqa_hip_timing_enable(suqa_hip_ctx* c, int on) {
+    SQE_CTX(c);
+    if (!on) sqe::timing_flush(c);
+    c->timing = on != 0;
+    return SUQA_OK;
+}
+
+int suqa_hip_timing_reset(suqa_hip_ctx* c) {
+    SQE_CTX(c);
+    sqe::timing_flush(c);
+    c->stats.clear();
+    return SUQA_OK;
+}
+
+int suqa_hip_timing_report(suqa_hip_ctx* c, suqa_hip_kstat* out, uint32_t max, uint32_t* n) {
+    SQE_CTX(c);
+    sqe::timing_flush(c);
+    uint32_t k = 0;
+    for (; k < c->stats.size() && k < max; ++k) {
+        std::memset(out[k].name, 0, sizeof(out[k].name));
+        std::strncpy(out[k].name, c->stats[k].name.c_str(), sizeof(out[k].name) - 1);
+        out[k].launches = c->stats[k].launches;
+        out[k].total_ms = c->stats[k].total_ms;
+        out[k].alg_bytes = c->stats[k].alg_bytes;
+    }
+    if (n) *n = (uint32_t)c->stats.size();
+    return SUQA_OK;
+}
+
+int suqa_hip_copy_kernel(suqa_hip_ctx* c, void* dst, const void* src, uint64_t nbytes) {
+    SQE_CTX(c);
+    if ((nbytes & 15u) || ((uintptr_t)dst & 15u) || ((uintptr_t)src & 15u))
+        return fail(SUQA_ERR_ARG, "copy needs 16-byte aligned pointers and size");
+    SQE_HIP(sqe::launch_copy(c, dst, src, nbytes));
+    return SUQA_OK;
+}
+
+int suqa_hip_device_alloc(uint64_t nbytes, void** ptr) {
+    if (!ptr) return fail(SUQA_ERR_ARG, "null pointer");
+    if (hipMalloc(ptr, nbytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SUQA_ERR_NOMEM, "hipMalloc failed");
+    }
+    return SUQA_OK;
+}
+
+int suqa_hip_device_free(void* ptr) {
+    SQE_HIP(hipFree(ptr));
+    return SUQA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// engine.hpp -- internals of the gfx950 state-vector engine shared by the
+// kernels (kernels.hip) and the C-ABI (capi.cpp).
+//
+// Every sweep visits a *selection*: the basis indices i with (i & M) == V,
+// enumerated densely by inserting the fixed bits of M into a counter k.
+// This is the reference's predicate form (e.g. suqa_cpu.hpp:86-92), but the
+// GPU only touches the selected amplitudes instead of testing all N.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "suqa_hip.h"
+
+namespace sqe {
+
+constexpr int kBlock = 256;     // 4 waves of 64 lanes
+constexpr int kUnroll = 4;      // items per thread per grid-stride step
+
+// Sorted fixed bit positions of a selection.
+struct Fixed {
+    uint32_t n;
+    uint32_t pos[32];
+};
+
+struct Sel {
+    Fixed fb;
+    uint64_t fval;     // values of the fixed bits
+    uint64_t count;    // number of selected indices = N >> fb.n
+};
+
+// Selection {i < 2^nq : (i & M) == V}; returns false when it is empty.
+bool make_sel(uint32_t nq, uint64_t M, uint64_t V, Sel* out);
+
+enum PairOp : int { OP_SWAP = 0, OP_Y = 1, OP_H = 2, OP_PAULI = 3 };
+
+// exp(i phi P) on a pair (a, b = a ^ xy):
+//   psi'_a = c psi_a + zeta_a s psi_b, zeta_a = base (-1)^{par(b & yz)},
+//   base = i^{nY+1}: base_imag = (nY even), base_neg = ((nY+1) mod 4 >= 2).
+struct PauliCoef {
+    double c, s;
+    uint64_t yz;       // Y|Z positions
+    uint32_t ny_odd;   // nY mod 2 == par(xy & yz)
+    uint32_t base_imag;
+    uint32_t base_neg;
+};
+
+struct KStat {
+    std::string name;
+    uint64_t launches = 0;
+    double total_ms = 0.0;
+    double alg_bytes = 0.0;
+};
+
+}  // namespace sqe
+
+struct suqa_hip_ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    uint32_t nq = 0;
+    uint64_t N = 0;
+    double* d_data = nullptr;
+    double* d_re = nullptr;
+    double* d_im = nullptr;
+    // reductions: per-block partials (2 per block), final result, host copy
+    uint32_t max_blocks = 0;
+    double* d_partials = nullptr;
+    double* d_result = nullptr;
+    double* h_result = nullptr;     // pinned, 4 doubles
+    // qms::apply_W coefficient table (fs1, fs2) per (E_old, E_new)
+    double* d_wtab = nullptr;
+    uint64_t wtab_L = 0;
+    bool wtab_valid = false;
+    double wtab_c = 0.0;
+    uint32_t wtab_mask = 0, wtab_b0 = 0;
+    double* d_wrow = nullptr;       // slow path: one row, L > 256
+    uint64_t wrow_cap = 0;
+    // one-shot QFT phase table (suqa.cu:611-625)
+    bool rphase_filled = false;
+    std::vector<double> rphase;
+    // timing
+    bool timing = false;
+    struct Rec {
+        int stat;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> evpool;
+    std::vector<sqe::KStat> stats;
+};
+
+namespace sqe {
+
+// Launchers (kernels.hip).  All enqueue on ctx->stream and return a HIP error.
+hipError_t launch_init(suqa_hip_ctx* c);
+hipError_t launch_scale(suqa_hip_ctx* c, double f);
+hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,
+                       const char* tag, double alg_bytes);
+hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
+                        const char* tag, double alg_bytes);
+hipError_t launch_phase_list(suqa_hip_ctx* c, const Sel& sel, uint32_t q0, uint32_t size_mask,
+                             const double* cs, const double* sn);
+// sum |psi|^2 over sel into d_result[0] (split_bit == 0) or into
+// d_result[0] (bit clear) and d_result[1] (bit set).
+hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
+                       double alg_bytes);
+hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale);
+hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_row, uint32_t wmask, uint32_t b0);
+hipError_t launch_copy(suqa_hip_ctx* c, void* dst, const void* src, uint64_t nbytes);
+
+// timing hooks (capi.cpp)
+void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes);
+void timing_end(suqa_hip_ctx* c);
+
+}  // namespace sqe

@@ -1,0 +1,541 @@
+// kernels.hip -- FP64 state-vector sweeps for gfx950 (MI355X).
+//
+// Layout: SoA re[N], im[N] in one HBM allocation (complex_defines.cuh:22-45).
+// Every kernel is a grid-stride sweep over a selection of basis indices
+// (engine.hpp: Sel).  When bit 0 of the index is free, a thread item covers
+// two consecutive amplitudes with 16-byte (double2) loads and stores, so a
+// wave reads 1 KiB contiguous per array and instruction.  kUnroll items per
+// thread are loaded before any is written, which keeps 4 x 4 x 16 B of loads
+// in flight per lane.  Arithmetic keeps the reference's operation order and
+// is compiled with -ffp-contract=off, so amplitudes match the reference CPU
+// path bit for bit (suqa_cpu.hpp:49-66 forms x*c - y*s, y*c + x*s).
+#include "engine.hpp"
+
+namespace sqe {
+
+namespace {
+
+constexpr double kTwoSqInv = 0.7071067811865475;   // suqa.cuh:57
+
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+// Inserts a zero at each fixed position (ascending), then ORs their values.
+__device__ __forceinline__ uint64_t expand(uint64_t k, const Fixed& fb, uint64_t fval) {
+    for (uint32_t j = 0; j < fb.n; ++j) {
+        const uint64_t lo = (1ull << fb.pos[j]) - 1ull;
+        k = (k & lo) | ((k & ~lo) << 1);
+    }
+    return k | fval;
+}
+
+template <int VEC>
+struct Vec;
+template <>
+struct Vec<1> {
+    double x[1];
+    __device__ __forceinline__ static Vec load(const double* p) {
+        Vec v;
+        v.x[0] = __builtin_nontemporal_load(p);
+        return v;
+    }
+    __device__ __forceinline__ void store(double* p) const { __builtin_nontemporal_store(x[0], p); }
+};
+template <>
+struct Vec<2> {
+    double x[2];
+    __device__ __forceinline__ static Vec load(const double* p) {
+        const dv2 t = __builtin_nontemporal_load(reinterpret_cast<const dv2*>(p));
+        Vec v;
+        v.x[0] = t.x;
+        v.x[1] = t.y;
+        return v;
+    }
+    __device__ __forceinline__ void store(double* p) const {
+        dv2 t;
+        t.x = x[0];
+        t.y = x[1];
+        __builtin_nontemporal_store(t, reinterpret_cast<dv2*>(p));
+    }
+};
+
+struct PairArgs {
+    Fixed fb;
+    uint64_t fval;
+    uint64_t nitems;   // items of VEC amplitudes
+    uint64_t xmask;
+    PauliCoef pc;
+};
+
+// psi'_a = c psi_a + zeta s psi_b, zeta in {1,-1,i,-i}; t = +-s carries the sign.
+__device__ __forceinline__ void pauli_out(double ar, double ai, double br, double bi, double c, double t,
+                                          bool imag, double& orr, double& oi) {
+    if (imag) {   // zeta = +-i: re = ar*c - (+-s)*bi, im = ai*c + (+-s)*br
+        orr = ar * c + bi * (-t);
+        oi = ai * c + br * t;
+    } else {      // zeta = +-1
+        orr = ar * c + br * t;
+        oi = ai * c + bi * t;
+    }
+}
+
+template <int OP, int VEC>
+__global__ __launch_bounds__(kBlock) void k_pair(double* __restrict__ re, double* __restrict__ im,
+                                                 const PairArgs a) {
+    const uint64_t step = (uint64_t)gridDim.x * kBlock * kUnroll;
+    for (uint64_t base = (uint64_t)blockIdx.x * kBlock * kUnroll + threadIdx.x; base < a.nitems;
+         base += step) {
+        uint64_t ia[kUnroll];
+        Vec<VEC> ar[kUnroll], ai[kUnroll], br[kUnroll], bi[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t it = base + (uint64_t)u * kBlock;
+            const uint64_t k = it < a.nitems ? it : 0;
+            ia[u] = expand(k * VEC, a.fb, a.fval);
+            const uint64_t ib = ia[u] ^ a.xmask;
+            ar[u] = Vec<VEC>::load(re + ia[u]);
+            ai[u] = Vec<VEC>::load(im + ia[u]);
+            br[u] = Vec<VEC>::load(re + ib);
+            bi[u] = Vec<VEC>::load(im + ib);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (base + (uint64_t)u * kBlock >= a.nitems) break;
+            Vec<VEC> oar, oai, obr, obi;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const double xr = ar[u].x[e], xi = ai[u].x[e], yr = br[u].x[e], yi = bi[u].x[e];
+                if (OP == OP_SWAP) {
+                    oar.x[e] = yr; oai.x[e] = yi; obr.x[e] = xr; obi.x[e] = xi;
+                } else if (OP == OP_Y) {   // suqa_cpu.hpp:116-118
+                    oar.x[e] = yi; oai.x[e] = yr * -1.0; obr.x[e] = xi * -1.0; obi.x[e] = xr;
+                } else if (OP == OP_H) {   // suqa_cpu.hpp:195-198
+                    oar.x[e] = kTwoSqInv * (xr + yr);
+                    oai.x[e] = kTwoSqInv * (xi + yi);
+                    obr.x[e] = kTwoSqInv * (xr - yr);
+                    obi.x[e] = kTwoSqInv * (xi - yi);
+                } else {                    // OP_PAULI
+                    const uint64_t ix = ia[u] + (uint64_t)e;
+                    const uint32_t par = (uint32_t)__popcll(ix & a.pc.yz) & 1u;
+                    const uint32_t neg_a = a.pc.base_neg ^ par ^ a.pc.ny_odd;
+                    const uint32_t neg_b = a.pc.base_neg ^ par;
+                    const double ta = neg_a ? -a.pc.s : a.pc.s;
+                    const double tb = neg_b ? -a.pc.s : a.pc.s;
+                    pauli_out(xr, xi, yr, yi, a.pc.c, ta, a.pc.base_imag, oar.x[e], oai.x[e]);
+                    pauli_out(yr, yi, xr, xi, a.pc.c, tb, a.pc.base_imag, obr.x[e], obi.x[e]);
+                }
+            }
+            const uint64_t ib = ia[u] ^ a.xmask;
+            oar.store(re + ia[u]);
+            oai.store(im + ia[u]);
+            obr.store(re + ib);
+            obi.store(im + ib);
+        }
+    }
+}
+
+struct PhaseArgs {
+    Fixed fb;
+    uint64_t fval;
+    uint64_t nitems;
+    double c, s;
+    uint64_t zmask;    // s changes sign where par(i & zmask) == 1
+};
+
+// util_rotate2 (suqa_cpu.hpp:49-54): re' = re*c - im*s, im' = im*c + re*s.
+template <int VEC>
+__global__ __launch_bounds__(kBlock) void k_phase(double* __restrict__ re, double* __restrict__ im,
+                                                  const PhaseArgs a) {
+    const uint64_t step = (uint64_t)gridDim.x * kBlock * kUnroll;
+    for (uint64_t base = (uint64_t)blockIdx.x * kBlock * kUnroll + threadIdx.x; base < a.nitems;
+         base += step) {
+        uint64_t ia[kUnroll];
+        Vec<VEC> xr[kUnroll], xi[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t it = base + (uint64_t)u * kBlock;
+            ia[u] = expand((it < a.nitems ? it : 0) * VEC, a.fb, a.fval);
+            xr[u] = Vec<VEC>::load(re + ia[u]);
+            xi[u] = Vec<VEC>::load(im + ia[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (base + (uint64_t)u * kBlock >= a.nitems) break;
+            Vec<VEC> orr, oi;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const uint32_t par = (uint32_t)__popcll((ia[u] + (uint64_t)e) & a.zmask) & 1u;
+                const double s = par ? -a.s : a.s;
+                const double r = xr[u].x[e], m = xi[u].x[e];
+                orr.x[e] = r * a.c - m * s;
+                oi.x[e] = m * a.c + r * s;
+            }
+            orr.store(re + ia[u]);
+            oi.store(im + ia[u]);
+        }
+    }
+}
+
+struct PhaseListArgs {
+    Fixed fb;
+    uint64_t fval;
+    uint64_t nitems;
+    uint32_t q0, size_mask;
+    double c[128];
+    double s[128];
+};
+
+__global__ __launch_bounds__(kBlock) void k_phase_list(double* __restrict__ re, double* __restrict__ im,
+                                                       const PhaseListArgs a) {
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t it = (uint64_t)blockIdx.x * kBlock + threadIdx.x; it < a.nitems; it += step) {
+        const uint64_t i = expand(it, a.fb, a.fval);
+        const uint32_t k = (uint32_t)(i >> a.q0) & a.size_mask;
+        const double c = a.c[k], s = a.s[k];
+        const double r = re[i], m = im[i];
+        re[i] = r * c - m * s;
+        im[i] = m * c + r * s;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+struct NormArgs {
+    Fixed fb;
+    uint64_t fval;
+    uint64_t nitems;
+    uint64_t split;    // 0: one sum; else sums for (i & split) == 0 and != 0
+};
+
+template <int VEC>
+__global__ __launch_bounds__(kBlock) void k_norm(const double* __restrict__ re, const double* __restrict__ im,
+                                                 const NormArgs a, double* __restrict__ partials) {
+    __shared__ double lds[2][kBlock / 64];
+    double acc0 = 0.0, acc1 = 0.0;
+    const uint64_t step = (uint64_t)gridDim.x * kBlock * kUnroll;
+    for (uint64_t base = (uint64_t)blockIdx.x * kBlock * kUnroll + threadIdx.x; base < a.nitems;
+         base += step) {
+        uint64_t ia[kUnroll];
+        Vec<VEC> xr[kUnroll], xi[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t it = base + (uint64_t)u * kBlock;
+            ia[u] = expand((it < a.nitems ? it : 0) * VEC, a.fb, a.fval);
+            xr[u] = Vec<VEC>::load(re + ia[u]);
+            xi[u] = Vec<VEC>::load(im + ia[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (base + (uint64_t)u * kBlock >= a.nitems) break;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const double r = xr[u].x[e], m = xi[u].x[e];
+                const double p = r * r + m * m;
+                if (a.split && ((ia[u] + (uint64_t)e) & a.split)) acc1 += p;
+                else acc0 += p;
+            }
+        }
+    }
+    acc0 = wave_sum(acc0);
+    acc1 = wave_sum(acc1);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][w] = acc0;
+        lds[1][w] = acc1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            s0 += lds[0][k];
+            s1 += lds[1][k];
+        }
+        partials[2 * blockIdx.x] = s0;
+        partials[2 * blockIdx.x + 1] = s1;
+    }
+}
+
+// One block: fixed-order sum of nparts (pairs of) partials.
+__global__ __launch_bounds__(kBlock) void k_norm_final(const double* __restrict__ partials, uint32_t nparts,
+                                                       double* __restrict__ out) {
+    __shared__ double lds[2][kBlock / 64];
+    double s0 = 0.0, s1 = 0.0;
+    for (uint32_t k = threadIdx.x; k < nparts; k += kBlock) {
+        s0 += partials[2 * k];
+        s1 += partials[2 * k + 1];
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][threadIdx.x >> 6] = s0;
+        lds[1][threadIdx.x >> 6] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            t0 += lds[0][k];
+            t1 += lds[1][k];
+        }
+        out[0] = t0;
+        out[1] = t1;
+    }
+}
+
+// measure_qbit collapse (suqa_cpu.hpp:544-549 then :36-39): the discarded half
+// becomes +0.0, the kept half is multiplied by scale = 1./norm.
+__global__ __launch_bounds__(kBlock) void k_collapse(double* __restrict__ re, double* __restrict__ im,
+                                                     uint64_t nitems, uint64_t qbit, uint64_t keepbit,
+                                                     double scale) {
+    const uint64_t step = (uint64_t)gridDim.x * kBlock * kUnroll;
+    for (uint64_t base = (uint64_t)blockIdx.x * kBlock * kUnroll + threadIdx.x; base < nitems; base += step) {
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t it = base + (uint64_t)u * kBlock;
+            if (it >= nitems) break;
+            const uint64_t i = it * 2;
+            Vec<2> r, m;
+            const bool k0 = (i & qbit) == keepbit;
+            const bool k1 = ((i + 1) & qbit) == keepbit;
+            if (k0 || k1) {
+                r = Vec<2>::load(re + i);
+                m = Vec<2>::load(im + i);
+            } else {
+                r.x[0] = r.x[1] = m.x[0] = m.x[1] = 0.0;
+            }
+            r.x[0] = k0 ? r.x[0] * scale : 0.0;
+            m.x[0] = k0 ? m.x[0] * scale : 0.0;
+            r.x[1] = k1 ? r.x[1] * scale : 0.0;
+            m.x[1] = k1 ? m.x[1] * scale : 0.0;
+            r.store(re + i);
+            m.store(im + i);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale(double* __restrict__ data, uint64_t nitems, double f) {
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t it = (uint64_t)blockIdx.x * kBlock + threadIdx.x; it < nitems; it += step) {
+        Vec<2> v = Vec<2>::load(data + 2 * it);
+        v.x[0] *= f;
+        v.x[1] *= f;
+        v.store(data + 2 * it);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_init(double* __restrict__ data, uint64_t nitems) {
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t it = (uint64_t)blockIdx.x * kBlock + threadIdx.x; it < nitems; it += step) {
+        Vec<2> v;
+        v.x[0] = it == 0 ? 1.0 : 0.0;
+        v.x[1] = 0.0;
+        v.store(data + 2 * it);
+    }
+}
+
+// qms::apply_W, dense form of qms.cuh:286-307; row[2*E] = (fs1, fs2) for E_old fixed.
+__global__ __launch_bounds__(kBlock) void k_qms_W(double* __restrict__ re, double* __restrict__ im,
+                                                  uint64_t half, const double* __restrict__ row,
+                                                  uint32_t wmask, uint32_t b0) {
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < half; j += step) {
+        const uint64_t i = j + half;
+        const uint32_t En = ((uint32_t)i & wmask) >> b0;
+        const double fs1 = row[2 * En], fs2 = row[2 * En + 1];
+        double t = re[j];
+        const double ri = re[i];
+        re[j] = fs2 * t + fs1 * ri;
+        re[i] = fs1 * t - fs2 * ri;
+        t = im[j];
+        const double mi = im[i];
+        im[j] = fs2 * t + fs1 * mi;
+        im[i] = fs1 * t - fs2 * mi;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(dv2* __restrict__ dst, const dv2* __restrict__ src,
+                                                 uint64_t n16) {
+    const uint64_t step = (uint64_t)gridDim.x * kBlock * kUnroll;
+    for (uint64_t base = (uint64_t)blockIdx.x * kBlock * kUnroll + threadIdx.x; base < n16; base += step) {
+        dv2 v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t it = base + (uint64_t)u * kBlock;
+            v[u] = __builtin_nontemporal_load(src + (it < n16 ? it : 0));
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t it = base + (uint64_t)u * kBlock;
+            if (it < n16) __builtin_nontemporal_store(v[u], dst + it);
+        }
+    }
+}
+
+inline uint32_t grid_for(const suqa_hip_ctx* c, uint64_t nitems, uint64_t per_block) {
+    uint64_t g = (nitems + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > c->max_blocks) g = c->max_blocks;
+    return (uint32_t)g;
+}
+
+inline bool bit0_free(const Sel& s) { return s.fb.n == 0 || s.fb.pos[0] != 0; }
+
+}  // namespace
+
+bool make_sel(uint32_t nq, uint64_t M, uint64_t V, Sel* out) {
+    const uint64_t all = (nq >= 64) ? ~0ull : ((1ull << nq) - 1ull);
+    if ((V & ~M) != 0ull) return false;       // (i & M) == V has no solution
+    if ((M & ~all) != 0ull) {
+        // fixed bits beyond the state: a solution needs those bits of V to be 0
+        if ((V & ~all) != 0ull) return false;
+        M &= all;
+    }
+    out->fb.n = 0;
+    for (uint32_t b = 0; b < nq; ++b)
+        if ((M >> b) & 1ull) out->fb.pos[out->fb.n++] = b;
+    out->fval = V & M;
+    out->count = (1ull << nq) >> out->fb.n;
+    return true;
+}
+
+hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,
+                       const char* tag, double alg_bytes) {
+    PairArgs a;
+    a.fb = sel.fb;
+    a.fval = sel.fval;
+    a.xmask = xmask;
+    a.pc = pc;
+    const bool v2 = bit0_free(sel) && sel.count >= 2 && (xmask & 1ull) == 0ull;
+    a.nitems = v2 ? sel.count / 2 : sel.count;
+    const uint32_t g = grid_for(c, a.nitems, (uint64_t)kBlock * kUnroll);
+    timing_begin(c, tag, alg_bytes);
+#define SQE_PAIR(OPV)                                                                             \
+    case OPV:                                                                                     \
+        if (v2) hipLaunchKernelGGL((k_pair<OPV, 2>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, \
+                                   c->d_im, a);                                                   \
+        else hipLaunchKernelGGL((k_pair<OPV, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re,    \
+                                c->d_im, a);                                                      \
+        break;
+    switch (op) {
+        SQE_PAIR(OP_SWAP)
+        SQE_PAIR(OP_Y)
+        SQE_PAIR(OP_H)
+        SQE_PAIR(OP_PAULI)
+    default: break;
+    }
+#undef SQE_PAIR
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
+                        const char* tag, double alg_bytes) {
+    PhaseArgs a;
+    a.fb = sel.fb;
+    a.fval = sel.fval;
+    a.c = cs;
+    a.s = sn;
+    a.zmask = zmask;
+    const bool v2 = bit0_free(sel) && sel.count >= 2;
+    a.nitems = v2 ? sel.count / 2 : sel.count;
+    const uint32_t g = grid_for(c, a.nitems, (uint64_t)kBlock * kUnroll);
+    timing_begin(c, tag, alg_bytes);
+    if (v2) hipLaunchKernelGGL(k_phase<2>, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    else hipLaunchKernelGGL(k_phase<1>, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_list(suqa_hip_ctx* c, const Sel& sel, uint32_t q0, uint32_t size_mask,
+                             const double* cs, const double* sn) {
+    PhaseListArgs a;
+    a.fb = sel.fb;
+    a.fval = sel.fval;
+    a.nitems = sel.count;
+    a.q0 = q0;
+    a.size_mask = size_mask;
+    for (uint32_t k = 0; k <= size_mask; ++k) {
+        a.c[k] = cs[k];
+        a.s[k] = sn[k];
+    }
+    const uint32_t g = grid_for(c, a.nitems, kBlock);
+    timing_begin(c, "phase_list", 32.0 * (double)sel.count);
+    hipLaunchKernelGGL(k_phase_list, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
+                       double alg_bytes) {
+    NormArgs a;
+    a.fb = sel.fb;
+    a.fval = sel.fval;
+    a.split = split_bit;
+    // VEC=2 needs both amplitudes of an item on the same side of the split.
+    const bool v2 = bit0_free(sel) && sel.count >= 2 && split_bit != 1ull;
+    a.nitems = v2 ? sel.count / 2 : sel.count;
+    const uint32_t g = grid_for(c, a.nitems, (uint64_t)kBlock * kUnroll);
+    timing_begin(c, tag, alg_bytes);
+    if (v2) hipLaunchKernelGGL(k_norm<2>, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
+    else hipLaunchKernelGGL(k_norm<1>, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
+    timing_end(c);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    timing_begin(c, "norm_final", 16.0 * g);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, c->stream, c->d_partials, g, c->d_result);
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale) {
+    const uint64_t nitems = c->N / 2;
+    const uint32_t g = grid_for(c, nitems, (uint64_t)kBlock * kUnroll);
+    // the discarded half is written without being read: 24 B per amplitude
+    timing_begin(c, "collapse", 24.0 * (double)c->N);
+    hipLaunchKernelGGL(k_collapse, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, nitems, qbit,
+                       keep ? qbit : 0ull, scale);
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(suqa_hip_ctx* c, double f) {
+    const uint64_t nitems = c->N;   // 2N doubles as N double2
+    const uint32_t g = grid_for(c, nitems, kBlock * 4);
+    timing_begin(c, "scale", 32.0 * (double)c->N);
+    hipLaunchKernelGGL(k_scale, dim3(g), dim3(kBlock), 0, c->stream, c->d_data, nitems, f);
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_init(suqa_hip_ctx* c) {
+    const uint64_t nitems = c->N;
+    const uint32_t g = grid_for(c, nitems, kBlock * 4);
+    timing_begin(c, "init", 16.0 * (double)c->N);
+    hipLaunchKernelGGL(k_init, dim3(g), dim3(kBlock), 0, c->stream, c->d_data, nitems);
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_row, uint32_t wmask, uint32_t b0) {
+    const uint64_t half = c->N / 2;
+    const uint32_t g = grid_for(c, half, kBlock * 4);
+    timing_begin(c, "qms_W", 32.0 * (double)c->N);
+    hipLaunchKernelGGL(k_qms_W, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, half, d_row, wmask, b0);
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(suqa_hip_ctx* c, void* dst, const void* src, uint64_t nbytes) {
+    const uint64_t n16 = nbytes / 16;
+    const uint32_t g = grid_for(c, n16, (uint64_t)kBlock * kUnroll);
+    timing_begin(c, "copy", 2.0 * (double)nbytes);
+    hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBlock), 0, c->stream, reinterpret_cast<dv2*>(dst),
+                       reinterpret_cast<const dv2*>(src), n16);
+    timing_end(c);
+    return hipGetLastError();
+}
+
+}  // namespace sqe
