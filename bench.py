@@ -98,13 +98,13 @@ def main():
     eng.synchronize()
     per = n
     for s in range(a.warmup):
-        capi.check(capi.lib().suqa_hip_run_gates(eng.ctx, capi.C.byref(arr, s * per * capi.C.sizeof(capi.Gate)), per))
+        eng.run_gates(arr, per, s * per)
     eng.synchronize()
     eng.timing(True)
     eng.timing_reset()
     t0 = time.perf_counter()
     for s in range(a.warmup, layers):
-        capi.check(capi.lib().suqa_hip_run_gates(eng.ctx, capi.C.byref(arr, s * per * capi.C.sizeof(capi.Gate)), per))
+        eng.run_gates(arr, per, s * per)
         if (s - a.warmup) % 50 == 49:
             log(f"  layer {s + 1}/{layers}")
     eng.synchronize()

@@ -259,9 +259,11 @@ class Engine:
     def synchronize(self):
         check(lib().suqa_hip_synchronize(self.ctx))
 
-    def run_gates(self, gates, n=None):
-        n = len(gates) if n is None else n
-        check(lib().suqa_hip_run_gates(self.ctx, gates, n))
+    def run_gates(self, gates, n=None, start=0):
+        """Applies gates[start:start+n] (a Gate array) asynchronously."""
+        n = len(gates) - start if n is None else n
+        ptr = C.cast(C.addressof(gates) + start * C.sizeof(Gate), C.POINTER(Gate))
+        check(lib().suqa_hip_run_gates(self.ctx, ptr, n))
 
     def apply_gate(self, g):
         self.run_gates((Gate * 1)(g), 1)

@@ -49,6 +49,8 @@ int read_result(suqa_hip_ctx* c, int n) {
 
 int free_state(suqa_hip_ctx* c) {
     if (c->d_data) SQE_HIP(hipFree(c->d_data));
+    if (c->d_partials) SQE_HIP(hipFree(c->d_partials));
+    c->d_partials = nullptr;
     c->d_data = c->d_re = c->d_im = nullptr;
     c->nq = 0;
     c->N = 0;
@@ -187,6 +189,12 @@ int suqa_hip_allocate_state(suqa_hip_ctx* c, uint32_t nq) {
     c->d_im = c->d_data + N;
     c->nq = nq;
     c->N = N;
+    if (hipMalloc(&p, sizeof(double) * 2 * sqe::partial_pairs_capacity(N)) != hipSuccess) {
+        (void)hipGetLastError();
+        free_state(c);
+        return fail(SUQA_ERR_NOMEM, "hipMalloc of the reduction partials failed");
+    }
+    c->d_partials = static_cast<double*>(p);
     return suqa_hip_init_state(c);
 }
 
@@ -199,11 +207,8 @@ int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out) {
     int ncu = 0;
     SQE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     c->num_cu = ncu > 0 ? ncu : 256;
-    c->max_blocks = (uint32_t)c->num_cu * 8u;
     SQE_HIP(hipStreamCreate(&c->stream));
     void* p = nullptr;
-    SQE_HIP(hipMalloc(&p, sizeof(double) * 2 * c->max_blocks));
-    c->d_partials = static_cast<double*>(p);
     SQE_HIP(hipMalloc(&p, sizeof(double) * 4));
     c->d_result = static_cast<double*>(p);
     SQE_HIP(hipHostMalloc(&p, sizeof(double) * 4, hipHostMallocDefault));
@@ -227,7 +232,6 @@ int suqa_hip_clear(suqa_hip_ctx* c) {
     }
     for (auto e : c->evpool) (void)hipEventDestroy(e);
     free_state(c);
-    if (c->d_partials) (void)hipFree(c->d_partials);
     if (c->d_result) (void)hipFree(c->d_result);
     if (c->h_result) (void)hipHostFree(c->h_result);
     if (c->d_wtab) (void)hipFree(c->d_wtab);

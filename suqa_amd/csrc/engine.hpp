@@ -16,8 +16,8 @@
 
 namespace sqe {
 
-constexpr int kBlock = 256;     // 4 waves of 64 lanes
-constexpr int kUnroll = 4;      // items per thread per grid-stride step
+constexpr int kBlock = 256;          // 4 waves of 64 lanes
+constexpr uint64_t kPartChunk = 2048; // partial pairs summed per block in the reduction tree
 
 // Sorted fixed bit positions of a selection.
 struct Fixed {
@@ -66,8 +66,7 @@ struct suqa_hip_ctx {
     double* d_re = nullptr;
     double* d_im = nullptr;
     // reductions: per-block partials (2 per block), final result, host copy
-    uint32_t max_blocks = 0;
-    double* d_partials = nullptr;
+    double* d_partials = nullptr;      // 2 * sqe::partial_pairs_capacity(N) doubles
     double* d_result = nullptr;
     double* h_result = nullptr;     // pinned, 4 doubles
     // qms::apply_W coefficient table (fs1, fs2) per (E_old, E_new)
@@ -110,6 +109,8 @@ hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, cons
 hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale);
 hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_row, uint32_t wmask, uint32_t b0);
 hipError_t launch_copy(suqa_hip_ctx* c, void* dst, const void* src, uint64_t nbytes);
+// Number of (sum, sum) partial pairs launch_norm needs for a 2^nq state.
+uint64_t partial_pairs_capacity(uint64_t N);
 
 // timing hooks (capi.cpp)
 void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes);
