@@ -126,16 +126,115 @@ __global__ __launch_bounds__(kBlock) void k_pair(double* __restrict__ re, double
     obi.store(im + ib);
 }
 
+// ---------------------------------------------------------------------------
+// Dense-low-bit mode.  A fixed bit or partner bit below kLowBits leaves every
+// 128-byte line (16 amplitudes) partly selected; skipping the unselected
+// amplitudes only makes the accesses strided.  Instead, sweep whole lines:
+// lanes 8m..8m+7 hold one 16-amplitude group as 8 double2, each element tests
+// its own membership (role a: (j & M) == V; role b: ((j ^ X) & M) == V),
+// and the partner j ^ X is read from the lane (j ^ X) & 15 of the same wave
+// (lane xor (X & 15) >> 1, element xor X & 1), or from the paired group
+// g ^ (X & ~15) that the same thread loads.  Unselected elements are written
+// back unchanged, so every store covers full lines.
+constexpr uint64_t kLowMask = 15;
+
+struct PairDLArgs {
+    Fixed fb;          // iteration: high fixed bits, plus the pivot of X & ~15
+    uint64_t fval;
+    uint64_t nitems;   // double2 items
+    uint64_t M, V;     // role-a predicate of the gate
+    uint64_t X;        // partner mask
+    uint64_t xhigh;    // X & ~15
+    int lanex;         // (X & 15) >> 1
+    int ex;            // X & 1
+    PauliCoef pc;
+};
+
+__device__ __forceinline__ Vec<2> shfl_pick(const Vec<2>& v, int lanex, int ex) {
+    Vec<2> r;
+    double a = v.x[0], b = v.x[1];
+    if (lanex) {
+        a = __shfl_xor(a, lanex, 64);
+        b = __shfl_xor(b, lanex, 64);
+    }
+    r.x[0] = ex ? b : a;
+    r.x[1] = ex ? a : b;
+    return r;
+}
+
+// Output of element j (value x) given its partner k = j ^ X (value y).
+template <int OP>
+__device__ __forceinline__ void dl_out(uint64_t j, const PairDLArgs& a, double xr, double xi, double yr, double yi,
+                                       double& orr, double& oi) {
+    const bool inA = (j & a.M) == a.V;
+    const bool inB = ((j ^ a.X) & a.M) == a.V;
+    orr = xr;
+    oi = xi;
+    if (!(inA || inB)) return;
+    if (OP == OP_SWAP) {
+        orr = yr;
+        oi = yi;
+    } else if (OP == OP_Y) {            // suqa_cpu.hpp:116-118
+        if (inA) { orr = yi; oi = yr * -1.0; }
+        else { orr = yi * -1.0; oi = yr; }
+    } else if (OP == OP_H) {            // suqa_cpu.hpp:195-198
+        if (inA) { orr = kTwoSqInv * (xr + yr); oi = kTwoSqInv * (xi + yi); }
+        else { orr = kTwoSqInv * (yr - xr); oi = kTwoSqInv * (yi - xi); }
+    } else {                            // OP_PAULI: psi_j' = c psi_j + base (-1)^{par(k&YZ)} s psi_k
+        const uint32_t par = (uint32_t)__popcll((j ^ a.X) & a.pc.yz) & 1u;
+        const double t = (a.pc.base_neg ^ par) ? -a.pc.s : a.pc.s;
+        pauli_out(xr, xi, yr, yi, a.pc.c, t, a.pc.base_imag, orr, oi);
+    }
+}
+
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_pair_dl(double* __restrict__ re, double* __restrict__ im,
+                                                    const PairDLArgs a) {
+    const uint64_t it0 = item_id();
+    const bool ok = it0 < a.nitems;
+    const uint64_t p = expand((ok ? it0 : 0) * 2, a.fb, a.fval);
+    const bool two = a.xhigh != 0;
+    const uint64_t q = p ^ a.xhigh;
+    const Vec<2> pr = Vec<2>::load(re + p), pi = Vec<2>::load(im + p);
+    Vec<2> qr = pr, qi = pi;
+    if (two) {
+        qr = Vec<2>::load(re + q);
+        qi = Vec<2>::load(im + q);
+    }
+    // partners of p's elements live in q's registers of lane ^ lanex
+    const Vec<2> yr = shfl_pick(qr, a.lanex, a.ex), yi = shfl_pick(qi, a.lanex, a.ex);
+    Vec<2> opr, opi;
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+        dl_out<OP>(p + (uint64_t)e, a, pr.x[e], pi.x[e], yr.x[e], yi.x[e], opr.x[e], opi.x[e]);
+    if (two) {
+        const Vec<2> zr = shfl_pick(pr, a.lanex, a.ex), zi = shfl_pick(pi, a.lanex, a.ex);
+        Vec<2> oqr, oqi;
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+            dl_out<OP>(q + (uint64_t)e, a, qr.x[e], qi.x[e], zr.x[e], zi.x[e], oqr.x[e], oqi.x[e]);
+        if (ok) {
+            oqr.store(re + q);
+            oqi.store(im + q);
+        }
+    }
+    if (ok) {
+        opr.store(re + p);
+        opi.store(im + p);
+    }
+}
+
 struct PhaseArgs {
     Fixed fb;
     uint64_t fval;
     uint64_t nitems;
     double c, s;
     uint64_t zmask;    // s changes sign where par(i & zmask) == 1
+    uint64_t M, V;     // dense-low mode: element i is rotated iff (i & M) == V
 };
 
 // util_rotate2 (suqa_cpu.hpp:49-54): re' = re*c - im*s, im' = im*c + re*s.
-template <int VEC>
+template <int VEC, bool DL>
 __global__ __launch_bounds__(kBlock) void k_phase(double* __restrict__ re, double* __restrict__ im,
                                                   const PhaseArgs a) {
     const uint64_t it = item_id();
@@ -145,9 +244,15 @@ __global__ __launch_bounds__(kBlock) void k_phase(double* __restrict__ re, doubl
     Vec<VEC> orr, oi;
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
-        const uint32_t par = (uint32_t)__popcll((ia + (uint64_t)e) & a.zmask) & 1u;
+        const uint64_t j = ia + (uint64_t)e;
+        const uint32_t par = (uint32_t)__popcll(j & a.zmask) & 1u;
         const double s = par ? -a.s : a.s;
         const double r = xr.x[e], m = xi.x[e];
+        if (DL && (j & a.M) != a.V) {
+            orr.x[e] = r;
+            oi.x[e] = m;
+            continue;
+        }
         orr.x[e] = r * a.c - m * s;
         oi.x[e] = m * a.c + r * s;
     }
@@ -187,9 +292,10 @@ struct NormArgs {
     uint64_t fval;
     uint64_t nitems;
     uint64_t split;    // 0: one sum; else sums for (i & split) == 0 and != 0
+    uint64_t M, V;     // dense-low mode: element i counts iff (i & M) == V
 };
 
-template <int VEC>
+template <int VEC, bool DL>
 __global__ __launch_bounds__(kBlock) void k_norm(const double* __restrict__ re, const double* __restrict__ im,
                                                  const NormArgs a, double* __restrict__ partials) {
     __shared__ double lds[2][kBlock / 64];
@@ -200,9 +306,10 @@ __global__ __launch_bounds__(kBlock) void k_norm(const double* __restrict__ re, 
         const Vec<VEC> xr = Vec<VEC>::load(re + ia), xi = Vec<VEC>::load(im + ia);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
+            const uint64_t j = ia + (uint64_t)e;
             const double r = xr.x[e], m = xi.x[e];
-            const double p = r * r + m * m;
-            if (a.split && ((ia + (uint64_t)e) & a.split)) acc1 += p;
+            const double p = (DL && (j & a.M) != a.V) ? 0.0 : r * r + m * m;
+            if (a.split && (j & a.split)) acc1 += p;
             else acc0 += p;
         }
     }
@@ -333,6 +440,12 @@ inline uint32_t grid_for(uint64_t nitems) {
 
 inline bool bit0_free(const Sel& s) { return s.fb.n == 0 || s.fb.pos[0] != 0; }
 
+inline uint64_t sel_mask(const Sel& s) {
+    uint64_t m = 0;
+    for (uint32_t k = 0; k < s.fb.n; ++k) m |= 1ull << s.fb.pos[k];
+    return m;
+}
+
 }  // namespace
 
 bool make_sel(uint32_t nq, uint64_t M, uint64_t V, Sel* out) {
@@ -353,6 +466,37 @@ bool make_sel(uint32_t nq, uint64_t M, uint64_t V, Sel* out) {
 
 hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,
                        const char* tag, double alg_bytes) {
+    const uint64_t M = sel_mask(sel), V = sel.fval;
+    if (((M | xmask) & kLowMask) != 0) {
+        PairDLArgs a;
+        a.M = M;
+        a.V = V;
+        a.X = xmask;
+        a.xhigh = xmask & ~kLowMask;
+        a.lanex = (int)((xmask & kLowMask) >> 1);
+        a.ex = (int)(xmask & 1ull);
+        a.pc = pc;
+        const uint64_t Mi = (M & ~kLowMask & ~a.xhigh) | (a.xhigh & (~a.xhigh + 1ull));
+        Sel it;
+        make_sel(c->nq, Mi, V & Mi, &it);
+        a.fb = it.fb;
+        a.fval = it.fval;
+        a.nitems = it.count / 2;
+        const uint32_t g = grid_for(a.nitems);
+        timing_begin(c, tag, alg_bytes);
+#define SQE_PAIR_DL(OPV) \
+    case OPV: hipLaunchKernelGGL((k_pair_dl<OPV>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a); break;
+        switch (op) {
+            SQE_PAIR_DL(OP_SWAP)
+            SQE_PAIR_DL(OP_Y)
+            SQE_PAIR_DL(OP_H)
+            SQE_PAIR_DL(OP_PAULI)
+        default: break;
+        }
+#undef SQE_PAIR_DL
+        timing_end(c);
+        return hipGetLastError();
+    }
     PairArgs a;
     a.fb = sel.fb;
     a.fval = sel.fval;
@@ -384,17 +528,24 @@ hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, 
 hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
                         const char* tag, double alg_bytes) {
     PhaseArgs a;
-    a.fb = sel.fb;
-    a.fval = sel.fval;
     a.c = cs;
     a.s = sn;
     a.zmask = zmask;
-    const bool v2 = bit0_free(sel) && sel.count >= 2;
-    a.nitems = v2 ? sel.count / 2 : sel.count;
+    a.M = sel_mask(sel);
+    a.V = sel.fval;
+    const bool dl = (a.M & kLowMask) != 0;
+    Sel it = sel;
+    if (dl) make_sel(c->nq, a.M & ~kLowMask, a.V & ~kLowMask, &it);
+    a.fb = it.fb;
+    a.fval = it.fval;
+    const bool v2 = it.count >= 2;
+    a.nitems = v2 ? it.count / 2 : it.count;
     const uint32_t g = grid_for(a.nitems);
     timing_begin(c, tag, alg_bytes);
-    if (v2) hipLaunchKernelGGL(k_phase<2>, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
-    else hipLaunchKernelGGL(k_phase<1>, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    if (dl && v2) hipLaunchKernelGGL((k_phase<2, true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    else if (dl) hipLaunchKernelGGL((k_phase<1, true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    else if (v2) hipLaunchKernelGGL((k_phase<2, false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    else hipLaunchKernelGGL((k_phase<1, false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
     timing_end(c);
     return hipGetLastError();
 }
@@ -427,16 +578,22 @@ uint64_t partial_pairs_capacity(uint64_t N) {
 hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
                        double alg_bytes) {
     NormArgs a;
-    a.fb = sel.fb;
-    a.fval = sel.fval;
     a.split = split_bit;
-    // VEC=2 needs both amplitudes of an item on the same side of the split.
-    const bool v2 = bit0_free(sel) && sel.count >= 2 && split_bit != 1ull;
-    a.nitems = v2 ? sel.count / 2 : sel.count;
+    a.M = sel_mask(sel);
+    a.V = sel.fval;
+    const bool dl = (a.M & kLowMask) != 0;
+    Sel it = sel;
+    if (dl) make_sel(c->nq, a.M & ~kLowMask, a.V & ~kLowMask, &it);
+    a.fb = it.fb;
+    a.fval = it.fval;
+    const bool v2 = it.count >= 2;
+    a.nitems = v2 ? it.count / 2 : it.count;
     const uint32_t g = grid_for(a.nitems);
     timing_begin(c, tag, alg_bytes);
-    if (v2) hipLaunchKernelGGL(k_norm<2>, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
-    else hipLaunchKernelGGL(k_norm<1>, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
+    if (dl && v2) hipLaunchKernelGGL((k_norm<2, true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
+    else if (dl) hipLaunchKernelGGL((k_norm<1, true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
+    else if (v2) hipLaunchKernelGGL((k_norm<2, false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
+    else hipLaunchKernelGGL((k_norm<1, false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
     timing_end(c);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

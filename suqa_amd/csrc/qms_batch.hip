@@ -1,0 +1,591 @@
+// qms_batch.hip -- thousands of independent QMS chains in one persistent kernel.
+//
+// One wavefront runs one chain at a time, taking chains from a work queue in
+// device memory.  The chain's state (2^n FP64 amplitudes, n = r + 4 <= 10,
+// SoA re/im) lives in the wave's slice of LDS (16 KiB at n = 10), so every
+// gate is a short loop over 64-lane strides with runtime qubit indices: the
+// code stays small for any register width and nothing spills.  A workgroup is
+// one wave, so __syncthreads() orders one gate's LDS writes before the next
+// gate's reads at the cost of a single-wave barrier.
+//
+// The control flow is metro_step (qms.cuh:373-545) with the same pcg32 draw
+// order; gate arithmetic keeps the reference's operation order
+// (suqa_cpu.hpp:49-66, 150-206; qms.cuh:286-307) and device code is built
+// with -ffp-contract=off, so a chain reproduces the reference CPU path's
+// accept/reject sequence for its seed.  Measurement sums are wave
+// reductions (another summation order than the reference's sequential
+// loop: amplitudes agree to ~1e-15, and outcomes are identical unless a
+// uniform draw falls within that distance of a probability).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "qms_batch.hpp"
+#include "suqa_hip.h"
+#include "suqa_qms.h"
+
+namespace qb {
+namespace {
+
+constexpr double kTwoSqInv = 0.7071067811865475;   // suqa.cuh:57
+constexpr int E0 = 3;                              // first energy qubit (qms.cuh:79-88)
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t ins0(uint32_t p, uint32_t q) {   // insert a zero bit at q
+    return ((p >> q) << (q + 1)) | (p & ((1u << q) - 1u));
+}
+
+// pcg32 of the `pcg` wrapper (pcg32.h:53-68, 121-130; Rand.hpp:257-273)
+struct Rng {
+    uint64_t state, inc;
+    __device__ void seed(uint64_t s) {
+        state = 0;
+        inc = (1ull << 1u) | 1u;
+        next();
+        state += s;
+        next();
+    }
+    __device__ uint32_t next() {
+        const uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+    }
+    __device__ double doub() {
+        return __longlong_as_double((long long)(((uint64_t)next() << 20) | 0x3ff0000000000000ULL)) - 1.0;
+    }
+};
+
+struct Chain {
+    double* re;          // LDS
+    double* im;
+    uint32_t r, n, N, L, acc;
+    uint32_t lane;
+    Rng rng;
+    const ChainDev* P;
+
+    __device__ void sync() const { __syncthreads(); }
+
+    // ------------------------------------------------------------ gates
+    __device__ void init_zero() {
+        for (uint32_t i = lane; i < N; i += 64) {
+            re[i] = i == 0 ? 1.0 : 0.0;
+            im[i] = 0.0;
+        }
+        sync();
+    }
+
+    // H on q (suqa_cpu.hpp:186-204) for states with (i & gc) == gc
+    __device__ void h(uint32_t q, uint32_t gc) {
+        for (uint32_t p = lane; p < N / 2; p += 64) {
+            const uint32_t a = ins0(p, q);
+            if ((a & gc) != gc) continue;
+            const uint32_t b = a | (1u << q);
+            const double a0r = re[a], a0i = im[a], a1r = re[b], a1i = im[b];
+            re[a] = kTwoSqInv * (a0r + a1r);
+            im[a] = kTwoSqInv * (a0i + a1i);
+            re[b] = kTwoSqInv * (a0r - a1r);
+            im[b] = kTwoSqInv * (a0i - a1i);
+        }
+        sync();
+    }
+
+    // X on qt for states whose control bits read cv (suqa_cpu.hpp:239-251)
+    __device__ void mcx(uint32_t qt, uint32_t cm, uint32_t cv) {
+        for (uint32_t p = lane; p < N / 2; p += 64) {
+            const uint32_t a = ins0(p, qt);
+            const uint32_t b = a | (1u << qt);
+            if ((b & cm) != cv) continue;
+            double t = re[a];
+            re[a] = re[b];
+            re[b] = t;
+            t = im[a];
+            im[a] = im[b];
+            im[b] = t;
+        }
+        sync();
+    }
+    __device__ void x(uint32_t q) { mcx(q, 1u << q, 1u << q); }
+    __device__ void cx(uint32_t qc, uint32_t qt) {
+        const uint32_t m = (1u << qc) | (1u << qt);
+        mcx(qt, m, m);
+    }
+
+    // util_rotate2 on states with all bits of m set (u1 / cu1 with q_mask 1)
+    __device__ void phase(uint32_t q1, uint32_t q2, double c, double s) {
+        const uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
+        const uint32_t m = (1u << q1) | (1u << q2);
+        const uint32_t cnt = q1 == q2 ? N / 2 : N / 4;
+        for (uint32_t p = lane; p < cnt; p += 64) {
+            const uint32_t i = (q1 == q2 ? ins0(p, lo) : ins0(ins0(p, lo), hi)) | m;
+            const double cpy = re[i];
+            re[i] = cpy * c - im[i] * s;
+            im[i] = im[i] * c + cpy * s;
+        }
+        sync();
+    }
+
+    // exp(i phi X_a X_b) on states with (i & gc) == gc (suqa_cpu.hpp:422-424)
+    __device__ void xx(uint32_t qa, uint32_t qb_, double c, double s, uint32_t gc) {
+        const uint32_t pv = qa < qb_ ? qa : qb_;
+        const uint32_t X = (1u << qa) | (1u << qb_);
+        for (uint32_t p = lane; p < N / 2; p += 64) {
+            const uint32_t j = ins0(p, pv);
+            if ((j & gc) != gc) continue;
+            const uint32_t k = j ^ X;
+            const double jr = re[j], ji = im[j], kr = re[k], ki = im[k];
+            re[j] = jr * c - ki * s;
+            im[j] = ji * c + kr * s;
+            re[k] = kr * c - ji * s;
+            im[k] = ki * c + jr * s;
+        }
+        sync();
+    }
+
+    // measure_qbit (suqa.cu:673-696): c = (rd < P(q=1)), collapse, renormalize
+    __device__ uint32_t measure(uint32_t q, double rd) {
+        double s0 = 0.0, s1 = 0.0;
+        for (uint32_t i = lane; i < N; i += 64) {
+            const double p = re[i] * re[i] + im[i] * im[i];
+            if ((i >> q) & 1u) s1 += p;
+            else s0 += p;
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        const uint32_t c = __builtin_amdgcn_readfirstlane((uint32_t)(rd < s1));
+        const double scale = 1. / sqrt(c ? s1 : s0);
+        for (uint32_t i = lane; i < N; i += 64) {
+            const bool keep = ((i >> q) & 1u) == c;
+            re[i] = keep ? re[i] * scale : 0.0;
+            im[i] = keep ? im[i] * scale : 0.0;
+        }
+        sync();
+        return c;
+    }
+
+    __device__ void reset(uint32_t q, double rd) {   // suqa.cu:730-738
+        if (measure(q, rd)) x(q);
+    }
+
+    // qms::apply_W, dense form (qms.cuh:286-307); acc is the top qubit
+    __device__ void W(uint32_t Eold) {
+        const double* row = P->wtab + 2 * (size_t)Eold * L;
+        const uint32_t half = N / 2;
+        for (uint32_t j = lane; j < half; j += 64) {
+            const uint32_t i = j + half;
+            const uint32_t En = (i >> E0) & (L - 1);
+            const double fs1 = row[2 * En], fs2 = row[2 * En + 1];
+            double t = re[j];
+            const double ri = re[i];
+            re[j] = fs2 * t + fs1 * ri;
+            re[i] = fs1 * t - fs2 * ri;
+            t = im[j];
+            const double ii = im[i];
+            im[j] = fs2 * t + fs1 * ii;
+            im[i] = fs1 * t - fs2 * ii;
+        }
+        sync();
+    }
+
+    // ------------------------------------------------------- QMS pieces
+    __device__ void system_init() {   // system.cu:6-11
+        init_zero();
+        x(1);
+        h(1, 0u);
+        cx(1, 0);
+    }
+
+    __device__ void cevolution(uint32_t trg, int dir) {   // qms.cuh:106-115, system.cu:21-26
+        const uint32_t gc = 1u << (E0 + trg);
+        const double c = P->xx_c[dir][trg], s = P->xx_s[dir][trg];
+        xx(1, 2, c, s, gc);
+        xx(2, 0, c, s, gc);
+        xx(0, 1, c, s, gc);
+    }
+
+    __device__ __noinline__ void Phi() {   // qms.cuh:153-176
+        for (uint32_t k = 0; k < r; ++k) h(E0 + k, 0u);
+        for (int trg = (int)r - 1; trg > -1; --trg) {
+            cevolution((uint32_t)trg, 0);
+            phase(E0 + trg, E0 + trg, P->u1_c[0][trg], P->u1_s[0][trg]);
+        }
+        for (uint32_t o = 0; o < r; ++o) {   // QFT^-1 (suqa.cu:649-661)
+            for (uint32_t in = 0; in < o; ++in) phase(E0 + in, E0 + o, P->qft_c[1][1 + o - in], P->qft_s[1][1 + o - in]);
+            h(E0 + o, 0u);
+        }
+    }
+
+    __device__ __noinline__ void Phi_inverse() {   // qms.cuh:179-198
+        for (int o = (int)r - 1; o >= 0; o--) {   // QFT (suqa.cu:635-647)
+            h(E0 + o, 0u);
+            for (int in = o - 1; in >= 0; in--) phase(E0 + in, E0 + o, P->qft_c[0][1 + o - in], P->qft_s[0][1 + o - in]);
+        }
+        for (uint32_t trg = 0; trg < r; ++trg) {
+            cevolution(trg, 1);
+            phase(E0 + trg, E0 + trg, P->u1_c[1][trg], P->u1_s[1][trg]);
+        }
+        for (uint32_t k = 0; k < r; ++k) h(E0 + k, 0u);
+    }
+
+    __device__ __noinline__ void U(uint32_t ci, uint32_t Eold) {   // qms.cuh:335-348
+        h(ci, 0u);
+        Phi();
+        W(Eold);
+    }
+    __device__ __noinline__ void U_inverse(uint32_t ci, uint32_t Eold) {   // qms.cuh:350-360
+        W(Eold);
+        Phi_inverse();
+        h(ci, 0u);
+    }
+
+    // measure_qbits(bm_enes, cs, extract_rands(r)): all draws first
+    __device__ __noinline__ uint32_t measure_enes() {
+        double rd[kMaxEne];
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
+            if (k < r) rd[k] = rng.doub();
+        uint32_t v = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
+            if (k < r) v += measure(E0 + k, rd[k]) << k;
+        return v;
+    }
+
+    __device__ __noinline__ void reset_enes_each_draw() {
+        for (uint32_t k = 0; k < r; ++k) reset(E0 + k, rng.doub());
+    }
+
+    __device__ void measure_rotation() {   // system.cu:76-83
+        h(0, 0u);
+        h(1, 0u);
+        h(2, 0u);
+        cx(0, 1);
+        cx(0, 2);
+        phase(2, 2, P->hp_c, P->hp_s);
+    }
+
+    __device__ __noinline__ double measure_X() {   // system.cu:99-119, op_vals {2,0,2,0,-2,0,-2,0}
+        measure_rotation();
+        const double r0 = rng.doub(), r1 = rng.doub(), r2 = rng.doub();
+        const uint32_t c0 = measure(0, r0), c1 = measure(1, r1), c2 = measure(2, r2);
+        measure_rotation();
+        const uint32_t meas = c0 | (c1 << 1) | (c2 << 2);
+        return (meas & 1u) ? 0.0 : ((meas & 4u) ? -2.0 : 2.0);
+    }
+
+    // metro_step (qms.cuh:373-545); returns the code, E/X through *Ed/*Xv when measured
+    __device__ __noinline__ int metro_step(bool take, uint32_t max_rev, double* Ed, double* Xv) {
+        int ret = 0;
+        {   // reset_non_syst_qbits (qms.cuh:128-150)
+            double rd[kMaxEne];
+#pragma unroll
+            for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
+                if (k < r) rd[k] = rng.doub();
+#pragma unroll
+            for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
+                if (k < r) reset(E0 + k, rd[k]);
+            reset(acc, rng.doub());
+        }
+        Phi();
+        const uint32_t Eold = measure_enes();
+        for (uint32_t k = 0; k < r; ++k)
+            if ((Eold >> k) & 1u) x(E0 + k);
+        uint32_t ci;
+        {   // draw_C (qms.cuh:207-216, system.cu:123)
+            const double e = rng.doub();
+            ci = e < 1. / 3 ? 0u : (e < 2. / 3 ? 1u : 2u);
+        }
+        U(ci, Eold);
+        if (measure(acc, rng.doub()) == 1u) {
+            const uint32_t Enew = measure_enes();
+            Phi_inverse();
+            if (!take) return 1;
+            *Ed = P->etab[Enew];
+            for (uint32_t k = 0; k < r; ++k)
+                if ((Enew >> k) & 1u) x(E0 + k);
+            *Xv = measure_X();
+            reset_enes_each_draw();
+            Phi();
+            (void)measure_enes();
+            Phi_inverse();
+            return 2;
+        }
+        U_inverse(ci, Eold);
+        uint32_t iters = 0;
+        while (iters < max_rev) {
+            Phi();
+            const uint32_t Enew = measure_enes();
+            Phi_inverse();
+            if (Eold == Enew) {
+                if (take) {
+                    *Ed = P->etab[Eold];
+                    reset_enes_each_draw();
+                    *Xv = measure_X();
+                    reset_enes_each_draw();
+                    Phi();
+                    (void)measure_enes();
+                    Phi_inverse();
+                    ret = 4;
+                } else {
+                    ret = 3;
+                }
+                break;
+            }
+            U(ci, Eold);
+            (void)measure(acc, rng.doub());
+            U_inverse(ci, Eold);
+            iters++;
+        }
+        if (iters == max_rev) return -1;
+        return ret;
+    }
+
+    // driver loop of qms.cu:132-149
+    __device__ void run(const BatchDev& b, uint32_t chain) {
+        rng.seed(P->seed);
+        system_init();
+        uint32_t s0 = 0, acc_count = 0, nm = 0;
+        int32_t* rets = b.rets + (size_t)chain * b.steps;
+        uint64_t* rnga = b.rng_after ? b.rng_after + (size_t)chain * b.steps : nullptr;
+        double* Eo = b.E + (size_t)chain * b.cap_meas;
+        double* Xo = b.X + (size_t)chain * b.cap_meas;
+        for (uint32_t s = 0; s < b.steps; ++s) {
+            const bool take = (s > s0 + (uint32_t)b.thermalization) && ((s - s0) % b.reset_each == 0u);
+            double Ed = 0.0, Xv = 0.0;
+            const int ret = metro_step(take, b.max_reverse, &Ed, &Xv);
+            if (ret == 2 || ret == 4) {
+                if (lane == 0 && nm < b.cap_meas) {
+                    Eo[nm] = Ed;
+                    Xo[nm] = Xv;
+                }
+                nm++;
+            }
+            if (ret < 0) {
+                system_init();
+                s0 = s + 1;
+            }
+            if (ret == 1 || ret == 2) acc_count++;
+            if (lane == 0) {
+                rets[s] = ret;
+                if (rnga) rnga[s] = rng.state;
+            }
+        }
+        if (lane == 0) {
+            b.n_meas[chain] = nm;
+            b.accepted[chain] = acc_count;
+        }
+    }
+};
+
+__global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
+    extern __shared__ double lds[];
+    const uint32_t lane = threadIdx.x;
+    for (;;) {
+        uint32_t chain = 0;
+        if (lane == 0) chain = atomicAdd(b.next_chain, 1u);
+        chain = __builtin_amdgcn_readfirstlane(__shfl(chain, 0, 64));
+        if (chain >= b.nchains) return;   // every wave reaches this once the queue is drained
+        Chain c;
+        c.P = &b.chains[chain];
+        c.r = c.P->r;
+        c.n = c.r + 4;
+        c.N = 1u << c.n;
+        c.L = 1u << c.r;
+        c.acc = c.n - 1;
+        c.lane = lane;
+        c.re = lds;
+        c.im = lds + c.N;
+        c.run(b, chain);
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
+// ----------------------------------------------------------------- host
+void fill_chain(const suqa_qms_chain& in, const suqa_qms_batch_params& p, ChainDev& d, std::vector<double>& wtab) {
+    // qms.cu:262-276 and qms.cuh:96-103
+    const uint32_t r = in.ene_qbits;
+    const uint32_t L = 1u << r;
+    const double t_PE_shift = p.ene_min;
+    const double t_PE_factor = (L - 1) / (double)(L * (p.ene_max - p.ene_min));
+    const double t = t_PE_factor * 8. * atan(1.0);
+    const double c_factor = in.beta / (t_PE_factor * L);
+    std::memset(&d, 0, sizeof(d));
+    d.seed = in.seed;
+    d.r = r;
+    for (uint32_t trg = 0; trg < r; ++trg) {
+        const double powr = (double)(1U << (r - 1 - trg));
+        // Phi: cevolution(-powr*t) -> pauli(XX, -(-powr*t)); u1(-powr*t*Emin)
+        const double tp = -powr * t;
+        sincos(-tp, &d.xx_s[0][trg], &d.xx_c[0][trg]);
+        sincos(-powr * t * t_PE_shift, &d.u1_s[0][trg], &d.u1_c[0][trg]);
+        // Phi^dag: cevolution(powr*t) -> pauli(XX, -(powr*t)); u1(powr*t*Emin)
+        const double ti = powr * t;
+        sincos(-ti, &d.xx_s[1][trg], &d.xx_c[1][trg]);
+        sincos(powr * t * t_PE_shift, &d.u1_s[1][trg], &d.u1_c[1][trg]);
+    }
+    // crm phases of the QFT: rphase_m[m] = 2 pi / 2^m (suqa.cu:611-633)
+    uint32_t cc = 1;
+    for (uint32_t m = 0; m < (uint32_t)kMaxEne + 2; ++m) {
+        const double ph = (2.0 * M_PI / (double)cc);
+        double neg = ph;
+        neg *= -1;
+        sincos(ph, &d.qft_s[0][m], &d.qft_c[0][m]);
+        sincos(neg, &d.qft_s[1][m], &d.qft_c[1][m]);
+        cc <<= 1;
+    }
+    sincos(M_PI * 0.5, &d.hp_s, &d.hp_c);
+    for (uint32_t k = 0; k < L; ++k) d.etab[k] = t_PE_shift + k / (double)(t_PE_factor * L);
+    wtab.resize(2 * (size_t)L * L);
+    for (uint32_t eo = 0; eo < L; ++eo)
+        for (uint32_t en = 0; en < L; ++en) {
+            const double dE = (en * c_factor - eo * c_factor);
+            double fs1 = 1.0, fs2 = 0.0;
+            if (dE > 0) {
+                fs1 = exp(-dE / 2.0);
+                fs2 = sqrt(1.0 - fs1 * fs1);
+            }
+            wtab[2 * ((size_t)eo * L + en)] = fs1;
+            wtab[2 * ((size_t)eo * L + en) + 1] = fs2;
+        }
+}
+
+std::string g_err;
+
+}  // namespace
+}  // namespace qb
+
+#define QB_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            fprintf(stderr, "suqa_qms_batch_run: %s: %s\n", #call, hipGetErrorString(e_));  \
+            rc = SUQA_ERR_HIP;                                                              \
+            goto done;                                                                      \
+        }                                                                                   \
+    } while (0)
+
+extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains, const suqa_qms_batch_params* p,
+                                  suqa_qms_batch_result* out) {
+    using namespace qb;
+    if (!chains || !p || !out || nchains == 0 || !out->rets || !out->n_meas || !out->accepted) return SUQA_ERR_ARG;
+    if (p->metro_steps == 0 || p->reset_each == 0 || p->max_reverse == 0 || p->pe_steps <= 0) return SUQA_ERR_ARG;
+    if (out->cap_meas && (!out->E || !out->X)) return SUQA_ERR_ARG;
+    for (uint32_t c = 0; c < nchains; ++c)
+        if (chains[c].ene_qbits < 2 || chains[c].ene_qbits > (uint32_t)kMaxEne || chains[c].seed == 0 ||
+            !(chains[c].beta > 0.0))
+            return SUQA_ERR_ARG;
+    int rc = SUQA_OK;
+    std::vector<ChainDev> hc(nchains);
+    std::vector<std::vector<double>> wt(nchains);
+    size_t wtotal = 0;
+    for (uint32_t c = 0; c < nchains; ++c) {
+        fill_chain(chains[c], *p, hc[c], wt[c]);
+        wtotal += wt[c].size();
+    }
+    ChainDev* d_chains = nullptr;
+    double *d_w = nullptr, *d_E = nullptr, *d_X = nullptr;
+    int32_t* d_rets = nullptr;
+    uint64_t* d_rng = nullptr;
+    uint32_t *d_nm = nullptr, *d_acc = nullptr, *d_next = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const size_t nsteps = (size_t)nchains * p->metro_steps;
+    const size_t nmeas = (size_t)nchains * out->cap_meas;
+    BatchDev b{};
+    int ncu = 256;
+    uint32_t grid = 0;
+    size_t lds_bytes = 0;
+    float ms = 0.f;
+    QB_HIP(hipStreamCreate(&st));
+    QB_HIP(hipMalloc(&d_w, sizeof(double) * wtotal));
+    {
+        std::vector<double> flat;
+        flat.reserve(wtotal);
+        for (uint32_t c = 0; c < nchains; ++c) {
+            hc[c].wtab = d_w + flat.size();
+            flat.insert(flat.end(), wt[c].begin(), wt[c].end());
+        }
+        QB_HIP(hipMemcpy(d_w, flat.data(), sizeof(double) * flat.size(), hipMemcpyHostToDevice));
+    }
+    QB_HIP(hipMalloc(&d_chains, sizeof(ChainDev) * nchains));
+    QB_HIP(hipMemcpy(d_chains, hc.data(), sizeof(ChainDev) * nchains, hipMemcpyHostToDevice));
+    QB_HIP(hipMalloc(&d_rets, sizeof(int32_t) * nsteps));
+    if (out->rng_after) QB_HIP(hipMalloc(&d_rng, sizeof(uint64_t) * nsteps));
+    QB_HIP(hipMalloc(&d_E, sizeof(double) * (nmeas ? nmeas : 1)));
+    QB_HIP(hipMalloc(&d_X, sizeof(double) * (nmeas ? nmeas : 1)));
+    QB_HIP(hipMalloc(&d_nm, sizeof(uint32_t) * nchains));
+    QB_HIP(hipMalloc(&d_acc, sizeof(uint32_t) * nchains));
+    QB_HIP(hipMalloc(&d_next, sizeof(uint32_t)));
+    b.chains = d_chains;
+    b.nchains = nchains;
+    b.steps = p->metro_steps;
+    b.reset_each = p->reset_each;
+    b.max_reverse = p->max_reverse;
+    b.thermalization = p->thermalization;
+    b.cap_meas = out->cap_meas;
+    b.next_chain = d_next;
+    b.rets = d_rets;
+    b.rng_after = d_rng;
+    b.E = d_E;
+    b.X = d_X;
+    b.n_meas = d_nm;
+    b.accepted = d_acc;
+    {
+        int dev = 0;
+        QB_HIP(hipGetDevice(&dev));
+        QB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    // one wave per chain, LDS sized for the widest chain
+    for (uint32_t c = 0; c < nchains; ++c) {
+        const size_t need = sizeof(double) * 2 * ((size_t)1 << (chains[c].ene_qbits + 4));
+        if (need > lds_bytes) lds_bytes = need;
+    }
+    grid = nchains < (uint32_t)ncu * 16u ? nchains : (uint32_t)ncu * 16u;
+    QB_HIP(hipEventCreate(&e0));
+    QB_HIP(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
+    QB_HIP(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_qms_batch, dim3(grid), dim3(64), lds_bytes, st, b);
+    QB_HIP(hipGetLastError());
+    QB_HIP(hipEventRecord(e1, st));
+    QB_HIP(hipStreamSynchronize(st));
+    QB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    out->seconds = ms * 1e-3;
+    QB_HIP(hipMemcpy(out->rets, d_rets, sizeof(int32_t) * nsteps, hipMemcpyDeviceToHost));
+    if (out->rng_after) QB_HIP(hipMemcpy(out->rng_after, d_rng, sizeof(uint64_t) * nsteps, hipMemcpyDeviceToHost));
+    if (nmeas) {
+        QB_HIP(hipMemcpy(out->E, d_E, sizeof(double) * nmeas, hipMemcpyDeviceToHost));
+        QB_HIP(hipMemcpy(out->X, d_X, sizeof(double) * nmeas, hipMemcpyDeviceToHost));
+    }
+    QB_HIP(hipMemcpy(out->n_meas, d_nm, sizeof(uint32_t) * nchains, hipMemcpyDeviceToHost));
+    QB_HIP(hipMemcpy(out->accepted, d_acc, sizeof(uint32_t) * nchains, hipMemcpyDeviceToHost));
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamSynchronize(st);
+    (void)hipFree(d_chains);
+    (void)hipFree(d_w);
+    hipFree(d_E);
+    hipFree(d_X);
+    (void)hipFree(d_rets);
+    (void)hipFree(d_rng);
+    (void)hipFree(d_nm);
+    (void)hipFree(d_acc);
+    (void)hipFree(d_next);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+}

@@ -1,0 +1,210 @@
+// qms.cpp -- qms:: on the MI355X engine.  Same control flow and random-draw
+// order as qms.cuh:373-545, so a run with a fixed pcg32 seed makes the same
+// accept/reject decisions as the reference CPU path.
+#include "qms.hpp"
+
+#include <stdexcept>
+
+#include "system.hpp"
+
+namespace qms {
+
+uint syst_qbits, syst_levels, ene_qbits, ene_levels, nqubits, Dim;
+uint max_reverse_attempts, metro_steps, reset_each;
+unsigned long long iseed = 0ULL;
+double t_phase_estimation, t_PE_factor, t_PE_shift;
+int n_phase_estimation;
+uint gCi, c_acc = 0;
+bool record_reverse = false;
+std::vector<uint> reverse_counters;
+pcg rangen(1ULL);
+std::vector<double> X_measures, E_measures;
+double c_factor;
+std::vector<uint> bm_syst, bm_enes_new;
+uint bm_acc;
+uint W_mask, W_mask_Enew;
+uint curr_E_old;
+double curr_E_old_d;
+
+void fill_bitmap() {
+    bm_syst.resize(syst_qbits);
+    bm_enes_new.resize(ene_qbits);
+    uint c = 0;
+    for (uint i = 0; i < syst_qbits; ++i) bm_syst[i] = c++;
+    for (uint i = 0; i < ene_qbits; ++i) bm_enes_new[i] = c++;
+    bm_acc = c;
+}
+
+void fill_W_utils(double beta, double t_PE_factor_) {
+    c_factor = beta / (t_PE_factor_ * ene_levels);
+    W_mask_Enew = 0U;
+    for (uint i = 0; i < ene_qbits; ++i) W_mask_Enew |= (1U << bm_enes_new[i]);
+    W_mask = (1U << bm_acc) | W_mask_Enew;
+}
+
+void cevolution(const double& t, const int& n, const uint& q_control, const bmReg& qsyst) {
+    if (qsyst.size() != syst_qbits)
+        throw std::runtime_error("ERROR: controlled evolution has wrong number of state qbits");
+    suqa::activate_gc_mask({q_control});
+    evolution(t, n);
+    suqa::deactivate_gc_mask({q_control});
+}
+
+uint creg_to_uint(const std::vector<uint>& c_reg) {
+    if (c_reg.size() < 1) throw std::runtime_error("ERROR: size of register zero.");
+    uint ret = c_reg[0];
+    for (uint j = 1U; j < c_reg.size(); ++j) ret += c_reg[j] << j;
+    return ret;
+}
+
+void reset_non_syst_qbits() {
+    std::vector<double> rgenerates(ene_qbits);
+    for (auto& el : rgenerates) el = rangen.doub();
+    suqa::apply_reset(bm_enes_new, rgenerates);
+    suqa::apply_reset(bm_acc, rangen.doub());
+}
+
+void apply_phase_estimation(const std::vector<uint>& q_syst, const std::vector<uint>& q_target, const double& t,
+                            const uint& n) {
+    suqa::apply_h(q_target);
+    for (int trg = (int)q_target.size() - 1; trg > -1; --trg) {
+        double powr = (double)(1U << (q_target.size() - 1 - trg));
+        cevolution(-powr * t, powr * n, q_target[trg], q_syst);
+        suqa::apply_u1(q_target[trg], -powr * t * t_PE_shift);
+    }
+    suqa::apply_qft_inverse(q_target);
+}
+
+void apply_phase_estimation_inverse(const std::vector<uint>& q_syst, const std::vector<uint>& q_target,
+                                    const double& t, const uint& n) {
+    suqa::apply_qft(q_target);
+    for (uint trg = 0; trg < q_target.size(); ++trg) {
+        double powr = (double)(1U << (q_target.size() - 1 - trg));
+        cevolution(powr * t, powr * n, q_target[trg], q_syst);
+        suqa::apply_u1(q_target[trg], powr * t * t_PE_shift);
+    }
+    suqa::apply_h(q_target);
+}
+
+void apply_Phi() { apply_phase_estimation(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation); }
+void apply_Phi_inverse() {
+    apply_phase_estimation_inverse(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation);
+}
+
+uint draw_C() {
+    std::vector<double> C_weigthsums = get_C_weigthsums();
+    double extract = rangen.doub();
+    for (uint Ci = 0U; Ci < C_weigthsums.size(); ++Ci)
+        if (extract < C_weigthsums[Ci]) return Ci;
+    return (uint)C_weigthsums.size();
+}
+
+// qms.cuh:312-329: one engine kernel; exp/sqrt of the acceptance rotation are
+// evaluated on the host with libm (as the reference CPU path does).
+void apply_W() {
+    int rc = suqa_hip_qms_apply_W(suqa::ctx, bm_acc, W_mask_Enew, bm_enes_new[0], c_factor, curr_E_old);
+    if (rc != SUQA_OK) throw std::runtime_error(std::string("ERROR: in qms::apply_W(): ") + suqa_hip_last_error());
+}
+
+void apply_W_inverse() { apply_W(); }
+
+void apply_U() {
+    apply_C(gCi);
+    apply_Phi();
+    apply_W();
+}
+
+void apply_U_inverse() {
+    apply_W_inverse();
+    apply_Phi_inverse();
+    apply_C_inverse(gCi);
+}
+
+std::vector<double> extract_rands(uint n) {
+    std::vector<double> ret(n);
+    for (auto& el : ret) el = rangen.doub();
+    return ret;
+}
+
+int metro_step(bool take_measure) {
+    // 1/2: accepted (not measured/measured), 3/4: rejected and restored,
+    // -1: rejected and not restored within max_reverse_attempts.
+    int ret = 0;
+    reset_non_syst_qbits();
+    apply_Phi();
+    std::vector<uint> c_E_news(ene_qbits, 0), c_E_olds(ene_qbits, 0);
+    uint Enew_meas;
+    suqa::measure_qbits(bm_enes_new, c_E_olds, extract_rands(ene_qbits));
+
+    curr_E_old = creg_to_uint(c_E_olds);
+    curr_E_old_d = t_PE_shift + curr_E_old / (double)(t_PE_factor * ene_levels);
+    for (uint ei = 0U; ei < ene_qbits; ++ei)
+        if (c_E_olds[ei]) suqa::apply_x(bm_enes_new[ei]);
+
+    gCi = draw_C();
+    apply_U();
+
+    suqa::measure_qbit(bm_acc, c_acc, rangen.doub());
+
+    if (c_acc == 1U) {
+        double Enew_meas_d;
+        suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
+        apply_Phi_inverse();
+        if (take_measure) {
+            Enew_meas_d = t_PE_shift + creg_to_uint(c_E_news) / (double)(t_PE_factor * ene_levels);
+            E_measures.push_back(Enew_meas_d);
+            for (uint ei = 0U; ei < ene_qbits; ++ei)
+                if (c_E_news[ei]) suqa::apply_x(bm_enes_new[ei]);
+            X_measures.push_back(measure_X(rangen));
+            for (uint ei = 0U; ei < ene_qbits; ++ei) suqa::apply_reset(bm_enes_new[ei], rangen.doub());
+            apply_Phi();
+            suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
+            apply_Phi_inverse();
+            ret = 2;
+        } else {
+            ret = 1;
+        }
+        return ret;
+    }
+
+    apply_U_inverse();
+    uint iters = 0;
+    while (iters < max_reverse_attempts) {
+        apply_Phi();
+        suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
+        Enew_meas = creg_to_uint(c_E_news);
+        apply_Phi_inverse();
+        if (curr_E_old == Enew_meas) {
+            if (take_measure) {
+                E_measures.push_back(curr_E_old_d);
+                for (uint ei = 0U; ei < ene_qbits; ++ei) suqa::apply_reset(bm_enes_new[ei], rangen.doub());
+                X_measures.push_back(measure_X(rangen));
+                for (uint ei = 0U; ei < ene_qbits; ++ei) suqa::apply_reset(bm_enes_new[ei], rangen.doub());
+                apply_Phi();
+                suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
+                apply_Phi_inverse();
+                ret = 4;
+            } else {
+                ret = 3;
+            }
+            break;
+        }
+        uint c_acc_trash;
+        apply_U();
+        suqa::measure_qbit(bm_acc, c_acc_trash, rangen.doub());
+        apply_U_inverse();
+        iters++;
+    }
+    if (record_reverse) reverse_counters.push_back(iters);
+    if (iters == max_reverse_attempts) return -1;
+    return ret;
+}
+
+void setup(double beta) {
+    fill_bitmap();
+    fill_W_utils(beta, t_PE_factor);
+}
+
+void clear() {}
+
+}  // namespace qms

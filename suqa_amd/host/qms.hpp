@@ -1,0 +1,50 @@
+// qms.hpp -- the reference's qms:: namespace (include/qms.cuh:30-566): the
+// Metropolis step of quantum Metropolis sampling on the suqa:: API, master
+// branch layout (one energy register).  Same globals, same functions.
+#pragma once
+#include <vector>
+
+#include "pcg.hpp"
+#include "suqa.hpp"
+
+namespace qms {
+
+extern uint syst_qbits, syst_levels, ene_qbits, ene_levels, nqubits, Dim;
+extern uint max_reverse_attempts, metro_steps, reset_each;
+extern unsigned long long iseed;
+extern double t_phase_estimation, t_PE_factor, t_PE_shift;
+extern int n_phase_estimation;
+extern uint gCi, c_acc;
+extern bool record_reverse;
+extern std::vector<uint> reverse_counters;
+extern pcg rangen;
+extern std::vector<double> X_measures, E_measures;
+extern double c_factor;
+extern std::vector<uint> bm_syst, bm_enes_new;
+extern uint bm_acc;
+extern uint W_mask, W_mask_Enew;
+extern uint curr_E_old;
+extern double curr_E_old_d;
+
+void fill_bitmap();                                                    // qms.cuh:79-88
+void fill_W_utils(double beta, double t_PE_factor);                    // qms.cuh:96-103
+void cevolution(const double& t, const int& n, const uint& q_control, const bmReg& qsyst);
+uint creg_to_uint(const std::vector<uint>& c_reg);
+void reset_non_syst_qbits();
+void apply_phase_estimation(const std::vector<uint>& q_syst, const std::vector<uint>& q_target, const double& t,
+                            const uint& n);
+void apply_phase_estimation_inverse(const std::vector<uint>& q_syst, const std::vector<uint>& q_target,
+                                    const double& t, const uint& n);
+void apply_Phi();
+void apply_Phi_inverse();
+uint draw_C();
+void apply_W();
+void apply_W_inverse();
+void apply_U();
+void apply_U_inverse();
+std::vector<double> extract_rands(uint n);
+int metro_step(bool take_measure);                                     // qms.cuh:373-545
+void setup(double beta);
+void clear();
+
+}  // namespace qms

@@ -1,0 +1,58 @@
+// system.cpp -- frustrated-triangle plug-in on the suqa:: API (system.cu).
+#include "system.hpp"
+
+#include <cmath>
+#include <stdexcept>
+
+void init_state() {   // an E = -1 eigenstate: CX(1->0) H(1) X(1) |000>
+    suqa::init_state();
+    suqa::apply_x(bm_spin[1]);
+    suqa::apply_h(bm_spin[1]);
+    suqa::apply_cx(bm_spin[1], bm_spin[0]);
+}
+
+static void exp_it_id_x_x(const bmReg& q, uint pos_id, double phase_t) {
+    suqa::apply_pauli_TP_rotation({q[(pos_id + 1) % 3], q[(pos_id + 2) % 3]}, {PAULI_X, PAULI_X}, phase_t);
+}
+
+void evolution(const double& t, const int& n) {   // the XX terms commute: exact for any n
+    (void)n;
+    for (uint iii = 0; iii < 3; ++iii) exp_it_id_x_x(bm_spin, iii, -t);
+}
+
+// Observable: basis change, measure the three spins, map to {2,0,2,0,-2,0,-2,0}
+// (system.cu:70-119).  The "antirotation" applies the rotation again.
+static const uint op_bits = 3;
+static const std::vector<double> op_vals = {2.0, 0.0, 2.0, 0.0, -2.0, 0.0, -2.0, 0.0};
+
+static void apply_measure_rotation() {
+    suqa::apply_h(bm_spin[0]);
+    suqa::apply_h(bm_spin[1]);
+    suqa::apply_h(bm_spin[2]);
+    suqa::apply_cx(bm_spin[0], bm_spin[1]);
+    suqa::apply_cx(bm_spin[0], bm_spin[2]);
+    suqa::apply_u1(bm_spin[2], M_PI * 0.5);
+}
+
+double measure_X(pcg& rgen) {
+    std::vector<uint> classics(op_bits);
+    apply_measure_rotation();
+    std::vector<double> rdoubs(op_bits);
+    for (auto& el : rdoubs) el = rgen.doub();
+    suqa::measure_qbits(bm_spin, classics, rdoubs);
+    apply_measure_rotation();
+    uint meas = 0U;
+    for (uint i = 0; i < op_bits; ++i) meas |= (classics[i] << i);
+    return op_vals[meas];
+}
+
+static const std::vector<double> C_weigthsums = {1. / 3, 2. / 3, 1.0};
+
+void apply_C(const uint& Ci) {
+    if (Ci > 2) throw std::runtime_error("ERROR: wrong move selection");
+    suqa::apply_h(bm_spin[Ci]);
+}
+
+void apply_C_inverse(const uint& Ci) { apply_C(Ci); }
+
+std::vector<double> get_C_weigthsums() { return C_weigthsums; }

@@ -35,6 +35,15 @@ def test_header_matches_exports(built):
     assert decl == exported(built, "suqa_hip_")
 
 
+def test_qms_header_matches_exports(built):
+    lib = os.path.join(ROOT, "suqa_amd", "lib", "libsuqa.so")
+    decl = declared("suqa_qms.h", "suqa_qms_")
+    assert decl == {"suqa_qms_run", "suqa_qms_batch_run"}
+    assert decl == exported(lib, "suqa_qms_")
+    from suqa_amd import qms
+    assert set(qms.SIGNATURES) == decl
+
+
 def test_binding_covers_header(built):
     from suqa_amd import capi
     assert set(capi.SIGNATURES) == declared("suqa_hip.h", "suqa_hip_")

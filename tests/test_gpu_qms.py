@@ -1,0 +1,101 @@
+"""QMS parity on the GPU: the plain FP64 path (suqa:: / qms:: C++ mirror over
+the engine) and the batched-chain kernel reproduce the reference's per-step
+accept/reject codes, pcg32 states and E/X measurements bit for bit."""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.common import QMS_FIXTURES, assert_bitwise, golden_state, hexarr, load, qms_params
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def qms():
+    from suqa_amd import qms
+    return qms
+
+
+def log(m):
+    print(m, flush=True)
+
+
+@pytest.mark.parametrize("fx", QMS_FIXTURES)
+def test_plain_qms_vs_golden(qms, fx):
+    d = load(fx)
+    p = qms_params(d)
+    log(f"plain QMS {fx}")
+    out = qms.run(p["beta"], p["steps"], p["reset_each"], p["r"], p["seed"], p["ene_min"], p["ene_max"],
+                  p["pe_steps"], p["thermalization"], p["max_reverse"], p["record_reverse"], final_state=True)
+    assert list(out["rets"]) == d["rets"]
+    assert [int(x) for x in out["rng_after"]] == [int(x) for x in d["rng_after"]]
+    assert out["accepted"] == d["accepted"]
+    assert_bitwise(out["E"], hexarr(d["E"]), "E")
+    assert_bitwise(out["X"], hexarr(d["X"]), "X")
+    if p["record_reverse"]:
+        assert list(out["reverse"]) == d["reverse_counters"]
+    re, im = golden_state(d["final_state"])
+    np.testing.assert_allclose(out["final"][0], re, atol=1e-12, rtol=0)
+    np.testing.assert_allclose(out["final"][1], im, atol=1e-12, rtol=0)
+
+
+def test_driver_output_file_matches_reference(qms):
+    log("suqa_qms driver file")
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "out.txt")
+        r = subprocess.run([qms.DRIVER, "1.0", "1000", "1", "4", out, "--seed", "42", "--ene-min", "-1", "--ene-max",
+                            "3", "--PE-steps", "10"], capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr
+        assert "acceptance" in r.stdout
+        with open(out, "rb") as f, open(os.path.join(ROOT, "tests", "golden", "qms_c1_out.txt"), "rb") as g:
+            assert f.read() == g.read()
+
+
+@pytest.mark.parametrize("fx", QMS_FIXTURES)
+def test_batch_single_chain_vs_golden(qms, fx):
+    d = load(fx)
+    p = qms_params(d)
+    log(f"batch QMS {fx}")
+    out = qms.batch_run([(p["beta"], p["r"], p["seed"])], p["steps"], p["reset_each"], p["max_reverse"], p["ene_min"],
+                        p["ene_max"], p["pe_steps"], p["thermalization"], rng_after=True)
+    assert list(out["rets"][0]) == d["rets"]
+    assert [int(x) for x in out["rng_after"][0]] == [int(x) for x in d["rng_after"]]
+    assert int(out["accepted"][0]) == d["accepted"]
+    nm = int(out["n_meas"][0])
+    assert nm == len(d["E"])
+    assert_bitwise(out["E"][0][:nm], hexarr(d["E"]), "E")
+    assert_bitwise(out["X"][0][:nm], hexarr(d["X"]), "X")
+
+
+def test_batch_mixed_chains_vs_oracle(qms):
+    """48 chains with register widths 2..6, betas and seeds all different, in one launch."""
+    rng = np.random.default_rng(9)
+    chains = [(float(0.1 + 4.9 * k / 47), int(2 + k % 5), int(1000 + k)) for k in range(48)]
+    rng.shuffle(chains)
+    steps, reset_each, therm, maxrev, emin, emax = 120, 2, 10, 30, -1.0, 3.0
+    log("batch mixed chains")
+    out = qms.batch_run(chains, steps, reset_each, maxrev, emin, emax, 10, therm, rng_after=True)
+    for c, (beta, r, seed) in enumerate(chains):
+        q = po.Qms(beta, r, seed, emin, emax, 10, maxrev)
+        L = po.lib()
+        L.so_qms_init_state(q.p)
+        s0 = 0
+        for s in range(steps):
+            take = (s > s0 + therm) and ((s - s0) % reset_each == 0)
+            ret = L.so_qms_metro_step(q.p, 1 if take else 0)
+            if ret < 0:
+                L.so_qms_init_state(q.p)
+                s0 = s + 1
+            assert out["rets"][c][s] == ret, f"chain {c} (beta={beta}, r={r}, seed={seed}) step {s}"
+            assert int(out["rng_after"][c][s]) == q.rng_state()
+        E, X = q.measures()
+        nm = int(out["n_meas"][c])
+        assert nm == len(E)
+        assert_bitwise(out["E"][c][:nm], np.array(E), f"E chain {c}")
+        assert_bitwise(out["X"][c][:nm], np.array(X), f"X chain {c}")
