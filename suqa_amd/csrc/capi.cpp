@@ -95,20 +95,21 @@ int check_qubits(const suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_t
     return SUQA_OK;
 }
 
-// crm of the QFT (suqa.cu:620-633): the phase table is filled by the first call.
+// crm of the QFT (suqa.cu:620-633).  The reference fills its phase table once,
+// sized by the first register; entry m is 2 pi / 2^m whatever the size, so a
+// larger register later extends the table instead of reading past its end.
 int qft_crm(suqa_hip_ctx* c, uint32_t qc, uint32_t qt, int m, int nqubits, uint32_t gc) {
-    if (!c->rphase_filled) {
-        c->rphase.assign((size_t)nqubits + 1, 0.0);
+    const size_t idx = (size_t)(m > 0 ? m : -m);
+    if (!c->rphase_filled || idx >= c->rphase.size()) {
+        const size_t sz = std::max((size_t)nqubits, idx) + 1;
+        c->rphase.assign(sz, 0.0);
         uint32_t cc = 1;
-        for (int i = 0; i < nqubits + 1; ++i) {
+        for (size_t i = 0; i < sz; ++i) {
             c->rphase[i] = (2.0 * M_PI / (double)cc);
             cc <<= 1;
         }
         c->rphase_filled = true;
     }
-    const size_t idx = (size_t)(m > 0 ? m : -m);
-    if (idx >= c->rphase.size())
-        return fail(SUQA_ERR_ARG, "QFT register larger than the first one (phase table filled once)");
     double rphase = c->rphase[idx];
     if (m <= 0) rphase *= -1;
     const uint32_t one = 1u;

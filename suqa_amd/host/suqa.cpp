@@ -155,7 +155,10 @@ void suqa::apply_pauli_TP_rotation_general(const bmReg& q_apply, const std::vect
 void suqa::apply_rx(uint q, double theta) { suqa::apply_pauli_TP_rotation({q}, {PAULI_X}, -theta / 2.0); }
 void suqa::apply_rz(uint q, double theta) { suqa::apply_pauli_TP_rotation({q}, {PAULI_Z}, -theta / 2.0); }
 
-// QFT (suqa.cu:611-661): no final swaps; the crm phases are filled once.
+// QFT (suqa.cu:611-661): no final swaps.  The crm phase table is filled by
+// the first call (suqa.cu:621-625); entry m is 2 pi / 2^m whatever the fill
+// size, so a later, larger register extends the table (where the reference
+// would read past its end) and every entry the reference defines is unchanged.
 static void fill_rphase(const uint& nqubits) {
     suqa::rphase_m.resize(nqubits + 1);
     uint c = 1;
@@ -166,14 +169,9 @@ static void fill_rphase(const uint& nqubits) {
 }
 
 static void qft_crm(const uint& q_control, const uint& q_target, const int& m, const int& nqubits) {
-    static bool activated = false;
-    if (!activated) {
-        fill_rphase(nqubits);
-        activated = true;
-    }
     const size_t idx = (size_t)(m > 0 ? m : -m);
-    if (idx >= suqa::rphase_m.size())
-        throw std::runtime_error("ERROR: QFT register larger than the first one (phase table filled once)");
+    if (suqa::rphase_m.empty() || idx >= suqa::rphase_m.size())
+        fill_rphase((uint)std::max<size_t>((size_t)nqubits, idx));
     double rphase = suqa::rphase_m[idx];
     if (m <= 0) rphase *= -1;
     suqa::apply_cu1(q_control, q_target, rphase);
