@@ -14,7 +14,18 @@ algorithmic bytes are 32 B per amplitude it modifies (H/X/RX/RZ 32N,
 U1/CX 16N, MCX3 4N).  roofline: the dominant kernel family, its algorithmic
 bytes over the sum of its per-launch HIP-event durations, against the
 8 TB/s HBM3E peak.  cpu_baseline: the oracle (a single-threaded restatement
-of the reference CPU path) on the first layer at 28 qubits.
+of the reference CPU path) on the first layer at 27 qubits.
+
+The QMS half of the metric is reported under "extra":
+  qms_plain  -- the reference's frustrated-triangle QMS (beta 1, 1000 steps,
+                4-qubit energy register, E in [-1,3], pcg32 seed 42) on the
+                plain FP64 GPU path (suqa_qms_run), Metropolis steps/s, with
+                its accept/reject sequence checked against the oracle;
+  qms_batch  -- 4096 independent chains (register widths 3..6 drawn with
+                pcg32 seed 5, betas 0.1..5.0, seeds 1000+i, 500 steps) in one
+                persistent kernel, chain-steps/s, a sample of chains checked
+                against the oracle;
+  cpu_baseline_qms -- the oracle on the same single chain, 1 core.
 """
 import argparse
 import json
@@ -52,7 +63,7 @@ def hbm_copy_calibration(capi, eng, nbytes=4 << 30, reps=5):
         capi.lib().suqa_hip_device_free(b)
 
 
-def cpu_baseline(n=28):
+def cpu_baseline(n=27):
     """Oracle (reference CPU path restated, 1 thread) on layer 0 of the same circuit at n qubits."""
     from oracle import pyoracle as po
     from suqa_amd import workloads as wl
@@ -72,6 +83,47 @@ def cpu_baseline(n=28):
                        f"{dt:.1f} s on 1 host core (oracle/suqa_oracle.c, gcc -O2)")
 
 
+def qms_legs(no_cpu):
+    """Plain FP64 GPU QMS (c1), batched chains (c5), their oracle checks and CPU rates."""
+    from suqa_amd import qms, workloads as wl
+    beta, steps, r, seed, emin, emax = 1.0, 1000, 4, 42, -1.0, 3.0
+    plain = qms.run(beta, steps, 1, r, seed, emin, emax, 10, 100, 100)
+    res = {"qms_plain": {"steps_per_s": round(steps / plain["seconds"], 1), "steps": steps,
+                         "accepted": int(plain["accepted"]), "seconds": round(plain["seconds"], 4),
+                         "config": "beta=1.0, 1000 steps, r=4 (8 qubits), E in [-1,3], seed 42, reset_each 1"}}
+    log(f"qms plain: {steps / plain['seconds']:.1f} steps/s")
+    rg = wl.Pcg32(5)
+    nch, bsteps = 4096, 500
+    chains = [(0.1 + 4.9 * i / (nch - 1), 3 + rg.randint(0, 4), 1000 + i) for i in range(nch)]
+    b = qms.batch_run(chains, bsteps, 1, 100, emin, emax, 10, 100)
+    rate = nch * bsteps / b["seconds"]
+    res["qms_batch"] = {"chain_steps_per_s": round(rate, 0), "chains": nch, "steps": bsteps,
+                        "kernel_seconds": round(b["seconds"], 4),
+                        "speedup_vs_plain_fp64_gpu": round(rate / (steps / plain["seconds"]), 1),
+                        "mean_acceptance": round(float(b["accepted"].mean()) / bsteps, 4), "dtype": "f64"}
+    log(f"qms batch: {rate:.0f} chain-steps/s")
+    if no_cpu:
+        return res
+    from oracle import pyoracle as po
+    q = po.Qms(beta, r, seed, emin, emax, 10, 100)
+    t0 = time.perf_counter()
+    acc, rets = q.run(steps, 1, 100)
+    dt = time.perf_counter() - t0
+    res["qms_plain"]["rets_match_oracle"] = bool(list(plain["rets"]) == rets)
+    res["cpu_baseline_qms"] = {"value": round(steps / dt, 1), "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": "the same 1000-step chain on the oracle (oracle/suqa_oracle.c, 1 core)"}
+    ok, t0, nchk = True, time.perf_counter(), 12
+    for c in range(0, nch, nch // nchk):
+        bb, rr, ss = chains[c]
+        qq = po.Qms(bb, rr, ss, emin, emax, 10, 100)
+        _, orets = qq.run(bsteps, 1, 100)
+        ok &= list(b["rets"][c]) == orets
+    dt = time.perf_counter() - t0
+    res["qms_batch"]["sample_chains_match_oracle"] = bool(ok)
+    res["qms_batch"]["oracle_chain_steps_per_s_1core"] = round(nchk * bsteps / dt, 1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +131,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-qms", action="store_true", help="skip the QMS legs (profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU sharding is not part of "
@@ -122,7 +175,8 @@ def main():
     tm = time.perf_counter() - tm
     norm = eng.vnorm()
     eng.close()
-    cpu = None if a.no_cpu else cpu_baseline(min(n, 28))
+    cpu = None if a.no_cpu else cpu_baseline(min(n, 27))
+    qres = {} if a.no_qms else qms_legs(a.no_cpu)
     fams = {k: dict(launches=v["launches"], ms=round(v["ms"], 3),
                     gbs=round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)) for k, v in rep.items()}
     out = {
@@ -143,8 +197,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": fam},
         "cpu_baseline": cpu,
-        "extra": {"copy_kernel_gbs": round(peak_copy, 1), "kernel_families": fams,
-                  "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm},
+        "extra": dict({"copy_kernel_gbs": round(peak_copy, 1), "kernel_families": fams,
+                       "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm}, **qres),
     }
     print(json.dumps(out), flush=True)
 

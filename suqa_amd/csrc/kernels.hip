@@ -88,42 +88,57 @@ __device__ __forceinline__ void pauli_out(double ar, double ai, double br, doubl
     }
 }
 
-template <int OP, int VEC>
+__device__ __forceinline__ Vec<2> shx2(const Vec<2>& v, int m) {
+    Vec<2> r;
+    r.x[0] = __shfl_xor(v.x[0], m, 64);
+    r.x[1] = __shfl_xor(v.x[1], m, 64);
+    return r;
+}
+
+// Output of element j of component `im_lane` (0: re, 1: im) whose value is v,
+// given the partner k = j ^ X: ps = partner, same component; pc = partner,
+// other component.  a_side: j plays role a (the reference's lower index).
+template <int OP>
+__device__ __forceinline__ double pair_out(bool a_side, bool im_lane, double v, double ps, double pc, uint64_t k,
+                                           const PauliCoef& pc_) {
+    if (OP == OP_SWAP) return ps;
+    if (OP == OP_Y) {   // suqa_cpu.hpp:116-118
+        if (a_side) return im_lane ? pc * -1.0 : pc;   // re_a' = im_b, im_a' = re_b * -1
+        return im_lane ? pc : pc * -1.0;               // re_b' = im_a * -1, im_b' = re_a
+    }
+    if (OP == OP_H) return a_side ? kTwoSqInv * (v + ps) : kTwoSqInv * (ps - v);   // suqa_cpu.hpp:195-198
+    // OP_PAULI: psi_j' = c psi_j + base (-1)^{par(k&YZ)} s psi_k
+    const uint32_t par = (uint32_t)__popcll(k & pc_.yz) & 1u;
+    const double t = (pc_.base_neg ^ par) ? -pc_.s : pc_.s;
+    if (!pc_.base_imag) return v * pc_.c + ps * t;
+    return im_lane ? v * pc_.c + pc * t : v * pc_.c + pc * (-t);
+}
+
+// Strided mode: a wave serves 16 double2 items; lanes 0-15 load re_a,
+// 16-31 im_a, 32-47 re_b, 48-63 im_b (one 16-byte stream per lane, the
+// layout that runs at copy speed), and the partner values come over
+// lane ^ 32 (same component) and lane ^ 48 (other component).
+template <int OP>
 __global__ __launch_bounds__(kBlock) void k_pair(double* __restrict__ re, double* __restrict__ im,
                                                  const PairArgs a) {
-    const uint64_t it = item_id();
-    if (it >= a.nitems) return;
-    const uint64_t ia = expand(it * VEC, a.fb, a.fval);
-    const uint64_t ib = ia ^ a.xmask;
-    const Vec<VEC> ar = Vec<VEC>::load(re + ia), ai = Vec<VEC>::load(im + ia);
-    const Vec<VEC> br = Vec<VEC>::load(re + ib), bi = Vec<VEC>::load(im + ib);
-    Vec<VEC> oar, oai, obr, obi;
+    const uint64_t t = item_id();
+    const int lane = threadIdx.x & 63;
+    const uint64_t it = (t >> 6) * 16 + (uint64_t)(lane & 15);
+    const bool ok = it < a.nitems;
+    const bool b_side = lane >= 32, im_lane = (lane >> 4) & 1;
+    const uint64_t ia = expand((ok ? it : 0) * 2, a.fb, a.fval);
+    const uint64_t x = b_side ? ia ^ a.xmask : ia;
+    double* base = im_lane ? im : re;
+    const Vec<2> v = Vec<2>::load(base + x);
+    const Vec<2> ps = shx2(v, 32);
+    Vec<2> pc;
+    if (OP == OP_Y || OP == OP_PAULI) pc = shx2(v, 48);
+    else pc = ps;
+    Vec<2> o;
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) {
-        const double xr = ar.x[e], xi = ai.x[e], yr = br.x[e], yi = bi.x[e];
-        if (OP == OP_SWAP) {
-            oar.x[e] = yr; oai.x[e] = yi; obr.x[e] = xr; obi.x[e] = xi;
-        } else if (OP == OP_Y) {   // suqa_cpu.hpp:116-118
-            oar.x[e] = yi; oai.x[e] = yr * -1.0; obr.x[e] = xi * -1.0; obi.x[e] = xr;
-        } else if (OP == OP_H) {   // suqa_cpu.hpp:195-198
-            oar.x[e] = kTwoSqInv * (xr + yr);
-            oai.x[e] = kTwoSqInv * (xi + yi);
-            obr.x[e] = kTwoSqInv * (xr - yr);
-            obi.x[e] = kTwoSqInv * (xi - yi);
-        } else {                    // OP_PAULI
-            const uint32_t par = (uint32_t)__popcll((ia + (uint64_t)e) & a.pc.yz) & 1u;
-            const uint32_t neg_a = a.pc.base_neg ^ par ^ a.pc.ny_odd;
-            const uint32_t neg_b = a.pc.base_neg ^ par;
-            const double ta = neg_a ? -a.pc.s : a.pc.s;
-            const double tb = neg_b ? -a.pc.s : a.pc.s;
-            pauli_out(xr, xi, yr, yi, a.pc.c, ta, a.pc.base_imag, oar.x[e], oai.x[e]);
-            pauli_out(yr, yi, xr, xi, a.pc.c, tb, a.pc.base_imag, obr.x[e], obi.x[e]);
-        }
-    }
-    oar.store(re + ia);
-    oai.store(im + ia);
-    obr.store(re + ib);
-    obi.store(im + ib);
+    for (int e = 0; e < 2; ++e)
+        o.x[e] = pair_out<OP>(!b_side, im_lane, v.x[e], ps.x[e], pc.x[e], (x + (uint64_t)e) ^ a.xmask, a.pc);
+    if (ok) o.store(base + x);
 }
 
 // ---------------------------------------------------------------------------
@@ -157,71 +172,45 @@ __device__ __forceinline__ Vec<2> shfl_pick(const Vec<2>& v, int lanex, int ex) 
         a = __shfl_xor(a, lanex, 64);
         b = __shfl_xor(b, lanex, 64);
     }
+    // element ex of the partner pair
     r.x[0] = ex ? b : a;
     r.x[1] = ex ? a : b;
     return r;
 }
 
 // Output of element j (value x) given its partner k = j ^ X (value y).
-template <int OP>
-__device__ __forceinline__ void dl_out(uint64_t j, const PairDLArgs& a, double xr, double xi, double yr, double yi,
-                                       double& orr, double& oi) {
-    const bool inA = (j & a.M) == a.V;
-    const bool inB = ((j ^ a.X) & a.M) == a.V;
-    orr = xr;
-    oi = xi;
-    if (!(inA || inB)) return;
-    if (OP == OP_SWAP) {
-        orr = yr;
-        oi = yi;
-    } else if (OP == OP_Y) {            // suqa_cpu.hpp:116-118
-        if (inA) { orr = yi; oi = yr * -1.0; }
-        else { orr = yi * -1.0; oi = yr; }
-    } else if (OP == OP_H) {            // suqa_cpu.hpp:195-198
-        if (inA) { orr = kTwoSqInv * (xr + yr); oi = kTwoSqInv * (xi + yi); }
-        else { orr = kTwoSqInv * (yr - xr); oi = kTwoSqInv * (yi - xi); }
-    } else {                            // OP_PAULI: psi_j' = c psi_j + base (-1)^{par(k&YZ)} s psi_k
-        const uint32_t par = (uint32_t)__popcll((j ^ a.X) & a.pc.yz) & 1u;
-        const double t = (a.pc.base_neg ^ par) ? -a.pc.s : a.pc.s;
-        pauli_out(xr, xi, yr, yi, a.pc.c, t, a.pc.base_imag, orr, oi);
-    }
-}
-
+// Dense-low mode, stream-split: with a partner group (xhigh != 0) lanes
+// 0-15 / 16-31 / 32-47 / 48-63 load re_P / im_P / re_Q / im_Q, 16 double2
+// (two 16-amplitude groups) each; without one, lanes 0-31 / 32-63 load re / im
+// of 32 double2.  Partner of element j: same component at lane ^ side ^ lanex,
+// other component at lane ^ side ^ comp ^ lanex, element ^ ex.
 template <int OP>
 __global__ __launch_bounds__(kBlock) void k_pair_dl(double* __restrict__ re, double* __restrict__ im,
                                                     const PairDLArgs a) {
-    const uint64_t it0 = item_id();
-    const bool ok = it0 < a.nitems;
-    const uint64_t p = expand((ok ? it0 : 0) * 2, a.fb, a.fval);
+    const uint64_t t = item_id();
+    const int lane = threadIdx.x & 63;
     const bool two = a.xhigh != 0;
-    const uint64_t q = p ^ a.xhigh;
-    const Vec<2> pr = Vec<2>::load(re + p), pi = Vec<2>::load(im + p);
-    Vec<2> qr = pr, qi = pi;
-    if (two) {
-        qr = Vec<2>::load(re + q);
-        qi = Vec<2>::load(im + q);
-    }
-    // partners of p's elements live in q's registers of lane ^ lanex
-    const Vec<2> yr = shfl_pick(qr, a.lanex, a.ex), yi = shfl_pick(qi, a.lanex, a.ex);
-    Vec<2> opr, opi;
+    const int per = two ? 16 : 32;
+    const int SIDE = two ? 32 : 0, COMP = two ? 16 : 32;
+    const uint64_t it = (t >> 6) * (uint64_t)per + (uint64_t)(lane & (per - 1));
+    const bool ok = it < a.nitems;
+    const bool q_side = two && lane >= 32;
+    const bool im_lane = (lane & COMP) != 0;
+    const uint64_t p = expand((ok ? it : 0) * 2, a.fb, a.fval);
+    const uint64_t x = q_side ? p ^ a.xhigh : p;
+    double* base = im_lane ? im : re;
+    const Vec<2> v = Vec<2>::load(base + x);
+    const Vec<2> ps = shfl_pick(v, SIDE ^ a.lanex, a.ex);
+    const Vec<2> pc = shfl_pick(v, SIDE ^ COMP ^ a.lanex, a.ex);
+    Vec<2> o;
 #pragma unroll
-    for (int e = 0; e < 2; ++e)
-        dl_out<OP>(p + (uint64_t)e, a, pr.x[e], pi.x[e], yr.x[e], yi.x[e], opr.x[e], opi.x[e]);
-    if (two) {
-        const Vec<2> zr = shfl_pick(pr, a.lanex, a.ex), zi = shfl_pick(pi, a.lanex, a.ex);
-        Vec<2> oqr, oqi;
-#pragma unroll
-        for (int e = 0; e < 2; ++e)
-            dl_out<OP>(q + (uint64_t)e, a, qr.x[e], qi.x[e], zr.x[e], zi.x[e], oqr.x[e], oqi.x[e]);
-        if (ok) {
-            oqr.store(re + q);
-            oqi.store(im + q);
-        }
+    for (int e = 0; e < 2; ++e) {
+        const uint64_t j = x + (uint64_t)e;
+        const bool inA = (j & a.M) == a.V;
+        const bool inB = ((j ^ a.X) & a.M) == a.V;
+        o.x[e] = (inA || inB) ? pair_out<OP>(inA, im_lane, v.x[e], ps.x[e], pc.x[e], j ^ a.X, a.pc) : v.x[e];
     }
-    if (ok) {
-        opr.store(re + p);
-        opi.store(im + p);
-    }
+    if (ok) o.store(base + x);
 }
 
 struct PhaseArgs {
@@ -234,30 +223,31 @@ struct PhaseArgs {
 };
 
 // util_rotate2 (suqa_cpu.hpp:49-54): re' = re*c - im*s, im' = im*c + re*s.
-template <int VEC, bool DL>
+// Stream-split: a wave serves 32 double2 items, lanes 0-31 load re and
+// lanes 32-63 im; each lane gets the other component over lane ^ 32.
+template <bool DL>
 __global__ __launch_bounds__(kBlock) void k_phase(double* __restrict__ re, double* __restrict__ im,
                                                   const PhaseArgs a) {
-    const uint64_t it = item_id();
-    if (it >= a.nitems) return;
-    const uint64_t ia = expand(it * VEC, a.fb, a.fval);
-    const Vec<VEC> xr = Vec<VEC>::load(re + ia), xi = Vec<VEC>::load(im + ia);
-    Vec<VEC> orr, oi;
+    const uint64_t t = item_id();
+    const int lane = threadIdx.x & 63;
+    const uint64_t it = (t >> 6) * 32 + (uint64_t)(lane & 31);
+    const bool ok = it < a.nitems;
+    const bool im_lane = lane >= 32;
+    const uint64_t ia = expand((ok ? it : 0) * 2, a.fb, a.fval);
+    double* base = im_lane ? im : re;
+    const Vec<2> v = Vec<2>::load(base + ia);
+    const Vec<2> o = shx2(v, 32);
+    Vec<2> out;
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) {
+    for (int e = 0; e < 2; ++e) {
         const uint64_t j = ia + (uint64_t)e;
         const uint32_t par = (uint32_t)__popcll(j & a.zmask) & 1u;
         const double s = par ? -a.s : a.s;
-        const double r = xr.x[e], m = xi.x[e];
-        if (DL && (j & a.M) != a.V) {
-            orr.x[e] = r;
-            oi.x[e] = m;
-            continue;
-        }
-        orr.x[e] = r * a.c - m * s;
-        oi.x[e] = m * a.c + r * s;
+        // re' = re*c - im*s, im' = im*c + re*s
+        out.x[e] = im_lane ? v.x[e] * a.c + o.x[e] * s : v.x[e] * a.c - o.x[e] * s;
+        if (DL && (j & a.M) != a.V) out.x[e] = v.x[e];
     }
-    orr.store(re + ia);
-    oi.store(im + ia);
+    if (ok) out.store(base + ia);
 }
 
 struct PhaseListArgs {
@@ -295,20 +285,22 @@ struct NormArgs {
     uint64_t M, V;     // dense-low mode: element i counts iff (i & M) == V
 };
 
-template <int VEC, bool DL>
+// Stream-split like k_phase: each lane squares its own component.
+template <bool DL>
 __global__ __launch_bounds__(kBlock) void k_norm(const double* __restrict__ re, const double* __restrict__ im,
                                                  const NormArgs a, double* __restrict__ partials) {
     __shared__ double lds[2][kBlock / 64];
     double acc0 = 0.0, acc1 = 0.0;
-    const uint64_t it = item_id();
+    const uint64_t t = item_id();
+    const int lane = threadIdx.x & 63;
+    const uint64_t it = (t >> 6) * 32 + (uint64_t)(lane & 31);
     if (it < a.nitems) {
-        const uint64_t ia = expand(it * VEC, a.fb, a.fval);
-        const Vec<VEC> xr = Vec<VEC>::load(re + ia), xi = Vec<VEC>::load(im + ia);
+        const uint64_t ia = expand(it * 2, a.fb, a.fval);
+        const Vec<2> v = Vec<2>::load((lane >= 32 ? im : re) + ia);
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) {
+        for (int e = 0; e < 2; ++e) {
             const uint64_t j = ia + (uint64_t)e;
-            const double r = xr.x[e], m = xi.x[e];
-            const double p = (DL && (j & a.M) != a.V) ? 0.0 : r * r + m * m;
+            const double p = (DL && (j & a.M) != a.V) ? 0.0 : v.x[e] * v.x[e];
             if (a.split && (j & a.split)) acc1 += p;
             else acc0 += p;
         }
@@ -367,24 +359,20 @@ __global__ __launch_bounds__(kBlock) void k_sum_pairs(const double* __restrict__
 __global__ __launch_bounds__(kBlock) void k_collapse(double* __restrict__ re, double* __restrict__ im,
                                                      uint64_t nitems, uint64_t qbit, uint64_t keepbit,
                                                      double scale) {
-    const uint64_t it = item_id();
+    const uint64_t t = item_id();
+    const int lane = threadIdx.x & 63;
+    const uint64_t it = (t >> 6) * 32 + (uint64_t)(lane & 31);
     if (it >= nitems) return;
     const uint64_t i = it * 2;
-    Vec<2> r, m;
+    double* base = lane >= 32 ? im : re;
     const bool k0 = (i & qbit) == keepbit;
     const bool k1 = ((i + 1) & qbit) == keepbit;
-    if (k0 || k1) {
-        r = Vec<2>::load(re + i);
-        m = Vec<2>::load(im + i);
-    } else {
-        r.x[0] = r.x[1] = m.x[0] = m.x[1] = 0.0;
-    }
-    r.x[0] = k0 ? r.x[0] * scale : 0.0;
-    m.x[0] = k0 ? m.x[0] * scale : 0.0;
-    r.x[1] = k1 ? r.x[1] * scale : 0.0;
-    m.x[1] = k1 ? m.x[1] * scale : 0.0;
-    r.store(re + i);
-    m.store(im + i);
+    Vec<2> v;
+    if (k0 || k1) v = Vec<2>::load(base + i);
+    else v.x[0] = v.x[1] = 0.0;
+    v.x[0] = k0 ? v.x[0] * scale : 0.0;
+    v.x[1] = k1 ? v.x[1] * scale : 0.0;
+    v.store(base + i);
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale(double* __restrict__ data, uint64_t nitems, double f) {
@@ -438,7 +426,6 @@ inline uint32_t grid_for(uint64_t nitems) {
     return (uint32_t)(g ? g : 8);
 }
 
-inline bool bit0_free(const Sel& s) { return s.fb.n == 0 || s.fb.pos[0] != 0; }
 
 inline uint64_t sel_mask(const Sel& s) {
     uint64_t m = 0;
@@ -482,7 +469,8 @@ hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, 
         a.fb = it.fb;
         a.fval = it.fval;
         a.nitems = it.count / 2;
-        const uint32_t g = grid_for(a.nitems);
+        const uint64_t per = a.xhigh ? 16 : 32;
+        const uint32_t g = grid_for(((a.nitems + per - 1) / per) * 64);
         timing_begin(c, tag, alg_bytes);
 #define SQE_PAIR_DL(OPV) \
     case OPV: hipLaunchKernelGGL((k_pair_dl<OPV>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a); break;
@@ -502,17 +490,11 @@ hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, 
     a.fval = sel.fval;
     a.xmask = xmask;
     a.pc = pc;
-    const bool v2 = bit0_free(sel) && sel.count >= 2 && (xmask & 1ull) == 0ull;
-    a.nitems = v2 ? sel.count / 2 : sel.count;
-    const uint32_t g = grid_for(a.nitems);
+    a.nitems = sel.count / 2;   // bits 0-3 are free here: double2 items
+    const uint32_t g = grid_for(((a.nitems + 15) / 16) * 64);
     timing_begin(c, tag, alg_bytes);
-#define SQE_PAIR(OPV)                                                                             \
-    case OPV:                                                                                     \
-        if (v2) hipLaunchKernelGGL((k_pair<OPV, 2>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, \
-                                   c->d_im, a);                                                   \
-        else hipLaunchKernelGGL((k_pair<OPV, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re,    \
-                                c->d_im, a);                                                      \
-        break;
+#define SQE_PAIR(OPV) \
+    case OPV: hipLaunchKernelGGL((k_pair<OPV>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a); break;
     switch (op) {
         SQE_PAIR(OP_SWAP)
         SQE_PAIR(OP_Y)
@@ -538,14 +520,11 @@ hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, u
     if (dl) make_sel(c->nq, a.M & ~kLowMask, a.V & ~kLowMask, &it);
     a.fb = it.fb;
     a.fval = it.fval;
-    const bool v2 = it.count >= 2;
-    a.nitems = v2 ? it.count / 2 : it.count;
-    const uint32_t g = grid_for(a.nitems);
+    a.nitems = it.count / 2;   // N >= 2 and bit 0 free: double2 items
+    const uint32_t g = grid_for(((a.nitems + 31) / 32) * 64);
     timing_begin(c, tag, alg_bytes);
-    if (dl && v2) hipLaunchKernelGGL((k_phase<2, true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
-    else if (dl) hipLaunchKernelGGL((k_phase<1, true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
-    else if (v2) hipLaunchKernelGGL((k_phase<2, false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
-    else hipLaunchKernelGGL((k_phase<1, false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    if (dl) hipLaunchKernelGGL((k_phase<true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    else hipLaunchKernelGGL((k_phase<false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
     timing_end(c);
     return hipGetLastError();
 }
@@ -586,14 +565,11 @@ hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, cons
     if (dl) make_sel(c->nq, a.M & ~kLowMask, a.V & ~kLowMask, &it);
     a.fb = it.fb;
     a.fval = it.fval;
-    const bool v2 = it.count >= 2;
-    a.nitems = v2 ? it.count / 2 : it.count;
-    const uint32_t g = grid_for(a.nitems);
+    a.nitems = it.count / 2;
+    const uint32_t g = grid_for(((a.nitems + 31) / 32) * 64);
     timing_begin(c, tag, alg_bytes);
-    if (dl && v2) hipLaunchKernelGGL((k_norm<2, true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
-    else if (dl) hipLaunchKernelGGL((k_norm<1, true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
-    else if (v2) hipLaunchKernelGGL((k_norm<2, false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
-    else hipLaunchKernelGGL((k_norm<1, false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
+    if (dl) hipLaunchKernelGGL((k_norm<true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
+    else hipLaunchKernelGGL((k_norm<false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
     timing_end(c);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -618,8 +594,8 @@ hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double
     const uint64_t nitems = c->N / 2;
     // the discarded half is written without being read: 24 B per amplitude
     timing_begin(c, "collapse", 24.0 * (double)c->N);
-    hipLaunchKernelGGL(k_collapse, dim3(grid_for(nitems)), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, nitems,
-                       qbit, keep ? qbit : 0ull, scale);
+    hipLaunchKernelGGL(k_collapse, dim3(grid_for(((nitems + 31) / 32) * 64)), dim3(kBlock), 0, c->stream, c->d_re,
+                       c->d_im, nitems, qbit, keep ? qbit : 0ull, scale);
     timing_end(c);
     return hipGetLastError();
 }

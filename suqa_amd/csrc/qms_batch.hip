@@ -579,8 +579,8 @@ done:
     if (st) (void)hipStreamSynchronize(st);
     (void)hipFree(d_chains);
     (void)hipFree(d_w);
-    hipFree(d_E);
-    hipFree(d_X);
+    (void)hipFree(d_E);
+    (void)hipFree(d_X);
     (void)hipFree(d_rets);
     (void)hipFree(d_rng);
     (void)hipFree(d_nm);
