@@ -37,6 +37,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes moved per launch relative to the algorithmic bytes, per kernel family, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) on this workload, scaled by the counts a
+# 4 GiB copy kernel gives (reads x2.000, writes x1.000): profiles/gate_sweep_n30_fp64.md.
+PMC_TRAFFIC_RATIO = {"h_pair": 1.000, "pauli_diag": 1.000, "pauli_pair": 1.000, "phase": 1.154,
+                     "swap_pair": 1.023}
 
 
 def log(msg):
@@ -169,6 +174,8 @@ def main():
     value = alg / wall / 1e9
     fam = max((k for k in rep if k not in ("norm_final",)), key=lambda k: rep[k]["ms"])
     achieved = rep[fam]["bytes"] / (rep[fam]["ms"] * 1e-3) / 1e9
+    alg_per_launch = rep[fam]["bytes"] / rep[fam]["launches"]
+    traffic = alg_per_launch * PMC_TRAFFIC_RATIO[fam] if fam in PMC_TRAFFIC_RATIO else None
     # the final measurements of the workload (not in the timed region)
     tm = time.perf_counter()
     outcomes = [eng.measure(q, r) for q, r in zip(mq, mr)]
@@ -195,7 +202,10 @@ def main():
         "config": {"workload": "plain FP64 gate sweep, random circuit {H,X,RX,RZ,U1,CX,MCX3}",
                    "qubits": n, "gates_per_step": per, "layers_total": layers, "seed": 7},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": fam},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else round(traffic), "kernel": fam,
+                     "alg_bytes_per_launch": round(alg_per_launch),
+                     "avg_launch_ms": round(rep[fam]["ms"] / rep[fam]["launches"], 4)},
         "cpu_baseline": cpu,
         "extra": dict({"copy_kernel_gbs": round(peak_copy, 1), "kernel_families": fams,
                        "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm}, **qres),
