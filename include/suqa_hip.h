@@ -171,6 +171,42 @@ int suqa_hip_copy_kernel(suqa_hip_ctx* ctx, void* dst, const void* src, uint64_t
 int suqa_hip_device_alloc(uint64_t nbytes, void** ptr);
 int suqa_hip_device_free(void* ptr);
 
+/* ---- FP32 state with fused blocks (the precision pattern; extension) --
+ * No counterpart in the reference.  Gates of a list are fused greedily, with
+ * commutation lookahead, into dense blocks of exactly `width` qubits
+ * (4..6; built in FP64, rounded to FP32) and applied with FP32 MFMA.  Norms,
+ * probabilities, the collapse scale and fidelities accumulate in FP64.
+ * Requires 9 <= nq <= 33 and nq >= width + 5. */
+typedef struct suqa_hip_f32 suqa_hip_f32;
+
+const char* suqa_hip_f32_last_error(void);
+int suqa_hip_f32_setup(uint32_t nq, suqa_hip_f32** out);
+int suqa_hip_f32_clear(suqa_hip_f32* c);
+int suqa_hip_f32_init_state(suqa_hip_f32* c);
+/* Fuses gates[0..ngates) into width-qubit blocks and applies them in order;
+ * *nblocks (may be NULL) receives the number of blocks (kernel launches). */
+int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t width,
+                           uint32_t* nblocks);
+/* Times the gate sample at every valid width on this context's state (which
+ * it changes) and returns the fastest; ms_per_gate[w - 4] gets each width's
+ * wall time per gate (-1 where the width does not fit). */
+int suqa_hip_f32_autotune(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t* best_width,
+                          double* ms_per_gate);
+int suqa_hip_f32_vnorm(suqa_hip_f32* c, double* norm);
+int suqa_hip_f32_prob_filter(suqa_hip_f32* c, const uint32_t* qs, const uint32_t* q_mask, uint32_t n, double* prob);
+/* measure_qbit semantics (suqa.cu:673-696) on the FP32 state, FP64 probabilities. */
+int suqa_hip_f32_measure_qbit(suqa_hip_f32* c, uint32_t q, double rdoub, uint32_t* cbit);
+/* |<ref|psi>|^2 against an FP64 engine state of the same size (FP64 sums). */
+int suqa_hip_f32_fidelity(suqa_hip_f32* c, suqa_hip_ctx* ref, double* fidelity);
+int suqa_hip_f32_download(suqa_hip_f32* c, float* re, float* im);
+int suqa_hip_f32_synchronize(suqa_hip_f32* c);
+/* Host only (no GPU): the fusion plan.  For each block b < *nblocks (at most
+ * max_blocks): qubits[6b..6b+width) sorted, gates_in[b], and the FP64 matrix
+ * mats[b * 2 * 4^width ...] as interleaved (re, im), row-major over the
+ * block index sum_j bit(qubits[j]) 2^j. */
+int suqa_hip_fuse_plan(const suqa_hip_gate* gates, uint32_t ngates, uint32_t nq, uint32_t width,
+                       uint32_t max_blocks, uint32_t* nblocks, uint32_t* qubits, uint32_t* gates_in, double* mats);
+
 #ifdef __cplusplus
 }
 #endif

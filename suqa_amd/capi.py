@@ -72,6 +72,20 @@ SIGNATURES = {
     "suqa_hip_copy_kernel": (C.c_int, [CTX, C.c_void_p, C.c_void_p, C.c_uint64]),
     "suqa_hip_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "suqa_hip_device_free": (C.c_int, [C.c_void_p]),
+    # FP32 fused-block engine (extension)
+    "suqa_hip_f32_last_error": (C.c_char_p, []),
+    "suqa_hip_f32_setup": (C.c_int, [U32, C.POINTER(CTX)]),
+    "suqa_hip_f32_clear": (C.c_int, [CTX]),
+    "suqa_hip_f32_init_state": (C.c_int, [CTX]),
+    "suqa_hip_f32_run_gates": (C.c_int, [CTX, C.POINTER(Gate), U32, U32, PU32]),
+    "suqa_hip_f32_autotune": (C.c_int, [CTX, C.POINTER(Gate), U32, PU32, PD]),
+    "suqa_hip_f32_vnorm": (C.c_int, [CTX, PD]),
+    "suqa_hip_f32_prob_filter": (C.c_int, [CTX, PU32, PU32, U32, PD]),
+    "suqa_hip_f32_measure_qbit": (C.c_int, [CTX, U32, C.c_double, PU32]),
+    "suqa_hip_f32_fidelity": (C.c_int, [CTX, CTX, PD]),
+    "suqa_hip_f32_download": (C.c_int, [CTX, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "suqa_hip_f32_synchronize": (C.c_int, [CTX]),
+    "suqa_hip_fuse_plan": (C.c_int, [C.POINTER(Gate), U32, U32, U32, U32, PU32, PU32, PU32, PD]),
 }
 
 
@@ -280,6 +294,96 @@ class Engine:
         check(lib().suqa_hip_timing_report(self.ctx, arr, 64, C.byref(n)))
         return {arr[k].name.decode(): dict(launches=arr[k].launches, ms=arr[k].total_ms, bytes=arr[k].alg_bytes)
                 for k in range(min(n.value, 64))}
+
+
+def check32(rc):
+    if rc != SUQA_OK:
+        raise SuqaError(rc, lib().suqa_hip_f32_last_error().decode())
+    return rc
+
+
+class F32Engine:
+    """FP32 state with gates fused into width-qubit blocks applied by MFMA."""
+
+    def __init__(self, nq):
+        self.ctx = CTX()
+        check32(lib().suqa_hip_f32_setup(nq, C.byref(self.ctx)))
+        self.nq, self.N = nq, 1 << nq
+
+    def close(self):
+        if self.ctx:
+            lib().suqa_hip_f32_clear(self.ctx)
+            self.ctx = CTX()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_state(self):
+        check32(lib().suqa_hip_f32_init_state(self.ctx))
+
+    def run_gates(self, gates, width, n=None, start=0):
+        n = len(gates) - start if n is None else n
+        ptr = C.cast(C.addressof(gates) + start * C.sizeof(Gate), C.POINTER(Gate))
+        nb = U32()
+        check32(lib().suqa_hip_f32_run_gates(self.ctx, ptr, n, width, C.byref(nb)))
+        return nb.value
+
+    def autotune(self, gates, n=None):
+        n = len(gates) if n is None else n
+        best = U32()
+        t = (C.c_double * 3)()
+        check32(lib().suqa_hip_f32_autotune(self.ctx, gates, n, C.byref(best), t))
+        return best.value, {4 + k: t[k] for k in range(3)}
+
+    def vnorm(self):
+        v = C.c_double()
+        check32(lib().suqa_hip_f32_vnorm(self.ctx, C.byref(v)))
+        return v.value
+
+    def prob_filter(self, qs, q_mask):
+        p = C.c_double()
+        check32(lib().suqa_hip_f32_prob_filter(self.ctx, _u32(qs), _u32(q_mask), len(qs), C.byref(p)))
+        return p.value
+
+    def measure(self, q, rdoub):
+        c = U32()
+        check32(lib().suqa_hip_f32_measure_qbit(self.ctx, q, rdoub, C.byref(c)))
+        return c.value
+
+    def fidelity(self, ref):
+        f = C.c_double()
+        check32(lib().suqa_hip_f32_fidelity(self.ctx, ref.ctx, C.byref(f)))
+        return f.value
+
+    def get(self):
+        re = np.empty(self.N, dtype=np.float32)
+        im = np.empty(self.N, dtype=np.float32)
+        check32(lib().suqa_hip_f32_download(self.ctx, re.ctypes.data_as(C.POINTER(C.c_float)),
+                                            im.ctypes.data_as(C.POINTER(C.c_float))))
+        return re, im
+
+    def synchronize(self):
+        check32(lib().suqa_hip_f32_synchronize(self.ctx))
+
+
+def fuse_plan(gates, nq, width, n=None):
+    """Host-only fusion plan: list of (qubits, gates_in, matrix as complex ndarray)."""
+    n = len(gates) if n is None else n
+    nb = U32()
+    check(lib().suqa_hip_fuse_plan(gates, n, nq, width, 0, C.byref(nb), None, None, None))
+    B = nb.value
+    qs = np.zeros(6 * B, dtype=np.uint32)
+    gi = np.zeros(B, dtype=np.uint32)
+    D = 1 << width
+    mats = np.zeros(2 * B * D * D)
+    check(lib().suqa_hip_fuse_plan(gates, n, nq, width, B, C.byref(nb), qs.ctypes.data_as(PU32),
+                                   gi.ctypes.data_as(PU32), mats.ctypes.data_as(PD)))
+    m = mats.reshape(B, D * D, 2)
+    m = (m[..., 0] + 1j * m[..., 1]).reshape(B, D, D)
+    return [(list(qs[6 * b:6 * b + width]), int(gi[b]), m[b]) for b in range(B)]
 
 
 def gates_from(seq):

@@ -1,0 +1,601 @@
+// fused.hip -- the precision pattern: FP32 state vector, gates fused into
+// dense K-qubit blocks (K = 4..6, host-built in FP64, rounded to FP32) and
+// applied with FP32 MFMA (v_mfma_f32_32x32x2f32) with FP32 accumulation.
+// Norms, measurement probabilities, the collapse scale and the fidelity
+// against an FP64 state are accumulated in FP64.
+//
+// Block kernel.  A block on qubits B (|B| = K) multiplies every group of 2^K
+// amplitudes that share the other bits.  A workgroup takes one tile: all
+// combinations of B and of the L lowest non-block bits (L = 7 -> 128 groups as
+// MFMA columns), i.e. 2^(K+L) amplitudes whose low address bits are all tile
+// bits, so the tile is a set of contiguous runs of >= 128 floats.
+//   1. load the tile (re, im) into LDS in compact bit order (16-byte loads);
+//   2. Y = A X with X = [Xr; Xi] (2^(K+1) x 2^L) and the real form
+//      A = [[Mr, -Mi], [Mi, Mr]]; a wave owns one 32-row block of A (held in
+//      VGPRs, host-permuted to the MFMA lane order) and RB 32-column blocks;
+//   3. write Y back into LDS, then to HBM.
+// MFMA 32x32x2 f32 operand layout: A[i][k] in lane i + 32k, B[k][j] in lane
+// j + 32k, D[i][j] in lane j + 32 * ((i / 4) % 2), register 4 * (i / 8) + i % 4.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fused.hpp"
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double dv2f __attribute__((ext_vector_type(2)));
+
+struct suqa_hip_f32 {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t nq = 0;
+    uint64_t N = 0;
+    float* d_data = nullptr;   // re[N], im[N]
+    float* d_re = nullptr;
+    float* d_im = nullptr;
+    double* d_partials = nullptr;
+    uint64_t partial_cap = 0;  // pairs
+    double* d_result = nullptr;
+    double* h_result = nullptr;
+    float* d_mats = nullptr;   // block matrices in MFMA lane order
+    size_t mats_cap = 0;       // floats
+};
+
+namespace sqf {
+namespace {
+
+thread_local std::string g_err;
+int fail(int code, const std::string& m) {
+    g_err = m;
+    return code;
+}
+#define SQF_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return fail(SUQA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ uint64_t item_id() {   // XCD-contiguous block remap (grid % 8 == 0)
+    const uint64_t G = gridDim.x, b = blockIdx.x;
+    return ((b & 7ull) * (G >> 3) + (b >> 3)) * kBlock + threadIdx.x;
+}
+inline uint32_t grid_for(uint64_t threads) {
+    uint64_t g = (threads + kBlock - 1) / kBlock;
+    g = (g + 7) & ~7ull;
+    return (uint32_t)(g ? g : 8);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+struct BlockArgs {
+    uint32_t tpos[kMaxK + kTileLow];   // tile bit positions, ascending (compact order)
+    uint32_t ntile;                    // K + L
+    uint32_t L;                        // non-block bits per tile (columns = 2^L)
+    uint32_t rowoff[1 << kMaxK];       // compact offset of block index r
+    uint32_t coloff[1 << kTileLow];    // compact offset of column c
+    uint64_t ntiles;
+    const float* A;                    // [RB][D/2][64] MFMA-ordered stacked matrix
+};
+
+__device__ __forceinline__ uint64_t tile_base(uint64_t t, const BlockArgs& a) {
+    for (uint32_t j = 0; j < a.ntile; ++j) {   // insert zeros at the tile bits (ascending)
+        const uint64_t lo = (1ull << a.tpos[j]) - 1ull;
+        t = (t & lo) | ((t & ~lo) << 1);
+    }
+    return t;
+}
+__device__ __forceinline__ uint64_t pdep_tile(uint32_t e, const BlockArgs& a) {
+    uint64_t o = 0;
+    for (uint32_t j = 0; j < a.ntile; ++j)
+        if ((e >> j) & 1u) o |= 1ull << a.tpos[j];
+    return o;
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_block(float* __restrict__ re, float* __restrict__ im, const BlockArgs a) {
+    constexpr int D = 2 << K;          // stacked dimension
+    constexpr int RB = D / 32;         // 32-row blocks of A (1, 2, 4)
+    constexpr int KS = D / 2;          // k-steps of 2
+    extern __shared__ float lds[];
+    const uint32_t tsz = 1u << a.ntile;
+    float* lre = lds;
+    float* lim = lds + tsz;
+    const uint64_t tile = ((uint64_t)(blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+    if (tile >= a.ntiles) return;      // whole workgroup leaves together
+    const uint64_t base = tile_base(tile, a);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+
+    // 1. tile -> LDS (compact order), 16-byte chunks
+    for (uint32_t ch = tid; ch < tsz / 4; ch += kBlock) {
+        const uint64_t g = base | pdep_tile(ch * 4, a);
+        const floatx4 vr = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(re + g));
+        const floatx4 vi = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(im + g));
+        *reinterpret_cast<floatx4*>(lre + ch * 4) = vr;
+        *reinterpret_cast<floatx4*>(lim + ch * 4) = vi;
+    }
+    __syncthreads();
+
+    // 2. MFMA: wave w owns row block rb and column blocks cb0 .. cb0+RB-1
+    const int wpr = 4 / RB;            // waves per row block
+    const int rb = w / wpr;
+    const int cb0 = (w % wpr) * RB;
+    const int ncb = 1 << (a.L - 5);    // 32-column blocks in the tile (1..4)
+    float areg[KS];
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) areg[kk] = a.A[((size_t)rb * KS + kk) * 64 + lane];
+    floatx16 acc[RB];
+#pragma unroll
+    for (int c = 0; c < RB; ++c) {
+        acc[c] = floatx16{0};
+        const int cb = cb0 + c;
+        if (cb >= ncb) continue;
+        const uint32_t coff = a.coloff[cb * 32 + (lane & 31)];
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const int R = 2 * kk + (lane >> 5);
+            const float b = R < (1 << K) ? lre[a.rowoff[R] + coff] : lim[a.rowoff[R - (1 << K)] + coff];
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(areg[kk], b, acc[c], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < RB; ++c) {
+        const int cb = cb0 + c;
+        if (cb >= ncb) continue;
+        const uint32_t coff = a.coloff[cb * 32 + (lane & 31)];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int i = rb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            if (i < (1 << K)) lre[a.rowoff[i] + coff] = acc[c][v];
+            else lim[a.rowoff[i - (1 << K)] + coff] = acc[c][v];
+        }
+    }
+    __syncthreads();
+
+    // 3. LDS -> tile
+    for (uint32_t ch = tid; ch < tsz / 4; ch += kBlock) {
+        const uint64_t g = base | pdep_tile(ch * 4, a);
+        __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lre + ch * 4), reinterpret_cast<floatx4*>(re + g));
+        __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lim + ch * 4), reinterpret_cast<floatx4*>(im + g));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_init32(float* __restrict__ d, uint64_t n4) {
+    const uint64_t it = item_id();
+    if (it >= n4) return;
+    floatx4 v = {0.f, 0.f, 0.f, 0.f};
+    if (it == 0) v.x = 1.f;
+    reinterpret_cast<floatx4*>(d)[it] = v;
+}
+
+// FP64 sums of |psi|^2 over (i & M) == V, split by bit `split` (0: one sum).
+// Lanes 0-31 read re, 32-63 im; a wave covers 32 x 4 amplitudes.
+__global__ __launch_bounds__(kBlock) void k_norm32(const float* __restrict__ re, const float* __restrict__ im,
+                                                   uint64_t n4, uint64_t M, uint64_t V, uint64_t split,
+                                                   double* __restrict__ partials) {
+    __shared__ double sh[2][kBlock / 64];
+    const uint64_t t = item_id();
+    const int lane = threadIdx.x & 63;
+    const uint64_t it = (t >> 6) * 32 + (uint64_t)(lane & 31);
+    double s0 = 0.0, s1 = 0.0;
+    if (it < n4) {
+        const floatx4 v = reinterpret_cast<const floatx4*>(lane >= 32 ? im : re)[it];
+        const uint64_t i = it * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint64_t j = i + (uint64_t)e;
+            if ((j & M) != V) continue;
+            const double x = (double)v[e];
+            if (split && (j & split)) s1 += x * x;
+            else s0 += x * x;
+        }
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = s0;
+        sh[1][threadIdx.x >> 6] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            a0 += sh[0][k];
+            a1 += sh[1][k];
+        }
+        partials[2 * blockIdx.x] = a0;
+        partials[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+// <ref|psi> in FP64: sum conj(r) p over all amplitudes (ref FP64, psi FP32).
+__global__ __launch_bounds__(kBlock) void k_overlap(const double* __restrict__ rre, const double* __restrict__ rim,
+                                                    const float* __restrict__ pre, const float* __restrict__ pim,
+                                                    uint64_t n2, double* __restrict__ partials) {
+    __shared__ double sh[2][kBlock / 64];
+    const uint64_t it = item_id();
+    double sr = 0.0, si = 0.0;
+    if (it < n2) {
+        const dv2f ar = reinterpret_cast<const dv2f*>(rre)[it], ai = reinterpret_cast<const dv2f*>(rim)[it];
+        const float2 br = reinterpret_cast<const float2*>(pre)[it], bi = reinterpret_cast<const float2*>(pim)[it];
+        sr = ar.x * (double)br.x + ai.x * (double)bi.x + ar.y * (double)br.y + ai.y * (double)bi.y;
+        si = ar.x * (double)bi.x - ai.x * (double)br.x + ar.y * (double)bi.y - ai.y * (double)br.y;
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = sr;
+        sh[1][threadIdx.x >> 6] = si;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            a0 += sh[0][k];
+            a1 += sh[1][k];
+        }
+        partials[2 * blockIdx.x] = a0;
+        partials[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_pairs32(const double* __restrict__ in, uint64_t npairs, uint64_t chunk,
+                                                        double* __restrict__ out) {
+    __shared__ double sh[2][kBlock / 64];
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk;
+    const uint64_t hi = lo + chunk < npairs ? lo + chunk : npairs;
+    double s0 = 0.0, s1 = 0.0;
+    for (uint64_t k = lo + threadIdx.x; k < hi; k += kBlock) {
+        s0 += in[2 * k];
+        s1 += in[2 * k + 1];
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = s0;
+        sh[1][threadIdx.x >> 6] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            t0 += sh[0][k];
+            t1 += sh[1][k];
+        }
+        out[2 * blockIdx.x] = t0;
+        out[2 * blockIdx.x + 1] = t1;
+    }
+}
+
+// Collapse on bit q == keep and rescale by the FP64 factor.
+__global__ __launch_bounds__(kBlock) void k_collapse32(float* __restrict__ re, float* __restrict__ im, uint64_t n4,
+                                                       uint64_t qbit, uint64_t keepbit, double scale) {
+    const uint64_t t = item_id();
+    const int lane = threadIdx.x & 63;
+    const uint64_t it = (t >> 6) * 32 + (uint64_t)(lane & 31);
+    if (it >= n4) return;
+    floatx4* p = reinterpret_cast<floatx4*>(lane >= 32 ? im : re) + it;
+    floatx4 v = *p;
+    const uint64_t i = it * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = (((i + (uint64_t)e) & qbit) == keepbit) ? (float)((double)v[e] * scale) : 0.f;
+    *p = v;
+}
+
+constexpr uint64_t kPartChunk = 2048;
+
+// Two-level fixed-order reduction of `g` partial pairs into d_result[0..1].
+hipError_t finish_sum(suqa_hip_f32* c, uint32_t g) {
+    const double* in = c->d_partials;
+    uint64_t np = g;
+    if (np > kPartChunk) {
+        const uint64_t g2 = (np + kPartChunk - 1) / kPartChunk;
+        double* out = c->d_partials + 2 * (uint64_t)g;
+        hipLaunchKernelGGL(k_sum_pairs32, dim3((uint32_t)g2), dim3(kBlock), 0, c->stream, in, np, kPartChunk, out);
+        in = out;
+        np = g2;
+    }
+    hipLaunchKernelGGL(k_sum_pairs32, dim3(1), dim3(kBlock), 0, c->stream, in, np, np, c->d_result);
+    return hipGetLastError();
+}
+
+int read2(suqa_hip_f32* c) {
+    SQF_HIP(hipMemcpyAsync(c->h_result, c->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SQF_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+int norm_split(suqa_hip_f32* c, uint64_t M, uint64_t V, uint64_t split) {
+    const uint64_t n4 = c->N / 4;
+    const uint32_t g = grid_for(((n4 + 31) / 32) * 64);
+    hipLaunchKernelGGL(k_norm32, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, n4, M, V, split,
+                       c->d_partials);
+    SQF_HIP(hipGetLastError());
+    SQF_HIP(finish_sum(c, g));
+    return read2(c);
+}
+
+// Host: tile geometry and MFMA-ordered matrices of one block.
+void block_args(const Block& b, uint32_t nq, BlockArgs& a) {
+    std::memset(&a, 0, sizeof(a));
+    uint64_t bm = 0;
+    for (uint32_t j = 0; j < b.K; ++j) bm |= 1ull << b.qubits[j];
+    std::vector<uint32_t> low;
+    for (uint32_t q = 0; q < nq && low.size() < (size_t)kTileLow; ++q)
+        if (!((bm >> q) & 1ull)) low.push_back(q);
+    a.L = (uint32_t)low.size();
+    uint64_t tm = bm;
+    for (uint32_t q : low) tm |= 1ull << q;
+    a.ntile = 0;
+    for (uint32_t q = 0; q < nq; ++q)
+        if ((tm >> q) & 1ull) a.tpos[a.ntile++] = q;
+    auto cpos = [&](uint32_t q) {
+        for (uint32_t j = 0; j < a.ntile; ++j)
+            if (a.tpos[j] == q) return j;
+        return 0u;
+    };
+    for (uint32_t r = 0; r < (1u << b.K); ++r) {
+        uint32_t o = 0;
+        for (uint32_t j = 0; j < b.K; ++j)
+            if ((r >> j) & 1u) o |= 1u << cpos(b.qubits[j]);
+        a.rowoff[r] = o;
+    }
+    for (uint32_t cidx = 0; cidx < (1u << a.L); ++cidx) {
+        uint32_t o = 0;
+        for (uint32_t j = 0; j < a.L; ++j)
+            if ((cidx >> j) & 1u) o |= 1u << cpos(low[j]);
+        a.coloff[cidx] = o;
+    }
+    a.ntiles = (1ull << nq) >> a.ntile;
+}
+
+// Stacked real form [[Mr, -Mi], [Mi, Mr]] in MFMA A-operand order:
+// out[(rb * KS + kk) * 64 + lane] = A[rb*32 + lane%32][2 kk + lane/32].
+void mfma_order(const Block& b, float* out) {
+    const uint32_t Dc = 1u << b.K, D = 2 * Dc, KS = D / 2, RB = D / 32;
+    auto A = [&](uint32_t i, uint32_t k) -> double {
+        const bool ii = i >= Dc, ki = k >= Dc;
+        const std::complex<double> m = b.m[(size_t)(i % Dc) * Dc + (k % Dc)];
+        if (!ii && !ki) return m.real();
+        if (!ii && ki) return -m.imag();
+        if (ii && !ki) return m.imag();
+        return m.real();
+    };
+    for (uint32_t rb = 0; rb < RB; ++rb)
+        for (uint32_t kk = 0; kk < KS; ++kk)
+            for (uint32_t l = 0; l < 64; ++l) out[((size_t)rb * KS + kk) * 64 + l] = (float)A(rb * 32 + l % 32, 2 * kk + l / 32);
+}
+
+int apply_blocks(suqa_hip_f32* c, const std::vector<Block>& blocks) {
+    if (blocks.empty()) return SUQA_OK;
+    std::vector<size_t> off(blocks.size());
+    size_t tot = 0;
+    for (size_t i = 0; i < blocks.size(); ++i) {
+        off[i] = tot;
+        tot += (size_t)(2u << blocks[i].K) * (2u << blocks[i].K);
+    }
+    std::vector<float> h(tot);
+    for (size_t i = 0; i < blocks.size(); ++i) mfma_order(blocks[i], h.data() + off[i]);
+    if (c->mats_cap < tot) {
+        if (c->d_mats) SQF_HIP(hipFree(c->d_mats));
+        c->d_mats = nullptr;
+        void* p = nullptr;
+        SQF_HIP(hipMalloc(&p, tot * sizeof(float)));
+        c->d_mats = static_cast<float*>(p);
+        c->mats_cap = tot;
+    }
+    SQF_HIP(hipMemcpyAsync(c->d_mats, h.data(), tot * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    for (size_t i = 0; i < blocks.size(); ++i) {
+        BlockArgs a;
+        block_args(blocks[i], c->nq, a);
+        a.A = c->d_mats + off[i];
+        const uint32_t g = (uint32_t)(((a.ntiles + 7) / 8) * 8);
+        const size_t lds = sizeof(float) * 2 * ((size_t)1 << a.ntile);
+        switch (blocks[i].K) {
+        case 4: hipLaunchKernelGGL(k_block<4>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a); break;
+        case 5: hipLaunchKernelGGL(k_block<5>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a); break;
+        case 6: hipLaunchKernelGGL(k_block<6>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a); break;
+        default: return fail(SUQA_ERR_ARG, "block width must be 4..6");
+        }
+        SQF_HIP(hipGetLastError());
+    }
+    // the host matrix buffer must outlive the copy
+    SQF_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+bool width_ok(uint32_t nq, uint32_t K) { return K >= (uint32_t)kMinK && K <= (uint32_t)kMaxK && nq >= K + 5; }
+
+}  // namespace
+}  // namespace sqf
+
+using namespace sqf;
+
+extern "C" {
+
+const char* suqa_hip_f32_last_error(void) { return sqf::g_err.c_str(); }
+
+int suqa_hip_f32_setup(uint32_t nq, suqa_hip_f32** out) {
+    if (!out) return fail(SUQA_ERR_ARG, "null output");
+    *out = nullptr;
+    if (nq < 9 || nq > 33) return fail(SUQA_ERR_ARG, "the FP32 engine needs 9 <= nq <= 33");
+    auto* c = new suqa_hip_f32();
+    c->nq = nq;
+    c->N = 1ull << nq;
+    void* p = nullptr;
+    auto cleanup = [&](int rc) {
+        suqa_hip_f32_clear(c);
+        return rc;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess)
+        return cleanup(fail(SUQA_ERR_HIP, "stream creation failed"));
+    if (hipMalloc(&p, 2 * c->N * sizeof(float)) != hipSuccess) return cleanup(fail(SUQA_ERR_NOMEM, "state allocation failed"));
+    c->d_data = static_cast<float*>(p);
+    c->d_re = c->d_data;
+    c->d_im = c->d_data + c->N;
+    c->partial_cap = c->N / 128 + 4096;
+    if (hipMalloc(&p, 2 * c->partial_cap * sizeof(double)) != hipSuccess)
+        return cleanup(fail(SUQA_ERR_NOMEM, "partials allocation failed"));
+    c->d_partials = static_cast<double*>(p);
+    if (hipMalloc(&p, 4 * sizeof(double)) != hipSuccess) return cleanup(fail(SUQA_ERR_NOMEM, "result allocation failed"));
+    c->d_result = static_cast<double*>(p);
+    if (hipHostMalloc(&p, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(SUQA_ERR_NOMEM, "pinned allocation failed"));
+    c->h_result = static_cast<double*>(p);
+    int rc = suqa_hip_f32_init_state(c);
+    if (rc) return cleanup(rc);
+    *out = c;
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_clear(suqa_hip_f32* c) {
+    if (!c) return SUQA_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->d_data) (void)hipFree(c->d_data);
+    if (c->d_partials) (void)hipFree(c->d_partials);
+    if (c->d_result) (void)hipFree(c->d_result);
+    if (c->h_result) (void)hipHostFree(c->h_result);
+    if (c->d_mats) (void)hipFree(c->d_mats);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_init_state(suqa_hip_f32* c) {
+    if (!c || !c->d_data) return fail(SUQA_ERR_ARG, "null context");
+    const uint64_t n4 = 2 * c->N / 4;
+    hipLaunchKernelGGL(k_init32, dim3(grid_for(n4)), dim3(kBlock), 0, c->stream, c->d_data, n4);
+    SQF_HIP(hipGetLastError());
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t width,
+                           uint32_t* nblocks) {
+    if (!c || !c->d_data || (ngates && !gates)) return fail(SUQA_ERR_ARG, "null argument");
+    if (!width_ok(c->nq, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
+    for (uint32_t i = 0; i < ngates; ++i) {
+        const suqa_hip_gate& g = gates[i];
+        if (g.target >= c->nq || g.kind < 0 || g.kind > SUQA_GATE_MCX || g.nctrl > 3) return fail(SUQA_ERR_ARG, "bad gate");
+        for (uint32_t k = 0; k < g.nctrl; ++k)
+            if (g.ctrl[k] >= c->nq) return fail(SUQA_ERR_ARG, "bad control");
+    }
+    const auto blocks = fuse(gates, ngates, c->nq, width, 256);
+    if (nblocks) *nblocks = (uint32_t)blocks.size();
+    return apply_blocks(c, blocks);
+}
+
+int suqa_hip_f32_autotune(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t* best_width,
+                          double* ms_per_gate) {
+    if (!c || !best_width) return fail(SUQA_ERR_ARG, "null argument");
+    double best = 1e300;
+    *best_width = 0;
+    for (uint32_t K = kMinK; K <= (uint32_t)kMaxK; ++K) {
+        if (ms_per_gate) ms_per_gate[K - kMinK] = -1.0;
+        if (!width_ok(c->nq, K)) continue;
+        const auto blocks = fuse(gates, ngates, c->nq, K, 256);
+        // warm-up on the first block, then time the whole sample
+        int rc = apply_blocks(c, std::vector<Block>(blocks.begin(), blocks.begin() + 1));
+        if (rc) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = apply_blocks(c, blocks);
+        if (rc) return rc;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        const double per = ms / std::max<uint32_t>(ngates, 1);
+        if (ms_per_gate) ms_per_gate[K - kMinK] = per;
+        if (per < best) {
+            best = per;
+            *best_width = K;
+        }
+    }
+    return *best_width ? SUQA_OK : fail(SUQA_ERR_ARG, "no fusion width fits this state");
+}
+
+int suqa_hip_f32_vnorm(suqa_hip_f32* c, double* norm) {
+    if (!c || !c->d_data) return fail(SUQA_ERR_ARG, "null context");
+    int rc = norm_split(c, 0, 0, 0);
+    if (rc) return rc;
+    if (norm) *norm = std::sqrt(c->h_result[0]);
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_prob_filter(suqa_hip_f32* c, const uint32_t* qs, const uint32_t* q_mask, uint32_t n, double* prob) {
+    if (!c || !c->d_data || (n && (!qs || !q_mask))) return fail(SUQA_ERR_ARG, "null argument");
+    uint64_t M = 0, V = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+        M |= 1ull << qs[k];
+        if (q_mask[k]) V |= (uint64_t)q_mask[k] << qs[k];
+    }
+    if (V & ~M) {
+        if (prob) *prob = 0.0;
+        return SUQA_OK;
+    }
+    int rc = norm_split(c, M, V, 0);
+    if (rc) return rc;
+    if (prob) *prob = c->h_result[0];
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_measure_qbit(suqa_hip_f32* c, uint32_t q, double rdoub, uint32_t* cout) {
+    if (!c || !c->d_data) return fail(SUQA_ERR_ARG, "null context");
+    if (q >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    const uint64_t qb = 1ull << q;
+    int rc = norm_split(c, 0, 0, qb);
+    if (rc) return rc;
+    const double p0 = c->h_result[0], p1 = c->h_result[1];
+    const uint32_t cc = (uint32_t)(rdoub < p1);
+    const double scale = 1. / std::sqrt(cc ? p1 : p0);
+    const uint64_t n4 = c->N / 4;
+    hipLaunchKernelGGL(k_collapse32, dim3(grid_for(((n4 + 31) / 32) * 64)), dim3(kBlock), 0, c->stream, c->d_re, c->d_im,
+                       n4, qb, cc ? qb : 0ull, scale);
+    SQF_HIP(hipGetLastError());
+    if (cout) *cout = cc;
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_fidelity(suqa_hip_f32* c, suqa_hip_ctx* ref, double* fidelity) {
+    if (!c || !c->d_data || !ref) return fail(SUQA_ERR_ARG, "null argument");
+    uint32_t rnq = 0;
+    uint64_t rN = 0;
+    double *rre = nullptr, *rim = nullptr;
+    if (suqa_hip_state_info(ref, &rnq, &rN, &rre, &rim) != SUQA_OK || rnq != c->nq)
+        return fail(SUQA_ERR_ARG, "reference state has another size");
+    SQF_HIP(hipDeviceSynchronize());   // the reference context runs on its own stream
+    const uint64_t n2 = c->N / 2;
+    const uint32_t g = grid_for(n2);
+    hipLaunchKernelGGL(k_overlap, dim3(g), dim3(kBlock), 0, c->stream, rre, rim, c->d_re, c->d_im, n2, c->d_partials);
+    SQF_HIP(hipGetLastError());
+    SQF_HIP(finish_sum(c, g));
+    int rc = read2(c);
+    if (rc) return rc;
+    if (fidelity) *fidelity = c->h_result[0] * c->h_result[0] + c->h_result[1] * c->h_result[1];
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_download(suqa_hip_f32* c, float* re, float* im) {
+    if (!c || !c->d_data) return fail(SUQA_ERR_ARG, "null context");
+    if (re) SQF_HIP(hipMemcpyAsync(re, c->d_re, c->N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (im) SQF_HIP(hipMemcpyAsync(im, c->d_im, c->N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SQF_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_synchronize(suqa_hip_f32* c) {
+    if (!c || !c->stream) return fail(SUQA_ERR_ARG, "null context");
+    SQF_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+}  // extern "C"
