@@ -26,6 +26,12 @@ The QMS half of the metric is reported under "extra":
                 persistent kernel, chain-steps/s, a sample of chains checked
                 against the oracle;
   cpu_baseline_qms -- the oracle on the same single chain, 1 core.
+
+and the precision pattern under "extra.fused": the same circuit on an FP32
+state with gates fused into K-qubit blocks (K autotuned over 4..6 on the
+first 10 layers) applied by FP32 MFMA, timed over all layers from |0> + H,
+with its fidelity |<psi64|psi32>|^2 against the FP64 run and the 8 final
+measurement outcomes compared.
 """
 import argparse
 import json
@@ -129,6 +135,40 @@ def qms_legs(no_cpu):
     return res
 
 
+def fused_leg(capi, wl, eng, gates, n, layers):
+    """FP32 fused-block run of the whole circuit; eng holds the FP64 state after all layers."""
+    f32 = capi.F32Engine(n)
+    try:
+        sample = capi.gates_from(gates[:10 * n])
+        best, tms = f32.autotune(sample)
+        log(f"fused: autotuned width {best} ({tms})")
+        arr = capi.gates_from(gates[:layers * n])
+        t0 = time.perf_counter()
+        plan = capi.fuse_plan(arr, n, best)
+        t_plan = time.perf_counter() - t0
+        f32.init_state()
+        f32.run_gates(capi.gates_from([wl.G(wl.H, q, [], 0.0) for q in range(n)]), best)
+        f32.synchronize()
+        t0 = time.perf_counter()
+        nb = f32.run_gates(arr, best)
+        f32.synchronize()
+        dt = time.perf_counter() - t0
+        fid = f32.fidelity(eng)
+        norm = f32.vnorm()
+        log(f"fused: {nb} blocks for {layers * n} gates in {dt:.3f} s, fidelity {fid!r}")
+        amp_bytes = 16 * (1 << n)     # an FP32 complex amplitude read and written per block
+        return f32, {
+            "width": best, "ms_per_gate_by_width": {str(k): (None if v < 0 else round(v, 4)) for k, v in tms.items()},
+            "gates": layers * n, "blocks": nb, "gates_per_block": round(layers * n / nb, 2),
+            "seconds": round(dt, 4), "layers_per_s": round(layers / dt, 3),
+            "ms_per_layer": round(dt / layers * 1e3, 3), "host_fusion_seconds": round(t_plan, 4),
+            "block_kernel_gbs": round(nb * amp_bytes / dt / 1e9, 1),
+            "fidelity_vs_fp64": fid, "norm": norm, "dtype": "f32 state, f32 MFMA, f64 reductions"}
+    except Exception:
+        f32.close()
+        raise
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +177,7 @@ def main():
     ap.add_argument("--n", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-qms", action="store_true", help="skip the QMS legs (profiling runs)")
+    ap.add_argument("--no-fused", action="store_true", help="skip the FP32 fused leg (profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU sharding is not part of "
@@ -176,12 +217,18 @@ def main():
     achieved = rep[fam]["bytes"] / (rep[fam]["ms"] * 1e-3) / 1e9
     alg_per_launch = rep[fam]["bytes"] / rep[fam]["launches"]
     traffic = alg_per_launch * PMC_TRAFFIC_RATIO[fam] if fam in PMC_TRAFFIC_RATIO else None
+    f32, fres = (None, None) if a.no_fused else fused_leg(capi, wl, eng, gates, n, layers)
     # the final measurements of the workload (not in the timed region)
     tm = time.perf_counter()
     outcomes = [eng.measure(q, r) for q, r in zip(mq, mr)]
     tm = time.perf_counter() - tm
     norm = eng.vnorm()
     eng.close()
+    if f32 is not None:
+        fres["measure_outcomes"] = [f32.measure(q, r) for q, r in zip(mq, mr)]
+        fres["outcomes_match_fp64"] = fres["measure_outcomes"] == outcomes
+        fres["speedup_vs_fp64_per_layer"] = round((wall / max(a.steps, 1)) / (fres["seconds"] / layers), 2)
+        f32.close()
     cpu = None if a.no_cpu else cpu_baseline(min(n, 27))
     qres = {} if a.no_qms else qms_legs(a.no_cpu)
     fams = {k: dict(launches=v["launches"], ms=round(v["ms"], 3),
@@ -208,7 +255,7 @@ def main():
                      "avg_launch_ms": round(rep[fam]["ms"] / rep[fam]["launches"], 4)},
         "cpu_baseline": cpu,
         "extra": dict({"copy_kernel_gbs": round(peak_copy, 1), "kernel_families": fams,
-                       "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm}, **qres),
+                       "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm, "fused": fres}, **qres),
     }
     print(json.dumps(out), flush=True)
 

@@ -152,6 +152,20 @@ typedef struct {
 /* Applies gates[0..ngates) in order, asynchronously on the context stream. */
 int suqa_hip_run_gates(suqa_hip_ctx* ctx, const suqa_hip_gate* gates, uint32_t ngates);
 
+/* One Pauli-string rotation exp(i phase P), P = prod_k sigma_{types[k]} on
+ * qs[k] (k < w <= 8), controlled on the gc_mask qubits: the operation
+ * suqa_hip_apply_pauli_rotation_general performs, as a list element. */
+typedef struct {
+    uint32_t w;
+    uint32_t qs[8];
+    uint32_t types[8];
+    uint32_t gc_mask;
+    double   phase;
+} suqa_hip_pauli_op;
+
+/* Applies ops[0..nops) in order, one sweep each (a Trotter step list). */
+int suqa_hip_run_paulis(suqa_hip_ctx* ctx, const suqa_hip_pauli_op* ops, uint32_t nops);
+
 typedef struct {
     char     name[32];     /* kernel family */
     uint64_t launches;
@@ -190,6 +204,13 @@ int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t
 /* Times the gate sample at every valid width on this context's state (which
  * it changes) and returns the fastest; ms_per_gate[w - 4] gets each width's
  * wall time per gate (-1 where the width does not fit). */
+/* The same fused application for Pauli-string rotations (each, with its
+ * controls, spanning at most `width` qubits). */
+int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint32_t nops, uint32_t width,
+                            uint32_t* nblocks);
+/* suqa::apply_qft_inverse (suqa.cu:649-661, no final swaps) on qs[0..n),
+ * its H and controlled-phase gates fused into width-qubit blocks. */
+int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, uint32_t width);
 int suqa_hip_f32_autotune(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t* best_width,
                           double* ms_per_gate);
 int suqa_hip_f32_vnorm(suqa_hip_f32* c, double* norm);

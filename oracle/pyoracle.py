@@ -295,3 +295,9 @@ def trotter_terms(nsys, nterms, seed):
     terms = (PauliTerm * nterms)()
     lib().so_trotter_terms_generate(nsys, nterms, seed, terms)
     return terms
+
+
+def qpe_trotter(st, nsys, nreg, terms, dt, steps):
+    """so_qpe_trotter: H on the register, controlled Trotter powers, inverse QFT."""
+    lib().so_qpe_trotter(st.p, nsys, nreg, terms, len(terms), dt, steps)
+

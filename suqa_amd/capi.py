@@ -31,6 +31,10 @@ class Gate(C.Structure):
     _fields_ = [("kind", C.c_int32), ("target", U32), ("nctrl", U32), ("ctrl", U32 * 3), ("angle", C.c_double)]
 
 
+class PauliOp(C.Structure):
+    _fields_ = [("w", U32), ("qs", U32 * 8), ("types", U32 * 8), ("gc_mask", U32), ("phase", C.c_double)]
+
+
 class KStat(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double),
                 ("alg_bytes", C.c_double)]
@@ -72,12 +76,15 @@ SIGNATURES = {
     "suqa_hip_copy_kernel": (C.c_int, [CTX, C.c_void_p, C.c_void_p, C.c_uint64]),
     "suqa_hip_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "suqa_hip_device_free": (C.c_int, [C.c_void_p]),
+    "suqa_hip_run_paulis": (C.c_int, [CTX, C.POINTER(PauliOp), U32]),
     # FP32 fused-block engine (extension)
     "suqa_hip_f32_last_error": (C.c_char_p, []),
     "suqa_hip_f32_setup": (C.c_int, [U32, C.POINTER(CTX)]),
     "suqa_hip_f32_clear": (C.c_int, [CTX]),
     "suqa_hip_f32_init_state": (C.c_int, [CTX]),
     "suqa_hip_f32_run_gates": (C.c_int, [CTX, C.POINTER(Gate), U32, U32, PU32]),
+    "suqa_hip_f32_run_paulis": (C.c_int, [CTX, C.POINTER(PauliOp), U32, U32, PU32]),
+    "suqa_hip_f32_qft_inverse": (C.c_int, [CTX, PU32, U32, U32]),
     "suqa_hip_f32_autotune": (C.c_int, [CTX, C.POINTER(Gate), U32, PU32, PD]),
     "suqa_hip_f32_vnorm": (C.c_int, [CTX, PD]),
     "suqa_hip_f32_prob_filter": (C.c_int, [CTX, PU32, PU32, U32, PD]),
@@ -279,6 +286,9 @@ class Engine:
         ptr = C.cast(C.addressof(gates) + start * C.sizeof(Gate), C.POINTER(Gate))
         check(lib().suqa_hip_run_gates(self.ctx, ptr, n))
 
+    def run_paulis(self, ops):
+        check(lib().suqa_hip_run_paulis(self.ctx, ops, len(ops)))
+
     def apply_gate(self, g):
         self.run_gates((Gate * 1)(g), 1)
 
@@ -330,6 +340,14 @@ class F32Engine:
         nb = U32()
         check32(lib().suqa_hip_f32_run_gates(self.ctx, ptr, n, width, C.byref(nb)))
         return nb.value
+
+    def run_paulis(self, ops, width):
+        nb = U32()
+        check32(lib().suqa_hip_f32_run_paulis(self.ctx, ops, len(ops), width, C.byref(nb)))
+        return nb.value
+
+    def qft_inverse(self, qs, width):
+        check32(lib().suqa_hip_f32_qft_inverse(self.ctx, _u32(qs), len(qs), width))
 
     def autotune(self, gates, n=None):
         n = len(gates) if n is None else n
@@ -384,6 +402,19 @@ def fuse_plan(gates, nq, width, n=None):
     m = mats.reshape(B, D * D, 2)
     m = (m[..., 0] + 1j * m[..., 1]).reshape(B, D, D)
     return [(list(qs[6 * b:6 * b + width]), int(gi[b]), m[b]) for b in range(B)]
+
+
+def pauli_ops(seq):
+    """ctypes PauliOp array from (qs, types, phase, gc_mask) tuples."""
+    arr = (PauliOp * len(seq))()
+    for a, (qs, types, phase, gc) in zip(arr, seq):
+        if len(qs) != len(types) or len(qs) > 8:
+            raise ValueError("a Pauli op needs matching qs/types of length <= 8")
+        a.w = len(qs)
+        for k, (q, t) in enumerate(zip(qs, types)):
+            a.qs[k], a.types[k] = q, t
+        a.gc_mask, a.phase = gc, phase
+    return arr
 
 
 def gates_from(seq):

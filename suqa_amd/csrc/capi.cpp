@@ -637,6 +637,18 @@ int suqa_hip_run_gates(suqa_hip_ctx* c, const suqa_hip_gate* gates, uint32_t nga
     return SUQA_OK;
 }
 
+int suqa_hip_run_paulis(suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t nops) {
+    SQE_CTX(c);
+    if (nops && !ops) return fail(SUQA_ERR_ARG, "null op list");
+    for (uint32_t k = 0; k < nops; ++k) {
+        if (ops[k].w > 8) return fail(SUQA_ERR_ARG, "Pauli op weight above 8");
+        int rc = suqa_hip_apply_pauli_rotation_general(c, ops[k].qs, ops[k].types, ops[k].w, ops[k].phase,
+                                                       ops[k].gc_mask);
+        if (rc) return rc;
+    }
+    return SUQA_OK;
+}
+
 int suqa_hip_timing_enable(suqa_hip_ctx* c, int on) {
     SQE_CTX(c);
     if (!on) sqe::timing_flush(c);

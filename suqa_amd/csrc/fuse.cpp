@@ -1,5 +1,5 @@
-// fuse.cpp -- host side of the precision pattern: gate matrices and greedy
-// fusion of a gate list into dense K-qubit blocks (computed in FP64).
+// fuse.cpp -- host side of the precision pattern: operation matrices and
+// greedy fusion of an operation list into dense K-qubit blocks (in FP64).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -8,54 +8,37 @@
 
 namespace sqf {
 
-using cd = std::complex<double>;
-
-// Workload gate -> (sorted qubits, 2^g x 2^g matrix in that qubit order).
-// H uses the reference's 1/sqrt(2) constant (suqa.cuh:57); RX(a) and RZ(a)
-// are exp(-i a P / 2) as apply_pauli_TP_rotation({q},{P},-a/2) computes them.
-void gate_matrix(const suqa_hip_gate& g, std::vector<uint32_t>& qs, std::vector<cd>& m) {
-    const double T = 0.7071067811865475;
-    qs.clear();
-    if (g.kind == SUQA_GATE_CX || g.kind == SUQA_GATE_MCX) {
-        for (uint32_t k = 0; k < g.nctrl; ++k) qs.push_back(g.ctrl[k]);
-        qs.push_back(g.target);
-        std::sort(qs.begin(), qs.end());
-        const uint32_t d = 1u << qs.size();
-        uint32_t tb = 0, cm = 0;
-        for (uint32_t j = 0; j < qs.size(); ++j) {
-            if (qs[j] == g.target) tb = 1u << j;
-            else cm |= 1u << j;
-        }
-        m.assign((size_t)d * d, cd(0, 0));
-        for (uint32_t in = 0; in < d; ++in) {
-            const uint32_t out = ((in & cm) == cm) ? (in ^ tb) : in;
-            m[(size_t)out * d + in] = 1.0;
-        }
-        return;
-    }
-    qs.push_back(g.target);
-    m.assign(4, cd(0, 0));
-    switch (g.kind) {
-    case SUQA_GATE_H: m = {T, T, T, -T}; break;
-    case SUQA_GATE_X: m = {0, 1, 1, 0}; break;
-    case SUQA_GATE_RX: {   // exp(i phi X), phi = -a/2: [[c, i s], [i s, c]]
-        const double ph = -g.angle / 2.0;
-        m = {std::cos(ph), cd(0, std::sin(ph)), cd(0, std::sin(ph)), std::cos(ph)};
-        break;
-    }
-    case SUQA_GATE_RZ: {   // exp(i phi Z): diag(e^{i phi}, e^{-i phi})
-        const double ph = -g.angle / 2.0;
-        m = {std::polar(1.0, ph), 0, 0, std::polar(1.0, -ph)};
-        break;
-    }
-    case SUQA_GATE_U1: m = {1, 0, 0, std::polar(1.0, g.angle)}; break;
-    default: break;
-    }
-}
-
 namespace {
 
-// Applies gate matrix gm (on block positions pos[]) to every column of the
+const double kT = 0.7071067811865475;   // the reference's 1/sqrt(2) (suqa.cuh:57)
+
+// Builds an op on the qubit set `qs` (any order, distinct) whose matrix is
+// f(in) applied column by column: f writes column `in` (indices over the
+// sorted qubits) into col[].
+template <class F>
+Op make_op(std::vector<uint32_t> qs, F f) {
+    std::sort(qs.begin(), qs.end());
+    Op o;
+    o.n = (uint32_t)qs.size();
+    for (uint32_t j = 0; j < o.n; ++j) o.qs[j] = qs[j];
+    const uint32_t d = 1u << o.n;
+    o.m.assign((size_t)d * d, cd(0, 0));
+    std::vector<cd> col(d);
+    for (uint32_t in = 0; in < d; ++in) {
+        std::fill(col.begin(), col.end(), cd(0, 0));
+        f(o, in, col);
+        for (uint32_t out = 0; out < d; ++out) o.m[(size_t)out * d + in] = col[out];
+    }
+    return o;
+}
+
+uint32_t bit_of(const Op& o, uint32_t q) {
+    for (uint32_t j = 0; j < o.n; ++j)
+        if (o.qs[j] == q) return 1u << j;
+    return 0;
+}
+
+// Applies op matrix gm (on block positions pos[]) to every column of the
 // block matrix B (D x D, row-major): B <- G B.
 void apply_to_block(std::vector<cd>& B, uint32_t K, const std::vector<uint32_t>& pos, const std::vector<cd>& gm) {
     const uint32_t D = 1u << K, gsz = (uint32_t)pos.size(), gd = 1u << gsz;
@@ -71,7 +54,7 @@ void apply_to_block(std::vector<cd>& B, uint32_t K, const std::vector<uint32_t>&
     std::vector<cd> v(gd), w(gd);
     for (uint32_t col = 0; col < D; ++col) {
         for (uint32_t r = 0; r < D; ++r) {
-            if (r & gmask) continue;   // r: the rest index with gate bits 0
+            if (r & gmask) continue;   // r: the rest index with the op's bits 0
             for (uint32_t x = 0; x < gd; ++x) v[x] = B[(size_t)(r | off[x]) * D + col];
             for (uint32_t y = 0; y < gd; ++y) {
                 cd s = 0;
@@ -85,12 +68,114 @@ void apply_to_block(std::vector<cd>& B, uint32_t K, const std::vector<uint32_t>&
 
 }  // namespace
 
-std::vector<Block> fuse(const suqa_hip_gate* g, uint32_t n, uint32_t nq, uint32_t width, uint32_t window) {
+// RX(a) and RZ(a) are exp(-i a P / 2) as apply_pauli_TP_rotation({q},{P},-a/2)
+// computes them; MCX flips the target when every control is 1.
+Op gate_op(const suqa_hip_gate& g) {
+    if (g.kind == SUQA_GATE_CX || g.kind == SUQA_GATE_MCX) {
+        std::vector<uint32_t> qs(g.ctrl, g.ctrl + g.nctrl);
+        qs.push_back(g.target);
+        return make_op(qs, [&](const Op& o, uint32_t in, std::vector<cd>& col) {
+            const uint32_t tb = bit_of(o, g.target);
+            uint32_t cm = 0;
+            for (uint32_t k = 0; k < g.nctrl; ++k) cm |= bit_of(o, g.ctrl[k]);
+            col[((in & cm) == cm) ? (in ^ tb) : in] = 1.0;
+        });
+    }
+    cd m[4] = {1, 0, 0, 1};
+    const double ph = -g.angle / 2.0;
+    switch (g.kind) {
+    case SUQA_GATE_H: m[0] = kT; m[1] = kT; m[2] = kT; m[3] = -kT; break;
+    case SUQA_GATE_X: m[0] = 0; m[1] = 1; m[2] = 1; m[3] = 0; break;
+    case SUQA_GATE_RX:   // exp(i ph X) = [[c, i s], [i s, c]]
+        m[0] = std::cos(ph); m[1] = cd(0, std::sin(ph)); m[2] = cd(0, std::sin(ph)); m[3] = std::cos(ph);
+        break;
+    case SUQA_GATE_RZ: m[0] = std::polar(1.0, ph); m[1] = 0; m[2] = 0; m[3] = std::polar(1.0, -ph); break;
+    case SUQA_GATE_U1: m[3] = std::polar(1.0, g.angle); break;
+    default: break;
+    }
+    Op o;
+    o.n = 1;
+    o.qs[0] = g.target;
+    o.m.assign(m, m + 4);
+    return o;
+}
+
+// exp(i phi P) = cos(phi) I + i sin(phi) P with P|x> = prod_k f_k |x ^ flips>,
+// f = 1 (X), i (-1)^b (Y), (-1)^b (Z); identity where a control bit is 0.
+bool pauli_op(const suqa_hip_pauli_op& p, Op& out) {
+    if (p.w > 8) return false;
+    std::vector<uint32_t> qs;
+    uint64_t seen = 0;
+    for (uint32_t k = 0; k < p.w; ++k) {
+        if (p.types[k] > 3u || p.qs[k] >= 32 || ((seen >> p.qs[k]) & 1ull)) return false;
+        seen |= 1ull << p.qs[k];
+        if (p.types[k] != 0u) qs.push_back(p.qs[k]);
+    }
+    for (uint32_t q = 0; q < 32; ++q)
+        if ((p.gc_mask >> q) & 1u) {
+            if ((seen >> q) & 1ull) return false;
+            qs.push_back(q);
+        }
+    if (qs.size() > (size_t)kMaxK) return false;
+    const double c = std::cos(p.phase), s = std::sin(p.phase);
+    out = make_op(qs, [&](const Op& o, uint32_t in, std::vector<cd>& col) {
+        uint32_t cm = 0;
+        for (uint32_t q = 0; q < 32; ++q)
+            if ((p.gc_mask >> q) & 1u) cm |= bit_of(o, q);
+        if ((in & cm) != cm) {
+            col[in] = 1.0;
+            return;
+        }
+        uint32_t flip = 0;
+        cd f = 1.0;
+        for (uint32_t k = 0; k < p.w; ++k) {
+            const uint32_t b = bit_of(o, p.qs[k]);
+            const bool one = (in & b) != 0;
+            switch (p.types[k]) {
+            case SUQA_PAULI_X: flip |= b; break;
+            case SUQA_PAULI_Y: flip |= b; f *= cd(0, one ? -1.0 : 1.0); break;
+            case SUQA_PAULI_Z: if (one) f = -f; break;
+            default: break;
+            }
+        }
+        col[in] += c;
+        col[in ^ flip] += cd(0, s) * f;
+    });
+    return true;
+}
+
+Op h_op(uint32_t q) {
+    suqa_hip_gate g{};
+    g.kind = SUQA_GATE_H;
+    g.target = q;
+    return gate_op(g);
+}
+
+Op cphase_op(uint32_t a, uint32_t b, double phase) {
+    return make_op({a, b}, [&](const Op&, uint32_t in, std::vector<cd>& col) {
+        col[in] = in == 3u ? std::polar(1.0, phase) : cd(1.0, 0.0);
+    });
+}
+
+// suqa.cu:649-661 order: for o = 0..n-1, the crm(-(1+o-in)) rotations with
+// in < o, then H on qs[o]; crm(m) is the controlled phase sign(m) 2 pi / 2^|m|.
+void qft_inverse_ops(const uint32_t* qs, uint32_t n, std::vector<Op>& out) {
+    for (uint32_t o = 0; o < n; ++o) {
+        for (uint32_t in = 0; in < o; ++in) {
+            const uint32_t m = 1 + (o - in);
+            out.push_back(cphase_op(qs[in], qs[o], -2.0 * M_PI / std::ldexp(1.0, (int)m)));
+        }
+        out.push_back(h_op(qs[o]));
+    }
+}
+
+std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width, uint32_t window) {
+    const uint32_t n = (uint32_t)ops.size();
     std::vector<Block> out;
     std::vector<char> used(n, 0);
-    std::vector<std::vector<uint32_t>> gq(n);
-    std::vector<std::vector<cd>> gm(n);
-    for (uint32_t i = 0; i < n; ++i) gate_matrix(g[i], gq[i], gm[i]);
+    std::vector<uint64_t> gmask(n, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < ops[i].n; ++j) gmask[i] |= 1ull << ops[i].qs[j];
     uint32_t first = 0;
     while (first < n) {
         while (first < n && used[first]) ++first;
@@ -101,8 +186,7 @@ std::vector<Block> fuse(const suqa_hip_gate* g, uint32_t n, uint32_t nq, uint32_
         for (uint32_t i = first; i < n && scanned < window; ++i) {
             if (used[i]) continue;
             ++scanned;
-            uint64_t G = 0;
-            for (uint32_t q : gq[i]) G |= 1ull << q;
+            const uint64_t G = gmask[i];
             if ((G & blocked) == 0 && (uint32_t)__builtin_popcountll(S | G) <= width) {
                 S |= G;
                 members.push_back(i);
@@ -126,10 +210,10 @@ std::vector<Block> fuse(const suqa_hip_gate* g, uint32_t n, uint32_t nq, uint32_
         for (uint32_t r = 0; r < D; ++r) b.m[(size_t)r * D + r] = 1.0;
         for (uint32_t i : members) {
             std::vector<uint32_t> pos;
-            for (uint32_t q : gq[i])
-                for (uint32_t j = 0; j < b.K; ++j)
-                    if (b.qubits[j] == q) pos.push_back(j);
-            apply_to_block(b.m, b.K, pos, gm[i]);
+            for (uint32_t j = 0; j < ops[i].n; ++j)
+                for (uint32_t k = 0; k < b.K; ++k)
+                    if (b.qubits[k] == ops[i].qs[j]) pos.push_back(k);
+            apply_to_block(b.m, b.K, pos, ops[i].m);
         }
         b.ngates = (uint32_t)members.size();
         out.push_back(std::move(b));
@@ -143,7 +227,16 @@ extern "C" int suqa_hip_fuse_plan(const suqa_hip_gate* gates, uint32_t ngates, u
                                   uint32_t max_blocks, uint32_t* nblocks, uint32_t* qubits, uint32_t* gates_in,
                                   double* mats) {
     if ((ngates && !gates) || !nblocks || width < 1 || width > 6 || nq < width || nq > 64) return SUQA_ERR_ARG;
-    const auto blocks = sqf::fuse(gates, ngates, nq, width, 256);
+    std::vector<sqf::Op> ops;
+    ops.reserve(ngates);
+    for (uint32_t i = 0; i < ngates; ++i) {
+        if (gates[i].nctrl > 3 || gates[i].target >= nq) return SUQA_ERR_ARG;
+        if (gates[i].kind == SUQA_GATE_CX || gates[i].kind == SUQA_GATE_MCX)
+            for (uint32_t k = 0; k < gates[i].nctrl; ++k)
+                if (gates[i].ctrl[k] >= nq) return SUQA_ERR_ARG;
+        ops.push_back(sqf::gate_op(gates[i]));
+    }
+    const auto blocks = sqf::fuse(ops, nq, width, 256);
     *nblocks = (uint32_t)blocks.size();
     const size_t msz = (size_t)1 << (2 * width);
     for (uint32_t b = 0; b < blocks.size() && b < max_blocks; ++b) {

@@ -378,29 +378,30 @@ void mfma_order(const Block& b, float* out) {
             for (uint32_t l = 0; l < 64; ++l) out[((size_t)rb * KS + kk) * 64 + l] = (float)A(rb * 32 + l % 32, 2 * kk + l / 32);
 }
 
-int apply_blocks(suqa_hip_f32* c, const std::vector<Block>& blocks) {
-    if (blocks.empty()) return SUQA_OK;
-    std::vector<size_t> off(blocks.size());
+// Uploads the MFMA-ordered matrices of blocks [lo, hi) and launches them.
+int apply_chunk(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, size_t hi) {
+    std::vector<size_t> off(hi - lo);
     size_t tot = 0;
-    for (size_t i = 0; i < blocks.size(); ++i) {
-        off[i] = tot;
+    for (size_t i = lo; i < hi; ++i) {
+        off[i - lo] = tot;
         tot += (size_t)(2u << blocks[i].K) * (2u << blocks[i].K);
     }
     std::vector<float> h(tot);
-    for (size_t i = 0; i < blocks.size(); ++i) mfma_order(blocks[i], h.data() + off[i]);
+    for (size_t i = lo; i < hi; ++i) mfma_order(blocks[i], h.data() + off[i - lo]);
     if (c->mats_cap < tot) {
         if (c->d_mats) SQF_HIP(hipFree(c->d_mats));
         c->d_mats = nullptr;
+        c->mats_cap = 0;
         void* p = nullptr;
         SQF_HIP(hipMalloc(&p, tot * sizeof(float)));
         c->d_mats = static_cast<float*>(p);
         c->mats_cap = tot;
     }
     SQF_HIP(hipMemcpyAsync(c->d_mats, h.data(), tot * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    for (size_t i = 0; i < blocks.size(); ++i) {
+    for (size_t i = lo; i < hi; ++i) {
         BlockArgs a;
         block_args(blocks[i], c->nq, a);
-        a.A = c->d_mats + off[i];
+        a.A = c->d_mats + off[i - lo];
         const uint32_t g = (uint32_t)(((a.ntiles + 7) / 8) * 8);
         const size_t lds = sizeof(float) * 2 * ((size_t)1 << a.ntile);
         switch (blocks[i].K) {
@@ -411,8 +412,33 @@ int apply_blocks(suqa_hip_f32* c, const std::vector<Block>& blocks) {
         }
         SQF_HIP(hipGetLastError());
     }
-    // the host matrix buffer must outlive the copy
+    // the host matrix buffer must outlive the copy, and the device buffer is
+    // rewritten by the next chunk
     SQF_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+// Applies the blocks in order, 256 at a time (at most 16 MiB of matrices).
+int apply_blocks(suqa_hip_f32* c, const std::vector<Block>& blocks) {
+    constexpr size_t kChunk = 256;
+    for (size_t lo = 0; lo < blocks.size(); lo += kChunk)
+        if (int rc = apply_chunk(c, blocks, lo, std::min(blocks.size(), lo + kChunk))) return rc;
+    return SUQA_OK;
+}
+
+// Workload gates must name distinct in-range qubits (a block's tile geometry
+// is built from them).
+int check_gates(const suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const suqa_hip_gate& g = gates[i];
+        if (g.target >= c->nq || g.kind < 0 || g.kind > SUQA_GATE_MCX || g.nctrl > 3) return fail(SUQA_ERR_ARG, "bad gate");
+        uint64_t seen = 1ull << g.target;
+        if (g.kind == SUQA_GATE_CX || g.kind == SUQA_GATE_MCX)
+            for (uint32_t k = 0; k < g.nctrl; ++k) {
+                if (g.ctrl[k] >= c->nq || ((seen >> g.ctrl[k]) & 1ull)) return fail(SUQA_ERR_ARG, "bad control");
+                seen |= 1ull << g.ctrl[k];
+            }
+    }
     return SUQA_OK;
 }
 
@@ -485,26 +511,55 @@ int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t
                            uint32_t* nblocks) {
     if (!c || !c->d_data || (ngates && !gates)) return fail(SUQA_ERR_ARG, "null argument");
     if (!width_ok(c->nq, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
-    for (uint32_t i = 0; i < ngates; ++i) {
-        const suqa_hip_gate& g = gates[i];
-        if (g.target >= c->nq || g.kind < 0 || g.kind > SUQA_GATE_MCX || g.nctrl > 3) return fail(SUQA_ERR_ARG, "bad gate");
-        for (uint32_t k = 0; k < g.nctrl; ++k)
-            if (g.ctrl[k] >= c->nq) return fail(SUQA_ERR_ARG, "bad control");
-    }
-    const auto blocks = fuse(gates, ngates, c->nq, width, 256);
+    if (int rc = check_gates(c, gates, ngates)) return rc;
+    std::vector<Op> ops;
+    ops.reserve(ngates);
+    for (uint32_t i = 0; i < ngates; ++i) ops.push_back(gate_op(gates[i]));
+    const auto blocks = fuse(ops, c->nq, width, 256);
     if (nblocks) *nblocks = (uint32_t)blocks.size();
     return apply_blocks(c, blocks);
 }
 
+int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint32_t nops, uint32_t width,
+                            uint32_t* nblocks) {
+    if (!c || !c->d_data || (nops && !ops)) return fail(SUQA_ERR_ARG, "null argument");
+    if (!width_ok(c->nq, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
+    std::vector<Op> list(nops);
+    for (uint32_t i = 0; i < nops; ++i) {
+        if (!pauli_op(ops[i], list[i])) return fail(SUQA_ERR_ARG, "malformed Pauli op or more than 6 qubits with controls");
+        if (list[i].n > width) return fail(SUQA_ERR_ARG, "Pauli op wider than the fusion width");
+        if (list[i].n && list[i].qs[list[i].n - 1] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    }
+    const auto blocks = fuse(list, c->nq, width, 256);
+    if (nblocks) *nblocks = (uint32_t)blocks.size();
+    return apply_blocks(c, blocks);
+}
+
+int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, uint32_t width) {
+    if (!c || !c->d_data || (n && !qs)) return fail(SUQA_ERR_ARG, "null argument");
+    if (!width_ok(c->nq, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
+    uint64_t seen = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (qs[k] >= c->nq || ((seen >> qs[k]) & 1ull)) return fail(SUQA_ERR_ARG, "bad register");
+        seen |= 1ull << qs[k];
+    }
+    std::vector<Op> list;
+    qft_inverse_ops(qs, n, list);
+    return apply_blocks(c, fuse(list, c->nq, width, 256));
+}
+
 int suqa_hip_f32_autotune(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t* best_width,
                           double* ms_per_gate) {
-    if (!c || !best_width) return fail(SUQA_ERR_ARG, "null argument");
+    if (!c || !c->d_data || !best_width || !gates || !ngates) return fail(SUQA_ERR_ARG, "null argument or empty sample");
+    if (int rc = check_gates(c, gates, ngates)) return rc;
     double best = 1e300;
     *best_width = 0;
     for (uint32_t K = kMinK; K <= (uint32_t)kMaxK; ++K) {
         if (ms_per_gate) ms_per_gate[K - kMinK] = -1.0;
         if (!width_ok(c->nq, K)) continue;
-        const auto blocks = fuse(gates, ngates, c->nq, K, 256);
+        std::vector<Op> ops;
+        for (uint32_t i = 0; i < ngates; ++i) ops.push_back(gate_op(gates[i]));
+        const auto blocks = fuse(ops, c->nq, K, 256);
         // warm-up on the first block, then time the whole sample
         int rc = apply_blocks(c, std::vector<Block>(blocks.begin(), blocks.begin() + 1));
         if (rc) return rc;

@@ -15,21 +15,38 @@ namespace sqf {
 constexpr int kTileLow = 7;        // non-block bits inside a tile: 128 MFMA columns
 constexpr int kMinK = 4, kMaxK = 6;
 
-// One fused block: sorted qubits and the 2^K x 2^K complex matrix
-// (row-major, acting on the block index sum_j b_j 2^j, b_j = bit qubits[j]).
+using cd = std::complex<double>;
+
+// One operation before fusion: sorted qubits and its 2^n x 2^n matrix
+// (row-major, acting on the local index sum_j b_j 2^j, b_j = bit qs[j]).
+struct Op {
+    uint32_t n = 0;
+    uint32_t qs[kMaxK] = {};
+    std::vector<cd> m;
+};
+
+// One fused block: the same, with exactly K qubits, and how many ops it holds.
 struct Block {
     uint32_t K;
     uint32_t qubits[kMaxK];
-    std::vector<std::complex<double>> m;
+    std::vector<cd> m;
     uint32_t ngates;
 };
 
-// Greedy fusion with commutation lookahead: a gate joins the open block if
+// Greedy fusion with commutation lookahead: an op joins the open block if
 // its qubits fit (at most width in total) and it shares no qubit with any
-// earlier gate left out of the block (so moving it forward is exact).
-std::vector<Block> fuse(const suqa_hip_gate* g, uint32_t n, uint32_t nq, uint32_t width, uint32_t window);
+// earlier op left out of the block (so moving it forward is exact).
+std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width, uint32_t window);
 
-// 2x2 / controlled matrices of the workload gates (suqa_hip.h SUQA_GATE_*).
-void gate_matrix(const suqa_hip_gate& g, std::vector<uint32_t>& qs, std::vector<std::complex<double>>& m);
+// Matrices of the workload gates (suqa_hip.h SUQA_GATE_*).
+Op gate_op(const suqa_hip_gate& g);
+// exp(i phase P) of a Pauli string, controlled on the gc_mask qubits; false
+// (and no op) if it spans more than kMaxK qubits or is malformed.
+bool pauli_op(const suqa_hip_pauli_op& p, Op& out);
+// H on q and the controlled phase diag(1,1,1,e^{i phase}) on {a, b}.
+Op h_op(uint32_t q);
+Op cphase_op(uint32_t a, uint32_t b, double phase);
+// The reference's no-swap inverse QFT on qs[0..n) as H / controlled-phase ops.
+void qft_inverse_ops(const uint32_t* qs, uint32_t n, std::vector<Op>& out);
 
 }  // namespace sqf

@@ -76,18 +76,6 @@ struct PairArgs {
     PauliCoef pc;
 };
 
-// psi'_a = c psi_a + zeta s psi_b, zeta in {1,-1,i,-i}; t = +-s carries the sign.
-__device__ __forceinline__ void pauli_out(double ar, double ai, double br, double bi, double c, double t,
-                                          bool imag, double& orr, double& oi) {
-    if (imag) {   // zeta = +-i: re = ar*c - (+-s)*bi, im = ai*c + (+-s)*br
-        orr = ar * c + bi * (-t);
-        oi = ai * c + br * t;
-    } else {      // zeta = +-1
-        orr = ar * c + br * t;
-        oi = ai * c + bi * t;
-    }
-}
-
 __device__ __forceinline__ Vec<2> shx2(const Vec<2>& v, int m) {
     Vec<2> r;
     r.x[0] = __shfl_xor(v.x[0], m, 64);

@@ -112,3 +112,43 @@ def trotter_terms(nsys, nterms, seed):
         t.coef = 2.0 * r.doub() - 1.0
         out.append(t)
     return out
+
+
+def trotter_step_ops(terms, dt, gc_mask=0):
+    """One first-order Trotter step exp(-i dt H), H = sum_t coef_t P_t, as
+    Pauli rotations exp(i (-coef dt) P) controlled on gc_mask, in term order."""
+    return [(list(t.qs), list(t.types), -t.coef * dt, gc_mask) for t in terms]
+
+
+def qpe_trotter(eng, nsys, nreg, terms, dt, steps, width=None):
+    """Phase estimation of U = (Trotter step)^steps on an nsys-qubit system
+    with an nreg-qubit register on qubits nsys..nsys+nreg-1: H on the
+    register; for trg = nreg-1 .. 0, U^(2^(nreg-1-trg)) controlled on
+    register qubit trg; the reference's no-swap inverse QFT on the register
+    (suqa.cu:649-661).  eng is a capi.Engine (FP64, one sweep per rotation)
+    or a capi.F32Engine (width-qubit fused blocks).  Returns the 2^nreg-bin
+    histogram from prob_filter in FP64, bin k = register value k (LSB on the
+    first register qubit)."""
+    from suqa_amd import capi
+    reg = [nsys + k for k in range(nreg)]
+    f32 = isinstance(eng, capi.F32Engine)
+    had = capi.gates_from([G(H, q, [], 0.0) for q in reg])
+    if f32:
+        eng.run_gates(had, width)
+    else:
+        eng.run_gates(had)
+    for trg in range(nreg - 1, -1, -1):
+        powr = 1 << (nreg - 1 - trg)
+        ops = capi.pauli_ops(trotter_step_ops(terms, dt, 1 << reg[trg]))
+        if f32:
+            allops = capi.pauli_ops(trotter_step_ops(terms, dt, 1 << reg[trg]) * (powr * steps))
+            eng.run_paulis(allops, width)
+        else:
+            for _ in range(powr * steps):
+                eng.run_paulis(ops)
+    if f32:
+        eng.qft_inverse(reg, width)
+    else:
+        eng.qft_inverse(reg)
+    return [eng.prob_filter(reg, [(k >> j) & 1 for j in range(nreg)]) for k in range(1 << nreg)]
+
