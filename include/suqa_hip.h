@@ -152,6 +152,20 @@ typedef struct {
 /* Applies gates[0..ngates) in order, asynchronously on the context stream. */
 int suqa_hip_run_gates(suqa_hip_ctx* ctx, const suqa_hip_gate* gates, uint32_t ngates);
 
+/* Gate passes (extension; the reference runs one sweep per gate).  With
+ * bits = t in 1..12, gate calls are queued and applied in passes: runs of
+ * consecutive gates whose flipped qubits fit in one t-qubit tile are applied
+ * to LDS-resident tiles of 2^t amplitudes in one HBM round trip, in order and
+ * with the one-sweep arithmetic (bit-identical results).  Every call that
+ * reads or exposes the state (norms, probabilities, measurements, apply_W,
+ * phase_list, download/upload, state_info, synchronize, timing_report)
+ * applies the queue first.  bits = 0 gives one sweep per gate. */
+int suqa_hip_set_tile_bits(suqa_hip_ctx* ctx, uint32_t bits);
+int suqa_hip_get_tile_bits(suqa_hip_ctx* ctx, uint32_t* bits);
+/* Counters since setup: passes launched, gates applied in passes, gates
+ * applied as lone sweeps while passes were on. */
+int suqa_hip_pass_stats(suqa_hip_ctx* ctx, uint64_t* passes, uint64_t* pass_gates, uint64_t* single_sweeps);
+
 /* One Pauli-string rotation exp(i phase P), P = prod_k sigma_{types[k]} on
  * qs[k] (k < w <= 8), controlled on the gc_mask qubits: the operation
  * suqa_hip_apply_pauli_rotation_general performs, as a list element. */

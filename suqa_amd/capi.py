@@ -77,6 +77,9 @@ SIGNATURES = {
     "suqa_hip_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "suqa_hip_device_free": (C.c_int, [C.c_void_p]),
     "suqa_hip_run_paulis": (C.c_int, [CTX, C.POINTER(PauliOp), U32]),
+    "suqa_hip_set_tile_bits": (C.c_int, [CTX, U32]),
+    "suqa_hip_get_tile_bits": (C.c_int, [CTX, PU32]),
+    "suqa_hip_pass_stats": (C.c_int, [CTX, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     # FP32 fused-block engine (extension)
     "suqa_hip_f32_last_error": (C.c_char_p, []),
     "suqa_hip_f32_setup": (C.c_int, [U32, C.POINTER(CTX)]),
@@ -288,6 +291,19 @@ class Engine:
 
     def run_paulis(self, ops):
         check(lib().suqa_hip_run_paulis(self.ctx, ops, len(ops)))
+
+    def set_tile_bits(self, bits):
+        check(lib().suqa_hip_set_tile_bits(self.ctx, bits))
+
+    def tile_bits(self):
+        v = U32()
+        check(lib().suqa_hip_get_tile_bits(self.ctx, C.byref(v)))
+        return v.value
+
+    def pass_stats(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().suqa_hip_pass_stats(self.ctx, C.byref(a), C.byref(b), C.byref(c)))
+        return {"passes": a.value, "pass_gates": b.value, "single_sweeps": c.value}
 
     def apply_gate(self, g):
         self.run_gates((Gate * 1)(g), 1)

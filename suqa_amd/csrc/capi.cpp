@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -47,7 +48,20 @@ int read_result(suqa_hip_ctx* c, int n) {
     return SUQA_OK;
 }
 
+// Tile bits of a new context: SUQA_HIP_TILE_BITS (0 = one sweep per gate,
+// 1..12) when set to a valid value, else kDefaultTileBits.
+constexpr uint32_t kDefaultTileBits = 0;
+uint32_t default_tile_bits() {
+    const char* e = std::getenv("SUQA_HIP_TILE_BITS");
+    if (!e || !*e) return kDefaultTileBits;
+    char* end = nullptr;
+    const long v = std::strtol(e, &end, 10);
+    if (*end || v < 0 || v > (long)sqe::kMaxTileBits) return kDefaultTileBits;
+    return (uint32_t)v;
+}
+
 int free_state(suqa_hip_ctx* c) {
+    c->queue.clear();
     if (c->d_data) SQE_HIP(hipFree(c->d_data));
     if (c->d_partials) SQE_HIP(hipFree(c->d_partials));
     c->d_partials = nullptr;
@@ -209,6 +223,7 @@ int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out) {
     SQE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     c->num_cu = ncu > 0 ? ncu : 256;
     SQE_HIP(hipStreamCreate(&c->stream));
+    c->tile_bits = default_tile_bits();
     void* p = nullptr;
     SQE_HIP(hipMalloc(&p, sizeof(double) * 4));
     c->d_result = static_cast<double*>(p);
@@ -584,6 +599,7 @@ int suqa_hip_qms_apply_W(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmask, uint32
 
 int suqa_hip_state_info(suqa_hip_ctx* c, uint32_t* nq, uint64_t* N, double** re, double** im) {
     SQE_CTX(c);
+    SQE_HIP(sqe::flush(c));
     if (nq) *nq = c->nq;
     if (N) *N = c->N;
     if (re) *re = c->d_re;
@@ -593,6 +609,7 @@ int suqa_hip_state_info(suqa_hip_ctx* c, uint32_t* nq, uint64_t* N, double** re,
 
 int suqa_hip_download(suqa_hip_ctx* c, double* re, double* im) {
     SQE_CTX(c);
+    SQE_HIP(sqe::flush(c));
     if (re) SQE_HIP(hipMemcpyAsync(re, c->d_re, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (im) SQE_HIP(hipMemcpyAsync(im, c->d_im, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SQE_HIP(hipStreamSynchronize(c->stream));
@@ -601,6 +618,7 @@ int suqa_hip_download(suqa_hip_ctx* c, double* re, double* im) {
 
 int suqa_hip_upload(suqa_hip_ctx* c, const double* re, const double* im) {
     SQE_CTX(c);
+    SQE_HIP(sqe::flush(c));
     if (re) SQE_HIP(hipMemcpyAsync(c->d_re, re, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (im) SQE_HIP(hipMemcpyAsync(c->d_im, im, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     SQE_HIP(hipStreamSynchronize(c->stream));
@@ -609,7 +627,32 @@ int suqa_hip_upload(suqa_hip_ctx* c, const double* re, const double* im) {
 
 int suqa_hip_synchronize(suqa_hip_ctx* c) {
     SQE_CTX(c);
+    SQE_HIP(sqe::flush(c));
     SQE_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+int suqa_hip_set_tile_bits(suqa_hip_ctx* c, uint32_t bits) {
+    SQE_CTX(c);
+    if (bits != 0 && (bits < sqe::kMinTileBits || bits > sqe::kMaxTileBits))
+        return fail(SUQA_ERR_ARG, "tile bits must be 0 (one sweep per gate) or 1..12");
+    SQE_HIP(sqe::flush(c));
+    c->tile_bits = bits;
+    return SUQA_OK;
+}
+
+int suqa_hip_get_tile_bits(suqa_hip_ctx* c, uint32_t* bits) {
+    SQE_CTX(c);
+    if (bits) *bits = c->tile_bits;
+    return SUQA_OK;
+}
+
+int suqa_hip_pass_stats(suqa_hip_ctx* c, uint64_t* passes, uint64_t* pass_gates, uint64_t* single_sweeps) {
+    SQE_CTX(c);
+    SQE_HIP(sqe::flush(c));
+    if (passes) *passes = c->passes;
+    if (pass_gates) *pass_gates = c->pass_gates;
+    if (single_sweeps) *single_sweeps = c->single_sweeps;
     return SUQA_OK;
 }
 
@@ -665,6 +708,7 @@ int suqa_hip_timing_reset(suqa_hip_ctx* c) {
 
 int suqa_hip_timing_report(suqa_hip_ctx* c, suqa_hip_kstat* out, uint32_t max, uint32_t* n) {
     SQE_CTX(c);
+    SQE_HIP(sqe::flush(c));
     sqe::timing_flush(c);
     uint32_t k = 0;
     for (; k < c->stats.size() && k < max; ++k) {

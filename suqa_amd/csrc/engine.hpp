@@ -47,6 +47,20 @@ struct PauliCoef {
     uint32_t base_neg;
 };
 
+// A gate sweep as the launchers receive it: a pair operation (OP_*) on the
+// pairs (i, i ^ X) with (i & M) == V, or a phase rotation (kPhaseOp) of the
+// amplitudes with (i & M) == V, s negated where par(i & zmask) == 1.
+constexpr int kPhaseOp = 4;
+struct GateDesc {
+    int kind;
+    uint64_t M, V, X;
+    PauliCoef pc;        // pair operations
+    double c, s;         // phase rotations
+    uint64_t zmask;
+    const char* tag;
+    double alg_bytes;
+};
+
 struct KStat {
     std::string name;
     uint64_t launches = 0;
@@ -80,6 +94,12 @@ struct suqa_hip_ctx {
     // one-shot QFT phase table (suqa.cu:611-625)
     bool rphase_filled = false;
     std::vector<double> rphase;
+    // gate passes: with tile_bits > 0, pair and phase sweeps are queued and
+    // applied in passes of up to kMaxPass gates over tiles of 2^tile_bits
+    // amplitudes held in LDS (tiled.hip); every other operation flushes first
+    uint32_t tile_bits = 0;
+    std::vector<sqe::GateDesc> queue;
+    uint64_t passes = 0, pass_gates = 0, single_sweeps = 0;
     // timing
     bool timing = false;
     struct Rec {
@@ -111,6 +131,15 @@ hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_row, uint32_t wmask, ui
 hipError_t launch_copy(suqa_hip_ctx* c, void* dst, const void* src, uint64_t nbytes);
 // Number of (sum, sum) partial pairs launch_norm needs for a 2^nq state.
 uint64_t partial_pairs_capacity(uint64_t N);
+
+// One-gate sweeps (kernels.hip), called directly or by the pass scheduler.
+hipError_t launch_pair_now(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,
+                           const char* tag, double alg_bytes);
+hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
+                            const char* tag, double alg_bytes);
+// Applies the queued gates in tile passes (tiled.hip); a no-op when empty.
+hipError_t flush(suqa_hip_ctx* c);
+constexpr uint32_t kMinTileBits = 1, kMaxTileBits = 12;
 
 // timing hooks (capi.cpp)
 void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes);

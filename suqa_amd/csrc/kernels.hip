@@ -12,12 +12,11 @@
 // is compiled with -ffp-contract=off, so amplitudes match the reference CPU
 // path bit for bit (suqa_cpu.hpp:49-66 forms x*c - y*s, y*c + x*s).
 #include "engine.hpp"
+#include "sweep_math.hpp"
 
 namespace sqe {
 
 namespace {
-
-constexpr double kTwoSqInv = 0.7071067811865475;   // suqa.cuh:57
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
 
@@ -81,25 +80,6 @@ __device__ __forceinline__ Vec<2> shx2(const Vec<2>& v, int m) {
     r.x[0] = __shfl_xor(v.x[0], m, 64);
     r.x[1] = __shfl_xor(v.x[1], m, 64);
     return r;
-}
-
-// Output of element j of component `im_lane` (0: re, 1: im) whose value is v,
-// given the partner k = j ^ X: ps = partner, same component; pc = partner,
-// other component.  a_side: j plays role a (the reference's lower index).
-template <int OP>
-__device__ __forceinline__ double pair_out(bool a_side, bool im_lane, double v, double ps, double pc, uint64_t k,
-                                           const PauliCoef& pc_) {
-    if (OP == OP_SWAP) return ps;
-    if (OP == OP_Y) {   // suqa_cpu.hpp:116-118
-        if (a_side) return im_lane ? pc * -1.0 : pc;   // re_a' = im_b, im_a' = re_b * -1
-        return im_lane ? pc : pc * -1.0;               // re_b' = im_a * -1, im_b' = re_a
-    }
-    if (OP == OP_H) return a_side ? kTwoSqInv * (v + ps) : kTwoSqInv * (ps - v);   // suqa_cpu.hpp:195-198
-    // OP_PAULI: psi_j' = c psi_j + base (-1)^{par(k&YZ)} s psi_k
-    const uint32_t par = (uint32_t)__popcll(k & pc_.yz) & 1u;
-    const double t = (pc_.base_neg ^ par) ? -pc_.s : pc_.s;
-    if (!pc_.base_imag) return v * pc_.c + ps * t;
-    return im_lane ? v * pc_.c + pc * t : v * pc_.c + pc * (-t);
 }
 
 // Strided mode: a wave serves 16 double2 items; lanes 0-15 load re_a,
@@ -231,8 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_phase(double* __restrict__ re, doubl
         const uint64_t j = ia + (uint64_t)e;
         const uint32_t par = (uint32_t)__popcll(j & a.zmask) & 1u;
         const double s = par ? -a.s : a.s;
-        // re' = re*c - im*s, im' = im*c + re*s
-        out.x[e] = im_lane ? v.x[e] * a.c + o.x[e] * s : v.x[e] * a.c - o.x[e] * s;
+        out.x[e] = phase_out(im_lane, v.x[e], o.x[e], a.c, s);
         if (DL && (j & a.M) != a.V) out.x[e] = v.x[e];
     }
     if (ok) out.store(base + ia);
@@ -441,6 +420,37 @@ bool make_sel(uint32_t nq, uint64_t M, uint64_t V, Sel* out) {
 
 hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,
                        const char* tag, double alg_bytes) {
+    if (c->tile_bits == 0) return launch_pair_now(c, op, sel, xmask, pc, tag, alg_bytes);
+    GateDesc d{};
+    d.kind = op;
+    d.M = sel_mask(sel);
+    d.V = sel.fval;
+    d.X = xmask;
+    d.pc = pc;
+    d.tag = tag;
+    d.alg_bytes = alg_bytes;
+    c->queue.push_back(d);
+    return hipSuccess;
+}
+
+hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
+                        const char* tag, double alg_bytes) {
+    if (c->tile_bits == 0) return launch_phase_now(c, sel, cs, sn, zmask, tag, alg_bytes);
+    GateDesc d{};
+    d.kind = kPhaseOp;
+    d.M = sel_mask(sel);
+    d.V = sel.fval;
+    d.c = cs;
+    d.s = sn;
+    d.zmask = zmask;
+    d.tag = tag;
+    d.alg_bytes = alg_bytes;
+    c->queue.push_back(d);
+    return hipSuccess;
+}
+
+hipError_t launch_pair_now(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,
+                           const char* tag, double alg_bytes) {
     const uint64_t M = sel_mask(sel), V = sel.fval;
     if (((M | xmask) & kLowMask) != 0) {
         PairDLArgs a;
@@ -495,8 +505,8 @@ hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, 
     return hipGetLastError();
 }
 
-hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
-                        const char* tag, double alg_bytes) {
+hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
+                            const char* tag, double alg_bytes) {
     PhaseArgs a;
     a.c = cs;
     a.s = sn;
@@ -519,6 +529,7 @@ hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, u
 
 hipError_t launch_phase_list(suqa_hip_ctx* c, const Sel& sel, uint32_t q0, uint32_t size_mask,
                              const double* cs, const double* sn) {
+    if (hipError_t e = flush(c)) return e;
     PhaseListArgs a;
     a.fb = sel.fb;
     a.fval = sel.fval;
@@ -544,6 +555,7 @@ uint64_t partial_pairs_capacity(uint64_t N) {
 
 hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
                        double alg_bytes) {
+    if (hipError_t e = flush(c)) return e;
     NormArgs a;
     a.split = split_bit;
     a.M = sel_mask(sel);
@@ -579,6 +591,7 @@ hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, cons
 }
 
 hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale) {
+    if (hipError_t e = flush(c)) return e;
     const uint64_t nitems = c->N / 2;
     // the discarded half is written without being read: 24 B per amplitude
     timing_begin(c, "collapse", 24.0 * (double)c->N);
@@ -589,6 +602,7 @@ hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double
 }
 
 hipError_t launch_scale(suqa_hip_ctx* c, double f) {
+    if (hipError_t e = flush(c)) return e;
     const uint64_t nitems = c->N;   // 2N doubles as N double2
     timing_begin(c, "scale", 32.0 * (double)c->N);
     hipLaunchKernelGGL(k_scale, dim3(grid_for(nitems)), dim3(kBlock), 0, c->stream, c->d_data, nitems, f);
@@ -597,6 +611,7 @@ hipError_t launch_scale(suqa_hip_ctx* c, double f) {
 }
 
 hipError_t launch_init(suqa_hip_ctx* c) {
+    if (hipError_t e = flush(c)) return e;
     const uint64_t nitems = c->N;
     timing_begin(c, "init", 16.0 * (double)c->N);
     hipLaunchKernelGGL(k_init, dim3(grid_for(nitems)), dim3(kBlock), 0, c->stream, c->d_data, nitems);
@@ -605,6 +620,7 @@ hipError_t launch_init(suqa_hip_ctx* c) {
 }
 
 hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_row, uint32_t wmask, uint32_t b0) {
+    if (hipError_t e = flush(c)) return e;
     const uint64_t half = c->N / 2;
     timing_begin(c, "qms_W", 32.0 * (double)c->N);
     hipLaunchKernelGGL(k_qms_W, dim3(grid_for(half)), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, half, d_row,
