@@ -4,17 +4,25 @@
 
 Workload: 30 qubits, complex FP64 (16 GiB in HBM), |0> then an H layer, then
 layers of 30 random gates from {H, X, RX, RZ, U1, CX, MCX(3)} drawn with
-pcg32 seed 7 (suqa_amd/workloads.py).  One step = one layer (30 gates, one
-HIP sweep each).  W warmup layers are applied untimed, then exactly K layers
-are timed between stream synchronizations; the default W=2, K=198 applies
-all 200 layers.  The 8 final measure_qbit calls run after the timed region.
+pcg32 seed 7 (suqa_amd/workloads.py).  One step = one layer (30 gates).
+The FP64 engine applies the gates in passes: runs of consecutive gates whose
+flipped qubits fit in one tile are applied to LDS-resident tiles in one HBM
+round trip, bit-identical to one sweep per gate.  The tile size is autotuned
+first (each candidate, and one sweep per gate, timed on 5 layers; the state
+is then re-initialised).  W warmup layers are applied untimed, then exactly
+K layers are timed between stream synchronizations; the default W=2, K=198
+applies all 200 layers.  The 8 final measure_qbit calls run after the timed
+region.
 
 value = algorithmic bytes of the timed layers / wall time (GB/s); a gate's
 algorithmic bytes are 32 B per amplitude it modifies (H/X/RX/RZ 32N,
-U1/CX 16N, MCX3 4N).  roofline: the dominant kernel family, its algorithmic
-bytes over the sum of its per-launch HIP-event durations, against the
-8 TB/s HBM3E peak.  cpu_baseline: the oracle (a single-threaded restatement
-of the reference CPU path) on the first layer at 27 qubits.
+U1/CX 16N, MCX3 4N), as if each gate were its own sweep, so value exceeds
+the HBM roof when passes hold several gates.  roofline: the dominant kernel
+(the tile pass), the HBM bytes it must move (every amplitude of the tiles it
+touches read and written once, 32 B) over the sum of its per-launch HIP-event
+durations, against the 8 TB/s HBM3E peak.  cpu_baseline: the oracle (a
+single-threaded restatement of the reference CPU path) on the first layer at
+27 qubits.
 
 The QMS half of the metric is reported under "extra":
   qms_plain  -- the reference's frustrated-triangle QMS (beta 1, 1000 steps,
@@ -178,6 +186,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-qms", action="store_true", help="skip the QMS legs (profiling runs)")
     ap.add_argument("--no-fused", action="store_true", help="skip the FP32 fused leg (profiling runs)")
+    ap.add_argument("--tile-bits", type=int, default=-1,
+                    help="FP64 tile size (0: one sweep per gate); default: autotuned")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU sharding is not part of "
@@ -192,9 +202,18 @@ def main():
     eng = capi.Engine(n)
     peak_copy = hbm_copy_calibration(capi, eng)
     log(f"copy-kernel bandwidth {peak_copy:.0f} GB/s")
+    tune = None
+    if a.tile_bits < 0:
+        best, tms = eng.autotune_tile_bits(capi.gates_from(gates[:5 * n]), (0, 8, 9, 10, 11, 12))
+        tune = {"tile_bits": best, "ms_per_gate": {str(k): round(v, 4) for k, v in tms.items()}}
+        log(f"autotuned tile bits {best}: {tms}")
+    else:
+        eng.set_tile_bits(a.tile_bits)
+    eng.init_state()
     for q in range(n):
         eng.h(q)
     eng.synchronize()
+    ps0 = eng.pass_stats()
     per = n
     for s in range(a.warmup):
         eng.run_gates(arr, per, s * per)
@@ -210,13 +229,19 @@ def main():
     t1 = time.perf_counter()
     rep = eng.timing_report()
     eng.timing(False)
+    ps1 = eng.pass_stats()
     wall = t1 - t0
     alg = sum(wl.gate_bytes(g, n) for g in gates[a.warmup * per:layers * per])
     value = alg / wall / 1e9
-    fam = max((k for k in rep if k not in ("norm_final",)), key=lambda k: rep[k]["ms"])
+    fam = max(rep, key=lambda k: rep[k]["ms"])
     achieved = rep[fam]["bytes"] / (rep[fam]["ms"] * 1e-3) / 1e9
     alg_per_launch = rep[fam]["bytes"] / rep[fam]["launches"]
     traffic = alg_per_launch * PMC_TRAFFIC_RATIO[fam] if fam in PMC_TRAFFIC_RATIO else None
+    eng_tile_bits = eng.tile_bits()
+    npass = ps1["passes"] - ps0["passes"]
+    pass_info = {"passes": npass, "gates_in_passes": ps1["pass_gates"] - ps0["pass_gates"],
+                 "single_sweeps": ps1["single_sweeps"] - ps0["single_sweeps"],
+                 "gates_per_pass": round((ps1["pass_gates"] - ps0["pass_gates"]) / npass, 2) if npass else None}
     f32, fres = (None, None) if a.no_fused else fused_leg(capi, wl, eng, gates, n, layers)
     # the final measurements of the workload (not in the timed region)
     tm = time.perf_counter()
@@ -246,8 +271,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: pcg32 seed-7 random circuit",
-        "config": {"workload": "plain FP64 gate sweep, random circuit {H,X,RX,RZ,U1,CX,MCX3}",
-                   "qubits": n, "gates_per_step": per, "layers_total": layers, "seed": 7},
+        "config": {"workload": "FP64 gate sweeps, random circuit {H,X,RX,RZ,U1,CX,MCX3}",
+                   "qubits": n, "gates_per_step": per, "layers_total": layers, "seed": 7,
+                   "tile_bits": eng_tile_bits},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic), "kernel": fam,
@@ -255,6 +281,7 @@ def main():
                      "avg_launch_ms": round(rep[fam]["ms"] / rep[fam]["launches"], 4)},
         "cpu_baseline": cpu,
         "extra": dict({"copy_kernel_gbs": round(peak_copy, 1), "kernel_families": fams,
+                       "tile_autotune": tune, "passes": pass_info,
                        "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm, "fused": fres}, **qres),
     }
     print(json.dumps(out), flush=True)

@@ -159,9 +159,16 @@ int suqa_hip_run_gates(suqa_hip_ctx* ctx, const suqa_hip_gate* gates, uint32_t n
  * with the one-sweep arithmetic (bit-identical results).  Every call that
  * reads or exposes the state (norms, probabilities, measurements, apply_W,
  * phase_list, download/upload, state_info, synchronize, timing_report)
- * applies the queue first.  bits = 0 gives one sweep per gate. */
+ * applies the queue first.  bits = 0 gives one sweep per gate.  A new
+ * context takes SUQA_HIP_TILE_BITS from the environment, else 11. */
 int suqa_hip_set_tile_bits(suqa_hip_ctx* ctx, uint32_t bits);
 int suqa_hip_get_tile_bits(suqa_hip_ctx* ctx, uint32_t* bits);
+/* Autotuner: applies the gate sample once per candidate tile size (so the
+ * state changes: re-initialise it afterwards), times it, keeps the fastest
+ * in *best and sets it; ms_per_gate[k] (may be NULL) gets each candidate's
+ * wall time per gate. */
+int suqa_hip_autotune_tile_bits(suqa_hip_ctx* ctx, const suqa_hip_gate* gates, uint32_t ngates,
+                                const uint32_t* candidates, uint32_t ncand, uint32_t* best, double* ms_per_gate);
 /* Counters since setup: passes launched, gates applied in passes, gates
  * applied as lone sweeps while passes were on. */
 int suqa_hip_pass_stats(suqa_hip_ctx* ctx, uint64_t* passes, uint64_t* pass_gates, uint64_t* single_sweeps);

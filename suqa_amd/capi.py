@@ -79,6 +79,7 @@ SIGNATURES = {
     "suqa_hip_run_paulis": (C.c_int, [CTX, C.POINTER(PauliOp), U32]),
     "suqa_hip_set_tile_bits": (C.c_int, [CTX, U32]),
     "suqa_hip_get_tile_bits": (C.c_int, [CTX, PU32]),
+    "suqa_hip_autotune_tile_bits": (C.c_int, [CTX, C.POINTER(Gate), U32, PU32, U32, PU32, PD]),
     "suqa_hip_pass_stats": (C.c_int, [CTX, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     # FP32 fused-block engine (extension)
     "suqa_hip_f32_last_error": (C.c_char_p, []),
@@ -141,12 +142,16 @@ def check(rc):
 class Engine:
     """One GPU state vector behind the reference's API names."""
 
-    def __init__(self, nq):
+    def __init__(self, nq, tile_bits=None):
+        """tile_bits: None keeps the library default (SUQA_HIP_TILE_BITS or 11);
+        0 applies one sweep per gate."""
         self.ctx = CTX()
         check(lib().suqa_hip_setup(nq, C.byref(self.ctx)))
         self.nq = nq
         self.N = 1 << nq
         self.gc_mask = 0
+        if tile_bits is not None:
+            self.set_tile_bits(tile_bits)
 
     def close(self):
         if self.ctx:
@@ -299,6 +304,14 @@ class Engine:
         v = U32()
         check(lib().suqa_hip_get_tile_bits(self.ctx, C.byref(v)))
         return v.value
+
+    def autotune_tile_bits(self, gates, candidates=(8, 9, 10, 11, 12)):
+        """Times the gate sample at each tile size (changing the state), keeps the fastest."""
+        best = U32()
+        t = (C.c_double * len(candidates))()
+        check(lib().suqa_hip_autotune_tile_bits(self.ctx, gates, len(gates), _u32(candidates), len(candidates),
+                                                C.byref(best), t))
+        return best.value, {b: t[k] for k, b in enumerate(candidates)}
 
     def pass_stats(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()

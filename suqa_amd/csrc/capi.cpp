@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -49,8 +50,9 @@ int read_result(suqa_hip_ctx* c, int n) {
 }
 
 // Tile bits of a new context: SUQA_HIP_TILE_BITS (0 = one sweep per gate,
-// 1..12) when set to a valid value, else kDefaultTileBits.
-constexpr uint32_t kDefaultTileBits = 0;
+// 1..12) when set to a valid value, else kDefaultTileBits (the fastest size
+// measured on the 30-qubit random circuit).
+constexpr uint32_t kDefaultTileBits = 11;
 uint32_t default_tile_bits() {
     const char* e = std::getenv("SUQA_HIP_TILE_BITS");
     if (!e || !*e) return kDefaultTileBits;
@@ -639,6 +641,36 @@ int suqa_hip_set_tile_bits(suqa_hip_ctx* c, uint32_t bits) {
     SQE_HIP(sqe::flush(c));
     c->tile_bits = bits;
     return SUQA_OK;
+}
+
+int suqa_hip_autotune_tile_bits(suqa_hip_ctx* c, const suqa_hip_gate* gates, uint32_t ngates,
+                                const uint32_t* candidates, uint32_t ncand, uint32_t* best, double* ms_per_gate) {
+    SQE_CTX(c);
+    if (!gates || !ngates || !candidates || !ncand || !best) return fail(SUQA_ERR_ARG, "empty sample or candidates");
+    for (uint32_t k = 0; k < ncand; ++k)
+        if (candidates[k] > sqe::kMaxTileBits) return fail(SUQA_ERR_ARG, "tile bits must be 0..12");
+    const uint32_t keep = c->tile_bits;
+    double best_ms = 1e300;
+    *best = keep;
+    for (uint32_t k = 0; k < ncand; ++k) {
+        int rc = suqa_hip_set_tile_bits(c, candidates[k]);
+        if (rc) return rc;
+        // warm-up on a few gates, then the timed sample
+        rc = suqa_hip_run_gates(c, gates, std::min<uint32_t>(ngates, 8));
+        if (!rc) rc = suqa_hip_synchronize(c);
+        if (rc) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = suqa_hip_run_gates(c, gates, ngates);
+        if (!rc) rc = suqa_hip_synchronize(c);
+        if (rc) return rc;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (ms_per_gate) ms_per_gate[k] = ms / ngates;
+        if (ms < best_ms) {
+            best_ms = ms;
+            *best = candidates[k];
+        }
+    }
+    return suqa_hip_set_tile_bits(c, *best);
 }
 
 int suqa_hip_get_tile_bits(suqa_hip_ctx* c, uint32_t* bits) {

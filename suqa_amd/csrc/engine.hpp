@@ -140,6 +140,7 @@ hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double s
 // Applies the queued gates in tile passes (tiled.hip); a no-op when empty.
 hipError_t flush(suqa_hip_ctx* c);
 constexpr uint32_t kMinTileBits = 1, kMaxTileBits = 12;
+constexpr size_t kAutoFlush = 256;   // queued gates that trigger a flush (keeps the GPU busy)
 
 // timing hooks (capi.cpp)
 void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes);

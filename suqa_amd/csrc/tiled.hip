@@ -10,8 +10,19 @@
 // anywhere: outside T they are constant over a tile and are tested once per
 // tile.  A pass is one kernel: each workgroup loads a tile (the 2^t
 // amplitudes that share the bits outside T) into LDS, applies the pass's
-// gates one after the other with a barrier between gates, and writes the
-// tile back, so the state crosses HBM once per pass instead of once per gate.
+// gates in order, and writes the tile back, so the state crosses HBM once per
+// pass instead of once per gate.
+//
+// Inside a pass, consecutive gates whose flipped bits fit in 3 tile bits F
+// form a register step: each thread takes the 8 amplitudes that F spans
+// (a group) from LDS into registers, applies the step's gates to them one
+// after the other and writes them back, so LDS is read and written once per
+// step.  A gate's selection and parity split into a part on F, the same for
+// every group and precomputed on the host as 8-bit tables over the group's 8
+// elements, and a part on the other tile bits, tested once per group; the
+// per-element work is the gate arithmetic alone.  A gate that flips more than
+// 3 bits gets a step of its own, applied pair by pair in LDS.
+//
 // Gates keep their order and every amplitude is computed with the one-gate
 // sweeps' arithmetic (sweep_math.hpp), so results are bit-identical to
 // applying the gates one sweep at a time.  A bit fixed to the same value by
@@ -19,6 +30,7 @@
 // enumeration: tiles outside the selection are never read.
 #include <algorithm>
 #include <cstring>
+#include <utility>
 
 #include "engine.hpp"
 #include "sweep_math.hpp"
@@ -28,15 +40,26 @@ namespace sqe {
 namespace {
 
 constexpr uint32_t kMaxPass = 24;
-constexpr uint32_t kLowFixed = 4;   // qubits 0..3: 16 amplitudes = one 128-byte line
+constexpr uint32_t kLowFixed = 4;                 // qubits 0..3: 16 amplitudes = one 128-byte line
+constexpr uint32_t kSub = 3, kLoc = 1u << kSub;   // register step: 3 bits, 8 amplitudes per thread
+constexpr uint32_t kMaxChunks = (1u << kMaxTileBits) / 2 / kBlock;   // 16-byte chunks per array per thread
 
 typedef double dv2t __attribute__((ext_vector_type(2)));
+typedef double dv8 __attribute__((ext_vector_type(8)));    // a thread's 8 register amplitudes
+typedef uint32_t uv8 __attribute__((ext_vector_type(8)));  // their tile offsets
 
 struct TileGate {
     uint32_t kind;        // OP_SWAP / OP_Y / OP_H / OP_PAULI or kPhaseOp
     uint32_t pivot;       // tile position of the lowest partner bit (pair operations)
-    uint32_t xc, mc, vc;  // partner mask and in-tile selection, in tile positions
-    uint32_t yzc;         // in-tile parity mask (Y|Z positions or the phase's zmask)
+    uint32_t xc, mc, vc;  // partner mask and selection in tile positions (LDS steps)
+    uint32_t yzc;         // parity mask in tile positions (Y|Z positions or the phase's zmask)
+    // register steps: the local part over the step's bits F ...
+    uint32_t xl;          // partner mask over F
+    uint32_t lsel;        // bit j: group element j meets the selection's part on F
+    uint32_t lpar;        // bit j: parity of element j's bits on F in the parity mask
+    // ... and the part on the other tile bits (tile positions, zero on F)
+    uint32_t mcg, vcg, yzg;
+    uint32_t gtriv;       // mcg == 0: every group meets the selection
     uint32_t base_imag, base_neg;
     uint64_t mo, vo;      // selection bits outside the tile (global positions)
     uint64_t yzo;         // parity bits outside the tile
@@ -49,9 +72,13 @@ struct TileArgs {
     uint32_t t;           // tile bits
     uint32_t nfix;        // positions fixed in the tile enumeration (tile bits + common selection)
     uint32_t ngates;
-    uint32_t pad;
+    uint32_t nsteps;
     uint8_t fixpos[64];   // ascending
     uint8_t tpos[16];     // tile bit positions, ascending (tile index order)
+    struct Step {
+        uint8_t f[kSub];  // tile positions of the step's bits, ascending; f[0] = 0xff: LDS step
+        uint8_t g0, g1;   // gates [g0, g1)
+    } step[kMaxPass];
     TileGate g[kMaxPass];
 };
 
@@ -63,6 +90,7 @@ __device__ __forceinline__ uint64_t tile_offset(uint32_t e, const TileArgs& a) {
     return o;
 }
 
+// LDS step: pair gate over the whole tile, pair (u, u ^ X) per thread.
 template <int OP>
 __device__ __forceinline__ void tile_pairs(double* lre, double* lim, uint32_t half, const TileGate& g, uint32_t opar) {
     PauliCoef pc;
@@ -97,6 +125,106 @@ __device__ __forceinline__ void tile_pairs(double* lre, double* lim, uint32_t ha
     }
 }
 
+// A thread's 8 register amplitudes.  Gate steps take and return them by
+// value: with references, the compiler merges the stores of the switch cases
+// below into one store through a pointer phi and the amplitudes end up in
+// scratch memory.
+struct RM {
+    dv8 r, m;
+};
+
+// Pair gate with local partner mask XL: pair (J, J ^ XL) for each J without
+// the lowest bit of XL, expanded at compile time.  The roles come from the
+// uniform table lsel; gpass (this group meets the selection's other bits)
+// and gpar (parity of the other bits) are per thread.
+template <int XL, int OP, int J>
+__device__ __forceinline__ RM reg_pair_one(RM x, const TileGate& g, const PauliCoef& pc, bool gpass, uint32_t gpar) {
+    constexpr int PIV = XL & -XL;
+    if constexpr ((J & PIV) == 0) {
+        constexpr int u = J, w = J ^ XL;
+        const bool su = (g.lsel >> u) & 1u;
+        const bool sw = !su && ((g.lsel >> w) & 1u);
+        if (su || sw) {
+            // element j's partner is j ^ X: u's is w and w's is u; the selected
+            // one plays the reference's index i (a_side)
+            const uint32_t par_u = OP == OP_PAULI ? (((g.lpar >> w) & 1u) ^ gpar) : 0u;
+            const uint32_t par_w = OP == OP_PAULI ? (((g.lpar >> u) & 1u) ^ gpar) : 0u;
+            const double ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
+            const double nru = pair_out_par<OP>(su, false, ru, rw, mw, par_u, pc);
+            const double nmu = pair_out_par<OP>(su, true, mu, mw, rw, par_u, pc);
+            const double nrw = pair_out_par<OP>(sw, false, rw, ru, mu, par_w, pc);
+            const double nmw = pair_out_par<OP>(sw, true, mw, mu, ru, par_w, pc);
+            if (g.gtriv) {
+                x.r[u] = nru;
+                x.m[u] = nmu;
+                x.r[w] = nrw;
+                x.m[w] = nmw;
+            } else {
+                x.r[u] = gpass ? nru : ru;
+                x.m[u] = gpass ? nmu : mu;
+                x.r[w] = gpass ? nrw : rw;
+                x.m[w] = gpass ? nmw : mw;
+            }
+        }
+    }
+    return x;
+}
+
+template <int XL, int OP, int... J>
+__device__ __forceinline__ RM reg_pairs_seq(RM x, const TileGate& g, bool gpass, uint32_t gpar,
+                                            std::integer_sequence<int, J...>) {
+    PauliCoef pc;
+    pc.c = g.c;
+    pc.s = g.s;
+    pc.yz = 0;
+    pc.ny_odd = 0;
+    pc.base_imag = g.base_imag;
+    pc.base_neg = g.base_neg;
+    ((x = reg_pair_one<XL, OP, J>(x, g, pc, gpass, gpar)), ...);
+    return x;
+}
+
+template <int OP>
+__device__ __forceinline__ RM reg_pairs(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
+    constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
+    switch (g.xl) {
+    case 1: return reg_pairs_seq<1, OP>(x, g, gpass, gpar, seq);
+    case 2: return reg_pairs_seq<2, OP>(x, g, gpass, gpar, seq);
+    case 3: return reg_pairs_seq<3, OP>(x, g, gpass, gpar, seq);
+    case 4: return reg_pairs_seq<4, OP>(x, g, gpass, gpar, seq);
+    case 5: return reg_pairs_seq<5, OP>(x, g, gpass, gpar, seq);
+    case 6: return reg_pairs_seq<6, OP>(x, g, gpass, gpar, seq);
+    default: return reg_pairs_seq<7, OP>(x, g, gpass, gpar, seq);
+    }
+}
+
+// Phase rotation of the selected register amplitudes.
+template <int J>
+__device__ __forceinline__ RM reg_phase_one(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
+    if ((g.lsel >> J) & 1u) {
+        const uint32_t par = ((g.lpar >> J) & 1u) ^ gpar;
+        const double sn = par ? -g.s : g.s;
+        const double rr = x.r[J], mm = x.m[J];
+        const double nr = phase_out(false, rr, mm, g.c, sn);
+        const double nm = phase_out(true, mm, rr, g.c, sn);
+        if (g.gtriv) {
+            x.r[J] = nr;
+            x.m[J] = nm;
+        } else {
+            x.r[J] = gpass ? nr : rr;
+            x.m[J] = gpass ? nm : mm;
+        }
+    }
+    return x;
+}
+
+template <int... J>
+__device__ __forceinline__ RM reg_phase(RM x, const TileGate& g, bool gpass, uint32_t gpar,
+                                        std::integer_sequence<int, J...>) {
+    ((x = reg_phase_one<J>(x, g, gpass, gpar)), ...);
+    return x;
+}
+
 __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double* __restrict__ im, const TileArgs a) {
     extern __shared__ double lds[];
     const uint32_t tsz = 1u << a.t;
@@ -112,59 +240,111 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
     }
     base |= a.fixval;
 
-    // 1. tile -> LDS, 16-byte accesses, four in flight per array per thread
-    constexpr uint32_t kU = 4;
-    for (uint32_t e0 = 2 * threadIdx.x; e0 < tsz; e0 += 2 * kBlock * kU) {
-        dv2t vr[kU], vi[kU];
+    // 1. tile -> LDS: chunk k of a thread is tile element e = 2 tid + k 2 kBlock;
+    //    every load of the thread is issued before the LDS writes
+    const uint32_t e0 = 2 * threadIdx.x;
+    const uint32_t nch = tsz / (2 * kBlock);   // 0 when the tile is smaller than 2 kBlock
+    const uint64_t toff = e0 < tsz ? (base | tile_offset(e0, a)) : 0ull;
+    if (nch == 0) {
+        if (e0 < tsz) {
+            const dv2t vr = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(re + toff));
+            const dv2t vi = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(im + toff));
+            *reinterpret_cast<dv2t*>(lre + e0) = vr;
+            *reinterpret_cast<dv2t*>(lim + e0) = vi;
+        }
+    } else {
+        dv2t vr[kMaxChunks], vi[kMaxChunks];
 #pragma unroll
-        for (uint32_t u = 0; u < kU; ++u) {
-            const uint32_t e = e0 + u * 2 * kBlock;
-            if (e < tsz) {
-                const uint64_t g = base | tile_offset(e, a);
-                vr[u] = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(re + g));
-                vi[u] = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(im + g));
+        for (uint32_t k = 0; k < kMaxChunks; ++k) {
+            if (k < nch) {
+                const uint64_t g = toff | tile_offset(k * 2 * kBlock, a);
+                vr[k] = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(re + g));
+                vi[k] = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(im + g));
             }
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kU; ++u) {
-            const uint32_t e = e0 + u * 2 * kBlock;
-            if (e < tsz) {
-                *reinterpret_cast<dv2t*>(lre + e) = vr[u];
-                *reinterpret_cast<dv2t*>(lim + e) = vi[u];
+        for (uint32_t k = 0; k < kMaxChunks; ++k) {
+            if (k < nch) {
+                *reinterpret_cast<dv2t*>(lre + e0 + k * 2 * kBlock) = vr[k];
+                *reinterpret_cast<dv2t*>(lim + e0 + k * 2 * kBlock) = vi[k];
             }
         }
     }
     __syncthreads();
 
-    // 2. the gates, in order
-    for (uint32_t k = 0; k < a.ngates; ++k) {
-        const TileGate& g = a.g[k];
-        if ((base & g.mo) != g.vo) continue;   // uniform: this tile is outside the gate's selection
-        const uint32_t opar = (uint32_t)__popcll(base & g.yzo) & 1u;
-        switch (g.kind) {
-        case kPhaseOp:
-            for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
-                if ((e & g.mc) != g.vc) continue;
-                const uint32_t par = ((uint32_t)__popc(e & g.yzc) & 1u) ^ opar;
-                const double s = par ? -g.s : g.s;
-                const double r = lre[e], m = lim[e];
-                lre[e] = phase_out(false, r, m, g.c, s);
-                lim[e] = phase_out(true, m, r, g.c, s);
+    // 2. the steps, in order
+    for (uint32_t st = 0; st < a.nsteps; ++st) {
+        const auto& S = a.step[st];
+        if (S.f[0] == 0xffu) {   // one gate flipping more than kSub bits: pairs in LDS
+            const TileGate& g = a.g[S.g0];
+            if ((base & g.mo) == g.vo) {
+                const uint32_t opar = (uint32_t)__popcll(base & g.yzo) & 1u;
+                switch (g.kind) {
+                case OP_SWAP: tile_pairs<OP_SWAP>(lre, lim, tsz / 2, g, opar); break;
+                case OP_Y: tile_pairs<OP_Y>(lre, lim, tsz / 2, g, opar); break;
+                case OP_H: tile_pairs<OP_H>(lre, lim, tsz / 2, g, opar); break;
+                default: tile_pairs<OP_PAULI>(lre, lim, tsz / 2, g, opar); break;
+                }
             }
-            break;
-        case OP_SWAP: tile_pairs<OP_SWAP>(lre, lim, tsz / 2, g, opar); break;
-        case OP_Y: tile_pairs<OP_Y>(lre, lim, tsz / 2, g, opar); break;
-        case OP_H: tile_pairs<OP_H>(lre, lim, tsz / 2, g, opar); break;
-        default: tile_pairs<OP_PAULI>(lre, lim, tsz / 2, g, opar); break;
+            __syncthreads();
+            continue;
+        }
+        uv8 off;
+#pragma unroll
+        for (uint32_t j = 0; j < kLoc; ++j)
+            off[j] = ((j & 1u) << S.f[0]) | (((j >> 1) & 1u) << S.f[1]) | (((j >> 2) & 1u) << S.f[2]);
+        for (uint32_t gi = threadIdx.x; gi < tsz / kLoc; gi += kBlock) {
+            uint32_t gb = gi;   // insert zeros at the step's bits (ascending)
+#pragma unroll
+            for (uint32_t q = 0; q < kSub; ++q) {
+                const uint32_t lo = (1u << S.f[q]) - 1u;
+                gb = (gb & lo) | ((gb & ~lo) << 1);
+            }
+            RM x;
+#pragma unroll
+            for (uint32_t j = 0; j < kLoc; ++j) {
+                x.r[j] = lre[gb | off[j]];
+                x.m[j] = lim[gb | off[j]];
+            }
+            for (uint32_t k = S.g0; k < S.g1; ++k) {
+                const TileGate& g = a.g[k];
+                if ((base & g.mo) != g.vo) continue;   // uniform: tile outside the gate's selection
+                const bool gpass = (gb & g.mcg) == g.vcg;
+                const uint32_t gpar = ((uint32_t)__popcll(base & g.yzo) + (uint32_t)__popc(gb & g.yzg)) & 1u;
+                switch (g.kind) {
+                case kPhaseOp: x = reg_phase(x, g, gpass, gpar, std::make_integer_sequence<int, (int)kLoc>{}); break;
+                case OP_SWAP: x = reg_pairs<OP_SWAP>(x, g, gpass, gpar); break;
+                case OP_Y: x = reg_pairs<OP_Y>(x, g, gpass, gpar); break;
+                case OP_H: x = reg_pairs<OP_H>(x, g, gpass, gpar); break;
+                default: x = reg_pairs<OP_PAULI>(x, g, gpass, gpar); break;
+                }
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kLoc; ++j) {
+                lre[gb | off[j]] = x.r[j];
+                lim[gb | off[j]] = x.m[j];
+            }
         }
         __syncthreads();
     }
 
     // 3. LDS -> tile
-    for (uint32_t e = 2 * threadIdx.x; e < tsz; e += 2 * kBlock) {
-        const uint64_t g = base | tile_offset(e, a);
-        __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lre + e), reinterpret_cast<dv2t*>(re + g));
-        __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lim + e), reinterpret_cast<dv2t*>(im + g));
+    if (nch == 0) {
+        if (e0 < tsz) {
+            __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lre + e0), reinterpret_cast<dv2t*>(re + toff));
+            __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lim + e0), reinterpret_cast<dv2t*>(im + toff));
+        }
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxChunks; ++k) {
+            if (k < nch) {
+                const uint64_t g = toff | tile_offset(k * 2 * kBlock, a);
+                __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lre + e0 + k * 2 * kBlock),
+                                            reinterpret_cast<dv2t*>(re + g));
+                __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lim + e0 + k * 2 * kBlock),
+                                            reinterpret_cast<dv2t*>(im + g));
+            }
+        }
     }
 }
 
@@ -181,6 +361,29 @@ hipError_t run_single(suqa_hip_ctx* c, const GateDesc& d) {
     ++c->single_sweeps;
     if (d.kind == kPhaseOp) return launch_phase_now(c, sel, d.c, d.s, d.zmask, d.tag, d.alg_bytes);
     return launch_pair_now(c, d.kind, sel, d.X, d.pc, d.tag, d.alg_bytes);
+}
+
+// Local tables of gate g for a register step on tile positions f[0..2].
+void local_tables(TileGate& g, const uint8_t* f) {
+    uint32_t F = 0, mcl = 0, vcl = 0, yzl = 0, xl = 0;
+    for (uint32_t m = 0; m < kSub; ++m) {
+        F |= 1u << f[m];
+        if ((g.mc >> f[m]) & 1u) mcl |= 1u << m;
+        if ((g.vc >> f[m]) & 1u) vcl |= 1u << m;
+        if ((g.yzc >> f[m]) & 1u) yzl |= 1u << m;
+        if ((g.xc >> f[m]) & 1u) xl |= 1u << m;
+    }
+    g.xl = xl;
+    g.lsel = 0;
+    g.lpar = 0;
+    for (uint32_t j = 0; j < kLoc; ++j) {
+        if ((j & mcl) == vcl) g.lsel |= 1u << j;
+        if (__builtin_popcount(j & yzl) & 1) g.lpar |= 1u << j;
+    }
+    g.mcg = g.mc & ~F;
+    g.vcg = g.vc & ~F;
+    g.yzg = g.yzc & ~F;
+    g.gtriv = g.mcg == 0;
 }
 
 hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, uint32_t t) {
@@ -222,6 +425,46 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
         mcom &= ~((g.vo ^ vcom) & mcom);
     }
     vcom &= mcom;
+    // register steps: runs of gates whose flipped tile bits fit in kSub bits
+    a.nsteps = 0;
+    uint32_t F = 0, g0 = 0;
+    auto close = [&](uint32_t g1, bool lds) {
+        auto& S = a.step[a.nsteps++];
+        S.g0 = (uint8_t)g0;
+        S.g1 = (uint8_t)g1;
+        if (lds) {
+            S.f[0] = S.f[1] = S.f[2] = 0xffu;
+            return;
+        }
+        // pad with tile bits that the step's gates select on, then high tile bits
+        uint32_t want = 0;
+        for (uint32_t k2 = g0; k2 < g1; ++k2) want |= a.g[k2].mc;
+        for (uint32_t p = 0; p < t && __builtin_popcount(F) < (int)kSub; ++p)
+            if (((want & ~F) >> p) & 1u) F |= 1u << p;
+        for (int p = (int)t - 1; p >= 0 && __builtin_popcount(F) < (int)kSub; --p) F |= 1u << p;
+        uint32_t qn = 0;
+        for (uint32_t p = 0; p < t; ++p)
+            if ((F >> p) & 1u) S.f[qn++] = (uint8_t)p;
+        for (uint32_t k2 = g0; k2 < g1; ++k2) local_tables(a.g[k2], S.f);
+    };
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t xc = a.g[i].xc;
+        if ((uint32_t)__builtin_popcount(xc) > kSub) {   // a step of its own, in LDS
+            if (i > g0) close(i, false);
+            g0 = i;
+            close(i + 1, true);
+            g0 = i + 1;
+            F = 0;
+            continue;
+        }
+        if ((uint32_t)__builtin_popcount(F | xc) > kSub) {
+            close(i, false);
+            g0 = i;
+            F = 0;
+        }
+        F |= xc;
+    }
+    if (n > g0) close(n, false);
     const uint64_t fix = T | mcom;
     a.nfix = 0;
     for (uint32_t b = 0; b < nq; ++b)
@@ -258,7 +501,7 @@ hipError_t flush(suqa_hip_ctx* c) {
             ++j;
         }
         hipError_t e;
-        if (j - i <= 1) {   // a lone gate (or one whose partner bits exceed the tile): one sweep
+        if (j - i <= 1 || t < kSub) {   // a lone gate, one whose partner bits exceed the tile, or a tiny state
             e = run_single(c, q[i]);
             j = i + 1;
         } else {

@@ -1,5 +1,5 @@
-"""Parity of the HIP engine (through the C-ABI) with the reference's golden
-vectors and the oracle.  Gates are bit-exact; anything downstream of a
+"""Parity of the HIP engine's one-sweep-per-gate kernels (through the C-ABI,
+tile passes off) with the reference's golden vectors and the oracle.  Gates are bit-exact; anything downstream of a
 reduction (measure, reset, vnormalize) is within 1e-12 with identical
 outcomes; prob_filter/vnorm values within 1e-14 relative."""
 import math
@@ -31,7 +31,7 @@ def test_gate_cases_vs_golden(capi):
     g = load("gates_n5.json")
     n = g["n"]
     log(f"gate cases n={n}")
-    eng = capi.Engine(n)
+    eng = capi.Engine(n, tile_bits=0)
     eng.qft(list(range(5)))
     prep(eng, n)
     re, im = golden_state(g["prep"])
@@ -63,7 +63,7 @@ def test_circuit_vs_golden(capi, fx):
     n = d["n"]
     log(f"circuit golden {fx}")
     gates, ng, mq, mr = po.circuit(n, d["layers"], d["seed"])
-    eng = capi.Engine(n)
+    eng = capi.Engine(n, tile_bits=0)
     for q in range(n):
         eng.h(q)
     eng.run_gates(capi.gates_from(gates[:ng]))
@@ -85,7 +85,7 @@ def test_circuit_n18_vs_oracle(capi):
     log(f"circuit n={n} layers={layers} vs oracle")
     gates, ng, mq, mr = po.circuit(n, layers, seed)
     st = po.State(n)
-    eng = capi.Engine(n)
+    eng = capi.Engine(n, tile_bits=0)
     for q in range(n):
         st.h(q)
         eng.h(q)
@@ -108,7 +108,7 @@ def test_pauli_general_vs_oracle(capi):
     """Any-weight exp(i phi P) in one pass vs the oracle's CNOT-ladder circuit."""
     n = 9
     rng = np.random.default_rng(5)
-    eng = capi.Engine(n)
+    eng = capi.Engine(n, tile_bits=0)
     st = po.State(n)
     v = rng.normal(size=(2, 1 << n))
     v /= np.linalg.norm(v)
@@ -137,7 +137,7 @@ def test_pauli_general_vs_oracle(capi):
 def test_reference_pauli_patterns_bitwise_vs_oracle(capi):
     n = 7
     rng = np.random.default_rng(11)
-    eng = capi.Engine(n)
+    eng = capi.Engine(n, tile_bits=0)
     st = po.State(n)
     v = rng.normal(size=(2, 1 << n))
     v /= np.linalg.norm(v)
@@ -164,7 +164,7 @@ def test_reference_pauli_patterns_bitwise_vs_oracle(capi):
 def test_phase_list_and_swap_vs_oracle(capi):
     n = 8
     rng = np.random.default_rng(3)
-    eng = capi.Engine(n)
+    eng = capi.Engine(n, tile_bits=0)
     st = po.State(n)
     v = rng.normal(size=(2, 1 << n))
     v /= np.linalg.norm(v)
@@ -201,7 +201,7 @@ def test_qms_W_bitwise_vs_oracle(capi):
     n = f.nqubits
     rng = np.random.default_rng(1)
     v = rng.normal(size=(2, 1 << n))
-    eng = capi.Engine(n)
+    eng = capi.Engine(n, tile_bits=0)
     st = f.st.contents
     N = 1 << n
     re = np.ctypeslib.as_array(st.re, shape=(N,))
@@ -236,7 +236,7 @@ def test_full_size_30_qubits_reversible(capi):
     n = 30
     log("full size n=30")
     gates, ng, mq, mr = po.circuit(n, 1, 7)
-    eng = capi.Engine(n)
+    eng = capi.Engine(n, tile_bits=0)
     for q in range(n):
         eng.h(q)
     eng.run_gates(capi.gates_from(gates[:ng]))
@@ -266,7 +266,7 @@ def test_low_and_high_bit_combinations_bitwise(capi):
     strided mode), bit-exact against the oracle."""
     n = 12
     rng = np.random.default_rng(21)
-    eng = capi.Engine(n)
+    eng = capi.Engine(n, tile_bits=0)
     st = po.State(n)
     v = rng.normal(size=(2, 1 << n))
     v /= np.linalg.norm(v)

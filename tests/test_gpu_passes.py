@@ -58,7 +58,10 @@ def test_gate_cases_in_passes_vs_golden(capi, bits):
             assert_bitwise(gre, re, tag + " re")
             assert_bitwise(gim, im, tag + " im")
     st = eng.pass_stats()
-    assert st["passes"] > 0
+    if bits >= 3:
+        assert st["passes"] > 0
+    else:   # tiles under 3 bits cannot hold a register step: every gate is its own sweep
+        assert st["passes"] == 0 and st["single_sweeps"] > 0
     eng.close()
 
 
