@@ -137,10 +137,34 @@ struct RM {
 // the lowest bit of XL, expanded at compile time.  The roles come from the
 // uniform table lsel; gpass (this group meets the selection's other bits)
 // and gpar (parity of the other bits) are per thread.
-template <int XL, int OP, int J>
+template <int XL, int OP, bool TRIV, int J>
 __device__ __forceinline__ RM reg_pair_one(RM x, const TileGate& g, const PauliCoef& pc, bool gpass, uint32_t gpar) {
     constexpr int PIV = XL & -XL;
-    if constexpr ((J & PIV) == 0) {
+    if constexpr ((J & PIV) == 0 && XL == PIV) {
+        // one flipped bit: every gate of this kind selects its pivot bit at 0
+        // (the host checks it), so the lower element u is the reference's i
+        constexpr int u = J, w = J ^ XL;
+        if ((g.lsel >> u) & 1u) {
+            const uint32_t par_u = OP == OP_PAULI ? (((g.lpar >> w) & 1u) ^ gpar) : 0u;
+            const uint32_t par_w = OP == OP_PAULI ? (((g.lpar >> u) & 1u) ^ gpar) : 0u;
+            const double ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
+            const double nru = pair_out_par<OP>(true, false, ru, rw, mw, par_u, pc);
+            const double nmu = pair_out_par<OP>(true, true, mu, mw, rw, par_u, pc);
+            const double nrw = pair_out_par<OP>(false, false, rw, ru, mu, par_w, pc);
+            const double nmw = pair_out_par<OP>(false, true, mw, mu, ru, par_w, pc);
+            if constexpr (TRIV) {
+                x.r[u] = nru;
+                x.m[u] = nmu;
+                x.r[w] = nrw;
+                x.m[w] = nmw;
+            } else {
+                x.r[u] = gpass ? nru : ru;
+                x.m[u] = gpass ? nmu : mu;
+                x.r[w] = gpass ? nrw : rw;
+                x.m[w] = gpass ? nmw : mw;
+            }
+        }
+    } else if constexpr ((J & PIV) == 0) {
         constexpr int u = J, w = J ^ XL;
         const bool su = (g.lsel >> u) & 1u;
         const bool sw = !su && ((g.lsel >> w) & 1u);
@@ -170,7 +194,7 @@ __device__ __forceinline__ RM reg_pair_one(RM x, const TileGate& g, const PauliC
     return x;
 }
 
-template <int XL, int OP, int... J>
+template <int XL, int OP, bool TRIV, int... J>
 __device__ __forceinline__ RM reg_pairs_seq(RM x, const TileGate& g, bool gpass, uint32_t gpar,
                                             std::integer_sequence<int, J...>) {
     PauliCoef pc;
@@ -180,21 +204,30 @@ __device__ __forceinline__ RM reg_pairs_seq(RM x, const TileGate& g, bool gpass,
     pc.ny_odd = 0;
     pc.base_imag = g.base_imag;
     pc.base_neg = g.base_neg;
-    ((x = reg_pair_one<XL, OP, J>(x, g, pc, gpass, gpar)), ...);
+    ((x = reg_pair_one<XL, OP, TRIV, J>(x, g, pc, gpass, gpar)), ...);
     return x;
 }
 
 template <int OP>
 __device__ __forceinline__ RM reg_pairs(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
     constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
+    // one-bit partner masks, with and without a selection on the other tile bits
+    if (g.gtriv) {
+        switch (g.xl) {
+        case 1: return reg_pairs_seq<1, OP, true>(x, g, gpass, gpar, seq);
+        case 2: return reg_pairs_seq<2, OP, true>(x, g, gpass, gpar, seq);
+        case 4: return reg_pairs_seq<4, OP, true>(x, g, gpass, gpar, seq);
+        default: break;
+        }
+    }
     switch (g.xl) {
-    case 1: return reg_pairs_seq<1, OP>(x, g, gpass, gpar, seq);
-    case 2: return reg_pairs_seq<2, OP>(x, g, gpass, gpar, seq);
-    case 3: return reg_pairs_seq<3, OP>(x, g, gpass, gpar, seq);
-    case 4: return reg_pairs_seq<4, OP>(x, g, gpass, gpar, seq);
-    case 5: return reg_pairs_seq<5, OP>(x, g, gpass, gpar, seq);
-    case 6: return reg_pairs_seq<6, OP>(x, g, gpass, gpar, seq);
-    default: return reg_pairs_seq<7, OP>(x, g, gpass, gpar, seq);
+    case 1: return reg_pairs_seq<1, OP, false>(x, g, gpass, gpar, seq);
+    case 2: return reg_pairs_seq<2, OP, false>(x, g, gpass, gpar, seq);
+    case 3: return reg_pairs_seq<3, OP, false>(x, g, gpass, gpar, seq);
+    case 4: return reg_pairs_seq<4, OP, false>(x, g, gpass, gpar, seq);
+    case 5: return reg_pairs_seq<5, OP, false>(x, g, gpass, gpar, seq);
+    case 6: return reg_pairs_seq<6, OP, false>(x, g, gpass, gpar, seq);
+    default: return reg_pairs_seq<7, OP, false>(x, g, gpass, gpar, seq);
     }
 }
 
@@ -449,7 +482,9 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     };
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t xc = a.g[i].xc;
-        if ((uint32_t)__builtin_popcount(xc) > kSub) {   // a step of its own, in LDS
+        // register steps assume that a one-bit partner mask is selected at 0
+        const bool pivot_zero = __builtin_popcount(xc) != 1 || (((a.g[i].mc & xc) != 0) && ((a.g[i].vc & xc) == 0));
+        if ((uint32_t)__builtin_popcount(xc) > kSub || !pivot_zero) {   // a step of its own, in LDS
             if (i > g0) close(i, false);
             g0 = i;
             close(i + 1, true);
