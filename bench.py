@@ -35,6 +35,14 @@ The QMS half of the metric is reported under "extra":
                 against the oracle;
   cpu_baseline_qms -- the oracle on the same single chain, 1 core.
 
+The Trotter + phase-estimation workload under "extra.trotter_qpe": 24 system
+qubits + a 6-qubit register (30 qubits), 96 Pauli strings of weight 2-4
+(pcg32 seed 11), dt 0.05, 20 Trotter steps per power.  Timed sample: H on
+the top register qubit, then its 20 controlled Trotter steps (1920
+controlled rotations), in FP64 (gate passes) and FP32 fused form, with the
+fidelity between them; the full QPE applies 63 such step groups
+(120,960 rotations), projected from the sample.
+
 and the precision pattern under "extra.fused": the same circuit on an FP32
 state with gates fused into K-qubit blocks (K autotuned over 4..6 on the
 first 10 layers) applied by FP32 MFMA, timed over all layers from |0> + H,
@@ -53,9 +61,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes moved per launch relative to the algorithmic bytes, per kernel family, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) on this workload, scaled by the counts a
-# 4 GiB copy kernel gives (reads x2.000, writes x1.000): profiles/gate_sweep_n30_fp64.md.
-PMC_TRAFFIC_RATIO = {"h_pair": 1.000, "pauli_diag": 1.000, "pauli_pair": 1.000, "phase": 1.154,
-                     "swap_pair": 1.023}
+# 4 GiB copy kernel gives (reads x2.000, writes x1.000): profiles/tile_pass_n30_fp64.md
+# (tile passes) and profiles/gate_sweep_n30_fp64.md (one sweep per gate).
+PMC_TRAFFIC_RATIO = {"tile_pass": 1.000, "h_pair": 1.000, "pauli_diag": 1.000, "pauli_pair": 1.000,
+                     "phase": 1.154, "swap_pair": 1.023}
 
 
 def log(msg):
@@ -177,6 +186,56 @@ def fused_leg(capi, wl, eng, gates, n, layers):
         raise
 
 
+def trotter_leg(capi, wl, no_fused):
+    """Controlled Trotter steps of the 24 + 6 qubit QPE: FP64 passes and FP32 fused, one sample."""
+    nsys, nreg, dt, steps = 24, 6, 0.05, 20
+    n = nsys + nreg
+    terms = wl.trotter_terms(nsys, 96, 11)
+    top = n - 1
+    ops = capi.pauli_ops(wl.trotter_step_ops(terms, dt, 1 << top))
+    e = capi.Engine(n)
+    try:
+        e.init_state()
+        e.h(top)
+        e.run_paulis(ops)          # warm-up step
+        e.synchronize()
+        ps0 = e.pass_stats()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            e.run_paulis(ops)
+        e.synchronize()
+        dt64 = time.perf_counter() - t0
+        ps1 = e.pass_stats()
+        rot = 96 * (steps - 1)
+        res = {"qubits": n, "terms": 96, "dt": dt, "steps_per_power": steps, "sample_rotations": rot,
+               "fp64_seconds": round(dt64, 4), "fp64_rotations_per_s": round(rot / dt64, 1),
+               "fp64_alg_gbs": round(rot * 16 * (1 << n) / dt64 / 1e9, 1),
+               "fp64_gates_per_pass": round((ps1["pass_gates"] - ps0["pass_gates"]) /
+                                            max(1, ps1["passes"] - ps0["passes"]), 2),
+               "fp64_full_qpe_projected_s": round(dt64 / rot * 96 * steps * 63, 1)}
+        log(f"trotter fp64: {rot / dt64:.0f} rotations/s")
+        if not no_fused:
+            f = capi.F32Engine(n)
+            try:
+                f.init_state()
+                f.run_gates(capi.gates_from([wl.G(wl.H, top, [], 0.0)]), 6)
+                f.run_paulis(ops, 6)       # warm-up step, as in FP64
+                f.synchronize()
+                t0 = time.perf_counter()
+                nb = f.run_paulis(capi.pauli_ops(wl.trotter_step_ops(terms, dt, 1 << top) * (steps - 1)), 6)
+                f.synchronize()
+                dt32 = time.perf_counter() - t0
+                res.update({"fp32_fused_seconds": round(dt32, 4), "fp32_blocks": nb,
+                            "fp32_rotations_per_s": round(rot / dt32, 1),
+                            "fp32_fidelity_vs_fp64": f.fidelity(e)})
+                log(f"trotter fp32: {rot / dt32:.0f} rotations/s, fidelity {res['fp32_fidelity_vs_fp64']!r}")
+            finally:
+                f.close()
+        return res
+    finally:
+        e.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +245,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-qms", action="store_true", help="skip the QMS legs (profiling runs)")
     ap.add_argument("--no-fused", action="store_true", help="skip the FP32 fused leg (profiling runs)")
+    ap.add_argument("--no-trotter", action="store_true", help="skip the Trotter + QPE leg (profiling runs)")
     ap.add_argument("--tile-bits", type=int, default=-1,
                     help="FP64 tile size (0: one sweep per gate); default: autotuned")
     a = ap.parse_args()
@@ -256,6 +316,7 @@ def main():
         f32.close()
     cpu = None if a.no_cpu else cpu_baseline(min(n, 27))
     qres = {} if a.no_qms else qms_legs(a.no_cpu)
+    tres = None if a.no_trotter else trotter_leg(capi, wl, a.no_fused)
     fams = {k: dict(launches=v["launches"], ms=round(v["ms"], 3),
                     gbs=round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)) for k, v in rep.items()}
     out = {
@@ -282,7 +343,8 @@ def main():
         "cpu_baseline": cpu,
         "extra": dict({"copy_kernel_gbs": round(peak_copy, 1), "kernel_families": fams,
                        "tile_autotune": tune, "passes": pass_info,
-                       "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm, "fused": fres}, **qres),
+                       "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm, "fused": fres,
+                       "trotter_qpe": tres}, **qres),
     }
     print(json.dumps(out), flush=True)
 
