@@ -60,6 +60,7 @@ struct TileGate {
     // ... and the part on the other tile bits (tile positions, zero on F)
     uint32_t mcg, vcg, yzg;
     uint32_t gtriv;       // mcg == 0: every group meets the selection
+    uint32_t dense;       // gtriv, and every element (pair) of a group is selected
     uint32_t base_imag, base_neg;
     uint64_t mo, vo;      // selection bits outside the tile (global positions)
     uint64_t yzo;         // parity bits outside the tile
@@ -133,68 +134,63 @@ struct RM {
     dv8 r, m;
 };
 
+// Register steps compute every element and pick results with selects, never
+// with branches on data: a branch around an update makes the compiler copy
+// the whole 16-double register set at the join.
+
 // Pair gate with local partner mask XL: pair (J, J ^ XL) for each J without
-// the lowest bit of XL, expanded at compile time.  The roles come from the
-// uniform table lsel; gpass (this group meets the selection's other bits)
+// the lowest bit of XL, expanded at compile time.  DENSE: every pair of the
+// group is selected (no test at all).  Otherwise the roles come from the
+// uniform table lsel, gpass (this group meets the selection's other bits)
 // and gpar (parity of the other bits) are per thread.
-template <int XL, int OP, bool TRIV, int J>
+template <int XL, int OP, bool DENSE, int J>
 __device__ __forceinline__ RM reg_pair_one(RM x, const TileGate& g, const PauliCoef& pc, bool gpass, uint32_t gpar) {
     constexpr int PIV = XL & -XL;
-    if constexpr ((J & PIV) == 0 && XL == PIV) {
-        // one flipped bit: every gate of this kind selects its pivot bit at 0
-        // (the host checks it), so the lower element u is the reference's i
+    if constexpr ((J & PIV) == 0) {
         constexpr int u = J, w = J ^ XL;
-        if ((g.lsel >> u) & 1u) {
+        const double ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
+        if constexpr (XL == PIV) {
+            // one flipped bit: gates of this kind select their pivot bit at 0 (the
+            // host checks it), so the lower element u is the reference's i
             const uint32_t par_u = OP == OP_PAULI ? (((g.lpar >> w) & 1u) ^ gpar) : 0u;
             const uint32_t par_w = OP == OP_PAULI ? (((g.lpar >> u) & 1u) ^ gpar) : 0u;
-            const double ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
             const double nru = pair_out_par<OP>(true, false, ru, rw, mw, par_u, pc);
             const double nmu = pair_out_par<OP>(true, true, mu, mw, rw, par_u, pc);
             const double nrw = pair_out_par<OP>(false, false, rw, ru, mu, par_w, pc);
             const double nmw = pair_out_par<OP>(false, true, mw, mu, ru, par_w, pc);
-            if constexpr (TRIV) {
+            if constexpr (DENSE) {
                 x.r[u] = nru;
                 x.m[u] = nmu;
                 x.r[w] = nrw;
                 x.m[w] = nmw;
             } else {
-                x.r[u] = gpass ? nru : ru;
-                x.m[u] = gpass ? nmu : mu;
-                x.r[w] = gpass ? nrw : rw;
-                x.m[w] = gpass ? nmw : mw;
+                const bool on = ((g.lsel >> u) & 1u) && (g.gtriv || gpass);
+                x.r[u] = on ? nru : ru;
+                x.m[u] = on ? nmu : mu;
+                x.r[w] = on ? nrw : rw;
+                x.m[w] = on ? nmw : mw;
             }
-        }
-    } else if constexpr ((J & PIV) == 0) {
-        constexpr int u = J, w = J ^ XL;
-        const bool su = (g.lsel >> u) & 1u;
-        const bool sw = !su && ((g.lsel >> w) & 1u);
-        if (su || sw) {
-            // element j's partner is j ^ X: u's is w and w's is u; the selected
-            // one plays the reference's index i (a_side)
+        } else {
+            // several flipped bits (SWAP, Pauli strings): either element may be i
+            const bool su = (g.lsel >> u) & 1u;
+            const bool sw = !su && ((g.lsel >> w) & 1u);
+            const bool on = (su || sw) && (g.gtriv || gpass);
             const uint32_t par_u = OP == OP_PAULI ? (((g.lpar >> w) & 1u) ^ gpar) : 0u;
             const uint32_t par_w = OP == OP_PAULI ? (((g.lpar >> u) & 1u) ^ gpar) : 0u;
-            const double ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
             const double nru = pair_out_par<OP>(su, false, ru, rw, mw, par_u, pc);
             const double nmu = pair_out_par<OP>(su, true, mu, mw, rw, par_u, pc);
             const double nrw = pair_out_par<OP>(sw, false, rw, ru, mu, par_w, pc);
             const double nmw = pair_out_par<OP>(sw, true, mw, mu, ru, par_w, pc);
-            if (g.gtriv) {
-                x.r[u] = nru;
-                x.m[u] = nmu;
-                x.r[w] = nrw;
-                x.m[w] = nmw;
-            } else {
-                x.r[u] = gpass ? nru : ru;
-                x.m[u] = gpass ? nmu : mu;
-                x.r[w] = gpass ? nrw : rw;
-                x.m[w] = gpass ? nmw : mw;
-            }
+            x.r[u] = on ? nru : ru;
+            x.m[u] = on ? nmu : mu;
+            x.r[w] = on ? nrw : rw;
+            x.m[w] = on ? nmw : mw;
         }
     }
     return x;
 }
 
-template <int XL, int OP, bool TRIV, int... J>
+template <int XL, int OP, bool DENSE, int... J>
 __device__ __forceinline__ RM reg_pairs_seq(RM x, const TileGate& g, bool gpass, uint32_t gpar,
                                             std::integer_sequence<int, J...>) {
     PauliCoef pc;
@@ -204,15 +200,17 @@ __device__ __forceinline__ RM reg_pairs_seq(RM x, const TileGate& g, bool gpass,
     pc.ny_odd = 0;
     pc.base_imag = g.base_imag;
     pc.base_neg = g.base_neg;
-    ((x = reg_pair_one<XL, OP, TRIV, J>(x, g, pc, gpass, gpar)), ...);
+    ((x = reg_pair_one<XL, OP, DENSE, J>(x, g, pc, gpass, gpar)), ...);
     return x;
 }
 
 template <int OP>
 __device__ __forceinline__ RM reg_pairs(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
     constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
-    // one-bit partner masks, with and without a selection on the other tile bits
-    if (g.gtriv) {
+    if (g.dense) {   // set by the host for one-bit partner masks only
+        // Each mask value has its own case and anything else falls through to the
+        // general path.  With `default:` standing for 4, hipcc (ROCm 7.2) merged the
+        // three OP_Y cases and left half of the xl = 4 outputs unassigned.
         switch (g.xl) {
         case 1: return reg_pairs_seq<1, OP, true>(x, g, gpass, gpar, seq);
         case 2: return reg_pairs_seq<2, OP, true>(x, g, gpass, gpar, seq);
@@ -227,35 +225,41 @@ __device__ __forceinline__ RM reg_pairs(RM x, const TileGate& g, bool gpass, uin
     case 4: return reg_pairs_seq<4, OP, false>(x, g, gpass, gpar, seq);
     case 5: return reg_pairs_seq<5, OP, false>(x, g, gpass, gpar, seq);
     case 6: return reg_pairs_seq<6, OP, false>(x, g, gpass, gpar, seq);
-    default: return reg_pairs_seq<7, OP, false>(x, g, gpass, gpar, seq);
+    case 7: return reg_pairs_seq<7, OP, false>(x, g, gpass, gpar, seq);
+    default: return x;   // no flipped bit: not a pair gate (never queued as one)
     }
 }
 
 // Phase rotation of the selected register amplitudes.
-template <int J>
+template <bool DENSE, int J>
 __device__ __forceinline__ RM reg_phase_one(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
-    if ((g.lsel >> J) & 1u) {
-        const uint32_t par = ((g.lpar >> J) & 1u) ^ gpar;
-        const double sn = par ? -g.s : g.s;
-        const double rr = x.r[J], mm = x.m[J];
-        const double nr = phase_out(false, rr, mm, g.c, sn);
-        const double nm = phase_out(true, mm, rr, g.c, sn);
-        if (g.gtriv) {
-            x.r[J] = nr;
-            x.m[J] = nm;
-        } else {
-            x.r[J] = gpass ? nr : rr;
-            x.m[J] = gpass ? nm : mm;
-        }
+    const uint32_t par = ((g.lpar >> J) & 1u) ^ gpar;
+    const double sn = par ? -g.s : g.s;
+    const double rr = x.r[J], mm = x.m[J];
+    const double nr = phase_out(false, rr, mm, g.c, sn);
+    const double nm = phase_out(true, mm, rr, g.c, sn);
+    if constexpr (DENSE) {
+        x.r[J] = nr;
+        x.m[J] = nm;
+    } else {
+        const bool on = ((g.lsel >> J) & 1u) && (g.gtriv || gpass);
+        x.r[J] = on ? nr : rr;
+        x.m[J] = on ? nm : mm;
     }
     return x;
 }
 
-template <int... J>
-__device__ __forceinline__ RM reg_phase(RM x, const TileGate& g, bool gpass, uint32_t gpar,
-                                        std::integer_sequence<int, J...>) {
-    ((x = reg_phase_one<J>(x, g, gpass, gpar)), ...);
+template <bool DENSE, int... J>
+__device__ __forceinline__ RM reg_phase_seq(RM x, const TileGate& g, bool gpass, uint32_t gpar,
+                                            std::integer_sequence<int, J...>) {
+    ((x = reg_phase_one<DENSE, J>(x, g, gpass, gpar)), ...);
     return x;
+}
+
+__device__ __forceinline__ RM reg_phase(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
+    constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
+    if (g.dense) return reg_phase_seq<true>(x, g, gpass, gpar, seq);
+    return reg_phase_seq<false>(x, g, gpass, gpar, seq);
 }
 
 __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double* __restrict__ im, const TileArgs a) {
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
                 const bool gpass = (gb & g.mcg) == g.vcg;
                 const uint32_t gpar = ((uint32_t)__popcll(base & g.yzo) + (uint32_t)__popc(gb & g.yzg)) & 1u;
                 switch (g.kind) {
-                case kPhaseOp: x = reg_phase(x, g, gpass, gpar, std::make_integer_sequence<int, (int)kLoc>{}); break;
+                case kPhaseOp: x = reg_phase(x, g, gpass, gpar); break;
                 case OP_SWAP: x = reg_pairs<OP_SWAP>(x, g, gpass, gpar); break;
                 case OP_Y: x = reg_pairs<OP_Y>(x, g, gpass, gpar); break;
                 case OP_H: x = reg_pairs<OP_H>(x, g, gpass, gpar); break;
@@ -417,6 +421,16 @@ void local_tables(TileGate& g, const uint8_t* f) {
     g.vcg = g.vc & ~F;
     g.yzg = g.yzc & ~F;
     g.gtriv = g.mcg == 0;
+    // dense: no selection left to test.  Phase: all 8 elements selected; one-bit
+    // pair: all 4 lower elements (pivot bit 0) selected
+    uint32_t need = 0xffu;
+    if (g.kind != (uint32_t)kPhaseOp) {
+        need = 0;
+        if (__builtin_popcount(xl) == 1)
+            for (uint32_t j = 0; j < kLoc; ++j)
+                if ((j & xl) == 0) need |= 1u << j;
+    }
+    g.dense = g.gtriv && need != 0 && (g.lsel & need) == need;
 }
 
 hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, uint32_t t) {

@@ -280,3 +280,53 @@ def test_full_size_passes_match_one_sweep_per_gate(capi):
             log(f"pass stats {e.pass_stats()}")
         e.close()
     assert vals[0] == vals[1]
+
+
+@pytest.mark.parametrize("n,bits", [(5, 5), (6, 5), (8, 6), (12, 11)])
+def test_every_gate_at_every_qubit_in_a_register_step(capi, n, bits):
+    """Each gate kind on each qubit, followed by a gate that selects on another
+    qubit (so both share a register step and the first one lands on every
+    local position), bit-identical to one sweep per gate.  Covers the
+    compile-time register layouts of every partner mask (xl = 1, 2, 4)."""
+    log(f"gate x qubit sweep n={n} tile bits {bits}")
+    rng = np.random.default_rng(n)
+    v = rng.normal(size=(2, 1 << n))
+    v /= np.linalg.norm(v)
+    ops = ("x", "y", "h", "pauliX", "pauliY", "pauliZ", "u1", "cx", "swap", "YY", "XZ")
+    bad = []
+    for op in ops:
+        for q in range(n):
+            q2 = (q + 1) % n
+            out = []
+            for b in (0, bits):
+                e = engine(capi, n, b)
+                e.set(v[0], v[1])
+                if op == "x":
+                    e.x(q)
+                elif op == "y":
+                    e.y(q)
+                elif op == "h":
+                    e.h(q)
+                elif op == "pauliX":
+                    e.pauli([q], [1], 0.37)
+                elif op == "pauliY":
+                    e.pauli([q], [2], 0.37)
+                elif op == "pauliZ":
+                    e.pauli([q], [3], 0.37)
+                elif op == "u1":
+                    e.u1(q, 0.37)
+                elif op == "cx":
+                    e.cx(q2, q)
+                elif op == "swap":
+                    e.swap(q, q2)
+                elif op == "YY":
+                    e.pauli([q, q2], [2, 2], 0.37)
+                else:
+                    e.pauli([q, q2], [1, 3], 0.37)
+                e.u1(q2, 0.11)
+                e.cx(q, q2)
+                out.append(e.get())
+                e.close()
+            if not (np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])):
+                bad.append((op, q))
+    assert not bad, f"passes differ from one sweep per gate for {bad}"
