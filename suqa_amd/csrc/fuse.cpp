@@ -101,7 +101,8 @@ Op gate_op(const suqa_hip_gate& g) {
 }
 
 // exp(i phi P) = cos(phi) I + i sin(phi) P with P|x> = prod_k f_k |x ^ flips>,
-// f = 1 (X), i (-1)^b (Y), (-1)^b (Z); identity where a control bit is 0.
+// f = 1 (X), i (-1)^b (Y), (-1)^b (Z), on the string's non-identity qubits;
+// the gc_mask qubits become the op's controls.
 bool pauli_op(const suqa_hip_pauli_op& p, Op& out) {
     if (p.w > 8) return false;
     std::vector<uint32_t> qs;
@@ -111,21 +112,10 @@ bool pauli_op(const suqa_hip_pauli_op& p, Op& out) {
         seen |= 1ull << p.qs[k];
         if (p.types[k] != 0u) qs.push_back(p.qs[k]);
     }
-    for (uint32_t q = 0; q < 32; ++q)
-        if ((p.gc_mask >> q) & 1u) {
-            if ((seen >> q) & 1ull) return false;
-            qs.push_back(q);
-        }
+    if (seen & (uint64_t)p.gc_mask) return false;
     if (qs.size() > (size_t)kMaxK) return false;
     const double c = std::cos(p.phase), s = std::sin(p.phase);
     out = make_op(qs, [&](const Op& o, uint32_t in, std::vector<cd>& col) {
-        uint32_t cm = 0;
-        for (uint32_t q = 0; q < 32; ++q)
-            if ((p.gc_mask >> q) & 1u) cm |= bit_of(o, q);
-        if ((in & cm) != cm) {
-            col[in] = 1.0;
-            return;
-        }
         uint32_t flip = 0;
         cd f = 1.0;
         for (uint32_t k = 0; k < p.w; ++k) {
@@ -141,6 +131,7 @@ bool pauli_op(const suqa_hip_pauli_op& p, Op& out) {
         col[in] += c;
         col[in ^ flip] += cd(0, s) * f;
     });
+    out.cmask = p.gc_mask;
     return true;
 }
 
@@ -180,29 +171,36 @@ std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width,
     while (first < n) {
         while (first < n && used[first]) ++first;
         if (first >= n) break;
-        uint64_t S = 0, blocked = 0;
+        const uint64_t cm = ops[first].cmask;
+        uint64_t S = 0, lt = 0, lc = 0;   // block targets; targets / controls of ops left out
         std::vector<uint32_t> members;
         uint32_t scanned = 0;
         for (uint32_t i = first; i < n && scanned < window; ++i) {
             if (used[i]) continue;
             ++scanned;
-            const uint64_t G = gmask[i];
-            if ((G & blocked) == 0 && (uint32_t)__builtin_popcountll(S | G) <= width) {
+            const uint64_t G = gmask[i], C = ops[i].cmask;
+            const bool commutes = (G & (lt | lc)) == 0 && (C & lt) == 0;
+            if (commutes && C == cm && (uint32_t)__builtin_popcountll(S | G) <= width) {
                 S |= G;
                 members.push_back(i);
                 used[i] = 1;
             } else {
-                blocked |= G;
+                lt |= G;
+                lc |= C;
             }
-            // nothing further can join once every qubit is blocked, or the block is
-            // full and all of its qubits are blocked
+            // nothing further can join once every qubit is blocked, the block's
+            // control is some left-out op's target, or the block is full and all
+            // of its qubits are blocked
             const bool full = (uint32_t)__builtin_popcountll(S) >= width;
-            if ((uint32_t)__builtin_popcountll(blocked) >= nq || (full && (blocked & S) == S)) break;
+            if ((uint32_t)__builtin_popcountll(lt | lc) >= nq || (lt & cm) != 0 || (full && ((lt | lc) & S) == S))
+                break;
         }
-        // pad the block to exactly `width` qubits with the lowest free qubits
-        for (uint32_t q = 0; q < nq && (uint32_t)__builtin_popcountll(S) < width; ++q) S |= 1ull << q;
+        // pad the block to exactly `width` qubits with the lowest free non-control qubits
+        for (uint32_t q = 0; q < nq && (uint32_t)__builtin_popcountll(S) < width; ++q)
+            if (!((cm >> q) & 1ull)) S |= 1ull << q;
         Block b;
         b.K = 0;
+        b.cmask = cm;
         for (uint32_t q = 0; q < nq; ++q)
             if ((S >> q) & 1ull) b.qubits[b.K++] = q;
         const uint32_t D = 1u << b.K;

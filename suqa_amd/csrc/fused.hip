@@ -85,16 +85,22 @@ struct BlockArgs {
     uint32_t L;                        // non-block bits per tile (columns = 2^L)
     uint32_t rowoff[1 << kMaxK];       // compact offset of block index r
     uint32_t coloff[1 << kTileLow];    // compact offset of column c
+    uint32_t nfix;                     // tile bits and control bits, ascending
+    uint8_t fixpos[64];
+    uint64_t fixval;                   // the controls (all 1)
     uint64_t ntiles;
     const float* A;                    // [RB][D/2][64] MFMA-ordered stacked matrix
 };
 
+// Tile t -> its base index: zeros inserted at the tile and control bits
+// (ascending), controls set.  A controlled block only visits tiles where every
+// control is 1: the others are left untouched.
 __device__ __forceinline__ uint64_t tile_base(uint64_t t, const BlockArgs& a) {
-    for (uint32_t j = 0; j < a.ntile; ++j) {   // insert zeros at the tile bits (ascending)
-        const uint64_t lo = (1ull << a.tpos[j]) - 1ull;
+    for (uint32_t j = 0; j < a.nfix; ++j) {
+        const uint64_t lo = (1ull << a.fixpos[j]) - 1ull;
         t = (t & lo) | ((t & ~lo) << 1);
     }
-    return t;
+    return t | a.fixval;
 }
 __device__ __forceinline__ uint64_t pdep_tile(uint32_t e, const BlockArgs& a) {
     uint64_t o = 0;
@@ -334,7 +340,7 @@ void block_args(const Block& b, uint32_t nq, BlockArgs& a) {
     for (uint32_t j = 0; j < b.K; ++j) bm |= 1ull << b.qubits[j];
     std::vector<uint32_t> low;
     for (uint32_t q = 0; q < nq && low.size() < (size_t)kTileLow; ++q)
-        if (!((bm >> q) & 1ull)) low.push_back(q);
+        if (!(((bm | b.cmask) >> q) & 1ull)) low.push_back(q);
     a.L = (uint32_t)low.size();
     uint64_t tm = bm;
     for (uint32_t q : low) tm |= 1ull << q;
@@ -358,7 +364,12 @@ void block_args(const Block& b, uint32_t nq, BlockArgs& a) {
             if ((cidx >> j) & 1u) o |= 1u << cpos(low[j]);
         a.coloff[cidx] = o;
     }
-    a.ntiles = (1ull << nq) >> a.ntile;
+    const uint64_t fix = tm | b.cmask;
+    a.nfix = 0;
+    for (uint32_t q = 0; q < nq; ++q)
+        if ((fix >> q) & 1ull) a.fixpos[a.nfix++] = (uint8_t)q;
+    a.fixval = b.cmask;
+    a.ntiles = (1ull << nq) >> a.nfix;
 }
 
 // Stacked real form [[Mr, -Mi], [Mi, Mr]] in MFMA A-operand order:
@@ -401,6 +412,7 @@ int apply_chunk(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, si
     for (size_t i = lo; i < hi; ++i) {
         BlockArgs a;
         block_args(blocks[i], c->nq, a);
+        if (a.L < 5) return fail(SUQA_ERR_ARG, "state too small for this block width and its controls");
         a.A = c->d_mats + off[i - lo];
         const uint32_t g = (uint32_t)(((a.ntiles + 7) / 8) * 8);
         const size_t lds = sizeof(float) * 2 * ((size_t)1 << a.ntile);
@@ -526,9 +538,10 @@ int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint3
     if (!width_ok(c->nq, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
     std::vector<Op> list(nops);
     for (uint32_t i = 0; i < nops; ++i) {
-        if (!pauli_op(ops[i], list[i])) return fail(SUQA_ERR_ARG, "malformed Pauli op or more than 6 qubits with controls");
+        if (!pauli_op(ops[i], list[i])) return fail(SUQA_ERR_ARG, "malformed Pauli op, a control on one of its qubits, or more than 6 qubits");
         if (list[i].n > width) return fail(SUQA_ERR_ARG, "Pauli op wider than the fusion width");
         if (list[i].n && list[i].qs[list[i].n - 1] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+        if (list[i].cmask >> c->nq) return fail(SUQA_ERR_ARG, "control qubit out of range");
     }
     const auto blocks = fuse(list, c->nq, width, 256);
     if (nblocks) *nblocks = (uint32_t)blocks.size();

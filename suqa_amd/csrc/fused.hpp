@@ -23,6 +23,7 @@ struct Op {
     uint32_t n = 0;
     uint32_t qs[kMaxK] = {};
     std::vector<cd> m;
+    uint64_t cmask = 0;   // control qubits (all must be 1), outside the matrix
 };
 
 // One fused block: the same, with exactly K qubits, and how many ops it holds.
@@ -31,17 +32,22 @@ struct Block {
     uint32_t qubits[kMaxK];
     std::vector<cd> m;
     uint32_t ngates;
+    uint64_t cmask = 0;   // common control of the block's ops: applied where all are 1
 };
 
-// Greedy fusion with commutation lookahead: an op joins the open block if
-// its qubits fit (at most width in total) and it shares no qubit with any
-// earlier op left out of the block (so moving it forward is exact).
+// Greedy fusion with commutation lookahead: an op joins the open block if it
+// has the block's control mask, its qubits fit (at most width in total) and
+// moving it ahead of every earlier op left out of the block is exact: its
+// qubits touch neither the targets nor the controls of those ops, and its
+// controls touch none of their targets (ops that share only controls
+// commute).
 std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width, uint32_t window);
 
 // Matrices of the workload gates (suqa_hip.h SUQA_GATE_*).
 Op gate_op(const suqa_hip_gate& g);
-// exp(i phase P) of a Pauli string, controlled on the gc_mask qubits; false
-// (and no op) if it spans more than kMaxK qubits or is malformed.
+// exp(i phase P) of a Pauli string on its non-identity qubits, controlled on
+// the gc_mask qubits (Op::cmask); false (and no op) if it is malformed or
+// acts on more than kMaxK qubits.
 bool pauli_op(const suqa_hip_pauli_op& p, Op& out);
 // H on q and the controlled phase diag(1,1,1,e^{i phase}) on {a, b}.
 Op h_op(uint32_t q);

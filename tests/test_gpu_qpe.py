@@ -66,9 +66,11 @@ def test_qpe_fp32_fused_vs_oracle(capi, oracle_result, width):
 
 def test_fp32_rejects_ops_wider_than_width(capi):
     f32 = capi.F32Engine(12)
-    ops = capi.pauli_ops([([0, 1, 2, 3], [1, 2, 3, 1], 0.1, 1 << 11)])   # 5 qubits with the control
+    ops = capi.pauli_ops([([0, 1, 2, 3, 4], [1, 2, 3, 1, 2], 0.1, 1 << 11)])   # 5 target qubits
     with pytest.raises(capi.SuqaError):
         f32.run_paulis(ops, 4)
+    with pytest.raises(capi.SuqaError):   # control on a qubit of the string
+        f32.run_paulis(capi.pauli_ops([([0, 1], [1, 3], 0.1, 1 << 1)]), 5)
     with pytest.raises(capi.SuqaError):
         f32.run_paulis(capi.pauli_ops([([0, 0], [1, 1], 0.1, 0)]), 5)      # repeated qubit
     f32.close()
