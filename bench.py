@@ -40,8 +40,11 @@ qubits + a 6-qubit register (30 qubits), 96 Pauli strings of weight 2-4
 (pcg32 seed 11), dt 0.05, 20 Trotter steps per power.  Timed sample: H on
 the top register qubit, then its 20 controlled Trotter steps (1920
 controlled rotations), in FP64 (gate passes) and FP32 fused form, with the
-fidelity between them; the full QPE applies 63 such step groups
-(120,960 rotations), projected from the sample.
+fidelity between them (FP32 block width timed at 4, 5, 6 on one step); the
+full QPE applies 63 such step groups (120,960 rotations), projected from
+the sample.  Then the whole QPE at full size with 2 Trotter steps per power
+(12,096 rotations) in both precisions: the 64-bin register histogram's L1
+and Linf error of FP32 against FP64.
 
 and the precision pattern under "extra.fused": the same circuit on an FP32
 state with gates fused into K-qubit blocks (K autotuned over 4..6 on the
@@ -217,18 +220,46 @@ def trotter_leg(capi, wl, no_fused):
         if not no_fused:
             f = capi.F32Engine(n)
             try:
+                # block width: one controlled Trotter step timed at each width
+                tw = {}
+                for wdt in (4, 5, 6):
+                    f.init_state()
+                    f.run_paulis(ops, wdt)
+                    f.synchronize()
+                    t0 = time.perf_counter()
+                    f.run_paulis(ops, wdt)
+                    f.synchronize()
+                    tw[wdt] = time.perf_counter() - t0
+                width = min(tw, key=tw.get)
                 f.init_state()
                 f.run_gates(capi.gates_from([wl.G(wl.H, top, [], 0.0)]), 6)
-                f.run_paulis(ops, 6)       # warm-up step, as in FP64
+                f.run_paulis(ops, width)   # warm-up step, as in FP64
                 f.synchronize()
                 t0 = time.perf_counter()
-                nb = f.run_paulis(capi.pauli_ops(wl.trotter_step_ops(terms, dt, 1 << top) * (steps - 1)), 6)
+                nb = f.run_paulis(capi.pauli_ops(wl.trotter_step_ops(terms, dt, 1 << top) * (steps - 1)), width)
                 f.synchronize()
                 dt32 = time.perf_counter() - t0
-                res.update({"fp32_fused_seconds": round(dt32, 4), "fp32_blocks": nb,
+                res.update({"fp32_width": width, "fp32_step_seconds_by_width": {str(k): round(v, 4) for k, v in tw.items()},
+                            "fp32_fused_seconds": round(dt32, 4), "fp32_blocks": nb,
                             "fp32_rotations_per_s": round(rot / dt32, 1),
                             "fp32_fidelity_vs_fp64": f.fidelity(e)})
-                log(f"trotter fp32: {rot / dt32:.0f} rotations/s, fidelity {res['fp32_fidelity_vs_fp64']!r}")
+                log(f"trotter fp32: {rot / dt32:.0f} rotations/s at width {width}, "
+                    f"fidelity {res['fp32_fidelity_vs_fp64']!r}")
+                # the whole QPE at full size with 2 Trotter steps per power: histograms
+                e.init_state()
+                t0 = time.perf_counter()
+                h64 = wl.qpe_trotter(e, nsys, nreg, terms, dt, 2)
+                t64 = time.perf_counter() - t0
+                f.init_state()
+                t0 = time.perf_counter()
+                h32 = wl.qpe_trotter(f, nsys, nreg, terms, dt, 2, width=width)
+                t32 = time.perf_counter() - t0
+                diff = [abs(x - y) for x, y in zip(h32, h64)]
+                res["qpe_steps2"] = {"rotations": 96 * 2 * 63, "fp64_seconds": round(t64, 2),
+                                     "fp32_seconds": round(t32, 2), "hist_l1_error": sum(diff),
+                                     "hist_linf_error": max(diff), "fp64_mode": max(range(64), key=lambda k: h64[k]),
+                                     "fp32_mode": max(range(64), key=lambda k: h32[k]), "fp64_hist_sum": sum(h64)}
+                log(f"qpe (2 steps/power): fp64 {t64:.1f} s, fp32 {t32:.1f} s, L1 {sum(diff):.3g}, Linf {max(diff):.3g}")
             finally:
                 f.close()
         return res

@@ -241,7 +241,8 @@ int suqa_hip_f32_vnorm(suqa_hip_f32* c, double* norm);
 int suqa_hip_f32_prob_filter(suqa_hip_f32* c, const uint32_t* qs, const uint32_t* q_mask, uint32_t n, double* prob);
 /* measure_qbit semantics (suqa.cu:673-696) on the FP32 state, FP64 probabilities. */
 int suqa_hip_f32_measure_qbit(suqa_hip_f32* c, uint32_t q, double rdoub, uint32_t* cbit);
-/* |<ref|psi>|^2 against an FP64 engine state of the same size (FP64 sums). */
+/* |<ref|psi>|^2 / (<ref|ref> <psi|psi>) against an FP64 engine state of the
+ * same size (FP64 sums). */
 int suqa_hip_f32_fidelity(suqa_hip_f32* c, suqa_hip_ctx* ref, double* fidelity);
 int suqa_hip_f32_download(suqa_hip_f32* c, float* re, float* im);
 int suqa_hip_f32_synchronize(suqa_hip_f32* c);

@@ -648,7 +648,14 @@ int suqa_hip_f32_fidelity(suqa_hip_f32* c, suqa_hip_ctx* ref, double* fidelity) 
     SQF_HIP(finish_sum(c, g));
     int rc = read2(c);
     if (rc) return rc;
-    if (fidelity) *fidelity = c->h_result[0] * c->h_result[0] + c->h_result[1] * c->h_result[1];
+    const double ov = c->h_result[0] * c->h_result[0] + c->h_result[1] * c->h_result[1];
+    // normalised by both norms (FP64 sums): rounding drift of the FP32 norm does
+    // not count as fidelity
+    double n32 = 0.0, n64 = 0.0;
+    rc = suqa_hip_f32_vnorm(c, &n32);
+    if (rc) return rc;
+    if (suqa_hip_vnorm(ref, &n64) != SUQA_OK) return fail(SUQA_ERR_HIP, "reference norm failed");
+    if (fidelity) *fidelity = ov / (n32 * n32 * n64 * n64);
     return SUQA_OK;
 }
 
