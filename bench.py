@@ -33,6 +33,10 @@ The QMS half of the metric is reported under "extra":
                 pcg32 seed 5, betas 0.1..5.0, seeds 1000+i, 500 steps) in one
                 persistent kernel, chain-steps/s, a sample of chains checked
                 against the oracle;
+  qms_batch_fp32 -- the same 4096 chains with FP32 amplitudes and FP64
+                probabilities: chain-steps/s, chains whose return codes equal
+                the FP64 run's, and per-beta <E> (8 bins of 512 chains)
+                against the FP64 run in units of its standard error;
   cpu_baseline_qms -- the oracle on the same single chain, 1 core.
 
 The Trotter + phase-estimation workload under "extra.trotter_qpe": 24 system
@@ -57,6 +61,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -133,6 +139,27 @@ def qms_legs(no_cpu):
                         "speedup_vs_plain_fp64_gpu": round(rate / (steps / plain["seconds"]), 1),
                         "mean_acceptance": round(float(b["accepted"].mean()) / bsteps, 4), "dtype": "f64"}
     log(f"qms batch: {rate:.0f} chain-steps/s")
+    # the same chains with FP32 amplitudes (FP64 probabilities): rate and per-beta
+    # thermal averages against the FP64 run
+    b32 = qms.batch_run(chains, bsteps, 1, 100, emin, emax, 10, 100, fp32=True)
+    rate32 = nch * bsteps / b32["seconds"]
+    same = int(sum(1 for c in range(nch) if np.array_equal(b32["rets"][c], b["rets"][c])))
+    bins = []
+    for k in range(8):   # 8 beta bins of 512 chains
+        idx = range(k * nch // 8, (k + 1) * nch // 8)
+
+        def chain_means(out):
+            return np.array([out["E"][c][:int(out["n_meas"][c])].mean() for c in idx if int(out["n_meas"][c])])
+        m64, m32 = chain_means(b), chain_means(b32)
+        se = m64.std(ddof=1) / np.sqrt(len(m64))
+        bins.append({"beta_min": round(chains[idx[0]][0], 3), "beta_max": round(chains[idx[-1]][0], 3),
+                     "E_fp64": round(float(m64.mean()), 6), "E_fp32": round(float(m32.mean()), 6),
+                     "fp64_stderr": round(float(se), 6), "diff_in_stderr": round(float(abs(m32.mean() - m64.mean()) / se), 3)})
+    res["qms_batch_fp32"] = {"chain_steps_per_s": round(rate32, 0), "kernel_seconds": round(b32["seconds"], 4),
+                             "speedup_vs_plain_fp64_gpu": round(rate32 / (steps / plain["seconds"]), 1),
+                             "chains_identical_to_fp64": same, "per_beta_E": bins,
+                             "dtype": "f32 amplitudes, f64 probabilities"}
+    log(f"qms batch fp32: {rate32:.0f} chain-steps/s, {same}/{nch} chains identical to FP64")
     if no_cpu:
         return res
     from oracle import pyoracle as po

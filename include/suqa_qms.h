@@ -65,6 +65,8 @@ typedef struct {
     uint32_t metro_steps, reset_each, max_reverse;
     double   ene_min, ene_max;
     int32_t  pe_steps, thermalization;
+    uint32_t fp32;            /* 0: FP64 chains (bit-exact with the reference CPU path);
+                                 1: FP32 amplitudes, FP64 probabilities and collapse scale */
 } suqa_qms_batch_params;
 
 /* Per chain outputs (arrays of nchains * metro_steps for rets; measurements

@@ -16,6 +16,11 @@
 // reductions (another summation order than the reference's sequential
 // loop: amplitudes agree to ~1e-15, and outcomes are identical unless a
 // uniform draw falls within that distance of a probability).
+//
+// FP32 mode (the precision pattern): the same kernel over float amplitudes
+// (half the LDS per chain), with every measurement probability summed and the
+// collapse scale computed in FP64.  Its chains follow the FP64 ones until a
+// uniform draw falls within the FP32 rounding of a probability.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -68,9 +73,10 @@ struct Rng {
     }
 };
 
+template <typename T>
 struct Chain {
-    double* re;          // LDS
-    double* im;
+    T* re;               // LDS
+    T* im;
     uint32_t r, n, N, L, acc;
     uint32_t lane;
     Rng rng;
@@ -81,8 +87,8 @@ struct Chain {
     // ------------------------------------------------------------ gates
     __device__ void init_zero() {
         for (uint32_t i = lane; i < N; i += 64) {
-            re[i] = i == 0 ? 1.0 : 0.0;
-            im[i] = 0.0;
+            re[i] = i == 0 ? T(1) : T(0);
+            im[i] = T(0);
         }
         sync();
     }
@@ -93,11 +99,11 @@ struct Chain {
             const uint32_t a = ins0(p, q);
             if ((a & gc) != gc) continue;
             const uint32_t b = a | (1u << q);
-            const double a0r = re[a], a0i = im[a], a1r = re[b], a1i = im[b];
-            re[a] = kTwoSqInv * (a0r + a1r);
-            im[a] = kTwoSqInv * (a0i + a1i);
-            re[b] = kTwoSqInv * (a0r - a1r);
-            im[b] = kTwoSqInv * (a0i - a1i);
+            const T a0r = re[a], a0i = im[a], a1r = re[b], a1i = im[b];
+            re[a] = T(kTwoSqInv) * (a0r + a1r);
+            im[a] = T(kTwoSqInv) * (a0i + a1i);
+            re[b] = T(kTwoSqInv) * (a0r - a1r);
+            im[b] = T(kTwoSqInv) * (a0i - a1i);
         }
         sync();
     }
@@ -108,7 +114,7 @@ struct Chain {
             const uint32_t a = ins0(p, qt);
             const uint32_t b = a | (1u << qt);
             if ((b & cm) != cv) continue;
-            double t = re[a];
+            T t = re[a];
             re[a] = re[b];
             re[b] = t;
             t = im[a];
@@ -124,13 +130,14 @@ struct Chain {
     }
 
     // util_rotate2 on states with all bits of m set (u1 / cu1 with q_mask 1)
-    __device__ void phase(uint32_t q1, uint32_t q2, double c, double s) {
+    __device__ void phase(uint32_t q1, uint32_t q2, double cd, double sd) {
+        const T c = T(cd), s = T(sd);
         const uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
         const uint32_t m = (1u << q1) | (1u << q2);
         const uint32_t cnt = q1 == q2 ? N / 2 : N / 4;
         for (uint32_t p = lane; p < cnt; p += 64) {
             const uint32_t i = (q1 == q2 ? ins0(p, lo) : ins0(ins0(p, lo), hi)) | m;
-            const double cpy = re[i];
+            const T cpy = re[i];
             re[i] = cpy * c - im[i] * s;
             im[i] = im[i] * c + cpy * s;
         }
@@ -138,14 +145,15 @@ struct Chain {
     }
 
     // exp(i phi X_a X_b) on states with (i & gc) == gc (suqa_cpu.hpp:422-424)
-    __device__ void xx(uint32_t qa, uint32_t qb_, double c, double s, uint32_t gc) {
+    __device__ void xx(uint32_t qa, uint32_t qb_, double cd, double sd, uint32_t gc) {
+        const T c = T(cd), s = T(sd);
         const uint32_t pv = qa < qb_ ? qa : qb_;
         const uint32_t X = (1u << qa) | (1u << qb_);
         for (uint32_t p = lane; p < N / 2; p += 64) {
             const uint32_t j = ins0(p, pv);
             if ((j & gc) != gc) continue;
             const uint32_t k = j ^ X;
-            const double jr = re[j], ji = im[j], kr = re[k], ki = im[k];
+            const T jr = re[j], ji = im[j], kr = re[k], ki = im[k];
             re[j] = jr * c - ki * s;
             im[j] = ji * c + kr * s;
             re[k] = kr * c - ji * s;
@@ -158,7 +166,8 @@ struct Chain {
     __device__ uint32_t measure(uint32_t q, double rd) {
         double s0 = 0.0, s1 = 0.0;
         for (uint32_t i = lane; i < N; i += 64) {
-            const double p = re[i] * re[i] + im[i] * im[i];
+            const double a = re[i], b = im[i];   // FP64 probabilities in both modes
+            const double p = a * a + b * b;
             if ((i >> q) & 1u) s1 += p;
             else s0 += p;
         }
@@ -168,8 +177,8 @@ struct Chain {
         const double scale = 1. / sqrt(c ? s1 : s0);
         for (uint32_t i = lane; i < N; i += 64) {
             const bool keep = ((i >> q) & 1u) == c;
-            re[i] = keep ? re[i] * scale : 0.0;
-            im[i] = keep ? im[i] * scale : 0.0;
+            re[i] = keep ? T((double)re[i] * scale) : T(0);
+            im[i] = keep ? T((double)im[i] * scale) : T(0);
         }
         sync();
         return c;
@@ -186,13 +195,13 @@ struct Chain {
         for (uint32_t j = lane; j < half; j += 64) {
             const uint32_t i = j + half;
             const uint32_t En = (i >> E0) & (L - 1);
-            const double fs1 = row[2 * En], fs2 = row[2 * En + 1];
-            double t = re[j];
-            const double ri = re[i];
+            const T fs1 = T(row[2 * En]), fs2 = T(row[2 * En + 1]);
+            T t = re[j];
+            const T ri = re[i];
             re[j] = fs2 * t + fs1 * ri;
             re[i] = fs1 * t - fs2 * ri;
             t = im[j];
-            const double ii = im[i];
+            const T ii = im[i];
             im[j] = fs2 * t + fs1 * ii;
             im[i] = fs1 * t - fs2 * ii;
         }
@@ -389,6 +398,7 @@ struct Chain {
     }
 };
 
+template <typename T>
 __global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
     extern __shared__ double lds[];
     const uint32_t lane = threadIdx.x;
@@ -397,7 +407,7 @@ __global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
         if (lane == 0) chain = atomicAdd(b.next_chain, 1u);
         chain = __builtin_amdgcn_readfirstlane(__shfl(chain, 0, 64));
         if (chain >= b.nchains) return;   // every wave reaches this once the queue is drained
-        Chain c;
+        Chain<T> c;
         c.P = &b.chains[chain];
         c.r = c.P->r;
         c.n = c.r + 4;
@@ -405,8 +415,8 @@ __global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
         c.L = 1u << c.r;
         c.acc = c.n - 1;
         c.lane = lane;
-        c.re = lds;
-        c.im = lds + c.N;
+        c.re = reinterpret_cast<T*>(lds);
+        c.im = c.re + c.N;
         c.run(b, chain);
         __syncthreads();
     }
@@ -551,7 +561,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     }
     // one wave per chain, LDS sized for the widest chain
     for (uint32_t c = 0; c < nchains; ++c) {
-        const size_t need = sizeof(double) * 2 * ((size_t)1 << (chains[c].ene_qbits + 4));
+        const size_t need = (p->fp32 ? sizeof(float) : sizeof(double)) * 2 * ((size_t)1 << (chains[c].ene_qbits + 4));
         if (need > lds_bytes) lds_bytes = need;
     }
     grid = nchains < (uint32_t)ncu * 16u ? nchains : (uint32_t)ncu * 16u;
@@ -559,7 +569,8 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     QB_HIP(hipEventCreate(&e1));
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
     QB_HIP(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_qms_batch, dim3(grid), dim3(64), lds_bytes, st, b);
+    if (p->fp32) hipLaunchKernelGGL(k_qms_batch<float>, dim3(grid), dim3(64), lds_bytes, st, b);
+    else hipLaunchKernelGGL(k_qms_batch<double>, dim3(grid), dim3(64), lds_bytes, st, b);
     QB_HIP(hipGetLastError());
     QB_HIP(hipEventRecord(e1, st));
     QB_HIP(hipStreamSynchronize(st));

@@ -99,3 +99,41 @@ def test_batch_mixed_chains_vs_oracle(qms):
         assert nm == len(E)
         assert_bitwise(out["E"][c][:nm], np.array(E), f"E chain {c}")
         assert_bitwise(out["X"][c][:nm], np.array(X), f"X chain {c}")
+
+
+def oracle_rets(beta, r, seed, steps, reset_each, therm, maxrev, emin, emax):
+    q = po.Qms(beta, r, seed, emin, emax, 10, maxrev)
+    L = po.lib()
+    L.so_qms_init_state(q.p)
+    s0, rets = 0, []
+    for s in range(steps):
+        take = (s > s0 + therm) and ((s - s0) % reset_each == 0)
+        ret = L.so_qms_metro_step(q.p, 1 if take else 0)
+        if ret < 0:
+            L.so_qms_init_state(q.p)
+            s0 = s + 1
+        rets.append(ret)
+    E, X = q.measures()
+    return rets, E, X
+
+
+def test_batch_fp32_chains_follow_fp64(qms):
+    """FP32 amplitudes with FP64 probabilities: a chain's decisions differ from
+    the FP64 oracle only when a uniform draw falls within FP32 rounding of a
+    probability, so nearly all of 48 chains x 120 steps reproduce the oracle's
+    return codes exactly, and those that do carry the same E and X values."""
+    chains = [(float(0.1 + 4.9 * k / 47), int(2 + k % 5), int(1000 + k)) for k in range(48)]
+    steps, reset_each, therm, maxrev, emin, emax = 120, 2, 10, 30, -1.0, 3.0
+    log("batch fp32 chains")
+    out = qms.batch_run(chains, steps, reset_each, maxrev, emin, emax, 10, therm, fp32=True)
+    same = 0
+    for c, (beta, r, seed) in enumerate(chains):
+        rets, E, X = oracle_rets(beta, r, seed, steps, reset_each, therm, maxrev, emin, emax)
+        if list(out["rets"][c]) == rets:
+            same += 1
+            nm = int(out["n_meas"][c])
+            assert nm == len(E)
+            assert_bitwise(out["E"][c][:nm], np.array(E), f"E chain {c}")
+            assert_bitwise(out["X"][c][:nm], np.array(X), f"X chain {c}")
+    log(f"{same} of {len(chains)} FP32 chains identical to the FP64 oracle")
+    assert same >= 44
