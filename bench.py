@@ -188,7 +188,7 @@ def fused_leg(capi, wl, eng, gates, n, layers):
     try:
         sample = capi.gates_from(gates[:10 * n])
         best, tms = f32.autotune(sample)
-        log(f"fused: autotuned width {best} ({tms})")
+        log(f"fused: autotuned width {best}, passes {f32.passes} ({tms})")
         arr = capi.gates_from(gates[:layers * n])
         t0 = time.perf_counter()
         plan = capi.fuse_plan(arr, n, best)
@@ -205,7 +205,8 @@ def fused_leg(capi, wl, eng, gates, n, layers):
         log(f"fused: {nb} blocks for {layers * n} gates in {dt:.3f} s, fidelity {fid!r}")
         amp_bytes = 16 * (1 << n)     # an FP32 complex amplitude read and written per block
         return f32, {
-            "width": best, "ms_per_gate_by_width": {str(k): (None if v < 0 else round(v, 4)) for k, v in tms.items()},
+            "width": best, "mode": "passes" if f32.passes else "blocks",
+            "ms_per_gate_by_mode_width": {k: (None if v < 0 else round(v, 4)) for k, v in tms.items()},
             "gates": layers * n, "blocks": nb, "gates_per_block": round(layers * n / nb, 2),
             "seconds": round(dt, 4), "layers_per_s": round(layers / dt, 3),
             "ms_per_layer": round(dt / layers * 1e3, 3), "host_fusion_seconds": round(t_plan, 4),
@@ -247,17 +248,21 @@ def trotter_leg(capi, wl, no_fused):
         if not no_fused:
             f = capi.F32Engine(n)
             try:
-                # block width: one controlled Trotter step timed at each width
+                # mode and block width: one controlled Trotter step timed at each
                 tw = {}
-                for wdt in (4, 5, 6):
-                    f.init_state()
-                    f.run_paulis(ops, wdt)
-                    f.synchronize()
-                    t0 = time.perf_counter()
-                    f.run_paulis(ops, wdt)
-                    f.synchronize()
-                    tw[wdt] = time.perf_counter() - t0
-                width = min(tw, key=tw.get)
+                for mode in ("blocks", "passes"):
+                    f.passes = mode == "passes"
+                    for wdt in (4, 5, 6):
+                        f.init_state()
+                        f.run_paulis(ops, wdt)
+                        f.synchronize()
+                        t0 = time.perf_counter()
+                        f.run_paulis(ops, wdt)
+                        f.synchronize()
+                        tw[f"{mode}{wdt}"] = time.perf_counter() - t0
+                best = min(tw, key=tw.get)
+                f.passes = best.startswith("passes")
+                width = int(best[-1])
                 f.init_state()
                 f.run_gates(capi.gates_from([wl.G(wl.H, top, [], 0.0)]), 6)
                 f.run_paulis(ops, width)   # warm-up step, as in FP64
@@ -266,11 +271,12 @@ def trotter_leg(capi, wl, no_fused):
                 nb = f.run_paulis(capi.pauli_ops(wl.trotter_step_ops(terms, dt, 1 << top) * (steps - 1)), width)
                 f.synchronize()
                 dt32 = time.perf_counter() - t0
-                res.update({"fp32_width": width, "fp32_step_seconds_by_width": {str(k): round(v, 4) for k, v in tw.items()},
+                res.update({"fp32_width": width, "fp32_mode": best[:-1],
+                            "fp32_step_seconds_by_mode_width": {k: round(v, 4) for k, v in tw.items()},
                             "fp32_fused_seconds": round(dt32, 4), "fp32_blocks": nb,
                             "fp32_rotations_per_s": round(rot / dt32, 1),
                             "fp32_fidelity_vs_fp64": f.fidelity(e)})
-                log(f"trotter fp32: {rot / dt32:.0f} rotations/s at width {width}, "
+                log(f"trotter fp32: {rot / dt32:.0f} rotations/s at {best}, "
                     f"fidelity {res['fp32_fidelity_vs_fp64']!r}")
                 # the whole QPE at full size with 2 Trotter steps per power: histograms
                 e.init_state()

@@ -218,13 +218,17 @@ const char* suqa_hip_f32_last_error(void);
 int suqa_hip_f32_setup(uint32_t nq, suqa_hip_f32** out);
 int suqa_hip_f32_clear(suqa_hip_f32* c);
 int suqa_hip_f32_init_state(suqa_hip_f32* c);
+/* Application mode of the fused blocks: 0 (the default) fuses the whole list
+ * and applies each block in its own launch (one HBM round trip per block);
+ * 1 cuts the list into passes of consecutive ops whose targets fit a 12-qubit
+ * tile, fuses each pass and applies all of its blocks in one round trip.
+ * Results agree to FP32 rounding (the fusion differs). */
+int suqa_hip_f32_set_passes(suqa_hip_f32* c, uint32_t on);
+int suqa_hip_f32_get_passes(const suqa_hip_f32* c, uint32_t* on);
 /* Fuses gates[0..ngates) into width-qubit blocks and applies them in order;
- * *nblocks (may be NULL) receives the number of blocks (kernel launches). */
+ * *nblocks (may be NULL) receives the number of blocks. */
 int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t width,
                            uint32_t* nblocks);
-/* Times the gate sample at every valid width on this context's state (which
- * it changes) and returns the fastest; ms_per_gate[w - 4] gets each width's
- * wall time per gate (-1 where the width does not fit). */
 /* The same fused application for Pauli-string rotations: each string's
  * non-identity qubits (at most `width`) carry the block matrix, and the
  * gc_mask controls select the tiles it is applied to (ops sharing a control
@@ -235,6 +239,10 @@ int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint3
 /* suqa::apply_qft_inverse (suqa.cu:649-661, no final swaps) on qs[0..n),
  * its H and controlled-phase gates fused into width-qubit blocks. */
 int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, uint32_t width);
+/* Times the gate sample at every valid width in both modes on this context's
+ * state (which it changes), returns the fastest width and leaves the fastest
+ * mode set; ms_per_gate[3 * mode + width - 4] (6 entries) gets the wall time
+ * per gate (-1 where the width does not fit). */
 int suqa_hip_f32_autotune(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t* best_width,
                           double* ms_per_gate);
 int suqa_hip_f32_vnorm(suqa_hip_f32* c, double* norm);

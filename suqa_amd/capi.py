@@ -90,6 +90,8 @@ SIGNATURES = {
     "suqa_hip_f32_run_paulis": (C.c_int, [CTX, C.POINTER(PauliOp), U32, U32, PU32]),
     "suqa_hip_f32_qft_inverse": (C.c_int, [CTX, PU32, U32, U32]),
     "suqa_hip_f32_autotune": (C.c_int, [CTX, C.POINTER(Gate), U32, PU32, PD]),
+    "suqa_hip_f32_set_passes": (C.c_int, [CTX, U32]),
+    "suqa_hip_f32_get_passes": (C.c_int, [CTX, PU32]),
     "suqa_hip_f32_vnorm": (C.c_int, [CTX, PD]),
     "suqa_hip_f32_prob_filter": (C.c_int, [CTX, PU32, PU32, U32, PD]),
     "suqa_hip_f32_measure_qbit": (C.c_int, [CTX, U32, C.c_double, PU32]),
@@ -378,12 +380,25 @@ class F32Engine:
     def qft_inverse(self, qs, width):
         check32(lib().suqa_hip_f32_qft_inverse(self.ctx, _u32(qs), len(qs), width))
 
+    @property
+    def passes(self):
+        """Application mode: False = one block per launch, True = block passes."""
+        v = U32()
+        check32(lib().suqa_hip_f32_get_passes(self.ctx, C.byref(v)))
+        return bool(v.value)
+
+    @passes.setter
+    def passes(self, on):
+        check32(lib().suqa_hip_f32_set_passes(self.ctx, 1 if on else 0))
+
     def autotune(self, gates, n=None):
+        """Times the sample at widths 4..6 in both modes, leaves the fastest mode
+        set; returns (width, {"blocks4": ms per gate, ..., "passes6": ...})."""
         n = len(gates) if n is None else n
         best = U32()
-        t = (C.c_double * 3)()
+        t = (C.c_double * 6)()
         check32(lib().suqa_hip_f32_autotune(self.ctx, gates, n, C.byref(best), t))
-        return best.value, {4 + k: t[k] for k in range(3)}
+        return best.value, {f"{m}{4 + k}": t[3 * i + k] for i, m in enumerate(("blocks", "passes")) for k in range(3)}
 
     def vnorm(self):
         v = C.c_double()

@@ -160,7 +160,8 @@ void qft_inverse_ops(const uint32_t* qs, uint32_t n, std::vector<Op>& out) {
     }
 }
 
-std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width, uint32_t window) {
+std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width, uint32_t window,
+                        uint64_t pad_allowed) {
     const uint32_t n = (uint32_t)ops.size();
     std::vector<Block> out;
     std::vector<char> used(n, 0);
@@ -195,9 +196,9 @@ std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width,
             if ((uint32_t)__builtin_popcountll(lt | lc) >= nq || (lt & cm) != 0 || (full && ((lt | lc) & S) == S))
                 break;
         }
-        // pad the block to exactly `width` qubits with the lowest free non-control qubits
+        // pad the block to exactly `width` qubits with the lowest free allowed non-control qubits
         for (uint32_t q = 0; q < nq && (uint32_t)__builtin_popcountll(S) < width; ++q)
-            if (!((cm >> q) & 1ull)) S |= 1ull << q;
+            if (!((cm >> q) & 1ull) && ((pad_allowed >> q) & 1ull)) S |= 1ull << q;
         Block b;
         b.K = 0;
         b.cmask = cm;

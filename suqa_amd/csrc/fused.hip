@@ -4,16 +4,18 @@
 // Norms, measurement probabilities, the collapse scale and the fidelity
 // against an FP64 state are accumulated in FP64.
 //
-// Block kernel.  A block on qubits B (|B| = K) multiplies every group of 2^K
-// amplitudes that share the other bits.  A workgroup takes one tile: all
-// combinations of B and of the L lowest non-block bits (L = 7 -> 128 groups as
-// MFMA columns), i.e. 2^(K+L) amplitudes whose low address bits are all tile
-// bits, so the tile is a set of contiguous runs of >= 128 floats.
-//   1. load the tile (re, im) into LDS in compact bit order (16-byte loads);
-//   2. Y = A X with X = [Xr; Xi] (2^(K+1) x 2^L) and the real form
-//      A = [[Mr, -Mi], [Mi, Mr]]; a wave owns one 32-row block of A (held in
-//      VGPRs, host-permuted to the MFMA lane order) and RB 32-column blocks;
-//   3. write Y back into LDS, then to HBM.
+// Block passes.  The op list is cut, in order, into passes of consecutive ops
+// whose target qubits fit one tile of t = 12 qubits (controls outside it);
+// each pass is fused into K-qubit blocks inside its tile (fuse.cpp) and one
+// kernel applies all of them: a workgroup loads a tile (2^12 amplitudes, the
+// lowest 5 non-control qubits always included so a tile is whole 128-byte
+// runs) into LDS, applies the blocks one after the other, and writes it back.
+// A block on K tile qubits is Y = A X with X = [Xr; Xi] (2^(K+1) x 2^(12-K),
+// the other tile bits as columns) and the real form A = [[Mr, -Mi], [Mi, Mr]];
+// a wave holds one 32-row block of A in VGPRs (host-permuted to the MFMA lane
+// order) and takes up to 2 of the (row block, 32-column block) pairs.  A block
+// whose ops share a control applies only where the control is 1; a control
+// common to the whole launch is fixed in the tile enumeration.
 // MFMA 32x32x2 f32 operand layout: A[i][k] in lane i + 32k, B[k][j] in lane
 // j + 32k, D[i][j] in lane j + 32 * ((i / 4) % 2), register 4 * (i / 8) + i % 4.
 #include <hip/hip_runtime.h>
@@ -45,6 +47,7 @@ struct suqa_hip_f32 {
     double* h_result = nullptr;
     float* d_mats = nullptr;   // block matrices in MFMA lane order
     size_t mats_cap = 0;       // floats
+    uint32_t passes = 0;       // 0: one block per launch, 1: block passes
 };
 
 namespace sqf {
@@ -79,6 +82,11 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// One block per launch: a workgroup takes one tile = the K block qubits plus the
+// 7 lowest other qubits (2^(K+7) amplitudes, whole 512-byte runs), applies the
+// block by MFMA and writes the tile back; the blocks come from fusing the whole
+// op list (fuse.cpp).
 struct BlockArgs {
     uint32_t tpos[kMaxK + kTileLow];   // tile bit positions, ascending (compact order)
     uint32_t ntile;                    // K + L
@@ -173,6 +181,140 @@ __global__ __launch_bounds__(kBlock) void k_block(float* __restrict__ re, float*
     // 3. LDS -> tile
     for (uint32_t ch = tid; ch < tsz / 4; ch += kBlock) {
         const uint64_t g = base | pdep_tile(ch * 4, a);
+        __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lre + ch * 4), reinterpret_cast<floatx4*>(re + g));
+        __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lim + ch * 4), reinterpret_cast<floatx4*>(im + g));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block passes: several fused blocks per HBM round trip.  A workgroup loads a
+// tile of 2^t amplitudes (the t tile qubits, which hold every block qubit of
+// the pass) into LDS, applies the pass's blocks one after the other by MFMA
+// on the LDS-resident tile, and writes it back.  A block on K tile qubits
+// multiplies 2^(t-K) columns (the other tile bits); the 4 waves share its
+// 2^(K+1)/32 row blocks x 2^(t-K)/32 column blocks, at most kItems each.
+constexpr uint32_t kPassTileBits = 12;   // 32 KiB of FP32 complex amplitudes per tile
+constexpr uint32_t kPassBlocks = 16;     // blocks per launch
+constexpr uint32_t kItems = 2;           // (row block, column block) pairs per wave at t = 12
+
+struct PassBlock {
+    uint32_t K;
+    uint8_t bpos[kMaxK];   // tile positions of the block qubits, ascending
+    uint8_t pad_[2];
+    uint64_t mo, vo;       // controls (outside the tile): applied where (base & mo) == vo
+    const float* A;        // MFMA-ordered stacked matrix
+};
+
+struct PassArgs {
+    uint32_t t, nfix, nblocks, pad_;
+    uint8_t tpos[16];      // tile bit positions, ascending
+    uint8_t fixpos[64];    // tile bits and common controls, ascending
+    uint64_t fixval;
+    uint64_t ntiles;
+    PassBlock b[kPassBlocks];
+};
+
+__device__ __forceinline__ uint64_t pass_offset(uint32_t e, const PassArgs& a) {
+    uint64_t o = 0;
+    for (uint32_t j = 0; j < a.t; ++j)
+        if ((e >> j) & 1u) o |= 1ull << a.tpos[j];
+    return o;
+}
+
+template <int K>
+__device__ __forceinline__ void pass_block(float* lre, float* lim, uint32_t t, const PassBlock& B) {
+    constexpr int Dc = 1 << K;
+    constexpr int RB = (2 * Dc) / 32;   // 32-row blocks of the stacked matrix (1, 2, 4)
+    constexpr int KS = Dc;              // k-steps of 2 rows
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t ncb = (1u << (t - K)) / 32;   // 32-column blocks
+    const uint32_t items = RB * ncb;
+    // tile offset of block row r: its bits deposited at the block positions (uniform)
+    auto rowo = [&](uint32_t r) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+            if ((r >> q) & 1u) o |= 1u << B.bpos[q];
+        return o;
+    };
+    const int rb = w % RB;              // 4 % RB == 0: every item of a wave has this row block
+    float areg[KS];
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) areg[kk] = B.A[((size_t)rb * KS + kk) * 64 + lane];
+    floatx16 acc[kItems];
+    uint32_t coff[kItems];
+#pragma unroll
+    for (uint32_t it = 0; it < kItems; ++it) {
+        acc[it] = floatx16{0};
+        coff[it] = 0;
+        const uint32_t item = (uint32_t)w + 4u * it;
+        if (item >= items) continue;
+        uint32_t col = (item / RB) * 32 + (uint32_t)(lane & 31);   // column index -> tile offset
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const uint32_t lo = (1u << B.bpos[q]) - 1u;
+            col = (col & lo) | ((col & ~lo) << 1);
+        }
+        coff[it] = col;
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            // rows 2kk, 2kk+1 of the stacked matrix: both real (2kk < Dc) or both imaginary;
+            // their offsets are uniform, the lane half picks one
+            const int R0 = (2 * kk) % Dc;   // even: row R0 + 1 adds block bit 0
+            const uint32_t o = rowo((uint32_t)R0) | ((uint32_t)(lane >> 5) << B.bpos[0]);
+            const float b = (2 * kk < Dc) ? lre[o + col] : lim[o + col];
+            acc[it] = __builtin_amdgcn_mfma_f32_32x32x2f32(areg[kk], b, acc[it], 0, 0, 0);
+        }
+    }
+    __syncthreads();   // every read of the block's input is done
+#pragma unroll
+    for (uint32_t it = 0; it < kItems; ++it) {
+        if ((uint32_t)w + 4u * it >= items) continue;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            // output row rb*32 + 8(v/4) + 4(lane/32) + v%4: uniform up to the lane half
+            const int i0 = rb * 32 + 8 * (v >> 2) + (v & 3);
+            const bool imag = i0 >= Dc;   // rows i0 and i0 + 4 lie in the same half (Dc >= 16)
+            const int r0 = imag ? i0 - Dc : i0;   // r0 % 8 < 4: row r0 + 4 adds block bit 2
+            const uint32_t o = rowo((uint32_t)r0) | ((uint32_t)(lane >> 5) << B.bpos[2]);
+            if (imag) lim[o + coff[it]] = acc[it][v];
+            else lre[o + coff[it]] = acc[it][v];
+        }
+    }
+    __syncthreads();
+}
+
+// All blocks of a launch have K = the fusion width.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_bpass(float* __restrict__ re, float* __restrict__ im, const PassArgs a) {
+    extern __shared__ float lds[];
+    const uint32_t tsz = 1u << a.t;
+    float* lre = lds;
+    float* lim = lds + tsz;
+    const uint64_t tile = ((uint64_t)(blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+    if (tile >= a.ntiles) return;      // whole workgroup leaves together
+    uint64_t base = tile;
+    for (uint32_t j = 0; j < a.nfix; ++j) {
+        const uint64_t lo = (1ull << a.fixpos[j]) - 1ull;
+        base = (base & lo) | ((base & ~lo) << 1);
+    }
+    base |= a.fixval;
+    const int tid = threadIdx.x;
+    for (uint32_t ch = tid; ch < tsz / 4; ch += kBlock) {   // tile bits 0, 1 are qubits 0, 1
+        const uint64_t g = base | pass_offset(ch * 4, a);
+        const floatx4 vr = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(re + g));
+        const floatx4 vi = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(im + g));
+        *reinterpret_cast<floatx4*>(lre + ch * 4) = vr;
+        *reinterpret_cast<floatx4*>(lim + ch * 4) = vi;
+    }
+    __syncthreads();
+    for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
+        const PassBlock& B = a.b[bi];
+        if ((base & B.mo) != B.vo) continue;   // uniform: a control of this tile is 0
+        pass_block<K>(lre, lim, a.t, B);
+    }
+    for (uint32_t ch = tid; ch < tsz / 4; ch += kBlock) {
+        const uint64_t g = base | pass_offset(ch * 4, a);
         __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lre + ch * 4), reinterpret_cast<floatx4*>(re + g));
         __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lim + ch * 4), reinterpret_cast<floatx4*>(im + g));
     }
@@ -333,6 +475,23 @@ int norm_split(suqa_hip_f32* c, uint64_t M, uint64_t V, uint64_t split) {
     return read2(c);
 }
 
+// Stacked real form [[Mr, -Mi], [Mi, Mr]] in MFMA A-operand order:
+// out[(rb * KS + kk) * 64 + lane] = A[rb*32 + lane%32][2 kk + lane/32].
+void mfma_order(const Block& b, float* out) {
+    const uint32_t Dc = 1u << b.K, D = 2 * Dc, KS = D / 2, RB = D / 32;
+    auto A = [&](uint32_t i, uint32_t k) -> double {
+        const bool ii = i >= Dc, ki = k >= Dc;
+        const std::complex<double> m = b.m[(size_t)(i % Dc) * Dc + (k % Dc)];
+        if (!ii && !ki) return m.real();
+        if (!ii && ki) return -m.imag();
+        if (ii && !ki) return m.imag();
+        return m.real();
+    };
+    for (uint32_t rb = 0; rb < RB; ++rb)
+        for (uint32_t kk = 0; kk < KS; ++kk)
+            for (uint32_t l = 0; l < 64; ++l) out[((size_t)rb * KS + kk) * 64 + l] = (float)A(rb * 32 + l % 32, 2 * kk + l / 32);
+}
+
 // Host: tile geometry and MFMA-ordered matrices of one block.
 void block_args(const Block& b, uint32_t nq, BlockArgs& a) {
     std::memset(&a, 0, sizeof(a));
@@ -370,23 +529,6 @@ void block_args(const Block& b, uint32_t nq, BlockArgs& a) {
         if ((fix >> q) & 1ull) a.fixpos[a.nfix++] = (uint8_t)q;
     a.fixval = b.cmask;
     a.ntiles = (1ull << nq) >> a.nfix;
-}
-
-// Stacked real form [[Mr, -Mi], [Mi, Mr]] in MFMA A-operand order:
-// out[(rb * KS + kk) * 64 + lane] = A[rb*32 + lane%32][2 kk + lane/32].
-void mfma_order(const Block& b, float* out) {
-    const uint32_t Dc = 1u << b.K, D = 2 * Dc, KS = D / 2, RB = D / 32;
-    auto A = [&](uint32_t i, uint32_t k) -> double {
-        const bool ii = i >= Dc, ki = k >= Dc;
-        const std::complex<double> m = b.m[(size_t)(i % Dc) * Dc + (k % Dc)];
-        if (!ii && !ki) return m.real();
-        if (!ii && ki) return -m.imag();
-        if (ii && !ki) return m.imag();
-        return m.real();
-    };
-    for (uint32_t rb = 0; rb < RB; ++rb)
-        for (uint32_t kk = 0; kk < KS; ++kk)
-            for (uint32_t l = 0; l < 64; ++l) out[((size_t)rb * KS + kk) * 64 + l] = (float)A(rb * 32 + l % 32, 2 * kk + l / 32);
 }
 
 // Uploads the MFMA-ordered matrices of blocks [lo, hi) and launches them.
@@ -436,6 +578,141 @@ int apply_blocks(suqa_hip_f32* c, const std::vector<Block>& blocks) {
     for (size_t lo = 0; lo < blocks.size(); lo += kChunk)
         if (int rc = apply_chunk(c, blocks, lo, std::min(blocks.size(), lo + kChunk))) return rc;
     return SUQA_OK;
+}
+
+// ---- block passes (host) ----------------------------------------------------
+
+struct PassLaunch {
+    PassArgs a;
+    size_t off[kPassBlocks];   // each block's matrix in the chunk buffer (floats)
+};
+
+// Uploads the chunk buffer, points the launches at it, launches them in order
+// and waits (the buffer is rewritten by the next chunk).
+int flush_passes(suqa_hip_f32* c, std::vector<PassLaunch>& L, std::vector<float>& buf) {
+    if (L.empty()) return SUQA_OK;
+    if (c->mats_cap < buf.size()) {
+        if (c->d_mats) SQF_HIP(hipFree(c->d_mats));
+        c->d_mats = nullptr;
+        c->mats_cap = 0;
+        void* p = nullptr;
+        SQF_HIP(hipMalloc(&p, buf.size() * sizeof(float)));
+        c->d_mats = static_cast<float*>(p);
+        c->mats_cap = buf.size();
+    }
+    SQF_HIP(hipMemcpyAsync(c->d_mats, buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    for (auto& l : L) {
+        for (uint32_t k = 0; k < l.a.nblocks; ++k) l.a.b[k].A = c->d_mats + l.off[k];
+        const uint32_t g = (uint32_t)(((l.a.ntiles + 7) / 8) * 8);
+        const size_t lds = sizeof(float) * 2 * ((size_t)1 << l.a.t);
+        switch (l.a.b[0].K) {
+        case 4: hipLaunchKernelGGL(k_bpass<4>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, l.a); break;
+        case 5: hipLaunchKernelGGL(k_bpass<5>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, l.a); break;
+        case 6: hipLaunchKernelGGL(k_bpass<6>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, l.a); break;
+        default: return fail(SUQA_ERR_ARG, "block width must be 4..6");
+        }
+        SQF_HIP(hipGetLastError());
+    }
+    SQF_HIP(hipStreamSynchronize(c->stream));
+    L.clear();
+    buf.clear();
+    return SUQA_OK;
+}
+
+// One launch of blocks [lo, hi) of a pass on tile qubits T.
+int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, size_t hi, uint64_t T, uint32_t t,
+               std::vector<PassLaunch>& L, std::vector<float>& buf) {
+    const uint32_t nq = c->nq;
+    PassLaunch l;
+    std::memset(&l, 0, sizeof(l));
+    PassArgs& a = l.a;
+    a.t = t;
+    uint32_t k = 0;
+    uint32_t pos[64] = {};
+    for (uint32_t q = 0; q < nq; ++q)
+        if ((T >> q) & 1ull) {
+            pos[q] = k;
+            a.tpos[k++] = (uint8_t)q;
+        }
+    uint64_t mcom = ~0ull;
+    a.nblocks = (uint32_t)(hi - lo);
+    for (size_t i = lo; i < hi; ++i) {
+        const Block& b = blocks[i];
+        PassBlock& B = a.b[i - lo];
+        B.K = b.K;
+        for (uint32_t j = 0; j < b.K; ++j) B.bpos[j] = (uint8_t)pos[b.qubits[j]];
+        B.mo = B.vo = b.cmask;
+        mcom &= b.cmask;
+        // MFMA-ordered stacked matrix, then the tile offsets of the block rows
+        const size_t D = (size_t)2 << b.K;
+        l.off[i - lo] = buf.size();
+        buf.resize(buf.size() + D * D);
+        mfma_order(b, buf.data() + l.off[i - lo]);
+    }
+    // controls shared by every block: tiles where one of them is 0 are not visited
+    const uint64_t fix = T | mcom;
+    for (uint32_t q = 0; q < nq; ++q)
+        if ((fix >> q) & 1ull) a.fixpos[a.nfix++] = (uint8_t)q;
+    a.fixval = mcom;
+    a.ntiles = (1ull << nq) >> a.nfix;
+    L.push_back(l);
+    return SUQA_OK;
+}
+
+// Applies ops in order: cut into passes of consecutive ops whose targets fit a
+// tile of t qubits (controls outside it), each pass fused into width-qubit
+// blocks inside its tile and applied by k_bpass, several blocks per HBM round
+// trip.
+int apply_ops(suqa_hip_f32* c, const std::vector<Op>& ops, uint32_t width, uint32_t* nblocks) {
+    const uint32_t nq = c->nq;
+    const uint32_t t = std::min<uint32_t>(kPassTileBits, nq);
+    if (t < width + 5) return fail(SUQA_ERR_ARG, "state too small for this block width");
+    std::vector<PassLaunch> L;
+    std::vector<float> buf;
+    uint32_t total = 0;
+    size_t i = 0;
+    while (i < ops.size()) {
+        uint64_t T = 0;   // the 5 lowest qubits that are not controls of the first op
+        for (uint32_t q = 0; q < nq && __builtin_popcountll(T) < 5; ++q)
+            if (!((ops[i].cmask >> q) & 1ull)) T |= 1ull << q;
+        uint64_t C = 0;
+        size_t j = i;
+        while (j < ops.size() && j - i < 64) {
+            uint64_t G = 0;
+            for (uint32_t k2 = 0; k2 < ops[j].n; ++k2) G |= 1ull << ops[j].qs[k2];
+            const uint64_t Cj = ops[j].cmask;
+            if ((Cj & (T | G)) != 0 || (G & C) != 0 || (uint32_t)__builtin_popcountll(T | G) > t) break;
+            T |= G;
+            C |= Cj;
+            ++j;
+        }
+        if (j == i) return fail(SUQA_ERR_ARG, "an op does not fit a tile");
+        // the tile loads move 4 consecutive amplitudes: qubits 0 and 1 must be tile bits
+        if ((T & 3ull) != 3ull) return fail(SUQA_ERR_ARG, "controls on qubits 0 or 1 are not supported by FP32 passes");
+        for (uint32_t q = 0; q < nq && (uint32_t)__builtin_popcountll(T) < t; ++q)
+            if (!((C >> q) & 1ull)) T |= 1ull << q;
+        if ((uint32_t)__builtin_popcountll(T) < t) return fail(SUQA_ERR_ARG, "state too small for a tile besides the controls");
+        const std::vector<Op> part(ops.begin() + (long)i, ops.begin() + (long)j);
+        const auto blocks = fuse(part, nq, width, 256, T);
+        total += (uint32_t)blocks.size();
+        for (size_t lo = 0; lo < blocks.size(); lo += kPassBlocks) {
+            add_launch(c, blocks, lo, std::min(blocks.size(), lo + kPassBlocks), T, t, L, buf);
+            if (buf.size() > (size_t)4 << 20)   // 16 MiB of matrices per upload
+                if (int rc = flush_passes(c, L, buf)) return rc;
+        }
+        i = j;
+    }
+    if (nblocks) *nblocks = total;
+    return flush_passes(c, L, buf);
+}
+
+// Applies ops in the context's mode: block passes, or the whole list fused
+// into blocks applied one launch each.
+int run_ops(suqa_hip_f32* c, const std::vector<Op>& ops, uint32_t width, uint32_t* nblocks) {
+    if (c->passes) return apply_ops(c, ops, width, nblocks);
+    const auto blocks = fuse(ops, c->nq, width, 256);
+    if (nblocks) *nblocks = (uint32_t)blocks.size();
+    return apply_blocks(c, blocks);
 }
 
 // Workload gates must name distinct in-range qubits (a block's tile geometry
@@ -527,9 +804,7 @@ int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t
     std::vector<Op> ops;
     ops.reserve(ngates);
     for (uint32_t i = 0; i < ngates; ++i) ops.push_back(gate_op(gates[i]));
-    const auto blocks = fuse(ops, c->nq, width, 256);
-    if (nblocks) *nblocks = (uint32_t)blocks.size();
-    return apply_blocks(c, blocks);
+    return run_ops(c, ops, width, nblocks);
 }
 
 int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint32_t nops, uint32_t width,
@@ -543,9 +818,7 @@ int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint3
         if (list[i].n && list[i].qs[list[i].n - 1] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
         if (list[i].cmask >> c->nq) return fail(SUQA_ERR_ARG, "control qubit out of range");
     }
-    const auto blocks = fuse(list, c->nq, width, 256);
-    if (nblocks) *nblocks = (uint32_t)blocks.size();
-    return apply_blocks(c, blocks);
+    return run_ops(c, list, width, nblocks);
 }
 
 int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, uint32_t width) {
@@ -558,35 +831,55 @@ int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, ui
     }
     std::vector<Op> list;
     qft_inverse_ops(qs, n, list);
-    return apply_blocks(c, fuse(list, c->nq, width, 256));
+    return run_ops(c, list, width, nullptr);
+}
+
+int suqa_hip_f32_set_passes(suqa_hip_f32* c, uint32_t on) {
+    if (!c) return fail(SUQA_ERR_ARG, "null context");
+    if (on > 1) return fail(SUQA_ERR_ARG, "mode must be 0 (one block per launch) or 1 (block passes)");
+    c->passes = on;
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_get_passes(const suqa_hip_f32* c, uint32_t* on) {
+    if (!c || !on) return fail(SUQA_ERR_ARG, "null argument");
+    *on = c->passes;
+    return SUQA_OK;
 }
 
 int suqa_hip_f32_autotune(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t* best_width,
                           double* ms_per_gate) {
     if (!c || !c->d_data || !best_width || !gates || !ngates) return fail(SUQA_ERR_ARG, "null argument or empty sample");
     if (int rc = check_gates(c, gates, ngates)) return rc;
+    std::vector<Op> ops;
+    for (uint32_t i = 0; i < ngates; ++i) ops.push_back(gate_op(gates[i]));
+    const std::vector<Op> warm(ops.begin(), ops.begin() + std::min<size_t>(ops.size(), 8));
     double best = 1e300;
+    uint32_t best_mode = 0;
     *best_width = 0;
-    for (uint32_t K = kMinK; K <= (uint32_t)kMaxK; ++K) {
-        if (ms_per_gate) ms_per_gate[K - kMinK] = -1.0;
-        if (!width_ok(c->nq, K)) continue;
-        std::vector<Op> ops;
-        for (uint32_t i = 0; i < ngates; ++i) ops.push_back(gate_op(gates[i]));
-        const auto blocks = fuse(ops, c->nq, K, 256);
-        // warm-up on the first block, then time the whole sample
-        int rc = apply_blocks(c, std::vector<Block>(blocks.begin(), blocks.begin() + 1));
-        if (rc) return rc;
-        const auto t0 = std::chrono::steady_clock::now();
-        rc = apply_blocks(c, blocks);
-        if (rc) return rc;
-        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        const double per = ms / std::max<uint32_t>(ngates, 1);
-        if (ms_per_gate) ms_per_gate[K - kMinK] = per;
-        if (per < best) {
-            best = per;
-            *best_width = K;
+    for (uint32_t mode = 0; mode < 2; ++mode)
+        for (uint32_t K = kMinK; K <= (uint32_t)kMaxK; ++K) {
+            const uint32_t slot = mode * (kMaxK - kMinK + 1) + (K - kMinK);
+            if (ms_per_gate) ms_per_gate[slot] = -1.0;
+            if (!width_ok(c->nq, K)) continue;
+            c->passes = mode;
+            // warm-up on the first ops, then time the whole sample
+            int rc = run_ops(c, warm, K, nullptr);
+            if (rc) return rc;
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = run_ops(c, ops, K, nullptr);
+            if (rc) return rc;
+            SQF_HIP(hipStreamSynchronize(c->stream));
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            const double per = ms / std::max<uint32_t>(ngates, 1);
+            if (ms_per_gate) ms_per_gate[slot] = per;
+            if (per < best) {
+                best = per;
+                *best_width = K;
+                best_mode = mode;
+            }
         }
-    }
+    c->passes = best_mode;
     return *best_width ? SUQA_OK : fail(SUQA_ERR_ARG, "no fusion width fits this state");
 }
 

@@ -12,7 +12,7 @@
 
 namespace sqf {
 
-constexpr int kTileLow = 7;        // non-block bits inside a tile: 128 MFMA columns
+constexpr int kTileLow = 7;        // non-block bits inside a one-block tile: 128 MFMA columns
 constexpr int kMinK = 4, kMaxK = 6;
 
 using cd = std::complex<double>;
@@ -41,7 +41,10 @@ struct Block {
 // qubits touch neither the targets nor the controls of those ops, and its
 // controls touch none of their targets (ops that share only controls
 // commute).
-std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width, uint32_t window);
+// Blocks are padded to exactly `width` qubits with the lowest free qubits of
+// pad_allowed (and never with a control qubit of the block).
+std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width, uint32_t window,
+                        uint64_t pad_allowed = ~0ull);
 
 // Matrices of the workload gates (suqa_hip.h SUQA_GATE_*).
 Op gate_op(const suqa_hip_gate& g);

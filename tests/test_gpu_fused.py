@@ -32,9 +32,10 @@ def circuit(n, layers, seed):
     return [wl.G(wl.H, q, [], 0.0) for q in range(n)] + gates
 
 
+@pytest.mark.parametrize("passes", [False, True], ids=["blocks", "passes"])
 @pytest.mark.parametrize("n,width", [(9, 4), (10, 5), (11, 6), (12, 4), (13, 5), (14, 6)])
-def test_fused_vs_fp64_and_oracle(capi, n, width):
-    log(f"fused n={n} width={width}")
+def test_fused_vs_fp64_and_oracle(capi, n, width, passes):
+    log(f"fused n={n} width={width} passes={passes}")
     allg = circuit(n, 24, 11 + n)
     arr = capi.gates_from(allg)
     e64 = capi.Engine(n)
@@ -42,6 +43,8 @@ def test_fused_vs_fp64_and_oracle(capi, n, width):
     e64.run_gates(arr)
     r64, i64 = e64.get()
     f32 = capi.F32Engine(n)
+    f32.passes = passes
+    assert f32.passes == passes
     f32.init_state()
     nb = f32.run_gates(arr, width)
     assert 0 < nb < len(allg)
@@ -71,15 +74,17 @@ def test_fused_vs_fp64_and_oracle(capi, n, width):
     e64.close()
 
 
-def test_fused_measure_matches_fp64(capi):
+@pytest.mark.parametrize("passes", [False, True], ids=["blocks", "passes"])
+def test_fused_measure_matches_fp64(capi, passes):
     n = 12
-    log(f"fused measure n={n}")
+    log(f"fused measure n={n} passes={passes}")
     allg = circuit(n, 16, 5)
     arr = capi.gates_from(allg)
     e64 = capi.Engine(n)
     e64.init_state()
     e64.run_gates(arr)
     f32 = capi.F32Engine(n)
+    f32.passes = passes
     f32.init_state()
     f32.run_gates(arr, 5)
     rng = wl.Pcg32(99)
@@ -109,6 +114,8 @@ def test_fused_rejects_bad_arguments(capi):
         f32.run_gates(arr, 3)
     with pytest.raises(capi.SuqaError):
         f32.measure(10, 0.5)
+    with pytest.raises(capi.SuqaError):
+        capi.check32(capi.lib().suqa_hip_f32_set_passes(f32.ctx, 2))   # modes are 0 and 1
     e64 = capi.Engine(11)
     with pytest.raises(capi.SuqaError):
         f32.fidelity(e64)
@@ -118,8 +125,9 @@ def test_fused_rejects_bad_arguments(capi):
 
 def test_fused_full_size_fidelity(capi):
     """30 qubits (FP64 16 GiB + FP32 8 GiB): the fused FP32 run of 10 layers of
-    the 30-qubit circuit keeps fidelity >= 1 - 1e-5 with the FP64 run; the
-    autotuner picks a width whose timing it reports."""
+    the 30-qubit circuit keeps fidelity >= 1 - 1e-5 with the FP64 run in
+    both modes; the autotuner picks a width and a mode whose timings it
+    reports."""
     n = 30
     log("fused full size n=30")
     gates, _, _ = wl.random_circuit(n, 10, 7)
@@ -133,14 +141,19 @@ def test_fused_full_size_fidelity(capi):
     f32 = capi.F32Engine(n)
     f32.init_state()
     had = capi.gates_from([wl.G(wl.H, q, [], 0.0) for q in range(n)])
-    f32.run_gates(had, 6)
-    f32.run_gates(arr, 5)
-    F = f32.fidelity(e64)
-    log(f"fidelity {F!r}")
-    assert F > 1 - 1e-5
-    assert abs(f32.vnorm() - 1.0) < 1e-5
+    for passes in (False, True):
+        f32.passes = passes
+        f32.init_state()
+        f32.run_gates(had, 6)
+        f32.run_gates(arr, 5)
+        F = f32.fidelity(e64)
+        log(f"passes={passes} fidelity {F!r}")
+        assert F > 1 - 1e-5
+        assert abs(f32.vnorm() - 1.0) < 1e-5
     best, t = f32.autotune(arr)
-    log(f"autotune {best} {t}")
-    assert best in (4, 5, 6) and all(v > 0 for v in t.values())
+    log(f"autotune {best} passes={f32.passes} {t}")
+    assert best in (4, 5, 6) and len(t) == 6 and all(v > 0 for v in t.values())
+    mode = "passes" if f32.passes else "blocks"
+    assert t[f"{mode}{best}"] == min(t.values())
     f32.close()
     e64.close()

@@ -53,11 +53,13 @@ def test_qpe_fp64_vs_oracle(capi, oracle_result):
     eng.close()
 
 
+@pytest.mark.parametrize("passes", [False, True], ids=["blocks", "passes"])
 @pytest.mark.parametrize("width", [5, 6])
-def test_qpe_fp32_fused_vs_oracle(capi, oracle_result, width):
-    log(f"qpe fp32 10+4 width {width}")
+def test_qpe_fp32_fused_vs_oracle(capi, oracle_result, width, passes):
+    log(f"qpe fp32 10+4 width {width} passes={passes}")
     hist = oracle_result[0]
     f32 = capi.F32Engine(NSYS + NREG)
+    f32.passes = passes
     f32.init_state()
     got = wl.qpe_trotter(f32, NSYS, NREG, wl.trotter_terms(NSYS, 96, 11), DT, STEPS, width=width)
     assert np.max(np.abs(np.array(got) - hist)) < 2e-5
@@ -76,12 +78,13 @@ def test_fp32_rejects_ops_wider_than_width(capi):
     f32.close()
 
 
-def test_controlled_trotter_step_full_size(capi):
+@pytest.mark.parametrize("passes", [False, True], ids=["blocks", "passes"])
+def test_controlled_trotter_step_full_size(capi, passes):
     """24 + 6 qubits: H on register qubit 29, one Trotter step of the 96
     terms controlled on it, FP32 fused vs FP64 (fidelity >= 1 - 1e-6), and
     the controlled half untouched where the control is 0."""
     n, nsys = 30, 24
-    log("controlled trotter step n=30")
+    log(f"controlled trotter step n=30 passes={passes}")
     terms = wl.trotter_terms(nsys, 96, 11)
     ops = capi.pauli_ops(wl.trotter_step_ops(terms, DT, 1 << 29))
     e64 = capi.Engine(n)
@@ -89,6 +92,7 @@ def test_controlled_trotter_step_full_size(capi):
     e64.h(29)
     e64.run_paulis(ops)
     f32 = capi.F32Engine(n)
+    f32.passes = passes
     f32.init_state()
     f32.run_gates(capi.gates_from([wl.G(wl.H, 29, [], 0.0)]), 5)
     nb = f32.run_paulis(ops, 6)
