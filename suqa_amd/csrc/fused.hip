@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "deposit.hpp"
 #include "fused.hpp"
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -88,34 +89,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 // block by MFMA and writes the tile back; the blocks come from fusing the whole
 // op list (fuse.cpp).
 struct BlockArgs {
-    uint32_t tpos[kMaxK + kTileLow];   // tile bit positions, ascending (compact order)
     uint32_t ntile;                    // K + L
     uint32_t L;                        // non-block bits per tile (columns = 2^L)
     uint32_t rowoff[1 << kMaxK];       // compact offset of block index r
     uint32_t coloff[1 << kTileLow];    // compact offset of column c
-    uint32_t nfix;                     // tile bits and control bits, ascending
-    uint8_t fixpos[64];
+    uint64_t tmask;                    // the tile's state bits (compact order = ascending)
+    uint64_t freemask;                 // state bits enumerated by the tile number
     uint64_t fixval;                   // the controls (all 1)
     uint64_t ntiles;
     const float* A;                    // [RB][D/2][64] MFMA-ordered stacked matrix
 };
-
-// Tile t -> its base index: zeros inserted at the tile and control bits
-// (ascending), controls set.  A controlled block only visits tiles where every
-// control is 1: the others are left untouched.
-__device__ __forceinline__ uint64_t tile_base(uint64_t t, const BlockArgs& a) {
-    for (uint32_t j = 0; j < a.nfix; ++j) {
-        const uint64_t lo = (1ull << a.fixpos[j]) - 1ull;
-        t = (t & lo) | ((t & ~lo) << 1);
-    }
-    return t | a.fixval;
-}
-__device__ __forceinline__ uint64_t pdep_tile(uint32_t e, const BlockArgs& a) {
-    uint64_t o = 0;
-    for (uint32_t j = 0; j < a.ntile; ++j)
-        if ((e >> j) & 1u) o |= 1ull << a.tpos[j];
-    return o;
-}
 
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_block(float* __restrict__ re, float* __restrict__ im, const BlockArgs a) {
@@ -128,16 +111,38 @@ __global__ __launch_bounds__(kBlock) void k_block(float* __restrict__ re, float*
     float* lim = lds + tsz;
     const uint64_t tile = ((uint64_t)(blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3));
     if (tile >= a.ntiles) return;      // whole workgroup leaves together
-    const uint64_t base = tile_base(tile, a);
+    // A controlled block only visits tiles where every control is 1: the
+    // others are left untouched.
+    const uint64_t base = deposit(tile, a.freemask) | a.fixval;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
 
-    // 1. tile -> LDS (compact order), 16-byte chunks
-    for (uint32_t ch = tid; ch < tsz / 4; ch += kBlock) {
-        const uint64_t g = base | pdep_tile(ch * 4, a);
-        const floatx4 vr = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(re + g));
-        const floatx4 vi = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(im + g));
-        *reinterpret_cast<floatx4*>(lre + ch * 4) = vr;
-        *reinterpret_cast<floatx4*>(lim + ch * 4) = vi;
+    // 1. tile -> LDS (compact order), 16-byte chunks: chunk i of a thread is
+    //    element 4 tid + 1024 i; every load is issued before the LDS writes
+    constexpr uint32_t kCh = (1u << (K + kTileLow)) / 4 / kBlock;   // chunks per thread at full size
+    const uint64_t toff = base | deposit(4u * tid, a.tmask);
+    uint64_t ibit[3];   // state offsets of tile bits 10, 11, 12
+#pragma unroll
+    for (uint32_t j = 0; j < 3; ++j) ibit[j] = deposit(4u * kBlock << j, a.tmask);
+    auto chunk_off = [&](uint32_t i) {
+        return ((i & 1u) ? ibit[0] : 0ull) | ((i & 2u) ? ibit[1] : 0ull) | ((i & 4u) ? ibit[2] : 0ull);
+    };
+    constexpr uint32_t kBatch = kCh < 4 ? kCh : 4;   // loads in flight per thread
+#pragma unroll
+    for (uint32_t i0 = 0; i0 < kCh; i0 += kBatch) {
+        floatx4 vr[kBatch], vi[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j)
+            if (tid + (i0 + j) * kBlock < tsz / 4) {
+                const uint64_t g = toff | chunk_off(i0 + j);
+                vr[j] = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(re + g));
+                vi[j] = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(im + g));
+            }
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j)
+            if (tid + (i0 + j) * kBlock < tsz / 4) {
+                *reinterpret_cast<floatx4*>(lre + 4 * (tid + (i0 + j) * kBlock)) = vr[j];
+                *reinterpret_cast<floatx4*>(lim + 4 * (tid + (i0 + j) * kBlock)) = vi[j];
+            }
     }
     __syncthreads();
 
@@ -179,11 +184,14 @@ __global__ __launch_bounds__(kBlock) void k_block(float* __restrict__ re, float*
     __syncthreads();
 
     // 3. LDS -> tile
-    for (uint32_t ch = tid; ch < tsz / 4; ch += kBlock) {
-        const uint64_t g = base | pdep_tile(ch * 4, a);
-        __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lre + ch * 4), reinterpret_cast<floatx4*>(re + g));
-        __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lim + ch * 4), reinterpret_cast<floatx4*>(im + g));
-    }
+#pragma unroll
+    for (uint32_t i = 0; i < kCh; ++i)
+        if (tid + i * kBlock < tsz / 4) {
+            const uint64_t g = toff | chunk_off(i);
+            const uint32_t e = 4 * (tid + i * kBlock);
+            __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lre + e), reinterpret_cast<floatx4*>(re + g));
+            __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lim + e), reinterpret_cast<floatx4*>(im + g));
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -197,28 +205,55 @@ constexpr uint32_t kPassTileBits = 12;   // 32 KiB of FP32 complex amplitudes pe
 constexpr uint32_t kPassBlocks = 16;     // blocks per launch
 constexpr uint32_t kItems = 2;           // (row block, column block) pairs per wave at t = 12
 
+// LDS layout of a tile: amplitude e sits at lds_sw(e) = e ^ h(e), h the XOR of
+// kSwVec[p] over the set bits p >= 5 of e.  h < 32 depends only on bits the
+// XOR leaves alone, so lds_sw is a bijection of [0, 2^t) that keeps aligned
+// groups of 4 together, and it is linear over GF(2).  Tile position p drives
+// the LDS banks (address mod 32) by the vector kSwVec[p]: 32 lanes that span 5
+// positions of independent vectors hit 32 distinct banks.  The vectors of
+// positions 5..11 come from a search over every 4-, 5- and 6-position block of
+// a 12-position tile: the 5 free positions picked by rank (see add_launch)
+// always span a space of rank >= 4, at most a 2-way conflict (1.06 on average).
+__host__ __device__ constexpr uint32_t kSwVecAt(uint32_t p) {
+    constexpr uint32_t v[12] = {1, 2, 4, 8, 16, 26, 7, 22, 21, 3, 28, 13};
+    return v[p];
+}
+__host__ __device__ inline uint32_t lds_sw(uint32_t e) {
+    uint32_t h = 0;
+#pragma unroll
+    for (uint32_t p = 5; p < kPassTileBits; ++p)
+        if ((e >> p) & 1u) h ^= kSwVecAt(p);
+    return e ^ h;
+}
+
 struct PassBlock {
     uint32_t K;
-    uint8_t bpos[kMaxK];   // tile positions of the block qubits, ascending
-    uint8_t pad_[2];
+    uint32_t rowsw[kMaxK];                    // lds_sw(1 << p) of the block positions, ascending
+    uint32_t colsw[kPassTileBits - kMinK];    // lds_sw(1 << p) of column bit j (bits 0..4: the lanes)
+    uint32_t pad_;
     uint64_t mo, vo;       // controls (outside the tile): applied where (base & mo) == vo
     const float* A;        // MFMA-ordered stacked matrix
 };
 
 struct PassArgs {
-    uint32_t t, nfix, nblocks, pad_;
-    uint8_t tpos[16];      // tile bit positions, ascending
-    uint8_t fixpos[64];    // tile bits and common controls, ascending
-    uint64_t fixval;
+    uint32_t t, nblocks;
+    uint64_t tmask;        // the tile's state bits (tile element e -> deposit(e, tmask))
+    uint64_t freemask;     // state bits enumerated by the tile number
+    uint64_t fixval;       // controls common to every block of the launch
     uint64_t ntiles;
     PassBlock b[kPassBlocks];
 };
 
-__device__ __forceinline__ uint64_t pass_offset(uint32_t e, const PassArgs& a) {
-    uint64_t o = 0;
-    for (uint32_t j = 0; j < a.t; ++j)
-        if ((e >> j) & 1u) o |= 1ull << a.tpos[j];
-    return o;
+// v[k ^ p] in slot k (an involution: also its own inverse).
+__device__ __forceinline__ floatx4 perm4(floatx4 v, uint32_t p) {
+    const bool s1 = p & 1u, s2 = p & 2u;
+    const float a0 = s1 ? v.y : v.x, a1 = s1 ? v.x : v.y, a2 = s1 ? v.w : v.z, a3 = s1 ? v.z : v.w;
+    floatx4 r;
+    r.x = s2 ? a2 : a0;
+    r.y = s2 ? a3 : a1;
+    r.z = s2 ? a0 : a2;
+    r.w = s2 ? a1 : a3;
+    return r;
 }
 
 template <int K>
@@ -227,58 +262,68 @@ __device__ __forceinline__ void pass_block(float* lre, float* lim, uint32_t t, c
     constexpr int RB = (2 * Dc) / 32;   // 32-row blocks of the stacked matrix (1, 2, 4)
     constexpr int KS = Dc;              // k-steps of 2 rows
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t ncb = (1u << (t - K)) / 32;   // 32-column blocks
-    const uint32_t items = RB * ncb;
-    // tile offset of block row r: its bits deposited at the block positions (uniform)
-    auto rowo = [&](uint32_t r) {
+    const uint32_t nc = t - K;                   // column bits (>= 5)
+    const uint32_t items = (uint32_t)RB << (nc - 5);
+    // swizzled tile offset of block row r: an XOR of the row bits' swizzles (uniform)
+    auto rows = [&](uint32_t r) {
         uint32_t o = 0;
 #pragma unroll
         for (int q = 0; q < K; ++q)
-            if ((r >> q) & 1u) o |= 1u << B.bpos[q];
+            if ((r >> q) & 1u) o ^= B.rowsw[q];
         return o;
     };
     const int rb = w % RB;              // 4 % RB == 0: every item of a wave has this row block
     float areg[KS];
 #pragma unroll
     for (int kk = 0; kk < KS; ++kk) areg[kk] = B.A[((size_t)rb * KS + kk) * 64 + lane];
-    floatx16 acc[kItems];
-    uint32_t coff[kItems];
+    uint32_t lanec = 0;                 // lane bits 0..4 -> column bits 0..4
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        if ((lane >> j) & 1) lanec ^= B.colsw[j];
+    const uint32_t rd_half = (lane >> 5) ? B.rowsw[0] : 0u;   // input rows 2kk, 2kk+1: block bit 0
+    const uint32_t wr_half = (lane >> 5) ? B.rowsw[2] : 0u;   // output rows i, i+4: block bit 2
+    uint32_t cs[kItems];
+    bool live[kItems];
 #pragma unroll
     for (uint32_t it = 0; it < kItems; ++it) {
-        acc[it] = floatx16{0};
-        coff[it] = 0;
         const uint32_t item = (uint32_t)w + 4u * it;
-        if (item >= items) continue;
-        uint32_t col = (item / RB) * 32 + (uint32_t)(lane & 31);   // column index -> tile offset
+        live[it] = item < items;
+        // a wave without this item reads a valid column of its row block and stores nothing
+        const uint32_t cb = live[it] ? item / RB : 0u;
+        uint32_t c = lanec;
+        for (uint32_t j = 5; j < nc; ++j)
+            if ((cb >> (j - 5)) & 1u) c ^= B.colsw[j];
+        cs[it] = c;
+    }
+    floatx16 acc[kItems];
 #pragma unroll
-        for (int q = 0; q < K; ++q) {
-            const uint32_t lo = (1u << B.bpos[q]) - 1u;
-            col = (col & lo) | ((col & ~lo) << 1);
-        }
-        coff[it] = col;
+    for (uint32_t it = 0; it < kItems; ++it) acc[it] = floatx16{0};
 #pragma unroll
-        for (int kk = 0; kk < KS; ++kk) {
-            // rows 2kk, 2kk+1 of the stacked matrix: both real (2kk < Dc) or both imaginary;
-            // their offsets are uniform, the lane half picks one
-            const int R0 = (2 * kk) % Dc;   // even: row R0 + 1 adds block bit 0
-            const uint32_t o = rowo((uint32_t)R0) | ((uint32_t)(lane >> 5) << B.bpos[0]);
-            const float b = (2 * kk < Dc) ? lre[o + col] : lim[o + col];
-            acc[it] = __builtin_amdgcn_mfma_f32_32x32x2f32(areg[kk], b, acc[it], 0, 0, 0);
-        }
+    for (int kk = 0; kk < KS; ++kk) {
+        // rows 2kk, 2kk+1 of the stacked matrix: both real (2kk < Dc) or both imaginary;
+        // R0 is even, row R0 + 1 adds block bit 0 (the lane half)
+        const uint32_t ro = rows((uint32_t)((2 * kk) % Dc)) ^ rd_half;
+        float* src = (2 * kk < Dc) ? lre : lim;
+        float b[kItems];
+#pragma unroll
+        for (uint32_t it = 0; it < kItems; ++it) b[it] = src[ro ^ cs[it]];
+#pragma unroll
+        for (uint32_t it = 0; it < kItems; ++it)
+            acc[it] = __builtin_amdgcn_mfma_f32_32x32x2f32(areg[kk], b[it], acc[it], 0, 0, 0);
     }
     __syncthreads();   // every read of the block's input is done
 #pragma unroll
     for (uint32_t it = 0; it < kItems; ++it) {
-        if ((uint32_t)w + 4u * it >= items) continue;
+        if (!live[it]) continue;   // uniform
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             // output row rb*32 + 8(v/4) + 4(lane/32) + v%4: uniform up to the lane half
             const int i0 = rb * 32 + 8 * (v >> 2) + (v & 3);
             const bool imag = i0 >= Dc;   // rows i0 and i0 + 4 lie in the same half (Dc >= 16)
             const int r0 = imag ? i0 - Dc : i0;   // r0 % 8 < 4: row r0 + 4 adds block bit 2
-            const uint32_t o = rowo((uint32_t)r0) | ((uint32_t)(lane >> 5) << B.bpos[2]);
-            if (imag) lim[o + coff[it]] = acc[it][v];
-            else lre[o + coff[it]] = acc[it][v];
+            const uint32_t o = rows((uint32_t)r0) ^ wr_half ^ cs[it];
+            if (imag) lim[o] = acc[it][v];
+            else lre[o] = acc[it][v];
         }
     }
     __syncthreads();
@@ -293,31 +338,46 @@ __global__ __launch_bounds__(kBlock) void k_bpass(float* __restrict__ re, float*
     float* lim = lds + tsz;
     const uint64_t tile = ((uint64_t)(blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3));
     if (tile >= a.ntiles) return;      // whole workgroup leaves together
-    uint64_t base = tile;
-    for (uint32_t j = 0; j < a.nfix; ++j) {
-        const uint64_t lo = (1ull << a.fixpos[j]) - 1ull;
-        base = (base & lo) | ((base & ~lo) << 1);
-    }
-    base |= a.fixval;
-    const int tid = threadIdx.x;
-    for (uint32_t ch = tid; ch < tsz / 4; ch += kBlock) {   // tile bits 0, 1 are qubits 0, 1
-        const uint64_t g = base | pass_offset(ch * 4, a);
-        const floatx4 vr = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(re + g));
-        const floatx4 vi = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(im + g));
-        *reinterpret_cast<floatx4*>(lre + ch * 4) = vr;
-        *reinterpret_cast<floatx4*>(lim + ch * 4) = vi;
-    }
+    const uint64_t base = deposit(tile, a.freemask) | a.fixval;
+    const uint32_t tid = threadIdx.x;
+    // tile bits 0, 1 are qubits 0, 1: 4 consecutive amplitudes, one aligned group
+    // in LDS.  Chunk i of a thread is element 4 tid + 1024 i; every load is
+    // issued before the LDS writes.
+    constexpr uint32_t kCh = (1u << kPassTileBits) / 4 / kBlock;
+    const uint64_t toff = base | deposit(4u * tid, a.tmask);
+    const uint64_t ibit0 = deposit(4u * kBlock, a.tmask), ibit1 = deposit(8u * kBlock, a.tmask);
+    auto chunk_off = [&](uint32_t i) { return ((i & 1u) ? ibit0 : 0ull) | ((i & 2u) ? ibit1 : 0ull); };
+    floatx4 vr[kCh], vi[kCh];
+#pragma unroll
+    for (uint32_t i = 0; i < kCh; ++i)
+        if (tid + i * kBlock < tsz / 4) {
+            const uint64_t g = toff | chunk_off(i);
+            vr[i] = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(re + g));
+            vi[i] = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(im + g));
+        }
+#pragma unroll
+    for (uint32_t i = 0; i < kCh; ++i)
+        if (tid + i * kBlock < tsz / 4) {
+            const uint32_t sw = lds_sw(4 * (tid + i * kBlock));
+            *reinterpret_cast<floatx4*>(lre + (sw & ~3u)) = perm4(vr[i], sw & 3u);
+            *reinterpret_cast<floatx4*>(lim + (sw & ~3u)) = perm4(vi[i], sw & 3u);
+        }
     __syncthreads();
     for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
         const PassBlock& B = a.b[bi];
         if ((base & B.mo) != B.vo) continue;   // uniform: a control of this tile is 0
         pass_block<K>(lre, lim, a.t, B);
     }
-    for (uint32_t ch = tid; ch < tsz / 4; ch += kBlock) {
-        const uint64_t g = base | pass_offset(ch * 4, a);
-        __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lre + ch * 4), reinterpret_cast<floatx4*>(re + g));
-        __builtin_nontemporal_store(*reinterpret_cast<const floatx4*>(lim + ch * 4), reinterpret_cast<floatx4*>(im + g));
-    }
+#pragma unroll
+    for (uint32_t i = 0; i < kCh; ++i)
+        if (tid + i * kBlock < tsz / 4) {
+            const uint64_t g = toff | chunk_off(i);
+            const uint32_t sw = lds_sw(4 * (tid + i * kBlock));
+            __builtin_nontemporal_store(perm4(*reinterpret_cast<const floatx4*>(lre + (sw & ~3u)), sw & 3u),
+                                        reinterpret_cast<floatx4*>(re + g));
+            __builtin_nontemporal_store(perm4(*reinterpret_cast<const floatx4*>(lim + (sw & ~3u)), sw & 3u),
+                                        reinterpret_cast<floatx4*>(im + g));
+        }
 }
 
 __global__ __launch_bounds__(kBlock) void k_init32(float* __restrict__ d, uint64_t n4) {
@@ -503,13 +563,10 @@ void block_args(const Block& b, uint32_t nq, BlockArgs& a) {
     a.L = (uint32_t)low.size();
     uint64_t tm = bm;
     for (uint32_t q : low) tm |= 1ull << q;
-    a.ntile = 0;
-    for (uint32_t q = 0; q < nq; ++q)
-        if ((tm >> q) & 1ull) a.tpos[a.ntile++] = q;
-    auto cpos = [&](uint32_t q) {
-        for (uint32_t j = 0; j < a.ntile; ++j)
-            if (a.tpos[j] == q) return j;
-        return 0u;
+    a.tmask = tm;
+    a.ntile = (uint32_t)__builtin_popcountll(tm);
+    auto cpos = [&](uint32_t q) {   // compact position of qubit q: tile bits below it
+        return (uint32_t)__builtin_popcountll(tm & ((1ull << q) - 1ull));
     };
     for (uint32_t r = 0; r < (1u << b.K); ++r) {
         uint32_t o = 0;
@@ -523,12 +580,9 @@ void block_args(const Block& b, uint32_t nq, BlockArgs& a) {
             if ((cidx >> j) & 1u) o |= 1u << cpos(low[j]);
         a.coloff[cidx] = o;
     }
-    const uint64_t fix = tm | b.cmask;
-    a.nfix = 0;
-    for (uint32_t q = 0; q < nq; ++q)
-        if ((fix >> q) & 1ull) a.fixpos[a.nfix++] = (uint8_t)q;
+    a.freemask = ((1ull << nq) - 1ull) & ~(tm | b.cmask);
     a.fixval = b.cmask;
-    a.ntiles = (1ull << nq) >> a.nfix;
+    a.ntiles = 1ull << __builtin_popcountll(a.freemask);
 }
 
 // Uploads the MFMA-ordered matrices of blocks [lo, hi) and launches them.
@@ -627,20 +681,37 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
     std::memset(&l, 0, sizeof(l));
     PassArgs& a = l.a;
     a.t = t;
+    a.tmask = T;
     uint32_t k = 0;
-    uint32_t pos[64] = {};
+    uint32_t pos[64] = {};   // tile position of each tile qubit
     for (uint32_t q = 0; q < nq; ++q)
-        if ((T >> q) & 1ull) {
-            pos[q] = k;
-            a.tpos[k++] = (uint8_t)q;
-        }
+        if ((T >> q) & 1ull) pos[q] = k++;
     uint64_t mcom = ~0ull;
     a.nblocks = (uint32_t)(hi - lo);
     for (size_t i = lo; i < hi; ++i) {
         const Block& b = blocks[i];
         PassBlock& B = a.b[i - lo];
         B.K = b.K;
-        for (uint32_t j = 0; j < b.K; ++j) B.bpos[j] = (uint8_t)pos[b.qubits[j]];
+        uint32_t inblock = 0;
+        for (uint32_t j = 0; j < b.K; ++j) {
+            B.rowsw[j] = lds_sw(1u << pos[b.qubits[j]]);
+            inblock |= 1u << pos[b.qubits[j]];
+        }
+        // column bits 0..4 (the lanes): non-block positions whose bank vectors
+        // raise the GF(2) rank, lowest first; then the other positions ascending
+        uint32_t nc = 0, taken = 0, basis[5] = {}, rank = 0;
+        for (uint32_t p = 0; p < t && nc < 5; ++p) {
+            if ((inblock >> p) & 1u) continue;
+            uint32_t v = kSwVecAt(p);
+            for (uint32_t r = 0; r < rank; ++r) v = std::min(v, v ^ basis[r]);
+            if (!v) continue;
+            basis[rank++] = v;
+            std::sort(basis, basis + rank, [](uint32_t x, uint32_t y) { return x > y; });
+            taken |= 1u << p;
+            B.colsw[nc++] = lds_sw(1u << p);
+        }
+        for (uint32_t p = 0; p < t; ++p)
+            if (!(((inblock | taken) >> p) & 1u)) B.colsw[nc++] = lds_sw(1u << p);
         B.mo = B.vo = b.cmask;
         mcom &= b.cmask;
         // MFMA-ordered stacked matrix, then the tile offsets of the block rows
@@ -650,11 +721,9 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
         mfma_order(b, buf.data() + l.off[i - lo]);
     }
     // controls shared by every block: tiles where one of them is 0 are not visited
-    const uint64_t fix = T | mcom;
-    for (uint32_t q = 0; q < nq; ++q)
-        if ((fix >> q) & 1ull) a.fixpos[a.nfix++] = (uint8_t)q;
+    a.freemask = ((1ull << nq) - 1ull) & ~(T | mcom);
     a.fixval = mcom;
-    a.ntiles = (1ull << nq) >> a.nfix;
+    a.ntiles = 1ull << __builtin_popcountll(a.freemask);
     L.push_back(l);
     return SUQA_OK;
 }

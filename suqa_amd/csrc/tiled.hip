@@ -32,6 +32,7 @@
 #include <cstring>
 #include <utility>
 
+#include "deposit.hpp"
 #include "engine.hpp"
 #include "sweep_math.hpp"
 
@@ -71,25 +72,18 @@ struct TileArgs {
     uint64_t ntiles;
     uint64_t fixval;      // common selection values outside the tile
     uint32_t t;           // tile bits
-    uint32_t nfix;        // positions fixed in the tile enumeration (tile bits + common selection)
     uint32_t ngates;
     uint32_t nsteps;
-    uint8_t fixpos[64];   // ascending
-    uint8_t tpos[16];     // tile bit positions, ascending (tile index order)
+    uint64_t tmask;       // the tile's state bits (tile element e -> deposit(e, tmask))
+    uint64_t freemask;    // state bits enumerated by the tile number (not in the tile,
+                          // not fixed by a selection common to every gate)
     struct Step {
-        uint8_t f[kSub];  // tile positions of the step's bits, ascending; f[0] = 0xff: LDS step
-        uint8_t g0, g1;   // gates [g0, g1)
+        uint32_t f[kSub]; // tile positions of the step's bits, ascending; f[0] = kLdsStep: LDS step
+        uint32_t g0, g1;  // gates [g0, g1)
     } step[kMaxPass];
     TileGate g[kMaxPass];
 };
-
-// Global offset of tile element e (tile positions -> state bits).
-__device__ __forceinline__ uint64_t tile_offset(uint32_t e, const TileArgs& a) {
-    uint64_t o = 0;
-    for (uint32_t j = 0; j < a.t; ++j)
-        if ((e >> j) & 1u) o |= 1ull << a.tpos[j];
-    return o;
-}
+constexpr uint32_t kLdsStep = 0xffu;
 
 // LDS step: pair gate over the whole tile, pair (u, u ^ X) per thread.
 template <int OP>
@@ -270,18 +264,20 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
     const uint64_t G = gridDim.x, b = blockIdx.x;
     const uint64_t tile = (b & 7ull) * (G >> 3) + (b >> 3);   // XCD-contiguous (grid % 8 == 0)
     if (tile >= a.ntiles) return;                             // whole workgroup leaves together
-    uint64_t base = tile;
-    for (uint32_t j = 0; j < a.nfix; ++j) {
-        const uint64_t lo = (1ull << a.fixpos[j]) - 1ull;
-        base = (base & lo) | ((base & ~lo) << 1);
-    }
-    base |= a.fixval;
+    const uint64_t base = deposit(tile, a.freemask) | a.fixval;
 
     // 1. tile -> LDS: chunk k of a thread is tile element e = 2 tid + k 2 kBlock;
     //    every load of the thread is issued before the LDS writes
     const uint32_t e0 = 2 * threadIdx.x;
     const uint32_t nch = tsz / (2 * kBlock);   // 0 when the tile is smaller than 2 kBlock
-    const uint64_t toff = e0 < tsz ? (base | tile_offset(e0, a)) : 0ull;
+    const uint64_t toff = e0 < tsz ? (base | deposit(e0, a.tmask)) : 0ull;
+    // state offsets of tile bits 9, 10, 11 (chunk k adds those of k's bits)
+    uint64_t kbit[3];
+#pragma unroll
+    for (uint32_t j = 0; j < 3; ++j) kbit[j] = deposit(2u * kBlock << j, a.tmask);
+    auto chunk_off = [&](uint32_t k) {
+        return ((k & 1u) ? kbit[0] : 0ull) | ((k & 2u) ? kbit[1] : 0ull) | ((k & 4u) ? kbit[2] : 0ull);
+    };
     if (nch == 0) {
         if (e0 < tsz) {
             const dv2t vr = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(re + toff));
@@ -294,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
 #pragma unroll
         for (uint32_t k = 0; k < kMaxChunks; ++k) {
             if (k < nch) {
-                const uint64_t g = toff | tile_offset(k * 2 * kBlock, a);
+                const uint64_t g = toff | chunk_off(k);
                 vr[k] = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(re + g));
                 vi[k] = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(im + g));
             }
@@ -312,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
     // 2. the steps, in order
     for (uint32_t st = 0; st < a.nsteps; ++st) {
         const auto& S = a.step[st];
-        if (S.f[0] == 0xffu) {   // one gate flipping more than kSub bits: pairs in LDS
+        if (S.f[0] == kLdsStep) {   // one gate flipping more than kSub bits: pairs in LDS
             const TileGate& g = a.g[S.g0];
             if ((base & g.mo) == g.vo) {
                 const uint32_t opar = (uint32_t)__popcll(base & g.yzo) & 1u;
@@ -375,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
 #pragma unroll
         for (uint32_t k = 0; k < kMaxChunks; ++k) {
             if (k < nch) {
-                const uint64_t g = toff | tile_offset(k * 2 * kBlock, a);
+                const uint64_t g = toff | chunk_off(k);
                 __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lre + e0 + k * 2 * kBlock),
                                             reinterpret_cast<dv2t*>(re + g));
                 __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lim + e0 + k * 2 * kBlock),
@@ -401,7 +397,7 @@ hipError_t run_single(suqa_hip_ctx* c, const GateDesc& d) {
 }
 
 // Local tables of gate g for a register step on tile positions f[0..2].
-void local_tables(TileGate& g, const uint8_t* f) {
+void local_tables(TileGate& g, const uint32_t* f) {
     uint32_t F = 0, mcl = 0, vcl = 0, yzl = 0, xl = 0;
     for (uint32_t m = 0; m < kSub; ++m) {
         F |= 1u << f[m];
@@ -439,27 +435,29 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     TileArgs a;
     std::memset(&a, 0, sizeof(a));
     a.t = t;
+    a.tmask = T;
+    uint8_t tpos[64];   // tile bit positions, ascending (tile index order)
     uint32_t k = 0;
     for (uint32_t b = 0; b < nq; ++b)
-        if ((T >> b) & 1ull) a.tpos[k++] = (uint8_t)b;
+        if ((T >> b) & 1ull) tpos[k++] = (uint8_t)b;
     a.ngates = n;
     uint64_t mcom = ~0ull, vcom = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const GateDesc& d = q[i];
         TileGate& g = a.g[i];
         g.kind = (uint32_t)d.kind;
-        g.mc = compact(d.M, a.tpos, t);
-        g.vc = compact(d.V, a.tpos, t);
+        g.mc = compact(d.M, tpos, t);
+        g.vc = compact(d.V, tpos, t);
         g.mo = d.M & ~T;
         g.vo = d.V & ~T;
         const uint64_t par = d.kind == kPhaseOp ? d.zmask : d.pc.yz;
-        g.yzc = compact(par, a.tpos, t);
+        g.yzc = compact(par, tpos, t);
         g.yzo = par & ~T;
         if (d.kind == kPhaseOp) {
             g.c = d.c;
             g.s = d.s;
         } else {
-            g.xc = compact(d.X, a.tpos, t);
+            g.xc = compact(d.X, tpos, t);
             g.pivot = (uint32_t)__builtin_ctz(g.xc);
             g.c = d.pc.c;
             g.s = d.pc.s;
@@ -477,10 +475,10 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     uint32_t F = 0, g0 = 0;
     auto close = [&](uint32_t g1, bool lds) {
         auto& S = a.step[a.nsteps++];
-        S.g0 = (uint8_t)g0;
-        S.g1 = (uint8_t)g1;
+        S.g0 = g0;
+        S.g1 = g1;
         if (lds) {
-            S.f[0] = S.f[1] = S.f[2] = 0xffu;
+            S.f[0] = S.f[1] = S.f[2] = kLdsStep;
             return;
         }
         // pad with tile bits that the step's gates select on, then high tile bits
@@ -491,7 +489,7 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
         for (int p = (int)t - 1; p >= 0 && __builtin_popcount(F) < (int)kSub; --p) F |= 1u << p;
         uint32_t qn = 0;
         for (uint32_t p = 0; p < t; ++p)
-            if ((F >> p) & 1u) S.f[qn++] = (uint8_t)p;
+            if ((F >> p) & 1u) S.f[qn++] = p;
         for (uint32_t k2 = g0; k2 < g1; ++k2) local_tables(a.g[k2], S.f);
     };
     for (uint32_t i = 0; i < n; ++i) {
@@ -515,11 +513,9 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     }
     if (n > g0) close(n, false);
     const uint64_t fix = T | mcom;
-    a.nfix = 0;
-    for (uint32_t b = 0; b < nq; ++b)
-        if ((fix >> b) & 1ull) a.fixpos[a.nfix++] = (uint8_t)b;
+    a.freemask = ((nq == 64 ? 0ull : (1ull << nq)) - 1ull) & ~fix;
     a.fixval = vcom;
-    a.ntiles = 1ull << (nq - a.nfix);
+    a.ntiles = 1ull << __builtin_popcountll(a.freemask);
     const uint64_t g8 = ((a.ntiles + 7) / 8) * 8;
     const size_t lds = sizeof(double) * 2 * ((size_t)1 << t);
     ++c->passes;
