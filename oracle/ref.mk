@@ -13,7 +13,7 @@ CXXFLAGS := -std=c++11 -O3 -DNDEBUG -w -I$(REF)/include
 
 OBJS := $(OUT)/suqa.o $(OUT)/system.o $(OUT)/Rand.o $(OUT)/io.o
 
-all: $(OUT)/qms $(OUT)/evo $(OUT)/ref_harness
+all: $(OUT)/qms $(OUT)/evo $(OUT)/ref_harness $(OUT)/qms_gatecount
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -32,5 +32,20 @@ $(OUT)/evo: $(OUT)/evo.o $(OBJS)
 
 $(OUT)/ref_harness: oracle/ref_harness.cpp $(OBJS)
 	$(CXX) $(CXXFLAGS) oracle/ref_harness.cpp $(OBJS) -o $@
+
+# the same qms driver with the gate-count instrumentation compiled in (-DGATECOUNT)
+GC_OBJS := $(OUT)/gc/qms.o $(OUT)/gc/suqa.o $(OUT)/gc/system.o $(OUT)/gc/Rand.o $(OUT)/gc/io.o
+
+$(OUT)/gc:
+	mkdir -p $(OUT)/gc
+
+$(OUT)/gc/%.o: $(REF)/src/%.cu | $(OUT)/gc
+	$(CXX) $(CXXFLAGS) -DGATECOUNT -x c++ -c $< -o $@
+
+$(OUT)/gc/%.o: $(REF)/src/%.cpp | $(OUT)/gc
+	$(CXX) $(CXXFLAGS) -DGATECOUNT -c $< -o $@
+
+$(OUT)/qms_gatecount: $(GC_OBJS)
+	$(CXX) $^ -o $@
 
 .PHONY: all

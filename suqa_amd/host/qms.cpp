@@ -17,6 +17,8 @@ int n_phase_estimation;
 uint gCi, c_acc = 0;
 bool record_reverse = false;
 std::vector<uint> reverse_counters;
+GateCounter gctr_global("global"), gctr_metrostep("metro step"), gctr_sample("sample"), gctr_measure("measure"),
+    gctr_reverse("reverse");
 pcg rangen(1ULL);
 std::vector<double> X_measures, E_measures;
 double c_factor;
@@ -104,6 +106,13 @@ uint draw_C() {
 void apply_W() {
     int rc = suqa_hip_qms_apply_W(suqa::ctx, bm_acc, W_mask_Enew, bm_enes_new[0], c_factor, curr_E_old);
     if (rc != SUQA_OK) throw std::runtime_error(std::string("ERROR: in qms::apply_W(): ") + suqa_hip_last_error());
+    // counted as L (L - 1) / 2 Toffolis controlled by both energy registers,
+    // gc_mask ignored (qms.cuh:323-328)
+    if (!suqa::gatecounters.counters.empty()) {
+        const GateRecord gr = GateCounter::n_ctrl_toffoli_gates(2 * ene_qbits);
+        const uint ntoff = (ene_levels * (ene_levels - 1)) / 2;
+        suqa::gatecounters.increment_g1g2(ntoff * gr.ng1, ntoff * gr.ng2);
+    }
 }
 
 void apply_W_inverse() { apply_W(); }
@@ -129,6 +138,8 @@ std::vector<double> extract_rands(uint n) {
 int metro_step(bool take_measure) {
     // 1/2: accepted (not measured/measured), 3/4: rejected and restored,
     // -1: rejected and not restored within max_reverse_attempts.
+    const bool counting = !suqa::gatecounters.counters.empty();
+    if (counting) gctr_metrostep.new_record();
     int ret = 0;
     reset_non_syst_qbits();
     apply_Phi();
@@ -151,6 +162,7 @@ int metro_step(bool take_measure) {
         suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
         apply_Phi_inverse();
         if (take_measure) {
+            if (counting) gctr_measure.new_record();
             Enew_meas_d = t_PE_shift + creg_to_uint(c_E_news) / (double)(t_PE_factor * ene_levels);
             E_measures.push_back(Enew_meas_d);
             for (uint ei = 0U; ei < ene_qbits; ++ei)
@@ -160,6 +172,10 @@ int metro_step(bool take_measure) {
             apply_Phi();
             suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
             apply_Phi_inverse();
+            if (counting) {
+                gctr_measure.deactivate();
+                gctr_sample.new_record();
+            }
             ret = 2;
         } else {
             ret = 1;
@@ -169,6 +185,7 @@ int metro_step(bool take_measure) {
 
     apply_U_inverse();
     uint iters = 0;
+    if (counting) gctr_reverse.new_record();
     while (iters < max_reverse_attempts) {
         apply_Phi();
         suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
@@ -195,6 +212,7 @@ int metro_step(bool take_measure) {
         apply_U_inverse();
         iters++;
     }
+    if (counting) gctr_reverse.deactivate();
     if (record_reverse) reverse_counters.push_back(iters);
     if (iters == max_reverse_attempts) return -1;
     return ret;

@@ -26,6 +26,10 @@ extern uint W_mask, W_mask_Enew;
 extern uint curr_E_old;
 extern double curr_E_old_d;
 
+// Gate-count records of the Metropolis step (qms.cuh:21-27, defined in
+// qms.cu:33-39): used when a driver registers them in suqa::gatecounters.
+extern GateCounter gctr_global, gctr_metrostep, gctr_sample, gctr_measure, gctr_reverse;
+
 void fill_bitmap();                                                    // qms.cuh:79-88
 void fill_W_utils(double beta, double t_PE_factor);                    // qms.cuh:96-103
 void cevolution(const double& t, const int& n, const uint& q_control, const bmReg& qsyst);

@@ -93,6 +93,16 @@ int main(int argc, char** argv) {
         auto t_start = std::chrono::high_resolution_clock::now();
         suqa::setup(qms::nqubits);
         qms::setup(p.beta);
+#ifdef GATECOUNT
+        // the gate-count build of the driver (qms.cu:97-105)
+        suqa::gatecounters.add_counter(&qms::gctr_global);
+        suqa::gatecounters.add_counter(&qms::gctr_metrostep);
+        suqa::gatecounters.add_counter(&qms::gctr_sample);
+        suqa::gatecounters.add_counter(&qms::gctr_measure);
+        suqa::gatecounters.add_counter(&qms::gctr_reverse);
+        qms::gctr_global.new_record();
+        qms::gctr_sample.new_record();
+#endif
         init_state();
         uint perc_mstep = (qms::metro_steps + 19) / 20;
         uint count_accepted = 0U;

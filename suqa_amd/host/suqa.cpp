@@ -10,6 +10,7 @@ uint suqa::gc_mask = 0U;
 uint suqa::nq = 0U;
 std::vector<double> suqa::rphase_m;
 suqa_hip_ctx* suqa::ctx = nullptr;
+GateCounterList suqa::gatecounters;
 
 namespace {
 
@@ -28,6 +29,35 @@ void refresh_view() {
     suqa::state.data_im = im;
 }
 
+// Gate counts of one gate with `extra` controls besides the gc_mask ones, as
+// an n-control Toffoli, `times` times (the reference's GATECOUNT blocks).
+void count_toffoli(uint extra, uint times = 1) {
+    if (suqa::gatecounters.counters.empty()) return;
+    const GateRecord gr = GateCounter::n_ctrl_toffoli_gates(suqa::gatecounters.gc_mask_set_qbits + extra);
+    suqa::gatecounters.increment_g1g2(times * gr.ng1, times * gr.ng2);
+}
+
+// Pauli-string rotation on qtar qubits, cnz of them X or Y (suqa.cu:582-603):
+// the rotation, a ladder of 2 (qtar - 1) controlled gates, and basis changes.
+void count_pauli(uint qtar, uint cnz) {
+    if (suqa::gatecounters.counters.empty() || qtar == 0) return;
+    const uint n = suqa::gatecounters.gc_mask_set_qbits;
+    const GateRecord g0 = GateCounter::n_ctrl_toffoli_gates(n), g1 = GateCounter::n_ctrl_toffoli_gates(n + 1);
+    suqa::gatecounters.increment_g1g2(g0);
+    suqa::gatecounters.increment_g1g2(2 * (qtar - 1) * g1.ng1, 2 * (qtar - 1) * g1.ng2);
+    if (cnz > 0) {
+        suqa::gatecounters.increment_g1g2(2 * g0.ng1, 2 * g0.ng2);
+        suqa::gatecounters.increment_g1g2(0, 2 * (cnz - 1));
+    }
+}
+
+uint count_non_z(const std::vector<uint>& types) {
+    uint k = 0;
+    for (uint t : types)
+        if (t != PAULI_Z) ++k;
+    return k;
+}
+
 }  // namespace
 
 void suqa::print_banner() {
@@ -36,10 +66,12 @@ void suqa::print_banner() {
 
 void suqa::activate_gc_mask(const bmReg& q_controls) {   // suqa.cu:45-52
     for (const auto& q : q_controls) suqa::gc_mask |= 1U << q;
+    gatecounters.update_cmask_setbits(gc_mask);
 }
 
 void suqa::deactivate_gc_mask(const bmReg& q_controls) { // suqa.cu:54-61
     for (const auto& q : q_controls) suqa::gc_mask &= ~(1U << q);
+    gatecounters.update_cmask_setbits(gc_mask);
 }
 
 double suqa::vnorm() {
@@ -67,11 +99,17 @@ void suqa::allocate_state(uint Dim) {   // suqa.cu:772-788 (Dim must be a power 
     refresh_view();
 }
 
-void suqa::apply_x(uint q) { chk(suqa_hip_apply_x(ctx, q, gc_mask), "suqa::apply_x()"); }
+void suqa::apply_x(uint q) {
+    chk(suqa_hip_apply_x(ctx, q, gc_mask), "suqa::apply_x()");
+    count_toffoli(0);
+}
 void suqa::apply_x(const bmReg& qs) {
     for (const auto& q : qs) suqa::apply_x(q);
 }
-void suqa::apply_y(uint q) { chk(suqa_hip_apply_y(ctx, q, gc_mask), "suqa::apply_y()"); }
+void suqa::apply_y(uint q) {
+    chk(suqa_hip_apply_y(ctx, q, gc_mask), "suqa::apply_y()");
+    count_toffoli(0);
+}
 void suqa::apply_y(const bmReg& qs) {
     for (const auto& q : qs) suqa::apply_y(q);
 }
@@ -79,7 +117,10 @@ void suqa::apply_z(uint q) { suqa::apply_u1(q, M_PI); }
 void suqa::apply_z(const bmReg& qs) {
     for (const auto& q : qs) suqa::apply_u1(q, M_PI);
 }
-void suqa::apply_h(uint q) { chk(suqa_hip_apply_h(ctx, q, gc_mask), "suqa::apply_h()"); }
+void suqa::apply_h(uint q) {
+    chk(suqa_hip_apply_h(ctx, q, gc_mask), "suqa::apply_h()");
+    count_toffoli(0);
+}
 void suqa::apply_h(const bmReg& qs) {
     for (const auto& q : qs) suqa::apply_h(q);
 }
@@ -99,40 +140,55 @@ void suqa::apply_s(const bmReg& qs) {
 void suqa::apply_u1(uint q, double phase) { suqa::apply_u1(q, 1U, phase); }
 void suqa::apply_u1(uint q, uint q_mask, double phase) {
     chk(suqa_hip_apply_u1(ctx, q, q_mask, phase, gc_mask), "suqa::apply_u1()");
+    count_toffoli(0);
 }
 
 void suqa::apply_cx(const uint& q_control, const uint& q_target, const uint& q_mask) {
     chk(suqa_hip_apply_mcx(ctx, &q_control, &q_mask, 1, q_target, gc_mask), "suqa::apply_cx()");
+    count_toffoli(1);
 }
 void suqa::apply_mcx(const bmReg& q_controls, const uint& q_target) {
     chk(suqa_hip_apply_mcx(ctx, q_controls.data(), nullptr, (uint32_t)q_controls.size(), q_target, gc_mask),
         "suqa::apply_mcx()");
+    count_toffoli((uint)q_controls.size());
 }
 void suqa::apply_mcx(const bmReg& q_controls, const bmReg& q_mask, const uint& q_target) {
     if (q_mask.size() < q_controls.size()) throw std::runtime_error("ERROR: in suqa::apply_mcx(): mask size");
     chk(suqa_hip_apply_mcx(ctx, q_controls.data(), q_mask.data(), (uint32_t)q_controls.size(), q_target, gc_mask),
         "suqa::apply_mcx()");
+    count_toffoli((uint)q_controls.size());
 }
 void suqa::apply_cu1(uint q_control, uint q_target, double phase, uint q_mask) {
     chk(suqa_hip_apply_mcu1(ctx, &q_control, &q_mask, 1, q_target, phase, gc_mask), "suqa::apply_cu1()");
+    count_toffoli(1);
 }
 void suqa::apply_mcu1(const bmReg& q_controls, const uint& q_target, double phase) {
     chk(suqa_hip_apply_mcu1(ctx, q_controls.data(), nullptr, (uint32_t)q_controls.size(), q_target, phase, gc_mask),
         "suqa::apply_mcu1()");
+    count_toffoli((uint)q_controls.size());
 }
 void suqa::apply_mcu1(const bmReg& q_controls, const bmReg& q_mask, const uint& q_target, double phase) {
     if (q_mask.size() < q_controls.size()) throw std::runtime_error("ERROR: in suqa::apply_mcu1(): mask size");
     chk(suqa_hip_apply_mcu1(ctx, q_controls.data(), q_mask.data(), (uint32_t)q_controls.size(), q_target, phase,
                             gc_mask),
         "suqa::apply_mcu1()");
+    count_toffoli((uint)q_controls.size());
 }
 void suqa::apply_swap(const uint& q1, const uint& q2) {
     chk(suqa_hip_apply_swap(ctx, q1, q2, gc_mask), "suqa::apply_swap()");
+    count_toffoli(1, 3);   // three CNOTs
 }
 
 void suqa::apply_phase_list(uint q0, uint q_size, const std::vector<double>& phases) {
     chk(suqa_hip_apply_phase_list(ctx, q0, q_size, phases.data(), (uint32_t)phases.size(), gc_mask),
         "suqa::apply_phase_list()");
+    // 2^(k+1) - 3 controlled phases on k qubits, each an (n+1)- and an n-control
+    // Toffoli (suqa.cu:442-453, the count its comment states)
+    if (!gatecounters.counters.empty()) {
+        const uint fact = (1U << (q_size + 1)) - 3U, n = gatecounters.gc_mask_set_qbits;
+        const GateRecord a = GateCounter::n_ctrl_toffoli_gates(n), b = GateCounter::n_ctrl_toffoli_gates(n + 1);
+        gatecounters.increment_g1g2(fact * (a.ng1 + b.ng1), fact * (a.ng2 + b.ng2));
+    }
 }
 
 void suqa::apply_pauli_TP_rotation(const bmReg& q_apply, const std::vector<uint>& types, double phase) {
@@ -142,6 +198,7 @@ void suqa::apply_pauli_TP_rotation(const bmReg& q_apply, const std::vector<uint>
     chk(suqa_hip_apply_pauli_TP_rotation(ctx, q_apply.data(), types.data(), (uint32_t)q_apply.size(), phase,
                                          gc_mask),
         "suqa::apply_pauli_TP_rotation()");
+    count_pauli((uint)q_apply.size(), count_non_z(types));
 }
 
 void suqa::apply_pauli_TP_rotation_general(const bmReg& q_apply, const std::vector<uint>& types, double phase) {
@@ -150,6 +207,7 @@ void suqa::apply_pauli_TP_rotation_general(const bmReg& q_apply, const std::vect
     chk(suqa_hip_apply_pauli_rotation_general(ctx, q_apply.data(), types.data(), (uint32_t)q_apply.size(), phase,
                                               gc_mask),
         "suqa::apply_pauli_TP_rotation_general()");
+    count_pauli((uint)q_apply.size(), count_non_z(types));
 }
 
 void suqa::apply_rx(uint q, double theta) { suqa::apply_pauli_TP_rotation({q}, {PAULI_X}, -theta / 2.0); }
@@ -228,4 +286,7 @@ void suqa::setup(uint num_qubits) {   // suqa.cu:792-844
     refresh_view();
 }
 
-void suqa::clear() { suqa::deallocate_state(); }
+void suqa::clear() {   // suqa.cu:846-866: the gate counts are printed last
+    suqa::deallocate_state();
+    gatecounters.print_counts();
+}

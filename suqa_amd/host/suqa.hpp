@@ -16,6 +16,7 @@
 #include <stdexcept>
 #include <vector>
 
+#include "gatecounters.hpp"
 #include "suqa_hip.h"
 
 typedef unsigned int uint;
@@ -45,6 +46,7 @@ extern uint gc_mask;
 extern uint nq;
 extern std::vector<double> rphase_m;   // QFT phases, filled once (suqa.cu:611-625)
 extern suqa_hip_ctx* ctx;              // the engine context behind the singleton
+extern GateCounterList gatecounters;   // gate-count instrumentation (suqa.cuh:89-91)
 
 void print_banner();
 

@@ -16,6 +16,8 @@ Fixtures (all JSON; doubles are C99 hex floats, exact):
   qms_c1_out.txt        the stock reference `qms` output file (E A lines)
                         for beta=1, 1000 steps, r=4, seed 42, E in [-1,3]
   evo_n3.txt            the stock reference `evo` output, 20 steps, dt=0.1
+  gatecount_*.txt       the gate-count table printed by the reference `qms`
+                        built with -DGATECOUNT (GATECOUNT_CASES below)
   circuit_*.json        the random-circuit family (gate list, state before
                         the measurements, outcomes, final state)
 """
@@ -37,6 +39,24 @@ QMS_CASES = [
 ]
 
 CIRCUITS = [("circuit_n10", 10, 200, 7), ("circuit_n6", 6, 20, 3)]
+
+# (name, qms command-line arguments after the output file)
+GATECOUNT_CASES = [
+    ("gatecount_c1", ["1.0", "1000", "1", "4"], ["--seed", "42", "--ene-min", "-1", "--ene-max", "3",
+                                                 "--PE-steps", "10"]),
+    ("gatecount_r3", ["2.0", "300", "2", "3"], ["--seed", "123", "--ene-min", "-1", "--ene-max", "3",
+                                                "--max-reverse", "3", "--thermalization", "10"]),
+]
+
+
+def gate_count_table(stdout):
+    """The lines from 'Gate counts' to the blank line after the table."""
+    lines = stdout.splitlines()
+    i = lines.index("Gate counts")
+    j = i + 3   # title, blank, header
+    while j < len(lines) and lines[j].strip():
+        j += 1
+    return "\n".join(lines[i:j]) + "\n"
 
 
 def run(cmd, **kw):
@@ -70,6 +90,14 @@ def main():
     if os.path.exists(tmp):
         os.remove(tmp)
     run([os.path.join(ROOT, "oracle", "_ref", "evo"), "20", "0.1", tmp], stdout=subprocess.DEVNULL)
+    for (name, pos, opts) in GATECOUNT_CASES:
+        tmp = out(name + "_out.tmp")
+        cmd = [os.path.join(ROOT, "oracle", "_ref", "qms_gatecount")] + pos + [tmp] + opts
+        print("+", " ".join(cmd), flush=True)
+        r = subprocess.run(cmd, check=True, capture_output=True, text=True)
+        os.remove(tmp)
+        with open(out(name + ".txt"), "w") as f:
+            f.write(gate_count_table(r.stdout))
     print("golden fixtures written to", HERE)
 
 
