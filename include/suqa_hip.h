@@ -123,6 +123,13 @@ int suqa_hip_prob_filter(suqa_hip_ctx* ctx, const uint32_t* qs, const uint32_t* 
  * evaluated on the host with libm (as the reference CPU path does). */
 int suqa_hip_qms_apply_W(suqa_hip_ctx* ctx, uint32_t q_acc, uint32_t W_mask_Enew,
                          uint32_t bm_enes0, double c_factor, uint32_t curr_E_old);
+/* The same rotation in the two-register layout (the commented bm_enes_old
+ * code of qms.cuh:72-85, 134-136, 201-202; extension): E_old is read from
+ * the register (W_mask_Eold, offset bm_enes_old0) of each basis state instead
+ * of a classical value.  Registers: contiguous, equal width <= 8 qubits,
+ * disjoint, below the acceptance qubit (the most significant). */
+int suqa_hip_qms_apply_W2(suqa_hip_ctx* ctx, uint32_t q_acc, uint32_t W_mask_Enew, uint32_t bm_enes_new0,
+                          uint32_t W_mask_Eold, uint32_t bm_enes_old0, double c_factor);
 
 /* ---- state access (ComplexVec view, suqa.cu:17) ----------------------- */
 int suqa_hip_state_info(suqa_hip_ctx* ctx, uint32_t* nq, uint64_t* N, double** dev_re,

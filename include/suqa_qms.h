@@ -33,6 +33,7 @@ typedef struct {
     int32_t  pe_steps;        /* --PE-steps (10) */
     int32_t  thermalization;  /* --thermalization (100) */
     int32_t  record_reverse;  /* --record-reverse */
+    int32_t  two_registers;   /* --ene-registers 2 (extension): E_old in its own register (r <= 8) */
 } suqa_qms_params;
 
 typedef struct {

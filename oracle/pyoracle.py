@@ -34,7 +34,9 @@ class _Qms(C.Structure):
                 ("t_phase_estimation", C.c_double), ("t_PE_factor", C.c_double), ("t_PE_shift", C.c_double),
                 ("n_phase_estimation", C.c_int), ("gCi", U32), ("c_acc", U32),
                 ("bm_syst", U32 * 3), ("bm_enes", U32 * 32), ("bm_acc", U32),
-                ("W_mask", U32), ("W_mask_Enew", U32), ("c_factor", C.c_double),
+                ("W_mask", U32), ("W_mask_Enew", U32),
+                ("two_registers", U32), ("bm_enes_old", U32 * 32), ("W_mask_Eold", U32),
+                ("c_factor", C.c_double),
                 ("curr_E_old", U32), ("curr_E_old_d", C.c_double), ("record_reverse", C.c_int),
                 ("E_measures", PD), ("X_measures", PD), ("n_meas", C.c_size_t), ("cap_meas", C.c_size_t),
                 ("reverse_counters", PU32), ("n_rev", C.c_size_t), ("cap_rev", C.c_size_t)]
@@ -79,6 +81,9 @@ def _load():
         "so_pcg32_next": (U32, [C.POINTER(_Pcg)]), "so_pcg32_doub": (C.c_double, [C.POINTER(_Pcg)]),
         "so_pcg32_randint": (C.c_int, [C.POINTER(_Pcg), C.c_int, C.c_int]),
         "so_qms_create": (C.POINTER(_Qms), [C.c_double, U32, U32, C.c_uint64, C.c_double, C.c_double, C.c_int, C.c_int]),
+        "so_qms_create2": (C.POINTER(_Qms), [C.c_double, U32, U32, C.c_uint64, C.c_double, C.c_double, C.c_int, C.c_int,
+                                             C.c_int]),
+        "so_qms_apply_W": (None, [C.POINTER(_Qms)]),
         "so_qms_destroy": (None, [C.POINTER(_Qms)]),
         "so_qms_init_state": (None, [C.POINTER(_Qms)]),
         "so_qms_metro_step": (C.c_int, [C.POINTER(_Qms), C.c_int]),
@@ -240,9 +245,9 @@ class Qms:
     """One QMS chain on the oracle (qms.cuh + system.cu + qms.cu loop)."""
 
     def __init__(self, beta, r, seed, ene_min=0.0, ene_max=1.0, pe_steps=10, max_reverse=100,
-                 record_reverse=False):
-        self.p = lib().so_qms_create(beta, r, max_reverse, seed, ene_min, ene_max, pe_steps,
-                                     1 if record_reverse else 0)
+                 record_reverse=False, two_registers=False):
+        self.p = lib().so_qms_create2(beta, r, max_reverse, seed, ene_min, ene_max, pe_steps,
+                                      1 if record_reverse else 0, 1 if two_registers else 0)
         if not self.p:
             raise ValueError("bad QMS parameters")
 
@@ -273,6 +278,14 @@ class Qms:
 
     def rng_state(self):
         return self.p.contents.rng.state
+
+    def set_state(self, re, im):
+        st = self.p.contents.st.contents
+        np.ctypeslib.as_array(st.re, shape=(st.N,))[:] = re
+        np.ctypeslib.as_array(st.im, shape=(st.N,))[:] = im
+
+    def apply_W(self):
+        lib().so_qms_apply_W(self.p)
 
     def state(self):
         st = self.p.contents.st.contents

@@ -13,7 +13,7 @@ CXXFLAGS := -std=c++11 -O3 -DNDEBUG -w -I$(REF)/include
 
 OBJS := $(OUT)/suqa.o $(OUT)/system.o $(OUT)/Rand.o $(OUT)/io.o
 
-all: $(OUT)/qms $(OUT)/evo $(OUT)/ref_harness $(OUT)/qms_gatecount
+all: $(OUT)/qms $(OUT)/evo $(OUT)/qsa $(OUT)/ref_harness $(OUT)/qms_gatecount
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -28,6 +28,9 @@ $(OUT)/qms: $(OUT)/qms.o $(OBJS)
 	$(CXX) $^ -o $@
 
 $(OUT)/evo: $(OUT)/evo.o $(OBJS)
+	$(CXX) $^ -o $@
+
+$(OUT)/qsa: $(OUT)/qsa.o $(OBJS)
 	$(CXX) $^ -o $@
 
 $(OUT)/ref_harness: oracle/ref_harness.cpp $(OBJS)

@@ -513,7 +513,12 @@ static double so_qms_doub(so_qms* q) {
 
 so_qms* so_qms_create(double beta, uint32_t ene_qbits, uint32_t max_reverse, uint64_t seed,
                       double ene_min, double ene_max, int pe_steps, int record_reverse) {
-    if (seed == 0 || ene_qbits == 0 || ene_qbits > 28) return NULL;
+    return so_qms_create2(beta, ene_qbits, max_reverse, seed, ene_min, ene_max, pe_steps, record_reverse, 0);
+}
+
+so_qms* so_qms_create2(double beta, uint32_t ene_qbits, uint32_t max_reverse, uint64_t seed,
+                       double ene_min, double ene_max, int pe_steps, int record_reverse, int two_registers) {
+    if (seed == 0 || ene_qbits == 0 || ene_qbits > 28 || (two_registers && ene_qbits > 13)) return NULL;
     so_qms* q = (so_qms*)calloc(1, sizeof(so_qms));
     if (!q) return NULL;
     /* qms.cu:56-84 */
@@ -524,7 +529,8 @@ so_qms* so_qms_create(double beta, uint32_t ene_qbits, uint32_t max_reverse, uin
     so_pcg32_seed(&q->rng, seed, 1u);
     q->syst_qbits = 3;
     q->ene_qbits = ene_qbits;
-    q->nqubits = q->syst_qbits + ene_qbits + 1;
+    q->two_registers = two_registers ? 1u : 0u;
+    q->nqubits = q->syst_qbits + (1u + q->two_registers) * ene_qbits + 1;
     q->ene_levels = 1u << ene_qbits;
     q->t_PE_shift = ene_min;
     q->t_PE_factor = (double)(q->ene_levels - 1u) / ((double)q->ene_levels * (ene_max - ene_min));
@@ -534,8 +540,13 @@ so_qms* so_qms_create(double beta, uint32_t ene_qbits, uint32_t max_reverse, uin
     /* qms.cuh:79-103 */
     uint32_t c = 0;
     for (uint32_t i = 0; i < q->syst_qbits; ++i) q->bm_syst[i] = c++;
+    if (q->two_registers)
+        for (uint32_t i = 0; i < ene_qbits; ++i) q->bm_enes_old[i] = c++;
     for (uint32_t i = 0; i < ene_qbits; ++i) q->bm_enes[i] = c++;
     q->bm_acc = c;
+    q->W_mask_Eold = 0u;
+    if (q->two_registers)
+        for (uint32_t i = 0; i < ene_qbits; ++i) q->W_mask_Eold |= 1u << q->bm_enes_old[i];
     q->c_factor = beta / (q->t_PE_factor * q->ene_levels);
     q->W_mask_Enew = 0u;
     for (uint32_t i = 0; i < ene_qbits; ++i) q->W_mask_Enew |= 1u << q->bm_enes[i];
@@ -617,18 +628,20 @@ static void so_qms_cevolution(so_qms* q, double t, int n, uint32_t qc) { /* qms.
     so_deactivate_gc_mask(q->st, &qc, 1);
 }
 
-void so_qms_apply_Phi(so_qms* q) {               /* qms.cuh:153-176 */
+static void so_qms_phase_estimation(so_qms* q, const uint32_t* reg) {   /* qms.cuh:153-176 */
     uint32_t r = q->ene_qbits;
     double t = q->t_phase_estimation;
     int n = q->n_phase_estimation;
-    for (uint32_t k = 0; k < r; ++k) so_apply_h(q->st, q->bm_enes[k]);
+    for (uint32_t k = 0; k < r; ++k) so_apply_h(q->st, reg[k]);
     for (int trg = (int)r - 1; trg > -1; --trg) {
         double powr = (double)(1u << (r - 1 - (uint32_t)trg));
-        so_qms_cevolution(q, -powr * t, (int)(powr * n), q->bm_enes[trg]);
-        so_apply_u1(q->st, q->bm_enes[trg], 1u, -powr * t * q->t_PE_shift);
+        so_qms_cevolution(q, -powr * t, (int)(powr * n), reg[trg]);
+        so_apply_u1(q->st, reg[trg], 1u, -powr * t * q->t_PE_shift);
     }
-    so_apply_qft_inverse(q->st, q->bm_enes, r);
+    so_apply_qft_inverse(q->st, reg, r);
 }
+
+void so_qms_apply_Phi(so_qms* q) { so_qms_phase_estimation(q, q->bm_enes); }   /* qms.cuh:203 */
 
 void so_qms_apply_Phi_inverse(so_qms* q) {       /* qms.cuh:179-198 */
     uint32_t r = q->ene_qbits;
@@ -651,7 +664,9 @@ void so_qms_apply_W(so_qms* q) {                 /* qms.cuh:286-307 (dense CPU) 
     for (uint64_t i = s->N / 2; i < s->N; ++i) {
         uint64_t j = i & ~(uint64_t)(1u << qa);
         uint32_t Enew = (((uint32_t)i) & wm) >> b0;
-        double dE = (Enew * c - q->curr_E_old * c);
+        /* two registers: E_old read from the basis state */
+        uint32_t Eold = q->two_registers ? (((uint32_t)i) & q->W_mask_Eold) >> q->bm_enes_old[0] : q->curr_E_old;
+        double dE = (Enew * c - Eold * c);
         if (dE > 0) {
             fs1 = exp(-dE / 2.0);
             fs2 = sqrt(1.0 - fs1 * fs1);
@@ -686,14 +701,19 @@ static uint32_t so_creg_to_uint(const uint32_t* c, uint32_t n) {   /* qms.cuh:11
 }
 
 /* measure_qbits(bm_enes, cs, extract_rands(r)): all draws first (qms.cuh:363-369) */
-static void so_qms_measure_enes(so_qms* q, uint32_t* cs) {
+static void so_qms_measure_reg(so_qms* q, const uint32_t* reg, uint32_t* cs) {
     double rd[32];
     for (uint32_t k = 0; k < q->ene_qbits; ++k) rd[k] = so_qms_doub(q);
-    for (uint32_t k = 0; k < q->ene_qbits; ++k) cs[k] = so_measure_qbit(q->st, q->bm_enes[k], rd[k]);
+    for (uint32_t k = 0; k < q->ene_qbits; ++k) cs[k] = so_measure_qbit(q->st, reg[k], rd[k]);
 }
+static void so_qms_measure_enes(so_qms* q, uint32_t* cs) { so_qms_measure_reg(q, q->bm_enes, cs); }
 
 static void so_qms_reset_non_syst(so_qms* q) {   /* qms.cuh:128-150 */
     double rd[32];
+    if (q->two_registers) {   /* the E_old register first (qms.cuh:134-136) */
+        for (uint32_t k = 0; k < q->ene_qbits; ++k) rd[k] = so_qms_doub(q);
+        for (uint32_t k = 0; k < q->ene_qbits; ++k) so_apply_reset(q->st, q->bm_enes_old[k], rd[k]);
+    }
     for (uint32_t k = 0; k < q->ene_qbits; ++k) rd[k] = so_qms_doub(q);
     for (uint32_t k = 0; k < q->ene_qbits; ++k) so_apply_reset(q->st, q->bm_enes[k], rd[k]);
     so_apply_reset(q->st, q->bm_acc, so_qms_doub(q));
@@ -704,12 +724,19 @@ int so_qms_metro_step(so_qms* q, int take_measure) {   /* qms.cuh:373-545 */
     uint32_t r = q->ene_qbits;
     uint32_t c_E_news[32] = {0}, c_E_olds[32] = {0};
     so_qms_reset_non_syst(q);
-    so_qms_apply_Phi(q);
-    so_qms_measure_enes(q, c_E_olds);
+    if (q->two_registers) {
+        /* E_old into its own register, which keeps the measured value */
+        so_qms_phase_estimation(q, q->bm_enes_old);
+        so_qms_measure_reg(q, q->bm_enes_old, c_E_olds);
+    } else {
+        so_qms_apply_Phi(q);
+        so_qms_measure_enes(q, c_E_olds);
+    }
     q->curr_E_old = so_creg_to_uint(c_E_olds, r);
     q->curr_E_old_d = q->t_PE_shift + q->curr_E_old / (double)(q->t_PE_factor * q->ene_levels);
-    for (uint32_t ei = 0; ei < r; ++ei)
-        if (c_E_olds[ei]) so_apply_x(q->st, q->bm_enes[ei]);
+    if (!q->two_registers)
+        for (uint32_t ei = 0; ei < r; ++ei)
+            if (c_E_olds[ei]) so_apply_x(q->st, q->bm_enes[ei]);
     q->gCi = so_qms_draw_C(q);
     so_qms_apply_U(q);
     q->c_acc = so_measure_qbit(q->st, q->bm_acc, so_qms_doub(q));

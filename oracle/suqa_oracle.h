@@ -111,6 +111,11 @@ typedef struct {
     uint32_t  bm_enes[32];
     uint32_t  bm_acc;
     uint32_t  W_mask, W_mask_Enew;
+    /* two-register layout (the commented bm_enes_old code, qms.cuh:72-85):
+     * E_old in its own register between the system and E_new */
+    uint32_t  two_registers;
+    uint32_t  bm_enes_old[32];
+    uint32_t  W_mask_Eold;
     double    c_factor;
     uint32_t  curr_E_old;
     double    curr_E_old_d;
@@ -127,6 +132,13 @@ typedef struct {
 so_qms* so_qms_create(double beta, uint32_t ene_qbits, uint32_t max_reverse,
                       uint64_t seed, double ene_min, double ene_max, int pe_steps,
                       int record_reverse);
+/* The same with two energy registers when two_registers != 0: bitmap syst
+ * 0..2, E_old 3..2+r, E_new 3+r..2+2r, acceptance 3+2r; the reset draws r
+ * doubles for E_old first; E_old is phase-estimated into its register and
+ * stays there; W reads E_old from the register. */
+so_qms* so_qms_create2(double beta, uint32_t ene_qbits, uint32_t max_reverse,
+                       uint64_t seed, double ene_min, double ene_max, int pe_steps,
+                       int record_reverse, int two_registers);
 void    so_qms_destroy(so_qms* q);
 void    so_qms_init_state(so_qms* q);                  /* system.cu:6-11 */
 void    so_qms_evolution(so_qms* q, double t, int n);  /* system.cu:21-26 */

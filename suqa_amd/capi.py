@@ -65,6 +65,7 @@ SIGNATURES = {
     "suqa_hip_apply_reset": (C.c_int, [CTX, U32, C.c_double, U32]),
     "suqa_hip_prob_filter": (C.c_int, [CTX, PU32, PU32, U32, PD]),
     "suqa_hip_qms_apply_W": (C.c_int, [CTX, U32, U32, U32, C.c_double, U32]),
+    "suqa_hip_qms_apply_W2": (C.c_int, [CTX, U32, U32, U32, U32, U32, C.c_double]),
     "suqa_hip_state_info": (C.c_int, [CTX, PU32, C.POINTER(C.c_uint64), C.POINTER(PD), C.POINTER(PD)]),
     "suqa_hip_download": (C.c_int, [CTX, PD, PD]),
     "suqa_hip_upload": (C.c_int, [CTX, PD, PD]),

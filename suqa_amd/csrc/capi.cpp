@@ -539,6 +539,53 @@ int suqa_hip_prob_filter(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* q_
     return SUQA_OK;
 }
 
+namespace {
+
+// fs1/fs2 exactly as qms.cuh:291-300 evaluates them (host libm).
+void w_coef(uint64_t En, uint64_t Eo, double cf, double* out) {
+    const double dE = ((double)(uint32_t)En * cf - (double)(uint32_t)Eo * cf);
+    if (dE > 0) {
+        out[0] = std::exp(-dE / 2.0);
+        out[1] = std::sqrt(1.0 - out[0] * out[0]);
+    } else {
+        out[0] = 1.0;
+        out[1] = 0.0;
+    }
+}
+
+// The (E_old, E_new) coefficient table of an L-level register in c->d_wtab
+// (L <= 256), rebuilt when c_factor, the mask or the offset change.
+int w_table(suqa_hip_ctx* c, uint32_t wmask, uint32_t b0, double cf, uint64_t L) {
+    if (c->wtab_valid && c->wtab_c == cf && c->wtab_mask == wmask && c->wtab_b0 == b0) return SUQA_OK;
+    std::vector<double> tab(2 * L * L);
+    for (uint64_t eo = 0; eo < L; ++eo)
+        for (uint64_t en = 0; en < L; ++en) w_coef(en, eo, cf, &tab[2 * (eo * L + en)]);
+    if (c->wtab_L < L) {
+        if (c->d_wtab) SQE_HIP(hipFree(c->d_wtab));
+        c->d_wtab = nullptr;
+        void* p = nullptr;
+        SQE_HIP(hipMalloc(&p, sizeof(double) * 2 * 256 * 256));
+        c->d_wtab = static_cast<double*>(p);
+        c->wtab_L = 256;
+    }
+    SQE_HIP(hipMemcpyAsync(c->d_wtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, c->stream));
+    SQE_HIP(hipStreamSynchronize(c->stream));
+    c->wtab_valid = true;
+    c->wtab_c = cf;
+    c->wtab_mask = wmask;
+    c->wtab_b0 = b0;
+    return SUQA_OK;
+}
+
+// An energy register mask: contiguous bits from b0.
+bool reg_mask_ok(uint32_t mask, uint32_t b0) {
+    if (b0 >= 32 || mask == 0) return false;
+    const uint32_t m = mask >> b0;
+    return (mask & ((1u << b0) - 1u)) == 0 && (m & (m + 1u)) == 0;
+}
+
+}  // namespace
+
 int suqa_hip_qms_apply_W(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmask, uint32_t b0, double cf,
                          uint32_t Eold) {
     SQE_CTX(c);
@@ -546,43 +593,13 @@ int suqa_hip_qms_apply_W(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmask, uint32
     if (b0 >= 32) return fail(SUQA_ERR_ARG, "bad energy register offset");
     const uint64_t L = (uint64_t)(wmask >> b0) + 1ull;
     if (L > (1ull << 20)) return fail(SUQA_ERR_ARG, "energy register too wide for apply_W");
-    // fs1/fs2 exactly as qms.cuh:291-300 evaluates them (host libm).
-    auto coef = [&](uint64_t En, uint64_t Eo, double* out) {
-        const double dE = ((double)(uint32_t)En * cf - (double)(uint32_t)Eo * cf);
-        if (dE > 0) {
-            out[0] = std::exp(-dE / 2.0);
-            out[1] = std::sqrt(1.0 - out[0] * out[0]);
-        } else {
-            out[0] = 1.0;
-            out[1] = 0.0;
-        }
-    };
     const double* row = nullptr;
     if (L <= 256 && Eold < L) {
-        if (!(c->wtab_valid && c->wtab_c == cf && c->wtab_mask == wmask && c->wtab_b0 == b0)) {
-            std::vector<double> tab(2 * L * L);
-            for (uint64_t eo = 0; eo < L; ++eo)
-                for (uint64_t en = 0; en < L; ++en) coef(en, eo, &tab[2 * (eo * L + en)]);
-            if (c->wtab_L < L) {
-                if (c->d_wtab) SQE_HIP(hipFree(c->d_wtab));
-                c->d_wtab = nullptr;
-                void* p = nullptr;
-                SQE_HIP(hipMalloc(&p, sizeof(double) * 2 * 256 * 256));
-                c->d_wtab = static_cast<double*>(p);
-                c->wtab_L = 256;
-            }
-            SQE_HIP(hipMemcpyAsync(c->d_wtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice,
-                                   c->stream));
-            SQE_HIP(hipStreamSynchronize(c->stream));
-            c->wtab_valid = true;
-            c->wtab_c = cf;
-            c->wtab_mask = wmask;
-            c->wtab_b0 = b0;
-        }
+        if (int rc = w_table(c, wmask, b0, cf, L)) return rc;
         row = c->d_wtab + 2 * (uint64_t)Eold * L;
     } else {
         std::vector<double> r(2 * L);
-        for (uint64_t en = 0; en < L; ++en) coef(en, Eold, &r[2 * en]);
+        for (uint64_t en = 0; en < L; ++en) w_coef(en, Eold, cf, &r[2 * en]);
         if (c->wrow_cap < L) {
             if (c->d_wrow) SQE_HIP(hipFree(c->d_wrow));
             c->d_wrow = nullptr;
@@ -595,7 +612,23 @@ int suqa_hip_qms_apply_W(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmask, uint32
         SQE_HIP(hipStreamSynchronize(c->stream));
         row = c->d_wrow;
     }
-    SQE_HIP(sqe::launch_qms_W(c, row, wmask, b0));
+    SQE_HIP(sqe::launch_qms_W(c, row, wmask, b0, 0u, 0u, (uint32_t)L));
+    return SUQA_OK;
+}
+
+int suqa_hip_qms_apply_W2(suqa_hip_ctx* c, uint32_t q_acc, uint32_t W_mask_Enew, uint32_t bm_enes_new0,
+                          uint32_t W_mask_Eold, uint32_t bm_enes_old0, double cf) {
+    SQE_CTX(c);
+    if (q_acc != c->nq - 1) return fail(SUQA_ERR_ARG, "apply_W needs the acceptance qubit to be the most significant");
+    if (!reg_mask_ok(W_mask_Enew, bm_enes_new0) || !reg_mask_ok(W_mask_Eold, bm_enes_old0))
+        return fail(SUQA_ERR_ARG, "energy registers must be contiguous bit ranges at their offsets");
+    if ((W_mask_Enew & W_mask_Eold) != 0 || ((W_mask_Enew | W_mask_Eold) >> q_acc) != 0)
+        return fail(SUQA_ERR_ARG, "energy registers overlap each other or the acceptance qubit");
+    const uint64_t L = (uint64_t)(W_mask_Enew >> bm_enes_new0) + 1ull;
+    if ((uint64_t)(W_mask_Eold >> bm_enes_old0) + 1ull != L) return fail(SUQA_ERR_ARG, "energy registers differ in width");
+    if (L > 256) return fail(SUQA_ERR_ARG, "energy registers too wide for the two-register apply_W (at most 8 qubits)");
+    if (int rc = w_table(c, W_mask_Enew, bm_enes_new0, cf, L)) return rc;
+    SQE_HIP(sqe::launch_qms_W(c, c->d_wtab, W_mask_Enew, bm_enes_new0, W_mask_Eold, bm_enes_old0, (uint32_t)L));
     return SUQA_OK;
 }
 

@@ -127,7 +127,12 @@ hipError_t launch_phase_list(suqa_hip_ctx* c, const Sel& sel, uint32_t q0, uint3
 hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
                        double alg_bytes);
 hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale);
-hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_row, uint32_t wmask, uint32_t b0);
+// W of qms.cuh:220-242 over the acceptance pairs (i, i without the top bit):
+// coefficients (fs1, fs2) at d_tab[2 (Eo L + En)], En = (i & wmask) >> b0 and
+// Eo = (i & omask) >> ob0 (omask = 0: Eo = 0, d_tab is the row of a classical
+// E_old).
+hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_tab, uint32_t wmask, uint32_t b0, uint32_t omask,
+                        uint32_t ob0, uint32_t L);
 hipError_t launch_copy(suqa_hip_ctx* c, void* dst, const void* src, uint64_t nbytes);
 // Number of (sum, sum) partial pairs launch_norm needs for a 2^nq state.
 uint64_t partial_pairs_capacity(uint64_t N);

@@ -362,12 +362,15 @@ __global__ __launch_bounds__(kBlock) void k_init(double* __restrict__ data, uint
 
 // qms::apply_W, dense form of qms.cuh:286-307; row[2*E] = (fs1, fs2) for E_old fixed.
 __global__ __launch_bounds__(kBlock) void k_qms_W(double* __restrict__ re, double* __restrict__ im,
-                                                  uint64_t half, const double* __restrict__ row,
-                                                  uint32_t wmask, uint32_t b0) {
+                                                  uint64_t half, const double* __restrict__ tab,
+                                                  uint32_t wmask, uint32_t b0, uint32_t omask, uint32_t ob0,
+                                                  uint32_t L) {
     const uint64_t j = item_id();
     if (j >= half) return;
     const uint64_t i = j + half;
     const uint32_t En = ((uint32_t)i & wmask) >> b0;
+    const uint32_t Eo = ((uint32_t)i & omask) >> ob0;
+    const double* row = tab + 2 * (uint64_t)Eo * L;
     const double fs1 = row[2 * En], fs2 = row[2 * En + 1];
     double t = re[j];
     const double ri = re[i];
@@ -619,12 +622,13 @@ hipError_t launch_init(suqa_hip_ctx* c) {
     return hipGetLastError();
 }
 
-hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_row, uint32_t wmask, uint32_t b0) {
+hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_tab, uint32_t wmask, uint32_t b0, uint32_t omask,
+                        uint32_t ob0, uint32_t L) {
     if (hipError_t e = flush(c)) return e;
     const uint64_t half = c->N / 2;
     timing_begin(c, "qms_W", 32.0 * (double)c->N);
-    hipLaunchKernelGGL(k_qms_W, dim3(grid_for(half)), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, half, d_row,
-                       wmask, b0);
+    hipLaunchKernelGGL(k_qms_W, dim3(grid_for(half)), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, half, d_tab,
+                       wmask, b0, omask, ob0, L);
     timing_end(c);
     return hipGetLastError();
 }

@@ -27,12 +27,17 @@ uint bm_acc;
 uint W_mask, W_mask_Enew;
 uint curr_E_old;
 double curr_E_old_d;
+bool two_registers = false;
+std::vector<uint> bm_enes_old;
+uint W_mask_Eold = 0U;
 
 void fill_bitmap() {
     bm_syst.resize(syst_qbits);
+    bm_enes_old.resize(two_registers ? ene_qbits : 0);
     bm_enes_new.resize(ene_qbits);
     uint c = 0;
     for (uint i = 0; i < syst_qbits; ++i) bm_syst[i] = c++;
+    for (uint& q : bm_enes_old) q = c++;
     for (uint i = 0; i < ene_qbits; ++i) bm_enes_new[i] = c++;
     bm_acc = c;
 }
@@ -41,6 +46,8 @@ void fill_W_utils(double beta, double t_PE_factor_) {
     c_factor = beta / (t_PE_factor_ * ene_levels);
     W_mask_Enew = 0U;
     for (uint i = 0; i < ene_qbits; ++i) W_mask_Enew |= (1U << bm_enes_new[i]);
+    W_mask_Eold = 0U;
+    for (uint q : bm_enes_old) W_mask_Eold |= 1U << q;
     W_mask = (1U << bm_acc) | W_mask_Enew;
 }
 
@@ -61,6 +68,10 @@ uint creg_to_uint(const std::vector<uint>& c_reg) {
 
 void reset_non_syst_qbits() {
     std::vector<double> rgenerates(ene_qbits);
+    if (two_registers) {   // the E_old register first (qms.cuh:134-136)
+        for (auto& el : rgenerates) el = rangen.doub();
+        suqa::apply_reset(bm_enes_old, rgenerates);
+    }
     for (auto& el : rgenerates) el = rangen.doub();
     suqa::apply_reset(bm_enes_new, rgenerates);
     suqa::apply_reset(bm_acc, rangen.doub());
@@ -89,6 +100,7 @@ void apply_phase_estimation_inverse(const std::vector<uint>& q_syst, const std::
 }
 
 void apply_Phi() { apply_phase_estimation(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation); }
+void apply_Phi_old() { apply_phase_estimation(bm_syst, bm_enes_old, t_phase_estimation, n_phase_estimation); }
 void apply_Phi_inverse() {
     apply_phase_estimation_inverse(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation);
 }
@@ -104,7 +116,9 @@ uint draw_C() {
 // qms.cuh:312-329: one engine kernel; exp/sqrt of the acceptance rotation are
 // evaluated on the host with libm (as the reference CPU path does).
 void apply_W() {
-    int rc = suqa_hip_qms_apply_W(suqa::ctx, bm_acc, W_mask_Enew, bm_enes_new[0], c_factor, curr_E_old);
+    int rc = two_registers ? suqa_hip_qms_apply_W2(suqa::ctx, bm_acc, W_mask_Enew, bm_enes_new[0], W_mask_Eold,
+                                                   bm_enes_old[0], c_factor)
+                           : suqa_hip_qms_apply_W(suqa::ctx, bm_acc, W_mask_Enew, bm_enes_new[0], c_factor, curr_E_old);
     if (rc != SUQA_OK) throw std::runtime_error(std::string("ERROR: in qms::apply_W(): ") + suqa_hip_last_error());
     // counted as L (L - 1) / 2 Toffolis controlled by both energy registers,
     // gc_mask ignored (qms.cuh:323-328)
@@ -142,15 +156,21 @@ int metro_step(bool take_measure) {
     if (counting) gctr_metrostep.new_record();
     int ret = 0;
     reset_non_syst_qbits();
-    apply_Phi();
     std::vector<uint> c_E_news(ene_qbits, 0), c_E_olds(ene_qbits, 0);
     uint Enew_meas;
-    suqa::measure_qbits(bm_enes_new, c_E_olds, extract_rands(ene_qbits));
+    if (two_registers) {   // E_old into its own register, which keeps the measured value
+        apply_Phi_old();
+        suqa::measure_qbits(bm_enes_old, c_E_olds, extract_rands(ene_qbits));
+    } else {
+        apply_Phi();
+        suqa::measure_qbits(bm_enes_new, c_E_olds, extract_rands(ene_qbits));
+    }
 
     curr_E_old = creg_to_uint(c_E_olds);
     curr_E_old_d = t_PE_shift + curr_E_old / (double)(t_PE_factor * ene_levels);
-    for (uint ei = 0U; ei < ene_qbits; ++ei)
-        if (c_E_olds[ei]) suqa::apply_x(bm_enes_new[ei]);
+    if (!two_registers)
+        for (uint ei = 0U; ei < ene_qbits; ++ei)
+            if (c_E_olds[ei]) suqa::apply_x(bm_enes_new[ei]);
 
     gCi = draw_C();
     apply_U();

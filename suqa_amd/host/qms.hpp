@@ -25,6 +25,13 @@ extern uint bm_acc;
 extern uint W_mask, W_mask_Enew;
 extern uint curr_E_old;
 extern double curr_E_old_d;
+// Two-register layout (the commented bm_enes_old code of qms.cuh:72-85,
+// 134-136, 201-202; extension, default off): E_old is phase-estimated into
+// its own register (qubits 3..2+r, E_new moves to 3+r..2+2r) and stays there;
+// the reset draws r doubles for it first; W reads E_old from the register.
+extern bool two_registers;
+extern std::vector<uint> bm_enes_old;
+extern uint W_mask_Eold;
 
 // Gate-count records of the Metropolis step (qms.cuh:21-27, defined in
 // qms.cu:33-39): used when a driver registers them in suqa::gatecounters.
@@ -41,6 +48,7 @@ void apply_phase_estimation_inverse(const std::vector<uint>& q_syst, const std::
                                     const double& t, const uint& n);
 void apply_Phi();
 void apply_Phi_inverse();
+void apply_Phi_old();   // qms.cuh:201 (two-register layout)
 uint draw_C();
 void apply_W();
 void apply_W_inverse();

@@ -4,8 +4,10 @@
 //   suqa_qms <beta> <metro steps> <reset each> <num ene qbits> <output file path>
 //            [--max-reverse N (100)] [--seed S (time)] [--ene-min E (0.0)]
 //            [--ene-max E (1.0)] [--PE-steps N (10)] [--thermalization N (100)]
-//            [--record-reverse]
+//            [--record-reverse] [--ene-registers 1|2 (1)]
 //
+// --ene-registers 2 (an extension) runs the two-register layout: E_old in its
+// own register, 3 + 2r + 1 qubits.
 // Output file: "# E A" header if the file is new, then "%.16lg %.16lg" lines
 // appended every ceil(steps/20) steps (measurements after the last such step
 // are not written, as in the reference); <out>_revcounts with
@@ -40,7 +42,7 @@ int main(int argc, char** argv) {
         printf("usage: %s <beta> <metro steps> <reset each> <num ene qbits> <output file path> [--max-reverse <max "
                "reverse attempts> (100)] [--seed <seed> (random)] [--ene-min <min energy> (0.0)] [--ene-max <max "
                "energy> (1.0)] [--PE-steps <steps of PE evolution> (10)] [--thermalization <steps> (100)] "
-               "[--record-reverse]\n",
+               "[--record-reverse] [--ene-registers <1|2> (1)]\n",
                argv[0]);
         exit(1);
     }
@@ -68,6 +70,11 @@ int main(int argc, char** argv) {
         if (const char* v = val("--ene-max")) p.ene_max = std::stod(v);
         if (const char* v = val("--PE-steps")) pe = (int)std::stod(v);
         if (const char* v = val("--thermalization")) therm = (int)std::stod(v);
+        if (const char* v = val("--ene-registers")) {
+            const int k = atoi(v);
+            if (k != 1 && k != 2) throw std::runtime_error("ERROR: argument <ene registers> must be 1 or 2");
+            p.two_registers = k == 2 ? 1 : 0;
+        }
         if (steps <= 0) throw std::runtime_error("ERROR: argument <metro steps> invalid");
         if (reset_each <= 0) throw std::runtime_error("ERROR: argument <reset each> non positive");
         if (ene_qbits <= 0) throw std::runtime_error("ERROR: argument <num ene qbits> non positive");

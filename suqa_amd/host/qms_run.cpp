@@ -21,11 +21,12 @@ void configure(const suqa_qms_params& p) {   // qms.cu:251-276
     qms::max_reverse_attempts = p.max_reverse;
     qms::n_phase_estimation = p.pe_steps;
     qms::record_reverse = p.record_reverse != 0;
+    qms::two_registers = p.two_registers != 0;
     qms::iseed = p.seed;
     if (qms::iseed > 0) qms::rangen.set_seed(qms::iseed);
     qms::iseed = qms::rangen.get_seed();
     qms::syst_qbits = (uint)syst_qbits;
-    qms::nqubits = qms::syst_qbits + qms::ene_qbits + 1;
+    qms::nqubits = qms::syst_qbits + (qms::two_registers ? 2 : 1) * qms::ene_qbits + 1;
     qms::Dim = (1U << qms::nqubits);
     qms::ene_levels = (1U << qms::ene_qbits);
     qms::syst_levels = (1U << qms::syst_qbits);
@@ -43,6 +44,8 @@ void validate(const suqa_qms_params& p) {   // parser_qms.hpp:130-157
     if (p.reset_each == 0) throw std::runtime_error("ERROR: argument <reset each> non positive");
     if (p.ene_qbits == 0) throw std::runtime_error("ERROR: argument <num ene qbits> non positive");
     if (p.ene_qbits > 28) throw std::runtime_error("ERROR: argument <num ene qbits> too large");
+    if (p.two_registers && p.ene_qbits > 8)
+        throw std::runtime_error("ERROR: argument <num ene qbits> too large for two energy registers (at most 8)");
     if (p.max_reverse == 0) throw std::runtime_error("ERROR: argument <max reverse attempts> non positive");
     if (p.pe_steps <= 0) throw std::runtime_error("ERROR: argument <steps of PE evolution> non positive");
 }

@@ -16,7 +16,7 @@ class Params(C.Structure):
     _fields_ = [("beta", C.c_double), ("metro_steps", C.c_uint32), ("reset_each", C.c_uint32),
                 ("ene_qbits", C.c_uint32), ("max_reverse", C.c_uint32), ("seed", C.c_uint64),
                 ("ene_min", C.c_double), ("ene_max", C.c_double), ("pe_steps", C.c_int32),
-                ("thermalization", C.c_int32), ("record_reverse", C.c_int32)]
+                ("thermalization", C.c_int32), ("record_reverse", C.c_int32), ("two_registers", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -71,10 +71,11 @@ def _ptr(a, ct):
 
 
 def run(beta, steps, reset_each, r, seed, ene_min=0.0, ene_max=1.0, pe_steps=10, thermalization=100,
-        max_reverse=100, record_reverse=False, final_state=False, cap_meas=None):
-    """The qms.cu driver loop on the GPU engine; returns a dict of per-step results."""
+        max_reverse=100, record_reverse=False, final_state=False, cap_meas=None, two_registers=False):
+    """The qms.cu driver loop on the GPU engine; returns a dict of per-step results.
+    two_registers: E_old in its own register (3 + 2r + 1 qubits)."""
     p = Params(beta, steps, reset_each, r, max_reverse, seed, ene_min, ene_max, pe_steps, thermalization,
-               1 if record_reverse else 0)
+               1 if record_reverse else 0, 1 if two_registers else 0)
     cap = steps if cap_meas is None else cap_meas
     rets = np.zeros(steps, dtype=np.int32)
     rng = np.zeros(steps, dtype=np.uint64)
@@ -85,7 +86,7 @@ def run(beta, steps, reset_each, r, seed, ene_min=0.0, ene_max=1.0, pe_steps=10,
     res.rets, res.rng_after = _ptr(rets, C.c_int32), _ptr(rng, C.c_uint64)
     res.E, res.X, res.cap_meas = _ptr(E, C.c_double), _ptr(X, C.c_double), cap
     res.reverse, res.cap_reverse = _ptr(rev, C.c_uint32), steps
-    N = 1 << (r + 4)
+    N = 1 << ((2 * r if two_registers else r) + 4)
     fre = np.zeros(N)
     fim = np.zeros(N)
     if final_state:

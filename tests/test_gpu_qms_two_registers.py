@@ -1,0 +1,71 @@
+"""The two-register QMS layout (E_old in its own register, 3 + 2r + 1 qubits;
+the configuration the c1/c5 workloads describe) on the GPU: the C++ qms::
+mirror over the engine, with suqa_hip_qms_apply_W2 reading E_old from the
+register, reproduces the oracle's two-register chain bit for bit (return
+codes, final pcg32 state, E/X measurements, reverse counters; final state
+within 1e-12), with gate passes on and off.  The oracle's two-register mode
+is pinned against the reference-pinned one-register path in
+tests/test_qms_two_registers.py."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.common import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# beta, r, seed, steps, reset_each, thermalization, max_reverse, E range
+CASES = [
+    (1.0, 4, 42, 300, 1, 50, 100, (-1.0, 3.0)),    # c1 as written: 12 qubits
+    (2.0, 3, 123, 300, 1, 10, 1, (-1.0, 3.0)),     # max_reverse 1: failed restorations (-1)
+    (0.5, 5, 7, 120, 2, 20, 100, (0.0, 1.0)),
+]
+
+
+@pytest.fixture(scope="module")
+def qms():
+    from suqa_amd import qms
+    return qms
+
+
+@pytest.mark.parametrize("tile_bits", ["0", "11"])
+@pytest.mark.parametrize("case", CASES, ids=[f"b{c[0]}-r{c[1]}-s{c[2]}" for c in CASES])
+def test_two_register_chain_vs_oracle(qms, case, tile_bits, monkeypatch):
+    beta, r, seed, steps, reset_each, therm, max_rev, (emin, emax) = case
+    print(f"two-register QMS {case} tile bits {tile_bits}", flush=True)
+    monkeypatch.setenv("SUQA_HIP_TILE_BITS", tile_bits)
+    out = qms.run(beta, steps, reset_each, r, seed, emin, emax, 10, therm, max_rev, True, final_state=True,
+                  two_registers=True)
+    o = po.Qms(beta, r, seed, emin, emax, 10, max_rev, True, two_registers=True)
+    acc, rets = o.run(steps, reset_each, therm)
+    assert list(out["rets"]) == rets
+    assert out["accepted"] == acc
+    assert int(out["rng_after"][-1]) == o.rng_state()
+    E, X = o.measures()
+    assert_bitwise(out["E"], np.array(E), "E")
+    assert_bitwise(out["X"], np.array(X), "X")
+    assert list(out["reverse"]) == o.reverse_counters()
+    re, im = o.state()
+    assert out["final"][0].shape == re.shape == (1 << (2 * r + 4),)
+    np.testing.assert_allclose(out["final"][0], re, atol=1e-12, rtol=0)
+    np.testing.assert_allclose(out["final"][1], im, atol=1e-12, rtol=0)
+    if max_rev == 1:
+        assert -1 in rets
+
+
+def test_driver_option(qms):
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "out.txt")
+        r = subprocess.run([qms.DRIVER, "1.0", "100", "1", "3", out, "--seed", "9", "--ene-min", "-1", "--ene-max",
+                            "3", "--thermalization", "10", "--ene-registers", "2"], capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout.splitlines()[0] == "10"    # qubits: 3 + 2 * 3 + 1
+        bad = subprocess.run([qms.DRIVER, "1.0", "10", "1", "3", out, "--ene-registers", "3"], capture_output=True,
+                             text=True, timeout=120)
+        assert bad.returncode != 0 and "ene registers" in (bad.stdout + bad.stderr)
