@@ -29,6 +29,9 @@ The QMS half of the metric is reported under "extra":
                 4-qubit energy register, E in [-1,3], pcg32 seed 42) on the
                 plain FP64 GPU path (suqa_qms_run), Metropolis steps/s, with
                 its accept/reject sequence checked against the oracle;
+  qms_plain_two_registers -- the same chain with E_old in its own register
+                (two 4-qubit registers, 12 qubits), checked against the
+                oracle's two-register chain;
   qms_batch  -- 4096 independent chains (register widths 3..6 drawn with
                 pcg32 seed 5, betas 0.1..5.0, seeds 1000+i, 500 steps) in one
                 persistent kernel, chain-steps/s, a sample of chains checked
@@ -129,6 +132,15 @@ def qms_legs(no_cpu):
                          "accepted": int(plain["accepted"]), "seconds": round(plain["seconds"], 4),
                          "config": "beta=1.0, 1000 steps, r=4 (8 qubits), E in [-1,3], seed 42, reset_each 1"}}
     log(f"qms plain: {steps / plain['seconds']:.1f} steps/s")
+    # the same chain in the two-register layout the workload describes (E_old in
+    # its own register: 3 + 2 * 4 + 1 = 12 qubits)
+    two = qms.run(beta, steps, 1, r, seed, emin, emax, 10, 100, 100, two_registers=True)
+    res["qms_plain_two_registers"] = {"steps_per_s": round(steps / two["seconds"], 1), "steps": steps,
+                                      "accepted": int(two["accepted"]), "seconds": round(two["seconds"], 4),
+                                      "mean_E": round(float(np.mean(two["E"])), 6),
+                                      "mean_E_one_register": round(float(np.mean(plain["E"])), 6),
+                                      "config": "as qms_plain, two 4-qubit energy registers (12 qubits)"}
+    log(f"qms plain, two registers: {steps / two['seconds']:.1f} steps/s")
     rg = wl.Pcg32(5)
     nch, bsteps = 4096, 500
     chains = [(0.1 + 4.9 * i / (nch - 1), 3 + rg.randint(0, 4), 1000 + i) for i in range(nch)]
@@ -163,6 +175,9 @@ def qms_legs(no_cpu):
     if no_cpu:
         return res
     from oracle import pyoracle as po
+    q2 = po.Qms(beta, r, seed, emin, emax, 10, 100, two_registers=True)
+    _, rets2 = q2.run(steps, 1, 100)
+    res["qms_plain_two_registers"]["rets_match_oracle"] = bool(list(two["rets"]) == rets2)
     q = po.Qms(beta, r, seed, emin, emax, 10, 100)
     t0 = time.perf_counter()
     acc, rets = q.run(steps, 1, 100)
