@@ -63,6 +63,7 @@ struct TileGate {
     uint32_t gtriv;       // mcg == 0: every group meets the selection
     uint32_t dense;       // gtriv, and every element (pair) of a group is selected
     uint32_t base_imag, base_neg;
+    uint32_t arm;         // register-step code path (reg_gate): kind x partner mask x dense
     uint64_t mo, vo;      // selection bits outside the tile (global positions)
     uint64_t yzo;         // parity bits outside the tile
     double c, s;
@@ -198,32 +199,6 @@ __device__ __forceinline__ RM reg_pairs_seq(RM x, const TileGate& g, bool gpass,
     return x;
 }
 
-template <int OP>
-__device__ __forceinline__ RM reg_pairs(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
-    constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
-    if (g.dense) {   // set by the host for one-bit partner masks only
-        // Each mask value has its own case and anything else falls through to the
-        // general path.  With `default:` standing for 4, hipcc (ROCm 7.2) merged the
-        // three OP_Y cases and left half of the xl = 4 outputs unassigned.
-        switch (g.xl) {
-        case 1: return reg_pairs_seq<1, OP, true>(x, g, gpass, gpar, seq);
-        case 2: return reg_pairs_seq<2, OP, true>(x, g, gpass, gpar, seq);
-        case 4: return reg_pairs_seq<4, OP, true>(x, g, gpass, gpar, seq);
-        default: break;
-        }
-    }
-    switch (g.xl) {
-    case 1: return reg_pairs_seq<1, OP, false>(x, g, gpass, gpar, seq);
-    case 2: return reg_pairs_seq<2, OP, false>(x, g, gpass, gpar, seq);
-    case 3: return reg_pairs_seq<3, OP, false>(x, g, gpass, gpar, seq);
-    case 4: return reg_pairs_seq<4, OP, false>(x, g, gpass, gpar, seq);
-    case 5: return reg_pairs_seq<5, OP, false>(x, g, gpass, gpar, seq);
-    case 6: return reg_pairs_seq<6, OP, false>(x, g, gpass, gpar, seq);
-    case 7: return reg_pairs_seq<7, OP, false>(x, g, gpass, gpar, seq);
-    default: return x;   // no flipped bit: not a pair gate (never queued as one)
-    }
-}
-
 // Phase rotation of the selected register amplitudes.
 template <bool DENSE, int J>
 __device__ __forceinline__ RM reg_phase_one(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
@@ -250,11 +225,40 @@ __device__ __forceinline__ RM reg_phase_seq(RM x, const TileGate& g, bool gpass,
     return x;
 }
 
-__device__ __forceinline__ RM reg_phase(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
+// One flat dispatch over every register-step code path, on the arm index the
+// host computes (local_tables): nested switches with a fall-through from the
+// dense cases made the compiler copy the 16 amplitudes into the result
+// registers before dispatching, on top of the copy back at the join.
+// Arms: 0/1 phase dense/general; 2 + 10 op + {0, 1, 2}: pair op with a
+// one-bit partner mask 1, 2, 4, dense; 2 + 10 op + 3 + (xl - 1): general.
+// Every arm is its own case: an earlier dense dispatch written as `case 1`,
+// `case 2`, `default` (= 4) was miscompiled by hipcc (ROCm 7.2), which merged
+// the OP_Y cases and left half of the xl = 4 outputs unassigned.
+#define SQE_PAIR_ARMS(OPV)                                                                                    \
+    case 2 + 10 * OPV + 0: return reg_pairs_seq<1, OPV, true>(x, g, gpass, gpar, seq);                        \
+    case 2 + 10 * OPV + 1: return reg_pairs_seq<2, OPV, true>(x, g, gpass, gpar, seq);                        \
+    case 2 + 10 * OPV + 2: return reg_pairs_seq<4, OPV, true>(x, g, gpass, gpar, seq);                        \
+    case 2 + 10 * OPV + 3: return reg_pairs_seq<1, OPV, false>(x, g, gpass, gpar, seq);                       \
+    case 2 + 10 * OPV + 4: return reg_pairs_seq<2, OPV, false>(x, g, gpass, gpar, seq);                       \
+    case 2 + 10 * OPV + 5: return reg_pairs_seq<3, OPV, false>(x, g, gpass, gpar, seq);                       \
+    case 2 + 10 * OPV + 6: return reg_pairs_seq<4, OPV, false>(x, g, gpass, gpar, seq);                       \
+    case 2 + 10 * OPV + 7: return reg_pairs_seq<5, OPV, false>(x, g, gpass, gpar, seq);                       \
+    case 2 + 10 * OPV + 8: return reg_pairs_seq<6, OPV, false>(x, g, gpass, gpar, seq);                       \
+    case 2 + 10 * OPV + 9: return reg_pairs_seq<7, OPV, false>(x, g, gpass, gpar, seq);
+
+__device__ __forceinline__ RM reg_gate(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
     constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
-    if (g.dense) return reg_phase_seq<true>(x, g, gpass, gpar, seq);
-    return reg_phase_seq<false>(x, g, gpass, gpar, seq);
+    switch (g.arm) {
+    case 0: return reg_phase_seq<true>(x, g, gpass, gpar, seq);
+    case 1: return reg_phase_seq<false>(x, g, gpass, gpar, seq);
+    SQE_PAIR_ARMS(OP_SWAP)
+    SQE_PAIR_ARMS(OP_Y)
+    SQE_PAIR_ARMS(OP_H)
+    SQE_PAIR_ARMS(OP_PAULI)
+    default: return x;   // never set by the host
+    }
 }
+#undef SQE_PAIR_ARMS
 
 __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double* __restrict__ im, const TileArgs a) {
     extern __shared__ double lds[];
@@ -344,13 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
                 if ((base & g.mo) != g.vo) continue;   // uniform: tile outside the gate's selection
                 const bool gpass = (gb & g.mcg) == g.vcg;
                 const uint32_t gpar = ((uint32_t)__popcll(base & g.yzo) + (uint32_t)__popc(gb & g.yzg)) & 1u;
-                switch (g.kind) {
-                case kPhaseOp: x = reg_phase(x, g, gpass, gpar); break;
-                case OP_SWAP: x = reg_pairs<OP_SWAP>(x, g, gpass, gpar); break;
-                case OP_Y: x = reg_pairs<OP_Y>(x, g, gpass, gpar); break;
-                case OP_H: x = reg_pairs<OP_H>(x, g, gpass, gpar); break;
-                default: x = reg_pairs<OP_PAULI>(x, g, gpass, gpar); break;
-                }
+                x = reg_gate(x, g, gpass, gpar);
             }
 #pragma unroll
             for (uint32_t j = 0; j < kLoc; ++j) {
@@ -427,6 +425,14 @@ void local_tables(TileGate& g, const uint32_t* f) {
                 if ((j & xl) == 0) need |= 1u << j;
     }
     g.dense = g.gtriv && need != 0 && (g.lsel & need) == need;
+    // the code path of reg_gate
+    if (g.kind == (uint32_t)kPhaseOp) {
+        g.arm = g.dense ? 0u : 1u;
+    } else {
+        const uint32_t base = 2u + 10u * g.kind;
+        if (g.dense && (xl == 1u || xl == 2u || xl == 4u)) g.arm = base + (xl == 1u ? 0u : xl == 2u ? 1u : 2u);
+        else g.arm = base + 3u + (xl - 1u);
+    }
 }
 
 hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, uint32_t t) {
