@@ -240,7 +240,7 @@ int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t
  * non-identity qubits (at most `width`) carry the block matrix, and the
  * gc_mask controls select the tiles it is applied to (ops sharing a control
  * mask fuse into one controlled block; tiles with a control at 0 are not
- * read).  Controls must not be qubits of the string. */
+ * read).  Controls must not be qubits of the string, nor qubits 0 or 1. */
 int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint32_t nops, uint32_t width,
                             uint32_t* nblocks);
 /* suqa::apply_qft_inverse (suqa.cu:649-661, no final swaps) on qs[0..n),

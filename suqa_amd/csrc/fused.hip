@@ -85,14 +85,19 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // ---------------------------------------------------------------------------
 // One block per launch: a workgroup takes one tile = the K block qubits plus the
-// 7 lowest other qubits (2^(K+7) amplitudes, whole 512-byte runs), applies the
-// block by MFMA and writes the tile back; the blocks come from fusing the whole
-// op list (fuse.cpp).
+// lowest other qubits, tile_bits1(K) in all (runs of at least 256 bytes),
+// applies the block by MFMA and writes the tile back; the blocks come from
+// fusing the whole op list (fuse.cpp).  11-qubit tiles (16 KiB) for K = 4 keep
+// its registers low enough for 8 workgroups per CU; K = 5, 6 take 12-qubit
+// tiles (32 KiB), which give them 2 (row block, column block) items per wave
+// (measured: K = 6 from 1.33 to 1.03 ms per gate, K = 4 slower at 12).
+constexpr uint32_t tile_bits1(uint32_t K) { return K == 4 ? 11u : 12u; }
+constexpr uint32_t kItems = 2;        // (row block, column block) pairs per wave, at most
 struct BlockArgs {
     uint32_t ntile;                    // K + L
     uint32_t L;                        // non-block bits per tile (columns = 2^L)
     uint32_t rowoff[1 << kMaxK];       // compact offset of block index r
-    uint32_t coloff[1 << kTileLow];    // compact offset of column c
+    uint32_t coloff[128];              // compact offset of column c (at most 2^(tile_bits1(K) - K))
     uint64_t tmask;                    // the tile's state bits (compact order = ascending)
     uint64_t freemask;                 // state bits enumerated by the tile number
     uint64_t fixval;                   // the controls (all 1)
@@ -118,7 +123,8 @@ __global__ __launch_bounds__(kBlock) void k_block(float* __restrict__ re, float*
 
     // 1. tile -> LDS (compact order), 16-byte chunks: chunk i of a thread is
     //    element 4 tid + 1024 i; every load is issued before the LDS writes
-    constexpr uint32_t kCh = (1u << (K + kTileLow)) / 4 / kBlock;   // chunks per thread at full size
+    constexpr uint32_t kCh = (1u << tile_bits1(K)) / 4 / kBlock;   // chunks per thread at full size
+    constexpr uint32_t kIt = 1u << (tile_bits1(K) - 11);           // items per wave at full size
     const uint64_t toff = base | deposit(4u * tid, a.tmask);
     uint64_t ibit[3];   // state offsets of tile bits 10, 11, 12
 #pragma unroll
@@ -147,38 +153,44 @@ __global__ __launch_bounds__(kBlock) void k_block(float* __restrict__ re, float*
     __syncthreads();
 
     // 2. MFMA: wave w owns row block rb and column blocks cb0 .. cb0+RB-1
-    const int wpr = 4 / RB;            // waves per row block
-    const int rb = w / wpr;
-    const int cb0 = (w % wpr) * RB;
-    const int ncb = 1 << (a.L - 5);    // 32-column blocks in the tile (1..4)
+    // (row block, 32-column block) items w + 4 it: 4 % RB == 0, so every item
+    // of a wave has row block w % RB; at full size there are 8 items, 2 per wave
+    const int rb = w % RB;
+    const uint32_t items = (uint32_t)RB << (a.L - 5);
     float areg[KS];
 #pragma unroll
     for (int kk = 0; kk < KS; ++kk) areg[kk] = a.A[((size_t)rb * KS + kk) * 64 + lane];
-    floatx16 acc[RB];
+    floatx16 acc[kIt];
+    uint32_t coff[kIt];
+    bool live[kIt];
 #pragma unroll
-    for (int c = 0; c < RB; ++c) {
-        acc[c] = floatx16{0};
-        const int cb = cb0 + c;
-        if (cb >= ncb) continue;
-        const uint32_t coff = a.coloff[cb * 32 + (lane & 31)];
+    for (uint32_t it = 0; it < kIt; ++it) {
+        const uint32_t item = (uint32_t)w + 4u * it;
+        live[it] = item < items;
+        // a wave without this item reads a valid column of its row block and stores nothing
+        coff[it] = a.coloff[(live[it] ? item / RB : 0u) * 32u + (uint32_t)(lane & 31)];
+        acc[it] = floatx16{0};
+    }
 #pragma unroll
-        for (int kk = 0; kk < KS; ++kk) {
-            const int R = 2 * kk + (lane >> 5);
-            const float b = R < (1 << K) ? lre[a.rowoff[R] + coff] : lim[a.rowoff[R - (1 << K)] + coff];
-            acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(areg[kk], b, acc[c], 0, 0, 0);
-        }
+    for (int kk = 0; kk < KS; ++kk) {
+        const int R = 2 * kk + (lane >> 5);
+        const float* src = R < (1 << K) ? lre + a.rowoff[R] : lim + a.rowoff[R - (1 << K)];
+        float b[kIt];
+#pragma unroll
+        for (uint32_t it = 0; it < kIt; ++it) b[it] = src[coff[it]];
+#pragma unroll
+        for (uint32_t it = 0; it < kIt; ++it)
+            acc[it] = __builtin_amdgcn_mfma_f32_32x32x2f32(areg[kk], b[it], acc[it], 0, 0, 0);
     }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < RB; ++c) {
-        const int cb = cb0 + c;
-        if (cb >= ncb) continue;
-        const uint32_t coff = a.coloff[cb * 32 + (lane & 31)];
+    for (uint32_t it = 0; it < kIt; ++it) {
+        if (!live[it]) continue;   // uniform
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int i = rb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-            if (i < (1 << K)) lre[a.rowoff[i] + coff] = acc[c][v];
-            else lim[a.rowoff[i - (1 << K)] + coff] = acc[c][v];
+            if (i < (1 << K)) lre[a.rowoff[i] + coff[it]] = acc[it][v];
+            else lim[a.rowoff[i - (1 << K)] + coff[it]] = acc[it][v];
         }
     }
     __syncthreads();
@@ -203,7 +215,6 @@ __global__ __launch_bounds__(kBlock) void k_block(float* __restrict__ re, float*
 // 2^(K+1)/32 row blocks x 2^(t-K)/32 column blocks, at most kItems each.
 constexpr uint32_t kPassTileBits = 12;   // 32 KiB of FP32 complex amplitudes per tile
 constexpr uint32_t kPassBlocks = 16;     // blocks per launch
-constexpr uint32_t kItems = 2;           // (row block, column block) pairs per wave at t = 12
 
 // LDS layout of a tile: amplitude e sits at lds_sw(e) = e ^ h(e), h the XOR of
 // kSwVec[p] over the set bits p >= 5 of e.  h < 32 depends only on bits the
@@ -558,7 +569,7 @@ void block_args(const Block& b, uint32_t nq, BlockArgs& a) {
     uint64_t bm = 0;
     for (uint32_t j = 0; j < b.K; ++j) bm |= 1ull << b.qubits[j];
     std::vector<uint32_t> low;
-    for (uint32_t q = 0; q < nq && low.size() < (size_t)kTileLow; ++q)
+    for (uint32_t q = 0; q < nq && low.size() < (size_t)(tile_bits1(b.K) - b.K); ++q)
         if (!(((bm | b.cmask) >> q) & 1ull)) low.push_back(q);
     a.L = (uint32_t)low.size();
     uint64_t tm = bm;
@@ -886,6 +897,8 @@ int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint3
         if (list[i].n > width) return fail(SUQA_ERR_ARG, "Pauli op wider than the fusion width");
         if (list[i].n && list[i].qs[list[i].n - 1] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
         if (list[i].cmask >> c->nq) return fail(SUQA_ERR_ARG, "control qubit out of range");
+        // tiles move 4 consecutive amplitudes: qubits 0 and 1 are always tile qubits
+        if (list[i].cmask & 3ull) return fail(SUQA_ERR_ARG, "controls on qubits 0 or 1 are not supported by the FP32 engine");
     }
     return run_ops(c, list, width, nblocks);
 }

@@ -12,7 +12,6 @@
 
 namespace sqf {
 
-constexpr int kTileLow = 7;        // non-block bits inside a one-block tile: 128 MFMA columns
 constexpr int kMinK = 4, kMaxK = 6;
 
 using cd = std::complex<double>;

@@ -75,6 +75,11 @@ def test_fp32_rejects_ops_wider_than_width(capi):
         f32.run_paulis(capi.pauli_ops([([0, 1], [1, 3], 0.1, 1 << 1)]), 5)
     with pytest.raises(capi.SuqaError):
         f32.run_paulis(capi.pauli_ops([([0, 0], [1, 1], 0.1, 0)]), 5)      # repeated qubit
+    for ctrl in (0, 1):                   # controls on qubits 0, 1: rejected in both modes, nothing launched
+        for passes in (False, True):
+            f32.passes = passes
+            with pytest.raises(capi.SuqaError):
+                f32.run_paulis(capi.pauli_ops([([4, 5], [1, 3], 0.1, 1 << ctrl)]), 5)
     f32.close()
 
 
