@@ -100,6 +100,33 @@ Op gate_op(const suqa_hip_gate& g) {
     return o;
 }
 
+Op cx_op(uint64_t ctrl, uint32_t tgt) {
+    std::vector<uint32_t> qs;
+    for (uint32_t q = 0; q < 64; ++q)
+        if ((ctrl >> q) & 1ull) qs.push_back(q);
+    qs.push_back(tgt);
+    return make_op(qs, [&](const Op& o, uint32_t in, std::vector<cd>& col) {
+        const uint32_t tb = bit_of(o, tgt);
+        const uint32_t cm = ((1u << o.n) - 1u) & ~tb;
+        col[((in & cm) == cm) ? (in ^ tb) : in] = 1.0;
+    });
+}
+
+PassOp pass_gate(const suqa_hip_gate& g) {
+    PassOp p;
+    p.op = gate_op(g);
+    if (g.kind == SUQA_GATE_CX || g.kind == SUQA_GATE_MCX) {
+        for (uint32_t k = 0; k < g.nctrl; ++k) p.ctrl |= 1ull << g.ctrl[k];
+        p.tgt = g.target;
+    } else if (g.kind == SUQA_GATE_RZ || g.kind == SUQA_GATE_U1) {
+        p.diag = true;
+        p.dq = g.target;
+        p.d0 = p.op.m[0];
+        p.d1 = p.op.m[3];
+    }
+    return p;
+}
+
 // exp(i phi P) = cos(phi) I + i sin(phi) P with P|x> = prod_k f_k |x ^ flips>,
 // f = 1 (X), i (-1)^b (Y), (-1)^b (Z), on the string's non-identity qubits;
 // the gc_mask qubits become the op's controls.

@@ -246,12 +246,22 @@ struct PassBlock {
     const float* A;        // MFMA-ordered stacked matrix
 };
 
+// A per-tile scalar of a pass: multiplies the tiles whose base meets
+// (base & mo) == vo (qubits outside the tile).
+struct PassScal {
+    uint64_t mo, vo;
+    double re, im;
+};
+constexpr uint32_t kPassScal = 24;   // scalars per pass
+
 struct PassArgs {
     uint32_t t, nblocks;
     uint64_t tmask;        // the tile's state bits (tile element e -> deposit(e, tmask))
     uint64_t freemask;     // state bits enumerated by the tile number
     uint64_t fixval;       // controls common to every block of the launch
     uint64_t ntiles;
+    uint32_t K, nscal;     // launch width; scalars applied when the tile is stored
+    PassScal sc[kPassScal];
     PassBlock b[kPassBlocks];
 };
 
@@ -379,15 +389,34 @@ __global__ __launch_bounds__(kBlock) void k_bpass(float* __restrict__ re, float*
         if ((base & B.mo) != B.vo) continue;   // uniform: a control of this tile is 0
         pass_block<K>(lre, lim, a.t, B);
     }
+    // the product of the scalars that select this tile, in FP64, rounded once
+    const bool scale = a.nscal != 0;
+    float fr = 1.f, fi = 0.f;
+    if (scale) {
+        double pr = 1.0, pi = 0.0;
+        for (uint32_t k = 0; k < a.nscal; ++k)
+            if ((base & a.sc[k].mo) == a.sc[k].vo) {
+                const double r = pr * a.sc[k].re - pi * a.sc[k].im;
+                pi = pr * a.sc[k].im + pi * a.sc[k].re;
+                pr = r;
+            }
+        fr = (float)pr;
+        fi = (float)pi;
+    }
 #pragma unroll
     for (uint32_t i = 0; i < kCh; ++i)
         if (tid + i * kBlock < tsz / 4) {
             const uint64_t g = toff | chunk_off(i);
             const uint32_t sw = lds_sw(4 * (tid + i * kBlock));
-            __builtin_nontemporal_store(perm4(*reinterpret_cast<const floatx4*>(lre + (sw & ~3u)), sw & 3u),
-                                        reinterpret_cast<floatx4*>(re + g));
-            __builtin_nontemporal_store(perm4(*reinterpret_cast<const floatx4*>(lim + (sw & ~3u)), sw & 3u),
-                                        reinterpret_cast<floatx4*>(im + g));
+            floatx4 vr = perm4(*reinterpret_cast<const floatx4*>(lre + (sw & ~3u)), sw & 3u);
+            floatx4 vi = perm4(*reinterpret_cast<const floatx4*>(lim + (sw & ~3u)), sw & 3u);
+            if (scale) {
+                const floatx4 nr = vr * fr - vi * fi;
+                vi = vr * fi + vi * fr;
+                vr = nr;
+            }
+            __builtin_nontemporal_store(vr, reinterpret_cast<floatx4*>(re + g));
+            __builtin_nontemporal_store(vi, reinterpret_cast<floatx4*>(im + g));
         }
 }
 
@@ -670,7 +699,7 @@ int flush_passes(suqa_hip_f32* c, std::vector<PassLaunch>& L, std::vector<float>
         for (uint32_t k = 0; k < l.a.nblocks; ++k) l.a.b[k].A = c->d_mats + l.off[k];
         const uint32_t g = (uint32_t)(((l.a.ntiles + 7) / 8) * 8);
         const size_t lds = sizeof(float) * 2 * ((size_t)1 << l.a.t);
-        switch (l.a.b[0].K) {
+        switch (l.a.K) {
         case 4: hipLaunchKernelGGL(k_bpass<4>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, l.a); break;
         case 5: hipLaunchKernelGGL(k_bpass<5>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, l.a); break;
         case 6: hipLaunchKernelGGL(k_bpass<6>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, l.a); break;
@@ -686,13 +715,19 @@ int flush_passes(suqa_hip_f32* c, std::vector<PassLaunch>& L, std::vector<float>
 
 // One launch of blocks [lo, hi) of a pass on tile qubits T.
 int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, size_t hi, uint64_t T, uint32_t t,
-               std::vector<PassLaunch>& L, std::vector<float>& buf) {
+               uint32_t width, const std::vector<PassScal>* scal, std::vector<PassLaunch>& L,
+               std::vector<float>& buf) {
     const uint32_t nq = c->nq;
     PassLaunch l;
     std::memset(&l, 0, sizeof(l));
     PassArgs& a = l.a;
     a.t = t;
     a.tmask = T;
+    a.K = width;
+    if (scal) {
+        a.nscal = (uint32_t)scal->size();
+        for (uint32_t k = 0; k < a.nscal; ++k) a.sc[k] = (*scal)[k];
+    }
     uint32_t k = 0;
     uint32_t pos[64] = {};   // tile position of each tile qubit
     for (uint32_t q = 0; q < nq; ++q)
@@ -731,7 +766,9 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
         buf.resize(buf.size() + D * D);
         mfma_order(b, buf.data() + l.off[i - lo]);
     }
-    // controls shared by every block: tiles where one of them is 0 are not visited
+    // controls shared by every block: tiles where one of them is 0 are not
+    // visited (unless the launch applies scalars, which every tile may need)
+    if (a.nblocks == 0 || a.nscal != 0) mcom = 0;
     a.freemask = ((1ull << nq) - 1ull) & ~(T | mcom);
     a.fixval = mcom;
     a.ntiles = 1ull << __builtin_popcountll(a.freemask);
@@ -740,10 +777,13 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
 }
 
 // Applies ops in order: cut into passes of consecutive ops whose targets fit a
-// tile of t qubits (controls outside it), each pass fused into width-qubit
-// blocks inside its tile and applied by k_bpass, several blocks per HBM round
-// trip.
-int apply_ops(suqa_hip_f32* c, const std::vector<Op>& ops, uint32_t width, uint32_t* nblocks) {
+// tile of t qubits, each pass fused into width-qubit blocks inside its tile
+// and applied by k_bpass, several blocks per HBM round trip.  What may stay
+// outside the tile does: the controls of a controlled X that are not tile
+// qubits become its block's control mask, and a one-qubit diagonal on a qubit
+// outside the tile becomes per-tile scalars (PassOp).  A qubit kept outside a
+// pass as a control or a scalar's selector cannot be a target later in it.
+int apply_ops(suqa_hip_f32* c, const std::vector<PassOp>& items, uint32_t width, uint32_t* nblocks) {
     const uint32_t nq = c->nq;
     const uint32_t t = std::min<uint32_t>(kPassTileBits, nq);
     if (t < width + 5) return fail(SUQA_ERR_ARG, "state too small for this block width");
@@ -751,19 +791,42 @@ int apply_ops(suqa_hip_f32* c, const std::vector<Op>& ops, uint32_t width, uint3
     std::vector<float> buf;
     uint32_t total = 0;
     size_t i = 0;
-    while (i < ops.size()) {
+    while (i < items.size()) {
         uint64_t T = 0;   // the 5 lowest qubits that are not controls of the first op
         for (uint32_t q = 0; q < nq && __builtin_popcountll(T) < 5; ++q)
-            if (!((ops[i].cmask >> q) & 1ull)) T |= 1ull << q;
-        uint64_t C = 0;
+            if (!((items[i].op.cmask >> q) & 1ull)) T |= 1ull << q;
+        uint64_t C = 0;   // qubits kept outside the tile by this pass
+        std::vector<Op> part;
+        std::vector<PassScal> scal;
         size_t j = i;
-        while (j < ops.size() && j - i < 64) {
+        while (j < items.size() && j - i < 64) {
+            const PassOp& it = items[j];
+            // at most nq - t qubits can stay outside a tile of t qubits
+            auto may_keep_out = [&](uint64_t m) { return (uint32_t)__builtin_popcountll(C | m) <= nq - t; };
+            if (it.diag && !((T >> it.dq) & 1ull) && may_keep_out(1ull << it.dq)) {
+                const uint64_t bq = 1ull << it.dq;
+                const uint32_t need = (it.d0 != 1.0 ? 1u : 0u) + (it.d1 != 1.0 ? 1u : 0u);
+                if (scal.size() + need > kPassScal) break;
+                if (it.d0 != 1.0) scal.push_back(PassScal{bq, 0ull, it.d0.real(), it.d0.imag()});
+                if (it.d1 != 1.0) scal.push_back(PassScal{bq, bq, it.d1.real(), it.d1.imag()});
+                C |= bq;
+                ++j;
+                continue;
+            }
+            Op op = it.op;
+            if (it.ctrl) {
+                const uint64_t inside = it.ctrl & T, outside = it.ctrl & ~T;
+                if (outside && may_keep_out(outside)) {
+                    op = cx_op(inside, it.tgt);
+                    op.cmask = outside;
+                }
+            }
             uint64_t G = 0;
-            for (uint32_t k2 = 0; k2 < ops[j].n; ++k2) G |= 1ull << ops[j].qs[k2];
-            const uint64_t Cj = ops[j].cmask;
-            if ((Cj & (T | G)) != 0 || (G & C) != 0 || (uint32_t)__builtin_popcountll(T | G) > t) break;
+            for (uint32_t k2 = 0; k2 < op.n; ++k2) G |= 1ull << op.qs[k2];
+            if ((op.cmask & (T | G)) != 0 || (G & C) != 0 || (uint32_t)__builtin_popcountll(T | G) > t) break;
             T |= G;
-            C |= Cj;
+            C |= op.cmask;
+            part.push_back(std::move(op));
             ++j;
         }
         if (j == i) return fail(SUQA_ERR_ARG, "an op does not fit a tile");
@@ -772,13 +835,15 @@ int apply_ops(suqa_hip_f32* c, const std::vector<Op>& ops, uint32_t width, uint3
         for (uint32_t q = 0; q < nq && (uint32_t)__builtin_popcountll(T) < t; ++q)
             if (!((C >> q) & 1ull)) T |= 1ull << q;
         if ((uint32_t)__builtin_popcountll(T) < t) return fail(SUQA_ERR_ARG, "state too small for a tile besides the controls");
-        const std::vector<Op> part(ops.begin() + (long)i, ops.begin() + (long)j);
         const auto blocks = fuse(part, nq, width, 256, T);
         total += (uint32_t)blocks.size();
-        for (size_t lo = 0; lo < blocks.size(); lo += kPassBlocks) {
-            add_launch(c, blocks, lo, std::min(blocks.size(), lo + kPassBlocks), T, t, L, buf);
+        // the scalars ride on the pass's last launch (they commute with its blocks)
+        for (size_t lo = 0; lo < blocks.size() || (lo == 0 && !scal.empty()); lo += kPassBlocks) {
+            const size_t hi = std::min(blocks.size(), lo + kPassBlocks);
+            add_launch(c, blocks, lo, hi, T, t, width, hi == blocks.size() ? &scal : nullptr, L, buf);
             if (buf.size() > (size_t)4 << 20)   // 16 MiB of matrices per upload
                 if (int rc = flush_passes(c, L, buf)) return rc;
+            if (blocks.empty()) break;
         }
         i = j;
     }
@@ -788,11 +853,20 @@ int apply_ops(suqa_hip_f32* c, const std::vector<Op>& ops, uint32_t width, uint3
 
 // Applies ops in the context's mode: block passes, or the whole list fused
 // into blocks applied one launch each.
-int run_ops(suqa_hip_f32* c, const std::vector<Op>& ops, uint32_t width, uint32_t* nblocks) {
-    if (c->passes) return apply_ops(c, ops, width, nblocks);
+int run_ops(suqa_hip_f32* c, const std::vector<PassOp>& items, uint32_t width, uint32_t* nblocks) {
+    if (c->passes) return apply_ops(c, items, width, nblocks);
+    std::vector<Op> ops;
+    ops.reserve(items.size());
+    for (const PassOp& it : items) ops.push_back(it.op);
     const auto blocks = fuse(ops, c->nq, width, 256);
     if (nblocks) *nblocks = (uint32_t)blocks.size();
     return apply_blocks(c, blocks);
+}
+
+std::vector<PassOp> as_pass_ops(std::vector<Op>&& ops) {
+    std::vector<PassOp> v(ops.size());
+    for (size_t k = 0; k < ops.size(); ++k) v[k].op = std::move(ops[k]);
+    return v;
 }
 
 // Workload gates must name distinct in-range qubits (a block's tile geometry
@@ -881,10 +955,10 @@ int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t
     if (!c || !c->d_data || (ngates && !gates)) return fail(SUQA_ERR_ARG, "null argument");
     if (!width_ok(c->nq, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
     if (int rc = check_gates(c, gates, ngates)) return rc;
-    std::vector<Op> ops;
-    ops.reserve(ngates);
-    for (uint32_t i = 0; i < ngates; ++i) ops.push_back(gate_op(gates[i]));
-    return run_ops(c, ops, width, nblocks);
+    std::vector<PassOp> items;
+    items.reserve(ngates);
+    for (uint32_t i = 0; i < ngates; ++i) items.push_back(pass_gate(gates[i]));
+    return run_ops(c, items, width, nblocks);
 }
 
 int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint32_t nops, uint32_t width,
@@ -900,7 +974,7 @@ int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint3
         // tiles move 4 consecutive amplitudes: qubits 0 and 1 are always tile qubits
         if (list[i].cmask & 3ull) return fail(SUQA_ERR_ARG, "controls on qubits 0 or 1 are not supported by the FP32 engine");
     }
-    return run_ops(c, list, width, nblocks);
+    return run_ops(c, as_pass_ops(std::move(list)), width, nblocks);
 }
 
 int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, uint32_t width) {
@@ -913,7 +987,7 @@ int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, ui
     }
     std::vector<Op> list;
     qft_inverse_ops(qs, n, list);
-    return run_ops(c, list, width, nullptr);
+    return run_ops(c, as_pass_ops(std::move(list)), width, nullptr);
 }
 
 int suqa_hip_f32_set_passes(suqa_hip_f32* c, uint32_t on) {
@@ -933,9 +1007,9 @@ int suqa_hip_f32_autotune(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t 
                           double* ms_per_gate) {
     if (!c || !c->d_data || !best_width || !gates || !ngates) return fail(SUQA_ERR_ARG, "null argument or empty sample");
     if (int rc = check_gates(c, gates, ngates)) return rc;
-    std::vector<Op> ops;
-    for (uint32_t i = 0; i < ngates; ++i) ops.push_back(gate_op(gates[i]));
-    const std::vector<Op> warm(ops.begin(), ops.begin() + std::min<size_t>(ops.size(), 8));
+    std::vector<PassOp> ops;
+    for (uint32_t i = 0; i < ngates; ++i) ops.push_back(pass_gate(gates[i]));
+    const std::vector<PassOp> warm(ops.begin(), ops.begin() + std::min<size_t>(ops.size(), 8));
     double best = 1e300;
     uint32_t best_mode = 0;
     *best_width = 0;

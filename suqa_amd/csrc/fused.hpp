@@ -51,6 +51,26 @@ Op gate_op(const suqa_hip_gate& g);
 // the gc_mask qubits (Op::cmask); false (and no op) if it is malformed or
 // acts on more than kMaxK qubits.
 bool pauli_op(const suqa_hip_pauli_op& p, Op& out);
+// An op as the pass builder of the block passes sees it: the dense form, and
+// the parts a pass may keep outside its tile instead.
+//  - ctrl != 0: a multi-controlled X on `tgt`; controls outside the tile
+//    become the block's control mask (cx_op over the target and the inside
+//    controls).
+//  - diag: diag(d0, d1) on qubit dq; outside the tile it is a per-tile scalar
+//    (d0 where the bit is 0, d1 where it is 1), which commutes with every op
+//    of the pass (no op of a pass targets a qubit outside its tile).
+struct PassOp {
+    Op op;
+    uint64_t ctrl = 0;
+    uint32_t tgt = 0;
+    bool diag = false;
+    uint32_t dq = 0;
+    cd d0 = 1.0, d1 = 1.0;
+};
+PassOp pass_gate(const suqa_hip_gate& g);
+// X on tgt controlled by the qubits of `ctrl` (all 1), dense over tgt + ctrl.
+Op cx_op(uint64_t ctrl, uint32_t tgt);
+
 // H on q and the controlled phase diag(1,1,1,e^{i phase}) on {a, b}.
 Op h_op(uint32_t q);
 Op cphase_op(uint32_t a, uint32_t b, double phase);
