@@ -199,6 +199,17 @@ std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width,
     while (first < n) {
         while (first < n && used[first]) ++first;
         if (first >= n) break;
+        if (ops[first].swap) {   // a controlled X on one qubit: its own swap block
+            Block b;
+            b.K = 1;
+            b.qubits[0] = ops[first].qs[0];
+            b.cmask = ops[first].cmask;
+            b.swap = true;
+            b.ngates = 1;
+            used[first] = 1;
+            out.push_back(std::move(b));
+            continue;
+        }
         const uint64_t cm = ops[first].cmask;
         uint64_t S = 0, lt = 0, lc = 0;   // block targets; targets / controls of ops left out
         std::vector<uint32_t> members;
@@ -208,7 +219,7 @@ std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width,
             ++scanned;
             const uint64_t G = gmask[i], C = ops[i].cmask;
             const bool commutes = (G & (lt | lc)) == 0 && (C & lt) == 0;
-            if (commutes && C == cm && (uint32_t)__builtin_popcountll(S | G) <= width) {
+            if (!ops[i].swap && commutes && C == cm && (uint32_t)__builtin_popcountll(S | G) <= width) {
                 S |= G;
                 members.push_back(i);
                 used[i] = 1;

@@ -238,7 +238,8 @@ __host__ __device__ inline uint32_t lds_sw(uint32_t e) {
 }
 
 struct PassBlock {
-    uint32_t K;
+    uint32_t K;                               // 0: a swap along tile bit pbit
+    uint32_t pbit;
     uint32_t rowsw[kMaxK];                    // lds_sw(1 << p) of the block positions, ascending
     uint32_t colsw[kPassTileBits - kMinK];    // lds_sw(1 << p) of column bit j (bits 0..4: the lanes)
     uint32_t pad_;
@@ -350,7 +351,24 @@ __device__ __forceinline__ void pass_block(float* lre, float* lim, uint32_t t, c
     __syncthreads();
 }
 
-// All blocks of a launch have K = the fusion width.
+// A controlled X on tile position p (rowsw[0] = lds_sw(1 << p)): swaps the
+// two halves of the tile along p (the controls were tested per tile).
+__device__ __forceinline__ void pass_swap(float* lre, float* lim, uint32_t t, const PassBlock& B) {
+    const uint32_t tsz = 1u << t, bit = B.pbit, swb = B.rowsw[0];
+    const uint32_t lo = bit - 1u;
+    for (uint32_t k = threadIdx.x; k < tsz / 2; k += kBlock) {
+        const uint32_t i = ((k & ~lo) << 1) | (k & lo);   // bit p of i is 0
+        const uint32_t a = lds_sw(i), b = a ^ swb;
+        const float r = lre[a], m = lim[a];
+        lre[a] = lre[b];
+        lim[a] = lim[b];
+        lre[b] = r;
+        lim[b] = m;
+    }
+    __syncthreads();
+}
+
+// All MFMA blocks of a launch have K = the fusion width; K = 0 marks a swap.
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_bpass(float* __restrict__ re, float* __restrict__ im, const PassArgs a) {
     extern __shared__ float lds[];
@@ -387,7 +405,8 @@ __global__ __launch_bounds__(kBlock) void k_bpass(float* __restrict__ re, float*
     for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
         const PassBlock& B = a.b[bi];
         if ((base & B.mo) != B.vo) continue;   // uniform: a control of this tile is 0
-        pass_block<K>(lre, lim, a.t, B);
+        if (B.K == 0) pass_swap(lre, lim, a.t, B);
+        else pass_block<K>(lre, lim, a.t, B);
     }
     // the product of the scalars that select this tile, in FP64, rounded once
     const bool scale = a.nscal != 0;
@@ -737,6 +756,15 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
     for (size_t i = lo; i < hi; ++i) {
         const Block& b = blocks[i];
         PassBlock& B = a.b[i - lo];
+        B.mo = B.vo = b.cmask;
+        mcom &= b.cmask;
+        if (b.swap) {
+            B.K = 0;
+            B.pbit = 1u << pos[b.qubits[0]];
+            B.rowsw[0] = lds_sw(B.pbit);
+            l.off[i - lo] = 0;
+            continue;
+        }
         B.K = b.K;
         uint32_t inblock = 0;
         for (uint32_t j = 0; j < b.K; ++j) {
@@ -758,8 +786,6 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
         }
         for (uint32_t p = 0; p < t; ++p)
             if (!(((inblock | taken) >> p) & 1u)) B.colsw[nc++] = lds_sw(1u << p);
-        B.mo = B.vo = b.cmask;
-        mcom &= b.cmask;
         // MFMA-ordered stacked matrix, then the tile offsets of the block rows
         const size_t D = (size_t)2 << b.K;
         l.off[i - lo] = buf.size();
@@ -819,6 +845,7 @@ int apply_ops(suqa_hip_f32* c, const std::vector<PassOp>& items, uint32_t width,
                 if (outside && may_keep_out(outside)) {
                     op = cx_op(inside, it.tgt);
                     op.cmask = outside;
+                    op.swap = inside == 0;   // X on the target alone: a swap in LDS
                 }
             }
             uint64_t G = 0;

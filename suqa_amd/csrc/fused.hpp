@@ -23,6 +23,7 @@ struct Op {
     uint32_t qs[kMaxK] = {};
     std::vector<cd> m;
     uint64_t cmask = 0;   // control qubits (all must be 1), outside the matrix
+    bool swap = false;    // X on qs[0] with cmask != 0: applied as a swap, never fused
 };
 
 // One fused block: the same, with exactly K qubits, and how many ops it holds.
@@ -32,6 +33,7 @@ struct Block {
     std::vector<cd> m;
     uint32_t ngates;
     uint64_t cmask = 0;   // common control of the block's ops: applied where all are 1
+    bool swap = false;    // a controlled X (Op::swap) on qubits[0], K = 1, no matrix
 };
 
 // Greedy fusion with commutation lookahead: an op joins the open block if it
@@ -41,7 +43,8 @@ struct Block {
 // controls touch none of their targets (ops that share only controls
 // commute).
 // Blocks are padded to exactly `width` qubits with the lowest free qubits of
-// pad_allowed (and never with a control qubit of the block).
+// pad_allowed (and never with a control qubit of the block).  Swap ops stay
+// single: each becomes a swap block where it is the first op left.
 std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width, uint32_t window,
                         uint64_t pad_allowed = ~0ull);
 
