@@ -220,7 +220,7 @@ def fused_leg(capi, wl, eng, gates, n, layers):
         log(f"fused: {nb} blocks for {layers * n} gates in {dt:.3f} s, fidelity {fid!r}")
         amp_bytes = 16 * (1 << n)     # an FP32 complex amplitude read and written per block
         return f32, {
-            "width": best, "mode": "passes" if f32.passes else "blocks",
+            "width": best, "mode": ["blocks", "passes", "sparse"][f32.passes],
             "ms_per_gate_by_mode_width": {k: (None if v < 0 else round(v, 4)) for k, v in tms.items()},
             "gates": layers * n, "blocks": nb, "gates_per_block": round(layers * n / nb, 2),
             "seconds": round(dt, 4), "layers_per_s": round(layers / dt, 3),
@@ -265,8 +265,9 @@ def trotter_leg(capi, wl, no_fused):
             try:
                 # mode and block width: one controlled Trotter step timed at each
                 tw = {}
-                for mode in ("blocks", "passes"):
-                    f.passes = mode == "passes"
+                modes = {"blocks": 0, "passes": 1, "sparse": 2}
+                for mode in modes:
+                    f.passes = modes[mode]
                     for wdt in (4, 5, 6):
                         f.init_state()
                         f.run_paulis(ops, wdt)
@@ -276,7 +277,7 @@ def trotter_leg(capi, wl, no_fused):
                         f.synchronize()
                         tw[f"{mode}{wdt}"] = time.perf_counter() - t0
                 best = min(tw, key=tw.get)
-                f.passes = best.startswith("passes")
+                f.passes = modes[best[:-1]]
                 width = int(best[-1])
                 f.init_state()
                 f.run_gates(capi.gates_from([wl.G(wl.H, top, [], 0.0)]), 6)

@@ -228,8 +228,11 @@ int suqa_hip_f32_init_state(suqa_hip_f32* c);
 /* Application mode of the fused blocks: 0 (the default) fuses the whole list
  * and applies each block in its own launch (one HBM round trip per block);
  * 1 cuts the list into passes of consecutive ops whose targets fit a 12-qubit
- * tile, fuses each pass and applies all of its blocks in one round trip.
- * Results agree to FP32 rounding (the fusion differs). */
+ * tile, fuses each pass and applies all of its blocks in one round trip;
+ * 2 is mode 1 with the Pauli rotations of suqa_hip_f32_run_paulis applied as
+ * pair updates in LDS instead of fused MFMA blocks (a rotation has two
+ * nonzeros per row).  Results agree to FP32 rounding (the arithmetic
+ * differs). */
 int suqa_hip_f32_set_passes(suqa_hip_f32* c, uint32_t on);
 int suqa_hip_f32_get_passes(const suqa_hip_f32* c, uint32_t* on);
 /* Fuses gates[0..ngates) into width-qubit blocks and applies them in order;

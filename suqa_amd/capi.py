@@ -383,14 +383,15 @@ class F32Engine:
 
     @property
     def passes(self):
-        """Application mode: False = one block per launch, True = block passes."""
+        """Application mode: 0 = one block per launch, 1 = block passes,
+        2 = block passes with sparse Pauli rotations (True/False are 1/0)."""
         v = U32()
         check32(lib().suqa_hip_f32_get_passes(self.ctx, C.byref(v)))
-        return bool(v.value)
+        return v.value
 
     @passes.setter
-    def passes(self, on):
-        check32(lib().suqa_hip_f32_set_passes(self.ctx, 1 if on else 0))
+    def passes(self, mode):
+        check32(lib().suqa_hip_f32_set_passes(self.ctx, int(mode)))
 
     def autotune(self, gates, n=None):
         """Times the sample at widths 4..6 in both modes, leaves the fastest mode

@@ -162,6 +162,25 @@ bool pauli_op(const suqa_hip_pauli_op& p, Op& out) {
     return true;
 }
 
+void pauli_rot(const suqa_hip_pauli_op& p, Op& out) {
+    // P|x> = i^ny (-1)^{|x & (Y|Z)|} |x ^ (X|Y)>, so
+    // exp(i phi P) psi (y) = cos phi psi(y) + i sin phi i^ny (-1)^{|(y ^ flips) & (Y|Z)|} psi(y ^ flips)
+    uint32_t ny = 0;
+    out.rflip = out.rpar = 0;
+    for (uint32_t k = 0; k < p.w; ++k) {
+        const uint64_t b = 1ull << p.qs[k];
+        if (p.types[k] == SUQA_PAULI_X || p.types[k] == SUQA_PAULI_Y) out.rflip |= b;
+        if (p.types[k] == SUQA_PAULI_Y || p.types[k] == SUQA_PAULI_Z) out.rpar |= b;
+        if (p.types[k] == SUQA_PAULI_Y) ++ny;
+    }
+    const double s = std::sin(p.phase);
+    static const cd ipow[4] = {cd(0, 1), cd(-1, 0), cd(0, -1), cd(1, 0)};   // i^(ny + 1)
+    out.rot = true;
+    out.rc = std::cos(p.phase);
+    out.rk = s * ipow[ny & 3u];
+    out.m.clear();
+}
+
 Op h_op(uint32_t q) {
     suqa_hip_gate g{};
     g.kind = SUQA_GATE_H;
@@ -199,12 +218,13 @@ std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width,
     while (first < n) {
         while (first < n && used[first]) ++first;
         if (first >= n) break;
-        if (ops[first].swap) {   // a controlled X on one qubit: its own swap block
+        if (ops[first].swap || ops[first].rot) {   // a controlled X or a sparse rotation: its own block
             Block b;
             b.K = 1;
             b.qubits[0] = ops[first].qs[0];
             b.cmask = ops[first].cmask;
-            b.swap = true;
+            b.swap = ops[first].swap;
+            b.rot = ops[first].rot ? &ops[first] : nullptr;
             b.ngates = 1;
             used[first] = 1;
             out.push_back(std::move(b));
@@ -219,7 +239,7 @@ std::vector<Block> fuse(const std::vector<Op>& ops, uint32_t nq, uint32_t width,
             ++scanned;
             const uint64_t G = gmask[i], C = ops[i].cmask;
             const bool commutes = (G & (lt | lc)) == 0 && (C & lt) == 0;
-            if (!ops[i].swap && commutes && C == cm && (uint32_t)__builtin_popcountll(S | G) <= width) {
+            if (!ops[i].swap && !ops[i].rot && commutes && C == cm && (uint32_t)__builtin_popcountll(S | G) <= width) {
                 S |= G;
                 members.push_back(i);
                 used[i] = 1;

@@ -48,7 +48,7 @@ struct suqa_hip_f32 {
     double* h_result = nullptr;
     float* d_mats = nullptr;   // block matrices in MFMA lane order
     size_t mats_cap = 0;       // floats
-    uint32_t passes = 0;       // 0: one block per launch, 1: block passes
+    uint32_t passes = 0;       // 0: one block per launch, 1: block passes, 2: passes with sparse Pauli rotations
 };
 
 namespace sqf {
@@ -238,8 +238,11 @@ __host__ __device__ inline uint32_t lds_sw(uint32_t e) {
 }
 
 struct PassBlock {
-    uint32_t K;                               // 0: a swap along tile bit pbit
+    uint32_t K;                               // 0: a swap along tile bit pbit, or a rotation (rot)
     uint32_t pbit;
+    uint32_t rot;                             // 1: sparse Pauli rotation, pbit its lowest flipped bit (0: diagonal)
+    uint32_t rflip, rpar;                     // rotation: flipped and parity tile bits
+    float rc, rkr, rki;                       // rotation: psi' = rc psi(y) + rk (-1)^{|(y^rflip)&rpar|} psi(y^rflip)
     uint32_t rowsw[kMaxK];                    // lds_sw(1 << p) of the block positions, ascending
     uint32_t colsw[kPassTileBits - kMinK];    // lds_sw(1 << p) of column bit j (bits 0..4: the lanes)
     uint32_t pad_;
@@ -368,7 +371,43 @@ __device__ __forceinline__ void pass_swap(float* lre, float* lim, uint32_t t, co
     __syncthreads();
 }
 
-// All MFMA blocks of a launch have K = the fusion width; K = 0 marks a swap.
+// A Pauli rotation exp(i phi P) on the tile as pair updates (B.rot):
+// psi'(y) = rc psi(y) + rk (-1)^{|(y ^ rflip) & rpar|} psi(y ^ rflip), with
+// rk = i^(ny+1) sin phi; a diagonal string (no flipped bit) scales each
+// amplitude by rc + rk (-1)^{|y & rpar|}.
+__device__ __forceinline__ void pass_rot(float* lre, float* lim, uint32_t t, const PassBlock& B) {
+    const uint32_t tsz = 1u << t;
+    const float c = B.rc, kr = B.rkr, ki = B.rki;
+    if (B.pbit == 0) {
+        for (uint32_t y = threadIdx.x; y < tsz; y += kBlock) {
+            const bool odd = (__popc(y & B.rpar) & 1u) != 0u;
+            const float fr = odd ? c - kr : c + kr, fi = odd ? -ki : ki;
+            const uint32_t a = lds_sw(y);
+            const float r = lre[a], m = lim[a];
+            lre[a] = r * fr - m * fi;
+            lim[a] = r * fi + m * fr;
+        }
+    } else {
+        const uint32_t lo = B.pbit - 1u, swf = lds_sw(B.rflip);
+        for (uint32_t k = threadIdx.x; k < tsz / 2; k += kBlock) {
+            const uint32_t y0 = ((k & ~lo) << 1) | (k & lo), y1 = y0 ^ B.rflip;   // pivot bit of y0 is 0
+            const uint32_t a0 = lds_sw(y0), a1 = a0 ^ swf;
+            const float ar = lre[a0], ai = lim[a0], br = lre[a1], bi = lim[a1];
+            const float s0 = (__popc(y0 & B.rpar) & 1u) ? -1.f : 1.f;
+            const float s1 = (__popc(y1 & B.rpar) & 1u) ? -1.f : 1.f;
+            const float tbr = s1 * (kr * br - ki * bi), tbi = s1 * (kr * bi + ki * br);
+            const float tar = s0 * (kr * ar - ki * ai), tai = s0 * (kr * ai + ki * ar);
+            lre[a0] = c * ar + tbr;
+            lim[a0] = c * ai + tbi;
+            lre[a1] = c * br + tar;
+            lim[a1] = c * bi + tai;
+        }
+    }
+    __syncthreads();
+}
+
+// All MFMA blocks of a launch have K = the fusion width; K = 0 marks a swap
+// or a sparse rotation.
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_bpass(float* __restrict__ re, float* __restrict__ im, const PassArgs a) {
     extern __shared__ float lds[];
@@ -405,8 +444,9 @@ __global__ __launch_bounds__(kBlock) void k_bpass(float* __restrict__ re, float*
     for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
         const PassBlock& B = a.b[bi];
         if ((base & B.mo) != B.vo) continue;   // uniform: a control of this tile is 0
-        if (B.K == 0) pass_swap(lre, lim, a.t, B);
-        else pass_block<K>(lre, lim, a.t, B);
+        if (B.K != 0) pass_block<K>(lre, lim, a.t, B);
+        else if (B.rot) pass_rot(lre, lim, a.t, B);
+        else pass_swap(lre, lim, a.t, B);
     }
     // the product of the scalars that select this tile, in FP64, rounded once
     const bool scale = a.nscal != 0;
@@ -765,6 +805,24 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
             l.off[i - lo] = 0;
             continue;
         }
+        if (b.rot) {   // qubit masks -> tile positions
+            auto tile_bits = [&](uint64_t m) {
+                uint32_t r = 0;
+                for (uint32_t q = 0; q < nq; ++q)
+                    if ((m >> q) & 1ull) r |= 1u << pos[q];
+                return r;
+            };
+            B.K = 0;
+            B.rot = 1;
+            B.rflip = tile_bits(b.rot->rflip);
+            B.rpar = tile_bits(b.rot->rpar);
+            B.pbit = B.rflip & (0u - B.rflip);
+            B.rc = (float)b.rot->rc;
+            B.rkr = (float)b.rot->rk.real();
+            B.rki = (float)b.rot->rk.imag();
+            l.off[i - lo] = 0;
+            continue;
+        }
         B.K = b.K;
         uint32_t inblock = 0;
         for (uint32_t j = 0; j < b.K; ++j) {
@@ -1000,6 +1058,7 @@ int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint3
         if (list[i].cmask >> c->nq) return fail(SUQA_ERR_ARG, "control qubit out of range");
         // tiles move 4 consecutive amplitudes: qubits 0 and 1 are always tile qubits
         if (list[i].cmask & 3ull) return fail(SUQA_ERR_ARG, "controls on qubits 0 or 1 are not supported by the FP32 engine");
+        if (c->passes == 2 && list[i].n) pauli_rot(ops[i], list[i]);
     }
     return run_ops(c, as_pass_ops(std::move(list)), width, nblocks);
 }
@@ -1019,7 +1078,7 @@ int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, ui
 
 int suqa_hip_f32_set_passes(suqa_hip_f32* c, uint32_t on) {
     if (!c) return fail(SUQA_ERR_ARG, "null context");
-    if (on > 1) return fail(SUQA_ERR_ARG, "mode must be 0 (one block per launch) or 1 (block passes)");
+    if (on > 2) return fail(SUQA_ERR_ARG, "mode must be 0 (one block per launch), 1 (block passes) or 2 (block passes, sparse Pauli rotations)");
     c->passes = on;
     return SUQA_OK;
 }

@@ -24,6 +24,12 @@ struct Op {
     std::vector<cd> m;
     uint64_t cmask = 0;   // control qubits (all must be 1), outside the matrix
     bool swap = false;    // X on qs[0] with cmask != 0: applied as a swap, never fused
+    // a Pauli rotation applied as pair updates, never fused (block passes with
+    // sparse rotations): psi'(y) = rc psi(y) + rk (-1)^{|(y ^ rflip) & rpar|} psi(y ^ rflip)
+    bool rot = false;
+    uint64_t rflip = 0, rpar = 0;
+    double rc = 1.0;
+    cd rk = 0.0;
 };
 
 // One fused block: the same, with exactly K qubits, and how many ops it holds.
@@ -34,6 +40,7 @@ struct Block {
     uint32_t ngates;
     uint64_t cmask = 0;   // common control of the block's ops: applied where all are 1
     bool swap = false;    // a controlled X (Op::swap) on qubits[0], K = 1, no matrix
+    const Op* rot = nullptr;   // a sparse Pauli rotation (Op::rot), no matrix
 };
 
 // Greedy fusion with commutation lookahead: an op joins the open block if it
@@ -54,6 +61,8 @@ Op gate_op(const suqa_hip_gate& g);
 // the gc_mask qubits (Op::cmask); false (and no op) if it is malformed or
 // acts on more than kMaxK qubits.
 bool pauli_op(const suqa_hip_pauli_op& p, Op& out);
+// The same rotation as a sparse op (Op::rot) on the same qubits.
+void pauli_rot(const suqa_hip_pauli_op& p, Op& out);
 // An op as the pass builder of the block passes sees it: the dense form, and
 // the parts a pass may keep outside its tile instead.
 //  - ctrl != 0: a multi-controlled X on `tgt`; controls outside the tile

@@ -115,7 +115,7 @@ def test_fused_rejects_bad_arguments(capi):
     with pytest.raises(capi.SuqaError):
         f32.measure(10, 0.5)
     with pytest.raises(capi.SuqaError):
-        capi.check32(capi.lib().suqa_hip_f32_set_passes(f32.ctx, 2))   # modes are 0 and 1
+        capi.check32(capi.lib().suqa_hip_f32_set_passes(f32.ctx, 3))   # modes are 0, 1 and 2
     e64 = capi.Engine(11)
     with pytest.raises(capi.SuqaError):
         f32.fidelity(e64)

@@ -1,5 +1,6 @@
 """Trotter + phase estimation on the GPU (FP64 engine, one sweep per
-controlled Pauli rotation; FP32 engine, rotations fused into blocks) against
+controlled Pauli rotation; FP32 engine, rotations fused into MFMA blocks or,
+in the sparse mode, applied as pair updates inside block passes) against
 the oracle at a reduced size (10 system + 4 register qubits, 96 terms of the
 seed-11 generator, dt 0.05, 2 Trotter steps per power), and FP32 against
 FP64 at the full 24 + 6 qubits on one controlled Trotter step.
@@ -53,7 +54,7 @@ def test_qpe_fp64_vs_oracle(capi, oracle_result):
     eng.close()
 
 
-@pytest.mark.parametrize("passes", [False, True], ids=["blocks", "passes"])
+@pytest.mark.parametrize("passes", [0, 1, 2], ids=["blocks", "passes", "sparse"])
 @pytest.mark.parametrize("width", [5, 6])
 def test_qpe_fp32_fused_vs_oracle(capi, oracle_result, width, passes):
     log(f"qpe fp32 10+4 width {width} passes={passes}")
@@ -75,15 +76,15 @@ def test_fp32_rejects_ops_wider_than_width(capi):
         f32.run_paulis(capi.pauli_ops([([0, 1], [1, 3], 0.1, 1 << 1)]), 5)
     with pytest.raises(capi.SuqaError):
         f32.run_paulis(capi.pauli_ops([([0, 0], [1, 1], 0.1, 0)]), 5)      # repeated qubit
-    for ctrl in (0, 1):                   # controls on qubits 0, 1: rejected in both modes, nothing launched
-        for passes in (False, True):
+    for ctrl in (0, 1):                   # controls on qubits 0, 1: rejected in every mode, nothing launched
+        for passes in (0, 1, 2):
             f32.passes = passes
             with pytest.raises(capi.SuqaError):
                 f32.run_paulis(capi.pauli_ops([([4, 5], [1, 3], 0.1, 1 << ctrl)]), 5)
     f32.close()
 
 
-@pytest.mark.parametrize("passes", [False, True], ids=["blocks", "passes"])
+@pytest.mark.parametrize("passes", [0, 1, 2], ids=["blocks", "passes", "sparse"])
 def test_controlled_trotter_step_full_size(capi, passes):
     """24 + 6 qubits: H on register qubit 29, one Trotter step of the 96
     terms controlled on it, FP32 fused vs FP64 (fidelity >= 1 - 1e-6), and
