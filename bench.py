@@ -51,7 +51,8 @@ fidelity between them (FP32 block width timed at 4, 5, 6 on one step); the
 full QPE applies 63 such step groups (120,960 rotations), projected from
 the sample.  Then the whole QPE at full size with 2 Trotter steps per power
 (12,096 rotations) in both precisions: the 64-bin register histogram's L1
-and Linf error of FP32 against FP64.
+and Linf error of FP32 against FP64; with --full-qpe also the whole QPE with
+20 steps per power (120,960 rotations, about 3 minutes) the same way.
 
 and the precision pattern under "extra.fused": the same circuit on an FP32
 state with gates fused into K-qubit blocks (K autotuned over 4..6 on the
@@ -232,7 +233,7 @@ def fused_leg(capi, wl, eng, gates, n, layers):
         raise
 
 
-def trotter_leg(capi, wl, no_fused):
+def trotter_leg(capi, wl, no_fused, full_qpe=False):
     """Controlled Trotter steps of the 24 + 6 qubit QPE: FP64 passes and FP32 fused, one sample."""
     nsys, nreg, dt, steps = 24, 6, 0.05, 20
     n = nsys + nreg
@@ -309,6 +310,24 @@ def trotter_leg(capi, wl, no_fused):
                                      "hist_linf_error": max(diff), "fp64_mode": max(range(64), key=lambda k: h64[k]),
                                      "fp32_mode": max(range(64), key=lambda k: h32[k]), "fp64_hist_sum": sum(h64)}
                 log(f"qpe (2 steps/power): fp64 {t64:.1f} s, fp32 {t32:.1f} s, L1 {sum(diff):.3g}, Linf {max(diff):.3g}")
+                if full_qpe:   # the whole QPE as specified: 20 Trotter steps per power, 120,960 rotations
+                    e.init_state()
+                    t0 = time.perf_counter()
+                    h64 = wl.qpe_trotter(e, nsys, nreg, terms, dt, steps)
+                    t64 = time.perf_counter() - t0
+                    log(f"full qpe fp64: {t64:.1f} s")
+                    f.init_state()
+                    t0 = time.perf_counter()
+                    h32 = wl.qpe_trotter(f, nsys, nreg, terms, dt, steps, width=width)
+                    t32 = time.perf_counter() - t0
+                    diff = [abs(x - y) for x, y in zip(h32, h64)]
+                    res["qpe_full"] = {"rotations": 96 * steps * 63, "fp64_seconds": round(t64, 2),
+                                       "fp32_seconds": round(t32, 2), "fp32_mode": best[:-1],
+                                       "hist_l1_error": sum(diff), "hist_linf_error": max(diff),
+                                       "fp64_mode": max(range(64), key=lambda k: h64[k]),
+                                       "fp32_mode_bin": max(range(64), key=lambda k: h32[k]),
+                                       "fp64_hist": [round(x, 12) for x in h64]}
+                    log(f"full qpe: fp64 {t64:.1f} s, fp32 {t32:.1f} s, L1 {sum(diff):.3g}, Linf {max(diff):.3g}")
             finally:
                 f.close()
         return res
@@ -326,6 +345,8 @@ def main():
     ap.add_argument("--no-qms", action="store_true", help="skip the QMS legs (profiling runs)")
     ap.add_argument("--no-fused", action="store_true", help="skip the FP32 fused leg (profiling runs)")
     ap.add_argument("--no-trotter", action="store_true", help="skip the Trotter + QPE leg (profiling runs)")
+    ap.add_argument("--full-qpe", action="store_true",
+                    help="also run the whole 24 + 6 qubit QPE (20 Trotter steps per power) in FP64 and FP32")
     ap.add_argument("--tile-bits", type=int, default=-1,
                     help="FP64 tile size (0: one sweep per gate); default: autotuned")
     a = ap.parse_args()
@@ -396,7 +417,7 @@ def main():
         f32.close()
     cpu = None if a.no_cpu else cpu_baseline(min(n, 27))
     qres = {} if a.no_qms else qms_legs(a.no_cpu)
-    tres = None if a.no_trotter else trotter_leg(capi, wl, a.no_fused)
+    tres = None if a.no_trotter else trotter_leg(capi, wl, a.no_fused, a.full_qpe)
     fams = {k: dict(launches=v["launches"], ms=round(v["ms"], 3),
                     gbs=round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)) for k, v in rep.items()}
     out = {
