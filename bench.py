@@ -40,6 +40,8 @@ The QMS half of the metric is reported under "extra":
                 probabilities: chain-steps/s, chains whose return codes equal
                 the FP64 run's, and per-beta <E> (8 bins of 512 chains)
                 against the FP64 run in units of its standard error;
+  qms_single_chain_device -- the c1 chain as a batch of one (one wavefront
+                 runs it on the device), its steps/s and rets against qms_plain;
   cpu_baseline_qms -- the oracle on the same single chain, 1 core.
 
 The Trotter + phase-estimation workload under "extra.trotter_qpe": 24 system
@@ -133,6 +135,14 @@ def qms_legs(no_cpu):
                          "accepted": int(plain["accepted"]), "seconds": round(plain["seconds"], 4),
                          "config": "beta=1.0, 1000 steps, r=4 (8 qubits), E in [-1,3], seed 42, reset_each 1"}}
     log(f"qms plain: {steps / plain['seconds']:.1f} steps/s")
+    # the same chain as the only chain of the batched kernel: one wavefront runs the
+    # whole chain on the device (no launches, no host round trips per measurement)
+    one = qms.batch_run([(beta, r, seed)], steps, 1, 100, emin, emax, 10, 100)
+    res["qms_single_chain_device"] = {
+        "steps_per_s": round(steps / one["seconds"], 1), "kernel_seconds": round(one["seconds"], 5),
+        "rets_match_plain": bool(np.array_equal(one["rets"][0], np.asarray(plain["rets"]))),
+        "config": "as qms_plain, run as a batch of one chain (one wavefront)"}
+    log(f"qms single chain on the device: {steps / one['seconds']:.1f} steps/s")
     # the same chain in the two-register layout the workload describes (E_old in
     # its own register: 3 + 2 * 4 + 1 = 12 qubits)
     two = qms.run(beta, steps, 1, r, seed, emin, emax, 10, 100, 100, two_registers=True)

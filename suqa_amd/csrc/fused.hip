@@ -247,6 +247,7 @@ struct PassBlock {
     uint32_t colsw[kPassTileBits - kMinK];    // lds_sw(1 << p) of column bit j (bits 0..4: the lanes)
     uint32_t pad_;
     uint64_t mo, vo;       // controls (outside the tile): applied where (base & mo) == vo
+    uint64_t zo;           // rotation: parity qubits outside the tile (constant over a tile)
     const float* A;        // MFMA-ordered stacked matrix
 };
 
@@ -374,10 +375,11 @@ __device__ __forceinline__ void pass_swap(float* lre, float* lim, uint32_t t, co
 // A Pauli rotation exp(i phi P) on the tile as pair updates (B.rot):
 // psi'(y) = rc psi(y) + rk (-1)^{|(y ^ rflip) & rpar|} psi(y ^ rflip), with
 // rk = i^(ny+1) sin phi; a diagonal string (no flipped bit) scales each
-// amplitude by rc + rk (-1)^{|y & rpar|}.
-__device__ __forceinline__ void pass_rot(float* lre, float* lim, uint32_t t, const PassBlock& B) {
+// amplitude by rc + rk (-1)^{|y & rpar|}.  Z qubits of the string outside the
+// tile add the same parity to every amplitude of the tile: zodd negates rk.
+__device__ __forceinline__ void pass_rot(float* lre, float* lim, uint32_t t, const PassBlock& B, bool zodd) {
     const uint32_t tsz = 1u << t;
-    const float c = B.rc, kr = B.rkr, ki = B.rki;
+    const float c = B.rc, kr = zodd ? -B.rkr : B.rkr, ki = zodd ? -B.rki : B.rki;
     if (B.pbit == 0) {
         for (uint32_t y = threadIdx.x; y < tsz; y += kBlock) {
             const bool odd = (__popc(y & B.rpar) & 1u) != 0u;
@@ -445,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_bpass(float* __restrict__ re, float*
         const PassBlock& B = a.b[bi];
         if ((base & B.mo) != B.vo) continue;   // uniform: a control of this tile is 0
         if (B.K != 0) pass_block<K>(lre, lim, a.t, B);
-        else if (B.rot) pass_rot(lre, lim, a.t, B);
+        else if (B.rot) pass_rot(lre, lim, a.t, B, (__popcll(base & B.zo) & 1u) != 0u);
         else pass_swap(lre, lim, a.t, B);
     }
     // the product of the scalars that select this tile, in FP64, rounded once
@@ -814,8 +816,9 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
             };
             B.K = 0;
             B.rot = 1;
-            B.rflip = tile_bits(b.rot->rflip);
-            B.rpar = tile_bits(b.rot->rpar);
+            B.rflip = tile_bits(b.rot->rflip);   // flipped qubits are tile qubits
+            B.rpar = tile_bits(b.rot->rpar & T);
+            B.zo = b.rot->rpar & ~T;               // Z qubits kept outside the tile
             B.pbit = B.rflip & (0u - B.rflip);
             B.rc = (float)b.rot->rc;
             B.rkr = (float)b.rot->rk.real();
@@ -864,9 +867,11 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
 // tile of t qubits, each pass fused into width-qubit blocks inside its tile
 // and applied by k_bpass, several blocks per HBM round trip.  What may stay
 // outside the tile does: the controls of a controlled X that are not tile
-// qubits become its block's control mask, and a one-qubit diagonal on a qubit
-// outside the tile becomes per-tile scalars (PassOp).  A qubit kept outside a
-// pass as a control or a scalar's selector cannot be a target later in it.
+// qubits become its block's control mask, a one-qubit diagonal on a qubit
+// outside the tile becomes per-tile scalars (PassOp), and the Z qubits of a
+// sparse rotation that are not tile qubits add a parity per tile.  A qubit
+// kept outside a pass as a control, a scalar's selector or a rotation's Z
+// qubit cannot be a target later in it.
 int apply_ops(suqa_hip_f32* c, const std::vector<PassOp>& items, uint32_t width, uint32_t* nblocks) {
     const uint32_t nq = c->nq;
     const uint32_t t = std::min<uint32_t>(kPassTileBits, nq);
@@ -908,9 +913,17 @@ int apply_ops(suqa_hip_f32* c, const std::vector<PassOp>& items, uint32_t width,
             }
             uint64_t G = 0;
             for (uint32_t k2 = 0; k2 < op.n; ++k2) G |= 1ull << op.qs[k2];
+            uint64_t Z = 0;   // a rotation's Z qubits outside the tile: a parity per tile
+            if (op.rot) {
+                const uint64_t zout = op.rpar & ~op.rflip & ~T;
+                if (zout && may_keep_out(op.cmask | zout)) {
+                    Z = zout;
+                    G &= ~zout;
+                }
+            }
             if ((op.cmask & (T | G)) != 0 || (G & C) != 0 || (uint32_t)__builtin_popcountll(T | G) > t) break;
             T |= G;
-            C |= op.cmask;
+            C |= op.cmask | Z;
             part.push_back(std::move(op));
             ++j;
         }

@@ -110,3 +110,51 @@ def test_controlled_trotter_step_full_size(capi, passes):
     assert abs(f32.prob_filter([29], [0]) - 0.5) < 1e-6
     f32.close()
     e64.close()
+
+
+def test_sparse_rotations_with_z_qubits_outside_the_tile(capi):
+    """FP32 sparse mode at 16 qubits (12-qubit tiles): Pauli strings with X/Y
+    on tile qubits and Z on qubits 12-15, some controlled on qubit 15.  The Z
+    qubits a pass keeps outside its tile enter as one parity per tile; the
+    result follows the FP64 engine to FP32 rounding, as in the mode that
+    brings every qubit of a string into the tile."""
+    n = 16
+    log("sparse rotations, Z qubits outside the tile, n=16")
+    rg = wl.Pcg32(23)
+    specs = []
+    for _ in range(60):
+        qs, types = [], []
+        for _ in range(1 + rg.randint(0, 2)):          # 1-2 flipped qubits among 2..11
+            q = 2 + rg.randint(0, 10)
+            if q not in qs:
+                qs.append(q)
+                types.append(1 + rg.randint(0, 2))     # X or Y
+        ctrl = 1 << 15 if rg.randint(0, 3) == 0 else 0
+        for _ in range(1 + rg.randint(0, 3)):          # 1-3 Z qubits among 12..14 (15 when uncontrolled)
+            q = 12 + rg.randint(0, 3 if ctrl else 4)
+            if q not in qs:
+                qs.append(q)
+                types.append(3)
+        specs.append((qs, types, 2.0 * rg.doub() - 1.0, ctrl))
+    ops = capi.pauli_ops(specs)
+    e64 = capi.Engine(n)
+    e64.init_state()
+    for q in range(n):
+        e64.h(q)
+    e64.run_paulis(ops)
+    fids = {}
+    for mode in (1, 2):
+        f32 = capi.F32Engine(n)
+        f32.passes = mode
+        f32.init_state()
+        f32.run_gates(capi.gates_from([wl.G(wl.H, q, [], 0.0) for q in range(n)]), 5)
+        f32.run_paulis(ops, 5)
+        fids[mode] = f32.fidelity(e64)
+        for qs in ([12, 13], [14, 15], [2, 12]):
+            for v in range(4):
+                mask = [(v >> j) & 1 for j in range(2)]
+                assert abs(f32.prob_filter(qs, mask) - e64.prob_filter(qs, mask)) < 1e-6
+        f32.close()
+    log(f"fidelity {fids!r}")
+    assert fids[1] > 1 - 1e-9 and fids[2] > 1 - 1e-9
+    e64.close()
