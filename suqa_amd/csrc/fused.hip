@@ -865,13 +865,16 @@ int add_launch(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, siz
 
 // Applies ops in order: cut into passes of consecutive ops whose targets fit a
 // tile of t qubits, each pass fused into width-qubit blocks inside its tile
-// and applied by k_bpass, several blocks per HBM round trip.  What may stay
-// outside the tile does: the controls of a controlled X that are not tile
-// qubits become its block's control mask, a one-qubit diagonal on a qubit
-// outside the tile becomes per-tile scalars (PassOp), and the Z qubits of a
-// sparse rotation that are not tile qubits add a parity per tile.  A qubit
-// kept outside a pass as a control, a scalar's selector or a rotation's Z
-// qubit cannot be a target later in it.
+// and applied by k_bpass, several blocks per HBM round trip.  The tile is
+// settled for the whole pass first, from what each op must flip: every qubit
+// of a dense op, the target of a controlled X, the flipped qubits of a sparse
+// rotation; a one-qubit diagonal needs none.  Then whatever of an op lies
+// outside the tile stays there: the controls of a controlled X become its
+// block's control mask, a one-qubit diagonal becomes per-tile scalars, and a
+// sparse rotation's Z qubits add a parity per tile.  None of these qubits is
+// flipped in the pass (flipped qubits are tile qubits), so the kept-out parts
+// commute with everything else in it.  The controls an op comes with
+// (Op::cmask, the gc_mask of a Pauli op) always stay outside the tile.
 int apply_ops(suqa_hip_f32* c, const std::vector<PassOp>& items, uint32_t width, uint32_t* nblocks) {
     const uint32_t nq = c->nq;
     const uint32_t t = std::min<uint32_t>(kPassTileBits, nq);
@@ -881,58 +884,72 @@ int apply_ops(suqa_hip_f32* c, const std::vector<PassOp>& items, uint32_t width,
     uint32_t total = 0;
     size_t i = 0;
     while (i < items.size()) {
-        uint64_t T = 0;   // the 5 lowest qubits that are not controls of the first op
+        // 1. the pass and its required tile qubits.  Qubits 0..4 (128-byte runs
+        //    of FP32 amplitudes) unless the first op comes with controls there.
+        uint64_t T = 0;
         for (uint32_t q = 0; q < nq && __builtin_popcountll(T) < 5; ++q)
             if (!((items[i].op.cmask >> q) & 1ull)) T |= 1ull << q;
-        uint64_t C = 0;   // qubits kept outside the tile by this pass
-        std::vector<Op> part;
-        std::vector<PassScal> scal;
+        uint64_t out = 0;    // controls the ops come with: never tile qubits
+        uint64_t refd = 0;   // qubits the ops may leave outside (controls, diagonals, Z)
+        uint32_t nscal = 0;  // scalars needed at most (diagonals outside the tile so far)
         size_t j = i;
         while (j < items.size() && j - i < 64) {
             const PassOp& it = items[j];
-            // at most nq - t qubits can stay outside a tile of t qubits
-            auto may_keep_out = [&](uint64_t m) { return (uint32_t)__builtin_popcountll(C | m) <= nq - t; };
-            if (it.diag && !((T >> it.dq) & 1ull) && may_keep_out(1ull << it.dq)) {
-                const uint64_t bq = 1ull << it.dq;
-                const uint32_t need = (it.d0 != 1.0 ? 1u : 0u) + (it.d1 != 1.0 ? 1u : 0u);
-                if (scal.size() + need > kPassScal) break;
-                if (it.d0 != 1.0) scal.push_back(PassScal{bq, 0ull, it.d0.real(), it.d0.imag()});
-                if (it.d1 != 1.0) scal.push_back(PassScal{bq, bq, it.d1.real(), it.d1.imag()});
-                C |= bq;
-                ++j;
-                continue;
+            const Op& op = it.op;
+            uint64_t need = 0, opt = 0;
+            uint32_t ns = 0;
+            if (it.diag) {
+                opt = 1ull << it.dq;
+                if (!((T >> it.dq) & 1ull)) ns = (it.d0 != 1.0 ? 1u : 0u) + (it.d1 != 1.0 ? 1u : 0u);
+            } else if (it.ctrl) {
+                need = 1ull << it.tgt;
+                opt = it.ctrl;
+            } else if (op.rot) {
+                need = op.rflip;
+                opt = op.rpar & ~op.rflip;
+            } else {
+                for (uint32_t k2 = 0; k2 < op.n; ++k2) need |= 1ull << op.qs[k2];
             }
-            Op op = it.op;
-            if (it.ctrl) {
-                const uint64_t inside = it.ctrl & T, outside = it.ctrl & ~T;
-                if (outside && may_keep_out(outside)) {
-                    op = cx_op(inside, it.tgt);
-                    op.cmask = outside;
-                    op.swap = inside == 0;   // X on the target alone: a swap in LDS
-                }
-            }
-            uint64_t G = 0;
-            for (uint32_t k2 = 0; k2 < op.n; ++k2) G |= 1ull << op.qs[k2];
-            uint64_t Z = 0;   // a rotation's Z qubits outside the tile: a parity per tile
-            if (op.rot) {
-                const uint64_t zout = op.rpar & ~op.rflip & ~T;
-                if (zout && may_keep_out(op.cmask | zout)) {
-                    Z = zout;
-                    G &= ~zout;
-                }
-            }
-            if ((op.cmask & (T | G)) != 0 || (G & C) != 0 || (uint32_t)__builtin_popcountll(T | G) > t) break;
-            T |= G;
-            C |= op.cmask | Z;
-            part.push_back(std::move(op));
+            if ((op.cmask & (T | need)) != 0 || (need & out) != 0 || (uint32_t)__builtin_popcountll(T | need) > t ||
+                nscal + ns > kPassScal)
+                break;
+            T |= need;
+            out |= op.cmask;
+            refd |= opt;
+            nscal += ns;
             ++j;
         }
         if (j == i) return fail(SUQA_ERR_ARG, "an op does not fit a tile");
         // the tile loads move 4 consecutive amplitudes: qubits 0 and 1 must be tile bits
         if ((T & 3ull) != 3ull) return fail(SUQA_ERR_ARG, "controls on qubits 0 or 1 are not supported by FP32 passes");
+        // 2. pad the tile: first qubits no op of the pass refers to, then the others
+        //    (a control, diagonal or Z qubit inside the tile joins its op's block)
         for (uint32_t q = 0; q < nq && (uint32_t)__builtin_popcountll(T) < t; ++q)
-            if (!((C >> q) & 1ull)) T |= 1ull << q;
+            if (!(((out | refd) >> q) & 1ull)) T |= 1ull << q;
+        for (uint32_t q = 0; q < nq && (uint32_t)__builtin_popcountll(T) < t; ++q)
+            if (!((out >> q) & 1ull)) T |= 1ull << q;
         if ((uint32_t)__builtin_popcountll(T) < t) return fail(SUQA_ERR_ARG, "state too small for a tile besides the controls");
+        // 3. the ops against the settled tile
+        std::vector<Op> part;
+        std::vector<PassScal> scal;
+        part.reserve(j - i);
+        for (size_t k2 = i; k2 < j; ++k2) {
+            const PassOp& it = items[k2];
+            if (it.diag && !((T >> it.dq) & 1ull)) {
+                const uint64_t bq = 1ull << it.dq;
+                if (it.d0 != 1.0) scal.push_back(PassScal{bq, 0ull, it.d0.real(), it.d0.imag()});
+                if (it.d1 != 1.0) scal.push_back(PassScal{bq, bq, it.d1.real(), it.d1.imag()});
+                continue;
+            }
+            Op op = it.op;
+            if (it.ctrl && (it.ctrl & ~T)) {
+                const uint64_t inside = it.ctrl & T;
+                op = cx_op(inside, it.tgt);
+                op.cmask = it.ctrl & ~T;
+                op.swap = inside == 0;   // X on the target alone: a swap in LDS
+            }
+            part.push_back(std::move(op));
+        }
         const auto blocks = fuse(part, nq, width, 256, T);
         total += (uint32_t)blocks.size();
         // the scalars ride on the pass's last launch (they commute with its blocks)
