@@ -42,7 +42,9 @@ The QMS half of the metric is reported under "extra":
                 against the FP64 run in units of its standard error;
   qms_single_chain_device -- the c1 chain as a batch of one (one wavefront
                  runs it on the device), its steps/s and rets against qms_plain;
-  cpu_baseline_qms -- the oracle on the same single chain, 1 core.
+  cpu_baseline_qms -- the oracle on the same single chain, 1 core; for the
+                 batch, a 64-chain sample on one oracle process per chain over
+                 the host cores (at most 16), its rets against the batch's.
 
 The Trotter + phase-estimation workload under "extra.trotter_qpe": 24 system
 qubits + a 6-qubit register (30 qubits), 96 Pauli strings of weight 2-4
@@ -126,6 +128,28 @@ def cpu_baseline(n=27):
                        f"{dt:.1f} s on 1 host core (oracle/suqa_oracle.c, gcc -O2)")
 
 
+def _oracle_chain(args):
+    """One chain on the oracle, in a worker process: (beta, r, seed, steps, emin, emax) -> rets."""
+    from oracle import pyoracle as po
+    beta, r, seed, steps, emin, emax = args
+    _, rets = po.Qms(beta, r, seed, emin, emax, 10, 100).run(steps, 1, 100)
+    return rets
+
+
+def oracle_chains_all_cores(chains, steps, emin, emax):
+    """The oracle on a sample of chains, one single-threaded process per chain over
+    the host cores the box gives a GPU (at most 16): (chain-steps/s, workers, rets)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    workers = max(1, min(16, os.cpu_count() or 1, len(chains)))
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        list(ex.map(_oracle_chain, [(1.0, 2, 1, 1, emin, emax)] * workers))   # start every worker
+        t0 = time.perf_counter()
+        rets = list(ex.map(_oracle_chain, [(b, r, s, steps, emin, emax) for (b, r, s) in chains]))
+        dt = time.perf_counter() - t0
+    return len(chains) * steps / dt, workers, rets
+
+
 def qms_legs(no_cpu):
     """Plain FP64 GPU QMS (c1), batched chains (c5), their oracle checks and CPU rates."""
     from suqa_amd import qms, workloads as wl
@@ -205,6 +229,19 @@ def qms_legs(no_cpu):
     dt = time.perf_counter() - t0
     res["qms_batch"]["sample_chains_match_oracle"] = bool(ok)
     res["qms_batch"]["oracle_chain_steps_per_s_1core"] = round(nchk * bsteps / dt, 1)
+    # the CPU baseline as the workload defines it: one oracle process per chain over
+    # the host cores, on a sample of 64 chains spread over the beta grid and widths
+    sample = list(range(0, nch, nch // 64))
+    try:
+        rate_cpu, workers, orets = oracle_chains_all_cores([chains[c] for c in sample], bsteps, emin, emax)
+    except Exception as ex:   # worker processes unavailable: keep the 1-core figure
+        res["qms_batch"]["oracle_all_cores_error"] = repr(ex)
+        return res
+    res["qms_batch"]["oracle_chain_steps_per_s_all_cores"] = round(rate_cpu, 1)
+    res["qms_batch"]["oracle_cores"] = workers
+    res["qms_batch"]["speedup_vs_oracle_all_cores"] = round(rate / rate_cpu, 1)
+    res["qms_batch"]["sample64_chains_match_oracle"] = bool(all(list(b["rets"][c]) == o for c, o in zip(sample, orets)))
+    log(f"oracle, {len(sample)} chains on {workers} processes: {rate_cpu:.0f} chain-steps/s")
     return res
 
 
