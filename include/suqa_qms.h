@@ -58,7 +58,7 @@ int suqa_qms_run(const suqa_qms_params* p, suqa_qms_result* out);
 /* ---- batched chains -------------------------------------------------- */
 typedef struct {
     double   beta;
-    uint32_t ene_qbits;       /* 1..7: register width of this chain (state 2^(r+4) amplitudes) */
+    uint32_t ene_qbits;       /* 2..6: register width of this chain (state 2^(r+4) amplitudes) */
     uint64_t seed;            /* pcg32 seed (> 0) */
 } suqa_qms_chain;
 
