@@ -109,8 +109,18 @@ int suqa_hip_apply_qft_inverse(suqa_hip_ctx* ctx, const uint32_t* qs, uint32_t n
 /* suqa::measure_qbit (suqa.cu:673-696): c = (rdoub < P(q=1)); collapse and
  * renormalize.  One fused reduction + one fused collapse/scale pass. */
 int suqa_hip_measure_qbit(suqa_hip_ctx* ctx, uint32_t q, double rdoub, uint32_t* c);
-/* suqa::apply_reset(uint,double) (suqa.cu:730-738): measure, X if 1. */
+/* suqa::apply_reset(uint,double) (suqa.cu:730-738): measure, X if 1.  On a
+ * state of at most 2^14 amplitudes the measurement, collapse and X are one
+ * launch and the outcome stays on the device (nothing waits for it). */
 int suqa_hip_apply_reset(suqa_hip_ctx* ctx, uint32_t q, double rdoub, uint32_t gc_mask);
+/* suqa::measure_qbits (suqa.cu:699-702): measure_qbit on qs[k] with rdoubs[k],
+ * in order, outcomes into cs[0..n).  On a state of at most 2^14 amplitudes
+ * the n measurements are one launch and one wait. */
+int suqa_hip_measure_qbits(suqa_hip_ctx* ctx, const uint32_t* qs, const double* rdoubs, uint32_t n, uint32_t* cs);
+/* suqa::apply_reset(bmReg, vector<double>) (suqa.cu:752-757): apply_reset on
+ * qs[k] with rdoubs[k], in order; one launch and no wait on a state of at
+ * most 2^14 amplitudes. */
+int suqa_hip_apply_resets(suqa_hip_ctx* ctx, const uint32_t* qs, const double* rdoubs, uint32_t n, uint32_t gc_mask);
 /* suqa::prob_filter (suqa.cu:705-727): sum |psi_i|^2 over i whose bits qs[k]
  * read q_mask[k]. */
 int suqa_hip_prob_filter(suqa_hip_ctx* ctx, const uint32_t* qs, const uint32_t* q_mask,

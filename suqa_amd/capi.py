@@ -63,6 +63,8 @@ SIGNATURES = {
     "suqa_hip_apply_qft_inverse": (C.c_int, [CTX, PU32, U32, U32]),
     "suqa_hip_measure_qbit": (C.c_int, [CTX, U32, C.c_double, PU32]),
     "suqa_hip_apply_reset": (C.c_int, [CTX, U32, C.c_double, U32]),
+    "suqa_hip_measure_qbits": (C.c_int, [CTX, PU32, C.POINTER(C.c_double), U32, PU32]),
+    "suqa_hip_apply_resets": (C.c_int, [CTX, PU32, C.POINTER(C.c_double), U32, U32]),
     "suqa_hip_prob_filter": (C.c_int, [CTX, PU32, PU32, U32, PD]),
     "suqa_hip_qms_apply_W": (C.c_int, [CTX, U32, U32, U32, C.c_double, U32]),
     "suqa_hip_qms_apply_W2": (C.c_int, [CTX, U32, U32, U32, U32, U32, C.c_double]),
@@ -261,6 +263,16 @@ class Engine:
 
     def reset(self, q, rdoub):
         check(lib().suqa_hip_apply_reset(self.ctx, q, rdoub, self.gc_mask))
+
+    def measure_qbits(self, qs, rdoubs):
+        cs = (U32 * max(1, len(qs)))()
+        rd = (C.c_double * max(1, len(rdoubs)))(*rdoubs)
+        check(lib().suqa_hip_measure_qbits(self.ctx, _u32(qs), rd, len(qs), cs))
+        return [cs[k] for k in range(len(qs))]
+
+    def resets(self, qs, rdoubs):
+        rd = (C.c_double * max(1, len(rdoubs)))(*rdoubs)
+        check(lib().suqa_hip_apply_resets(self.ctx, _u32(qs), rd, len(qs), self.gc_mask))
 
     def prob_filter(self, qs, q_mask):
         p = C.c_double()

@@ -231,6 +231,10 @@ int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out) {
     c->d_result = static_cast<double*>(p);
     SQE_HIP(hipHostMalloc(&p, sizeof(double) * 4, hipHostMallocDefault));
     c->h_result = static_cast<double*>(p);
+    SQE_HIP(hipMalloc(&p, sizeof(uint32_t) * sqe::kMeasMax));
+    c->d_flags = static_cast<uint32_t*>(p);
+    SQE_HIP(hipHostMalloc(&p, sizeof(uint32_t) * sqe::kMeasMax, hipHostMallocDefault));
+    c->h_flags = static_cast<uint32_t*>(p);
     int rc = suqa_hip_allocate_state(c, nq);
     if (rc) {
         std::string keep = g_err;
@@ -252,6 +256,8 @@ int suqa_hip_clear(suqa_hip_ctx* c) {
     free_state(c);
     if (c->d_result) (void)hipFree(c->d_result);
     if (c->h_result) (void)hipHostFree(c->h_result);
+    if (c->d_flags) (void)hipFree(c->d_flags);
+    if (c->h_flags) (void)hipHostFree(c->h_flags);
     if (c->d_wtab) (void)hipFree(c->d_wtab);
     if (c->d_wrow) (void)hipFree(c->d_wrow);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -490,9 +496,31 @@ int suqa_hip_apply_qft_inverse(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, 
     return SUQA_OK;
 }
 
+// Small states: measurements (and resets) in one single-workgroup launch,
+// outcomes copied back only when the caller asks for them.
+static int measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, bool reset,
+                         uint32_t gc, uint32_t* cs) {
+    for (uint32_t k0 = 0; k0 < n; k0 += sqe::kMeasMax) {
+        const uint32_t m = std::min<uint32_t>(sqe::kMeasMax, n - k0);
+        SQE_HIP(sqe::launch_measure_small(c, qs + k0, rds + k0, m, reset ? ~0u : 0u, gc));
+        if (cs) {
+            SQE_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c->stream));
+            SQE_HIP(hipStreamSynchronize(c->stream));
+            for (uint32_t k = 0; k < m; ++k) cs[k0 + k] = c->h_flags[k];
+        }
+    }
+    return SUQA_OK;
+}
+
 int suqa_hip_measure_qbit(suqa_hip_ctx* c, uint32_t q, double rdoub, uint32_t* cout) {
     SQE_CTX(c);
     if (q >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    if (c->nq <= sqe::kMeasSmallBits) {
+        uint32_t cc = 0;
+        int rc = measure_small(c, &q, &rdoub, 1, false, 0u, &cc);
+        if (rc == SUQA_OK && cout) *cout = cc;
+        return rc;
+    }
     Sel sel;
     sqe::make_sel(c->nq, 0, 0, &sel);
     const uint64_t qb = 1ull << q;
@@ -510,10 +538,35 @@ int suqa_hip_measure_qbit(suqa_hip_ctx* c, uint32_t q, double rdoub, uint32_t* c
 }
 
 int suqa_hip_apply_reset(suqa_hip_ctx* c, uint32_t q, double rdoub, uint32_t gc) {
+    SQE_CTX(c);
+    if (q >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    if (c->nq <= sqe::kMeasSmallBits) return measure_small(c, &q, &rdoub, 1, true, gc, nullptr);
     uint32_t cc = 0;
     int rc = suqa_hip_measure_qbit(c, q, rdoub, &cc);
     if (rc) return rc;
     if (cc) return suqa_hip_apply_x(c, q, gc);
+    return SUQA_OK;
+}
+
+int suqa_hip_measure_qbits(suqa_hip_ctx* c, const uint32_t* qs, const double* rdoubs, uint32_t n, uint32_t* cs) {
+    SQE_CTX(c);
+    if (n && (!qs || !rdoubs || !cs)) return fail(SUQA_ERR_ARG, "null argument");
+    for (uint32_t k = 0; k < n; ++k)
+        if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    if (n && c->nq <= sqe::kMeasSmallBits) return measure_small(c, qs, rdoubs, n, false, 0u, cs);
+    for (uint32_t k = 0; k < n; ++k)
+        if (int rc = suqa_hip_measure_qbit(c, qs[k], rdoubs[k], &cs[k])) return rc;
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_resets(suqa_hip_ctx* c, const uint32_t* qs, const double* rdoubs, uint32_t n, uint32_t gc) {
+    SQE_CTX(c);
+    if (n && (!qs || !rdoubs)) return fail(SUQA_ERR_ARG, "null argument");
+    for (uint32_t k = 0; k < n; ++k)
+        if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    if (n && c->nq <= sqe::kMeasSmallBits) return measure_small(c, qs, rdoubs, n, true, gc, nullptr);
+    for (uint32_t k = 0; k < n; ++k)
+        if (int rc = suqa_hip_apply_reset(c, qs[k], rdoubs[k], gc)) return rc;
     return SUQA_OK;
 }
 

@@ -83,6 +83,8 @@ struct suqa_hip_ctx {
     double* d_partials = nullptr;      // 2 * sqe::partial_pairs_capacity(N) doubles
     double* d_result = nullptr;
     double* h_result = nullptr;     // pinned, 4 doubles
+    uint32_t* d_flags = nullptr;    // outcomes of launch_measure_small (sqe::kMeasMax)
+    uint32_t* h_flags = nullptr;    // pinned copy
     // qms::apply_W coefficient table (fs1, fs2) per (E_old, E_new)
     double* d_wtab = nullptr;
     uint64_t wtab_L = 0;
@@ -127,6 +129,14 @@ hipError_t launch_phase_list(suqa_hip_ctx* c, const Sel& sel, uint32_t q0, uint3
 hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
                        double alg_bytes);
 hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale);
+// Measurements of a state of at most 2^kMeasSmallBits amplitudes in one
+// workgroup: for k < n in order, p0 and p1 of qubit qs[k], the outcome
+// c = (rds[k] < p1) into ctx->d_flags[k], the collapse scaled by 1/sqrt(p_c),
+// and when bit k of `resets` is set and c = 1, X on qs[k] over the indices
+// with the gc bits set.  n <= kMeasMax.
+constexpr uint32_t kMeasSmallBits = 14, kMeasMax = 16;
+hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, uint32_t resets,
+                                uint32_t gc);
 // W of qms.cuh:220-242 over the acceptance pairs (i, i without the top bit):
 // coefficients (fs1, fs2) at d_tab[2 (Eo L + En)], En = (i & wmask) >> b0 and
 // Eo = (i & omask) >> ob0 (omask = 0: Eo = 0, d_tab is the row of a classical

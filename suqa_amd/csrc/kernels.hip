@@ -342,6 +342,77 @@ __global__ __launch_bounds__(kBlock) void k_collapse(double* __restrict__ re, do
     v.store(base + i);
 }
 
+// launch_measure_small (engine.hpp): one workgroup, the whole state.  The
+// probabilities are a fixed-order block reduction; every thread reads the
+// same two sums from LDS, so the outcome and the scale are uniform.
+struct MeasArgs {
+    uint64_t N;
+    uint32_t n, resets, gc, pad_;
+    uint32_t q[kMeasMax];
+    double rd[kMeasMax];
+};
+
+__global__ __launch_bounds__(kBlock) void k_measure_small(double* __restrict__ re, double* __restrict__ im,
+                                                          const MeasArgs a, uint32_t* __restrict__ out) {
+    __shared__ double red[2][kBlock / 64];
+    __shared__ double fin[2];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = 0; k < a.n; ++k) {
+        const uint32_t q = a.q[k];
+        const uint64_t qb = 1ull << q;
+        double s0 = 0.0, s1 = 0.0;
+        for (uint64_t i = tid; i < a.N; i += kBlock) {
+            const double x = re[i], y = im[i];
+            const double p = x * x + y * y;
+            if (i & qb) s1 += p;
+            else s0 += p;
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        if ((tid & 63u) == 0u) {
+            red[0][tid >> 6] = s0;
+            red[1][tid >> 6] = s1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double t0 = 0.0, t1 = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) {
+                t0 += red[0][w];
+                t1 += red[1][w];
+            }
+            fin[0] = t0;
+            fin[1] = t1;
+        }
+        __syncthreads();
+        const double p0 = fin[0], p1 = fin[1];
+        const uint32_t cbit = a.rd[k] < p1 ? 1u : 0u;   // suqa.cu:690
+        const double scale = 1. / sqrt(cbit ? p1 : p0);  // set_ampl_to_zero + vnormalize (suqa.cu:694-695)
+        const uint64_t keep = cbit ? qb : 0ull;
+        for (uint64_t i = tid; i < a.N; i += kBlock) {
+            const bool kp = (i & qb) == keep;
+            re[i] = kp ? re[i] * scale : 0.0;
+            im[i] = kp ? im[i] * scale : 0.0;
+        }
+        if (tid == 0) out[k] = cbit;
+        __syncthreads();
+        if (((a.resets >> k) & 1u) && cbit) {   // apply_reset: X on q (suqa.cu:735-737)
+            const uint64_t M = (uint64_t)a.gc | qb, V = M & ~qb;
+            for (uint64_t pr = tid; pr < a.N / 2; pr += kBlock) {
+                const uint64_t i = ((pr >> q) << (q + 1)) | (pr & (qb - 1ull));
+                if ((i & M) != V) continue;
+                const uint64_t j = i | qb;
+                const double tr = re[i], ti = im[i];
+                re[i] = re[j];
+                im[i] = im[j];
+                re[j] = tr;
+                im[j] = ti;
+            }
+            __syncthreads();
+        }
+        __syncthreads();   // fin[] is rewritten by the next measurement
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scale(double* __restrict__ data, uint64_t nitems, double f) {
     const uint64_t it = item_id();
     if (it >= nitems) return;
@@ -600,6 +671,26 @@ hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double
     timing_begin(c, "collapse", 24.0 * (double)c->N);
     hipLaunchKernelGGL(k_collapse, dim3(grid_for(((nitems + 31) / 32) * 64)), dim3(kBlock), 0, c->stream, c->d_re,
                        c->d_im, nitems, qbit, keep ? qbit : 0ull, scale);
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, uint32_t resets,
+                                uint32_t gc) {
+    if (hipError_t e = flush(c)) return e;
+    if (n == 0 || n > kMeasMax || c->nq > kMeasSmallBits) return hipErrorInvalidValue;
+    MeasArgs a{};
+    a.N = c->N;
+    a.n = n;
+    a.resets = resets;
+    a.gc = gc;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (qs[k] >= c->nq) return hipErrorInvalidValue;
+        a.q[k] = qs[k];
+        a.rd[k] = rds[k];
+    }
+    timing_begin(c, "measure_small", 40.0 * (double)c->N * n);
+    hipLaunchKernelGGL(k_measure_small, dim3(1), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_flags);
     timing_end(c);
     return hipGetLastError();
 }

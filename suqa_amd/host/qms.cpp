@@ -73,8 +73,11 @@ void reset_non_syst_qbits() {
         suqa::apply_reset(bm_enes_old, rgenerates);
     }
     for (auto& el : rgenerates) el = rangen.doub();
-    suqa::apply_reset(bm_enes_new, rgenerates);
-    suqa::apply_reset(bm_acc, rangen.doub());
+    // the new register, then the acceptance qubit with the next draw: one call
+    std::vector<uint> qs(bm_enes_new);
+    qs.push_back(bm_acc);
+    rgenerates.push_back(rangen.doub());
+    suqa::apply_reset(qs, rgenerates);
 }
 
 void apply_phase_estimation(const std::vector<uint>& q_syst, const std::vector<uint>& q_target, const double& t,
@@ -188,7 +191,7 @@ int metro_step(bool take_measure) {
             for (uint ei = 0U; ei < ene_qbits; ++ei)
                 if (c_E_news[ei]) suqa::apply_x(bm_enes_new[ei]);
             X_measures.push_back(measure_X(rangen));
-            for (uint ei = 0U; ei < ene_qbits; ++ei) suqa::apply_reset(bm_enes_new[ei], rangen.doub());
+            suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));   // the same draws, in order
             apply_Phi();
             suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
             apply_Phi_inverse();
@@ -214,9 +217,9 @@ int metro_step(bool take_measure) {
         if (curr_E_old == Enew_meas) {
             if (take_measure) {
                 E_measures.push_back(curr_E_old_d);
-                for (uint ei = 0U; ei < ene_qbits; ++ei) suqa::apply_reset(bm_enes_new[ei], rangen.doub());
+                suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));   // the same draws, in order
                 X_measures.push_back(measure_X(rangen));
-                for (uint ei = 0U; ei < ene_qbits; ++ei) suqa::apply_reset(bm_enes_new[ei], rangen.doub());
+                suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));   // the same draws, in order
                 apply_Phi();
                 suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
                 apply_Phi_inverse();

@@ -260,17 +260,32 @@ void suqa::measure_qbit(uint q, uint& c, double rdoub) {
 }
 
 void suqa::measure_qbits(const bmReg& qs, std::vector<uint>& cs, const std::vector<double>& rdoubs) {
-    for (uint k = 0U; k < qs.size(); ++k) suqa::measure_qbit(qs[k], cs[k], rdoubs[k]);
+    const uint32_t n = (uint32_t)qs.size();
+    std::vector<uint32_t> out(n);
+    chk(suqa_hip_measure_qbits(ctx, qs.data(), rdoubs.data(), n, out.data()), "suqa::measure_qbits()");
+    for (uint32_t k = 0; k < n; ++k) cs[k] = out[k];
 }
 
+// With gate counters registered, the X of a reset is counted when the outcome
+// is 1 (as the reference's reset counts its apply_x), so the host needs the
+// outcome; otherwise the engine applies measurement and X without a wait.
 void suqa::apply_reset(uint q, double rdoub) {   // suqa.cu:730-738
-    uint c;
-    suqa::measure_qbit(q, c, rdoub);
-    if (c) suqa::apply_x(q);
+    if (!gatecounters.counters.empty()) {
+        uint c;
+        suqa::measure_qbit(q, c, rdoub);
+        if (c) suqa::apply_x(q);
+        return;
+    }
+    chk(suqa_hip_apply_reset(ctx, q, rdoub, gc_mask), "suqa::apply_reset()");
 }
 
-void suqa::apply_reset(const bmReg& qs, std::vector<double> rdoubs) {
-    for (uint i = 0; i < qs.size(); ++i) suqa::apply_reset(qs[i], rdoubs[i]);
+void suqa::apply_reset(const bmReg& qs, std::vector<double> rdoubs) {   // suqa.cu:752-757
+    if (!gatecounters.counters.empty()) {
+        for (uint i = 0; i < qs.size(); ++i) suqa::apply_reset(qs[i], rdoubs[i]);
+        return;
+    }
+    chk(suqa_hip_apply_resets(ctx, qs.data(), rdoubs.data(), (uint32_t)qs.size(), gc_mask),
+        "suqa::apply_reset()");
 }
 
 void suqa::prob_filter(const bmReg& qs, const std::vector<uint>& q_mask, double& prob) {

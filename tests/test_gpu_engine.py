@@ -311,3 +311,43 @@ def test_low_and_high_bit_combinations_bitwise(capi):
         gre, gim = eng.get()
         np.testing.assert_allclose(gre, st.re(), atol=1e-14, rtol=0)
     eng.close()
+
+
+@pytest.mark.parametrize("n", [6, 12, 14, 15])
+def test_measure_qbits_and_resets_vs_oracle(capi, n):
+    """suqa_hip_measure_qbits / suqa_hip_apply_resets (one launch on states of
+    at most 2^14 amplitudes, the per-qubit path above) against the oracle's
+    sequential measure_qbit / apply_reset: identical outcomes, states within
+    1e-12; resets under an active gc_mask (their X is controlled) included."""
+    log(f"measure_qbits / resets n={n}")
+    rng = np.random.default_rng(40 + n)
+    eng = capi.Engine(n)
+    st = po.State(n)
+    prep(eng, n)
+    prep(st, n)
+    for rnd in range(3):
+        qs = [int(q) for q in rng.permutation(n)[:min(n, 18)]]
+        rds = [float(x) for x in rng.random(len(qs))]
+        cs = eng.measure_qbits(qs, rds)
+        assert cs == [st.measure(q, r) for q, r in zip(qs, rds)]
+        for q in range(n):   # mix the state again so later outcomes are not forced
+            eng.h(q)
+            st.h(q)
+        rqs = [int(q) for q in rng.permutation(n)[:3]]
+        rr = [float(x) for x in rng.random(3)]
+        ctl = [q for q in range(n) if q not in rqs][:1] if rnd == 2 else []
+        eng.activate_gc_mask(ctl)
+        st.activate_gc_mask(ctl)
+        eng.resets(rqs, rr)
+        for q, r in zip(rqs, rr):
+            st.reset(q, r)
+        eng.deactivate_gc_mask(ctl)
+        st.deactivate_gc_mask(ctl)
+        for q in range(n):
+            eng.h(q)
+            st.h(q)
+        gre, gim = eng.get()
+        np.testing.assert_allclose(gre, st.re(), atol=1e-12, rtol=0)
+        np.testing.assert_allclose(gim, st.im(), atol=1e-12, rtol=0)
+    assert abs(eng.vnorm() - 1.0) < 1e-12
+    eng.close()
