@@ -21,7 +21,10 @@
 // every group and precomputed on the host as 8-bit tables over the group's 8
 // elements, and a part on the other tile bits, tested once per group; the
 // per-element work is the gate arithmetic alone.  A gate that flips more than
-// 3 bits gets a step of its own, applied pair by pair in LDS.
+// 3 bits gets a step of its own, applied pair by pair in LDS, and so does
+// every gate when the whole state is one tile of at most 2^9 amplitudes (a
+// latency-bound workgroup, where a pair per thread beats 8 amplitudes per
+// thread on tsz / 8 threads).
 //
 // Gates keep their order and every amplitude is computed with the one-gate
 // sweeps' arithmetic (sweep_math.hpp), so results are bit-identical to
@@ -43,6 +46,7 @@ namespace {
 constexpr uint32_t kMaxPass = 24;
 constexpr uint32_t kLowFixed = 4;                 // qubits 0..3: 16 amplitudes = one 128-byte line
 constexpr uint32_t kSub = 3, kLoc = 1u << kSub;   // register step: 3 bits, 8 amplitudes per thread
+constexpr uint32_t kTinyBits = 9;                 // one-tile states up to 2^9: LDS steps only
 constexpr uint32_t kMaxChunks = (1u << kMaxTileBits) / 2 / kBlock;   // 16-byte chunks per array per thread
 
 typedef double dv2t __attribute__((ext_vector_type(2)));
@@ -118,6 +122,19 @@ __device__ __forceinline__ void tile_pairs(double* lre, double* lim, uint32_t ha
         lim[ia] = pair_out_par<OP>(true, true, ma, mb, rb, par_a, pc);
         lre[ib] = pair_out_par<OP>(false, false, rb, ra, ma, par_b, pc);
         lim[ib] = pair_out_par<OP>(false, true, mb, ma, ra, par_b, pc);
+    }
+}
+
+// LDS step: phase rotation over the whole tile, element per thread (the
+// arithmetic of reg_phase_one: s negated where the parity is odd).
+__device__ __forceinline__ void tile_phase(double* lre, double* lim, uint32_t tsz, const TileGate& g, uint32_t opar) {
+    for (uint32_t y = threadIdx.x; y < tsz; y += kBlock) {
+        if ((y & g.mc) != g.vc) continue;
+        const uint32_t par = ((uint32_t)__popc(y & g.yzc) & 1u) ^ opar;
+        const double sn = par ? -g.s : g.s;
+        const double rr = lre[y], mm = lim[y];
+        lre[y] = phase_out(false, rr, mm, g.c, sn);
+        lim[y] = phase_out(true, mm, rr, g.c, sn);
     }
 }
 
@@ -320,7 +337,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
                 case OP_SWAP: tile_pairs<OP_SWAP>(lre, lim, tsz / 2, g, opar); break;
                 case OP_Y: tile_pairs<OP_Y>(lre, lim, tsz / 2, g, opar); break;
                 case OP_H: tile_pairs<OP_H>(lre, lim, tsz / 2, g, opar); break;
-                default: tile_pairs<OP_PAULI>(lre, lim, tsz / 2, g, opar); break;
+                case OP_PAULI: tile_pairs<OP_PAULI>(lre, lim, tsz / 2, g, opar); break;
+                default: tile_phase(lre, lim, tsz, g, opar); break;
                 }
             }
             __syncthreads();
@@ -476,6 +494,15 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
         mcom &= ~((g.vo ^ vcom) & mcom);
     }
     vcom &= mcom;
+    const uint64_t fix = T | mcom;
+    a.freemask = ((nq == 64 ? 0ull : (1ull << nq)) - 1ull) & ~fix;
+    a.fixval = vcom;
+    a.ntiles = 1ull << __builtin_popcountll(a.freemask);
+    // A state that is one small tile (t <= kTinyBits) is one workgroup and the
+    // pass is latency-bound: every gate takes an LDS step of its own (a pair or
+    // an element per thread, every thread busy) instead of sharing register
+    // steps among tsz / 8 threads.
+    const bool tiny = t <= kTinyBits && a.ntiles == 1;
     // register steps: runs of gates whose flipped tile bits fit in kSub bits
     a.nsteps = 0;
     uint32_t F = 0, g0 = 0;
@@ -502,7 +529,7 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
         const uint32_t xc = a.g[i].xc;
         // register steps assume that a one-bit partner mask is selected at 0
         const bool pivot_zero = __builtin_popcount(xc) != 1 || (((a.g[i].mc & xc) != 0) && ((a.g[i].vc & xc) == 0));
-        if ((uint32_t)__builtin_popcount(xc) > kSub || !pivot_zero) {   // a step of its own, in LDS
+        if (tiny || (uint32_t)__builtin_popcount(xc) > kSub || !pivot_zero) {   // a step of its own, in LDS
             if (i > g0) close(i, false);
             g0 = i;
             close(i + 1, true);
@@ -518,10 +545,6 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
         F |= xc;
     }
     if (n > g0) close(n, false);
-    const uint64_t fix = T | mcom;
-    a.freemask = ((nq == 64 ? 0ull : (1ull << nq)) - 1ull) & ~fix;
-    a.fixval = vcom;
-    a.ntiles = 1ull << __builtin_popcountll(a.freemask);
     const uint64_t g8 = ((a.ntiles + 7) / 8) * 8;
     const size_t lds = sizeof(double) * 2 * ((size_t)1 << t);
     ++c->passes;

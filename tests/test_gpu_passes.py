@@ -282,12 +282,14 @@ def test_full_size_passes_match_one_sweep_per_gate(capi):
     assert vals[0] == vals[1]
 
 
-@pytest.mark.parametrize("n,bits", [(5, 5), (6, 5), (8, 6), (12, 11)])
+@pytest.mark.parametrize("n,bits", [(7, 5), (6, 5), (8, 6), (12, 11), (5, 5), (9, 11)])
 def test_every_gate_at_every_qubit_in_a_register_step(capi, n, bits):
     """Each gate kind on each qubit, followed by a gate that selects on another
     qubit (so both share a register step and the first one lands on every
     local position), bit-identical to one sweep per gate.  Covers the
-    compile-time register layouts of every partner mask (xl = 1, 2, 4)."""
+    compile-time register layouts of every partner mask (xl = 1, 2, 4), and
+    (n = 5, 9: the state is one tile of at most 2^9 amplitudes) the passes
+    that give every gate an LDS step of its own."""
     log(f"gate x qubit sweep n={n} tile bits {bits}")
     rng = np.random.default_rng(n)
     v = rng.normal(size=(2, 1 << n))
