@@ -40,6 +40,9 @@ The QMS half of the metric is reported under "extra":
                 probabilities: chain-steps/s, chains whose return codes equal
                 the FP64 run's, and per-beta <E> (8 bins of 512 chains)
                 against the FP64 run in units of its standard error;
+  qms_batch_two_registers -- the same kind of batch in the two-register
+                layout (3 + 2r + 1 qubits per chain) for r in {3, 4}, FP64
+                and FP32, 4 sample chains checked against the oracle;
   qms_single_chain_device -- the c1 chain as a batch of one (one wavefront
                  runs it on the device), its steps/s and rets against qms_plain;
   cpu_baseline_qms -- the oracle on the same single chain, 1 core; for the
@@ -150,6 +153,12 @@ def oracle_chains_all_cores(chains, steps, emin, emax):
     return len(chains) * steps / dt, workers, rets
 
 
+def po_two_register_chain(beta, r, seed, steps, emin, emax):
+    """One two-register chain on the oracle: (accepted, rets)."""
+    from oracle import pyoracle as po
+    return po.Qms(beta, r, seed, emin, emax, 10, 100, two_registers=True).run(steps, 1, 100)
+
+
 def qms_legs(no_cpu):
     """Plain FP64 GPU QMS (c1), batched chains (c5), their oracle checks and CPU rates."""
     from suqa_amd import qms, workloads as wl
@@ -207,8 +216,29 @@ def qms_legs(no_cpu):
                              "chains_identical_to_fp64": same, "per_beta_E": bins,
                              "dtype": "f32 amplitudes, f64 probabilities"}
     log(f"qms batch fp32: {rate32:.0f} chain-steps/s, {same}/{nch} chains identical to FP64")
+    # the batch in the two-register layout (E_old in its own register, 3 + 2r + 1
+    # qubits per chain) for the widths whose FP64 state fits a workgroup's LDS
+    rg2 = wl.Pcg32(5)
+    chains2 = [(0.1 + 4.9 * i / (nch - 1), 3 + rg2.randint(0, 2), 1000 + i) for i in range(nch)]
+    b2 = qms.batch_run(chains2, bsteps, 1, 100, emin, emax, 10, 100, two_registers=True)
+    b2f = qms.batch_run(chains2, bsteps, 1, 100, emin, emax, 10, 100, two_registers=True, fp32=True)
+    res["qms_batch_two_registers"] = {
+        "chain_steps_per_s": round(nch * bsteps / b2["seconds"], 0),
+        "fp32_chain_steps_per_s": round(nch * bsteps / b2f["seconds"], 0),
+        "fp32_chains_identical_to_fp64": int(sum(1 for c in range(nch) if np.array_equal(b2f["rets"][c], b2["rets"][c]))),
+        "chains": nch, "steps": bsteps, "register_widths": [3, 4], "qubits_per_chain": [10, 12],
+        "config": "as qms_batch, E_old in its own register; widths 5 and 6 (14 and 16 qubits) exceed a "
+                  "workgroup's LDS in FP64"}
+    log(f"qms batch, two registers: {nch * bsteps / b2['seconds']:.0f} chain-steps/s "
+        f"(fp32 {nch * bsteps / b2f['seconds']:.0f})")
     if no_cpu:
         return res
+    okb2 = True
+    for c in range(0, nch, nch // 4):
+        bb, rr, ss = chains2[c]
+        _, orets = po_two_register_chain(bb, rr, ss, bsteps, emin, emax)
+        okb2 &= list(b2["rets"][c]) == orets
+    res["qms_batch_two_registers"]["sample_chains_match_oracle"] = bool(okb2)
     from oracle import pyoracle as po
     q2 = po.Qms(beta, r, seed, emin, emax, 10, 100, two_registers=True)
     _, rets2 = q2.run(steps, 1, 100)

@@ -68,6 +68,10 @@ typedef struct {
     int32_t  pe_steps, thermalization;
     uint32_t fp32;            /* 0: FP64 chains (bit-exact with the reference CPU path);
                                  1: FP32 amplitudes, FP64 probabilities and collapse scale */
+    uint32_t two_registers;   /* 1: E_old in its own register (the commented-out layout of
+                                 qms.cuh:72-85, 134-136, 201-202): 2r + 4 qubits per chain,
+                                 held in LDS, so 2^(2r+4) amplitudes must fit a workgroup's
+                                 LDS (the call fails otherwise) */
 } suqa_qms_batch_params;
 
 /* Per chain outputs (arrays of nchains * metro_steps for rets; measurements

@@ -23,6 +23,7 @@
 // uniform draw falls within the FP32 rounding of a probability.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -78,6 +79,8 @@ struct Chain {
     T* re;               // LDS
     T* im;
     uint32_t r, n, N, L, acc;
+    uint32_t en, eo;     // first qubit of the E_new register, of the E_old register (two registers)
+    bool two;            // E_old in its own register (qms.cuh:72-85, 134-136, 201-202)
     uint32_t lane;
     Rng rng;
     const ChainDev* P;
@@ -188,13 +191,15 @@ struct Chain {
         if (measure(q, rd)) x(q);
     }
 
-    // qms::apply_W, dense form (qms.cuh:286-307); acc is the top qubit
+    // qms::apply_W, dense form (qms.cuh:286-307); acc is the top qubit.  With
+    // two registers E_old is read per amplitude from its register.
     __device__ void W(uint32_t Eold) {
-        const double* row = P->wtab + 2 * (size_t)Eold * L;
         const uint32_t half = N / 2;
         for (uint32_t j = lane; j < half; j += 64) {
             const uint32_t i = j + half;
-            const uint32_t En = (i >> E0) & (L - 1);
+            const uint32_t En = (i >> en) & (L - 1);
+            const uint32_t Eo = two ? (i >> eo) & (L - 1) : Eold;
+            const double* row = P->wtab + 2 * (size_t)Eo * L;
             const T fs1 = T(row[2 * En]), fs2 = T(row[2 * En + 1]);
             T t = re[j];
             const T ri = re[i];
@@ -216,36 +221,37 @@ struct Chain {
         cx(1, 0);
     }
 
-    __device__ void cevolution(uint32_t trg, int dir) {   // qms.cuh:106-115, system.cu:21-26
-        const uint32_t gc = 1u << (E0 + trg);
+    __device__ void cevolution(uint32_t base, uint32_t trg, int dir) {   // qms.cuh:106-115, system.cu:21-26
+        const uint32_t gc = 1u << (base + trg);
         const double c = P->xx_c[dir][trg], s = P->xx_s[dir][trg];
         xx(1, 2, c, s, gc);
         xx(2, 0, c, s, gc);
         xx(0, 1, c, s, gc);
     }
 
-    __device__ __noinline__ void Phi() {   // qms.cuh:153-176
-        for (uint32_t k = 0; k < r; ++k) h(E0 + k, 0u);
+    __device__ __noinline__ void Phi(uint32_t b) {   // qms.cuh:153-176 on the register at qubit b
+        for (uint32_t k = 0; k < r; ++k) h(b + k, 0u);
         for (int trg = (int)r - 1; trg > -1; --trg) {
-            cevolution((uint32_t)trg, 0);
-            phase(E0 + trg, E0 + trg, P->u1_c[0][trg], P->u1_s[0][trg]);
+            cevolution(b, (uint32_t)trg, 0);
+            phase(b + trg, b + trg, P->u1_c[0][trg], P->u1_s[0][trg]);
         }
         for (uint32_t o = 0; o < r; ++o) {   // QFT^-1 (suqa.cu:649-661)
-            for (uint32_t in = 0; in < o; ++in) phase(E0 + in, E0 + o, P->qft_c[1][1 + o - in], P->qft_s[1][1 + o - in]);
-            h(E0 + o, 0u);
+            for (uint32_t in = 0; in < o; ++in) phase(b + in, b + o, P->qft_c[1][1 + o - in], P->qft_s[1][1 + o - in]);
+            h(b + o, 0u);
         }
     }
+    __device__ void Phi() { Phi(en); }
 
     __device__ __noinline__ void Phi_inverse() {   // qms.cuh:179-198
         for (int o = (int)r - 1; o >= 0; o--) {   // QFT (suqa.cu:635-647)
-            h(E0 + o, 0u);
-            for (int in = o - 1; in >= 0; in--) phase(E0 + in, E0 + o, P->qft_c[0][1 + o - in], P->qft_s[0][1 + o - in]);
+            h(en + o, 0u);
+            for (int in = o - 1; in >= 0; in--) phase(en + in, en + o, P->qft_c[0][1 + o - in], P->qft_s[0][1 + o - in]);
         }
         for (uint32_t trg = 0; trg < r; ++trg) {
-            cevolution(trg, 1);
-            phase(E0 + trg, E0 + trg, P->u1_c[1][trg], P->u1_s[1][trg]);
+            cevolution(en, trg, 1);
+            phase(en + trg, en + trg, P->u1_c[1][trg], P->u1_s[1][trg]);
         }
-        for (uint32_t k = 0; k < r; ++k) h(E0 + k, 0u);
+        for (uint32_t k = 0; k < r; ++k) h(en + k, 0u);
     }
 
     __device__ __noinline__ void U(uint32_t ci, uint32_t Eold) {   // qms.cuh:335-348
@@ -260,7 +266,7 @@ struct Chain {
     }
 
     // measure_qbits(bm_enes, cs, extract_rands(r)): all draws first
-    __device__ __noinline__ uint32_t measure_enes() {
+    __device__ __noinline__ uint32_t measure_enes(uint32_t b) {
         double rd[kMaxEne];
 #pragma unroll
         for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
@@ -268,12 +274,24 @@ struct Chain {
         uint32_t v = 0;
 #pragma unroll
         for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
-            if (k < r) v += measure(E0 + k, rd[k]) << k;
+            if (k < r) v += measure(b + k, rd[k]) << k;
         return v;
+    }
+    __device__ uint32_t measure_enes() { return measure_enes(en); }
+
+    // r draws, then the resets of the register at qubit b in order
+    __device__ __noinline__ void reset_register(uint32_t b) {
+        double rd[kMaxEne];
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
+            if (k < r) rd[k] = rng.doub();
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
+            if (k < r) reset(b + k, rd[k]);
     }
 
     __device__ __noinline__ void reset_enes_each_draw() {
-        for (uint32_t k = 0; k < r; ++k) reset(E0 + k, rng.doub());
+        for (uint32_t k = 0; k < r; ++k) reset(en + k, rng.doub());
     }
 
     __device__ void measure_rotation() {   // system.cu:76-83
@@ -297,20 +315,20 @@ struct Chain {
     // metro_step (qms.cuh:373-545); returns the code, E/X through *Ed/*Xv when measured
     __device__ __noinline__ int metro_step(bool take, uint32_t max_rev, double* Ed, double* Xv) {
         int ret = 0;
-        {   // reset_non_syst_qbits (qms.cuh:128-150)
-            double rd[kMaxEne];
-#pragma unroll
-            for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
-                if (k < r) rd[k] = rng.doub();
-#pragma unroll
-            for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
-                if (k < r) reset(E0 + k, rd[k]);
-            reset(acc, rng.doub());
+        // reset_non_syst_qbits (qms.cuh:128-150): the E_old register first
+        if (two) reset_register(eo);
+        reset_register(en);
+        reset(acc, rng.doub());
+        uint32_t Eold;
+        if (two) {   // E_old into its own register, which keeps the measured value
+            Phi(eo);
+            Eold = measure_enes(eo);
+        } else {
+            Phi();
+            Eold = measure_enes();
+            for (uint32_t k = 0; k < r; ++k)
+                if ((Eold >> k) & 1u) x(en + k);
         }
-        Phi();
-        const uint32_t Eold = measure_enes();
-        for (uint32_t k = 0; k < r; ++k)
-            if ((Eold >> k) & 1u) x(E0 + k);
         uint32_t ci;
         {   // draw_C (qms.cuh:207-216, system.cu:123)
             const double e = rng.doub();
@@ -323,7 +341,7 @@ struct Chain {
             if (!take) return 1;
             *Ed = P->etab[Enew];
             for (uint32_t k = 0; k < r; ++k)
-                if ((Enew >> k) & 1u) x(E0 + k);
+                if ((Enew >> k) & 1u) x(en + k);
             *Xv = measure_X();
             reset_enes_each_draw();
             Phi();
@@ -403,17 +421,21 @@ __global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
     extern __shared__ double lds[];
     const uint32_t lane = threadIdx.x;
     for (;;) {
-        uint32_t chain = 0;
-        if (lane == 0) chain = atomicAdd(b.next_chain, 1u);
-        chain = __builtin_amdgcn_readfirstlane(__shfl(chain, 0, 64));
-        if (chain >= b.nchains) return;   // every wave reaches this once the queue is drained
+        uint32_t pos = 0;
+        if (lane == 0) pos = b.first + atomicAdd(b.next_chain, 1u);
+        pos = __builtin_amdgcn_readfirstlane(__shfl(pos, 0, 64));
+        if (pos >= b.nchains) return;   // every wave reaches this once the queue is drained
+        const uint32_t chain = b.order[pos];
         Chain<T> c;
         c.P = &b.chains[chain];
         c.r = c.P->r;
-        c.n = c.r + 4;
+        c.two = b.two_registers != 0;
+        c.n = c.two ? 2 * c.r + 4 : c.r + 4;
         c.N = 1u << c.n;
         c.L = 1u << c.r;
         c.acc = c.n - 1;
+        c.eo = E0;
+        c.en = c.two ? E0 + c.r : E0;
         c.lane = lane;
         c.re = reinterpret_cast<T*>(lds);
         c.im = c.re + c.N;
@@ -510,7 +532,9 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     double *d_w = nullptr, *d_E = nullptr, *d_X = nullptr;
     int32_t* d_rets = nullptr;
     uint64_t* d_rng = nullptr;
-    uint32_t *d_nm = nullptr, *d_acc = nullptr, *d_next = nullptr;
+    uint32_t *d_nm = nullptr, *d_acc = nullptr, *d_next = nullptr, *d_order = nullptr;
+    std::vector<size_t> need(nchains);
+    std::vector<uint32_t> order(nchains);
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const size_t nsteps = (size_t)nchains * p->metro_steps;
@@ -540,7 +564,19 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     QB_HIP(hipMalloc(&d_nm, sizeof(uint32_t) * nchains));
     QB_HIP(hipMalloc(&d_acc, sizeof(uint32_t) * nchains));
     QB_HIP(hipMalloc(&d_next, sizeof(uint32_t)));
+    // chains grouped by the LDS their state needs, widest first: when LDS
+    // limits the resident waves, one launch per group, so narrow chains are not
+    // held to the widest chain's occupancy
+    for (uint32_t c = 0; c < nchains; ++c) {
+        const uint32_t n = p->two_registers ? 2 * chains[c].ene_qbits + 4 : chains[c].ene_qbits + 4;
+        need[c] = (p->fp32 ? sizeof(float) : sizeof(double)) * 2 * ((size_t)1 << n);
+        order[c] = c;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return need[x] > need[y]; });
+    QB_HIP(hipMalloc(&d_order, sizeof(uint32_t) * nchains));
+    QB_HIP(hipMemcpy(d_order, order.data(), sizeof(uint32_t) * nchains, hipMemcpyHostToDevice));
     b.chains = d_chains;
+    b.order = d_order;
     b.nchains = nchains;
     b.steps = p->metro_steps;
     b.reset_each = p->reset_each;
@@ -554,24 +590,49 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     b.X = d_X;
     b.n_meas = d_nm;
     b.accepted = d_acc;
+    b.two_registers = p->two_registers ? 1u : 0u;
     {
         int dev = 0;
         QB_HIP(hipGetDevice(&dev));
         QB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    // one wave per chain, LDS sized for the widest chain
-    for (uint32_t c = 0; c < nchains; ++c) {
-        const size_t need = (p->fp32 ? sizeof(float) : sizeof(double)) * 2 * ((size_t)1 << (chains[c].ene_qbits + 4));
-        if (need > lds_bytes) lds_bytes = need;
+    lds_bytes = need[order[0]];   // the widest chain
+    {
+        int dev = 0, lds_max = 0;
+        QB_HIP(hipGetDevice(&dev));
+        QB_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        if (lds_bytes > (size_t)lds_max) {
+            fprintf(stderr, "suqa_qms_batch_run: a chain's state (%zu bytes) exceeds the %d bytes of LDS of a workgroup\n",
+                    lds_bytes, lds_max);
+            rc = SUQA_ERR_ARG;
+            goto done;
+        }
     }
-    grid = nchains < (uint32_t)ncu * 16u ? nchains : (uint32_t)ncu * 16u;
+    {
+        // One launch sized for the widest chain when every wave of the grid fits
+        // in LDS at that size; otherwise one launch per group of equal LDS need.
+        int dev = 0, lds_cu = 0;
+        QB_HIP(hipGetDevice(&dev));
+        QB_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+        const uint64_t waves_cu = (std::min<uint64_t>(nchains, (uint64_t)ncu * 16u) + ncu - 1) / ncu;
+        if (waves_cu * lds_bytes <= (uint64_t)lds_cu)
+            for (uint32_t c = 0; c < nchains; ++c) need[c] = lds_bytes;
+    }
     QB_HIP(hipEventCreate(&e0));
     QB_HIP(hipEventCreate(&e1));
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
     QB_HIP(hipEventRecord(e0, st));
-    if (p->fp32) hipLaunchKernelGGL(k_qms_batch<float>, dim3(grid), dim3(64), lds_bytes, st, b);
-    else hipLaunchKernelGGL(k_qms_batch<double>, dim3(grid), dim3(64), lds_bytes, st, b);
-    QB_HIP(hipGetLastError());
+    for (uint32_t lo = 0; lo < nchains;) {   // one wave per chain, LDS sized for the group
+        uint32_t hi = lo;
+        while (hi < nchains && need[order[hi]] == need[order[lo]]) ++hi;
+        b.first = lo;
+        b.nchains = hi;
+        grid = hi - lo < (uint32_t)ncu * 16u ? hi - lo : (uint32_t)ncu * 16u;
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
+        if (p->fp32) hipLaunchKernelGGL(k_qms_batch<float>, dim3(grid), dim3(64), need[order[lo]], st, b);
+        else hipLaunchKernelGGL(k_qms_batch<double>, dim3(grid), dim3(64), need[order[lo]], st, b);
+        QB_HIP(hipGetLastError());
+        lo = hi;
+    }
     QB_HIP(hipEventRecord(e1, st));
     QB_HIP(hipStreamSynchronize(st));
     QB_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -597,6 +658,7 @@ done:
     (void)hipFree(d_nm);
     (void)hipFree(d_acc);
     (void)hipFree(d_next);
+    (void)hipFree(d_order);
     if (st) (void)hipStreamDestroy(st);
     return rc;
 }

@@ -27,6 +27,8 @@ struct ChainDev {
 
 struct BatchDev {
     const ChainDev* chains;
+    const uint32_t* order;                  // chain indices; this launch runs order[first .. nchains)
+    uint32_t first;
     uint32_t nchains;
     uint32_t steps, reset_each, max_reverse;
     int32_t thermalization;
@@ -38,6 +40,7 @@ struct BatchDev {
     double* X;
     uint32_t* n_meas;                       // [nchains]
     uint32_t* accepted;                     // [nchains]
+    uint32_t two_registers;                 // E_old in its own register: 2r + 4 qubits per chain
 };
 
 }  // namespace qb

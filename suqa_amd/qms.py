@@ -34,7 +34,7 @@ class ChainSpec(C.Structure):
 class BatchParams(C.Structure):
     _fields_ = [("metro_steps", C.c_uint32), ("reset_each", C.c_uint32), ("max_reverse", C.c_uint32),
                 ("ene_min", C.c_double), ("ene_max", C.c_double), ("pe_steps", C.c_int32),
-                ("thermalization", C.c_int32), ("fp32", C.c_uint32)]
+                ("thermalization", C.c_int32), ("fp32", C.c_uint32), ("two_registers", C.c_uint32)]
 
 
 class BatchResult(C.Structure):
@@ -101,12 +101,14 @@ def run(beta, steps, reset_each, r, seed, ene_min=0.0, ene_max=1.0, pe_steps=10,
 
 
 def batch_run(chains, steps, reset_each=1, max_reverse=100, ene_min=0.0, ene_max=1.0, pe_steps=10,
-              thermalization=100, cap_meas=None, rng_after=False, fp32=False):
+              thermalization=100, cap_meas=None, rng_after=False, fp32=False, two_registers=False):
     """chains: list of (beta, r, seed).  Returns per-chain arrays.  fp32: FP32
-    amplitudes with FP64 probabilities (the precision pattern)."""
+    amplitudes with FP64 probabilities (the precision pattern).  two_registers:
+    E_old in its own register (2r + 4 qubits per chain, held in LDS)."""
     n = len(chains)
     specs = (ChainSpec * n)(*[ChainSpec(b, r, s) for (b, r, s) in chains])
-    bp = BatchParams(steps, reset_each, max_reverse, ene_min, ene_max, pe_steps, thermalization, 1 if fp32 else 0)
+    bp = BatchParams(steps, reset_each, max_reverse, ene_min, ene_max, pe_steps, thermalization, 1 if fp32 else 0,
+                     1 if two_registers else 0)
     cap = steps if cap_meas is None else cap_meas
     rets = np.zeros((n, steps), dtype=np.int32)
     rng = np.zeros((n, steps), dtype=np.uint64) if rng_after else None

@@ -69,3 +69,44 @@ def test_driver_option(qms):
         bad = subprocess.run([qms.DRIVER, "1.0", "10", "1", "3", out, "--ene-registers", "3"], capture_output=True,
                              text=True, timeout=120)
         assert bad.returncode != 0 and "ene registers" in (bad.stdout + bad.stderr)
+
+
+# beta, r, seed: one launch of the batched kernel with uneven widths (the
+# widest, r = 4, is a 12-qubit chain: 64 KiB of LDS in FP64)
+BATCH = [(1.0, 4, 42), (2.0, 3, 123), (0.5, 2, 7), (3.0, 4, 99), (0.2, 3, 5)]
+
+
+def test_two_register_batch_vs_oracle(qms):
+    """The batched kernel with two_registers: every chain reproduces the
+    oracle's two-register chain bit for bit (return codes, final pcg32 state,
+    acceptances, E/X)."""
+    steps, reset_each, therm, max_rev, emin, emax = 160, 1, 20, 100, -1.0, 3.0
+    print("two-register batch", flush=True)
+    out = qms.batch_run(BATCH, steps, reset_each, max_rev, emin, emax, 10, therm, rng_after=True, two_registers=True)
+    for c, (beta, r, seed) in enumerate(BATCH):
+        o = po.Qms(beta, r, seed, emin, emax, 10, max_rev, two_registers=True)
+        acc, rets = o.run(steps, reset_each, therm)
+        assert list(out["rets"][c]) == rets, f"chain {c} (beta={beta}, r={r}, seed={seed})"
+        assert int(out["accepted"][c]) == acc
+        assert int(out["rng_after"][c][-1]) == o.rng_state()
+        E, X = o.measures()
+        nm = int(out["n_meas"][c])
+        assert nm == len(E)
+        assert_bitwise(out["E"][c][:nm], np.array(E), f"E chain {c}")
+        assert_bitwise(out["X"][c][:nm], np.array(X), f"X chain {c}")
+
+
+def test_two_register_batch_fp32_and_limits(qms):
+    """FP32 two-register chains follow the FP64 ones (decisions differ only when
+    a draw falls within FP32 rounding of a probability); a chain whose state
+    does not fit a workgroup's LDS (r = 6 in FP64: 16 qubits, 1 MiB) is
+    rejected before anything is launched."""
+    steps, emin, emax = 120, -1.0, 3.0
+    print("two-register batch fp32", flush=True)
+    o64 = qms.batch_run(BATCH, steps, 1, 100, emin, emax, 10, 20, two_registers=True)
+    o32 = qms.batch_run(BATCH, steps, 1, 100, emin, emax, 10, 20, two_registers=True, fp32=True)
+    same = sum(1 for c in range(len(BATCH)) if np.array_equal(o64["rets"][c], o32["rets"][c]))
+    assert same >= len(BATCH) - 1
+    from suqa_amd import capi
+    with pytest.raises(capi.SuqaError):
+        qms.batch_run([(1.0, 6, 3)], 10, 1, 100, emin, emax, 10, 2, two_registers=True)
