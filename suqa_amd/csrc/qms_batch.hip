@@ -74,16 +74,22 @@ struct Rng {
     }
 };
 
-template <typename T>
-struct Chain {
-    T* re;               // LDS
-    T* im;
+// What every chain layout holds.
+struct ChainBase {
     uint32_t r, n, N, L, acc;
     uint32_t en, eo;     // first qubit of the E_new register, of the E_old register (two registers)
     bool two;            // E_old in its own register (qms.cuh:72-85, 134-136, 201-202)
-    uint32_t lane;
+    uint32_t lane;       // 0 on the one thread that writes a chain's outputs
     Rng rng;
     const ChainDev* P;
+};
+
+// The chain's state in the wave's LDS slice: one wavefront per chain, gates
+// as loops over 64-lane strides.
+template <typename T>
+struct LdsOps : ChainBase {
+    T* re;               // LDS
+    T* im;
 
     __device__ void sync() const { __syncthreads(); }
 
@@ -212,6 +218,161 @@ struct Chain {
         }
         sync();
     }
+
+};
+
+// One amplitude per thread (FP64, N <= 1024 threads, one workgroup per chain):
+// a gate exchanges with its partner lane (a shuffle inside the wavefront, LDS
+// across wavefronts) and every thread updates its own amplitude with the
+// arithmetic of LdsOps, so a gate costs one exchange instead of a loop.  For
+// chains that cannot fill the GPU (a single chain): latency, not throughput.
+struct RegOps : ChainBase {
+    double vr, vi;       // amplitude idx
+    uint32_t idx;
+    double* xr;          // LDS exchange buffers (N each) and reduction partials
+    double* xi;
+    double* red;
+
+    __device__ void sync() const { __syncthreads(); }
+
+    // amplitude idx ^ m; every thread of the workgroup calls it (m is uniform)
+    __device__ void partner(uint32_t m, double& pr, double& pi) {
+        if (m < 64u) {
+            pr = __shfl_xor(vr, (int)m, 64);
+            pi = __shfl_xor(vi, (int)m, 64);
+        } else {
+            xr[idx] = vr;
+            xi[idx] = vi;
+            __syncthreads();
+            pr = xr[idx ^ m];
+            pi = xi[idx ^ m];
+            __syncthreads();   // the next exchange overwrites the buffers
+        }
+    }
+
+    __device__ void init_zero() {
+        vr = idx == 0 ? 1.0 : 0.0;
+        vi = 0.0;
+    }
+
+    __device__ void h(uint32_t q, uint32_t gc) {   // suqa_cpu.hpp:186-204
+        const uint32_t qb = 1u << q;
+        double pr, pi;
+        partner(qb, pr, pi);
+        if (((idx & ~qb) & gc) != gc) return;
+        if (idx & qb) {   // b: kTwoSqInv * (a0 - a1)
+            vr = kTwoSqInv * (pr - vr);
+            vi = kTwoSqInv * (pi - vi);
+        } else {          // a: kTwoSqInv * (a0 + a1)
+            vr = kTwoSqInv * (vr + pr);
+            vi = kTwoSqInv * (vi + pi);
+        }
+    }
+
+    __device__ void mcx(uint32_t qt, uint32_t cm, uint32_t cv) {   // suqa_cpu.hpp:239-251
+        const uint32_t qb = 1u << qt;
+        double pr, pi;
+        partner(qb, pr, pi);
+        if (((idx | qb) & cm) != cv) return;
+        vr = pr;
+        vi = pi;
+    }
+    __device__ void x(uint32_t q) { mcx(q, 1u << q, 1u << q); }
+    __device__ void cx(uint32_t qc, uint32_t qt) {
+        const uint32_t m = (1u << qc) | (1u << qt);
+        mcx(qt, m, m);
+    }
+
+    __device__ void phase(uint32_t q1, uint32_t q2, double c, double s) {   // util_rotate2
+        const uint32_t m = (1u << q1) | (1u << q2);
+        if ((idx & m) != m) return;
+        const double cpy = vr;
+        vr = cpy * c - vi * s;
+        vi = vi * c + cpy * s;
+    }
+
+    __device__ void xx(uint32_t qa, uint32_t qb_, double c, double s, uint32_t gc) {   // suqa_cpu.hpp:422-424
+        const uint32_t pv = qa < qb_ ? qa : qb_;
+        const uint32_t X = (1u << qa) | (1u << qb_);
+        double pr, pi;
+        partner(X, pr, pi);
+        const uint32_t j = (idx >> pv) & 1u ? idx ^ X : idx;
+        if ((j & gc) != gc) return;
+        const double nr = vr * c - pi * s;   // both ends: own * c -/+ partner's other component * s
+        vi = vi * c + pr * s;
+        vr = nr;
+    }
+
+    __device__ uint32_t measure(uint32_t q, double rd) {   // suqa.cu:673-696
+        const double p = vr * vr + vi * vi;
+        const bool one = (idx >> q) & 1u;
+        double s0 = wave_sum(one ? 0.0 : p), s1 = wave_sum(one ? p : 0.0);
+        const uint32_t w = idx >> 6, nw = N >> 6;
+        if ((idx & 63u) == 0u) {
+            red[2 * w] = s0;
+            red[2 * w + 1] = s1;
+        }
+        __syncthreads();
+        double t0 = 0.0, t1 = 0.0;
+        for (uint32_t k = 0; k < nw; ++k) {
+            t0 += red[2 * k];
+            t1 += red[2 * k + 1];
+        }
+        __syncthreads();   // red is rewritten by the next measurement
+        const uint32_t cbit = rd < t1 ? 1u : 0u;
+        const double scale = 1. / sqrt(cbit ? t1 : t0);
+        const bool keep = (uint32_t)one == cbit;
+        vr = keep ? vr * scale : 0.0;
+        vi = keep ? vi * scale : 0.0;
+        return cbit;
+    }
+
+    __device__ void reset(uint32_t q, double rd) {   // suqa.cu:730-738
+        if (measure(q, rd)) x(q);
+    }
+
+    __device__ void W(uint32_t Eold) {   // qms.cuh:286-307: pairs (j, j + N/2)
+        const uint32_t half = N / 2;
+        double pr, pi;
+        partner(half, pr, pi);
+        const uint32_t i = idx | half;
+        const uint32_t En = (i >> en) & (L - 1);
+        const uint32_t Eo = two ? (i >> eo) & (L - 1) : Eold;
+        const double* row = P->wtab + 2 * (size_t)Eo * L;
+        const double fs1 = row[2 * En], fs2 = row[2 * En + 1];
+        if (idx < half) {   // j: fs2 t + fs1 ri
+            vr = fs2 * vr + fs1 * pr;
+            vi = fs2 * vi + fs1 * pi;
+        } else {            // i: fs1 t - fs2 ri
+            vr = fs1 * pr - fs2 * vr;
+            vi = fs1 * pi - fs2 * vi;
+        }
+    }
+};
+
+// The QMS control flow over either state layout.
+template <class Ops>
+struct Chain : Ops {
+    using Ops::r;
+    using Ops::n;
+    using Ops::N;
+    using Ops::L;
+    using Ops::acc;
+    using Ops::en;
+    using Ops::eo;
+    using Ops::two;
+    using Ops::lane;
+    using Ops::rng;
+    using Ops::P;
+    using Ops::init_zero;
+    using Ops::h;
+    using Ops::x;
+    using Ops::cx;
+    using Ops::phase;
+    using Ops::xx;
+    using Ops::measure;
+    using Ops::reset;
+    using Ops::W;
 
     // ------------------------------------------------------- QMS pieces
     __device__ void system_init() {   // system.cu:6-11
@@ -426,7 +587,7 @@ __global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
         pos = __builtin_amdgcn_readfirstlane(__shfl(pos, 0, 64));
         if (pos >= b.nchains) return;   // every wave reaches this once the queue is drained
         const uint32_t chain = b.order[pos];
-        Chain<T> c;
+        Chain<LdsOps<T>> c;
         c.P = &b.chains[chain];
         c.r = c.P->r;
         c.two = b.two_registers != 0;
@@ -442,6 +603,30 @@ __global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
         c.run(b, chain);
         __syncthreads();
     }
+}
+
+// One chain per workgroup of N threads, one amplitude per thread (RegOps).
+__global__ __launch_bounds__(1024) void k_qms_chain_reg(const BatchDev b) {
+    extern __shared__ double lds[];
+    const uint32_t pos = b.first + blockIdx.x;
+    if (pos >= b.nchains) return;   // uniform
+    const uint32_t chain = b.order[pos];
+    Chain<RegOps> c;
+    c.P = &b.chains[chain];
+    c.r = c.P->r;
+    c.two = b.two_registers != 0;
+    c.n = c.two ? 2 * c.r + 4 : c.r + 4;
+    c.N = 1u << c.n;
+    c.L = 1u << c.r;
+    c.acc = c.n - 1;
+    c.eo = E0;
+    c.en = c.two ? E0 + c.r : E0;
+    c.idx = threadIdx.x;
+    c.lane = threadIdx.x;
+    c.xr = lds;
+    c.xi = lds + c.N;
+    c.red = lds + 2 * c.N;
+    c.run(b, chain);
 }
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
@@ -544,6 +729,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     uint32_t grid = 0;
     size_t lds_bytes = 0;
     float ms = 0.f;
+    bool reg = false;
     QB_HIP(hipStreamCreate(&st));
     QB_HIP(hipMalloc(&d_w, sizeof(double) * wtotal));
     {
@@ -608,7 +794,11 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
             goto done;
         }
     }
-    {
+    // Fewer FP64 chains than CUs, each of at most 1024 amplitudes: a workgroup
+    // per chain with one amplitude per thread (latency), else a wavefront per
+    // chain (throughput).
+    reg = !p->fp32 && need[order[0]] <= 2 * sizeof(double) * 1024 && nchains <= (uint32_t)ncu;
+    if (!reg) {
         // One launch sized for the widest chain when every wave of the grid fits
         // in LDS at that size; otherwise one launch per group of equal LDS need.
         int dev = 0, lds_cu = 0;
@@ -628,8 +818,14 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
         b.nchains = hi;
         grid = hi - lo < (uint32_t)ncu * 16u ? hi - lo : (uint32_t)ncu * 16u;
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
-        if (p->fp32) hipLaunchKernelGGL(k_qms_batch<float>, dim3(grid), dim3(64), need[order[lo]], st, b);
-        else hipLaunchKernelGGL(k_qms_batch<double>, dim3(grid), dim3(64), need[order[lo]], st, b);
+        if (reg) {
+            const uint32_t Ng = (uint32_t)(need[order[lo]] / (2 * sizeof(double)));   // amplitudes per chain
+            hipLaunchKernelGGL(k_qms_chain_reg, dim3(hi - lo), dim3(Ng), (2 * (size_t)Ng + 32) * sizeof(double), st, b);
+        } else if (p->fp32) {
+            hipLaunchKernelGGL(k_qms_batch<float>, dim3(grid), dim3(64), need[order[lo]], st, b);
+        } else {
+            hipLaunchKernelGGL(k_qms_batch<double>, dim3(grid), dim3(64), need[order[lo]], st, b);
+        }
         QB_HIP(hipGetLastError());
         lo = hi;
     }

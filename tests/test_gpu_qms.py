@@ -73,15 +73,21 @@ def test_batch_single_chain_vs_golden(qms, fx):
     assert_bitwise(out["X"][0][:nm], hexarr(d["X"]), "X")
 
 
-def test_batch_mixed_chains_vs_oracle(qms):
-    """48 chains with register widths 2..6, betas and seeds all different, in one launch."""
+@pytest.mark.parametrize("nch", [48, 300], ids=["workgroup-per-chain", "wave-per-chain"])
+def test_batch_mixed_chains_vs_oracle(qms, nch):
+    """Chains with register widths 2..6, betas and seeds all different, in one
+    call: fewer chains than CUs run a workgroup per chain (one amplitude per
+    thread), more run a wavefront per chain (state in LDS); both bit-exact.
+    The 300-chain case checks every 6th chain."""
     rng = np.random.default_rng(9)
-    chains = [(float(0.1 + 4.9 * k / 47), int(2 + k % 5), int(1000 + k)) for k in range(48)]
+    chains = [(float(0.1 + 4.9 * k / (nch - 1)), int(2 + k % 5), int(1000 + k)) for k in range(nch)]
     rng.shuffle(chains)
     steps, reset_each, therm, maxrev, emin, emax = 120, 2, 10, 30, -1.0, 3.0
     log("batch mixed chains")
     out = qms.batch_run(chains, steps, reset_each, maxrev, emin, emax, 10, therm, rng_after=True)
     for c, (beta, r, seed) in enumerate(chains):
+        if nch > 48 and c % 6:
+            continue
         q = po.Qms(beta, r, seed, emin, emax, 10, maxrev)
         L = po.lib()
         L.so_qms_init_state(q.p)

@@ -76,14 +76,17 @@ def test_driver_option(qms):
 BATCH = [(1.0, 4, 42), (2.0, 3, 123), (0.5, 2, 7), (3.0, 4, 99), (0.2, 3, 5)]
 
 
-def test_two_register_batch_vs_oracle(qms):
+@pytest.mark.parametrize("chains", [BATCH, [(1.5, 3, 11), (0.7, 2, 13), (2.5, 3, 17)]], ids=["wave-per-chain", "workgroup-per-chain"])
+def test_two_register_batch_vs_oracle(qms, chains):
     """The batched kernel with two_registers: every chain reproduces the
     oracle's two-register chain bit for bit (return codes, final pcg32 state,
-    acceptances, E/X)."""
+    acceptances, E/X).  A batch with a 12-qubit chain runs a wavefront per
+    chain (state in LDS); a few chains of at most 1024 amplitudes run a
+    workgroup per chain (one amplitude per thread)."""
     steps, reset_each, therm, max_rev, emin, emax = 160, 1, 20, 100, -1.0, 3.0
-    print("two-register batch", flush=True)
-    out = qms.batch_run(BATCH, steps, reset_each, max_rev, emin, emax, 10, therm, rng_after=True, two_registers=True)
-    for c, (beta, r, seed) in enumerate(BATCH):
+    print(f"two-register batch {chains}", flush=True)
+    out = qms.batch_run(chains, steps, reset_each, max_rev, emin, emax, 10, therm, rng_after=True, two_registers=True)
+    for c, (beta, r, seed) in enumerate(chains):
         o = po.Qms(beta, r, seed, emin, emax, 10, max_rev, two_registers=True)
         acc, rets = o.run(steps, reset_each, therm)
         assert list(out["rets"][c]) == rets, f"chain {c} (beta={beta}, r={r}, seed={seed})"
