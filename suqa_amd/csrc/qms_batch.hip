@@ -390,7 +390,7 @@ struct Chain : Ops {
         xx(0, 1, c, s, gc);
     }
 
-    __device__ __noinline__ void Phi(uint32_t b) {   // qms.cuh:153-176 on the register at qubit b
+    __device__ void Phi(uint32_t b) {   // qms.cuh:153-176 on the register at qubit b
         for (uint32_t k = 0; k < r; ++k) h(b + k, 0u);
         for (int trg = (int)r - 1; trg > -1; --trg) {
             cevolution(b, (uint32_t)trg, 0);
@@ -403,7 +403,7 @@ struct Chain : Ops {
     }
     __device__ void Phi() { Phi(en); }
 
-    __device__ __noinline__ void Phi_inverse() {   // qms.cuh:179-198
+    __device__ void Phi_inverse() {   // qms.cuh:179-198
         for (int o = (int)r - 1; o >= 0; o--) {   // QFT (suqa.cu:635-647)
             h(en + o, 0u);
             for (int in = o - 1; in >= 0; in--) phase(en + in, en + o, P->qft_c[0][1 + o - in], P->qft_s[0][1 + o - in]);
@@ -415,19 +415,19 @@ struct Chain : Ops {
         for (uint32_t k = 0; k < r; ++k) h(en + k, 0u);
     }
 
-    __device__ __noinline__ void U(uint32_t ci, uint32_t Eold) {   // qms.cuh:335-348
+    __device__ void U(uint32_t ci, uint32_t Eold) {   // qms.cuh:335-348
         h(ci, 0u);
         Phi();
         W(Eold);
     }
-    __device__ __noinline__ void U_inverse(uint32_t ci, uint32_t Eold) {   // qms.cuh:350-360
+    __device__ void U_inverse(uint32_t ci, uint32_t Eold) {   // qms.cuh:350-360
         W(Eold);
         Phi_inverse();
         h(ci, 0u);
     }
 
     // measure_qbits(bm_enes, cs, extract_rands(r)): all draws first
-    __device__ __noinline__ uint32_t measure_enes(uint32_t b) {
+    __device__ uint32_t measure_enes(uint32_t b) {
         double rd[kMaxEne];
 #pragma unroll
         for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
@@ -441,7 +441,7 @@ struct Chain : Ops {
     __device__ uint32_t measure_enes() { return measure_enes(en); }
 
     // r draws, then the resets of the register at qubit b in order
-    __device__ __noinline__ void reset_register(uint32_t b) {
+    __device__ void reset_register(uint32_t b) {
         double rd[kMaxEne];
 #pragma unroll
         for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
@@ -451,7 +451,7 @@ struct Chain : Ops {
             if (k < r) reset(b + k, rd[k]);
     }
 
-    __device__ __noinline__ void reset_enes_each_draw() {
+    __device__ void reset_enes_each_draw() {
         for (uint32_t k = 0; k < r; ++k) reset(en + k, rng.doub());
     }
 
@@ -464,7 +464,7 @@ struct Chain : Ops {
         phase(2, 2, P->hp_c, P->hp_s);
     }
 
-    __device__ __noinline__ double measure_X() {   // system.cu:99-119, op_vals {2,0,2,0,-2,0,-2,0}
+    __device__ double measure_X() {   // system.cu:99-119, op_vals {2,0,2,0,-2,0,-2,0}
         measure_rotation();
         const double r0 = rng.doub(), r1 = rng.doub(), r2 = rng.doub();
         const uint32_t c0 = measure(0, r0), c1 = measure(1, r1), c2 = measure(2, r2);
@@ -474,7 +474,7 @@ struct Chain : Ops {
     }
 
     // metro_step (qms.cuh:373-545); returns the code, E/X through *Ed/*Xv when measured
-    __device__ __noinline__ int metro_step(bool take, uint32_t max_rev, double* Ed, double* Xv) {
+    __device__ int metro_step(bool take, uint32_t max_rev, double* Ed, double* Xv) {
         int ret = 0;
         // reset_non_syst_qbits (qms.cuh:128-150): the E_old register first
         if (two) reset_register(eo);
