@@ -174,7 +174,7 @@ def qms_legs(no_cpu):
     res["qms_single_chain_device"] = {
         "steps_per_s": round(steps / one["seconds"], 1), "kernel_seconds": round(one["seconds"], 5),
         "rets_match_plain": bool(np.array_equal(one["rets"][0], np.asarray(plain["rets"]))),
-        "config": "as qms_plain, run as a batch of one chain (one wavefront)"}
+        "config": "as qms_plain, run as a batch of one chain (one workgroup, one amplitude per thread)"}
     log(f"qms single chain on the device: {steps / one['seconds']:.1f} steps/s")
     # the same chain in the two-register layout the workload describes (E_old in
     # its own register: 3 + 2 * 4 + 1 = 12 qubits)
