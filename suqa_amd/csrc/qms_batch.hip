@@ -729,7 +729,6 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     uint32_t grid = 0;
     size_t lds_bytes = 0;
     float ms = 0.f;
-    bool reg = false;
     QB_HIP(hipStreamCreate(&st));
     QB_HIP(hipMalloc(&d_w, sizeof(double) * wtotal));
     {
@@ -794,11 +793,10 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
             goto done;
         }
     }
-    // Fewer FP64 chains than CUs, each of at most 1024 amplitudes: a workgroup
-    // per chain with one amplitude per thread (latency), else a wavefront per
-    // chain (throughput).
-    reg = !p->fp32 && need[order[0]] <= 2 * sizeof(double) * 1024 && nchains <= (uint32_t)ncu;
-    if (!reg) {
+    // FP64 chains of at most 1024 amplitudes run a workgroup per chain with one
+    // amplitude per thread (RegOps); the others a wavefront per chain with the
+    // state in LDS (LdsOps).
+    if (p->fp32 || need[order[0]] > 2 * sizeof(double) * 1024) {
         // One launch sized for the widest chain when every wave of the grid fits
         // in LDS at that size; otherwise one launch per group of equal LDS need.
         int dev = 0, lds_cu = 0;
@@ -818,7 +816,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
         b.nchains = hi;
         grid = hi - lo < (uint32_t)ncu * 16u ? hi - lo : (uint32_t)ncu * 16u;
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
-        if (reg) {
+        if (!p->fp32 && need[order[lo]] <= 2 * sizeof(double) * 1024) {
             const uint32_t Ng = (uint32_t)(need[order[lo]] / (2 * sizeof(double)));   // amplitudes per chain
             hipLaunchKernelGGL(k_qms_chain_reg, dim3(hi - lo), dim3(Ng), (2 * (size_t)Ng + 32) * sizeof(double), st, b);
         } else if (p->fp32) {

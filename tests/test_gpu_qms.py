@@ -73,12 +73,12 @@ def test_batch_single_chain_vs_golden(qms, fx):
     assert_bitwise(out["X"][0][:nm], hexarr(d["X"]), "X")
 
 
-@pytest.mark.parametrize("nch", [48, 300], ids=["workgroup-per-chain", "wave-per-chain"])
+@pytest.mark.parametrize("nch", [48, 300], ids=["48-chains", "300-chains"])
 def test_batch_mixed_chains_vs_oracle(qms, nch):
     """Chains with register widths 2..6, betas and seeds all different, in one
-    call: fewer chains than CUs run a workgroup per chain (one amplitude per
-    thread), more run a wavefront per chain (state in LDS); both bit-exact.
-    The 300-chain case checks every 6th chain."""
+    call (FP64 chains of at most 1024 amplitudes: a workgroup per chain, one
+    amplitude per thread, launched per register width), bit-exact.  The
+    300-chain case checks every 6th chain."""
     rng = np.random.default_rng(9)
     chains = [(float(0.1 + 4.9 * k / (nch - 1)), int(2 + k % 5), int(1000 + k)) for k in range(nch)]
     rng.shuffle(chains)

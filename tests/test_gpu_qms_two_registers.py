@@ -76,13 +76,13 @@ def test_driver_option(qms):
 BATCH = [(1.0, 4, 42), (2.0, 3, 123), (0.5, 2, 7), (3.0, 4, 99), (0.2, 3, 5)]
 
 
-@pytest.mark.parametrize("chains", [BATCH, [(1.5, 3, 11), (0.7, 2, 13), (2.5, 3, 17)]], ids=["wave-per-chain", "workgroup-per-chain"])
+@pytest.mark.parametrize("chains", [BATCH, [(1.5, 3, 11), (0.7, 2, 13), (2.5, 3, 17)]], ids=["r2-4", "r2-3"])
 def test_two_register_batch_vs_oracle(qms, chains):
     """The batched kernel with two_registers: every chain reproduces the
     oracle's two-register chain bit for bit (return codes, final pcg32 state,
-    acceptances, E/X).  A batch with a 12-qubit chain runs a wavefront per
-    chain (state in LDS); a few chains of at most 1024 amplitudes run a
-    workgroup per chain (one amplitude per thread)."""
+    acceptances, E/X).  The 12-qubit chains (r = 4) run a wavefront per chain
+    with the state in LDS, the others a workgroup per chain with one
+    amplitude per thread."""
     steps, reset_each, therm, max_rev, emin, emax = 160, 1, 20, 100, -1.0, 3.0
     print(f"two-register batch {chains}", flush=True)
     out = qms.batch_run(chains, steps, reset_each, max_rev, emin, emax, 10, therm, rng_after=True, two_registers=True)
