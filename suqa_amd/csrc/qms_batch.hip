@@ -1,12 +1,14 @@
-// qms_batch.hip -- thousands of independent QMS chains in one persistent kernel.
+// qms_batch.hip -- thousands of independent QMS chains in a few launches.
 //
-// One wavefront runs one chain at a time, taking chains from a work queue in
-// device memory.  The chain's state (2^n FP64 amplitudes, n = r + 4 <= 10,
-// SoA re/im) lives in the wave's slice of LDS (16 KiB at n = 10), so every
-// gate is a short loop over 64-lane strides with runtime qubit indices: the
-// code stays small for any register width and nothing spills.  A workgroup is
-// one wave, so __syncthreads() orders one gate's LDS writes before the next
-// gate's reads at the cost of a single-wave barrier.
+// Two layouts of a chain's state (2^n amplitudes, n = r + 4, or 2r + 4 with
+// E_old in its own register), with one control flow (Chain<Ops>):
+// * RegOps, FP64 chains of at most 1024 amplitudes: a workgroup of N threads per
+//   chain, one amplitude per thread.  A gate exchanges each amplitude with its
+//   partner lane (a shuffle inside the wavefront, LDS and a barrier across
+//   wavefronts); a measurement is a block reduction.
+// * LdsOps, wider chains and FP32 chains: one wavefront per chain, taking chains from a work
+//   queue in device memory, the state in the wave's slice of LDS, every gate
+//   a short loop over 64-lane strides.
 //
 // The control flow is metro_step (qms.cuh:373-545) with the same pcg32 draw
 // order; gate arithmetic keeps the reference's operation order
@@ -17,9 +19,9 @@
 // loop: amplitudes agree to ~1e-15, and outcomes are identical unless a
 // uniform draw falls within that distance of a probability).
 //
-// FP32 mode (the precision pattern): the same kernel over float amplitudes
-// (half the LDS per chain), with every measurement probability summed and the
-// collapse scale computed in FP64.  Its chains follow the FP64 ones until a
+// FP32 mode (the precision pattern): the same kernels over float amplitudes,
+// with every measurement probability summed and the collapse scale computed
+// in FP64.  Its chains follow the FP64 ones until a
 // uniform draw falls within the FP32 rounding of a probability.
 #include <hip/hip_runtime.h>
 
@@ -221,22 +223,23 @@ struct LdsOps : ChainBase {
 
 };
 
-// One amplitude per thread (FP64, N <= 1024 threads, one workgroup per chain):
+// One amplitude per thread (N <= 1024 threads, one workgroup per chain):
 // a gate exchanges with its partner lane (a shuffle inside the wavefront, LDS
 // across wavefronts) and every thread updates its own amplitude with the
 // arithmetic of LdsOps, so a gate costs one exchange instead of a loop.  For
 // chains that cannot fill the GPU (a single chain): latency, not throughput.
+template <typename T>
 struct RegOps : ChainBase {
-    double vr, vi;       // amplitude idx
+    T vr, vi;            // amplitude idx
     uint32_t idx;
-    double* xr;          // LDS exchange buffers (N each) and reduction partials
-    double* xi;
+    T* xr;               // LDS exchange buffers (N each) and reduction partials
+    T* xi;
     double* red;
 
     __device__ void sync() const { __syncthreads(); }
 
     // amplitude idx ^ m; every thread of the workgroup calls it (m is uniform)
-    __device__ void partner(uint32_t m, double& pr, double& pi) {
+    __device__ void partner(uint32_t m, T& pr, T& pi) {
         if (m < 64u) {
             pr = __shfl_xor(vr, (int)m, 64);
             pi = __shfl_xor(vi, (int)m, 64);
@@ -251,27 +254,27 @@ struct RegOps : ChainBase {
     }
 
     __device__ void init_zero() {
-        vr = idx == 0 ? 1.0 : 0.0;
-        vi = 0.0;
+        vr = idx == 0 ? T(1) : T(0);
+        vi = T(0);
     }
 
     __device__ void h(uint32_t q, uint32_t gc) {   // suqa_cpu.hpp:186-204
         const uint32_t qb = 1u << q;
-        double pr, pi;
+        T pr, pi;
         partner(qb, pr, pi);
         if (((idx & ~qb) & gc) != gc) return;
         if (idx & qb) {   // b: kTwoSqInv * (a0 - a1)
-            vr = kTwoSqInv * (pr - vr);
-            vi = kTwoSqInv * (pi - vi);
+            vr = T(kTwoSqInv) * (pr - vr);
+            vi = T(kTwoSqInv) * (pi - vi);
         } else {          // a: kTwoSqInv * (a0 + a1)
-            vr = kTwoSqInv * (vr + pr);
-            vi = kTwoSqInv * (vi + pi);
+            vr = T(kTwoSqInv) * (vr + pr);
+            vi = T(kTwoSqInv) * (vi + pi);
         }
     }
 
     __device__ void mcx(uint32_t qt, uint32_t cm, uint32_t cv) {   // suqa_cpu.hpp:239-251
         const uint32_t qb = 1u << qt;
-        double pr, pi;
+        T pr, pi;
         partner(qb, pr, pi);
         if (((idx | qb) & cm) != cv) return;
         vr = pr;
@@ -283,28 +286,31 @@ struct RegOps : ChainBase {
         mcx(qt, m, m);
     }
 
-    __device__ void phase(uint32_t q1, uint32_t q2, double c, double s) {   // util_rotate2
+    __device__ void phase(uint32_t q1, uint32_t q2, double cd, double sd) {   // util_rotate2
+        const T c = T(cd), s = T(sd);
         const uint32_t m = (1u << q1) | (1u << q2);
         if ((idx & m) != m) return;
-        const double cpy = vr;
+        const T cpy = vr;
         vr = cpy * c - vi * s;
         vi = vi * c + cpy * s;
     }
 
-    __device__ void xx(uint32_t qa, uint32_t qb_, double c, double s, uint32_t gc) {   // suqa_cpu.hpp:422-424
+    __device__ void xx(uint32_t qa, uint32_t qb_, double cd, double sd, uint32_t gc) {   // suqa_cpu.hpp:422-424
+        const T c = T(cd), s = T(sd);
         const uint32_t pv = qa < qb_ ? qa : qb_;
         const uint32_t X = (1u << qa) | (1u << qb_);
-        double pr, pi;
+        T pr, pi;
         partner(X, pr, pi);
         const uint32_t j = (idx >> pv) & 1u ? idx ^ X : idx;
         if ((j & gc) != gc) return;
-        const double nr = vr * c - pi * s;   // both ends: own * c -/+ partner's other component * s
+        const T nr = vr * c - pi * s;   // both ends: own * c -/+ partner's other component * s
         vi = vi * c + pr * s;
         vr = nr;
     }
 
     __device__ uint32_t measure(uint32_t q, double rd) {   // suqa.cu:673-696
-        const double p = vr * vr + vi * vi;
+        const double x = vr, y = vi;   // FP64 probabilities in both precisions
+        const double p = x * x + y * y;
         const bool one = (idx >> q) & 1u;
         double s0 = wave_sum(one ? 0.0 : p), s1 = wave_sum(one ? p : 0.0);
         const uint32_t w = idx >> 6, nw = N >> 6;
@@ -322,8 +328,8 @@ struct RegOps : ChainBase {
         const uint32_t cbit = rd < t1 ? 1u : 0u;
         const double scale = 1. / sqrt(cbit ? t1 : t0);
         const bool keep = (uint32_t)one == cbit;
-        vr = keep ? vr * scale : 0.0;
-        vi = keep ? vi * scale : 0.0;
+        vr = keep ? T((double)vr * scale) : T(0);
+        vi = keep ? T((double)vi * scale) : T(0);
         return cbit;
     }
 
@@ -333,13 +339,13 @@ struct RegOps : ChainBase {
 
     __device__ void W(uint32_t Eold) {   // qms.cuh:286-307: pairs (j, j + N/2)
         const uint32_t half = N / 2;
-        double pr, pi;
+        T pr, pi;
         partner(half, pr, pi);
         const uint32_t i = idx | half;
         const uint32_t En = (i >> en) & (L - 1);
         const uint32_t Eo = two ? (i >> eo) & (L - 1) : Eold;
         const double* row = P->wtab + 2 * (size_t)Eo * L;
-        const double fs1 = row[2 * En], fs2 = row[2 * En + 1];
+        const T fs1 = T(row[2 * En]), fs2 = T(row[2 * En + 1]);
         if (idx < half) {   // j: fs2 t + fs1 ri
             vr = fs2 * vr + fs1 * pr;
             vi = fs2 * vi + fs1 * pi;
@@ -606,12 +612,13 @@ __global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
 }
 
 // One chain per workgroup of N threads, one amplitude per thread (RegOps).
+template <typename T>
 __global__ __launch_bounds__(1024) void k_qms_chain_reg(const BatchDev b) {
     extern __shared__ double lds[];
     const uint32_t pos = b.first + blockIdx.x;
     if (pos >= b.nchains) return;   // uniform
     const uint32_t chain = b.order[pos];
-    Chain<RegOps> c;
+    Chain<RegOps<T>> c;
     c.P = &b.chains[chain];
     c.r = c.P->r;
     c.two = b.two_registers != 0;
@@ -623,9 +630,9 @@ __global__ __launch_bounds__(1024) void k_qms_chain_reg(const BatchDev b) {
     c.en = c.two ? E0 + c.r : E0;
     c.idx = threadIdx.x;
     c.lane = threadIdx.x;
-    c.xr = lds;
-    c.xi = lds + c.N;
-    c.red = lds + 2 * c.N;
+    c.red = lds;   // 32 doubles, then the exchange buffers
+    c.xr = reinterpret_cast<T*>(lds + 32);
+    c.xi = c.xr + c.N;
     c.run(b, chain);
 }
 
@@ -729,6 +736,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     uint32_t grid = 0;
     size_t lds_bytes = 0;
     float ms = 0.f;
+    const size_t elt = p->fp32 ? sizeof(float) : sizeof(double);
     QB_HIP(hipStreamCreate(&st));
     QB_HIP(hipMalloc(&d_w, sizeof(double) * wtotal));
     {
@@ -794,9 +802,10 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
         }
     }
     // FP64 chains of at most 1024 amplitudes run a workgroup per chain with one
-    // amplitude per thread (RegOps); the others a wavefront per chain with the
-    // state in LDS (LdsOps).
-    if (p->fp32 || need[order[0]] > 2 * sizeof(double) * 1024) {
+    // amplitude per thread (RegOps); wider ones, and FP32 chains (half the LDS
+    // per chain: more of them resident, 5.8 against 4.6 M chain-steps/s), a
+    // wavefront per chain with the state in LDS (LdsOps).
+    if (p->fp32 || need[order[0]] > 2 * elt * 1024) {
         // One launch sized for the widest chain when every wave of the grid fits
         // in LDS at that size; otherwise one launch per group of equal LDS need.
         int dev = 0, lds_cu = 0;
@@ -816,9 +825,10 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
         b.nchains = hi;
         grid = hi - lo < (uint32_t)ncu * 16u ? hi - lo : (uint32_t)ncu * 16u;
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
-        if (!p->fp32 && need[order[lo]] <= 2 * sizeof(double) * 1024) {
-            const uint32_t Ng = (uint32_t)(need[order[lo]] / (2 * sizeof(double)));   // amplitudes per chain
-            hipLaunchKernelGGL(k_qms_chain_reg, dim3(hi - lo), dim3(Ng), (2 * (size_t)Ng + 32) * sizeof(double), st, b);
+        if (!p->fp32 && need[order[lo]] <= 2 * elt * 1024) {
+            const uint32_t Ng = (uint32_t)(need[order[lo]] / (2 * elt));   // amplitudes per chain
+            const size_t sh = 32 * sizeof(double) + 2 * (size_t)Ng * elt;
+            hipLaunchKernelGGL(k_qms_chain_reg<double>, dim3(hi - lo), dim3(Ng), sh, st, b);
         } else if (p->fp32) {
             hipLaunchKernelGGL(k_qms_batch<float>, dim3(grid), dim3(64), need[order[lo]], st, b);
         } else {
