@@ -21,10 +21,9 @@
 // every group and precomputed on the host as 8-bit tables over the group's 8
 // elements, and a part on the other tile bits, tested once per group; the
 // per-element work is the gate arithmetic alone.  A gate that flips more than
-// 3 bits gets a step of its own, applied pair by pair in LDS, and so does
-// every gate when the whole state is one tile of at most 2^9 amplitudes (a
-// latency-bound workgroup, where a pair per thread beats 8 amplitudes per
-// thread on tsz / 8 threads).
+// 3 bits gets a step of its own, applied pair by pair in LDS.  A state that
+// is one tile of at most 2^10 amplitudes (a latency-bound workgroup) runs
+// k_tile_tiny instead: one amplitude per thread, one exchange per gate.
 //
 // Gates keep their order and every amplitude is computed with the one-gate
 // sweeps' arithmetic (sweep_math.hpp), so results are bit-identical to
@@ -46,7 +45,7 @@ namespace {
 constexpr uint32_t kMaxPass = 24;
 constexpr uint32_t kLowFixed = 4;                 // qubits 0..3: 16 amplitudes = one 128-byte line
 constexpr uint32_t kSub = 3, kLoc = 1u << kSub;   // register step: 3 bits, 8 amplitudes per thread
-constexpr uint32_t kTinyBits = 9;                 // one-tile states up to 2^9: LDS steps only
+constexpr uint32_t kTinyBits = 10;                // one-tile states up to 2^10: k_tile_tiny
 constexpr uint32_t kMaxChunks = (1u << kMaxTileBits) / 2 / kBlock;   // 16-byte chunks per array per thread
 
 typedef double dv2t __attribute__((ext_vector_type(2)));
@@ -122,19 +121,6 @@ __device__ __forceinline__ void tile_pairs(double* lre, double* lim, uint32_t ha
         lim[ia] = pair_out_par<OP>(true, true, ma, mb, rb, par_a, pc);
         lre[ib] = pair_out_par<OP>(false, false, rb, ra, ma, par_b, pc);
         lim[ib] = pair_out_par<OP>(false, true, mb, ma, ra, par_b, pc);
-    }
-}
-
-// LDS step: phase rotation over the whole tile, element per thread (the
-// arithmetic of reg_phase_one: s negated where the parity is odd).
-__device__ __forceinline__ void tile_phase(double* lre, double* lim, uint32_t tsz, const TileGate& g, uint32_t opar) {
-    for (uint32_t y = threadIdx.x; y < tsz; y += kBlock) {
-        if ((y & g.mc) != g.vc) continue;
-        const uint32_t par = ((uint32_t)__popc(y & g.yzc) & 1u) ^ opar;
-        const double sn = par ? -g.s : g.s;
-        const double rr = lre[y], mm = lim[y];
-        lre[y] = phase_out(false, rr, mm, g.c, sn);
-        lim[y] = phase_out(true, mm, rr, g.c, sn);
     }
 }
 
@@ -337,8 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
                 case OP_SWAP: tile_pairs<OP_SWAP>(lre, lim, tsz / 2, g, opar); break;
                 case OP_Y: tile_pairs<OP_Y>(lre, lim, tsz / 2, g, opar); break;
                 case OP_H: tile_pairs<OP_H>(lre, lim, tsz / 2, g, opar); break;
-                case OP_PAULI: tile_pairs<OP_PAULI>(lre, lim, tsz / 2, g, opar); break;
-                default: tile_phase(lre, lim, tsz, g, opar); break;
+                default: tile_pairs<OP_PAULI>(lre, lim, tsz / 2, g, opar); break;
                 }
             }
             __syncthreads();
@@ -395,6 +380,87 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
             }
         }
     }
+}
+
+// A pass over a state that is one tile of at most 2^kTinyBits amplitudes: a
+// workgroup of 2^t threads, one amplitude per thread.  A pair gate exchanges
+// each amplitude with its partner lane (a shuffle inside the wavefront, LDS
+// across wavefronts) and every thread computes its own outputs with the
+// arithmetic and roles of tile_pairs; a phase gate is element-wise as in
+// tile_phase.  Latency-bound states take one exchange per gate instead of
+// register steps on tsz / 8 threads.
+__global__ __launch_bounds__(1024) void k_tile_tiny(double* __restrict__ re, double* __restrict__ im, const TileArgs a) {
+    extern __shared__ double lds[];
+    const uint32_t tsz = 1u << a.t;
+    const uint32_t y = threadIdx.x;
+    const uint64_t base = a.fixval;   // one tile: every other state bit is fixed by the pass
+    const uint64_t off = base | deposit(y, a.tmask);
+    double vr = re[off], vi = im[off];
+    for (uint32_t k = 0; k < a.ngates; ++k) {
+        const TileGate& g = a.g[k];
+        if ((base & g.mo) != g.vo) continue;   // uniform: the tile is outside the gate's selection
+        const uint32_t opar = (uint32_t)__popcll(base & g.yzo) & 1u;
+        if (g.kind == (uint32_t)kPhaseOp) {
+            if ((y & g.mc) == g.vc) {
+                const uint32_t par = ((uint32_t)__popc(y & g.yzc) & 1u) ^ opar;
+                const double sn = par ? -g.s : g.s;
+                const double rr = vr, mm = vi;
+                vr = phase_out(false, rr, mm, g.c, sn);
+                vi = phase_out(true, mm, rr, g.c, sn);
+            }
+            continue;
+        }
+        const uint32_t w = y ^ g.xc;   // every thread takes part in the exchange
+        double pr, pi;
+        if (g.xc < 64u) {
+            pr = __shfl_xor(vr, (int)g.xc, 64);
+            pi = __shfl_xor(vi, (int)g.xc, 64);
+        } else {
+            lds[y] = vr;
+            lds[tsz + y] = vi;
+            __syncthreads();
+            pr = lds[w];
+            pi = lds[tsz + w];
+            __syncthreads();   // the next exchange overwrites the buffers
+        }
+        // roles of tile_pairs: u has the pivot bit 0; the selected one of (u, u ^ X) is the a side
+        const uint32_t u = ((y >> g.pivot) & 1u) ? w : y;
+        const uint32_t uw = u ^ g.xc;
+        const bool su = (u & g.mc) == g.vc;
+        if (!su && (uw & g.mc) != g.vc) continue;
+        const bool is_a = y == (su ? u : uw);
+        const uint32_t par = ((uint32_t)__popc(w & g.yzc) & 1u) ^ opar;   // parity of the partner
+        PauliCoef pc;
+        pc.c = g.c;
+        pc.s = g.s;
+        pc.yz = 0;
+        pc.ny_odd = 0;
+        pc.base_imag = g.base_imag;
+        pc.base_neg = g.base_neg;
+        double nr, ni;
+        switch (g.kind) {
+        case OP_SWAP:
+            nr = pair_out_par<OP_SWAP>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_SWAP>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        case OP_Y:
+            nr = pair_out_par<OP_Y>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_Y>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        case OP_H:
+            nr = pair_out_par<OP_H>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_H>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        default:
+            nr = pair_out_par<OP_PAULI>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_PAULI>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        }
+        vr = nr;
+        vi = ni;
+    }
+    re[off] = vr;
+    im[off] = vi;
 }
 
 uint32_t compact(uint64_t m, const uint8_t* tpos, uint32_t t) {
@@ -499,9 +565,8 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     a.fixval = vcom;
     a.ntiles = 1ull << __builtin_popcountll(a.freemask);
     // A state that is one small tile (t <= kTinyBits) is one workgroup and the
-    // pass is latency-bound: every gate takes an LDS step of its own (a pair or
-    // an element per thread, every thread busy) instead of sharing register
-    // steps among tsz / 8 threads.
+    // pass is latency-bound: k_tile_tiny, one amplitude per thread (the steps
+    // below are not used).
     const bool tiny = t <= kTinyBits && a.ntiles == 1;
     // register steps: runs of gates whose flipped tile bits fit in kSub bits
     a.nsteps = 0;
@@ -529,7 +594,7 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
         const uint32_t xc = a.g[i].xc;
         // register steps assume that a one-bit partner mask is selected at 0
         const bool pivot_zero = __builtin_popcount(xc) != 1 || (((a.g[i].mc & xc) != 0) && ((a.g[i].vc & xc) == 0));
-        if (tiny || (uint32_t)__builtin_popcount(xc) > kSub || !pivot_zero) {   // a step of its own, in LDS
+        if ((uint32_t)__builtin_popcount(xc) > kSub || !pivot_zero) {   // a step of its own, in LDS
             if (i > g0) close(i, false);
             g0 = i;
             close(i + 1, true);
@@ -550,7 +615,10 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     ++c->passes;
     c->pass_gates += n;
     timing_begin(c, "tile_pass", 32.0 * (double)((uint64_t)1 << t) * (double)a.ntiles);
-    hipLaunchKernelGGL(k_tile, dim3((uint32_t)g8), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a);
+    if (tiny)
+        hipLaunchKernelGGL(k_tile_tiny, dim3(1), dim3(1u << t), lds, c->stream, c->d_re, c->d_im, a);
+    else
+        hipLaunchKernelGGL(k_tile, dim3((uint32_t)g8), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a);
     timing_end(c);
     return hipGetLastError();
 }
