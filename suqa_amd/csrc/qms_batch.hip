@@ -232,9 +232,10 @@ template <typename T>
 struct RegOps : ChainBase {
     T vr, vi;            // amplitude idx
     uint32_t idx;
-    T* xr;               // LDS exchange buffers (N each) and reduction partials
-    T* xi;
-    double* red;
+    T* xr;               // LDS exchange buffers (2 x N each, alternating) and
+    T* xi;               // reduction partials (2 x 32, alternating): one barrier
+    double* red;         // per exchange or reduction, as the previous use of a
+    uint32_t xb, rb;     // buffer ended before the barrier of the use in between
 
     __device__ void sync() const { __syncthreads(); }
 
@@ -244,12 +245,14 @@ struct RegOps : ChainBase {
             pr = __shfl_xor(vr, (int)m, 64);
             pi = __shfl_xor(vi, (int)m, 64);
         } else {
-            xr[idx] = vr;
-            xi[idx] = vi;
+            T* br = xr + xb * N;
+            T* bi = xi + xb * N;
+            br[idx] = vr;
+            bi[idx] = vi;
             __syncthreads();
-            pr = xr[idx ^ m];
-            pi = xi[idx ^ m];
-            __syncthreads();   // the next exchange overwrites the buffers
+            pr = br[idx ^ m];
+            pi = bi[idx ^ m];
+            xb ^= 1u;
         }
     }
 
@@ -314,17 +317,18 @@ struct RegOps : ChainBase {
         const bool one = (idx >> q) & 1u;
         double s0 = wave_sum(one ? 0.0 : p), s1 = wave_sum(one ? p : 0.0);
         const uint32_t w = idx >> 6, nw = N >> 6;
+        double* pr = red + 32 * rb;
         if ((idx & 63u) == 0u) {
-            red[2 * w] = s0;
-            red[2 * w + 1] = s1;
+            pr[2 * w] = s0;
+            pr[2 * w + 1] = s1;
         }
         __syncthreads();
         double t0 = 0.0, t1 = 0.0;
         for (uint32_t k = 0; k < nw; ++k) {
-            t0 += red[2 * k];
-            t1 += red[2 * k + 1];
+            t0 += pr[2 * k];
+            t1 += pr[2 * k + 1];
         }
-        __syncthreads();   // red is rewritten by the next measurement
+        rb ^= 1u;
         const uint32_t cbit = rd < t1 ? 1u : 0u;
         const double scale = 1. / sqrt(cbit ? t1 : t0);
         const bool keep = (uint32_t)one == cbit;
@@ -630,9 +634,11 @@ __global__ __launch_bounds__(1024) void k_qms_chain_reg(const BatchDev b) {
     c.en = c.two ? E0 + c.r : E0;
     c.idx = threadIdx.x;
     c.lane = threadIdx.x;
-    c.red = lds;   // 32 doubles, then the exchange buffers
-    c.xr = reinterpret_cast<T*>(lds + 32);
-    c.xi = c.xr + c.N;
+    c.red = lds;   // 2 x 32 doubles, then the exchange buffers
+    c.xr = reinterpret_cast<T*>(lds + 64);
+    c.xi = c.xr + 2 * c.N;
+    c.xb = 0;
+    c.rb = 0;
     c.run(b, chain);
 }
 
@@ -827,7 +833,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
         if (!p->fp32 && need[order[lo]] <= 2 * elt * 1024) {
             const uint32_t Ng = (uint32_t)(need[order[lo]] / (2 * elt));   // amplitudes per chain
-            const size_t sh = 32 * sizeof(double) + 2 * (size_t)Ng * elt;
+            const size_t sh = 64 * sizeof(double) + 4 * (size_t)Ng * elt;
             hipLaunchKernelGGL(k_qms_chain_reg<double>, dim3(hi - lo), dim3(Ng), sh, st, b);
         } else if (p->fp32) {
             hipLaunchKernelGGL(k_qms_batch<float>, dim3(grid), dim3(64), need[order[lo]], st, b);

@@ -396,6 +396,7 @@ __global__ __launch_bounds__(1024) void k_tile_tiny(double* __restrict__ re, dou
     const uint64_t base = a.fixval;   // one tile: every other state bit is fixed by the pass
     const uint64_t off = base | deposit(y, a.tmask);
     double vr = re[off], vi = im[off];
+    uint32_t xb = 0;   // alternating exchange buffers: one barrier per exchange
     for (uint32_t k = 0; k < a.ngates; ++k) {
         const TileGate& g = a.g[k];
         if ((base & g.mo) != g.vo) continue;   // uniform: the tile is outside the gate's selection
@@ -415,13 +416,14 @@ __global__ __launch_bounds__(1024) void k_tile_tiny(double* __restrict__ re, dou
         if (g.xc < 64u) {
             pr = __shfl_xor(vr, (int)g.xc, 64);
             pi = __shfl_xor(vi, (int)g.xc, 64);
-        } else {
-            lds[y] = vr;
-            lds[tsz + y] = vi;
+        } else {   // the previous use of this buffer ended before the last exchange's barrier
+            double* bx = lds + 2 * tsz * xb;
+            bx[y] = vr;
+            bx[tsz + y] = vi;
             __syncthreads();
-            pr = lds[w];
-            pi = lds[tsz + w];
-            __syncthreads();   // the next exchange overwrites the buffers
+            pr = bx[w];
+            pi = bx[tsz + w];
+            xb ^= 1u;
         }
         // roles of tile_pairs: u has the pivot bit 0; the selected one of (u, u ^ X) is the a side
         const uint32_t u = ((y >> g.pivot) & 1u) ? w : y;
@@ -616,7 +618,7 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     c->pass_gates += n;
     timing_begin(c, "tile_pass", 32.0 * (double)((uint64_t)1 << t) * (double)a.ntiles);
     if (tiny)
-        hipLaunchKernelGGL(k_tile_tiny, dim3(1), dim3(1u << t), lds, c->stream, c->d_re, c->d_im, a);
+        hipLaunchKernelGGL(k_tile_tiny, dim3(1), dim3(1u << t), 2 * lds, c->stream, c->d_re, c->d_im, a);
     else
         hipLaunchKernelGGL(k_tile, dim3((uint32_t)g8), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a);
     timing_end(c);
