@@ -44,9 +44,37 @@ namespace {
 constexpr double kTwoSqInv = 0.7071067811865475;   // suqa.cuh:57
 constexpr int E0 = 3;                              // first energy qubit (qms.cuh:79-88)
 
+// The value of lane id ^ m for m = 1, 2, 4, 8 (DPP moves inside a row of 16
+// lanes: xor 1, 2 are quad permutes; xor 4 = mirror within 8, then xor 3;
+// xor 8 = mirror within 16, then within 8), on both dwords of a double.
+template <int C1, int C2>
+__device__ __forceinline__ double dpp_xor(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    int lo = (int)(uint32_t)u, hi = (int)(uint32_t)(u >> 32);
+    lo = __builtin_amdgcn_mov_dpp(lo, C1, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, C1, 0xF, 0xF, true);
+    if constexpr (C2 >= 0) {
+        lo = __builtin_amdgcn_mov_dpp(lo, C2, 0xF, 0xF, true);
+        hi = __builtin_amdgcn_mov_dpp(hi, C2, 0xF, 0xF, true);
+    }
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Butterfly sum over the 64 lanes (every lane ends with a sum of all; lane
+// 0's is the one used): DPP moves for xor 1..8, a swizzle for 16, a permute
+// for 32 (12 two-dword permutes before: 24 LDS-path round trips).
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    v += dpp_xor<0xB1, -1>(v);
+    v += dpp_xor<0x4E, -1>(v);
+    v += dpp_xor<0x141, 0x1B>(v);
+    v += dpp_xor<0x140, 0x141>(v);
+    {
+        const uint64_t u = (uint64_t)__double_as_longlong(v);
+        const int lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)u, 0x401F);   // xor 16
+        const int hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), 0x401F);
+        v += __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+    }
+    v += __shfl_xor(v, 32, 64);
     return v;
 }
 
@@ -311,40 +339,11 @@ struct RegOps : ChainBase {
         vr = nr;
     }
 
-    // Lane 0 of each wavefront ends with the wave's sum (butterfly: DPP moves
-    // for lane xor 1, 2, 4, 8, a swizzle for 16, a permute for 32).
-    template <int C1, int C2>
-    static __device__ __forceinline__ double dpp_xor(double v) {
-        const uint64_t u = (uint64_t)__double_as_longlong(v);
-        int lo = (int)(uint32_t)u, hi = (int)(uint32_t)(u >> 32);
-        lo = __builtin_amdgcn_mov_dpp(lo, C1, 0xF, 0xF, true);
-        hi = __builtin_amdgcn_mov_dpp(hi, C1, 0xF, 0xF, true);
-        if constexpr (C2 >= 0) {
-            lo = __builtin_amdgcn_mov_dpp(lo, C2, 0xF, 0xF, true);
-            hi = __builtin_amdgcn_mov_dpp(hi, C2, 0xF, 0xF, true);
-        }
-        return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
-    }
-    static __device__ __forceinline__ double wave_sum_dpp(double v) {
-        v += dpp_xor<0xB1, -1>(v);     // xor 1
-        v += dpp_xor<0x4E, -1>(v);     // xor 2
-        v += dpp_xor<0x141, 0x1B>(v);  // xor 4: mirror within 8, then xor 3
-        v += dpp_xor<0x140, 0x141>(v); // xor 8: mirror within 16, then within 8
-        {
-            const uint64_t u = (uint64_t)__double_as_longlong(v);
-            const int lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)u, 0x401F);   // xor 16
-            const int hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), 0x401F);
-            v += __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
-        }
-        v += __shfl_xor(v, 32, 64);
-        return v;
-    }
-
     __device__ uint32_t measure(uint32_t q, double rd) {   // suqa.cu:673-696
         const double x = vr, y = vi;   // FP64 probabilities in both precisions
         const double p = x * x + y * y;
         const bool one = (idx >> q) & 1u;
-        double s0 = wave_sum_dpp(one ? 0.0 : p), s1 = wave_sum_dpp(one ? p : 0.0);
+        double s0 = wave_sum(one ? 0.0 : p), s1 = wave_sum(one ? p : 0.0);
         const uint32_t w = idx >> 6, nw = N >> 6;
         double* pr = red + 32 * rb;
         if ((idx & 63u) == 0u) {
