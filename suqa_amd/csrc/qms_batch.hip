@@ -121,10 +121,10 @@ struct LdsOps : ChainBase {
     T* re;               // LDS
     T* im;
 
-    __device__ void sync() const { __syncthreads(); }
+    __device__ __forceinline__ void sync() const { __syncthreads(); }
 
     // ------------------------------------------------------------ gates
-    __device__ void init_zero() {
+    __device__ __forceinline__ void init_zero() {
         for (uint32_t i = lane; i < N; i += 64) {
             re[i] = i == 0 ? T(1) : T(0);
             im[i] = T(0);
@@ -133,7 +133,7 @@ struct LdsOps : ChainBase {
     }
 
     // H on q (suqa_cpu.hpp:186-204) for states with (i & gc) == gc
-    __device__ void h(uint32_t q, uint32_t gc) {
+    __device__ __forceinline__ void h(uint32_t q, uint32_t gc) {
         for (uint32_t p = lane; p < N / 2; p += 64) {
             const uint32_t a = ins0(p, q);
             if ((a & gc) != gc) continue;
@@ -148,7 +148,7 @@ struct LdsOps : ChainBase {
     }
 
     // X on qt for states whose control bits read cv (suqa_cpu.hpp:239-251)
-    __device__ void mcx(uint32_t qt, uint32_t cm, uint32_t cv) {
+    __device__ __forceinline__ void mcx(uint32_t qt, uint32_t cm, uint32_t cv) {
         for (uint32_t p = lane; p < N / 2; p += 64) {
             const uint32_t a = ins0(p, qt);
             const uint32_t b = a | (1u << qt);
@@ -162,14 +162,14 @@ struct LdsOps : ChainBase {
         }
         sync();
     }
-    __device__ void x(uint32_t q) { mcx(q, 1u << q, 1u << q); }
-    __device__ void cx(uint32_t qc, uint32_t qt) {
+    __device__ __forceinline__ void x(uint32_t q) { mcx(q, 1u << q, 1u << q); }
+    __device__ __forceinline__ void cx(uint32_t qc, uint32_t qt) {
         const uint32_t m = (1u << qc) | (1u << qt);
         mcx(qt, m, m);
     }
 
     // util_rotate2 on states with all bits of m set (u1 / cu1 with q_mask 1)
-    __device__ void phase(uint32_t q1, uint32_t q2, double cd, double sd) {
+    __device__ __forceinline__ void phase(uint32_t q1, uint32_t q2, double cd, double sd) {
         const T c = T(cd), s = T(sd);
         const uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
         const uint32_t m = (1u << q1) | (1u << q2);
@@ -184,7 +184,7 @@ struct LdsOps : ChainBase {
     }
 
     // exp(i phi X_a X_b) on states with (i & gc) == gc (suqa_cpu.hpp:422-424)
-    __device__ void xx(uint32_t qa, uint32_t qb_, double cd, double sd, uint32_t gc) {
+    __device__ __forceinline__ void xx(uint32_t qa, uint32_t qb_, double cd, double sd, uint32_t gc) {
         const T c = T(cd), s = T(sd);
         const uint32_t pv = qa < qb_ ? qa : qb_;
         const uint32_t X = (1u << qa) | (1u << qb_);
@@ -202,7 +202,7 @@ struct LdsOps : ChainBase {
     }
 
     // measure_qbit (suqa.cu:673-696): c = (rd < P(q=1)), collapse, renormalize
-    __device__ uint32_t measure(uint32_t q, double rd) {
+    __device__ __forceinline__ uint32_t measure(uint32_t q, double rd) {
         double s0 = 0.0, s1 = 0.0;
         for (uint32_t i = lane; i < N; i += 64) {
             const double a = re[i], b = im[i];   // FP64 probabilities in both modes
@@ -223,13 +223,13 @@ struct LdsOps : ChainBase {
         return c;
     }
 
-    __device__ void reset(uint32_t q, double rd) {   // suqa.cu:730-738
+    __device__ __forceinline__ void reset(uint32_t q, double rd) {   // suqa.cu:730-738
         if (measure(q, rd)) x(q);
     }
 
     // qms::apply_W, dense form (qms.cuh:286-307); acc is the top qubit.  With
     // two registers E_old is read per amplitude from its register.
-    __device__ void W(uint32_t Eold) {
+    __device__ __forceinline__ void W(uint32_t Eold) {
         const uint32_t half = N / 2;
         for (uint32_t j = lane; j < half; j += 64) {
             const uint32_t i = j + half;
@@ -265,10 +265,10 @@ struct RegOps : ChainBase {
     double* red;         // per exchange or reduction, as the previous use of a
     uint32_t xb, rb;     // buffer ended before the barrier of the use in between
 
-    __device__ void sync() const { __syncthreads(); }
+    __device__ __forceinline__ void sync() const { __syncthreads(); }
 
     // amplitude idx ^ m; every thread of the workgroup calls it (m is uniform)
-    __device__ void partner(uint32_t m, T& pr, T& pi) {
+    __device__ __forceinline__ void partner(uint32_t m, T& pr, T& pi) {
         if (m < 64u) {
             pr = __shfl_xor(vr, (int)m, 64);
             pi = __shfl_xor(vi, (int)m, 64);
@@ -284,12 +284,12 @@ struct RegOps : ChainBase {
         }
     }
 
-    __device__ void init_zero() {
+    __device__ __forceinline__ void init_zero() {
         vr = idx == 0 ? T(1) : T(0);
         vi = T(0);
     }
 
-    __device__ void h(uint32_t q, uint32_t gc) {   // suqa_cpu.hpp:186-204
+    __device__ __forceinline__ void h(uint32_t q, uint32_t gc) {   // suqa_cpu.hpp:186-204
         const uint32_t qb = 1u << q;
         T pr, pi;
         partner(qb, pr, pi);
@@ -303,7 +303,7 @@ struct RegOps : ChainBase {
         }
     }
 
-    __device__ void mcx(uint32_t qt, uint32_t cm, uint32_t cv) {   // suqa_cpu.hpp:239-251
+    __device__ __forceinline__ void mcx(uint32_t qt, uint32_t cm, uint32_t cv) {   // suqa_cpu.hpp:239-251
         const uint32_t qb = 1u << qt;
         T pr, pi;
         partner(qb, pr, pi);
@@ -311,13 +311,13 @@ struct RegOps : ChainBase {
         vr = pr;
         vi = pi;
     }
-    __device__ void x(uint32_t q) { mcx(q, 1u << q, 1u << q); }
-    __device__ void cx(uint32_t qc, uint32_t qt) {
+    __device__ __forceinline__ void x(uint32_t q) { mcx(q, 1u << q, 1u << q); }
+    __device__ __forceinline__ void cx(uint32_t qc, uint32_t qt) {
         const uint32_t m = (1u << qc) | (1u << qt);
         mcx(qt, m, m);
     }
 
-    __device__ void phase(uint32_t q1, uint32_t q2, double cd, double sd) {   // util_rotate2
+    __device__ __forceinline__ void phase(uint32_t q1, uint32_t q2, double cd, double sd) {   // util_rotate2
         const T c = T(cd), s = T(sd);
         const uint32_t m = (1u << q1) | (1u << q2);
         if ((idx & m) != m) return;
@@ -326,7 +326,7 @@ struct RegOps : ChainBase {
         vi = vi * c + cpy * s;
     }
 
-    __device__ void xx(uint32_t qa, uint32_t qb_, double cd, double sd, uint32_t gc) {   // suqa_cpu.hpp:422-424
+    __device__ __forceinline__ void xx(uint32_t qa, uint32_t qb_, double cd, double sd, uint32_t gc) {   // suqa_cpu.hpp:422-424
         const T c = T(cd), s = T(sd);
         const uint32_t pv = qa < qb_ ? qa : qb_;
         const uint32_t X = (1u << qa) | (1u << qb_);
@@ -339,7 +339,7 @@ struct RegOps : ChainBase {
         vr = nr;
     }
 
-    __device__ uint32_t measure(uint32_t q, double rd) {   // suqa.cu:673-696
+    __device__ __forceinline__ uint32_t measure(uint32_t q, double rd) {   // suqa.cu:673-696
         const double x = vr, y = vi;   // FP64 probabilities in both precisions
         const double p = x * x + y * y;
         const bool one = (idx >> q) & 1u;
@@ -365,11 +365,11 @@ struct RegOps : ChainBase {
         return cbit;
     }
 
-    __device__ void reset(uint32_t q, double rd) {   // suqa.cu:730-738
+    __device__ __forceinline__ void reset(uint32_t q, double rd) {   // suqa.cu:730-738
         if (measure(q, rd)) x(q);
     }
 
-    __device__ void W(uint32_t Eold) {   // qms.cuh:286-307: pairs (j, j + N/2)
+    __device__ __forceinline__ void W(uint32_t Eold) {   // qms.cuh:286-307: pairs (j, j + N/2)
         const uint32_t half = N / 2;
         T pr, pi;
         partner(half, pr, pi);
@@ -413,14 +413,14 @@ struct Chain : Ops {
     using Ops::W;
 
     // ------------------------------------------------------- QMS pieces
-    __device__ void system_init() {   // system.cu:6-11
+    __device__ __forceinline__ void system_init() {   // system.cu:6-11
         init_zero();
         x(1);
         h(1, 0u);
         cx(1, 0);
     }
 
-    __device__ void cevolution(uint32_t base, uint32_t trg, int dir) {   // qms.cuh:106-115, system.cu:21-26
+    __device__ __forceinline__ void cevolution(uint32_t base, uint32_t trg, int dir) {   // qms.cuh:106-115, system.cu:21-26
         const uint32_t gc = 1u << (base + trg);
         const double c = P->xx_c[dir][trg], s = P->xx_s[dir][trg];
         xx(1, 2, c, s, gc);
@@ -428,7 +428,7 @@ struct Chain : Ops {
         xx(0, 1, c, s, gc);
     }
 
-    __device__ void Phi(uint32_t b) {   // qms.cuh:153-176 on the register at qubit b
+    __device__ __forceinline__ void Phi(uint32_t b) {   // qms.cuh:153-176 on the register at qubit b
         for (uint32_t k = 0; k < r; ++k) h(b + k, 0u);
         for (int trg = (int)r - 1; trg > -1; --trg) {
             cevolution(b, (uint32_t)trg, 0);
@@ -439,9 +439,9 @@ struct Chain : Ops {
             h(b + o, 0u);
         }
     }
-    __device__ void Phi() { Phi(en); }
+    __device__ __forceinline__ void Phi() { Phi(en); }
 
-    __device__ void Phi_inverse() {   // qms.cuh:179-198
+    __device__ __forceinline__ void Phi_inverse() {   // qms.cuh:179-198
         for (int o = (int)r - 1; o >= 0; o--) {   // QFT (suqa.cu:635-647)
             h(en + o, 0u);
             for (int in = o - 1; in >= 0; in--) phase(en + in, en + o, P->qft_c[0][1 + o - in], P->qft_s[0][1 + o - in]);
@@ -453,19 +453,19 @@ struct Chain : Ops {
         for (uint32_t k = 0; k < r; ++k) h(en + k, 0u);
     }
 
-    __device__ void U(uint32_t ci, uint32_t Eold) {   // qms.cuh:335-348
+    __device__ __forceinline__ void U(uint32_t ci, uint32_t Eold) {   // qms.cuh:335-348
         h(ci, 0u);
         Phi();
         W(Eold);
     }
-    __device__ void U_inverse(uint32_t ci, uint32_t Eold) {   // qms.cuh:350-360
+    __device__ __forceinline__ void U_inverse(uint32_t ci, uint32_t Eold) {   // qms.cuh:350-360
         W(Eold);
         Phi_inverse();
         h(ci, 0u);
     }
 
     // measure_qbits(bm_enes, cs, extract_rands(r)): all draws first
-    __device__ uint32_t measure_enes(uint32_t b) {
+    __device__ __forceinline__ uint32_t measure_enes(uint32_t b) {
         double rd[kMaxEne];
 #pragma unroll
         for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
@@ -476,10 +476,10 @@ struct Chain : Ops {
             if (k < r) v += measure(b + k, rd[k]) << k;
         return v;
     }
-    __device__ uint32_t measure_enes() { return measure_enes(en); }
+    __device__ __forceinline__ uint32_t measure_enes() { return measure_enes(en); }
 
     // r draws, then the resets of the register at qubit b in order
-    __device__ void reset_register(uint32_t b) {
+    __device__ __forceinline__ void reset_register(uint32_t b) {
         double rd[kMaxEne];
 #pragma unroll
         for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
@@ -489,11 +489,11 @@ struct Chain : Ops {
             if (k < r) reset(b + k, rd[k]);
     }
 
-    __device__ void reset_enes_each_draw() {
+    __device__ __forceinline__ void reset_enes_each_draw() {
         for (uint32_t k = 0; k < r; ++k) reset(en + k, rng.doub());
     }
 
-    __device__ void measure_rotation() {   // system.cu:76-83
+    __device__ __forceinline__ void measure_rotation() {   // system.cu:76-83
         h(0, 0u);
         h(1, 0u);
         h(2, 0u);
@@ -502,7 +502,7 @@ struct Chain : Ops {
         phase(2, 2, P->hp_c, P->hp_s);
     }
 
-    __device__ double measure_X() {   // system.cu:99-119, op_vals {2,0,2,0,-2,0,-2,0}
+    __device__ __forceinline__ double measure_X() {   // system.cu:99-119, op_vals {2,0,2,0,-2,0,-2,0}
         measure_rotation();
         const double r0 = rng.doub(), r1 = rng.doub(), r2 = rng.doub();
         const uint32_t c0 = measure(0, r0), c1 = measure(1, r1), c2 = measure(2, r2);
@@ -512,7 +512,7 @@ struct Chain : Ops {
     }
 
     // metro_step (qms.cuh:373-545); returns the code, E/X through *Ed/*Xv when measured
-    __device__ int metro_step(bool take, uint32_t max_rev, double* Ed, double* Xv) {
+    __device__ __forceinline__ int metro_step(bool take, uint32_t max_rev, double* Ed, double* Xv) {
         int ret = 0;
         // reset_non_syst_qbits (qms.cuh:128-150): the E_old register first
         if (two) reset_register(eo);
@@ -579,7 +579,7 @@ struct Chain : Ops {
     }
 
     // driver loop of qms.cu:132-149
-    __device__ void run(const BatchDev& b, uint32_t chain) {
+    __device__ __forceinline__ void run(const BatchDev& b, uint32_t chain) {
         rng.seed(P->seed);
         system_init();
         uint32_t s0 = 0, acc_count = 0, nm = 0;
@@ -771,6 +771,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     const size_t nmeas = (size_t)nchains * out->cap_meas;
     BatchDev b{};
     int ncu = 256;
+    bool reg = false;   // the per-amplitude layout for FP64 one-register chains
     uint32_t grid = 0;
     size_t lds_bytes = 0;
     float ms = 0.f;
@@ -839,11 +840,14 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
             goto done;
         }
     }
-    // FP64 chains of at most 1024 amplitudes run a workgroup per chain with one
-    // amplitude per thread (RegOps); wider ones, and FP32 chains (half the LDS
-    // per chain: more of them resident, 5.8 against 4.6 M chain-steps/s), a
-    // wavefront per chain with the state in LDS (LdsOps).
-    if (p->fp32 || need[order[0]] > 2 * elt * 1024) {
+    // FP64 one-register chains of at most 1024 amplitudes run a workgroup per
+    // chain with one amplitude per thread (RegOps: 6.3 against 5.9 M
+    // chain-steps/s for the 4096-chain batch with r in 3..6); wider ones,
+    // two-register chains (1.20 against 1.14 M with r in 3..4) and FP32 chains
+    // (half the LDS per chain: 11 M) a wavefront per chain with the state in
+    // LDS (LdsOps).
+    reg = !p->fp32 && !p->two_registers;
+    if (!reg || need[order[0]] > 2 * elt * 1024) {
         // One launch sized for the widest chain when every wave of the grid fits
         // in LDS at that size; otherwise one launch per group of equal LDS need.
         int dev = 0, lds_cu = 0;
@@ -863,7 +867,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
         b.nchains = hi;
         grid = hi - lo < (uint32_t)ncu * 16u ? hi - lo : (uint32_t)ncu * 16u;
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
-        if (!p->fp32 && need[order[lo]] <= 2 * elt * 1024) {
+        if (reg && need[order[lo]] <= 2 * elt * 1024) {
             const uint32_t Ng = (uint32_t)(need[order[lo]] / (2 * elt));   // amplitudes per chain
             const size_t sh = 64 * sizeof(double) + 4 * (size_t)Ng * elt;
             // more chains than four waves per SIMD hold at once: the occupancy
