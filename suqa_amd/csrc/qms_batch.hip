@@ -357,7 +357,9 @@ struct RegOps : ChainBase {
             t1 += pr[2 * k + 1];
         }
         rb ^= 1u;
-        const uint32_t cbit = rd < t1 ? 1u : 0u;
+        // every thread read the same partials: uniform, and said so, so that the
+        // control flow that depends on it stays scalar
+        const uint32_t cbit = __builtin_amdgcn_readfirstlane(rd < t1 ? 1u : 0u);
         const double scale = 1. / sqrt(cbit ? t1 : t0);
         const bool keep = (uint32_t)one == cbit;
         vr = keep ? T((double)vr * scale) : T(0);
@@ -464,29 +466,23 @@ struct Chain : Ops {
         h(ci, 0u);
     }
 
-    // measure_qbits(bm_enes, cs, extract_rands(r)): all draws first
+    // measure_qbits(bm_enes, cs, extract_rands(r)): the r draws first, then the
+    // measurements in order; the draws are replayed from a copy of the
+    // generator, so the loop stays rolled and holds no array
     __device__ __forceinline__ uint32_t measure_enes(uint32_t b) {
-        double rd[kMaxEne];
-#pragma unroll
-        for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
-            if (k < r) rd[k] = rng.doub();
+        Rng d = rng;
+        for (uint32_t k = 0; k < r; ++k) (void)rng.doub();
         uint32_t v = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
-            if (k < r) v += measure(b + k, rd[k]) << k;
+        for (uint32_t k = 0; k < r; ++k) v |= measure(b + k, d.doub()) << k;
         return v;
     }
-    __device__ __forceinline__ uint32_t measure_enes() { return measure_enes(en); }
 
+    __device__ __forceinline__ uint32_t measure_enes() { return measure_enes(en); }
     // r draws, then the resets of the register at qubit b in order
     __device__ __forceinline__ void reset_register(uint32_t b) {
-        double rd[kMaxEne];
-#pragma unroll
-        for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
-            if (k < r) rd[k] = rng.doub();
-#pragma unroll
-        for (uint32_t k = 0; k < (uint32_t)kMaxEne; ++k)
-            if (k < r) reset(b + k, rd[k]);
+        Rng d = rng;
+        for (uint32_t k = 0; k < r; ++k) (void)rng.doub();
+        for (uint32_t k = 0; k < r; ++k) reset(b + k, d.doub());
     }
 
     __device__ __forceinline__ void reset_enes_each_draw() {
@@ -503,11 +499,14 @@ struct Chain : Ops {
     }
 
     __device__ __forceinline__ double measure_X() {   // system.cu:99-119, op_vals {2,0,2,0,-2,0,-2,0}
-        measure_rotation();
-        const double r0 = rng.doub(), r1 = rng.doub(), r2 = rng.doub();
-        const uint32_t c0 = measure(0, r0), c1 = measure(1, r1), c2 = measure(2, r2);
-        measure_rotation();
-        const uint32_t meas = c0 | (c1 << 1) | (c2 << 2);
+        Rng d = rng;   // three draws, then the three measurements
+        for (int k = 0; k < 3; ++k) (void)rng.doub();
+        uint32_t meas = 0;
+        for (int rep = 0; rep < 2; ++rep) {
+            measure_rotation();
+            if (rep == 0)
+                for (uint32_t q = 0; q < 3; ++q) meas |= measure(q, d.doub()) << q;
+        }
         return (meas & 1u) ? 0.0 : ((meas & 4u) ? -2.0 : 2.0);
     }
 
