@@ -114,18 +114,20 @@ struct ChainBase {
     const ChainDev* P;
 };
 
-// The chain's state in the wave's LDS slice: one wavefront per chain, gates
-// as loops over 64-lane strides.
-template <typename T>
+// The chain's state in the workgroup's LDS: TW wavefronts per chain (one for
+// most chains; more where LDS holds only a few chains per CU), gates as loops
+// over 64 * TW-lane strides.
+template <typename T, int TW>
 struct LdsOps : ChainBase {
     T* re;               // LDS
     T* im;
+    double* red;         // LDS: the waves' partial sums of a measurement (TW > 1)
 
     __device__ __forceinline__ void sync() const { __syncthreads(); }
 
     // ------------------------------------------------------------ gates
     __device__ __forceinline__ void init_zero() {
-        for (uint32_t i = lane; i < N; i += 64) {
+        for (uint32_t i = lane; i < N; i += 64 * TW) {
             re[i] = i == 0 ? T(1) : T(0);
             im[i] = T(0);
         }
@@ -134,7 +136,7 @@ struct LdsOps : ChainBase {
 
     // H on q (suqa_cpu.hpp:186-204) for states with (i & gc) == gc
     __device__ __forceinline__ void h(uint32_t q, uint32_t gc) {
-        for (uint32_t p = lane; p < N / 2; p += 64) {
+        for (uint32_t p = lane; p < N / 2; p += 64 * TW) {
             const uint32_t a = ins0(p, q);
             if ((a & gc) != gc) continue;
             const uint32_t b = a | (1u << q);
@@ -149,7 +151,7 @@ struct LdsOps : ChainBase {
 
     // X on qt for states whose control bits read cv (suqa_cpu.hpp:239-251)
     __device__ __forceinline__ void mcx(uint32_t qt, uint32_t cm, uint32_t cv) {
-        for (uint32_t p = lane; p < N / 2; p += 64) {
+        for (uint32_t p = lane; p < N / 2; p += 64 * TW) {
             const uint32_t a = ins0(p, qt);
             const uint32_t b = a | (1u << qt);
             if ((b & cm) != cv) continue;
@@ -174,7 +176,7 @@ struct LdsOps : ChainBase {
         const uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
         const uint32_t m = (1u << q1) | (1u << q2);
         const uint32_t cnt = q1 == q2 ? N / 2 : N / 4;
-        for (uint32_t p = lane; p < cnt; p += 64) {
+        for (uint32_t p = lane; p < cnt; p += 64 * TW) {
             const uint32_t i = (q1 == q2 ? ins0(p, lo) : ins0(ins0(p, lo), hi)) | m;
             const T cpy = re[i];
             re[i] = cpy * c - im[i] * s;
@@ -188,7 +190,7 @@ struct LdsOps : ChainBase {
         const T c = T(cd), s = T(sd);
         const uint32_t pv = qa < qb_ ? qa : qb_;
         const uint32_t X = (1u << qa) | (1u << qb_);
-        for (uint32_t p = lane; p < N / 2; p += 64) {
+        for (uint32_t p = lane; p < N / 2; p += 64 * TW) {
             const uint32_t j = ins0(p, pv);
             if ((j & gc) != gc) continue;
             const uint32_t k = j ^ X;
@@ -204,7 +206,7 @@ struct LdsOps : ChainBase {
     // measure_qbit (suqa.cu:673-696): c = (rd < P(q=1)), collapse, renormalize
     __device__ __forceinline__ uint32_t measure(uint32_t q, double rd) {
         double s0 = 0.0, s1 = 0.0;
-        for (uint32_t i = lane; i < N; i += 64) {
+        for (uint32_t i = lane; i < N; i += 64 * TW) {
             const double a = re[i], b = im[i];   // FP64 probabilities in both modes
             const double p = a * a + b * b;
             if ((i >> q) & 1u) s1 += p;
@@ -212,9 +214,22 @@ struct LdsOps : ChainBase {
         }
         s0 = wave_sum(s0);
         s1 = wave_sum(s1);
+        if constexpr (TW > 1) {   // the last use of red ended at a barrier (the collapse below)
+            if ((lane & 63u) == 0u) {
+                red[2 * (lane >> 6)] = s0;
+                red[2 * (lane >> 6) + 1] = s1;
+            }
+            __syncthreads();
+            s0 = 0.0;
+            s1 = 0.0;
+            for (int w = 0; w < TW; ++w) {
+                s0 += red[2 * w];
+                s1 += red[2 * w + 1];
+            }
+        }
         const uint32_t c = __builtin_amdgcn_readfirstlane((uint32_t)(rd < s1));
         const double scale = 1. / sqrt(c ? s1 : s0);
-        for (uint32_t i = lane; i < N; i += 64) {
+        for (uint32_t i = lane; i < N; i += 64 * TW) {
             const bool keep = ((i >> q) & 1u) == c;
             re[i] = keep ? T((double)re[i] * scale) : T(0);
             im[i] = keep ? T((double)im[i] * scale) : T(0);
@@ -231,7 +246,7 @@ struct LdsOps : ChainBase {
     // two registers E_old is read per amplitude from its register.
     __device__ __forceinline__ void W(uint32_t Eold) {
         const uint32_t half = N / 2;
-        for (uint32_t j = lane; j < half; j += 64) {
+        for (uint32_t j = lane; j < half; j += 64 * TW) {
             const uint32_t i = j + half;
             const uint32_t En = (i >> en) & (L - 1);
             const uint32_t Eo = two ? (i >> eo) & (L - 1) : Eold;
@@ -614,17 +629,25 @@ struct Chain : Ops {
     }
 };
 
-template <typename T>
-__global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
+template <typename T, int TW>
+__global__ __launch_bounds__(64 * TW) void k_qms_batch(const BatchDev b) {
     extern __shared__ double lds[];
     const uint32_t lane = threadIdx.x;
     for (;;) {
         uint32_t pos = 0;
-        if (lane == 0) pos = b.first + atomicAdd(b.next_chain, 1u);
-        pos = __builtin_amdgcn_readfirstlane(__shfl(pos, 0, 64));
-        if (pos >= b.nchains) return;   // every wave reaches this once the queue is drained
+        if constexpr (TW == 1) {
+            if (lane == 0) pos = b.first + atomicAdd(b.next_chain, 1u);
+            pos = __builtin_amdgcn_readfirstlane(__shfl(pos, 0, 64));
+        } else {   // through LDS; the previous chain's last reads ended at a barrier
+            uint32_t* q = reinterpret_cast<uint32_t*>(lds);
+            if (lane == 0) *q = b.first + atomicAdd(b.next_chain, 1u);
+            __syncthreads();
+            pos = __builtin_amdgcn_readfirstlane(*q);
+            __syncthreads();
+        }
+        if (pos >= b.nchains) return;   // every workgroup reaches this once the queue is drained
         const uint32_t chain = b.order[pos];
-        Chain<LdsOps<T>> c;
+        Chain<LdsOps<T, TW>> c;
         c.P = &b.chains[chain];
         c.r = c.P->r;
         c.two = b.two_registers != 0;
@@ -637,6 +660,7 @@ __global__ __launch_bounds__(64) void k_qms_batch(const BatchDev b) {
         c.lane = lane;
         c.re = reinterpret_cast<T*>(lds);
         c.im = c.re + c.N;
+        c.red = reinterpret_cast<double*>(c.im + c.N);
         c.run(b, chain);
         __syncthreads();
     }
@@ -769,7 +793,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     const size_t nsteps = (size_t)nchains * p->metro_steps;
     const size_t nmeas = (size_t)nchains * out->cap_meas;
     BatchDev b{};
-    int ncu = 256;
+    int ncu = 256, lds_max = 0, lds_cu = 0;
     bool reg = false;   // the per-amplitude layout for FP64 one-register chains
     uint32_t grid = 0;
     size_t lds_bytes = 0;
@@ -826,12 +850,11 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
         int dev = 0;
         QB_HIP(hipGetDevice(&dev));
         QB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        QB_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        QB_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     }
     lds_bytes = need[order[0]];   // the widest chain
     {
-        int dev = 0, lds_max = 0;
-        QB_HIP(hipGetDevice(&dev));
-        QB_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         if (lds_bytes > (size_t)lds_max) {
             fprintf(stderr, "suqa_qms_batch_run: a chain's state (%zu bytes) exceeds the %d bytes of LDS of a workgroup\n",
                     lds_bytes, lds_max);
@@ -849,9 +872,6 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     if (!reg || need[order[0]] > 2 * elt * 1024) {
         // One launch sized for the widest chain when every wave of the grid fits
         // in LDS at that size; otherwise one launch per group of equal LDS need.
-        int dev = 0, lds_cu = 0;
-        QB_HIP(hipGetDevice(&dev));
-        QB_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
         const uint64_t waves_cu = (std::min<uint64_t>(nchains, (uint64_t)ncu * 16u) + ncu - 1) / ncu;
         if (waves_cu * lds_bytes <= (uint64_t)lds_cu)
             for (uint32_t c = 0; c < nchains; ++c) need[c] = lds_bytes;
@@ -876,10 +896,31 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
                 hipLaunchKernelGGL((k_qms_chain_reg<double, 8>), dim3(hi - lo), dim3(Ng), sh, st, b);
             else
                 hipLaunchKernelGGL((k_qms_chain_reg<double, 4>), dim3(hi - lo), dim3(Ng), sh, st, b);
-        } else if (p->fp32) {
-            hipLaunchKernelGGL(k_qms_batch<float>, dim3(grid), dim3(64), need[order[lo]], st, b);
         } else {
-            hipLaunchKernelGGL(k_qms_batch<double>, dim3(grid), dim3(64), need[order[lo]], st, b);
+            // waves per chain: where LDS holds fewer than 8 chains per CU, 2 or 4
+            // waves per chain keep the SIMDs busy (the measurement's partial sums
+            // follow the state)
+            const uint64_t per_cu = lds_cu / need[order[lo]];
+            const int tw = per_cu >= 8 ? 1 : (per_cu >= 4 ? 2 : 4);
+            const size_t sh = need[order[lo]] + (tw > 1 ? 2 * sizeof(double) * tw : 0);
+            if (sh > (size_t)lds_max) {
+                fprintf(stderr, "suqa_qms_batch_run: %zu bytes of LDS per chain exceed the %d of a workgroup\n", sh,
+                        lds_max);
+                rc = SUQA_ERR_ARG;
+                goto done;
+            }
+#define QB_LAUNCH_LDS(TYPE, TWV) \
+    hipLaunchKernelGGL((k_qms_batch<TYPE, TWV>), dim3(grid), dim3(64 * TWV), sh, st, b)
+            if (p->fp32) {
+                if (tw == 1) QB_LAUNCH_LDS(float, 1);
+                else if (tw == 2) QB_LAUNCH_LDS(float, 2);
+                else QB_LAUNCH_LDS(float, 4);
+            } else {
+                if (tw == 1) QB_LAUNCH_LDS(double, 1);
+                else if (tw == 2) QB_LAUNCH_LDS(double, 2);
+                else QB_LAUNCH_LDS(double, 4);
+            }
+#undef QB_LAUNCH_LDS
         }
         QB_HIP(hipGetLastError());
         lo = hi;
