@@ -2,13 +2,15 @@
 //
 // Two layouts of a chain's state (2^n amplitudes, n = r + 4, or 2r + 4 with
 // E_old in its own register), with one control flow (Chain<Ops>):
-// * RegOps, FP64 chains of at most 1024 amplitudes: a workgroup of N threads per
-//   chain, one amplitude per thread.  A gate exchanges each amplitude with its
-//   partner lane (a shuffle inside the wavefront, LDS and a barrier across
-//   wavefronts); a measurement is a block reduction.
-// * LdsOps, wider chains and FP32 chains: one wavefront per chain, taking chains from a work
-//   queue in device memory, the state in the wave's slice of LDS, every gate
-//   a short loop over 64-lane strides.
+// * RegOps, batches of FP64 one-register chains no larger than the CU count
+//   (latency first): a workgroup of N threads per chain, one amplitude per
+//   thread.  A gate exchanges each amplitude with its partner lane (a shuffle
+//   inside the wavefront, LDS and a barrier across wavefronts); a measurement
+//   is a block reduction.
+// * LdsOps, everything else (throughput): 1 to 8 wavefronts per chain, the
+//   fewer chains LDS holds per CU the more, taking chains from a work queue in
+//   device memory, the state in the workgroup's LDS, every gate a short loop
+//   over the workgroup's lanes.
 //
 // The control flow is metro_step (qms.cuh:373-545) with the same pcg32 draw
 // order; gate arithmetic keeps the reference's operation order
@@ -114,9 +116,9 @@ struct ChainBase {
     const ChainDev* P;
 };
 
-// The chain's state in the workgroup's LDS: TW wavefronts per chain (one for
-// most chains; more where LDS holds only a few chains per CU), gates as loops
-// over 64 * TW-lane strides.
+// The chain's state in the workgroup's LDS: TW wavefronts per chain (1 to 8,
+// more the fewer chains LDS holds per CU), gates as loops over 64 * TW-lane
+// strides.
 template <typename T, int TW>
 struct LdsOps : ChainBase {
     T* re;               // LDS
@@ -667,12 +669,8 @@ __global__ __launch_bounds__(64 * TW) void k_qms_batch(const BatchDev b) {
 }
 
 // One chain per workgroup of N threads, one amplitude per thread (RegOps).
-// W is the fewest waves per SIMD the register allocation must allow: 4 (what
-// 1024 threads a workgroup imply; ~100 VGPRs, no spills) runs one chain
-// fastest, 8 (64 VGPRs, a few spills) keeps twice the workgroups resident per
-// CU and runs a large batch ~20 % faster.
-template <typename T, int W>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_qms_chain_reg(const BatchDev b) {
+template <typename T>
+__global__ __launch_bounds__(1024) void k_qms_chain_reg(const BatchDev b) {
     extern __shared__ double lds[];
     const uint32_t pos = b.first + blockIdx.x;
     if (pos >= b.nchains) return;   // uniform
@@ -862,13 +860,13 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
             goto done;
         }
     }
-    // FP64 one-register chains of at most 1024 amplitudes run a workgroup per
-    // chain with one amplitude per thread (RegOps: 6.3 against 5.9 M
-    // chain-steps/s for the 4096-chain batch with r in 3..6); wider ones,
-    // two-register chains (1.20 against 1.14 M with r in 3..4) and FP32 chains
-    // (half the LDS per chain: 11 M) a wavefront per chain with the state in
-    // LDS (LdsOps).
-    reg = !p->fp32 && !p->two_registers;
+    // Batches of no more chains than CUs (latency: a single chain runs 27 k
+    // steps/s against 14 k in LDS) of FP64 one-register chains of at most 1024
+    // amplitudes run a workgroup per chain with one amplitude per thread
+    // (RegOps); larger batches, wider, two-register and FP32 chains run in LDS
+    // with 1 to 8 wavefronts per chain (LdsOps: 8.3 against 6.3 M chain-steps/s
+    // for 4096 FP64 chains with r in 3..6).
+    reg = !p->fp32 && !p->two_registers && nchains <= (uint32_t)ncu;
     if (!reg || need[order[0]] > 2 * elt * 1024) {
         // One launch sized for the widest chain when every wave of the grid fits
         // in LDS at that size; otherwise one launch per group of equal LDS need.
@@ -889,19 +887,13 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
         if (reg && need[order[lo]] <= 2 * elt * 1024) {
             const uint32_t Ng = (uint32_t)(need[order[lo]] / (2 * elt));   // amplitudes per chain
             const size_t sh = 64 * sizeof(double) + 4 * (size_t)Ng * elt;
-            // more chains than four waves per SIMD hold at once: the occupancy
-            // variant; otherwise the one without spills
-            const uint32_t fit4 = (uint32_t)ncu * std::max<uint32_t>(1u, 1024u / Ng);
-            if (hi - lo > fit4)
-                hipLaunchKernelGGL((k_qms_chain_reg<double, 8>), dim3(hi - lo), dim3(Ng), sh, st, b);
-            else
-                hipLaunchKernelGGL((k_qms_chain_reg<double, 4>), dim3(hi - lo), dim3(Ng), sh, st, b);
+            hipLaunchKernelGGL(k_qms_chain_reg<double>, dim3(hi - lo), dim3(Ng), sh, st, b);
         } else {
-            // waves per chain: where LDS holds fewer than 8 chains per CU, 2 or 4
-            // waves per chain keep the SIMDs busy (the measurement's partial sums
-            // follow the state)
+            // waves per chain: the fewer chains LDS holds per CU, the more waves
+            // each (up to 8), so the SIMDs stay busy; the measurement's partial
+            // sums follow the state
             const uint64_t per_cu = lds_cu / need[order[lo]];
-            const int tw = per_cu >= 8 ? 1 : (per_cu >= 4 ? 2 : 4);
+            const int tw = per_cu >= 32 ? 1 : (per_cu >= 16 ? 2 : (per_cu >= 4 ? 4 : 8));
             const size_t sh = need[order[lo]] + (tw > 1 ? 2 * sizeof(double) * tw : 0);
             if (sh > (size_t)lds_max) {
                 fprintf(stderr, "suqa_qms_batch_run: %zu bytes of LDS per chain exceed the %d of a workgroup\n", sh,
@@ -914,11 +906,13 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
             if (p->fp32) {
                 if (tw == 1) QB_LAUNCH_LDS(float, 1);
                 else if (tw == 2) QB_LAUNCH_LDS(float, 2);
-                else QB_LAUNCH_LDS(float, 4);
+                else if (tw == 4) QB_LAUNCH_LDS(float, 4);
+                else QB_LAUNCH_LDS(float, 8);
             } else {
                 if (tw == 1) QB_LAUNCH_LDS(double, 1);
                 else if (tw == 2) QB_LAUNCH_LDS(double, 2);
-                else QB_LAUNCH_LDS(double, 4);
+                else if (tw == 4) QB_LAUNCH_LDS(double, 4);
+                else QB_LAUNCH_LDS(double, 8);
             }
 #undef QB_LAUNCH_LDS
         }

@@ -123,12 +123,15 @@ def oracle_rets(beta, r, seed, steps, reset_each, therm, maxrev, emin, emax):
     return rets, E, X
 
 
-def test_batch_fp32_chains_follow_fp64(qms):
+@pytest.mark.parametrize("widths", [5, 2], ids=["r2-6", "r2-3"])
+def test_batch_fp32_chains_follow_fp64(qms, widths):
     """FP32 amplitudes with FP64 probabilities: a chain's decisions differ from
     the FP64 oracle only when a uniform draw falls within FP32 rounding of a
     probability, so nearly all of 48 chains x 120 steps reproduce the oracle's
-    return codes exactly, and those that do carry the same E and X values."""
-    chains = [(float(0.1 + 4.9 * k / 47), int(2 + k % 5), int(1000 + k)) for k in range(48)]
+    return codes exactly, and those that do carry the same E and X values.
+    r in 2..6 runs two wavefronts per chain (8 KiB of LDS for the widest);
+    r in 2..3 (1 KiB) one wavefront per chain."""
+    chains = [(float(0.1 + 4.9 * k / 47), int(2 + k % widths), int(1000 + k)) for k in range(48)]
     steps, reset_each, therm, maxrev, emin, emax = 120, 2, 10, 30, -1.0, 3.0
     log("batch fp32 chains")
     out = qms.batch_run(chains, steps, reset_each, maxrev, emin, emax, 10, therm, fp32=True)
