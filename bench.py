@@ -43,7 +43,7 @@ The QMS half of the metric is reported under "extra":
   qms_batch_two_registers -- the same kind of batch in the two-register
                 layout (3 + 2r + 1 qubits per chain) for r in {3, 4}, FP64
                 and FP32, 4 sample chains checked against the oracle;
-  qms_single_chain_device -- the c1 chain as a batch of one (one wavefront
+  qms_single_chain_device -- the c1 chain as a batch of one (one workgroup
                  runs it on the device), its steps/s and rets against qms_plain;
   cpu_baseline_qms -- the oracle on the same single chain, 1 core; for the
                  batch, a 64-chain sample on one oracle process per chain over
@@ -168,7 +168,7 @@ def qms_legs(no_cpu):
                          "accepted": int(plain["accepted"]), "seconds": round(plain["seconds"], 4),
                          "config": "beta=1.0, 1000 steps, r=4 (8 qubits), E in [-1,3], seed 42, reset_each 1"}}
     log(f"qms plain: {steps / plain['seconds']:.1f} steps/s")
-    # the same chain as the only chain of the batched kernel: one wavefront runs the
+    # the same chain as the only chain of the batched kernel: one workgroup runs the
     # whole chain on the device (no launches, no host round trips per measurement)
     one = qms.batch_run([(beta, r, seed)], steps, 1, 100, emin, emax, 10, 100)
     res["qms_single_chain_device"] = {
