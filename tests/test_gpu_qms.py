@@ -73,14 +73,16 @@ def test_batch_single_chain_vs_golden(qms, fx):
     assert_bitwise(out["X"][0][:nm], hexarr(d["X"]), "X")
 
 
-@pytest.mark.parametrize("nch", [48, 300], ids=["48-chains", "300-chains"])
-def test_batch_mixed_chains_vs_oracle(qms, nch):
-    """Chains with register widths 2..6, betas and seeds all different, in one
-    call (FP64 chains of at most 1024 amplitudes: a workgroup per chain, one
-    amplitude per thread, launched per register width), bit-exact.  The
-    300-chain case checks every 6th chain."""
+@pytest.mark.parametrize("nch,widths", [(48, (2, 3, 4, 5, 6)), (300, (2, 3, 4, 5, 6)), (300, (2, 3)), (300, (5,))],
+                         ids=["48-chains", "300-chains", "300-chains-r2-3", "300-chains-r5"])
+def test_batch_mixed_chains_vs_oracle(qms, nch, widths):
+    """Chains with the given register widths, betas and seeds all different,
+    in one call, bit-exact.  48 chains (no more than the CUs) run a workgroup
+    per chain with one amplitude per thread, launched per register width; 300
+    chains run in LDS with 4 (widest r = 6: 16 KiB per chain), 1 (r <= 3) or 2
+    (r = 5) wavefronts per chain.  The 300-chain cases check every 6th chain."""
     rng = np.random.default_rng(9)
-    chains = [(float(0.1 + 4.9 * k / (nch - 1)), int(2 + k % 5), int(1000 + k)) for k in range(nch)]
+    chains = [(float(0.1 + 4.9 * k / (nch - 1)), int(widths[k % len(widths)]), int(1000 + k)) for k in range(nch)]
     rng.shuffle(chains)
     steps, reset_each, therm, maxrev, emin, emax = 120, 2, 10, 30, -1.0, 3.0
     log("batch mixed chains")
