@@ -8,9 +8,10 @@
  * per-step results handed back.
  *
  * suqa_qms_batch_run: many independent chains (frustrated triangle, uneven
- * energy-register widths, betas and seeds) in one persistent GPU kernel, each
+ * energy-register widths, betas and seeds) in a few GPU launches (one per
+ * group of equal register width, chains taken from a device work queue), each
  * chain following the same metro_step control flow and pcg32 draw order.
- * No counterpart in the reference (north_star's batching pattern).
+ * No counterpart in the reference, which runs one chain per process.
  */
 #ifndef SUQA_QMS_H
 #define SUQA_QMS_H

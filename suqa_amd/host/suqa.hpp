@@ -90,7 +90,7 @@ void apply_phase_list(uint q0, uint q_size, const std::vector<double>& phases);
 
 void apply_pauli_TP_rotation(const bmReg& q_apply, const std::vector<uint>& pauli_TPtype, double phase);
 
-// Extensions (north_star gate set and the reference's TODO at suqa.cu:459):
+// Extensions (the rotation gates RX and RZ, and the reference's TODO at suqa.cu:459):
 // RX(a) = exp(-i a X/2), RZ(a) = exp(-i a Z/2), any-weight Pauli rotation.
 void apply_rx(uint q, double theta);
 void apply_rz(uint q, double theta);
