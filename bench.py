@@ -428,8 +428,8 @@ def main():
                     help="FP64 tile size (0: one sweep per gate); default: autotuned")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU sharding is not part of "
-              f"this build", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); the engine has no multi-GPU "
+              f"sharding", file=sys.stderr)
         sys.exit(2)
     from suqa_amd import capi, workloads as wl
     n = a.n

@@ -2,7 +2,7 @@
  * suqa_oracle.h -- CPU restatement of SUQA's dense FP64 state-vector engine,
  * its PCG32 random source and the QMS Metropolis step (frustrated triangle).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine:
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle the HIP engine is tested against:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product path (suqa_amd/) never links or calls it.
  *
