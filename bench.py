@@ -160,6 +160,23 @@ def po_two_register_chain(beta, r, seed, steps, emin, emax):
     return po.Qms(beta, r, seed, emin, emax, 10, 100, two_registers=True).run(steps, 1, 100)
 
 
+def qms_batch_only():
+    """The 4096-chain batches of the QMS legs alone, FP64 then FP32, one untimed
+    run each first (a profiling mode: no gate sweeps, no plain chain)."""
+    from suqa_amd import qms, workloads as wl
+    emin, emax = -1.0, 3.0
+    rg = wl.Pcg32(5)
+    nch, bsteps = 4096, 500
+    chains = [(0.1 + 4.9 * i / (nch - 1), 3 + rg.randint(0, 4), 1000 + i) for i in range(nch)]
+    out = {}
+    for name, fp32 in (("qms_batch", False), ("qms_batch_fp32", True)):
+        qms.batch_run(chains, bsteps, 1, 100, emin, emax, 10, 100, fp32=fp32)
+        b = qms.batch_run(chains, bsteps, 1, 100, emin, emax, 10, 100, fp32=fp32)
+        out[name] = round(nch * bsteps / b["seconds"], 0)
+        log(f"{name}: {out[name]:.0f} chain-steps/s")
+    return out
+
+
 def qms_legs(no_cpu):
     """Plain FP64 GPU QMS (c1), batched chains (c5), their oracle checks and CPU rates."""
     from suqa_amd import qms, workloads as wl
@@ -430,6 +447,8 @@ def main():
     ap.add_argument("--no-qms", action="store_true", help="skip the QMS legs (profiling runs)")
     ap.add_argument("--no-fused", action="store_true", help="skip the FP32 fused leg (profiling runs)")
     ap.add_argument("--no-trotter", action="store_true", help="skip the Trotter + QPE leg (profiling runs)")
+    ap.add_argument("--qms-batch-only", action="store_true",
+                    help="only the 4096-chain QMS batches, FP64 and FP32 (profiling runs)")
     ap.add_argument("--full-qpe", action="store_true",
                     help="also run the whole 24 + 6 qubit QPE (20 Trotter steps per power) in FP64 and FP32")
     ap.add_argument("--tile-bits", type=int, default=-1,
@@ -439,6 +458,9 @@ def main():
         print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); the engine has no multi-GPU "
               f"sharding", file=sys.stderr)
         sys.exit(2)
+    if a.qms_batch_only:
+        print(json.dumps({"mode": "qms_batch_only", **qms_batch_only()}), flush=True)
+        return
     from suqa_amd import capi, workloads as wl
     n = a.n
     layers = a.warmup + a.steps
