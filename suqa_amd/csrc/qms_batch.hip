@@ -121,8 +121,10 @@ struct ChainBase {
 // strides.
 template <typename T, int TW>
 struct LdsOps : ChainBase {
-    T* re;               // LDS
-    T* im;
+    struct alignas(2 * sizeof(T)) Amp {
+        T r, i;
+    };
+    Amp* st;             // LDS: amplitude i at st[i], real and imaginary parts adjacent (one access)
     double* red;         // LDS: the waves' partial sums of a measurement (TW > 1)
 
     __device__ __forceinline__ void sync() const { __syncthreads(); }
@@ -130,8 +132,8 @@ struct LdsOps : ChainBase {
     // ------------------------------------------------------------ gates
     __device__ __forceinline__ void init_zero() {
         for (uint32_t i = lane; i < N; i += 64 * TW) {
-            re[i] = i == 0 ? T(1) : T(0);
-            im[i] = T(0);
+            st[i].r = i == 0 ? T(1) : T(0);
+            st[i].i = T(0);
         }
         sync();
     }
@@ -142,11 +144,11 @@ struct LdsOps : ChainBase {
             const uint32_t a = ins0(p, q);
             if ((a & gc) != gc) continue;
             const uint32_t b = a | (1u << q);
-            const T a0r = re[a], a0i = im[a], a1r = re[b], a1i = im[b];
-            re[a] = T(kTwoSqInv) * (a0r + a1r);
-            im[a] = T(kTwoSqInv) * (a0i + a1i);
-            re[b] = T(kTwoSqInv) * (a0r - a1r);
-            im[b] = T(kTwoSqInv) * (a0i - a1i);
+            const T a0r = st[a].r, a0i = st[a].i, a1r = st[b].r, a1i = st[b].i;
+            st[a].r = T(kTwoSqInv) * (a0r + a1r);
+            st[a].i = T(kTwoSqInv) * (a0i + a1i);
+            st[b].r = T(kTwoSqInv) * (a0r - a1r);
+            st[b].i = T(kTwoSqInv) * (a0i - a1i);
         }
         sync();
     }
@@ -157,12 +159,12 @@ struct LdsOps : ChainBase {
             const uint32_t a = ins0(p, qt);
             const uint32_t b = a | (1u << qt);
             if ((b & cm) != cv) continue;
-            T t = re[a];
-            re[a] = re[b];
-            re[b] = t;
-            t = im[a];
-            im[a] = im[b];
-            im[b] = t;
+            T t = st[a].r;
+            st[a].r = st[b].r;
+            st[b].r = t;
+            t = st[a].i;
+            st[a].i = st[b].i;
+            st[b].i = t;
         }
         sync();
     }
@@ -180,9 +182,9 @@ struct LdsOps : ChainBase {
         const uint32_t cnt = q1 == q2 ? N / 2 : N / 4;
         for (uint32_t p = lane; p < cnt; p += 64 * TW) {
             const uint32_t i = (q1 == q2 ? ins0(p, lo) : ins0(ins0(p, lo), hi)) | m;
-            const T cpy = re[i];
-            re[i] = cpy * c - im[i] * s;
-            im[i] = im[i] * c + cpy * s;
+            const T cpy = st[i].r;
+            st[i].r = cpy * c - st[i].i * s;
+            st[i].i = st[i].i * c + cpy * s;
         }
         sync();
     }
@@ -196,11 +198,11 @@ struct LdsOps : ChainBase {
             const uint32_t j = ins0(p, pv);
             if ((j & gc) != gc) continue;
             const uint32_t k = j ^ X;
-            const T jr = re[j], ji = im[j], kr = re[k], ki = im[k];
-            re[j] = jr * c - ki * s;
-            im[j] = ji * c + kr * s;
-            re[k] = kr * c - ji * s;
-            im[k] = ki * c + jr * s;
+            const T jr = st[j].r, ji = st[j].i, kr = st[k].r, ki = st[k].i;
+            st[j].r = jr * c - ki * s;
+            st[j].i = ji * c + kr * s;
+            st[k].r = kr * c - ji * s;
+            st[k].i = ki * c + jr * s;
         }
         sync();
     }
@@ -209,7 +211,7 @@ struct LdsOps : ChainBase {
     __device__ __forceinline__ uint32_t measure(uint32_t q, double rd) {
         double s0 = 0.0, s1 = 0.0;
         for (uint32_t i = lane; i < N; i += 64 * TW) {
-            const double a = re[i], b = im[i];   // FP64 probabilities in both modes
+            const double a = st[i].r, b = st[i].i;   // FP64 probabilities in both modes
             const double p = a * a + b * b;
             if ((i >> q) & 1u) s1 += p;
             else s0 += p;
@@ -233,8 +235,8 @@ struct LdsOps : ChainBase {
         const double scale = 1. / sqrt(c ? s1 : s0);
         for (uint32_t i = lane; i < N; i += 64 * TW) {
             const bool keep = ((i >> q) & 1u) == c;
-            re[i] = keep ? T((double)re[i] * scale) : T(0);
-            im[i] = keep ? T((double)im[i] * scale) : T(0);
+            st[i].r = keep ? T((double)st[i].r * scale) : T(0);
+            st[i].i = keep ? T((double)st[i].i * scale) : T(0);
         }
         sync();
         return c;
@@ -254,14 +256,14 @@ struct LdsOps : ChainBase {
             const uint32_t Eo = two ? (i >> eo) & (L - 1) : Eold;
             const double* row = P->wtab + 2 * (size_t)Eo * L;
             const T fs1 = T(row[2 * En]), fs2 = T(row[2 * En + 1]);
-            T t = re[j];
-            const T ri = re[i];
-            re[j] = fs2 * t + fs1 * ri;
-            re[i] = fs1 * t - fs2 * ri;
-            t = im[j];
-            const T ii = im[i];
-            im[j] = fs2 * t + fs1 * ii;
-            im[i] = fs1 * t - fs2 * ii;
+            T t = st[j].r;
+            const T ri = st[i].r;
+            st[j].r = fs2 * t + fs1 * ri;
+            st[i].r = fs1 * t - fs2 * ri;
+            t = st[j].i;
+            const T ii = st[i].i;
+            st[j].i = fs2 * t + fs1 * ii;
+            st[i].i = fs1 * t - fs2 * ii;
         }
         sync();
     }
@@ -660,9 +662,8 @@ __global__ __launch_bounds__(64 * TW) void k_qms_batch(const BatchDev b) {
         c.eo = E0;
         c.en = c.two ? E0 + c.r : E0;
         c.lane = lane;
-        c.re = reinterpret_cast<T*>(lds);
-        c.im = c.re + c.N;
-        c.red = reinterpret_cast<double*>(c.im + c.N);
+        c.st = reinterpret_cast<typename LdsOps<T, TW>::Amp*>(lds);
+        c.red = reinterpret_cast<double*>(c.st + c.N);
         c.run(b, chain);
         __syncthreads();
     }
