@@ -147,6 +147,10 @@ int suqa_hip_state_info(suqa_hip_ctx* ctx, uint32_t* nq, uint64_t* N, double** d
 int suqa_hip_download(suqa_hip_ctx* ctx, double* re, double* im);
 int suqa_hip_upload(suqa_hip_ctx* ctx, const double* re, const double* im);
 int suqa_hip_synchronize(suqa_hip_ctx* ctx);
+/* Extension (no counterpart in the reference): a checksum of the state,
+ * out3 = {sum re, sum im, sum_i i |psi_i|^2 / N}, FP64 tree sums over the
+ * whole state (deterministic for a given nq). */
+int suqa_hip_checksum(suqa_hip_ctx* ctx, double* out3);
 
 /* ---- workload runner and timing (extensions) -------------------------- */
 /* Gate kinds of the random-circuit workload. RX(a) = exp(-i a X/2),

@@ -70,9 +70,11 @@ typedef struct {
     uint32_t fp32;            /* 0: FP64 chains (bit-exact with the reference CPU path);
                                  1: FP32 amplitudes, FP64 probabilities and collapse scale */
     uint32_t two_registers;   /* 1: E_old in its own register (the commented-out layout of
-                                 qms.cuh:72-85, 134-136, 201-202): 2r + 4 qubits per chain,
-                                 held in LDS, so 2^(2r+4) amplitudes must fit a workgroup's
-                                 LDS (the call fails otherwise) */
+                                 qms.cuh:72-85, 134-136, 201-202, with the draws of its
+                                 "comparison with 2enereg" lines 134-136, 433, 479):
+                                 2r + 4 qubits per chain, r in 2..6, stored exactly in
+                                 2^(r+4) amplitudes (E_old is definite outside its own
+                                 phase estimation) */
 } suqa_qms_batch_params;
 
 /* Per chain outputs (arrays of nchains * metro_steps for rets; measurements

@@ -35,7 +35,7 @@ class _Qms(C.Structure):
                 ("n_phase_estimation", C.c_int), ("gCi", U32), ("c_acc", U32),
                 ("bm_syst", U32 * 3), ("bm_enes", U32 * 32), ("bm_acc", U32),
                 ("W_mask", U32), ("W_mask_Enew", U32),
-                ("two_registers", U32), ("bm_enes_old", U32 * 32), ("W_mask_Eold", U32),
+                ("two_registers", U32), ("bm_enes_old", U32 * 32), ("W_mask_Eold", U32), ("compare_draws", U32),
                 ("c_factor", C.c_double),
                 ("curr_E_old", U32), ("curr_E_old_d", C.c_double), ("record_reverse", C.c_int),
                 ("E_measures", PD), ("X_measures", PD), ("n_meas", C.c_size_t), ("cap_meas", C.c_size_t),
@@ -245,9 +245,12 @@ class Qms:
     """One QMS chain on the oracle (qms.cuh + system.cu + qms.cu loop)."""
 
     def __init__(self, beta, r, seed, ene_min=0.0, ene_max=1.0, pe_steps=10, max_reverse=100,
-                 record_reverse=False, two_registers=False):
+                 record_reverse=False, two_registers=False, compare_draws=False):
+        """two_registers: E_old in its own register; compare_draws: one register
+        with the draws of the reference's "comparison with 2enereg" lines."""
+        mode = 1 if two_registers else (2 if compare_draws else 0)
         self.p = lib().so_qms_create2(beta, r, max_reverse, seed, ene_min, ene_max, pe_steps,
-                                      1 if record_reverse else 0, 1 if two_registers else 0)
+                                      1 if record_reverse else 0, mode)
         if not self.p:
             raise ValueError("bad QMS parameters")
 

@@ -9,7 +9,10 @@
 //   gates   <out>                      -- per-API golden amplitudes at n=5
 //   qms     <beta> <steps> <reset_each> <r> <seed> <ene_min> <ene_max>
 //           <pe_steps> <therm> <max_rev> <record_rev> <out>
-//   circuit <n> <layers> <seed> <out>  -- random-circuit family (see gen_circuit)
+//   circuit <n> <layers> <seed> <out>  -- random-circuit family (see mode_circuit)
+//   circuit_summary <n> <layers> <seed> <out>
+//                                      -- the same circuit: checksum and 64 sample
+//                                         amplitudes before and after the measurements
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -279,15 +282,9 @@ static void mode_qms(int argc, char** argv) {
 // with pcg(seed); targets and controls distinct; angles 2*pi*doub().
 // Then measure_qbit on min(8,n) distinct random qubits, one doub() each.
 // RX(a) = exp(-i a X/2) = pauli({q},{X},-a/2); RZ likewise with Z.
-static void mode_circuit(char** argv) {
-    uint n = (uint)atoi(argv[2]);
-    uint layers = (uint)atoi(argv[3]);
-    unsigned long long seed = strtoull(argv[4], nullptr, 10);
-    pcg rng;
-    rng.set_seed(seed);
-    suqa::setup(n);
-    for (uint q = 0; q < n; ++q) suqa::apply_h(q);
-    fprintf(g_out, "{\"n\":%u,\"layers\":%u,\"seed\":%llu,\"gates\":[", n, layers, seed);
+// Applies the random layers with pcg `rng` (after the H layer); with `log`
+// each gate is written to g_out as [kind, target, c0, c1, c2, "angle"].
+static void apply_random_layers(uint n, uint layers, pcg& rng, bool log) {
     bool first = true;
     for (uint l = 0; l < layers; ++l) {
         for (uint g = 0; g < n; ++g) {
@@ -311,13 +308,16 @@ static void mode_circuit(char** argv) {
             case 5: suqa::apply_cx(ctr[0], t); break;
             case 6: suqa::apply_mcx(ctr, t); break;
             }
-            fprintf(g_out, "%s[%d,%u,%u,%u,%u,\"%a\"]", first ? "" : ",", kind, t,
-                    nc > 0 ? ctr[0] : 0U, nc > 1 ? ctr[1] : 0U, nc > 2 ? ctr[2] : 0U, ang);
+            if (log)
+                fprintf(g_out, "%s[%d,%u,%u,%u,%u,\"%a\"]", first ? "" : ",", kind, t,
+                        nc > 0 ? ctr[0] : 0U, nc > 1 ? ctr[1] : 0U, nc > 2 ? ctr[2] : 0U, ang);
             first = false;
         }
     }
-    fprintf(g_out, "],");
-    dump_state("pre_measure");
+}
+
+// The measurement plan: min(8,n) distinct random qubits, one doub() each.
+static void measure_random(uint n, pcg& rng) {
     uint nm = n < 8 ? n : 8;
     std::vector<uint> mq;
     while (mq.size() < nm) {
@@ -326,22 +326,95 @@ static void mode_circuit(char** argv) {
         for (uint x : mq) dup |= (x == q);
         if (!dup) mq.push_back(q);
     }
-    fprintf(g_out, ",\"measure\":[");
+    fprintf(g_out, "\"measure\":[");
     for (uint k = 0; k < nm; ++k) {
         double rd = rng.doub();
         uint c;
         suqa::measure_qbit(mq[k], c, rd);
         fprintf(g_out, "%s[%u,\"%a\",%u]", k ? "," : "", mq[k], rd, c);
     }
+    fprintf(g_out, "]");
+}
+
+// Random-circuit family of the gate-throughput workload: |0>, H on every
+// qubit, `layers` x n gates drawn uniformly from {H,X,RX,RZ,U1,CX,MCX(3)}
+// with pcg(seed); targets and controls distinct; angles 2*pi*doub().
+// Then measure_qbit on min(8,n) distinct random qubits, one doub() each.
+// RX(a) = exp(-i a X/2) = pauli({q},{X},-a/2); RZ likewise with Z.
+static void mode_circuit(char** argv) {
+    uint n = (uint)atoi(argv[2]);
+    uint layers = (uint)atoi(argv[3]);
+    unsigned long long seed = strtoull(argv[4], nullptr, 10);
+    pcg rng;
+    rng.set_seed(seed);
+    suqa::setup(n);
+    for (uint q = 0; q < n; ++q) suqa::apply_h(q);
+    fprintf(g_out, "{\"n\":%u,\"layers\":%u,\"seed\":%llu,\"gates\":[", n, layers, seed);
+    apply_random_layers(n, layers, rng, true);
     fprintf(g_out, "],");
+    dump_state("pre_measure");
+    fprintf(g_out, ",");
+    measure_random(n, rng);
+    fprintf(g_out, ",");
     dump_state("final");
+    fprintf(g_out, "}\n");
+    suqa::clear();
+}
+
+// Pairwise (recursive halving) sum of f(i) over [lo, hi).
+template <class F>
+static double pairwise(uint64_t lo, uint64_t hi, const F& f) {
+    if (hi - lo <= 8) {
+        double s = 0.0;
+        for (uint64_t i = lo; i < hi; ++i) s += f(i);
+        return s;
+    }
+    uint64_t mid = lo + (hi - lo) / 2;
+    return pairwise(lo, mid, f) + pairwise(mid, hi, f);
+}
+
+// Checksum of the state (sum re, sum im, sum i |psi_i|^2 / N; pairwise sums)
+// and the amplitudes at 64 fixed indices i_k = (k * 2654435761 + 12345) mod N.
+static void dump_summary(const char* key) {
+    const uint64_t N = suqa::state.size();
+    const double* re = suqa::state.data_re;
+    const double* im = suqa::state.data_im;
+    double sre = pairwise(0, N, [&](uint64_t i) { return re[i]; });
+    double sim = pairwise(0, N, [&](uint64_t i) { return im[i]; });
+    double sidx = pairwise(0, N, [&](uint64_t i) { return (double)i * (re[i] * re[i] + im[i] * im[i]); }) / (double)N;
+    fprintf(g_out, "\"%s\":{\"sum_re\":\"%a\",\"sum_im\":\"%a\",\"sum_index_prob\":\"%a\",\"samples\":[", key, sre,
+            sim, sidx);
+    for (uint64_t k = 0; k < 64; ++k) {
+        uint64_t i = (k * 2654435761ULL + 12345ULL) % N;
+        fprintf(g_out, "%s[%llu,\"%a\",\"%a\"]", k ? "," : "", (unsigned long long)i, re[i], im[i]);
+    }
+    fprintf(g_out, "]}");
+}
+
+// The same circuit at sizes too large to store whole: the checksum and 64
+// sample amplitudes of the state before the measurements and after them.
+static void mode_circuit_summary(char** argv) {
+    uint n = (uint)atoi(argv[2]);
+    uint layers = (uint)atoi(argv[3]);
+    unsigned long long seed = strtoull(argv[4], nullptr, 10);
+    pcg rng;
+    rng.set_seed(seed);
+    suqa::setup(n);
+    for (uint q = 0; q < n; ++q) suqa::apply_h(q);
+    fprintf(g_out, "{\"n\":%u,\"layers\":%u,\"seed\":%llu,", n, layers, seed);
+    apply_random_layers(n, layers, rng, false);
+    dump_summary("pre_measure");
+    fprintf(g_out, ",");
+    measure_random(n, rng);
+    fprintf(g_out, ",");
+    dump_summary("final");
     fprintf(g_out, "}\n");
     suqa::clear();
 }
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s pcg|gates|qms|circuit ... <out.json>\n", argv[0]);
+        fprintf(stderr, "usage: %s pcg|gates|qms|circuit|circuit_summary ... <out.json>\n", argv[0]);
         return 2;
     }
     std::string mode = argv[1];
@@ -351,6 +424,7 @@ int main(int argc, char** argv) {
     else if (mode == "gates") mode_gates();
     else if (mode == "qms" && argc == 14) mode_qms(argc, argv);
     else if (mode == "circuit" && argc == 6) mode_circuit(argv);
+    else if (mode == "circuit_summary" && argc == 6) mode_circuit_summary(argv);
     else { fprintf(stderr, "bad arguments\n"); return 2; }
     fclose(g_out);
     return 0;

@@ -518,7 +518,9 @@ so_qms* so_qms_create(double beta, uint32_t ene_qbits, uint32_t max_reverse, uin
 
 so_qms* so_qms_create2(double beta, uint32_t ene_qbits, uint32_t max_reverse, uint64_t seed,
                        double ene_min, double ene_max, int pe_steps, int record_reverse, int two_registers) {
-    if (seed == 0 || ene_qbits == 0 || ene_qbits > 28 || (two_registers && ene_qbits > 13)) return NULL;
+    if (seed == 0 || ene_qbits == 0 || ene_qbits > 28 || two_registers < 0 || two_registers > SO_QMS_COMPARE_DRAWS ||
+        (two_registers == 1 && ene_qbits > 13))
+        return NULL;
     so_qms* q = (so_qms*)calloc(1, sizeof(so_qms));
     if (!q) return NULL;
     /* qms.cu:56-84 */
@@ -529,7 +531,8 @@ so_qms* so_qms_create2(double beta, uint32_t ene_qbits, uint32_t max_reverse, ui
     so_pcg32_seed(&q->rng, seed, 1u);
     q->syst_qbits = 3;
     q->ene_qbits = ene_qbits;
-    q->two_registers = two_registers ? 1u : 0u;
+    q->two_registers = two_registers == 1 ? 1u : 0u;
+    q->compare_draws = two_registers == SO_QMS_COMPARE_DRAWS ? 1u : 0u;
     q->nqubits = q->syst_qbits + (1u + q->two_registers) * ene_qbits + 1;
     q->ene_levels = 1u << ene_qbits;
     q->t_PE_shift = ene_min;
@@ -713,6 +716,8 @@ static void so_qms_reset_non_syst(so_qms* q) {   /* qms.cuh:128-150 */
     if (q->two_registers) {   /* the E_old register first (qms.cuh:134-136) */
         for (uint32_t k = 0; k < q->ene_qbits; ++k) rd[k] = so_qms_doub(q);
         for (uint32_t k = 0; k < q->ene_qbits; ++k) so_apply_reset(q->st, q->bm_enes_old[k], rd[k]);
+    } else if (q->compare_draws) {   /* the same draws, nothing to reset */
+        for (uint32_t k = 0; k < q->ene_qbits; ++k) (void)so_qms_doub(q);
     }
     for (uint32_t k = 0; k < q->ene_qbits; ++k) rd[k] = so_qms_doub(q);
     for (uint32_t k = 0; k < q->ene_qbits; ++k) so_apply_reset(q->st, q->bm_enes[k], rd[k]);
@@ -746,8 +751,12 @@ int so_qms_metro_step(so_qms* q, int take_measure) {   /* qms.cuh:373-545 */
         if (take_measure) {
             double e = q->t_PE_shift + so_creg_to_uint(c_E_news, r) / (double)(q->t_PE_factor * q->ene_levels);
             so_qms_push_E(q, e);
-            for (uint32_t ei = 0; ei < r; ++ei)
+            for (uint32_t ei = 0; ei < r; ++ei) {
+                /* qms.cuh:431-434: the comparison draw before each flip */
+                if (q->two_registers) so_apply_reset(q->st, q->bm_enes_old[ei], so_qms_doub(q));
+                else if (q->compare_draws) so_apply_reset(q->st, q->bm_enes[ei], so_qms_doub(q));
                 if (c_E_news[ei]) so_apply_x(q->st, q->bm_enes[ei]);
+            }
             so_qms_push_X(q, so_qms_measure_X(q));
             for (uint32_t ei = 0; ei < r; ++ei) so_apply_reset(q->st, q->bm_enes[ei], so_qms_doub(q));
             so_qms_apply_Phi(q);
@@ -763,6 +772,10 @@ int so_qms_metro_step(so_qms* q, int take_measure) {   /* qms.cuh:373-545 */
     uint32_t iters = 0;
     while (iters < q->max_reverse_attempts) {
         so_qms_apply_Phi(q);
+        /* qms.cuh:478-479: the E_old readout's r draws (two registers: the
+         * register's value is the one measured at the start of the step) */
+        if (q->two_registers || q->compare_draws)
+            for (uint32_t k = 0; k < r; ++k) (void)so_qms_doub(q);
         so_qms_measure_enes(q, c_E_news);
         uint32_t Enew_meas = so_creg_to_uint(c_E_news, r);
         so_qms_apply_Phi_inverse(q);

@@ -116,6 +116,10 @@ typedef struct {
     uint32_t  two_registers;
     uint32_t  bm_enes_old[32];
     uint32_t  W_mask_Eold;
+    /* one register with the extra draws of the reference's "comparison with
+     * 2enereg" debug lines (qms.cuh:134-136, 433, 479): the draws the
+     * two-register chain makes, at the same points of metro_step */
+    uint32_t  compare_draws;
     double    c_factor;
     uint32_t  curr_E_old;
     double    curr_E_old_d;
@@ -132,10 +136,23 @@ typedef struct {
 so_qms* so_qms_create(double beta, uint32_t ene_qbits, uint32_t max_reverse,
                       uint64_t seed, double ene_min, double ene_max, int pe_steps,
                       int record_reverse);
-/* The same with two energy registers when two_registers != 0: bitmap syst
- * 0..2, E_old 3..2+r, E_new 3+r..2+2r, acceptance 3+2r; the reset draws r
- * doubles for E_old first; E_old is phase-estimated into its register and
- * stays there; W reads E_old from the register. */
+/* two_registers == 1: two energy registers.  Bitmap syst 0..2, E_old
+ * 3..2+r, E_new 3+r..2+2r, acceptance 3+2r.  Draws at the points of the
+ * reference's "comparison with 2enereg" lines:
+ *   - the reset draws r doubles for E_old first and resets it
+ *     (qms.cuh:134-136);
+ *   - E_old is phase-estimated into its register and stays there; W reads
+ *     E_old from the register;
+ *   - an accepted, measured step resets E_old qubit ei with one draw before
+ *     the flip of E_new qubit ei (qms.cuh:431-434);
+ *   - each restoration attempt reads E_old out after its Phi: r draws
+ *     (qms.cuh:478-479); the register holds the value measured at the start
+ *     of the step (only diagonal reads have acted on it since), so the
+ *     readout is that value.
+ * two_registers == SO_QMS_COMPARE_DRAWS: one register with those draws (the
+ * debug lines as written: nothing reset at 134-136, E_new qubit ei reset at
+ * 433, r draws discarded at 479). */
+#define SO_QMS_COMPARE_DRAWS 2
 so_qms* so_qms_create2(double beta, uint32_t ene_qbits, uint32_t max_reverse,
                        uint64_t seed, double ene_min, double ene_max, int pe_steps,
                        int record_reverse, int two_registers);

@@ -72,6 +72,7 @@ SIGNATURES = {
     "suqa_hip_download": (C.c_int, [CTX, PD, PD]),
     "suqa_hip_upload": (C.c_int, [CTX, PD, PD]),
     "suqa_hip_synchronize": (C.c_int, [CTX]),
+    "suqa_hip_checksum": (C.c_int, [CTX, PD]),
     "suqa_hip_run_gates": (C.c_int, [CTX, C.POINTER(Gate), U32]),
     "suqa_hip_timing_enable": (C.c_int, [CTX, C.c_int]),
     "suqa_hip_timing_reset": (C.c_int, [CTX]),
@@ -302,6 +303,12 @@ class Engine:
 
     def synchronize(self):
         check(lib().suqa_hip_synchronize(self.ctx))
+
+    def checksum(self):
+        """(sum re, sum im, sum_i i |psi_i|^2 / N), FP64 tree sums on the device."""
+        out = (C.c_double * 3)()
+        check(lib().suqa_hip_checksum(self.ctx, out))
+        return tuple(out)
 
     def run_gates(self, gates, n=None, start=0):
         """Applies gates[start:start+n] (a Gate array) asynchronously."""

@@ -282,6 +282,18 @@ int suqa_hip_vnorm(suqa_hip_ctx* c, double* norm) {
     return SUQA_OK;
 }
 
+int suqa_hip_checksum(suqa_hip_ctx* c, double* out3) {
+    SQE_CTX(c);
+    if (!out3) return fail(SUQA_ERR_ARG, "null argument");
+    SQE_HIP(sqe::launch_checksum(c));
+    int rc = read_result(c, 3);
+    if (rc) return rc;
+    out3[0] = c->h_result[0];
+    out3[1] = c->h_result[1];
+    out3[2] = c->h_result[2] / (double)c->N;
+    return SUQA_OK;
+}
+
 int suqa_hip_vnormalize(suqa_hip_ctx* c) {
     double v = 0.0;
     int rc = suqa_hip_vnorm(c, &v);

@@ -129,6 +129,8 @@ hipError_t launch_phase_list(suqa_hip_ctx* c, const Sel& sel, uint32_t q0, uint3
 hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
                        double alg_bytes);
 hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale);
+// d_result[0..2] = sum re, sum im, sum i |psi_i|^2 over the whole state.
+hipError_t launch_checksum(suqa_hip_ctx* c);
 // Measurements of a state of at most 2^kMeasSmallBits amplitudes in one
 // workgroup: for k < n in order, p0 and p1 of qubit qs[k], the outcome
 // c = (rds[k] < p1) into ctx->d_flags[k], the collapse scaled by 1/sqrt(p_c),

@@ -291,6 +291,42 @@ __global__ __launch_bounds__(kBlock) void k_norm(const double* __restrict__ re, 
     }
 }
 
+// Checksum sums over the whole state, one amplitude per thread, into a
+// (sum, sum) pair per block: mode 0 -> (sum re, sum im), mode 1 ->
+// (sum i |psi_i|^2, 0).
+__global__ __launch_bounds__(kBlock) void k_checksum(const double* __restrict__ re, const double* __restrict__ im,
+                                                     uint64_t N, int mode, double* __restrict__ partials) {
+    __shared__ double lds[2][kBlock / 64];
+    double acc0 = 0.0, acc1 = 0.0;
+    const uint64_t i = item_id();
+    if (i < N) {
+        const double r = re[i], m = im[i];
+        if (mode == 0) {
+            acc0 = r;
+            acc1 = m;
+        } else {
+            acc0 = (double)i * (r * r + m * m);
+        }
+    }
+    acc0 = wave_sum(acc0);
+    acc1 = wave_sum(acc1);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][w] = acc0;
+        lds[1][w] = acc1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            s0 += lds[0][k];
+            s1 += lds[1][k];
+        }
+        partials[2 * blockIdx.x] = s0;
+        partials[2 * blockIdx.x + 1] = s1;
+    }
+}
+
 // Fixed-order sum of pairs of partials: block k reduces pairs
 // [k * chunk, (k+1) * chunk) into out[2k], out[2k+1].
 __global__ __launch_bounds__(kBlock) void k_sum_pairs(const double* __restrict__ in, uint64_t npairs,
@@ -662,6 +698,30 @@ hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, cons
     hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(kBlock), 0, c->stream, in, np, np, c->d_result);
     timing_end(c);
     return hipGetLastError();
+}
+
+hipError_t launch_checksum(suqa_hip_ctx* c) {
+    if (hipError_t e = flush(c)) return e;
+    const uint32_t g = grid_for(c->N);
+    for (int mode = 0; mode < 2; ++mode) {
+        timing_begin(c, "checksum", 16.0 * (double)c->N);
+        hipLaunchKernelGGL(k_checksum, dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, c->N, mode,
+                           c->d_partials);
+        const double* in = c->d_partials;
+        uint64_t np = g;
+        if (np > kPartChunk) {
+            const uint64_t g2 = (np + kPartChunk - 1) / kPartChunk;
+            double* out = c->d_partials + 2 * (uint64_t)grid_for(c->N);
+            hipLaunchKernelGGL(k_sum_pairs, dim3((uint32_t)g2), dim3(kBlock), 0, c->stream, in, np,
+                               (uint64_t)kPartChunk, out);
+            in = out;
+            np = g2;
+        }
+        hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(kBlock), 0, c->stream, in, np, np, c->d_result + 2 * mode);
+        timing_end(c);
+        if (hipError_t e = hipGetLastError()) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale) {

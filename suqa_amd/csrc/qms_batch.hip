@@ -1,8 +1,21 @@
 // qms_batch.hip -- thousands of independent QMS chains in a few launches.
 //
-// Two layouts of a chain's state (2^n amplitudes, n = r + 4, or 2r + 4 with
-// E_old in its own register), with one control flow (Chain<Ops>):
-// * RegOps, batches of FP64 one-register chains no larger than the CU count
+// A chain's state is 2^(r+4) amplitudes: 3 system qubits, the r-qubit
+// register slot at qubit 3 and the acceptance qubit on top.  With E_old in
+// its own register (two registers, 2r + 4 qubits in the reference layout) the
+// state is stored in the same 2^(r+4) amplitudes, exactly: the E_old register
+// reads one definite value on every nonzero amplitude except during its own
+// phase estimation, and there E_new and the acceptance qubit read 0.  So the
+// slot holds E_old while it is phase-estimated and measured, and E_new after;
+// the E_old value is kept in a scalar (vold), every gate that reads it (W)
+// takes the scalar, and a measurement of one of its qubits (a reset) is a
+// norm and a scale of the stored amplitudes (measure_fixed).  Every other
+// amplitude of the full state is exactly zero and stays zero under every
+// gate the chain applies, so the stored ones are computed exactly as the
+// full-state chain computes them.
+//
+// Two layouts of those amplitudes, with one control flow (Chain<Ops>):
+// * RegOps, batches of FP64 chains no larger than the CU count
 //   (latency first): a workgroup of N threads per chain, one amplitude per
 //   thread.  A gate exchanges each amplitude with its partner lane (a shuffle
 //   inside the wavefront, LDS and a barrier across wavefronts); a measurement
@@ -109,8 +122,9 @@ struct Rng {
 // What every chain layout holds.
 struct ChainBase {
     uint32_t r, n, N, L, acc;
-    uint32_t en, eo;     // first qubit of the E_new register, of the E_old register (two registers)
+    uint32_t en;         // first qubit of the register slot (E0)
     bool two;            // E_old in its own register (qms.cuh:72-85, 134-136, 201-202)
+    uint32_t vold;       // two registers: the value the E_old register holds (definite, see Chain)
     uint32_t lane;       // 0 on the one thread that writes a chain's outputs
     Rng rng;
     const ChainDev* P;
@@ -246,15 +260,41 @@ struct LdsOps : ChainBase {
         if (measure(q, rd)) x(q);
     }
 
-    // qms::apply_W, dense form (qms.cuh:286-307); acc is the top qubit.  With
-    // two registers E_old is read per amplitude from its register.
+    // measure_qbit of a qubit that is not stored and reads bitv on every
+    // stored amplitude: P(1) is the whole norm or 0, the stored amplitudes are
+    // the kept half and are scaled as measure() scales it (an outcome of
+    // probability 0 leaves 0 * 1/sqrt(0) = NaN, as it would in the full state)
+    __device__ __forceinline__ uint32_t measure_fixed(uint32_t bitv, double rd) {
+        double s = 0.0;
+        for (uint32_t i = lane; i < N; i += 64 * TW) {
+            const double a = st[i].r, b = st[i].i;
+            s += a * a + b * b;
+        }
+        s = wave_sum(s);
+        if constexpr (TW > 1) {   // the last use of red ended at a barrier
+            if ((lane & 63u) == 0u) red[2 * (lane >> 6)] = s;
+            __syncthreads();
+            s = 0.0;
+            for (int w = 0; w < TW; ++w) s += red[2 * w];
+        }
+        const uint32_t c = __builtin_amdgcn_readfirstlane((uint32_t)(rd < (bitv ? s : 0.0)));
+        const double f = c == bitv ? 1. / sqrt(s) : __builtin_nan("");
+        for (uint32_t i = lane; i < N; i += 64 * TW) {
+            st[i].r = T((double)st[i].r * f);
+            st[i].i = T((double)st[i].i * f);
+        }
+        sync();
+        return c;
+    }
+
+    // qms::apply_W, dense form (qms.cuh:286-307); acc is the top qubit, E_old
+    // the classical value (two registers: the value the E_old register holds).
     __device__ __forceinline__ void W(uint32_t Eold) {
         const uint32_t half = N / 2;
         for (uint32_t j = lane; j < half; j += 64 * TW) {
             const uint32_t i = j + half;
             const uint32_t En = (i >> en) & (L - 1);
-            const uint32_t Eo = two ? (i >> eo) & (L - 1) : Eold;
-            const double* row = P->wtab + 2 * (size_t)Eo * L;
+            const double* row = P->wtab + 2 * (size_t)Eold * L;
             const T fs1 = T(row[2 * En]), fs2 = T(row[2 * En + 1]);
             T t = st[j].r;
             const T ri = st[i].r;
@@ -390,14 +430,32 @@ struct RegOps : ChainBase {
         if (measure(q, rd)) x(q);
     }
 
+    // measure_qbit of a qubit that is not stored and reads bitv everywhere
+    // (LdsOps::measure_fixed)
+    __device__ __forceinline__ uint32_t measure_fixed(uint32_t bitv, double rd) {
+        const double x = vr, y = vi;
+        const double s = wave_sum(x * x + y * y);
+        const uint32_t w = idx >> 6, nw = N >> 6;
+        double* pr = red + 32 * rb;
+        if ((idx & 63u) == 0u) pr[2 * w] = s;
+        __syncthreads();
+        double t = 0.0;
+        for (uint32_t k = 0; k < nw; ++k) t += pr[2 * k];
+        rb ^= 1u;
+        const uint32_t c = __builtin_amdgcn_readfirstlane(rd < (bitv ? t : 0.0) ? 1u : 0u);
+        const double f = c == bitv ? 1. / sqrt(t) : __builtin_nan("");
+        vr = T((double)vr * f);
+        vi = T((double)vi * f);
+        return c;
+    }
+
     __device__ __forceinline__ void W(uint32_t Eold) {   // qms.cuh:286-307: pairs (j, j + N/2)
         const uint32_t half = N / 2;
         T pr, pi;
         partner(half, pr, pi);
         const uint32_t i = idx | half;
         const uint32_t En = (i >> en) & (L - 1);
-        const uint32_t Eo = two ? (i >> eo) & (L - 1) : Eold;
-        const double* row = P->wtab + 2 * (size_t)Eo * L;
+        const double* row = P->wtab + 2 * (size_t)Eold * L;
         const T fs1 = T(row[2 * En]), fs2 = T(row[2 * En + 1]);
         if (idx < half) {   // j: fs2 t + fs1 ri
             vr = fs2 * vr + fs1 * pr;
@@ -418,8 +476,8 @@ struct Chain : Ops {
     using Ops::L;
     using Ops::acc;
     using Ops::en;
-    using Ops::eo;
     using Ops::two;
+    using Ops::vold;
     using Ops::lane;
     using Ops::rng;
     using Ops::P;
@@ -431,11 +489,13 @@ struct Chain : Ops {
     using Ops::xx;
     using Ops::measure;
     using Ops::reset;
+    using Ops::measure_fixed;
     using Ops::W;
 
     // ------------------------------------------------------- QMS pieces
     __device__ __forceinline__ void system_init() {   // system.cu:6-11
         init_zero();
+        vold = 0u;
         x(1);
         h(1, 0u);
         cx(1, 0);
@@ -449,7 +509,8 @@ struct Chain : Ops {
         xx(0, 1, c, s, gc);
     }
 
-    __device__ __forceinline__ void Phi(uint32_t b) {   // qms.cuh:153-176 on the register at qubit b
+    __device__ __forceinline__ void Phi() {   // qms.cuh:153-176 on the register slot
+        const uint32_t b = en;
         for (uint32_t k = 0; k < r; ++k) h(b + k, 0u);
         for (int trg = (int)r - 1; trg > -1; --trg) {
             cevolution(b, (uint32_t)trg, 0);
@@ -460,7 +521,6 @@ struct Chain : Ops {
             h(b + o, 0u);
         }
     }
-    __device__ __forceinline__ void Phi() { Phi(en); }
 
     __device__ __forceinline__ void Phi_inverse() {   // qms.cuh:179-198
         for (int o = (int)r - 1; o >= 0; o--) {   // QFT (suqa.cu:635-647)
@@ -488,20 +548,32 @@ struct Chain : Ops {
     // measure_qbits(bm_enes, cs, extract_rands(r)): the r draws first, then the
     // measurements in order; the draws are replayed from a copy of the
     // generator, so the loop stays rolled and holds no array
-    __device__ __forceinline__ uint32_t measure_enes(uint32_t b) {
+    __device__ __forceinline__ uint32_t measure_enes() {
         Rng d = rng;
         for (uint32_t k = 0; k < r; ++k) (void)rng.doub();
         uint32_t v = 0;
-        for (uint32_t k = 0; k < r; ++k) v |= measure(b + k, d.doub()) << k;
+        for (uint32_t k = 0; k < r; ++k) v |= measure(en + k, d.doub()) << k;
         return v;
     }
 
-    __device__ __forceinline__ uint32_t measure_enes() { return measure_enes(en); }
-    // r draws, then the resets of the register at qubit b in order
-    __device__ __forceinline__ void reset_register(uint32_t b) {
+    // r draws, then the resets of the register slot in order
+    __device__ __forceinline__ void reset_register() {
         Rng d = rng;
         for (uint32_t k = 0; k < r; ++k) (void)rng.doub();
-        for (uint32_t k = 0; k < r; ++k) reset(b + k, d.doub());
+        for (uint32_t k = 0; k < r; ++k) reset(en + k, d.doub());
+    }
+
+    // apply_reset of E_old qubit k (two registers): the qubit reads bit k of
+    // vold on every nonzero amplitude, so it is measured by measure_fixed and
+    // reads 0 after the reset's X
+    __device__ __forceinline__ void reset_old_qubit(uint32_t k, double rd) {
+        (void)measure_fixed((vold >> k) & 1u, rd);
+        vold &= ~(1u << k);
+    }
+    __device__ __forceinline__ void reset_old_register() {   // qms.cuh:134-136: r draws, then the resets
+        Rng d = rng;
+        for (uint32_t k = 0; k < r; ++k) (void)rng.doub();
+        for (uint32_t k = 0; k < r; ++k) reset_old_qubit(k, d.doub());
     }
 
     __device__ __forceinline__ void reset_enes_each_draw() {
@@ -533,19 +605,19 @@ struct Chain : Ops {
     __device__ __forceinline__ int metro_step(bool take, uint32_t max_rev, double* Ed, double* Xv) {
         int ret = 0;
         // reset_non_syst_qbits (qms.cuh:128-150): the E_old register first
-        if (two) reset_register(eo);
-        reset_register(en);
+        if (two) reset_old_register();
+        reset_register();
         reset(acc, rng.doub());
-        uint32_t Eold;
-        if (two) {   // E_old into its own register, which keeps the measured value
-            Phi(eo);
-            Eold = measure_enes(eo);
-        } else {
-            Phi();
-            Eold = measure_enes();
-            for (uint32_t k = 0; k < r; ++k)
-                if ((Eold >> k) & 1u) x(en + k);
-        }
+        // Phi and the E_old measurement on the register slot.  One register:
+        // the measured value is flipped back to 0.  Two registers: E_new and
+        // the acceptance qubit read 0 here, so the slot holds E_old during its
+        // phase estimation; once measured, E_old is vold and the slot, flipped
+        // back to 0, is E_new (the same amplitudes, moved by the same X gates)
+        Phi();
+        const uint32_t Eold = measure_enes();
+        for (uint32_t k = 0; k < r; ++k)
+            if ((Eold >> k) & 1u) x(en + k);
+        if (two) vold = Eold;
         uint32_t ci;
         {   // draw_C (qms.cuh:207-216, system.cu:123)
             const double e = rng.doub();
@@ -557,8 +629,10 @@ struct Chain : Ops {
             Phi_inverse();
             if (!take) return 1;
             *Ed = P->etab[Enew];
-            for (uint32_t k = 0; k < r; ++k)
+            for (uint32_t k = 0; k < r; ++k) {   // two registers: E_old qubit k reset first (qms.cuh:431-434)
+                if (two) reset_old_qubit(k, rng.doub());
                 if ((Enew >> k) & 1u) x(en + k);
+            }
             *Xv = measure_X();
             reset_enes_each_draw();
             Phi();
@@ -570,6 +644,8 @@ struct Chain : Ops {
         uint32_t iters = 0;
         while (iters < max_rev) {
             Phi();
+            if (two)   // the E_old readout's r draws (qms.cuh:478-479); the register holds vold
+                for (uint32_t k = 0; k < r; ++k) (void)rng.doub();
             const uint32_t Enew = measure_enes();
             Phi_inverse();
             if (Eold == Enew) {
@@ -655,12 +731,11 @@ __global__ __launch_bounds__(64 * TW) void k_qms_batch(const BatchDev b) {
         c.P = &b.chains[chain];
         c.r = c.P->r;
         c.two = b.two_registers != 0;
-        c.n = c.two ? 2 * c.r + 4 : c.r + 4;
+        c.n = c.r + 4;
         c.N = 1u << c.n;
         c.L = 1u << c.r;
         c.acc = c.n - 1;
-        c.eo = E0;
-        c.en = c.two ? E0 + c.r : E0;
+        c.en = E0;
         c.lane = lane;
         c.st = reinterpret_cast<typename LdsOps<T, TW>::Amp*>(lds);
         c.red = reinterpret_cast<double*>(c.st + c.N);
@@ -680,12 +755,11 @@ __global__ __launch_bounds__(1024) void k_qms_chain_reg(const BatchDev b) {
     c.P = &b.chains[chain];
     c.r = c.P->r;
     c.two = b.two_registers != 0;
-    c.n = c.two ? 2 * c.r + 4 : c.r + 4;
+    c.n = c.r + 4;
     c.N = 1u << c.n;
     c.L = 1u << c.r;
     c.acc = c.n - 1;
-    c.eo = E0;
-    c.en = c.two ? E0 + c.r : E0;
+    c.en = E0;
     c.idx = threadIdx.x;
     c.lane = threadIdx.x;
     c.red = lds;   // 2 x 32 doubles, then the exchange buffers
@@ -822,7 +896,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     // limits the resident waves, one launch per group, so narrow chains are not
     // held to the widest chain's occupancy
     for (uint32_t c = 0; c < nchains; ++c) {
-        const uint32_t n = p->two_registers ? 2 * chains[c].ene_qbits + 4 : chains[c].ene_qbits + 4;
+        const uint32_t n = chains[c].ene_qbits + 4;   // both layouts (two registers: see Chain)
         need[c] = (p->fp32 ? sizeof(float) : sizeof(double)) * 2 * ((size_t)1 << n);
         order[c] = c;
     }
@@ -867,7 +941,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     // (RegOps); larger batches, wider, two-register and FP32 chains run in LDS
     // with 1 to 8 wavefronts per chain (LdsOps: 8.3 against 6.3 M chain-steps/s
     // for 4096 FP64 chains with r in 3..6).
-    reg = !p->fp32 && !p->two_registers && nchains <= (uint32_t)ncu;
+    reg = !p->fp32 && nchains <= (uint32_t)ncu;
     if (!reg || need[order[0]] > 2 * elt * 1024) {
         // One launch sized for the widest chain when every wave of the grid fits
         // in LDS at that size; otherwise one launch per group of equal LDS need.

@@ -188,8 +188,12 @@ int metro_step(bool take_measure) {
             if (counting) gctr_measure.new_record();
             Enew_meas_d = t_PE_shift + creg_to_uint(c_E_news) / (double)(t_PE_factor * ene_levels);
             E_measures.push_back(Enew_meas_d);
-            for (uint ei = 0U; ei < ene_qbits; ++ei)
+            for (uint ei = 0U; ei < ene_qbits; ++ei) {
+                // two registers: E_old qubit ei reset with one draw before each
+                // flip (the "comparison with 2enereg" draw of qms.cuh:431-434)
+                if (two_registers) suqa::apply_reset(bm_enes_old[ei], rangen.doub());
                 if (c_E_news[ei]) suqa::apply_x(bm_enes_new[ei]);
+            }
             X_measures.push_back(measure_X(rangen));
             suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));   // the same draws, in order
             apply_Phi();
@@ -211,6 +215,9 @@ int metro_step(bool take_measure) {
     if (counting) gctr_reverse.new_record();
     while (iters < max_reverse_attempts) {
         apply_Phi();
+        // two registers: the E_old readout takes r draws (qms.cuh:478-479); the
+        // register still holds the value measured at the start of the step
+        if (two_registers) (void)extract_rands(ene_qbits);
         suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
         Enew_meas = creg_to_uint(c_E_news);
         apply_Phi_inverse();

@@ -104,7 +104,8 @@ def batch_run(chains, steps, reset_each=1, max_reverse=100, ene_min=0.0, ene_max
               thermalization=100, cap_meas=None, rng_after=False, fp32=False, two_registers=False):
     """chains: list of (beta, r, seed).  Returns per-chain arrays.  fp32: FP32
     amplitudes with FP64 probabilities (the precision pattern).  two_registers:
-    E_old in its own register (2r + 4 qubits per chain, held in LDS)."""
+    E_old in its own register (2r + 4 qubits per chain, r in 2..6, stored in
+    2^(r+4) amplitudes: E_old is definite outside its own phase estimation)."""
     n = len(chains)
     specs = (ChainSpec * n)(*[ChainSpec(b, r, s) for (b, r, s) in chains])
     bp = BatchParams(steps, reset_each, max_reverse, ene_min, ene_max, pe_steps, thermalization, 1 if fp32 else 0,

@@ -116,3 +116,43 @@ def qms_params(d):
 
 
 QMS_FIXTURES = ["qms_c1.json", "qms_r3_b05.json", "qms_r3_maxrev1.json", "qms_r5_b3.json", "qms_r6_b01.json"]
+
+
+def pairwise_sum(x):
+    """The harness's pairwise sum (oracle/ref_harness.cpp: recursive halving,
+    ranges of at most 8 summed left to right) of a power-of-two-sized array,
+    in the same order, so bit-identical for identical inputs."""
+    x = np.asarray(x, dtype=np.float64)
+    n = x.size
+    if n <= 8:
+        s = 0.0
+        for v in x:
+            s += float(v)
+        return s
+    a = x.reshape(-1, 8)
+    s = np.zeros(a.shape[0])
+    for k in range(8):
+        s = s + a[:, k]
+    while s.size > 1:
+        s = s[0::2] + s[1::2]
+    return float(s[0])
+
+
+def SAMPLE_INDEX(k, N):
+    return (k * 2654435761 + 12345) % N
+
+
+def state_summary(re, im):
+    """(sum re, sum im, sum i |psi_i|^2 / N) by pairwise sums and the 64
+    sample amplitudes, as oracle/ref_harness.cpp's circuit_summary writes them."""
+    N = re.size
+    idx = np.arange(N, dtype=np.float64)
+    sums = (pairwise_sum(re), pairwise_sum(im), pairwise_sum(idx * (re * re + im * im)) / N)
+    samples = [(SAMPLE_INDEX(k, N), re[SAMPLE_INDEX(k, N)], im[SAMPLE_INDEX(k, N)]) for k in range(64)]
+    return sums, samples
+
+
+def golden_summary(d):
+    sums = tuple(float.fromhex(d[k]) for k in ("sum_re", "sum_im", "sum_index_prob"))
+    samples = [(int(i), float.fromhex(r), float.fromhex(m)) for (i, r, m) in d["samples"]]
+    return sums, samples

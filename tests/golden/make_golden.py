@@ -20,6 +20,10 @@ Fixtures (all JSON; doubles are C99 hex floats, exact):
                         built with -DGATECOUNT (GATECOUNT_CASES below)
   circuit_*.json        the random-circuit family (gate list, state before
                         the measurements, outcomes, final state)
+  circuit_*_summary.json  the same family at sizes too large to store
+                        (n = 24: the 30-qubit workload's circuit family):
+                        checksum + 64 sample amplitudes before and after
+                        the measurements, outcomes
 """
 import argparse
 import os
@@ -39,6 +43,10 @@ QMS_CASES = [
 ]
 
 CIRCUITS = [("circuit_n10", 10, 200, 7), ("circuit_n6", 6, 20, 3)]
+# the same family where the state is too large to store: checksum (pairwise
+# sums of re, im and i |psi_i|^2 / N) and 64 sample amplitudes, before and
+# after the measurements (the n = 24 one takes ~3 minutes on one core)
+SUMMARY_CIRCUITS = [("circuit_n10_summary", 10, 200, 7), ("circuit_n24_summary", 24, 200, 7)]
 
 # (name, qms command-line arguments after the output file)
 GATECOUNT_CASES = [
@@ -79,6 +87,9 @@ def main():
             stdout=subprocess.DEVNULL)
     for (name, n, layers, seed) in CIRCUITS:
         run([h, "circuit", str(n), str(layers), str(seed), out(name + ".json")],
+            stdout=subprocess.DEVNULL)
+    for (name, n, layers, seed) in SUMMARY_CIRCUITS:
+        run([h, "circuit_summary", str(n), str(layers), str(seed), out(name + ".json")],
             stdout=subprocess.DEVNULL)
     # stock drivers: output files, byte for byte
     tmp = out("qms_c1_out.txt")

@@ -5,7 +5,8 @@ register, reproduces the oracle's two-register chain bit for bit (return
 codes, final pcg32 state, E/X measurements, reverse counters; final state
 within 1e-12), with gate passes on and off.  The oracle's two-register mode
 is pinned against the reference-pinned one-register path in
-tests/test_qms_two_registers.py."""
+tests/test_qms_two_registers.py (identical return codes to the one-register
+chain that makes the reference's "comparison with 2enereg" draws)."""
 import os
 import subprocess
 import tempfile
@@ -71,45 +72,68 @@ def test_driver_option(qms):
         assert bad.returncode != 0 and "ene registers" in (bad.stdout + bad.stderr)
 
 
-# beta, r, seed: one launch of the batched kernel with uneven widths (the
-# widest, r = 4, is a 12-qubit chain: 64 KiB of LDS in FP64)
-BATCH = [(1.0, 4, 42), (2.0, 3, 123), (0.5, 2, 7), (3.0, 4, 99), (0.2, 3, 5)]
+# beta, r, seed: one launch of the batched kernel with uneven widths, r = 2..6
+# (the reference layout's 8 to 16 qubits per chain)
+BATCH = [(1.0, 4, 42), (2.0, 3, 123), (0.5, 2, 7), (3.0, 6, 99), (0.2, 5, 5), (1.3, 6, 31), (0.8, 5, 77)]
 
 
-@pytest.mark.parametrize("chains", [BATCH, [(1.5, 3, 11), (0.7, 2, 13), (2.5, 3, 17)]], ids=["r2-4", "r2-3"])
-def test_two_register_batch_vs_oracle(qms, chains):
-    """The batched kernel with two_registers: every chain reproduces the
-    oracle's two-register chain bit for bit (return codes, final pcg32 state,
-    acceptances, E/X).  The 12-qubit chains (r = 4) run a wavefront per chain
-    with the state in LDS, the others a workgroup per chain with one
-    amplitude per thread."""
-    steps, reset_each, therm, max_rev, emin, emax = 160, 1, 20, 100, -1.0, 3.0
-    print(f"two-register batch {chains}", flush=True)
+def check_chain_vs_oracle(out, c, beta, r, seed, steps, reset_each, therm, max_rev, emin, emax):
+    o = po.Qms(beta, r, seed, emin, emax, 10, max_rev, two_registers=True)
+    acc, rets = o.run(steps, reset_each, therm)
+    assert list(out["rets"][c]) == rets, f"chain {c} (beta={beta}, r={r}, seed={seed})"
+    assert int(out["accepted"][c]) == acc
+    assert int(out["rng_after"][c][-1]) == o.rng_state()
+    E, X = o.measures()
+    nm = int(out["n_meas"][c])
+    assert nm == len(E)
+    assert_bitwise(out["E"][c][:nm], np.array(E), f"E chain {c}")
+    assert_bitwise(out["X"][c][:nm], np.array(X), f"X chain {c}")
+    return rets
+
+
+@pytest.mark.parametrize("max_rev", [100, 2])
+def test_two_register_batch_vs_oracle(qms, max_rev):
+    """The batched kernel with two_registers: every chain (r = 2..6, 8 to 16
+    qubits in the full layout, stored in 2^(r+4) amplitudes) reproduces the
+    oracle's full-state two-register chain bit for bit (return codes, final
+    pcg32 state, acceptances, E/X).  Seven chains run a workgroup per chain
+    with one amplitude per thread; max_rev 2 makes restorations fail (-1)."""
+    steps, reset_each, therm, emin, emax = 60, 1, 5, -1.0, 3.0
+    print(f"two-register batch {BATCH} max_rev {max_rev}", flush=True)
+    out = qms.batch_run(BATCH, steps, reset_each, max_rev, emin, emax, 10, therm, rng_after=True, two_registers=True)
+    fails = 0
+    for c, (beta, r, seed) in enumerate(BATCH):
+        rets = check_chain_vs_oracle(out, c, beta, r, seed, steps, reset_each, therm, max_rev, emin, emax)
+        fails += rets.count(-1)
+    if max_rev == 2:
+        assert fails > 0
+
+
+def test_two_register_batch_lds_layout_vs_oracle(qms):
+    """300 two-register chains with r = 2..6 run in LDS (1 to 4 wavefronts
+    per chain); every 25th chain is checked against the oracle's full-state
+    two-register chain, bit for bit, over an inexact energy range ([0, 1])."""
+    nch, steps, reset_each, therm, max_rev, emin, emax = 300, 50, 2, 5, 30, 0.0, 1.0
+    chains = [(float(0.1 + 4.9 * k / (nch - 1)), 2 + k % 5, 500 + k) for k in range(nch)]
+    print("two-register batch, 300 chains", flush=True)
     out = qms.batch_run(chains, steps, reset_each, max_rev, emin, emax, 10, therm, rng_after=True, two_registers=True)
-    for c, (beta, r, seed) in enumerate(chains):
-        o = po.Qms(beta, r, seed, emin, emax, 10, max_rev, two_registers=True)
-        acc, rets = o.run(steps, reset_each, therm)
-        assert list(out["rets"][c]) == rets, f"chain {c} (beta={beta}, r={r}, seed={seed})"
-        assert int(out["accepted"][c]) == acc
-        assert int(out["rng_after"][c][-1]) == o.rng_state()
-        E, X = o.measures()
-        nm = int(out["n_meas"][c])
-        assert nm == len(E)
-        assert_bitwise(out["E"][c][:nm], np.array(E), f"E chain {c}")
-        assert_bitwise(out["X"][c][:nm], np.array(X), f"X chain {c}")
+    for c in range(0, nch, 25):
+        beta, r, seed = chains[c]
+        check_chain_vs_oracle(out, c, beta, r, seed, steps, reset_each, therm, max_rev, emin, emax)
 
 
-def test_two_register_batch_fp32_and_limits(qms):
-    """FP32 two-register chains follow the FP64 ones (decisions differ only when
-    a draw falls within FP32 rounding of a probability); a chain whose state
-    does not fit a workgroup's LDS (r = 6 in FP64: 16 qubits, 1 MiB) is
-    rejected before anything is launched."""
+def test_two_register_batch_fp32_follows_fp64(qms):
+    """FP32 two-register chains (r = 2..6) follow the FP64 ones (bit-exact with
+    the oracle above): decisions differ only when a draw falls within FP32
+    rounding of a probability, so nearly all 48 chains x 120 steps agree."""
+    chains = [(float(0.1 + 4.9 * k / 47), 2 + k % 5, 2000 + k) for k in range(48)]
     steps, emin, emax = 120, -1.0, 3.0
     print("two-register batch fp32", flush=True)
-    o64 = qms.batch_run(BATCH, steps, 1, 100, emin, emax, 10, 20, two_registers=True)
-    o32 = qms.batch_run(BATCH, steps, 1, 100, emin, emax, 10, 20, two_registers=True, fp32=True)
-    same = sum(1 for c in range(len(BATCH)) if np.array_equal(o64["rets"][c], o32["rets"][c]))
-    assert same >= len(BATCH) - 1
+    o64 = qms.batch_run(chains, steps, 1, 100, emin, emax, 10, 20, two_registers=True)
+    o32 = qms.batch_run(chains, steps, 1, 100, emin, emax, 10, 20, two_registers=True, fp32=True)
+    same = sum(1 for c in range(len(chains)) if np.array_equal(o64["rets"][c], o32["rets"][c]))
+    print(f"{same} of {len(chains)} FP32 chains identical to FP64", flush=True)
+    assert same >= 44
     from suqa_amd import capi
-    with pytest.raises(capi.SuqaError):
-        qms.batch_run([(1.0, 6, 3)], 10, 1, 100, emin, emax, 10, 2, two_registers=True)
+    with pytest.raises(capi.SuqaError):   # register widths are 2..6
+        qms.batch_run([(1.0, 7, 3)], 10, 1, 100, emin, emax, 10, 2, two_registers=True)

@@ -10,8 +10,8 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests.common import (QMS_FIXTURES, assert_bitwise, golden_state, hexarr, load, prep, qms_params,
-                          run_case)
+from tests.common import (QMS_FIXTURES, assert_bitwise, golden_state, golden_summary, hexarr, load, prep,
+                          qms_params, run_case, state_summary)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -132,6 +132,25 @@ def test_circuit_bitwise(fx):
     re, im = golden_state(d["final"])
     assert_bitwise(st.re(), re, "final re")
     assert_bitwise(st.im(), im, "final im")
+
+
+def test_circuit_summary_definition():
+    """The checksum and sample amplitudes tests compute from a state
+    (tests/common.state_summary) equal, bit for bit, what the reference harness
+    writes for the same state (the 10-qubit circuit, both stored in full);
+    the 24-qubit summary fixture is for the same circuit family and seed."""
+    full, summ = load("circuit_n10.json"), load("circuit_n10_summary.json")
+    assert (summ["n"], summ["layers"], summ["seed"]) == (full["n"], full["layers"], full["seed"])
+    for key in ("pre_measure", "final"):
+        sums, samples = state_summary(*golden_state(full[key]))
+        gsums, gsamples = golden_summary(summ[key])
+        assert sums == gsums, key
+        assert samples == gsamples, key
+    assert summ["measure"] == full["measure"]
+    big = load("circuit_n24_summary.json")
+    assert (big["n"], big["layers"], big["seed"]) == (24, 200, 7)
+    _, _, mq, mr = po.circuit(24, 200, 7)
+    assert [(q, float.fromhex(r)) for (q, r, _) in big["measure"]] == list(zip(mq, mr))
 
 
 def test_qms_driver_output_file_identical():
