@@ -220,21 +220,33 @@ int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out) {
     *out = nullptr;
     if (nq < 1 || nq > 32) return fail(SUQA_ERR_ARG, "nq must be in [1, 32]");
     auto* c = new suqa_hip_ctx();
-    SQE_HIP(hipGetDevice(&c->device));
+    // every failure below frees what was made so far (suqa_hip_clear) and
+    // keeps the failing call's message
+#define SETUP_HIP(call)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            const std::string msg_ = std::string(#call) + ": " + hipGetErrorString(e_);        \
+            suqa_hip_clear(c);                                                                 \
+            return fail(SUQA_ERR_HIP, msg_);                                                   \
+        }                                                                                      \
+    } while (0)
+    SETUP_HIP(hipGetDevice(&c->device));
     int ncu = 0;
-    SQE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    SETUP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     c->num_cu = ncu > 0 ? ncu : 256;
-    SQE_HIP(hipStreamCreate(&c->stream));
+    SETUP_HIP(hipStreamCreate(&c->stream));
     c->tile_bits = default_tile_bits();
     void* p = nullptr;
-    SQE_HIP(hipMalloc(&p, sizeof(double) * 4));
+    SETUP_HIP(hipMalloc(&p, sizeof(double) * 4));
     c->d_result = static_cast<double*>(p);
-    SQE_HIP(hipHostMalloc(&p, sizeof(double) * 4, hipHostMallocDefault));
+    SETUP_HIP(hipHostMalloc(&p, sizeof(double) * 4, hipHostMallocDefault));
     c->h_result = static_cast<double*>(p);
-    SQE_HIP(hipMalloc(&p, sizeof(uint32_t) * sqe::kMeasMax));
+    SETUP_HIP(hipMalloc(&p, sizeof(uint32_t) * sqe::kMeasMax));
     c->d_flags = static_cast<uint32_t*>(p);
-    SQE_HIP(hipHostMalloc(&p, sizeof(uint32_t) * sqe::kMeasMax, hipHostMallocDefault));
+    SETUP_HIP(hipHostMalloc(&p, sizeof(uint32_t) * sqe::kMeasMax, hipHostMallocDefault));
     c->h_flags = static_cast<uint32_t*>(p);
+#undef SETUP_HIP
     int rc = suqa_hip_allocate_state(c, nq);
     if (rc) {
         std::string keep = g_err;
