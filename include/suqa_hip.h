@@ -126,6 +126,20 @@ int suqa_hip_apply_resets(suqa_hip_ctx* ctx, const uint32_t* qs, const double* r
 int suqa_hip_prob_filter(suqa_hip_ctx* ctx, const uint32_t* qs, const uint32_t* q_mask,
                          uint32_t n, double* prob);
 
+/* No counterpart in the reference: suqa::prob_filter (suqa.cu:705-727) on a
+ * state the caller holds, for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing: the reduction partials live in a fixed
+ * device array of the library (4 MiB per GPU, held until the process ends),
+ * so 1 <= nq <= 26.  d_re, d_im: the state's 2^nq real and imaginary parts in
+ * device memory; d_prob: one device double that receives the probability.
+ * Asynchronous on `stream` (a hipStream_t; NULL: the null stream), no host
+ * synchronization; the same kernels as suqa_hip_prob_filter, so the same
+ * result bit for bit.  Calls that overlap on one GPU share the array and
+ * race: serialize them (one stream, or events). */
+int suqa_hip_prob_filter_noalloc(const double* d_re, const double* d_im, uint32_t nq, const uint32_t* qs,
+                                 const uint32_t* q_mask, uint32_t n, double* d_prob, void* stream);
+
 /* ---- QMS -------------------------------------------------------------- */
 /* qms::apply_W (qms.cuh:220-329): acceptance rotation on ancilla q_acc (the
  * most significant qubit) driven by the energy register read by

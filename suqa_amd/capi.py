@@ -80,6 +80,7 @@ SIGNATURES = {
     "suqa_hip_copy_kernel": (C.c_int, [CTX, C.c_void_p, C.c_void_p, C.c_uint64]),
     "suqa_hip_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "suqa_hip_device_free": (C.c_int, [C.c_void_p]),
+    "suqa_hip_prob_filter_noalloc": (C.c_int, [PD, PD, U32, PU32, PU32, U32, PD, C.c_void_p]),
     "suqa_hip_run_paulis": (C.c_int, [CTX, C.POINTER(PauliOp), U32]),
     "suqa_hip_set_tile_bits": (C.c_int, [CTX, U32]),
     "suqa_hip_get_tile_bits": (C.c_int, [CTX, PU32]),

@@ -137,6 +137,8 @@ int qft_crm(suqa_hip_ctx* c, uint32_t qc, uint32_t qt, int m, int nqubits, uint3
 // ------------------------------------------------------------------ timing
 namespace sqe {
 
+void set_last_error(const std::string& msg) { g_err = msg; }
+
 void timing_flush(suqa_hip_ctx* c) {
     if (c->recs.empty()) return;
     (void)hipStreamSynchronize(c->stream);
@@ -598,12 +600,7 @@ int suqa_hip_prob_filter(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* q_
     SQE_CTX(c);
     if (n && (!qs || !q_mask)) return fail(SUQA_ERR_ARG, "null argument");
     uint32_t mqs = 0u, mask = 0u;                    // suqa.cu:706-713
-    for (uint32_t k = 0; k < n; ++k) {
-        if (qs[k] >= 32) return fail(SUQA_ERR_ARG, "qubit index out of range");
-        mqs |= 1u << qs[k];
-    }
-    for (uint32_t k = 0; k < n; ++k)
-        if (q_mask[k]) mask |= q_mask[k] << qs[k];
+    if (!sqe::prob_filter_masks(qs, q_mask, n, &mqs, &mask)) return fail(SUQA_ERR_ARG, "qubit index out of range");
     Sel sel;
     if (!sqe::make_sel(c->nq, mqs, mask, &sel)) {
         if (prob) *prob = 0.0;

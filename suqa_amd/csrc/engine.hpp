@@ -115,6 +115,22 @@ struct suqa_hip_ctx {
 
 namespace sqe {
 
+// The selection of suqa::prob_filter (suqa.cu:706-713): bits qs[k] of the
+// index must read q_mask[k].  False when a qubit index is 32 or more.
+inline bool prob_filter_masks(const uint32_t* qs, const uint32_t* q_mask, uint32_t n, uint32_t* mqs,
+                              uint32_t* mask) {
+    uint32_t m = 0u, v = 0u;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (qs[k] >= 32) return false;
+        m |= 1u << qs[k];
+    }
+    for (uint32_t k = 0; k < n; ++k)
+        if (q_mask[k]) v |= q_mask[k] << qs[k];
+    *mqs = m;
+    *mask = v;
+    return true;
+}
+
 // Launchers (kernels.hip).  All enqueue on ctx->stream and return a HIP error.
 hipError_t launch_init(suqa_hip_ctx* c);
 hipError_t launch_scale(suqa_hip_ctx* c, double f);
@@ -128,6 +144,14 @@ hipError_t launch_phase_list(suqa_hip_ctx* c, const Sel& sel, uint32_t q0, uint3
 // d_result[0] (bit clear) and d_result[1] (bit set).
 hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
                        double alg_bytes);
+// The reduction of launch_norm on any state: k_norm writes one (sum, sum) pair
+// per block into partials, a fixed tree of k_sum_pairs reduces them into
+// result[0..1].  partials holds 2 * partial_pairs_capacity(2^nq) doubles;
+// every call writes what it reads.  With timed != nullptr the launches are
+// timed on that context.
+hipError_t launch_norm_tree(const double* re, const double* im, uint32_t nq, const Sel& sel, uint64_t split_bit,
+                            double* partials, double* result, hipStream_t st, suqa_hip_ctx* timed, const char* tag,
+                            double alg_bytes);
 hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale);
 // d_result[0..2] = sum re, sum im, sum i |psi_i|^2 over the whole state.
 hipError_t launch_checksum(suqa_hip_ctx* c);
@@ -158,6 +182,9 @@ hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double s
 hipError_t flush(suqa_hip_ctx* c);
 constexpr uint32_t kMinTileBits = 1, kMaxTileBits = 12;
 constexpr size_t kAutoFlush = 256;   // queued gates that trigger a flush (keeps the GPU busy)
+
+// the message suqa_hip_last_error() returns on this thread (capi.cpp)
+void set_last_error(const std::string& msg);
 
 // timing hooks (capi.cpp)
 void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes);

@@ -663,41 +663,47 @@ uint64_t partial_pairs_capacity(uint64_t N) {
     return p1 + (p1 + kPartChunk - 1) / kPartChunk + 8;
 }
 
-hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
-                       double alg_bytes) {
-    if (hipError_t e = flush(c)) return e;
+hipError_t launch_norm_tree(const double* re, const double* im, uint32_t nq, const Sel& sel, uint64_t split_bit,
+                            double* partials, double* result, hipStream_t st, suqa_hip_ctx* timed, const char* tag,
+                            double alg_bytes) {
     NormArgs a;
     a.split = split_bit;
     a.M = sel_mask(sel);
     a.V = sel.fval;
     const bool dl = (a.M & kLowMask) != 0;
     Sel it = sel;
-    if (dl) make_sel(c->nq, a.M & ~kLowMask, a.V & ~kLowMask, &it);
+    if (dl) make_sel(nq, a.M & ~kLowMask, a.V & ~kLowMask, &it);
     a.fb = it.fb;
     a.fval = it.fval;
     a.nitems = it.count / 2;
     const uint32_t g = grid_for(((a.nitems + 31) / 32) * 64);
-    timing_begin(c, tag, alg_bytes);
-    if (dl) hipLaunchKernelGGL((k_norm<true>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
-    else hipLaunchKernelGGL((k_norm<false>), dim3(g), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_partials);
-    timing_end(c);
+    if (timed) timing_begin(timed, tag, alg_bytes);
+    if (dl) hipLaunchKernelGGL((k_norm<true>), dim3(g), dim3(kBlock), 0, st, re, im, a, partials);
+    else hipLaunchKernelGGL((k_norm<false>), dim3(g), dim3(kBlock), 0, st, re, im, a, partials);
+    if (timed) timing_end(timed);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // deterministic tree: g pairs -> ceil(g / kPartChunk) pairs -> 1 pair
-    const double* in = c->d_partials;
+    const double* in = partials;
     uint64_t np = g;
-    timing_begin(c, "norm_final", 16.0 * g);
+    if (timed) timing_begin(timed, "norm_final", 16.0 * g);
     if (np > kPartChunk) {
         const uint64_t g2 = (np + kPartChunk - 1) / kPartChunk;
-        double* out = c->d_partials + 2 * (uint64_t)grid_for(c->N);
-        hipLaunchKernelGGL(k_sum_pairs, dim3((uint32_t)g2), dim3(kBlock), 0, c->stream, in, np,
-                           (uint64_t)kPartChunk, out);
+        double* out = partials + 2 * (uint64_t)grid_for(1ull << nq);
+        hipLaunchKernelGGL(k_sum_pairs, dim3((uint32_t)g2), dim3(kBlock), 0, st, in, np, (uint64_t)kPartChunk, out);
         in = out;
         np = g2;
     }
-    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(kBlock), 0, c->stream, in, np, np, c->d_result);
-    timing_end(c);
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(kBlock), 0, st, in, np, np, result);
+    if (timed) timing_end(timed);
     return hipGetLastError();
+}
+
+hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
+                       double alg_bytes) {
+    if (hipError_t e = flush(c)) return e;
+    return launch_norm_tree(c->d_re, c->d_im, c->nq, sel, split_bit, c->d_partials, c->d_result, c->stream, c, tag,
+                            alg_bytes);
 }
 
 hipError_t launch_checksum(suqa_hip_ctx* c) {
