@@ -227,6 +227,7 @@ typedef struct {
     uint64_t launches;
     double   total_ms;     /* sum of per-launch HIP-event durations */
     double   alg_bytes;    /* sum of algorithmic HBM bytes */
+    double   flops;        /* sum of MFMA flops (FP32 block kernels; 0 elsewhere) */
 } suqa_hip_kstat;
 
 /* Per-launch HIP events around every kernel launch on the context stream. */
@@ -292,6 +293,13 @@ int suqa_hip_f32_measure_qbit(suqa_hip_f32* c, uint32_t q, double rdoub, uint32_
 int suqa_hip_f32_fidelity(suqa_hip_f32* c, suqa_hip_ctx* ref, double* fidelity);
 int suqa_hip_f32_download(suqa_hip_f32* c, float* re, float* im);
 int suqa_hip_f32_synchronize(suqa_hip_f32* c);
+/* Per-launch HIP events around every block kernel (k_block<K> as "blockK",
+ * k_bpass<K> as "bpassK"): algorithmic HBM bytes (16 B per amplitude of the
+ * tiles a launch visits: FP32 re and im read and written) and MFMA flops
+ * (8 * 2^K per amplitude per dense block). */
+int suqa_hip_f32_timing_enable(suqa_hip_f32* c, int on);
+int suqa_hip_f32_timing_reset(suqa_hip_f32* c);
+int suqa_hip_f32_timing_report(suqa_hip_f32* c, suqa_hip_kstat* out, uint32_t max, uint32_t* n);
 /* Host only (no GPU): the fusion plan.  For each block b < *nblocks (at most
  * max_blocks): qubits[6b..6b+width) sorted, gates_in[b], and the FP64 matrix
  * mats[b * 2 * 4^width ...] as interleaved (re, im), row-major over the

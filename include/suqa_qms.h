@@ -88,6 +88,9 @@ typedef struct {
     uint32_t* n_meas;         /* [nchains] */
     uint32_t* accepted;       /* [nchains] */
     double    seconds;        /* out: kernel wall time */
+    uint64_t* lds_accesses;   /* [nchains] or NULL: algorithmic LDS bytes of each chain (the
+                                 amplitude reads and writes its gates and measurements must
+                                 make); 0 for chains of the one-amplitude-per-thread layout */
 } suqa_qms_batch_result;
 
 int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains, const suqa_qms_batch_params* p,

@@ -2,7 +2,7 @@
  * suqa_oracle.c -- CPU restatement of the reference's dense FP64 path.
  * TEST INFRASTRUCTURE ONLY (the oracle tests compare against, never the product); see the header.
  *
- * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).
+ * Build: gcc -O3 -ffp-contract=off -fPIC -shared (oracle/Makefile).
  * Loops keep the reference's order: full sweeps over i = 0..N-1 with a
  * bit-mask predicate, sequential FP64 sums, host sincos from libm.
  */

@@ -37,7 +37,7 @@ class PauliOp(C.Structure):
 
 class KStat(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double),
-                ("alg_bytes", C.c_double)]
+                ("alg_bytes", C.c_double), ("flops", C.c_double)]
 
 
 # name -> (restype, argtypes); every entry point of include/suqa_hip.h
@@ -103,6 +103,9 @@ SIGNATURES = {
     "suqa_hip_f32_fidelity": (C.c_int, [CTX, CTX, PD]),
     "suqa_hip_f32_download": (C.c_int, [CTX, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "suqa_hip_f32_synchronize": (C.c_int, [CTX]),
+    "suqa_hip_f32_timing_enable": (C.c_int, [CTX, C.c_int]),
+    "suqa_hip_f32_timing_reset": (C.c_int, [CTX]),
+    "suqa_hip_f32_timing_report": (C.c_int, [CTX, C.POINTER(KStat), U32, PU32]),
     "suqa_hip_fuse_plan": (C.c_int, [C.POINTER(Gate), U32, U32, U32, U32, PU32, PU32, PU32, PD]),
 }
 
@@ -451,6 +454,20 @@ class F32Engine:
 
     def synchronize(self):
         check32(lib().suqa_hip_f32_synchronize(self.ctx))
+
+    def timing(self, on):
+        check32(lib().suqa_hip_f32_timing_enable(self.ctx, 1 if on else 0))
+
+    def timing_reset(self):
+        check32(lib().suqa_hip_f32_timing_reset(self.ctx))
+
+    def timing_report(self):
+        """{family: launches, ms, bytes (algorithmic HBM), flops (MFMA)} of the block kernels."""
+        arr = (KStat * 64)()
+        n = U32()
+        check32(lib().suqa_hip_f32_timing_report(self.ctx, arr, 64, C.byref(n)))
+        return {arr[k].name.decode(): dict(launches=arr[k].launches, ms=arr[k].total_ms, bytes=arr[k].alg_bytes,
+                                           flops=arr[k].flops) for k in range(min(n.value, 64))}
 
 
 def fuse_plan(gates, nq, width, n=None):

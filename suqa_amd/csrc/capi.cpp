@@ -856,6 +856,7 @@ int suqa_hip_timing_report(suqa_hip_ctx* c, suqa_hip_kstat* out, uint32_t max, u
         out[k].launches = c->stats[k].launches;
         out[k].total_ms = c->stats[k].total_ms;
         out[k].alg_bytes = c->stats[k].alg_bytes;
+        out[k].flops = 0.0;
     }
     if (n) *n = (uint32_t)c->stats.size();
     return SUQA_OK;

@@ -49,6 +49,20 @@ struct suqa_hip_f32 {
     float* d_mats = nullptr;   // block matrices in MFMA lane order
     size_t mats_cap = 0;       // floats
     uint32_t passes = 0;       // 0: one block per launch, 1: block passes, 2: passes with sparse Pauli rotations
+    // per-launch HIP events of the block kernels (suqa_hip_f32_timing_*)
+    bool timing = false;
+    struct Rec {
+        int stat;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> evpool;
+    struct Stat {
+        std::string name;
+        uint64_t launches = 0;
+        double ms = 0.0, bytes = 0.0, flops = 0.0;
+    };
+    std::vector<Stat> stats;
 };
 
 namespace sqf {
@@ -66,6 +80,49 @@ int fail(int code, const std::string& m) {
     } while (0)
 
 constexpr int kBlock = 256;
+
+// ---- per-launch timing ------------------------------------------------------
+hipEvent_t take_event(suqa_hip_f32* c) {
+    if (!c->evpool.empty()) {
+        hipEvent_t e = c->evpool.back();
+        c->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+// Brackets one launch: begin before, end after; bytes are the algorithmic HBM
+// bytes of the launch, flops its MFMA flops.
+int timing_begin(suqa_hip_f32* c, const std::string& name, double bytes, double flops) {
+    if (!c->timing) return -1;
+    size_t k = 0;
+    while (k < c->stats.size() && c->stats[k].name != name) ++k;
+    if (k == c->stats.size()) {
+        c->stats.emplace_back();
+        c->stats.back().name = name;
+    }
+    c->stats[k].launches++;
+    c->stats[k].bytes += bytes;
+    c->stats[k].flops += flops;
+    suqa_hip_f32::Rec r{(int)k, take_event(c), take_event(c)};
+    (void)hipEventRecord(r.e0, c->stream);
+    c->recs.push_back(r);
+    return (int)k;
+}
+void timing_end(suqa_hip_f32* c, int k) {
+    if (k >= 0) (void)hipEventRecord(c->recs.back().e1, c->stream);
+}
+// After the stream has drained: the recorded durations into the totals.
+void timing_collect(suqa_hip_f32* c) {
+    for (auto& r : c->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) c->stats[r.stat].ms += ms;
+        c->evpool.push_back(r.e0);
+        c->evpool.push_back(r.e1);
+    }
+    c->recs.clear();
+}
 
 __device__ __forceinline__ uint64_t item_id() {   // XCD-contiguous block remap (grid % 8 == 0)
     const uint64_t G = gridDim.x, b = blockIdx.x;
@@ -713,17 +770,22 @@ int apply_chunk(suqa_hip_f32* c, const std::vector<Block>& blocks, size_t lo, si
         a.A = c->d_mats + off[i - lo];
         const uint32_t g = (uint32_t)(((a.ntiles + 7) / 8) * 8);
         const size_t lds = sizeof(float) * 2 * ((size_t)1 << a.ntile);
+        const double amps = (double)a.ntiles * (double)(1ull << a.ntile);
+        const int tk = timing_begin(c, "block" + std::to_string(blocks[i].K), 16.0 * amps,
+                                    8.0 * (double)(1u << blocks[i].K) * amps);
         switch (blocks[i].K) {
         case 4: hipLaunchKernelGGL(k_block<4>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a); break;
         case 5: hipLaunchKernelGGL(k_block<5>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a); break;
         case 6: hipLaunchKernelGGL(k_block<6>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a); break;
         default: return fail(SUQA_ERR_ARG, "block width must be 4..6");
         }
+        timing_end(c, tk);
         SQF_HIP(hipGetLastError());
     }
     // the host matrix buffer must outlive the copy, and the device buffer is
     // rewritten by the next chunk
     SQF_HIP(hipStreamSynchronize(c->stream));
+    timing_collect(c);
     return SUQA_OK;
 }
 
@@ -760,15 +822,22 @@ int flush_passes(suqa_hip_f32* c, std::vector<PassLaunch>& L, std::vector<float>
         for (uint32_t k = 0; k < l.a.nblocks; ++k) l.a.b[k].A = c->d_mats + l.off[k];
         const uint32_t g = (uint32_t)(((l.a.ntiles + 7) / 8) * 8);
         const size_t lds = sizeof(float) * 2 * ((size_t)1 << l.a.t);
+        const double amps = (double)l.a.ntiles * (double)(1ull << l.a.t);
+        uint32_t dense = 0;   // MFMA blocks of the pass (swaps and sparse rotations use none)
+        for (uint32_t k = 0; k < l.a.nblocks; ++k) dense += l.a.b[k].K ? 1u : 0u;
+        const int tk = timing_begin(c, "bpass" + std::to_string(l.a.K), 16.0 * amps,
+                                    8.0 * (double)(1u << l.a.K) * amps * dense);
         switch (l.a.K) {
         case 4: hipLaunchKernelGGL(k_bpass<4>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, l.a); break;
         case 5: hipLaunchKernelGGL(k_bpass<5>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, l.a); break;
         case 6: hipLaunchKernelGGL(k_bpass<6>, dim3(g), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, l.a); break;
         default: return fail(SUQA_ERR_ARG, "block width must be 4..6");
         }
+        timing_end(c, tk);
         SQF_HIP(hipGetLastError());
     }
     SQF_HIP(hipStreamSynchronize(c->stream));
+    timing_collect(c);
     L.clear();
     buf.clear();
     return SUQA_OK;
@@ -1047,6 +1116,11 @@ int suqa_hip_f32_setup(uint32_t nq, suqa_hip_f32** out) {
 int suqa_hip_f32_clear(suqa_hip_f32* c) {
     if (!c) return SUQA_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& r : c->recs) {
+        (void)hipEventDestroy(r.e0);
+        (void)hipEventDestroy(r.e1);
+    }
+    for (auto e : c->evpool) (void)hipEventDestroy(e);
     if (c->d_data) (void)hipFree(c->d_data);
     if (c->d_partials) (void)hipFree(c->d_partials);
     if (c->d_result) (void)hipFree(c->d_result);
@@ -1229,6 +1303,37 @@ int suqa_hip_f32_download(suqa_hip_f32* c, float* re, float* im) {
     if (re) SQF_HIP(hipMemcpyAsync(re, c->d_re, c->N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (im) SQF_HIP(hipMemcpyAsync(im, c->d_im, c->N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     SQF_HIP(hipStreamSynchronize(c->stream));
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_timing_enable(suqa_hip_f32* c, int on) {
+    if (!c) return fail(SUQA_ERR_ARG, "null context");
+    c->timing = on != 0;
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_timing_reset(suqa_hip_f32* c) {
+    if (!c) return fail(SUQA_ERR_ARG, "null context");
+    SQF_HIP(hipStreamSynchronize(c->stream));
+    timing_collect(c);
+    c->stats.clear();
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_timing_report(suqa_hip_f32* c, suqa_hip_kstat* out, uint32_t max, uint32_t* n) {
+    if (!c || (max && !out)) return fail(SUQA_ERR_ARG, "null argument");
+    SQF_HIP(hipStreamSynchronize(c->stream));
+    timing_collect(c);
+    uint32_t k = 0;
+    for (; k < c->stats.size() && k < max; ++k) {
+        std::memset(&out[k], 0, sizeof(out[k]));
+        std::strncpy(out[k].name, c->stats[k].name.c_str(), sizeof(out[k].name) - 1);
+        out[k].launches = c->stats[k].launches;
+        out[k].total_ms = c->stats[k].ms;
+        out[k].alg_bytes = c->stats[k].bytes;
+        out[k].flops = c->stats[k].flops;
+    }
+    if (n) *n = (uint32_t)c->stats.size();
     return SUQA_OK;
 }
 

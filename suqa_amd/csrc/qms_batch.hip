@@ -126,6 +126,7 @@ struct ChainBase {
     bool two;            // E_old in its own register (qms.cuh:72-85, 134-136, 201-202)
     uint32_t vold;       // two registers: the value the E_old register holds (definite, see Chain)
     uint32_t lane;       // 0 on the one thread that writes a chain's outputs
+    uint64_t acc_amps;   // LdsOps: amplitude accesses (reads + writes) the gates and measurements must make
     Rng rng;
     const ChainDev* P;
 };
@@ -145,6 +146,7 @@ struct LdsOps : ChainBase {
 
     // ------------------------------------------------------------ gates
     __device__ __forceinline__ void init_zero() {
+        acc_amps += N;
         for (uint32_t i = lane; i < N; i += 64 * TW) {
             st[i].r = i == 0 ? T(1) : T(0);
             st[i].i = T(0);
@@ -154,6 +156,7 @@ struct LdsOps : ChainBase {
 
     // H on q (suqa_cpu.hpp:186-204) for states with (i & gc) == gc
     __device__ __forceinline__ void h(uint32_t q, uint32_t gc) {
+        acc_amps += 4ull * ((N / 2) >> __builtin_popcount(gc));
         for (uint32_t p = lane; p < N / 2; p += 64 * TW) {
             const uint32_t a = ins0(p, q);
             if ((a & gc) != gc) continue;
@@ -169,6 +172,7 @@ struct LdsOps : ChainBase {
 
     // X on qt for states whose control bits read cv (suqa_cpu.hpp:239-251)
     __device__ __forceinline__ void mcx(uint32_t qt, uint32_t cm, uint32_t cv) {
+        acc_amps += 4ull * ((N / 2) >> (__builtin_popcount(cm) - 1));
         for (uint32_t p = lane; p < N / 2; p += 64 * TW) {
             const uint32_t a = ins0(p, qt);
             const uint32_t b = a | (1u << qt);
@@ -194,6 +198,7 @@ struct LdsOps : ChainBase {
         const uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
         const uint32_t m = (1u << q1) | (1u << q2);
         const uint32_t cnt = q1 == q2 ? N / 2 : N / 4;
+        acc_amps += 2ull * cnt;
         for (uint32_t p = lane; p < cnt; p += 64 * TW) {
             const uint32_t i = (q1 == q2 ? ins0(p, lo) : ins0(ins0(p, lo), hi)) | m;
             const T cpy = st[i].r;
@@ -208,6 +213,7 @@ struct LdsOps : ChainBase {
         const T c = T(cd), s = T(sd);
         const uint32_t pv = qa < qb_ ? qa : qb_;
         const uint32_t X = (1u << qa) | (1u << qb_);
+        acc_amps += 4ull * ((N / 2) >> __builtin_popcount(gc));
         for (uint32_t p = lane; p < N / 2; p += 64 * TW) {
             const uint32_t j = ins0(p, pv);
             if ((j & gc) != gc) continue;
@@ -223,6 +229,7 @@ struct LdsOps : ChainBase {
 
     // measure_qbit (suqa.cu:673-696): c = (rd < P(q=1)), collapse, renormalize
     __device__ __forceinline__ uint32_t measure(uint32_t q, double rd) {
+        acc_amps += N + N / 2 + N;   // read all, read the kept half, write all
         double s0 = 0.0, s1 = 0.0;
         for (uint32_t i = lane; i < N; i += 64 * TW) {
             const double a = st[i].r, b = st[i].i;   // FP64 probabilities in both modes
@@ -265,6 +272,7 @@ struct LdsOps : ChainBase {
     // the kept half and are scaled as measure() scales it (an outcome of
     // probability 0 leaves 0 * 1/sqrt(0) = NaN, as it would in the full state)
     __device__ __forceinline__ uint32_t measure_fixed(uint32_t bitv, double rd) {
+        acc_amps += 3ull * N;   // read all twice, write all
         double s = 0.0;
         for (uint32_t i = lane; i < N; i += 64 * TW) {
             const double a = st[i].r, b = st[i].i;
@@ -291,6 +299,7 @@ struct LdsOps : ChainBase {
     // the classical value (two registers: the value the E_old register holds).
     __device__ __forceinline__ void W(uint32_t Eold) {
         const uint32_t half = N / 2;
+        acc_amps += 2ull * N;
         for (uint32_t j = lane; j < half; j += 64 * TW) {
             const uint32_t i = j + half;
             const uint32_t En = (i >> en) & (L - 1);
@@ -705,6 +714,7 @@ struct Chain : Ops {
         if (lane == 0) {
             b.n_meas[chain] = nm;
             b.accepted[chain] = acc_count;
+            if (b.lds_amps) b.lds_amps[chain] = this->acc_amps;
         }
     }
 };
@@ -737,6 +747,7 @@ __global__ __launch_bounds__(64 * TW) void k_qms_batch(const BatchDev b) {
         c.acc = c.n - 1;
         c.en = E0;
         c.lane = lane;
+        c.acc_amps = 0;
         c.st = reinterpret_cast<typename LdsOps<T, TW>::Amp*>(lds);
         c.red = reinterpret_cast<double*>(c.st + c.N);
         c.run(b, chain);
@@ -762,6 +773,7 @@ __global__ __launch_bounds__(1024) void k_qms_chain_reg(const BatchDev b) {
     c.en = E0;
     c.idx = threadIdx.x;
     c.lane = threadIdx.x;
+    c.acc_amps = 0;
     c.red = lds;   // 2 x 32 doubles, then the exchange buffers
     c.xr = reinterpret_cast<T*>(lds + 64);
     c.xi = c.xr + 2 * c.N;
@@ -859,6 +871,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     int32_t* d_rets = nullptr;
     uint64_t* d_rng = nullptr;
     uint32_t *d_nm = nullptr, *d_acc = nullptr, *d_next = nullptr, *d_order = nullptr;
+    uint64_t* d_lds = nullptr;
     std::vector<size_t> need(nchains);
     std::vector<uint32_t> order(nchains);
     hipStream_t st = nullptr;
@@ -892,6 +905,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     QB_HIP(hipMalloc(&d_nm, sizeof(uint32_t) * nchains));
     QB_HIP(hipMalloc(&d_acc, sizeof(uint32_t) * nchains));
     QB_HIP(hipMalloc(&d_next, sizeof(uint32_t)));
+    if (out->lds_accesses) QB_HIP(hipMalloc(&d_lds, sizeof(uint64_t) * nchains));
     // chains grouped by the LDS their state needs, widest first: when LDS
     // limits the resident waves, one launch per group, so narrow chains are not
     // held to the widest chain's occupancy
@@ -919,6 +933,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     b.n_meas = d_nm;
     b.accepted = d_acc;
     b.two_registers = p->two_registers ? 1u : 0u;
+    b.lds_amps = d_lds;
     {
         int dev = 0;
         QB_HIP(hipGetDevice(&dev));
@@ -1006,6 +1021,10 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     }
     QB_HIP(hipMemcpy(out->n_meas, d_nm, sizeof(uint32_t) * nchains, hipMemcpyDeviceToHost));
     QB_HIP(hipMemcpy(out->accepted, d_acc, sizeof(uint32_t) * nchains, hipMemcpyDeviceToHost));
+    if (d_lds) {   // chains of the per-amplitude layout count nothing (0)
+        QB_HIP(hipMemcpy(out->lds_accesses, d_lds, sizeof(uint64_t) * nchains, hipMemcpyDeviceToHost));
+        for (uint32_t c = 0; c < nchains; ++c) out->lds_accesses[c] *= 2 * elt;   // accesses -> bytes
+    }
 done:
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1020,6 +1039,7 @@ done:
     (void)hipFree(d_acc);
     (void)hipFree(d_next);
     (void)hipFree(d_order);
+    (void)hipFree(d_lds);
     if (st) (void)hipStreamDestroy(st);
     return rc;
 }

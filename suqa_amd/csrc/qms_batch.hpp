@@ -41,6 +41,7 @@ struct BatchDev {
     uint32_t* n_meas;                       // [nchains]
     uint32_t* accepted;                     // [nchains]
     uint32_t two_registers;                 // E_old in its own register: 2r + 4 qubits per chain
+    uint64_t* lds_amps;                     // [nchains] LDS amplitude accesses per chain, or null
 };
 
 }  // namespace qb

@@ -40,7 +40,8 @@ class BatchParams(C.Structure):
 class BatchResult(C.Structure):
     _fields_ = [("rets", C.POINTER(C.c_int32)), ("rng_after", C.POINTER(C.c_uint64)),
                 ("E", C.POINTER(C.c_double)), ("X", C.POINTER(C.c_double)), ("cap_meas", C.c_uint32),
-                ("n_meas", C.POINTER(C.c_uint32)), ("accepted", C.POINTER(C.c_uint32)), ("seconds", C.c_double)]
+                ("n_meas", C.POINTER(C.c_uint32)), ("accepted", C.POINTER(C.c_uint32)), ("seconds", C.c_double),
+                ("lds_accesses", C.POINTER(C.c_uint64))]
 
 
 SIGNATURES = {
@@ -101,11 +102,12 @@ def run(beta, steps, reset_each, r, seed, ene_min=0.0, ene_max=1.0, pe_steps=10,
 
 
 def batch_run(chains, steps, reset_each=1, max_reverse=100, ene_min=0.0, ene_max=1.0, pe_steps=10,
-              thermalization=100, cap_meas=None, rng_after=False, fp32=False, two_registers=False):
+              thermalization=100, cap_meas=None, rng_after=False, fp32=False, two_registers=False, lds_bytes=False):
     """chains: list of (beta, r, seed).  Returns per-chain arrays.  fp32: FP32
     amplitudes with FP64 probabilities (the precision pattern).  two_registers:
     E_old in its own register (2r + 4 qubits per chain, r in 2..6, stored in
-    2^(r+4) amplitudes: E_old is definite outside its own phase estimation)."""
+    2^(r+4) amplitudes: E_old is definite outside its own phase estimation).
+    lds_bytes: also return each chain's algorithmic LDS bytes."""
     n = len(chains)
     specs = (ChainSpec * n)(*[ChainSpec(b, r, s) for (b, r, s) in chains])
     bp = BatchParams(steps, reset_each, max_reverse, ene_min, ene_max, pe_steps, thermalization, 1 if fp32 else 0,
@@ -122,7 +124,9 @@ def batch_run(chains, steps, reset_each=1, max_reverse=100, ene_min=0.0, ene_max
     res.rng_after = _ptr(rng, C.c_uint64) if rng_after else C.POINTER(C.c_uint64)()
     res.E, res.X, res.cap_meas = _ptr(E, C.c_double), _ptr(X, C.c_double), cap
     res.n_meas, res.accepted = _ptr(nm, C.c_uint32), _ptr(acc, C.c_uint32)
+    lds = np.zeros(n, dtype=np.uint64) if lds_bytes else None
+    res.lds_accesses = _ptr(lds, C.c_uint64) if lds_bytes else C.POINTER(C.c_uint64)()
     rc = lib().suqa_qms_batch_run(specs, n, C.byref(bp), C.byref(res))
     if rc != 0:
         raise capi.SuqaError(rc, "suqa_qms_batch_run failed (see stderr)")
-    return dict(rets=rets, rng_after=rng, E=E, X=X, n_meas=nm, accepted=acc, seconds=res.seconds)
+    return dict(rets=rets, rng_after=rng, E=E, X=X, n_meas=nm, accepted=acc, seconds=res.seconds, lds_bytes=lds)
