@@ -481,6 +481,18 @@ def trotter_leg(capi, wl, no_fused, full_qpe=True, no_cpu=False):
                         "fp32_roofline": kernel_roofline(rep32, fam32, mfma_tflops=157.3),
                         "fp32_fidelity_vs_fp64": f.fidelity(e)})
             log(f"trotter fp32: {rot / dt32:.0f} rotations/s at {best}, fidelity {res['fp32_fidelity_vs_fp64']!r}")
+        # the same controlled steps replayed from one capture, as the QPE below
+        # applies them (after the fidelity above: this changes the state)
+        e.graph_begin()
+        e.run_paulis(ops)
+        g = e.graph_end()
+        e.graph_launch(g)
+        e.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            e.graph_launch(g)
+        e.synchronize()
+        res["fp64_graph_replay_rotations_per_s"] = round(rot / (time.perf_counter() - t0), 1)
         # the whole QPE as specified (20 Trotter steps per power, 120,960 controlled
         # rotations), or with 2 steps per power when full_qpe is off
         qs = steps if full_qpe else 2

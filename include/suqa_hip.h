@@ -115,7 +115,8 @@ int suqa_hip_measure_qbit(suqa_hip_ctx* ctx, uint32_t q, double rdoub, uint32_t*
 int suqa_hip_apply_reset(suqa_hip_ctx* ctx, uint32_t q, double rdoub, uint32_t gc_mask);
 /* suqa::measure_qbits (suqa.cu:699-702): measure_qbit on qs[k] with rdoubs[k],
  * in order, outcomes into cs[0..n).  On a state of at most 2^14 amplitudes
- * the n measurements are one launch and one wait. */
+ * the n measurements are one launch and one wait; with cs == NULL (outcomes
+ * not wanted) they are one launch and nothing waits. */
 int suqa_hip_measure_qbits(suqa_hip_ctx* ctx, const uint32_t* qs, const double* rdoubs, uint32_t n, uint32_t* cs);
 /* suqa::apply_reset(bmReg, vector<double>) (suqa.cu:752-757): apply_reset on
  * qs[k] with rdoubs[k], in order; one launch and no wait on a state of at
@@ -165,6 +166,19 @@ int suqa_hip_synchronize(suqa_hip_ctx* ctx);
  * out3 = {sum re, sum im, sum_i i |psi_i|^2 / N}, FP64 tree sums over the
  * whole state (deterministic for a given nq). */
 int suqa_hip_checksum(suqa_hip_ctx* ctx, double* out3);
+
+/* ---- captured launch sequences (extension) ----------------------------
+ * No counterpart in the reference, which launches every kernel from the
+ * host.  graph_begin applies the queued gates, then captures the context
+ * stream: gate calls until graph_end are recorded (the passes they form),
+ * not run; graph_end returns an id; graph_launch applies the queued gates,
+ * then replays the sequence with the same results as the calls themselves.
+ * Entry points that read a value back (norms, probabilities, measurements)
+ * fail while a capture is open.  A graph holds the state's device
+ * addresses: allocate_state drops every graph of the context. */
+int suqa_hip_graph_begin(suqa_hip_ctx* ctx);
+int suqa_hip_graph_end(suqa_hip_ctx* ctx, uint32_t* graph);
+int suqa_hip_graph_launch(suqa_hip_ctx* ctx, uint32_t graph);
 
 /* ---- workload runner and timing (extensions) -------------------------- */
 /* Gate kinds of the random-circuit workload. RX(a) = exp(-i a X/2),

@@ -73,6 +73,9 @@ SIGNATURES = {
     "suqa_hip_upload": (C.c_int, [CTX, PD, PD]),
     "suqa_hip_synchronize": (C.c_int, [CTX]),
     "suqa_hip_checksum": (C.c_int, [CTX, PD]),
+    "suqa_hip_graph_begin": (C.c_int, [CTX]),
+    "suqa_hip_graph_end": (C.c_int, [CTX, PU32]),
+    "suqa_hip_graph_launch": (C.c_int, [CTX, U32]),
     "suqa_hip_run_gates": (C.c_int, [CTX, C.POINTER(Gate), U32]),
     "suqa_hip_timing_enable": (C.c_int, [CTX, C.c_int]),
     "suqa_hip_timing_reset": (C.c_int, [CTX]),
@@ -307,6 +310,18 @@ class Engine:
 
     def synchronize(self):
         check(lib().suqa_hip_synchronize(self.ctx))
+
+    def graph_begin(self):
+        """Start capturing the context stream (gate calls are recorded, not run)."""
+        check(lib().suqa_hip_graph_begin(self.ctx))
+
+    def graph_end(self):
+        g = U32()
+        check(lib().suqa_hip_graph_end(self.ctx, C.byref(g)))
+        return g.value
+
+    def graph_launch(self, g):
+        check(lib().suqa_hip_graph_launch(self.ctx, g))
 
     def checksum(self):
         """(sum re, sum im, sum_i i |psi_i|^2 / N), FP64 tree sums on the device."""

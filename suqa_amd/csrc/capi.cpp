@@ -44,6 +44,7 @@ int fail(int code, const std::string& msg) {
 uint32_t popc32(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
 
 int read_result(suqa_hip_ctx* c, int n) {
+    if (c->capturing) return fail(SUQA_ERR_ARG, "a value cannot be read back while a graph is being captured");
     SQE_HIP(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     SQE_HIP(hipStreamSynchronize(c->stream));
     return SUQA_OK;
@@ -62,8 +63,15 @@ uint32_t default_tile_bits() {
     return (uint32_t)v;
 }
 
+void drop_graphs(suqa_hip_ctx* c) {
+    for (auto& g : c->graphs)
+        if (g) (void)hipGraphExecDestroy(g);
+    c->graphs.clear();
+}
+
 int free_state(suqa_hip_ctx* c) {
     c->queue.clear();
+    drop_graphs(c);   // they hold the state's addresses
     if (c->d_data) SQE_HIP(hipFree(c->d_data));
     if (c->d_partials) SQE_HIP(hipFree(c->d_partials));
     c->d_partials = nullptr;
@@ -163,7 +171,7 @@ static hipEvent_t get_event(suqa_hip_ctx* c) {
 }
 
 void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes) {
-    if (!c->timing) return;
+    if (!c->timing || c->capturing) return;
     if (c->recs.size() >= 4096) timing_flush(c);
     int idx = -1;
     for (size_t k = 0; k < c->stats.size(); ++k)
@@ -180,7 +188,7 @@ void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes) {
 }
 
 void timing_end(suqa_hip_ctx* c) {
-    if (!c->timing || c->recs.empty()) return;
+    if (!c->timing || c->capturing || c->recs.empty()) return;
     (void)hipEventRecord(c->recs.back().e1, c->stream);
 }
 
@@ -261,6 +269,11 @@ int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out) {
 
 int suqa_hip_clear(suqa_hip_ctx* c) {
     if (!c) return SUQA_OK;
+    if (c->capturing) {
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(c->stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+        c->capturing = false;
+    }
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& r : c->recs) {
         (void)hipEventDestroy(r.e0);
@@ -526,6 +539,7 @@ int suqa_hip_apply_qft_inverse(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, 
 // outcomes copied back only when the caller asks for them.
 static int measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, bool reset,
                          uint32_t gc, uint32_t* cs) {
+    if (cs && c->capturing) return fail(SUQA_ERR_ARG, "a measurement cannot be read back while a graph is being captured");
     for (uint32_t k0 = 0; k0 < n; k0 += sqe::kMeasMax) {
         const uint32_t m = std::min<uint32_t>(sqe::kMeasMax, n - k0);
         SQE_HIP(sqe::launch_measure_small(c, qs + k0, rds + k0, m, reset ? ~0u : 0u, gc));
@@ -576,12 +590,12 @@ int suqa_hip_apply_reset(suqa_hip_ctx* c, uint32_t q, double rdoub, uint32_t gc)
 
 int suqa_hip_measure_qbits(suqa_hip_ctx* c, const uint32_t* qs, const double* rdoubs, uint32_t n, uint32_t* cs) {
     SQE_CTX(c);
-    if (n && (!qs || !rdoubs || !cs)) return fail(SUQA_ERR_ARG, "null argument");
+    if (n && (!qs || !rdoubs)) return fail(SUQA_ERR_ARG, "null argument");
     for (uint32_t k = 0; k < n; ++k)
         if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
     if (n && c->nq <= sqe::kMeasSmallBits) return measure_small(c, qs, rdoubs, n, false, 0u, cs);
     for (uint32_t k = 0; k < n; ++k)
-        if (int rc = suqa_hip_measure_qbit(c, qs[k], rdoubs[k], &cs[k])) return rc;
+        if (int rc = suqa_hip_measure_qbit(c, qs[k], rdoubs[k], cs ? &cs[k] : nullptr)) return rc;
     return SUQA_OK;
 }
 
@@ -859,6 +873,46 @@ int suqa_hip_timing_report(suqa_hip_ctx* c, suqa_hip_kstat* out, uint32_t max, u
         out[k].flops = 0.0;
     }
     if (n) *n = (uint32_t)c->stats.size();
+    return SUQA_OK;
+}
+
+int suqa_hip_graph_begin(suqa_hip_ctx* c) {
+    SQE_CTX(c);
+    if (c->capturing) return fail(SUQA_ERR_ARG, "a graph capture is already open on this context");
+    SQE_HIP(sqe::flush(c));   // queued gates run before the captured sequence, not in it
+    SQE_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    return SUQA_OK;
+}
+
+int suqa_hip_graph_end(suqa_hip_ctx* c, uint32_t* graph) {
+    SQE_CTX(c);
+    if (!graph) return fail(SUQA_ERR_ARG, "null argument");
+    if (!c->capturing) return fail(SUQA_ERR_ARG, "no graph capture is open on this context");
+    const hipError_t ef = sqe::flush(c);   // the captured gates' passes
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+    c->capturing = false;
+    if (ef != hipSuccess || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return fail(SUQA_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(ef != hipSuccess ? ef : ee));
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(SUQA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    c->graphs.push_back(x);
+    *graph = (uint32_t)(c->graphs.size() - 1);
+    return SUQA_OK;
+}
+
+int suqa_hip_graph_launch(suqa_hip_ctx* c, uint32_t graph) {
+    SQE_CTX(c);
+    if (c->capturing) return fail(SUQA_ERR_ARG, "a graph cannot be launched while one is being captured");
+    if (graph >= c->graphs.size() || !c->graphs[graph]) return fail(SUQA_ERR_ARG, "unknown graph");
+    SQE_HIP(sqe::flush(c));   // gates queued before it run first
+    SQE_HIP(hipGraphLaunch(c->graphs[graph], c->stream));
     return SUQA_OK;
 }
 

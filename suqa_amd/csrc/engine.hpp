@@ -102,6 +102,10 @@ struct suqa_hip_ctx {
     uint32_t tile_bits = 0;
     std::vector<sqe::GateDesc> queue;
     uint64_t passes = 0, pass_gates = 0, single_sweeps = 0;
+    // captured launch sequences (suqa_hip_graph_*): replayable graphs on the
+    // context stream; they hold the state's device addresses
+    bool capturing = false;
+    std::vector<hipGraphExec_t> graphs;
     // timing
     bool timing = false;
     struct Rec {

@@ -388,7 +388,18 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
 // across wavefronts) and every thread computes its own outputs with the
 // arithmetic and roles of tile_pairs; a phase gate is element-wise as in
 // tile_phase.  Latency-bound states take one exchange per gate instead of
-// register steps on tsz / 8 threads.
+// register steps on tsz / 8 threads.  The pass's gate descriptors are copied
+// from the kernel arguments into LDS by all threads at once first: read gate by
+// gate from the argument segment, each gate waited for its own scalar loads
+// (about half a microsecond per gate).
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
+}
+__device__ __forceinline__ double unid(double v) {
+    return __longlong_as_double((long long)uni64((uint64_t)__double_as_longlong(v)));
+}
+
 __global__ __launch_bounds__(1024) void k_tile_tiny(double* __restrict__ re, double* __restrict__ im, const TileArgs a) {
     extern __shared__ double lds[];
     const uint32_t tsz = 1u << a.t;
@@ -396,9 +407,30 @@ __global__ __launch_bounds__(1024) void k_tile_tiny(double* __restrict__ re, dou
     const uint64_t base = a.fixval;   // one tile: every other state bit is fixed by the pass
     const uint64_t off = base | deposit(y, a.tmask);
     double vr = re[off], vi = im[off];
+    TileGate* sg = reinterpret_cast<TileGate*>(lds + 4 * (size_t)tsz);   // after the exchange buffers
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&a.g[0]);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(sg);
+        const uint32_t nw = a.ngates * (uint32_t)(sizeof(TileGate) / 4);
+        for (uint32_t i = y; i < nw; i += tsz) dst[i] = src[i];
+    }
+    __syncthreads();
     uint32_t xb = 0;   // alternating exchange buffers: one barrier per exchange
     for (uint32_t k = 0; k < a.ngates; ++k) {
-        const TileGate& g = a.g[k];
+        TileGate g;   // the fields used here, made uniform (every lane read the same LDS words)
+        g.kind = uni(sg[k].kind);
+        g.pivot = uni(sg[k].pivot);
+        g.xc = uni(sg[k].xc);
+        g.mc = uni(sg[k].mc);
+        g.vc = uni(sg[k].vc);
+        g.yzc = uni(sg[k].yzc);
+        g.base_imag = uni(sg[k].base_imag);
+        g.base_neg = uni(sg[k].base_neg);
+        g.mo = uni64(sg[k].mo);
+        g.vo = uni64(sg[k].vo);
+        g.yzo = uni64(sg[k].yzo);
+        g.c = unid(sg[k].c);
+        g.s = unid(sg[k].s);
         if ((base & g.mo) != g.vo) continue;   // uniform: the tile is outside the gate's selection
         const uint32_t opar = (uint32_t)__popcll(base & g.yzo) & 1u;
         if (g.kind == (uint32_t)kPhaseOp) {
@@ -618,7 +650,8 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     c->pass_gates += n;
     timing_begin(c, "tile_pass", 32.0 * (double)((uint64_t)1 << t) * (double)a.ntiles);
     if (tiny)
-        hipLaunchKernelGGL(k_tile_tiny, dim3(1), dim3(1u << t), 2 * lds, c->stream, c->d_re, c->d_im, a);
+        hipLaunchKernelGGL(k_tile_tiny, dim3(1), dim3(1u << t), 2 * lds + sizeof(TileGate) * kMaxPass, c->stream,
+                           c->d_re, c->d_im, a);
     else
         hipLaunchKernelGGL(k_tile, dim3((uint32_t)g8), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a);
     timing_end(c);

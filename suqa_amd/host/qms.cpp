@@ -3,6 +3,8 @@
 // accept/reject decisions as the reference CPU path.
 #include "qms.hpp"
 
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 
 #include "system.hpp"
@@ -30,6 +32,59 @@ double curr_E_old_d;
 bool two_registers = false;
 std::vector<uint> bm_enes_old;
 uint W_mask_Eold = 0U;
+bool use_graphs = false;
+
+namespace {
+
+// Captured launch sequences of the fixed parts of a step (suqa_hip_graph_*):
+// Phi, Phi^dag and C_i Phi / Phi^dag C_i depend only on the run's parameters,
+// so each is recorded the first time it is applied and replayed after that.
+// Not with gate counters registered (a replay counts nothing).
+struct Graph {
+    bool ok = false;
+    uint32_t id = 0;
+};
+Graph g_phi, g_phi_old, g_phi_inv, g_cphi[3], g_phi_inv_c[3];
+
+void drop_graphs() {
+    g_phi = g_phi_old = g_phi_inv = Graph{};
+    for (auto& g : g_cphi) g = Graph{};
+    for (auto& g : g_phi_inv_c) g = Graph{};
+}
+
+void chk_graph(int rc) {
+    if (rc != SUQA_OK) throw std::runtime_error(std::string("ERROR: graph capture: ") + suqa_hip_last_error());
+}
+
+// measure_qbits whose outcomes the step never reads (the E re-collapse after
+// the X measurement): the same measurements and draws, and nothing waits for
+// the outcomes.  With gate counters the reference's call is kept.
+void measure_discarded(const bmReg& reg, const std::vector<double>& rdoubs) {
+    if (!suqa::gatecounters.counters.empty()) {
+        std::vector<uint> cs(reg.size());
+        suqa::measure_qbits(reg, cs, rdoubs);
+        return;
+    }
+    if (suqa_hip_measure_qbits(suqa::ctx, reg.data(), rdoubs.data(), (uint32_t)reg.size(), nullptr) != SUQA_OK)
+        throw std::runtime_error(std::string("ERROR: in suqa::measure_qbits(): ") + suqa_hip_last_error());
+}
+
+template <class F>
+void replay(Graph& g, F apply) {
+    if (!use_graphs || !suqa::gatecounters.counters.empty()) {
+        apply();
+        return;
+    }
+    if (!g.ok) {
+        chk_graph(suqa_hip_graph_begin(suqa::ctx));
+        apply();   // recorded, not run
+        chk_graph(suqa_hip_graph_end(suqa::ctx, &g.id));
+        g.ok = true;
+    }
+    chk_graph(suqa_hip_graph_launch(suqa::ctx, g.id));
+}
+
+}  // namespace
 
 void fill_bitmap() {
     bm_syst.resize(syst_qbits);
@@ -102,10 +157,15 @@ void apply_phase_estimation_inverse(const std::vector<uint>& q_syst, const std::
     suqa::apply_h(q_target);
 }
 
-void apply_Phi() { apply_phase_estimation(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation); }
-void apply_Phi_old() { apply_phase_estimation(bm_syst, bm_enes_old, t_phase_estimation, n_phase_estimation); }
+void apply_Phi() {
+    replay(g_phi, [] { apply_phase_estimation(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation); });
+}
+void apply_Phi_old() {
+    replay(g_phi_old, [] { apply_phase_estimation(bm_syst, bm_enes_old, t_phase_estimation, n_phase_estimation); });
+}
 void apply_Phi_inverse() {
-    apply_phase_estimation_inverse(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation);
+    replay(g_phi_inv,
+           [] { apply_phase_estimation_inverse(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation); });
 }
 
 uint draw_C() {
@@ -135,15 +195,31 @@ void apply_W() {
 void apply_W_inverse() { apply_W(); }
 
 void apply_U() {
-    apply_C(gCi);
-    apply_Phi();
+    if (gCi < 3) {
+        const uint ci = gCi;
+        replay(g_cphi[ci], [ci] {
+            apply_C(ci);
+            apply_phase_estimation(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation);
+        });
+    } else {
+        apply_C(gCi);
+        apply_Phi();
+    }
     apply_W();
 }
 
 void apply_U_inverse() {
     apply_W_inverse();
-    apply_Phi_inverse();
-    apply_C_inverse(gCi);
+    if (gCi < 3) {
+        const uint ci = gCi;
+        replay(g_phi_inv_c[ci], [ci] {
+            apply_phase_estimation_inverse(bm_syst, bm_enes_new, t_phase_estimation, n_phase_estimation);
+            apply_C_inverse(ci);
+        });
+    } else {
+        apply_Phi_inverse();
+        apply_C_inverse(gCi);
+    }
 }
 
 std::vector<double> extract_rands(uint n) {
@@ -182,7 +258,8 @@ int metro_step(bool take_measure) {
 
     if (c_acc == 1U) {
         double Enew_meas_d;
-        suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
+        if (take_measure) suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
+        else measure_discarded(bm_enes_new, extract_rands(ene_qbits));   // E_new is not recorded
         apply_Phi_inverse();
         if (take_measure) {
             if (counting) gctr_measure.new_record();
@@ -197,7 +274,7 @@ int metro_step(bool take_measure) {
             X_measures.push_back(measure_X(rangen));
             suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));   // the same draws, in order
             apply_Phi();
-            suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
+            measure_discarded(bm_enes_new, extract_rands(ene_qbits));   // E re-collapse
             apply_Phi_inverse();
             if (counting) {
                 gctr_measure.deactivate();
@@ -228,7 +305,7 @@ int metro_step(bool take_measure) {
                 X_measures.push_back(measure_X(rangen));
                 suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));   // the same draws, in order
                 apply_Phi();
-                suqa::measure_qbits(bm_enes_new, c_E_news, extract_rands(ene_qbits));
+                measure_discarded(bm_enes_new, extract_rands(ene_qbits));   // E re-collapse
                 apply_Phi_inverse();
                 ret = 4;
             } else {
@@ -251,8 +328,11 @@ int metro_step(bool take_measure) {
 void setup(double beta) {
     fill_bitmap();
     fill_W_utils(beta, t_PE_factor);
+    drop_graphs();   // a new engine context holds none of them
+    const char* e = std::getenv("SUQA_QMS_GRAPHS");
+    use_graphs = e && std::strcmp(e, "1") == 0;
 }
 
-void clear() {}
+void clear() { drop_graphs(); }
 
 }  // namespace qms

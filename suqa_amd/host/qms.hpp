@@ -32,6 +32,12 @@ extern double curr_E_old_d;
 extern bool two_registers;
 extern std::vector<uint> bm_enes_old;
 extern uint W_mask_Eold;
+// Extension: the fixed launch sequences of a step (Phi, Phi^dag, C_i Phi,
+// Phi^dag C_i) captured once per run and replayed (suqa_hip_graph_*); setup()
+// turns it on when SUQA_QMS_GRAPHS=1.  Off by default: each sequence is one or
+// two kernel launches, and a graph launch costs more host time than they do.
+// Never with gate counters.
+extern bool use_graphs;
 
 // Gate-count records of the Metropolis step (qms.cuh:21-27, defined in
 // qms.cu:33-39): used when a driver registers them in suqa::gatecounters.

@@ -120,7 +120,7 @@ def trotter_step_ops(terms, dt, gc_mask=0):
     return [(list(t.qs), list(t.types), -t.coef * dt, gc_mask) for t in terms]
 
 
-def qpe_trotter(eng, nsys, nreg, terms, dt, steps, width=None):
+def qpe_trotter(eng, nsys, nreg, terms, dt, steps, width=None, graphs=True):
     """Phase estimation of U = (Trotter step)^steps on an nsys-qubit system
     with an nreg-qubit register on qubits nsys..nsys+nreg-1: H on the
     register; for trg = nreg-1 .. 0, U^(2^(nreg-1-trg)) controlled on
@@ -128,7 +128,10 @@ def qpe_trotter(eng, nsys, nreg, terms, dt, steps, width=None):
     (suqa.cu:649-661).  eng is a capi.Engine (FP64, one sweep per rotation)
     or a capi.F32Engine (width-qubit fused blocks).  Returns the 2^nreg-bin
     histogram from prob_filter in FP64, bin k = register value k (LSB on the
-    first register qubit)."""
+    first register qubit).  FP64 with graphs: each register qubit's controlled
+    Trotter step is captured once (suqa_hip_graph_*) and the capture is
+    replayed 2^(nreg-1-trg) * steps times (the same launches, the same
+    results)."""
     from suqa_amd import capi
     reg = [nsys + k for k in range(nreg)]
     f32 = isinstance(eng, capi.F32Engine)
@@ -143,6 +146,12 @@ def qpe_trotter(eng, nsys, nreg, terms, dt, steps, width=None):
         if f32:
             allops = capi.pauli_ops(trotter_step_ops(terms, dt, 1 << reg[trg]) * (powr * steps))
             eng.run_paulis(allops, width)
+        elif graphs:
+            eng.graph_begin()
+            eng.run_paulis(ops)
+            g = eng.graph_end()
+            for _ in range(powr * steps):
+                eng.graph_launch(g)
         else:
             for _ in range(powr * steps):
                 eng.run_paulis(ops)

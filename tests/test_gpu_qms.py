@@ -26,11 +26,15 @@ def log(m):
     print(m, flush=True)
 
 
+@pytest.mark.parametrize("graphs", ["1", "0"], ids=["graphs", "no-graphs"])
 @pytest.mark.parametrize("fx", QMS_FIXTURES)
-def test_plain_qms_vs_golden(qms, fx):
+def test_plain_qms_vs_golden(qms, fx, graphs, monkeypatch):
+    """With the step's fixed launch sequences captured once and replayed
+    (default) and with every gate call launched."""
+    monkeypatch.setenv("SUQA_QMS_GRAPHS", graphs)
     d = load(fx)
     p = qms_params(d)
-    log(f"plain QMS {fx}")
+    log(f"plain QMS {fx} graphs {graphs}")
     out = qms.run(p["beta"], p["steps"], p["reset_each"], p["r"], p["seed"], p["ene_min"], p["ene_max"],
                   p["pe_steps"], p["thermalization"], p["max_reverse"], p["record_reverse"], final_state=True)
     assert list(out["rets"]) == d["rets"]
