@@ -10,6 +10,9 @@
 //   qms     <beta> <steps> <reset_each> <r> <seed> <ene_min> <ene_max>
 //           <pe_steps> <therm> <max_rev> <record_rev> <out>
 //   circuit <n> <layers> <seed> <out>  -- random-circuit family (see mode_circuit)
+//   qsa     <beta> <sampling> <ene qbits> <szegedy qbits> <seed> <ene_min> <ene_max>
+//           <ene_min_szegedy> <ene_max_szegedy> <pe_steps> <annealing sequences>
+//           <max annealing steps> <out>    -- the QSA loop of qsa.cu, bounded
 //   circuit_summary <n> <layers> <seed> <out>
 //                                      -- the same circuit: checksum and 64 sample
 //                                         amplitudes before and after the measurements
@@ -24,6 +27,7 @@
 #include "suqa.cuh"
 #include "system.cuh"
 #include "qms.cuh"
+#include "qsa.cuh"
 
 static FILE* g_out = nullptr;
 
@@ -200,6 +204,26 @@ static void mode_gates() {
     cs.push_back({"prob_filter", {0, 1, 2, 3, 4}, {1, 1, 0, 0, 1}, 0.0, {}});
     cs.push_back({"vnorm", {}, {}, 0.0, {}});
     cs.push_back({"vnormalize", {}, {}, 0.0, {}});
+    // gates on qubits of the active gc_mask (the QSA layout applies such gates):
+    // the reference's loops then update coinciding slots
+    for (const auto& gc : std::vector<std::vector<uint>>{{1}, {1, 3}}) {
+        for (uint q : {1U, 3U}) {
+            for (const char* op : {"x", "y", "h", "z", "s"}) cs.push_back({op, {q}, {}, 0.0, gc});
+            for (uint p = 1; p <= 3; ++p) cs.push_back({"pauli", {q}, {p}, 0.6180339, gc});
+        }
+        cs.push_back({"swap", {1, 4}, {}, 0.0, gc});
+        cs.push_back({"swap", {4, 1}, {}, 0.0, gc});
+        cs.push_back({"swap", {3, 1}, {}, 0.0, gc});
+        cs.push_back({"swap", {0, 3}, {}, 0.0, gc});
+        for (uint a = 1; a <= 3; ++a)
+            for (uint b = 1; b <= 3; ++b) {
+                cs.push_back({"pauli", {1, 4}, {a, b}, 0.4321, gc});
+                cs.push_back({"pauli", {4, 1}, {a, b}, -1.111, gc});
+                cs.push_back({"pauli", {3, 1}, {a, b}, 0.777, gc});
+            }
+        cs.push_back({"cx", {1, 3}, {}, 0.0, gc});
+        cs.push_back({"cu1", {3, 1}, {}, 1.234, gc});
+    }
     for (const auto& c : cs) run_case(c, n);
     fprintf(g_out, "]}\n");
     suqa::clear();
@@ -412,6 +436,102 @@ static void mode_circuit_summary(char** argv) {
     suqa::clear();
 }
 
+// ------------------------------------------------------------------ qsa
+// The QSA driver loop of qsa.cu:115-262 on the reference's qsa:: functions,
+// bounded: it stops once `max_ann` annealing steps have run (the stock
+// driver's samples can take ~10^5 restarts).  Records, per annealing step,
+// the sample, restart and step indices, the move drawn, the Szegedy
+// register's readout and the pcg32 state after its measurement; per finished
+// sample the energy, the X value and the restart count; the final state.
+static void mode_qsa(char** argv) {
+    const double beta = atof(argv[2]);
+    const uint sampling = (uint)atoi(argv[3]);
+    const uint ene_q = (uint)atoi(argv[4]), sz_q = (uint)atoi(argv[5]);
+    const unsigned long long seed = strtoull(argv[6], nullptr, 10);
+    const double emin = atof(argv[7]), emax = atof(argv[8]), emin_sz = atof(argv[9]), emax_sz = atof(argv[10]);
+    const int pe = atoi(argv[11]);
+    const uint ann_seq = (uint)atoi(argv[12]), max_ann = (uint)atoi(argv[13]);
+    // qsa.cu:115-160
+    qsa::beta = beta;
+    qsa::ene_qbits = ene_q;
+    qsa::szegedy_qbits = sz_q;
+    qsa::n_phase_estimation = pe;
+    qsa::iseed = seed;
+    qsa::rangen.set_seed(qsa::iseed);
+    qsa::iseed = qsa::rangen.get_seed();
+    qsa::syst_qbits = (uint)syst_qbits;
+    qsa::nqubits = qsa::syst_qbits + qsa::ene_qbits + qsa::szegedy_qbits + 1;
+    qsa::Dim = (1U << qsa::nqubits);
+    qsa::ene_levels = (1U << qsa::ene_qbits);
+    double szegedy_levels = (1U << qsa::szegedy_qbits);
+    qsa::t_PE_shift = emin;
+    qsa::t_PE_factor = (qsa::ene_levels - 1) / (double)(qsa::ene_levels * (emax - emin));
+    qsa::t_phase_estimation = qsa::t_PE_factor * 8. * atan(1.0);
+    qsa::t_PE_shift_szegedy = emin_sz;
+    qsa::t_PE_factor_szegedy = (szegedy_levels - 1) / (double)((szegedy_levels) * (emax_sz - emin_sz));
+    qsa::t_phase_estimation_szegedy = qsa::t_PE_factor_szegedy * 8. * atan(1.0);
+    suqa::setup(qsa::nqubits);
+    qsa::setup();
+    fprintf(g_out, "{\"beta\":\"%a\",\"sampling\":%u,\"ene_qbits\":%u,\"szegedy_qbits\":%u,\"seed\":%llu,"
+                   "\"ene_min\":\"%a\",\"ene_max\":\"%a\",\"ene_min_szegedy\":\"%a\",\"ene_max_szegedy\":\"%a\","
+                   "\"pe_steps\":%d,\"annealing_sequences\":%u,\"max_annealing_steps\":%u,\"steps\":[",
+            beta, sampling, ene_q, sz_q, seed, emin, emax, emin_sz, emax_sz, pe, ann_seq, max_ann);
+    int rejection1 = 0;
+    uint nann = 0;
+    bool first = true;
+    double beta_i = qsa::beta / ((double)ann_seq);
+    std::vector<std::string> samples;
+    for (uint iiii = 0; iiii < sampling && nann < max_ann;) {
+        std::vector<uint> c_Ene_test(qsa::szegedy_qbits, 0);
+        std::vector<uint> c_Ene(qsa::szegedy_qbits, 0);
+        int control = 0;
+        uint attempt = 0;
+        while (control == 0 && nann < max_ann) {
+            int control2 = 0;
+            uint move = qsa::draw_C();
+            qsa::reset_non_syst_qbits();
+            qsa_init_state();
+            for (uint s = 0U; s < ann_seq; s++) {
+                move = qsa::draw_C();
+                qsa::apply_PE_szegedy(qsa::bm_states, qsa::bm_szegedy, (double)(s + 1) * beta_i, move);
+                suqa::measure_qbits(qsa::bm_szegedy, c_Ene_test, qsa::extract_rands(qsa::szegedy_qbits));
+                ++nann;
+                fprintf(g_out, "%s[%u,%u,%u,%u,%u,\"%llu\"]", first ? "" : ",", iiii, attempt, s, move,
+                        qsa::creg_to_uint(c_Ene_test), (unsigned long long)qsa::rangen.get_state().state);
+                first = false;
+                if (qsa::creg_to_uint(c_Ene_test) != 0U) {
+                    control2 = 1;
+                    rejection1++;
+                    break;
+                }
+                if (nann >= max_ann) break;
+            }
+            if (control2 == 0 && nann <= max_ann) control = 1;
+            ++attempt;
+        }
+        if (control == 0) break;   // bounded: stopped inside a sample
+        qsa::apply_phase_estimation_measure(qsa::bm_states, qsa::bm_szegedy, qsa::t_phase_estimation_szegedy,
+                                            qsa::n_phase_estimation);
+        suqa::measure_qbits(qsa::bm_szegedy, c_Ene, qsa::extract_rands(qsa::szegedy_qbits));
+        double Ene_measure = qsa::t_PE_shift_szegedy +
+                             qsa::creg_to_uint(c_Ene) / (double)(qsa::t_PE_factor_szegedy * szegedy_levels);
+        double X = measure_X(qsa::rangen);
+        char buf[256];
+        snprintf(buf, sizeof buf, "[\"%a\",\"%a\",%d,\"%llu\"]", Ene_measure, X, rejection1,
+                 (unsigned long long)qsa::rangen.get_state().state);
+        samples.push_back(buf);
+        ++iiii;
+    }
+    fprintf(g_out, "],\"samples\":[");
+    for (size_t k = 0; k < samples.size(); ++k) fprintf(g_out, "%s%s", k ? "," : "", samples[k].c_str());
+    fprintf(g_out, "],\"annealing_steps\":%u,\"final_rng\":\"%llu\",", nann,
+            (unsigned long long)qsa::rangen.get_state().state);
+    dump_state("final_state");
+    fprintf(g_out, "}\n");
+    qsa::clear();
+    suqa::clear();
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
         fprintf(stderr, "usage: %s pcg|gates|qms|circuit|circuit_summary ... <out.json>\n", argv[0]);
@@ -425,6 +545,7 @@ int main(int argc, char** argv) {
     else if (mode == "qms" && argc == 14) mode_qms(argc, argv);
     else if (mode == "circuit" && argc == 6) mode_circuit(argv);
     else if (mode == "circuit_summary" && argc == 6) mode_circuit_summary(argv);
+    else if (mode == "qsa" && argc == 15) mode_qsa(argv);
     else { fprintf(stderr, "bad arguments\n"); return 2; }
     fclose(g_out);
     return 0;

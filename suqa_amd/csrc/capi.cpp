@@ -107,13 +107,13 @@ int pauli_masks(suqa_hip_ctx* c, uint32_t xm, uint32_t ym, uint32_t zm, double p
     return SUQA_OK;
 }
 
-int check_qubits(const suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_t gc) {
+int check_qubits(const suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_t gc, bool gc_overlap_ok = false) {
     uint32_t seen = 0;
     for (uint32_t k = 0; k < n; ++k) {
         if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
         const uint32_t b = 1u << qs[k];
         if (seen & b) return fail(SUQA_ERR_ARG, "repeated qubit in a Pauli string");
-        if (gc & b) return fail(SUQA_ERR_ARG, "Pauli rotation on a qubit of the active gc_mask");
+        if ((gc & b) && !gc_overlap_ok) return fail(SUQA_ERR_ARG, "Pauli rotation on a qubit of the active gc_mask");
         seen |= b;
     }
     return SUQA_OK;
@@ -348,9 +348,13 @@ int suqa_hip_apply_y(suqa_hip_ctx* c, uint32_t q, uint32_t gc) { return pair_1q(
 int suqa_hip_apply_h(suqa_hip_ctx* c, uint32_t q, uint32_t gc) {
     SQE_CTX(c);
     if (q >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
-    if (gc & (1u << q)) return fail(SUQA_ERR_ARG, "apply_h on a qubit of the active gc_mask");
     const uint64_t T = 1ull << q;
     Sel sel;
+    if (gc & (1u << q)) {   // the reference's loop on coinciding slots (suqa_cpu.hpp:186-198)
+        if (sqe::make_sel(c->nq, (uint64_t)gc | T, gc, &sel))
+            SQE_HIP(sqe::launch_overlap(c, sqe::kOvH, sel, T, 0, 0, 0, 0.0, 0.0));
+        return SUQA_OK;
+    }
     if (!sqe::make_sel(c->nq, (uint64_t)gc | T, gc, &sel)) return SUQA_OK;
     PauliCoef pc{};
     SQE_HIP(sqe::launch_pair(c, sqe::OP_H, sel, T, pc, "h_pair", 64.0 * (double)sel.count));
@@ -414,9 +418,13 @@ int suqa_hip_apply_swap(suqa_hip_ctx* c, uint32_t q1, uint32_t q2, uint32_t gc) 
     if (q1 >= c->nq || q2 >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
     if (q1 == q2) return SUQA_OK;                    // the reference swaps each amplitude with itself
     const uint32_t m1 = 1u << q1, m2 = 1u << q2;
-    if (gc & (m1 | m2)) return fail(SUQA_ERR_ARG, "apply_swap on a qubit of the active gc_mask");
     // suqa_cpu.hpp:303-309: (i & (gc|m1|m2)) == gc, swap i|m1 <-> i|m2
     Sel sel;
+    if (gc & (m1 | m2)) {   // on coinciding slots
+        if (sqe::make_sel(c->nq, (uint64_t)(gc | m1 | m2), gc, &sel))
+            SQE_HIP(sqe::launch_overlap(c, sqe::kOvSwap, sel, m1, m2, 0, 0, 0.0, 0.0));
+        return SUQA_OK;
+    }
     if (!sqe::make_sel(c->nq, (uint64_t)(gc | m1 | m2), (uint64_t)(gc | m1), &sel)) return SUQA_OK;
     PauliCoef pc{};
     SQE_HIP(sqe::launch_pair(c, sqe::OP_SWAP, sel, (uint64_t)(m1 | m2), pc, "swap_pair",
@@ -444,7 +452,7 @@ int suqa_hip_apply_pauli_TP_rotation(suqa_hip_ctx* c, const uint32_t* qs, const 
     if (n == 0 || n > 3)
         return fail(SUQA_ERR_UNSUPPORTED, "ERROR: unimplemented pauli TP rotation in the selected configuration");
     if (!qs || !types) return fail(SUQA_ERR_ARG, "null argument");
-    int rc = check_qubits(c, qs, n, gc);
+    int rc = check_qubits(c, qs, n, gc, n <= 2);
     if (rc) return rc;
     uint32_t t[3], q[3];
     for (uint32_t k = 0; k < n; ++k) {
@@ -461,13 +469,33 @@ int suqa_hip_apply_pauli_TP_rotation(suqa_hip_ctx* c, const uint32_t* qs, const 
         else if (type == SUQA_PAULI_Y) ym |= 1u << qq;
         else if (type == SUQA_PAULI_Z) zm |= 1u << qq;
     };
+    uint32_t over = 0;   // gate qubits inside the active gc_mask
+    for (uint32_t k = 0; k < n; ++k) over |= gc & (1u << q[k]);
     if (n == 1) {
         if (t[0] < 1 || t[0] > 3) return SUQA_OK;    // suqa_cpu.hpp:360-370: no matching case
+        if (over) {   // the reference's loop on coinciding slots
+            double sph, cph;
+            sincos(phase, &sph, &cph);
+            const uint64_t m1 = 1ull << q[0];
+            Sel sel;
+            if (sqe::make_sel(c->nq, (uint64_t)gc | m1, gc, &sel))
+                SQE_HIP(sqe::launch_overlap(c, sqe::kOvPauli1, sel, m1, 0, t[0], 0, cph, sph));
+            return SUQA_OK;
+        }
         add(t[0], q[0]);
     } else if (n == 2) {
         if (t[0] > t[1]) sw(0, 1);                   // suqa.cu:506-509
         const bool known = t[0] >= 1 && t[1] <= 3 && t[0] <= t[1];
         if (!known) return SUQA_OK;                  // suqa_cpu.hpp:422-440: no matching case
+        if (over) {   // the reference's loop on coinciding slots
+            double sph, cph;
+            sincos(phase, &sph, &cph);
+            const uint64_t m1 = 1ull << q[0], m2 = 1ull << q[1];
+            Sel sel;
+            if (sqe::make_sel(c->nq, (uint64_t)gc | m1 | m2, gc, &sel))
+                SQE_HIP(sqe::launch_overlap(c, sqe::kOvPauli2, sel, m1, m2, t[0], t[1], cph, sph));
+            return SUQA_OK;
+        }
         if (t[0] == SUQA_PAULI_X && t[1] == SUQA_PAULI_Y) {
             // suqa_cpu.hpp:431-433: the reference's X(q1)Y(q2) block rotates
             // about Y on q1 and X on q2.

@@ -156,6 +156,12 @@ hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, cons
 hipError_t launch_norm_tree(const double* re, const double* im, uint32_t nq, const Sel& sel, uint64_t split_bit,
                             double* partials, double* result, hipStream_t st, suqa_hip_ctx* timed, const char* tag,
                             double alg_bytes);
+// A gate whose qubits lie in the active gc_mask, with the reference's dense
+// loop bodies on coinciding slots (kernels.hip, k_overlap): selection
+// (i0 & (gc | gate bits)) == gc, slots i0 | {0, m1, m2, m1 | m2}.
+constexpr int kOvH = 0, kOvSwap = 1, kOvPauli1 = 2, kOvPauli2 = 3;
+hipError_t launch_overlap(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t m1, uint64_t m2, uint32_t p1, uint32_t p2,
+                          double cs, double sn);
 hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale);
 // d_result[0..2] = sum re, sum im, sum i |psi_i|^2 over the whole state.
 hipError_t launch_checksum(suqa_hip_ctx* c);

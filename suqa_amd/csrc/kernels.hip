@@ -291,6 +291,87 @@ __global__ __launch_bounds__(kBlock) void k_norm(const double* __restrict__ re, 
     }
 }
 
+// Gates whose qubits lie in the active gc_mask.  The reference's dense loops
+// (suqa_cpu.hpp:150-206, 268-317, 338-463) select i0 with
+// (i0 & (gc | gate bits)) == gc and update the slots i0 | (subsets of the gate
+// bits).  A gate bit inside gc is already set in i0, so slots coincide, and
+// the loop body's reads and writes, done in order on the same addresses, make
+// another operator than the gate's (H zeroes the selection, a swap becomes a
+// controlled X, an XX rotation turns twice).  One thread per selected i0 runs
+// the body of the reference's case on the slots k0 = i0, k1 = i0 | m1,
+// k2 = i0 | m2, k3 = k1 | k2 through memory, in the reference's order.
+__device__ __forceinline__ void rot2m(double* a, double* b, double c, double s) {   // suqa_cpu.hpp:49-54
+    const double cpy = *a;
+    *a = cpy * c - (*b) * s;
+    *b = (*b) * c + cpy * s;
+}
+__device__ __forceinline__ void rot4m(double* a, double* b, double* c, double* d, double ct, double st) {
+    rot2m(a, b, ct, st);
+    rot2m(c, d, ct, st);
+}
+__device__ __forceinline__ void swapm(double* a, double* b) {
+    const double cpy = *a;
+    *a = *b;
+    *b = cpy;
+}
+
+struct OverlapArgs {
+    Fixed fb;
+    uint64_t fval, nitems;
+    uint64_t m1, m2;
+    int op;              // kOvH, kOvSwap, kOvPauli1, kOvPauli2
+    uint32_t p1, p2;     // Pauli types (sorted as the reference sorts them)
+    double c, s;
+};
+
+__global__ __launch_bounds__(kBlock) void k_overlap(double* re, double* im, const OverlapArgs a) {
+    const uint64_t it = item_id();
+    if (it >= a.nitems) return;
+    const uint64_t k0 = expand(it, a.fb, a.fval);
+    const uint64_t k1 = k0 | a.m1, k2 = k0 | a.m2, k3 = k1 | k2;
+    const double ct = a.c, st = a.s;
+    switch (a.op) {
+    case kOvH: {   // suqa_cpu.hpp:186-198
+        const double a0r = re[k0], a0i = im[k0], a1r = re[k1], a1i = im[k1];
+        re[k0] = kTwoSqInv * (a0r + a1r);
+        im[k0] = kTwoSqInv * (a0i + a1i);
+        re[k1] = kTwoSqInv * (a0r - a1r);
+        im[k1] = kTwoSqInv * (a0i - a1i);
+        break;
+    }
+    case kOvSwap:   // suqa_cpu.hpp:303-309
+        swapm(&re[k1], &re[k2]);
+        swapm(&im[k1], &im[k2]);
+        break;
+    case kOvPauli1:   // suqa_cpu.hpp:360-370
+        if (a.p1 == 1) rot4m(&re[k0], &im[k1], &re[k1], &im[k0], ct, st);
+        else if (a.p1 == 2) rot4m(&re[k0], &re[k1], &im[k0], &im[k1], ct, -st);
+        else if (a.p1 == 3) rot4m(&re[k0], &im[k0], &im[k1], &re[k1], ct, st);
+        break;
+    default:   // kOvPauli2, suqa_cpu.hpp:422-440
+        if (a.p1 == 1 && a.p2 == 1) {
+            rot4m(&re[k0], &im[k3], &re[k3], &im[k0], ct, st);
+            rot4m(&re[k1], &im[k2], &re[k2], &im[k1], ct, st);
+        } else if (a.p1 == 2 && a.p2 == 2) {
+            rot4m(&re[k0], &im[k3], &re[k3], &im[k0], ct, -st);
+            rot4m(&re[k1], &im[k2], &re[k2], &im[k1], ct, st);
+        } else if (a.p1 == 3 && a.p2 == 3) {
+            rot4m(&re[k0], &im[k0], &re[k3], &im[k3], ct, st);
+            rot4m(&re[k1], &im[k1], &re[k2], &im[k2], ct, -st);
+        } else if (a.p1 == 1 && a.p2 == 2) {
+            rot4m(&re[k0], &re[k3], &im[k0], &im[k3], ct, -st);
+            rot4m(&re[k2], &re[k1], &im[k2], &im[k1], ct, -st);
+        } else if (a.p1 == 1 && a.p2 == 3) {
+            rot4m(&re[k0], &im[k1], &re[k1], &im[k0], ct, st);
+            rot4m(&re[k2], &im[k3], &re[k3], &im[k2], ct, -st);
+        } else if (a.p1 == 2 && a.p2 == 3) {
+            rot4m(&re[k0], &re[k1], &im[k0], &im[k1], ct, -st);
+            rot4m(&re[k2], &re[k3], &im[k2], &im[k3], ct, st);
+        }
+        break;
+    }
+}
+
 // Checksum sums over the whole state, one amplitude per thread, into a
 // (sum, sum) pair per block: mode 0 -> (sum re, sum im), mode 1 ->
 // (sum i |psi_i|^2, 0).
@@ -704,6 +785,26 @@ hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, cons
     if (hipError_t e = flush(c)) return e;
     return launch_norm_tree(c->d_re, c->d_im, c->nq, sel, split_bit, c->d_partials, c->d_result, c->stream, c, tag,
                             alg_bytes);
+}
+
+hipError_t launch_overlap(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t m1, uint64_t m2, uint32_t p1, uint32_t p2,
+                          double cs, double sn) {
+    if (hipError_t e = flush(c)) return e;
+    OverlapArgs a;
+    a.fb = sel.fb;
+    a.fval = sel.fval;
+    a.nitems = sel.count;
+    a.m1 = m1;
+    a.m2 = m2;
+    a.op = op;
+    a.p1 = p1;
+    a.p2 = p2;
+    a.c = cs;
+    a.s = sn;
+    timing_begin(c, "overlap", 64.0 * (double)sel.count);
+    hipLaunchKernelGGL(k_overlap, dim3(grid_for(a.nitems)), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a);
+    timing_end(c);
+    return hipGetLastError();
 }
 
 hipError_t launch_checksum(suqa_hip_ctx* c) {

@@ -8,8 +8,9 @@
 //    reference) keeps working on device pointers.
 //  - Errors: every failing engine call throws std::runtime_error, the
 //    reference's exception type (suqa.cu:421, 478, 555, 563, 579).
-//  - Gates whose qubits overlap the active gc_mask, where the reference
-//    computes a degenerate result (H, swap, Pauli rotations), throw instead.
+//  - Gates whose qubits lie in the active gc_mask give the reference's
+//    results (its loops on coinciding slots), as the QSA layout needs;
+//    three-qubit Pauli rotations there throw.
 #pragma once
 #include <cstdint>
 #include <cstdio>

@@ -56,3 +56,37 @@ void apply_C(const uint& Ci) {
 void apply_C_inverse(const uint& Ci) { apply_C(Ci); }
 
 std::vector<double> get_C_weigthsums() { return C_weigthsums; }
+
+// ---- QSA hooks
+void qsa_init_state() {   // spins in |+++>, copied onto the tilde spins by CX
+    suqa::init_state();
+    for (uint k = 0; k < 3; ++k) suqa::apply_h(bm_spin[k]);
+    for (uint k = 0; k < 3; ++k) suqa::apply_cx(bm_spin[k], bm_spin_tilde[k]);
+}
+
+void evolution_szegedy(const double& t, const int& n) {   // tilde pairs by -t, spin pairs by +t, interleaved
+    (void)n;
+    for (uint i = 0; i < 3; i++) {
+        suqa::apply_pauli_TP_rotation({bm_spin_tilde[i % 3], bm_spin_tilde[(1 + i) % 3]}, {PAULI_X, PAULI_X}, -t);
+        suqa::apply_pauli_TP_rotation({bm_spin[i % 3], bm_spin[(1 + i) % 3]}, {PAULI_X, PAULI_X}, t);
+    }
+}
+
+void evolution_measure(const double& t, const int& n) {
+    (void)n;
+    for (uint i = 0; i < 3; i++)
+        suqa::apply_pauli_TP_rotation({bm_spin[i % 3], bm_spin[(1 + i) % 3]}, {PAULI_X, PAULI_X}, -t);
+}
+
+void evolution_tracing(const double& t, const int& n) {
+    (void)n;
+    for (uint i = 0; i < 3; i++)
+        suqa::apply_pauli_TP_rotation({bm_spin_tilde[i % 3], bm_spin_tilde[(1 + i) % 3]}, {PAULI_X, PAULI_X}, -t);
+}
+
+void qsa_apply_C(const uint& Ci) {
+    if (Ci > 2) throw std::runtime_error("ERROR: wrong move selection");
+    suqa::apply_h(bm_spin_tilde[Ci]);
+}
+
+void qsa_apply_C_inverse(const uint& Ci) { qsa_apply_C(Ci); }

@@ -16,3 +16,14 @@ double measure_X(pcg& rgen);                     // system.cu:99-119
 void apply_C(const uint& Ci);                    // system.cu:125-129
 void apply_C_inverse(const uint& Ci);            // system.cu:131-133
 std::vector<double> get_C_weigthsums();          // system.cu:151
+
+// QSA hooks (system.cuh:32-41, system.cu:29-66): the second copy of the spins
+// and the Szegedy-walk evolutions.  The reference's QSA keeps syst_qbits = 3,
+// so bm_spin_tilde shares qubits 3..5 with its energy and Szegedy registers.
+const bmReg bm_spin_tilde = {3, 4, 5};
+void qsa_init_state();                                  // system.cu:29-37
+void evolution_szegedy(const double& t, const int& n);  // system.cu:39-52
+void evolution_measure(const double& t, const int& n);  // system.cu:54-59
+void evolution_tracing(const double& t, const int& n);  // system.cu:60-65
+void qsa_apply_C(const uint& Ci);                       // system.cu:136-143
+void qsa_apply_C_inverse(const uint& Ci);               // system.cu:145-150

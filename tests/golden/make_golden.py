@@ -20,6 +20,11 @@ Fixtures (all JSON; doubles are C99 hex floats, exact):
                         built with -DGATECOUNT (GATECOUNT_CASES below)
   circuit_*.json        the random-circuit family (gate list, state before
                         the measurements, outcomes, final state)
+  qsa_*.json            QSA traces: qsa.cu's loop on the reference qsa:: functions,
+                        bounded by a number of annealing steps (per step the move,
+                        the Szegedy readout and the pcg32 state; per sample E, X,
+                        restarts; the final state)
+  qsa_out_*.txt         the stock reference `qsa` output file (E X restarts lines)
   circuit_*_summary.json  the same family at sizes too large to store
                         (n = 24: the 30-qubit workload's circuit family):
                         checksum + 64 sample amplitudes before and after
@@ -47,6 +52,17 @@ CIRCUITS = [("circuit_n10", 10, 200, 7), ("circuit_n6", 6, 20, 3)]
 # sums of re, im and i |psi_i|^2 / N) and 64 sample amplitudes, before and
 # after the measurements (the n = 24 one takes ~3 minutes on one core)
 SUMMARY_CIRCUITS = [("circuit_n10_summary", 10, 200, 7), ("circuit_n24_summary", 24, 200, 7)]
+
+# (name, beta, sampling, ene qbits, szegedy qbits, seed, ene_min, ene_max, ene_min_sz, ene_max_sz,
+#  PE steps, annealing sequences, bound on annealing steps): QSA traces (qsa.cu's loop, bounded)
+QSA_CASES = [
+    ("qsa_e2s2", 0.25, 3, 2, 2, 7, -4.0, 8.0, -1.0, 3.0, 1, 3, 80),
+    ("qsa_e1s2", 1.0, 4, 1, 2, 11, -4.0, 8.0, -1.0, 3.0, 1, 2, 100),
+    ("qsa_e2s1", 0.5, 3, 2, 1, 5, -4.0, 8.0, -1.0, 3.0, 2, 2, 80),
+    ("qsa_e2s3_default_seq", 0.25, 1, 2, 3, 3, -4.0, 8.0, -1.0, 3.0, 1, 100, 25),
+]
+# the stock qsa driver's output file: (name, arguments after the output file)
+QSA_OUT_CASES = [("qsa_out_e2s2", ["0.25", "3", "2", "2"], ["--seed", "7", "--annealing-sequences", "3"])]
 
 # (name, qms command-line arguments after the output file)
 GATECOUNT_CASES = [
@@ -91,6 +107,14 @@ def main():
     for (name, n, layers, seed) in SUMMARY_CIRCUITS:
         run([h, "circuit_summary", str(n), str(layers), str(seed), out(name + ".json")],
             stdout=subprocess.DEVNULL)
+    for (name, beta, sampling, e, sz, seed, emin, emax, emin_s, emax_s, pe, ann, bound) in QSA_CASES:
+        run([h, "qsa", repr(beta), str(sampling), str(e), str(sz), str(seed), repr(emin), repr(emax), repr(emin_s),
+             repr(emax_s), str(pe), str(ann), str(bound), out(name + ".json")], stdout=subprocess.DEVNULL)
+    for (name, pos, opts) in QSA_OUT_CASES:
+        tmp = out(name + ".txt")
+        if os.path.exists(tmp):
+            os.remove(tmp)
+        run([os.path.join(ROOT, "oracle", "_ref", "qsa")] + pos + [tmp] + opts, stdout=subprocess.DEVNULL)
     # stock drivers: output files, byte for byte
     tmp = out("qms_c1_out.txt")
     if os.path.exists(tmp):

@@ -44,6 +44,15 @@ def test_qms_header_matches_exports(built):
     assert set(qms.SIGNATURES) == decl
 
 
+def test_qsa_header_matches_exports(built):
+    lib = os.path.join(ROOT, "suqa_amd", "lib", "libsuqa.so")
+    decl = declared("suqa_qsa.h", "suqa_qsa_")
+    assert decl == {"suqa_qsa_run"}
+    assert decl == exported(lib, "suqa_qsa_")
+    from suqa_amd import qsa
+    assert set(qsa.SIGNATURES) == decl
+
+
 def test_binding_covers_header(built):
     from suqa_amd import capi
     assert set(capi.SIGNATURES) == declared("suqa_hip.h", "suqa_hip_")
