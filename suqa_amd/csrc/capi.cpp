@@ -194,6 +194,83 @@ void timing_end(suqa_hip_ctx* c) {
 
 }  // namespace sqe
 
+namespace {
+
+// Symplectic form of a controlled Pauli rotation: flipped qubits (X|Y),
+// phase qubits (Y|Z), controls.
+struct PauliSym {
+    uint32_t x, z, g;
+};
+
+PauliSym pauli_sym(const suqa_hip_pauli_op& o) {
+    PauliSym p{0u, 0u, o.gc_mask};
+    for (uint32_t k = 0; k < o.w && k < 8; ++k) {
+        const uint32_t b = o.qs[k] < 32 ? 1u << o.qs[k] : 0u;
+        if (o.types[k] == SUQA_PAULI_X || o.types[k] == SUQA_PAULI_Y) p.x |= b;
+        if (o.types[k] == SUQA_PAULI_Y || o.types[k] == SUQA_PAULI_Z) p.z |= b;
+    }
+    return p;
+}
+
+// Two controlled rotations exp(i a P) and exp(i b Q) commute when neither
+// flips a control of the other (their projectors then commute with the other
+// rotation) and P, Q commute (an even number of positions where they differ
+// and neither is I).
+bool commute(const PauliSym& a, const PauliSym& b) {
+    if ((a.g & b.x) || (b.g & a.x)) return false;
+    return (__builtin_popcount((a.x & b.z) ^ (a.z & b.x)) & 1) == 0;
+}
+
+// The order in which run_paulis queues a list with gate passes on: passes
+// packed greedily, an op moving ahead only past ops it commutes with, so the
+// product is the list's.  A pass is a run of at most `max_pass` ops whose
+// flipped qubits fit, with qubits 0..low-1, in `t` qubits (the tile the
+// flush forms, tiled.hip), so the flush cuts the reordered queue into exactly
+// these passes.  Lookahead over `window` ops.
+std::vector<uint32_t> pass_order(const suqa_hip_pauli_op* ops, uint32_t n, uint32_t t, uint32_t low, uint32_t max_pass,
+                                 uint32_t window) {
+    std::vector<PauliSym> sym(n);
+    for (uint32_t k = 0; k < n; ++k) sym[k] = pauli_sym(ops[k]);
+    std::vector<uint32_t> rest(n), order;
+    for (uint32_t k = 0; k < n; ++k) rest[k] = k;
+    order.reserve(n);
+    const uint32_t lowmask = (uint32_t)((1ull << std::min(low, t)) - 1ull);
+    while (!rest.empty()) {
+        uint32_t T = lowmask, count = 0;
+        bool took = true;
+        while (took && count < max_pass && !rest.empty()) {
+            took = false;
+            const size_t lim = std::min<size_t>(rest.size(), window);
+            for (size_t j = 0; j < lim; ++j) {
+                const PauliSym& s = sym[rest[j]];
+                if ((uint32_t)__builtin_popcount(T | s.x) > t) continue;
+                bool ok = true;   // commutes with every op it would move ahead of
+                for (size_t i = 0; i < j && ok; ++i) ok = commute(sym[rest[i]], s);
+                if (!ok) continue;
+                T |= s.x;
+                ++count;
+                order.push_back(rest[j]);
+                rest.erase(rest.begin() + (ptrdiff_t)j);
+                took = true;
+                break;
+            }
+        }
+        if (count == 0) {   // the first op alone (its flips exceed the tile)
+            order.push_back(rest.front());
+            rest.erase(rest.begin());
+        }
+    }
+    return order;
+}
+
+}  // namespace
+
+namespace sqe {
+std::vector<uint32_t> pauli_pass_order(const suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t n) {
+    return pass_order(ops, n, std::min(c->tile_bits, c->nq), kPassLowFixed, kPassMaxGates, 128);
+}
+}  // namespace sqe
+
 // ------------------------------------------------------------ entry points
 extern "C" {
 
@@ -247,6 +324,10 @@ int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out) {
     c->num_cu = ncu > 0 ? ncu : 256;
     SETUP_HIP(hipStreamCreate(&c->stream));
     c->tile_bits = default_tile_bits();
+    {
+        const char* e = std::getenv("SUQA_HIP_PAULI_ORDER");
+        c->pauli_reorder = !(e && std::strcmp(e, "0") == 0);
+    }
     void* p = nullptr;
     SETUP_HIP(hipMalloc(&p, sizeof(double) * 4));
     c->d_result = static_cast<double*>(p);
@@ -864,14 +945,19 @@ int suqa_hip_run_gates(suqa_hip_ctx* c, const suqa_hip_gate* gates, uint32_t nga
 int suqa_hip_run_paulis(suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t nops) {
     SQE_CTX(c);
     if (nops && !ops) return fail(SUQA_ERR_ARG, "null op list");
-    for (uint32_t k = 0; k < nops; ++k) {
+    for (uint32_t k = 0; k < nops; ++k)
         if (ops[k].w > 8) return fail(SUQA_ERR_ARG, "Pauli op weight above 8");
-        int rc = suqa_hip_apply_pauli_rotation_general(c, ops[k].qs, ops[k].types, ops[k].w, ops[k].phase,
-                                                       ops[k].gc_mask);
+    std::vector<uint32_t> order;
+    if (c->tile_bits > 0 && c->pauli_reorder) order = sqe::pauli_pass_order(c, ops, nops);
+    for (uint32_t k = 0; k < nops; ++k) {
+        const suqa_hip_pauli_op& o = ops[order.empty() ? k : order[k]];
+        int rc = suqa_hip_apply_pauli_rotation_general(c, o.qs, o.types, o.w, o.phase, o.gc_mask);
         if (rc) return rc;
     }
     return SUQA_OK;
 }
+
+
 
 int suqa_hip_timing_enable(suqa_hip_ctx* c, int on) {
     SQE_CTX(c);

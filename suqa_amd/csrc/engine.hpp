@@ -100,6 +100,7 @@ struct suqa_hip_ctx {
     // applied in passes of up to kMaxPass gates over tiles of 2^tile_bits
     // amplitudes held in LDS (tiled.hip); every other operation flushes first
     uint32_t tile_bits = 0;
+    bool pauli_reorder = true;   // run_paulis packs passes with commuting ops moved ahead
     std::vector<sqe::GateDesc> queue;
     uint64_t passes = 0, pass_gates = 0, single_sweeps = 0;
     // captured launch sequences (suqa_hip_graph_*): replayable graphs on the
@@ -191,6 +192,10 @@ hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double s
 // Applies the queued gates in tile passes (tiled.hip); a no-op when empty.
 hipError_t flush(suqa_hip_ctx* c);
 constexpr uint32_t kMinTileBits = 1, kMaxTileBits = 12;
+constexpr uint32_t kPassLowFixed = 4;    // qubits 0..3 are in every pass tile (one 128-byte line)
+constexpr uint32_t kPassMaxGates = 24;   // gates per pass
+// run_paulis' order of an op list with gate passes on (capi.cpp)
+std::vector<uint32_t> pauli_pass_order(const suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t n);
 constexpr size_t kAutoFlush = 256;   // queued gates that trigger a flush (keeps the GPU busy)
 
 // the message suqa_hip_last_error() returns on this thread (capi.cpp)

@@ -42,8 +42,8 @@ namespace sqe {
 
 namespace {
 
-constexpr uint32_t kMaxPass = 24;
-constexpr uint32_t kLowFixed = 4;                 // qubits 0..3: 16 amplitudes = one 128-byte line
+constexpr uint32_t kMaxPass = kPassMaxGates;
+constexpr uint32_t kLowFixed = kPassLowFixed;     // qubits 0..3: 16 amplitudes = one 128-byte line
 constexpr uint32_t kSub = 3, kLoc = 1u << kSub;   // register step: 3 bits, 8 amplitudes per thread
 constexpr uint32_t kTinyBits = 10;                // one-tile states up to 2^10: k_tile_tiny
 constexpr uint32_t kMaxChunks = (1u << kMaxTileBits) / 2 / kBlock;   // 16-byte chunks per array per thread
