@@ -389,7 +389,8 @@ def fused_leg(capi, wl, eng, gates, n, layers, no_cpu):
         roof = kernel_roofline(rep, fam, mfma_tflops=157.3)
         log(f"fused: {fam} {roof}")
         out = {
-            "width": best, "mode": ["blocks", "passes", "sparse"][f32.passes],
+            "width": None if f32.passes == 3 else best, "mode": ["blocks", "passes", "sparse", "gates"][f32.passes],
+            "tile_bits": f32.tile_bits if f32.passes == 3 else None,
             "ms_per_gate_by_mode_width": {k: (None if v < 0 else round(v, 4)) for k, v in tms.items()},
             "layers": layers, "gates": layers * n, "blocks": nb, "gates_per_block": round(layers * n / nb, 2),
             "seconds": round(dt, 4), "layers_per_s": round(layers / dt, 3),
@@ -446,21 +447,24 @@ def trotter_leg(capi, wl, no_fused, full_qpe=True, no_cpu=False):
         if not no_fused:
             f = capi.F32Engine(n)
             # mode and block width: one controlled Trotter step timed at each
-            tw = {}
-            modes = {"blocks": 0, "passes": 1, "sparse": 2}
-            for mode in modes:
-                f.passes = modes[mode]
+            tw, cand = {}, {}
+            for name, mode in (("blocks", 0), ("passes", 1), ("sparse", 2)):
                 for wdt in (4, 5, 6):
-                    f.init_state()
-                    f.run_paulis(ops, wdt)
-                    f.synchronize()
-                    t0 = time.perf_counter()
-                    f.run_paulis(ops, wdt)
-                    f.synchronize()
-                    tw[f"{mode}{wdt}"] = time.perf_counter() - t0
+                    cand[f"{name}{wdt}"] = (mode, wdt, 12)
+            for tb in (12, 13):   # gate passes: no fusion, the width is ignored
+                cand[f"gates{tb}"] = (3, 4, tb)
+            for key, (mode, wdt, tb) in cand.items():
+                f.passes = mode
+                f.tile_bits = tb
+                f.init_state()
+                f.run_paulis(ops, wdt)
+                f.synchronize()
+                t0 = time.perf_counter()
+                f.run_paulis(ops, wdt)
+                f.synchronize()
+                tw[key] = time.perf_counter() - t0
             best = min(tw, key=tw.get)
-            f.passes = modes[best[:-1]]
-            width = int(best[-1])
+            f.passes, width, f.tile_bits = cand[best]
             f.init_state()
             f.run_gates(capi.gates_from([wl.G(wl.H, top, [], 0.0)]), 6)
             f.run_paulis(ops, width)   # warm-up step, as in FP64
@@ -474,7 +478,7 @@ def trotter_leg(capi, wl, no_fused, full_qpe=True, no_cpu=False):
             rep32 = f.timing_report()
             f.timing(False)
             fam32 = max(rep32, key=lambda k: rep32[k]["ms"])
-            res.update({"fp32_width": width, "fp32_mode": best[:-1],
+            res.update({"fp32_width": None if f.passes == 3 else width, "fp32_mode": best,
                         "fp32_step_seconds_by_mode_width": {k: round(v, 4) for k, v in tw.items()},
                         "fp32_fused_seconds": round(dt32, 4), "fp32_blocks": nb,
                         "fp32_rotations_per_s": round(rot / dt32, 1),

@@ -274,10 +274,21 @@ int suqa_hip_f32_init_state(suqa_hip_f32* c);
  * tile, fuses each pass and applies all of its blocks in one round trip;
  * 2 is mode 1 with the Pauli rotations of suqa_hip_f32_run_paulis applied as
  * pair updates in LDS instead of fused MFMA blocks (a rotation has two
- * nonzeros per row).  Results agree to FP32 rounding (the arithmetic
- * differs). */
+ * nonzeros per row); 3 fuses nothing: every gate becomes the FP64 engine's
+ * sweep (the gates of suqa_hip_run_gates, the rotations of
+ * suqa_hip_run_paulis, the H and phase gates of suqa_hip_apply_qft_inverse)
+ * and the gate-pass scheduler of the FP64 engine applies them in FP32 VALU
+ * arithmetic, in passes over LDS tiles of suqa_hip_f32_set_tile_bits qubits
+ * (the width argument is then ignored, *nblocks receives the number of
+ * passes, and Pauli strings keep the FP64 engine's limits: weight <= 8, no
+ * control on a qubit of the string).  Results agree to FP32 rounding (the
+ * arithmetic differs). */
 int suqa_hip_f32_set_passes(suqa_hip_f32* c, uint32_t on);
 int suqa_hip_f32_get_passes(const suqa_hip_f32* c, uint32_t* on);
+/* Mode 3's pass tile: 12 (the default; 32 KiB of LDS per workgroup) or 13
+ * qubits (64 KiB). */
+int suqa_hip_f32_set_tile_bits(suqa_hip_f32* c, uint32_t bits);
+int suqa_hip_f32_get_tile_bits(const suqa_hip_f32* c, uint32_t* bits);
 /* Fuses gates[0..ngates) into width-qubit blocks and applies them in order;
  * *nblocks (may be NULL) receives the number of blocks. */
 int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t width,
@@ -292,10 +303,12 @@ int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint3
 /* suqa::apply_qft_inverse (suqa.cu:649-661, no final swaps) on qs[0..n),
  * its H and controlled-phase gates fused into width-qubit blocks. */
 int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, uint32_t width);
-/* Times the gate sample at every valid width in both modes on this context's
- * state (which it changes), returns the fastest width and leaves the fastest
- * mode set; ms_per_gate[3 * mode + width - 4] (6 entries) gets the wall time
- * per gate (-1 where the width does not fit). */
+/* Times the gate sample on this context's state (which it changes) in modes
+ * 0 and 1 at every valid width and in mode 3 at tile bits 12 and 13, leaves
+ * the fastest mode (and mode 3's tile bits) set and returns the fastest
+ * block width of modes 0/1 in *best_width; ms_per_gate (8 entries) gets the
+ * wall time per gate: [3 * mode + width - 4] for modes 0, 1 and
+ * [6 + bits - 12] for mode 3 (-1 where it does not fit). */
 int suqa_hip_f32_autotune(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t* best_width,
                           double* ms_per_gate);
 int suqa_hip_f32_vnorm(suqa_hip_f32* c, double* norm);
@@ -308,9 +321,10 @@ int suqa_hip_f32_fidelity(suqa_hip_f32* c, suqa_hip_ctx* ref, double* fidelity);
 int suqa_hip_f32_download(suqa_hip_f32* c, float* re, float* im);
 int suqa_hip_f32_synchronize(suqa_hip_f32* c);
 /* Per-launch HIP events around every block kernel (k_block<K> as "blockK",
- * k_bpass<K> as "bpassK"): algorithmic HBM bytes (16 B per amplitude of the
- * tiles a launch visits: FP32 re and im read and written) and MFMA flops
- * (8 * 2^K per amplitude per dense block). */
+ * k_bpass<K> as "bpassK", mode 3's k_tile<float> as "tile_pass32"):
+ * algorithmic HBM bytes (16 B per amplitude of the tiles a launch visits:
+ * FP32 re and im read and written) and MFMA flops (8 * 2^K per amplitude per
+ * dense block; 0 for the gate passes). */
 int suqa_hip_f32_timing_enable(suqa_hip_f32* c, int on);
 int suqa_hip_f32_timing_reset(suqa_hip_f32* c);
 int suqa_hip_f32_timing_report(suqa_hip_f32* c, suqa_hip_kstat* out, uint32_t max, uint32_t* n);

@@ -100,6 +100,8 @@ SIGNATURES = {
     "suqa_hip_f32_autotune": (C.c_int, [CTX, C.POINTER(Gate), U32, PU32, PD]),
     "suqa_hip_f32_set_passes": (C.c_int, [CTX, U32]),
     "suqa_hip_f32_get_passes": (C.c_int, [CTX, PU32]),
+    "suqa_hip_f32_set_tile_bits": (C.c_int, [CTX, U32]),
+    "suqa_hip_f32_get_tile_bits": (C.c_int, [CTX, PU32]),
     "suqa_hip_f32_vnorm": (C.c_int, [CTX, PD]),
     "suqa_hip_f32_prob_filter": (C.c_int, [CTX, PU32, PU32, U32, PD]),
     "suqa_hip_f32_measure_qbit": (C.c_int, [CTX, U32, C.c_double, PU32]),
@@ -422,7 +424,9 @@ class F32Engine:
     @property
     def passes(self):
         """Application mode: 0 = one block per launch, 1 = block passes,
-        2 = block passes with sparse Pauli rotations (True/False are 1/0)."""
+        2 = block passes with sparse Pauli rotations, 3 = gate passes (no
+        fusion: the FP64 engine's sweeps in FP32 tile passes).  True/False
+        are 1/0."""
         v = U32()
         check32(lib().suqa_hip_f32_get_passes(self.ctx, C.byref(v)))
         return v.value
@@ -431,14 +435,28 @@ class F32Engine:
     def passes(self, mode):
         check32(lib().suqa_hip_f32_set_passes(self.ctx, int(mode)))
 
+    @property
+    def tile_bits(self):
+        """Tile qubits of a mode-3 gate pass (12 or 13)."""
+        v = U32()
+        check32(lib().suqa_hip_f32_get_tile_bits(self.ctx, C.byref(v)))
+        return v.value
+
+    @tile_bits.setter
+    def tile_bits(self, bits):
+        check32(lib().suqa_hip_f32_set_tile_bits(self.ctx, int(bits)))
+
     def autotune(self, gates, n=None):
-        """Times the sample at widths 4..6 in both modes, leaves the fastest mode
-        set; returns (width, {"blocks4": ms per gate, ..., "passes6": ...})."""
+        """Times the sample in modes 0, 1 at widths 4..6 and in mode 3 at tile
+        bits 12, 13; leaves the fastest mode set and returns (best block width,
+        {"blocks4": ms per gate, ..., "passes6": ..., "gates12": ..., "gates13": ...})."""
         n = len(gates) if n is None else n
         best = U32()
-        t = (C.c_double * 6)()
+        t = (C.c_double * 8)()
         check32(lib().suqa_hip_f32_autotune(self.ctx, gates, n, C.byref(best), t))
-        return best.value, {f"{m}{4 + k}": t[3 * i + k] for i, m in enumerate(("blocks", "passes")) for k in range(3)}
+        out = {f"{m}{4 + k}": t[3 * i + k] for i, m in enumerate(("blocks", "passes")) for k in range(3)}
+        out.update({f"gates{12 + k}": t[6 + k] for k in range(2)})
+        return best.value, out
 
     def vnorm(self):
         v = C.c_double()

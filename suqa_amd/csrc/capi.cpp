@@ -38,7 +38,7 @@ int fail(int code, const std::string& msg) {
 
 #define SQE_CTX(c)                                                                             \
     do {                                                                                       \
-        if (!(c) || !(c)->d_data) return fail(SUQA_ERR_ARG, "null or cleared context");        \
+        if (!(c) || (!(c)->d_data && !(c)->f32)) return fail(SUQA_ERR_ARG, "null or cleared context"); \
     } while (0)
 
 uint32_t popc32(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
@@ -190,6 +190,29 @@ void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes) {
 void timing_end(suqa_hip_ctx* c) {
     if (!c->timing || c->capturing || c->recs.empty()) return;
     (void)hipEventRecord(c->recs.back().e1, c->stream);
+}
+
+suqa_hip_ctx* f32_gate_queue(int device, hipStream_t stream, uint32_t nq, float* re, float* im) {
+    if (nq < 1 || nq > 33 || !re || !im) return nullptr;
+    auto* q = new suqa_hip_ctx();
+    q->device = device;
+    q->stream = stream;   // the FP32 engine's stream (not owned)
+    q->nq = nq;
+    q->N = 1ull << nq;
+    q->f32 = true;
+    q->f32_re = re;
+    q->f32_im = im;
+    q->tile_bits = kMaxTileBits32 - 1;
+    const char* e = std::getenv("SUQA_HIP_PAULI_ORDER");
+    q->pauli_reorder = !(e && std::strcmp(e, "0") == 0);
+    return q;
+}
+
+void f32_gate_queue_free(suqa_hip_ctx* q) {
+    if (!q) return;
+    timing_flush(q);
+    for (auto e : q->evpool) (void)hipEventDestroy(e);
+    delete q;
 }
 
 }  // namespace sqe

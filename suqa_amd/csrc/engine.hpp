@@ -101,6 +101,12 @@ struct suqa_hip_ctx {
     // amplitudes held in LDS (tiled.hip); every other operation flushes first
     uint32_t tile_bits = 0;
     bool pauli_reorder = true;   // run_paulis packs passes with commuting ops moved ahead
+    // the FP32 engine's gate queue (sqe::f32_gate_queue): no FP64 state of its
+    // own; gates queue as above and its passes run on the float state f32_re,
+    // f32_im (one-gate sweeps and every other operation are refused)
+    bool f32 = false;
+    float* f32_re = nullptr;
+    float* f32_im = nullptr;
     std::vector<sqe::GateDesc> queue;
     uint64_t passes = 0, pass_gates = 0, single_sweeps = 0;
     // captured launch sequences (suqa_hip_graph_*): replayable graphs on the
@@ -192,6 +198,7 @@ hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double s
 // Applies the queued gates in tile passes (tiled.hip); a no-op when empty.
 hipError_t flush(suqa_hip_ctx* c);
 constexpr uint32_t kMinTileBits = 1, kMaxTileBits = 12;
+constexpr uint32_t kMaxTileBits32 = 13;   // float tiles (64 KiB of LDS)
 constexpr uint32_t kPassLowFixed = 4;    // qubits 0..3 are in every pass tile (one 128-byte line)
 constexpr uint32_t kPassMaxGates = 24;   // gates per pass
 // run_paulis' order of an op list with gate passes on (capi.cpp)
@@ -200,6 +207,15 @@ constexpr size_t kAutoFlush = 256;   // queued gates that trigger a flush (keeps
 
 // the message suqa_hip_last_error() returns on this thread (capi.cpp)
 void set_last_error(const std::string& msg);
+
+// A gate queue over an FP32 state of 2^nq amplitudes (re, im on `stream`),
+// for the FP32 engine's gate passes: the FP64 gate entry points
+// (suqa_hip_run_gates, suqa_hip_run_paulis, suqa_hip_apply_qft_inverse) accept
+// it and queue their gates, flush() applies them as float tile passes of
+// tile_bits qubits.  Nothing is allocated on the device.  (capi.cpp)
+suqa_hip_ctx* f32_gate_queue(int device, hipStream_t stream, uint32_t nq, float* re, float* im);
+void f32_gate_queue_free(suqa_hip_ctx* q);
+void timing_flush(suqa_hip_ctx* c);
 
 // timing hooks (capi.cpp)
 void timing_begin(suqa_hip_ctx* c, const char* tag, double alg_bytes);

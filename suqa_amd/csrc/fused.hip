@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "deposit.hpp"
+#include "engine.hpp"
 #include "fused.hpp"
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -48,7 +49,10 @@ struct suqa_hip_f32 {
     double* h_result = nullptr;
     float* d_mats = nullptr;   // block matrices in MFMA lane order
     size_t mats_cap = 0;       // floats
-    uint32_t passes = 0;       // 0: one block per launch, 1: block passes, 2: passes with sparse Pauli rotations
+    uint32_t passes = 0;       // 0: one block per launch, 1: block passes, 2: passes with sparse Pauli rotations,
+                               // 3: gate passes (gates queued as on the FP64 engine, float tile passes)
+    suqa_hip_ctx* gq = nullptr;   // mode 3: the gate queue over this state (sqe::f32_gate_queue)
+    uint32_t gate_tile_bits = 12; // mode 3: tile qubits of a pass (12: 32 KiB of LDS, 13: 64 KiB)
     // per-launch HIP events of the block kernels (suqa_hip_f32_timing_*)
     bool timing = false;
     struct Rec {
@@ -1053,6 +1057,38 @@ std::vector<PassOp> as_pass_ops(std::vector<Op>&& ops) {
     return v;
 }
 
+// Mode 3 (gate passes): the gates go through the FP64 engine's gate entry
+// points on a gate queue whose state is this float state, so they become the
+// same pair and phase sweeps, and its pass scheduler applies them as float
+// tile passes (tiled.hip, k_tile<float>): every gate is computed on its own,
+// in FP32 VALU arithmetic, with no fused matrices.  Each call applies what it
+// queued before it returns.
+int gate_queue(suqa_hip_f32* c, suqa_hip_ctx** out) {
+    if (!c->gq) {
+        c->gq = sqe::f32_gate_queue(c->device, c->stream, c->nq, c->d_re, c->d_im);
+        if (!c->gq) return fail(SUQA_ERR_NOMEM, "gate queue");
+    }
+    c->gq->tile_bits = std::min(c->gate_tile_bits, c->nq);
+    c->gq->timing = c->timing;
+    *out = c->gq;
+    return SUQA_OK;
+}
+// rc: what the entry point returned (its message is suqa_hip_last_error's)
+int gate_queue_done(suqa_hip_f32* c, int rc, uint32_t* npasses, uint64_t passes_before) {
+    if (rc != SUQA_OK) {
+        c->gq->queue.clear();
+        return fail(rc, suqa_hip_last_error());
+    }
+    const hipError_t e = sqe::flush(c->gq);
+    if (e != hipSuccess) {
+        c->gq->queue.clear();
+        const std::string why = suqa_hip_last_error();
+        return fail(SUQA_ERR_HIP, std::string("gate pass: ") + hipGetErrorString(e) + (why.empty() ? "" : " (" + why + ")"));
+    }
+    if (npasses) *npasses = (uint32_t)(c->gq->passes - passes_before);
+    return SUQA_OK;
+}
+
 // Workload gates must name distinct in-range qubits (a block's tile geometry
 // is built from them).
 int check_gates(const suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t n) {
@@ -1070,6 +1106,8 @@ int check_gates(const suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t n) {
 }
 
 bool width_ok(uint32_t nq, uint32_t K) { return K >= (uint32_t)kMinK && K <= (uint32_t)kMaxK && nq >= K + 5; }
+// mode 3 takes any width (it fuses nothing)
+bool width_ok(const suqa_hip_f32* c, uint32_t K) { return c->passes == 3 || width_ok(c->nq, K); }
 
 }  // namespace
 }  // namespace sqf
@@ -1121,6 +1159,7 @@ int suqa_hip_f32_clear(suqa_hip_f32* c) {
         (void)hipEventDestroy(r.e1);
     }
     for (auto e : c->evpool) (void)hipEventDestroy(e);
+    sqe::f32_gate_queue_free(c->gq);
     if (c->d_data) (void)hipFree(c->d_data);
     if (c->d_partials) (void)hipFree(c->d_partials);
     if (c->d_result) (void)hipFree(c->d_result);
@@ -1142,8 +1181,14 @@ int suqa_hip_f32_init_state(suqa_hip_f32* c) {
 int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t ngates, uint32_t width,
                            uint32_t* nblocks) {
     if (!c || !c->d_data || (ngates && !gates)) return fail(SUQA_ERR_ARG, "null argument");
-    if (!width_ok(c->nq, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
+    if (!width_ok(c, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
     if (int rc = check_gates(c, gates, ngates)) return rc;
+    if (c->passes == 3) {
+        suqa_hip_ctx* q = nullptr;
+        if (int rc = gate_queue(c, &q)) return rc;
+        const uint64_t p0 = q->passes;
+        return gate_queue_done(c, suqa_hip_run_gates(q, gates, ngates), nblocks, p0);
+    }
     std::vector<PassOp> items;
     items.reserve(ngates);
     for (uint32_t i = 0; i < ngates; ++i) items.push_back(pass_gate(gates[i]));
@@ -1153,7 +1198,13 @@ int suqa_hip_f32_run_gates(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t
 int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint32_t nops, uint32_t width,
                             uint32_t* nblocks) {
     if (!c || !c->d_data || (nops && !ops)) return fail(SUQA_ERR_ARG, "null argument");
-    if (!width_ok(c->nq, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
+    if (!width_ok(c, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
+    if (c->passes == 3) {
+        suqa_hip_ctx* q = nullptr;
+        if (int rc = gate_queue(c, &q)) return rc;
+        const uint64_t p0 = q->passes;
+        return gate_queue_done(c, suqa_hip_run_paulis(q, ops, nops), nblocks, p0);
+    }
     std::vector<Op> list(nops);
     for (uint32_t i = 0; i < nops; ++i) {
         if (!pauli_op(ops[i], list[i])) return fail(SUQA_ERR_ARG, "malformed Pauli op, a control on one of its qubits, or more than 6 qubits");
@@ -1169,11 +1220,16 @@ int suqa_hip_f32_run_paulis(suqa_hip_f32* c, const suqa_hip_pauli_op* ops, uint3
 
 int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, uint32_t width) {
     if (!c || !c->d_data || (n && !qs)) return fail(SUQA_ERR_ARG, "null argument");
-    if (!width_ok(c->nq, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
+    if (!width_ok(c, width)) return fail(SUQA_ERR_ARG, "fusion width must be 4..6 and nq >= width + 5");
     uint64_t seen = 0;
     for (uint32_t k = 0; k < n; ++k) {
         if (qs[k] >= c->nq || ((seen >> qs[k]) & 1ull)) return fail(SUQA_ERR_ARG, "bad register");
         seen |= 1ull << qs[k];
+    }
+    if (c->passes == 3) {
+        suqa_hip_ctx* q = nullptr;
+        if (int rc = gate_queue(c, &q)) return rc;
+        return gate_queue_done(c, suqa_hip_apply_qft_inverse(q, qs, n, 0), nullptr, q->passes);
     }
     std::vector<Op> list;
     qft_inverse_ops(qs, n, list);
@@ -1182,7 +1238,7 @@ int suqa_hip_f32_qft_inverse(suqa_hip_f32* c, const uint32_t* qs, uint32_t n, ui
 
 int suqa_hip_f32_set_passes(suqa_hip_f32* c, uint32_t on) {
     if (!c) return fail(SUQA_ERR_ARG, "null context");
-    if (on > 2) return fail(SUQA_ERR_ARG, "mode must be 0 (one block per launch), 1 (block passes) or 2 (block passes, sparse Pauli rotations)");
+    if (on > 3) return fail(SUQA_ERR_ARG, "mode must be 0 (one block per launch), 1 (block passes), 2 (block passes, sparse Pauli rotations) or 3 (gate passes)");
     c->passes = on;
     return SUQA_OK;
 }
@@ -1200,33 +1256,56 @@ int suqa_hip_f32_autotune(suqa_hip_f32* c, const suqa_hip_gate* gates, uint32_t 
     std::vector<PassOp> ops;
     for (uint32_t i = 0; i < ngates; ++i) ops.push_back(pass_gate(gates[i]));
     const std::vector<PassOp> warm(ops.begin(), ops.begin() + std::min<size_t>(ops.size(), 8));
-    double best = 1e300;
-    uint32_t best_mode = 0;
+    const uint32_t nwarm = std::min<uint32_t>(ngates, 8);
+    double best = 1e300, best_block = 1e300;
+    uint32_t best_mode = 0, best_tb = c->gate_tile_bits;
     *best_width = 0;
-    for (uint32_t mode = 0; mode < 2; ++mode)
-        for (uint32_t K = kMinK; K <= (uint32_t)kMaxK; ++K) {
-            const uint32_t slot = mode * (kMaxK - kMinK + 1) + (K - kMinK);
-            if (ms_per_gate) ms_per_gate[slot] = -1.0;
-            if (!width_ok(c->nq, K)) continue;
-            c->passes = mode;
-            // warm-up on the first ops, then time the whole sample
-            int rc = run_ops(c, warm, K, nullptr);
-            if (rc) return rc;
-            const auto t0 = std::chrono::steady_clock::now();
-            rc = run_ops(c, ops, K, nullptr);
-            if (rc) return rc;
-            SQF_HIP(hipStreamSynchronize(c->stream));
-            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            const double per = ms / std::max<uint32_t>(ngates, 1);
-            if (ms_per_gate) ms_per_gate[slot] = per;
-            if (per < best) {
-                best = per;
-                *best_width = K;
-                best_mode = mode;
-            }
+    // modes 0 and 1 at widths 4..6 (slots 0..5), mode 3 at tile bits 12, 13 (slots 6, 7)
+    for (uint32_t slot = 0; slot < 8; ++slot) {
+        const uint32_t mode = slot < 3 ? 0u : slot < 6 ? 1u : 3u;
+        const uint32_t K = mode == 3 ? (uint32_t)kMinK : kMinK + slot % 3;
+        const uint32_t tb = mode == 3 ? 12u + (slot - 6u) : c->gate_tile_bits;
+        if (ms_per_gate) ms_per_gate[slot] = -1.0;
+        if (!width_ok(c->nq, K) || (mode == 3 && tb > c->nq)) continue;
+        c->passes = mode;
+        const uint32_t keep_tb = c->gate_tile_bits;
+        c->gate_tile_bits = tb;
+        // warm-up on the first ops, then time the whole sample
+        int rc = mode == 3 ? suqa_hip_f32_run_gates(c, gates, nwarm, K, nullptr) : run_ops(c, warm, K, nullptr);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!rc) rc = mode == 3 ? suqa_hip_f32_run_gates(c, gates, ngates, K, nullptr) : run_ops(c, ops, K, nullptr);
+        c->gate_tile_bits = keep_tb;
+        if (rc) return rc;
+        SQF_HIP(hipStreamSynchronize(c->stream));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        const double per = ms / std::max<uint32_t>(ngates, 1);
+        if (ms_per_gate) ms_per_gate[slot] = per;
+        if (mode != 3 && per < best_block) {
+            best_block = per;
+            *best_width = K;
         }
+        if (per < best) {
+            best = per;
+            best_mode = mode;
+            best_tb = tb;
+        }
+    }
     c->passes = best_mode;
+    c->gate_tile_bits = best_tb;
     return *best_width ? SUQA_OK : fail(SUQA_ERR_ARG, "no fusion width fits this state");
+}
+
+int suqa_hip_f32_set_tile_bits(suqa_hip_f32* c, uint32_t bits) {
+    if (!c) return fail(SUQA_ERR_ARG, "null context");
+    if (bits < 12 || bits > sqe::kMaxTileBits32) return fail(SUQA_ERR_ARG, "gate-pass tile bits must be 12 or 13");
+    c->gate_tile_bits = bits;
+    return SUQA_OK;
+}
+
+int suqa_hip_f32_get_tile_bits(const suqa_hip_f32* c, uint32_t* bits) {
+    if (!c || !bits) return fail(SUQA_ERR_ARG, "null argument");
+    *bits = c->gate_tile_bits;
+    return SUQA_OK;
 }
 
 int suqa_hip_f32_vnorm(suqa_hip_f32* c, double* norm) {
@@ -1309,6 +1388,7 @@ int suqa_hip_f32_download(suqa_hip_f32* c, float* re, float* im) {
 int suqa_hip_f32_timing_enable(suqa_hip_f32* c, int on) {
     if (!c) return fail(SUQA_ERR_ARG, "null context");
     c->timing = on != 0;
+    if (c->gq) c->gq->timing = c->timing;
     return SUQA_OK;
 }
 
@@ -1317,6 +1397,10 @@ int suqa_hip_f32_timing_reset(suqa_hip_f32* c) {
     SQF_HIP(hipStreamSynchronize(c->stream));
     timing_collect(c);
     c->stats.clear();
+    if (c->gq) {
+        sqe::timing_flush(c->gq);
+        c->gq->stats.clear();
+    }
     return SUQA_OK;
 }
 
@@ -1324,16 +1408,27 @@ int suqa_hip_f32_timing_report(suqa_hip_f32* c, suqa_hip_kstat* out, uint32_t ma
     if (!c || (max && !out)) return fail(SUQA_ERR_ARG, "null argument");
     SQF_HIP(hipStreamSynchronize(c->stream));
     timing_collect(c);
-    uint32_t k = 0;
-    for (; k < c->stats.size() && k < max; ++k) {
-        std::memset(&out[k], 0, sizeof(out[k]));
-        std::strncpy(out[k].name, c->stats[k].name.c_str(), sizeof(out[k].name) - 1);
-        out[k].launches = c->stats[k].launches;
-        out[k].total_ms = c->stats[k].ms;
-        out[k].alg_bytes = c->stats[k].bytes;
-        out[k].flops = c->stats[k].flops;
+    std::vector<suqa_hip_f32::Stat> all = c->stats;
+    if (c->gq) {   // the gate passes of mode 3 (VALU: no MFMA flops)
+        sqe::timing_flush(c->gq);
+        for (const auto& g : c->gq->stats) {
+            all.emplace_back();
+            all.back().name = g.name;
+            all.back().launches = g.launches;
+            all.back().ms = g.total_ms;
+            all.back().bytes = g.alg_bytes;
+        }
     }
-    if (n) *n = (uint32_t)c->stats.size();
+    uint32_t k = 0;
+    for (; k < all.size() && k < max; ++k) {
+        std::memset(&out[k], 0, sizeof(out[k]));
+        std::strncpy(out[k].name, all[k].name.c_str(), sizeof(out[k].name) - 1);
+        out[k].launches = all[k].launches;
+        out[k].total_ms = all[k].ms;
+        out[k].alg_bytes = all[k].bytes;
+        out[k].flops = all[k].flops;
+    }
+    if (n) *n = (uint32_t)all.size();
     return SUQA_OK;
 }
 

@@ -642,6 +642,7 @@ hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, u
 
 hipError_t launch_pair_now(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,
                            const char* tag, double alg_bytes) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     const uint64_t M = sel_mask(sel), V = sel.fval;
     if (((M | xmask) & kLowMask) != 0) {
         PairDLArgs a;
@@ -698,6 +699,7 @@ hipError_t launch_pair_now(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xma
 
 hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
                             const char* tag, double alg_bytes) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     PhaseArgs a;
     a.c = cs;
     a.s = sn;
@@ -720,6 +722,7 @@ hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double s
 
 hipError_t launch_phase_list(suqa_hip_ctx* c, const Sel& sel, uint32_t q0, uint32_t size_mask,
                              const double* cs, const double* sn) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
     PhaseListArgs a;
     a.fb = sel.fb;
@@ -782,6 +785,7 @@ hipError_t launch_norm_tree(const double* re, const double* im, uint32_t nq, con
 
 hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, const char* tag,
                        double alg_bytes) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
     return launch_norm_tree(c->d_re, c->d_im, c->nq, sel, split_bit, c->d_partials, c->d_result, c->stream, c, tag,
                             alg_bytes);
@@ -789,6 +793,7 @@ hipError_t launch_norm(suqa_hip_ctx* c, const Sel& sel, uint64_t split_bit, cons
 
 hipError_t launch_overlap(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t m1, uint64_t m2, uint32_t p1, uint32_t p2,
                           double cs, double sn) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
     OverlapArgs a;
     a.fb = sel.fb;
@@ -808,6 +813,7 @@ hipError_t launch_overlap(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t m1, 
 }
 
 hipError_t launch_checksum(suqa_hip_ctx* c) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
     const uint32_t g = grid_for(c->N);
     for (int mode = 0; mode < 2; ++mode) {
@@ -832,6 +838,7 @@ hipError_t launch_checksum(suqa_hip_ctx* c) {
 }
 
 hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double scale) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
     const uint64_t nitems = c->N / 2;
     // the discarded half is written without being read: 24 B per amplitude
@@ -844,6 +851,7 @@ hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double
 
 hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, uint32_t resets,
                                 uint32_t gc) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
     if (n == 0 || n > kMeasMax || c->nq > kMeasSmallBits) return hipErrorInvalidValue;
     MeasArgs a{};
@@ -863,6 +871,7 @@ hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const doubl
 }
 
 hipError_t launch_scale(suqa_hip_ctx* c, double f) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
     const uint64_t nitems = c->N;   // 2N doubles as N double2
     timing_begin(c, "scale", 32.0 * (double)c->N);
@@ -872,6 +881,7 @@ hipError_t launch_scale(suqa_hip_ctx* c, double f) {
 }
 
 hipError_t launch_init(suqa_hip_ctx* c) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
     const uint64_t nitems = c->N;
     timing_begin(c, "init", 16.0 * (double)c->N);
@@ -892,6 +902,7 @@ hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_tab, uint32_t wmask, ui
 }
 
 hipError_t launch_copy(suqa_hip_ctx* c, void* dst, const void* src, uint64_t nbytes) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     const uint64_t n16 = nbytes / 16;
     timing_begin(c, "copy", 2.0 * (double)nbytes);
     hipLaunchKernelGGL(k_copy, dim3(grid_for(n16)), dim3(kBlock), 0, c->stream, reinterpret_cast<dv2*>(dst),

@@ -1,4 +1,5 @@
-// tiled.hip -- multi-gate passes of the FP64 engine.
+// tiled.hip -- multi-gate passes of the FP64 engine (and of the FP32
+// engine's gate-pass mode, on a float state with the same kernels).
 //
 // With tile passes on (suqa_hip_set_tile_bits), the gate sweeps that the API
 // calls produce (a pair operation or a phase rotation over a selection) are
@@ -32,6 +33,7 @@
 // enumeration: tiles outside the selection are never read.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 
 #include "deposit.hpp"
@@ -43,14 +45,28 @@ namespace sqe {
 namespace {
 
 constexpr uint32_t kMaxPass = kPassMaxGates;
-constexpr uint32_t kLowFixed = kPassLowFixed;     // qubits 0..3: 16 amplitudes = one 128-byte line
+constexpr uint32_t kLowFixed = kPassLowFixed;     // qubits 0..3: 16 amplitudes, one 128-byte line in FP64
 constexpr uint32_t kSub = 3, kLoc = 1u << kSub;   // register step: 3 bits, 8 amplitudes per thread
 constexpr uint32_t kTinyBits = 10;                // one-tile states up to 2^10: k_tile_tiny
-constexpr uint32_t kMaxChunks = (1u << kMaxTileBits) / 2 / kBlock;   // 16-byte chunks per array per thread
 
-typedef double dv2t __attribute__((ext_vector_type(2)));
-typedef double dv8 __attribute__((ext_vector_type(8)));    // a thread's 8 register amplitudes
-typedef uint32_t uv8 __attribute__((ext_vector_type(8)));  // their tile offsets
+// Amplitude types: double (the FP64 engine) or float (the FP32 engine's gate
+// passes).  A tile moves between HBM and LDS in 16-byte chunks of kVec
+// amplitudes per array; tiles have at most kMaxBits bits.
+template <typename T>
+struct AmpT;
+template <>
+struct AmpT<double> {
+    typedef double v8 __attribute__((ext_vector_type(8)));   // a thread's 8 register amplitudes
+    typedef double chunk __attribute__((ext_vector_type(2)));
+    static constexpr uint32_t kVec = 2, kMaxBits = kMaxTileBits;
+};
+template <>
+struct AmpT<float> {
+    typedef float v8 __attribute__((ext_vector_type(8)));
+    typedef float chunk __attribute__((ext_vector_type(4)));
+    static constexpr uint32_t kVec = 4, kMaxBits = kMaxTileBits32;
+};
+typedef uint32_t uv8 __attribute__((ext_vector_type(8)));  // tile offsets of the 8 register amplitudes
 
 struct TileGate {
     uint32_t kind;        // OP_SWAP / OP_Y / OP_H / OP_PAULI or kPhaseOp
@@ -67,6 +83,7 @@ struct TileGate {
     uint32_t dense;       // gtriv, and every element (pair) of a group is selected
     uint32_t base_imag, base_neg;
     uint32_t arm;         // register-step code path (reg_gate): kind x partner mask x dense
+    float cf, sf;         // c, s rounded to FP32 (float passes)
     uint64_t mo, vo;      // selection bits outside the tile (global positions)
     uint64_t yzo;         // parity bits outside the tile
     double c, s;
@@ -89,16 +106,34 @@ struct TileArgs {
 };
 constexpr uint32_t kLdsStep = 0xffu;
 
+// A gate's coefficients in the pass's amplitude type (FP64: the exact values).
+template <typename T>
+__device__ __forceinline__ T gate_c(const TileGate& g);
+template <>
+__device__ __forceinline__ double gate_c<double>(const TileGate& g) { return g.c; }
+template <>
+__device__ __forceinline__ float gate_c<float>(const TileGate& g) { return g.cf; }
+template <typename T>
+__device__ __forceinline__ T gate_s(const TileGate& g);
+template <>
+__device__ __forceinline__ double gate_s<double>(const TileGate& g) { return g.s; }
+template <>
+__device__ __forceinline__ float gate_s<float>(const TileGate& g) { return g.sf; }
+
+template <typename T>
+struct Coef {   // what pair_out_par reads of a pair gate
+    T c, s;
+    uint32_t base_imag, base_neg;
+};
+template <typename T>
+__device__ __forceinline__ Coef<T> coef(const TileGate& g) {
+    return Coef<T>{gate_c<T>(g), gate_s<T>(g), g.base_imag, g.base_neg};
+}
+
 // LDS step: pair gate over the whole tile, pair (u, u ^ X) per thread.
-template <int OP>
-__device__ __forceinline__ void tile_pairs(double* lre, double* lim, uint32_t half, const TileGate& g, uint32_t opar) {
-    PauliCoef pc;
-    pc.c = g.c;
-    pc.s = g.s;
-    pc.yz = 0;
-    pc.ny_odd = 0;
-    pc.base_imag = g.base_imag;
-    pc.base_neg = g.base_neg;
+template <int OP, typename T>
+__device__ __forceinline__ void tile_pairs(T* lre, T* lim, uint32_t half, const TileGate& g, uint32_t opar) {
+    const Coef<T> pc = coef<T>(g);
     const uint32_t lo = (1u << g.pivot) - 1u;
     for (uint32_t p = threadIdx.x; p < half; p += kBlock) {
         const uint32_t u = ((p & ~lo) << 1) | (p & lo);
@@ -113,7 +148,7 @@ __device__ __forceinline__ void tile_pairs(double* lre, double* lim, uint32_t ha
         } else {
             continue;
         }
-        const double ra = lre[ia], ma = lim[ia], rb = lre[ib], mb = lim[ib];
+        const T ra = lre[ia], ma = lim[ia], rb = lre[ib], mb = lim[ib];
         // the partner of a is b and vice versa
         const uint32_t par_a = OP == OP_PAULI ? (((uint32_t)__popc(ib & g.yzc) & 1u) ^ opar) : 0u;
         const uint32_t par_b = OP == OP_PAULI ? (((uint32_t)__popc(ia & g.yzc) & 1u) ^ opar) : 0u;
@@ -128,34 +163,36 @@ __device__ __forceinline__ void tile_pairs(double* lre, double* lim, uint32_t ha
 // value: with references, the compiler merges the stores of the switch cases
 // below into one store through a pointer phi and the amplitudes end up in
 // scratch memory.
+template <typename T>
 struct RM {
-    dv8 r, m;
+    typename AmpT<T>::v8 r, m;
 };
 
 // Register steps compute every element and pick results with selects, never
 // with branches on data: a branch around an update makes the compiler copy
-// the whole 16-double register set at the join.
+// the whole register set at the join.
 
 // Pair gate with local partner mask XL: pair (J, J ^ XL) for each J without
 // the lowest bit of XL, expanded at compile time.  DENSE: every pair of the
 // group is selected (no test at all).  Otherwise the roles come from the
 // uniform table lsel, gpass (this group meets the selection's other bits)
 // and gpar (parity of the other bits) are per thread.
-template <int XL, int OP, bool DENSE, int J>
-__device__ __forceinline__ RM reg_pair_one(RM x, const TileGate& g, const PauliCoef& pc, bool gpass, uint32_t gpar) {
+template <int XL, int OP, bool DENSE, int J, typename T>
+__device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, const Coef<T>& pc, bool gpass,
+                                              uint32_t gpar) {
     constexpr int PIV = XL & -XL;
     if constexpr ((J & PIV) == 0) {
         constexpr int u = J, w = J ^ XL;
-        const double ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
+        const T ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
         if constexpr (XL == PIV) {
             // one flipped bit: gates of this kind select their pivot bit at 0 (the
             // host checks it), so the lower element u is the reference's i
             const uint32_t par_u = OP == OP_PAULI ? (((g.lpar >> w) & 1u) ^ gpar) : 0u;
             const uint32_t par_w = OP == OP_PAULI ? (((g.lpar >> u) & 1u) ^ gpar) : 0u;
-            const double nru = pair_out_par<OP>(true, false, ru, rw, mw, par_u, pc);
-            const double nmu = pair_out_par<OP>(true, true, mu, mw, rw, par_u, pc);
-            const double nrw = pair_out_par<OP>(false, false, rw, ru, mu, par_w, pc);
-            const double nmw = pair_out_par<OP>(false, true, mw, mu, ru, par_w, pc);
+            const T nru = pair_out_par<OP>(true, false, ru, rw, mw, par_u, pc);
+            const T nmu = pair_out_par<OP>(true, true, mu, mw, rw, par_u, pc);
+            const T nrw = pair_out_par<OP>(false, false, rw, ru, mu, par_w, pc);
+            const T nmw = pair_out_par<OP>(false, true, mw, mu, ru, par_w, pc);
             if constexpr (DENSE) {
                 x.r[u] = nru;
                 x.m[u] = nmu;
@@ -175,10 +212,10 @@ __device__ __forceinline__ RM reg_pair_one(RM x, const TileGate& g, const PauliC
             const bool on = (su || sw) && (g.gtriv || gpass);
             const uint32_t par_u = OP == OP_PAULI ? (((g.lpar >> w) & 1u) ^ gpar) : 0u;
             const uint32_t par_w = OP == OP_PAULI ? (((g.lpar >> u) & 1u) ^ gpar) : 0u;
-            const double nru = pair_out_par<OP>(su, false, ru, rw, mw, par_u, pc);
-            const double nmu = pair_out_par<OP>(su, true, mu, mw, rw, par_u, pc);
-            const double nrw = pair_out_par<OP>(sw, false, rw, ru, mu, par_w, pc);
-            const double nmw = pair_out_par<OP>(sw, true, mw, mu, ru, par_w, pc);
+            const T nru = pair_out_par<OP>(su, false, ru, rw, mw, par_u, pc);
+            const T nmu = pair_out_par<OP>(su, true, mu, mw, rw, par_u, pc);
+            const T nrw = pair_out_par<OP>(sw, false, rw, ru, mu, par_w, pc);
+            const T nmw = pair_out_par<OP>(sw, true, mw, mu, ru, par_w, pc);
             x.r[u] = on ? nru : ru;
             x.m[u] = on ? nmu : mu;
             x.r[w] = on ? nrw : rw;
@@ -188,28 +225,23 @@ __device__ __forceinline__ RM reg_pair_one(RM x, const TileGate& g, const PauliC
     return x;
 }
 
-template <int XL, int OP, bool DENSE, int... J>
-__device__ __forceinline__ RM reg_pairs_seq(RM x, const TileGate& g, bool gpass, uint32_t gpar,
-                                            std::integer_sequence<int, J...>) {
-    PauliCoef pc;
-    pc.c = g.c;
-    pc.s = g.s;
-    pc.yz = 0;
-    pc.ny_odd = 0;
-    pc.base_imag = g.base_imag;
-    pc.base_neg = g.base_neg;
+template <int XL, int OP, bool DENSE, typename T, int... J>
+__device__ __forceinline__ RM<T> reg_pairs_seq(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar,
+                                               std::integer_sequence<int, J...>) {
+    const Coef<T> pc = coef<T>(g);
     ((x = reg_pair_one<XL, OP, DENSE, J>(x, g, pc, gpass, gpar)), ...);
     return x;
 }
 
 // Phase rotation of the selected register amplitudes.
-template <bool DENSE, int J>
-__device__ __forceinline__ RM reg_phase_one(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
+template <bool DENSE, int J, typename T>
+__device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar) {
     const uint32_t par = ((g.lpar >> J) & 1u) ^ gpar;
-    const double sn = par ? -g.s : g.s;
-    const double rr = x.r[J], mm = x.m[J];
-    const double nr = phase_out(false, rr, mm, g.c, sn);
-    const double nm = phase_out(true, mm, rr, g.c, sn);
+    const T s = gate_s<T>(g);
+    const T sn = par ? -s : s;
+    const T rr = x.r[J], mm = x.m[J];
+    const T nr = phase_out(false, rr, mm, gate_c<T>(g), sn);
+    const T nm = phase_out(true, mm, rr, gate_c<T>(g), sn);
     if constexpr (DENSE) {
         x.r[J] = nr;
         x.m[J] = nm;
@@ -221,9 +253,9 @@ __device__ __forceinline__ RM reg_phase_one(RM x, const TileGate& g, bool gpass,
     return x;
 }
 
-template <bool DENSE, int... J>
-__device__ __forceinline__ RM reg_phase_seq(RM x, const TileGate& g, bool gpass, uint32_t gpar,
-                                            std::integer_sequence<int, J...>) {
+template <bool DENSE, typename T, int... J>
+__device__ __forceinline__ RM<T> reg_phase_seq(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar,
+                                               std::integer_sequence<int, J...>) {
     ((x = reg_phase_one<DENSE, J>(x, g, gpass, gpar)), ...);
     return x;
 }
@@ -249,7 +281,8 @@ __device__ __forceinline__ RM reg_phase_seq(RM x, const TileGate& g, bool gpass,
     case 2 + 10 * OPV + 8: return reg_pairs_seq<6, OPV, false>(x, g, gpass, gpar, seq);                       \
     case 2 + 10 * OPV + 9: return reg_pairs_seq<7, OPV, false>(x, g, gpass, gpar, seq);
 
-__device__ __forceinline__ RM reg_gate(RM x, const TileGate& g, bool gpass, uint32_t gpar) {
+template <typename T>
+__device__ __forceinline__ RM<T> reg_gate(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar) {
     constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
     switch (g.arm) {
     case 0: return reg_phase_seq<true>(x, g, gpass, gpar, seq);
@@ -263,50 +296,55 @@ __device__ __forceinline__ RM reg_gate(RM x, const TileGate& g, bool gpass, uint
 }
 #undef SQE_PAIR_ARMS
 
-__global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double* __restrict__ im, const TileArgs a) {
-    extern __shared__ double lds[];
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restrict__ im, const TileArgs a) {
+    typedef typename AmpT<T>::chunk CH;
+    constexpr uint32_t V = AmpT<T>::kVec;
+    constexpr uint32_t kMaxChunks = (1u << AmpT<T>::kMaxBits) / V / kBlock;   // chunks per array per thread
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t tsz = 1u << a.t;
-    double* lre = lds;
-    double* lim = lds + tsz;
+    T* lre = reinterpret_cast<T*>(lds_raw);
+    T* lim = lre + tsz;
     const uint64_t G = gridDim.x, b = blockIdx.x;
     const uint64_t tile = (b & 7ull) * (G >> 3) + (b >> 3);   // XCD-contiguous (grid % 8 == 0)
     if (tile >= a.ntiles) return;                             // whole workgroup leaves together
     const uint64_t base = deposit(tile, a.freemask) | a.fixval;
 
-    // 1. tile -> LDS: chunk k of a thread is tile element e = 2 tid + k 2 kBlock;
-    //    every load of the thread is issued before the LDS writes
-    const uint32_t e0 = 2 * threadIdx.x;
-    const uint32_t nch = tsz / (2 * kBlock);   // 0 when the tile is smaller than 2 kBlock
+    // 1. tile -> LDS: chunk k of a thread is tile elements e = V tid + k V kBlock
+    //    onwards (tile bits 0..log2 V - 1 are state qubits 0.. of one line); every
+    //    load of the thread is issued before the LDS writes
+    const uint32_t e0 = V * threadIdx.x;
+    const uint32_t nch = tsz / (V * kBlock);   // 0 when the tile is smaller than V kBlock
     const uint64_t toff = e0 < tsz ? (base | deposit(e0, a.tmask)) : 0ull;
-    // state offsets of tile bits 9, 10, 11 (chunk k adds those of k's bits)
+    // state offsets of the tile bits that chunk index bits 0, 1, 2 stand for
     uint64_t kbit[3];
 #pragma unroll
-    for (uint32_t j = 0; j < 3; ++j) kbit[j] = deposit(2u * kBlock << j, a.tmask);
+    for (uint32_t j = 0; j < 3; ++j) kbit[j] = deposit(V * kBlock << j, a.tmask);
     auto chunk_off = [&](uint32_t k) {
         return ((k & 1u) ? kbit[0] : 0ull) | ((k & 2u) ? kbit[1] : 0ull) | ((k & 4u) ? kbit[2] : 0ull);
     };
     if (nch == 0) {
         if (e0 < tsz) {
-            const dv2t vr = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(re + toff));
-            const dv2t vi = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(im + toff));
-            *reinterpret_cast<dv2t*>(lre + e0) = vr;
-            *reinterpret_cast<dv2t*>(lim + e0) = vi;
+            const CH vr = __builtin_nontemporal_load(reinterpret_cast<const CH*>(re + toff));
+            const CH vi = __builtin_nontemporal_load(reinterpret_cast<const CH*>(im + toff));
+            *reinterpret_cast<CH*>(lre + e0) = vr;
+            *reinterpret_cast<CH*>(lim + e0) = vi;
         }
     } else {
-        dv2t vr[kMaxChunks], vi[kMaxChunks];
+        CH vr[kMaxChunks], vi[kMaxChunks];
 #pragma unroll
         for (uint32_t k = 0; k < kMaxChunks; ++k) {
             if (k < nch) {
                 const uint64_t g = toff | chunk_off(k);
-                vr[k] = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(re + g));
-                vi[k] = __builtin_nontemporal_load(reinterpret_cast<const dv2t*>(im + g));
+                vr[k] = __builtin_nontemporal_load(reinterpret_cast<const CH*>(re + g));
+                vi[k] = __builtin_nontemporal_load(reinterpret_cast<const CH*>(im + g));
             }
         }
 #pragma unroll
         for (uint32_t k = 0; k < kMaxChunks; ++k) {
             if (k < nch) {
-                *reinterpret_cast<dv2t*>(lre + e0 + k * 2 * kBlock) = vr[k];
-                *reinterpret_cast<dv2t*>(lim + e0 + k * 2 * kBlock) = vi[k];
+                *reinterpret_cast<CH*>(lre + e0 + k * V * kBlock) = vr[k];
+                *reinterpret_cast<CH*>(lim + e0 + k * V * kBlock) = vi[k];
             }
         }
     }
@@ -340,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
                 const uint32_t lo = (1u << S.f[q]) - 1u;
                 gb = (gb & lo) | ((gb & ~lo) << 1);
             }
-            RM x;
+            RM<T> x;
 #pragma unroll
             for (uint32_t j = 0; j < kLoc; ++j) {
                 x.r[j] = lre[gb | off[j]];
@@ -365,18 +403,18 @@ __global__ __launch_bounds__(kBlock) void k_tile(double* __restrict__ re, double
     // 3. LDS -> tile
     if (nch == 0) {
         if (e0 < tsz) {
-            __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lre + e0), reinterpret_cast<dv2t*>(re + toff));
-            __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lim + e0), reinterpret_cast<dv2t*>(im + toff));
+            __builtin_nontemporal_store(*reinterpret_cast<const CH*>(lre + e0), reinterpret_cast<CH*>(re + toff));
+            __builtin_nontemporal_store(*reinterpret_cast<const CH*>(lim + e0), reinterpret_cast<CH*>(im + toff));
         }
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < kMaxChunks; ++k) {
             if (k < nch) {
                 const uint64_t g = toff | chunk_off(k);
-                __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lre + e0 + k * 2 * kBlock),
-                                            reinterpret_cast<dv2t*>(re + g));
-                __builtin_nontemporal_store(*reinterpret_cast<const dv2t*>(lim + e0 + k * 2 * kBlock),
-                                            reinterpret_cast<dv2t*>(im + g));
+                __builtin_nontemporal_store(*reinterpret_cast<const CH*>(lre + e0 + k * V * kBlock),
+                                            reinterpret_cast<CH*>(re + g));
+                __builtin_nontemporal_store(*reinterpret_cast<const CH*>(lim + e0 + k * V * kBlock),
+                                            reinterpret_cast<CH*>(im + g));
             }
         }
     }
@@ -399,14 +437,17 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v) {
 __device__ __forceinline__ double unid(double v) {
     return __longlong_as_double((long long)uni64((uint64_t)__double_as_longlong(v)));
 }
+__device__ __forceinline__ float unif(float v) { return __uint_as_float(uni(__float_as_uint(v))); }
 
-__global__ __launch_bounds__(1024) void k_tile_tiny(double* __restrict__ re, double* __restrict__ im, const TileArgs a) {
-    extern __shared__ double lds[];
+template <typename T>
+__global__ __launch_bounds__(1024) void k_tile_tiny(T* __restrict__ re, T* __restrict__ im, const TileArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* lds = reinterpret_cast<T*>(lds_raw);
     const uint32_t tsz = 1u << a.t;
     const uint32_t y = threadIdx.x;
     const uint64_t base = a.fixval;   // one tile: every other state bit is fixed by the pass
     const uint64_t off = base | deposit(y, a.tmask);
-    double vr = re[off], vi = im[off];
+    T vr = re[off], vi = im[off];
     TileGate* sg = reinterpret_cast<TileGate*>(lds + 4 * (size_t)tsz);   // after the exchange buffers
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(&a.g[0]);
@@ -429,27 +470,33 @@ __global__ __launch_bounds__(1024) void k_tile_tiny(double* __restrict__ re, dou
         g.mo = uni64(sg[k].mo);
         g.vo = uni64(sg[k].vo);
         g.yzo = uni64(sg[k].yzo);
-        g.c = unid(sg[k].c);
-        g.s = unid(sg[k].s);
+        if constexpr (std::is_same<T, double>::value) {
+            g.c = unid(sg[k].c);
+            g.s = unid(sg[k].s);
+        } else {
+            g.cf = unif(sg[k].cf);
+            g.sf = unif(sg[k].sf);
+        }
         if ((base & g.mo) != g.vo) continue;   // uniform: the tile is outside the gate's selection
         const uint32_t opar = (uint32_t)__popcll(base & g.yzo) & 1u;
         if (g.kind == (uint32_t)kPhaseOp) {
             if ((y & g.mc) == g.vc) {
                 const uint32_t par = ((uint32_t)__popc(y & g.yzc) & 1u) ^ opar;
-                const double sn = par ? -g.s : g.s;
-                const double rr = vr, mm = vi;
-                vr = phase_out(false, rr, mm, g.c, sn);
-                vi = phase_out(true, mm, rr, g.c, sn);
+                const T s = gate_s<T>(g);
+                const T sn = par ? -s : s;
+                const T rr = vr, mm = vi;
+                vr = phase_out(false, rr, mm, gate_c<T>(g), sn);
+                vi = phase_out(true, mm, rr, gate_c<T>(g), sn);
             }
             continue;
         }
         const uint32_t w = y ^ g.xc;   // every thread takes part in the exchange
-        double pr, pi;
+        T pr, pi;
         if (g.xc < 64u) {
             pr = __shfl_xor(vr, (int)g.xc, 64);
             pi = __shfl_xor(vi, (int)g.xc, 64);
         } else {   // the previous use of this buffer ended before the last exchange's barrier
-            double* bx = lds + 2 * tsz * xb;
+            T* bx = lds + 2 * tsz * xb;
             bx[y] = vr;
             bx[tsz + y] = vi;
             __syncthreads();
@@ -464,14 +511,8 @@ __global__ __launch_bounds__(1024) void k_tile_tiny(double* __restrict__ re, dou
         if (!su && (uw & g.mc) != g.vc) continue;
         const bool is_a = y == (su ? u : uw);
         const uint32_t par = ((uint32_t)__popc(w & g.yzc) & 1u) ^ opar;   // parity of the partner
-        PauliCoef pc;
-        pc.c = g.c;
-        pc.s = g.s;
-        pc.yz = 0;
-        pc.ny_odd = 0;
-        pc.base_imag = g.base_imag;
-        pc.base_neg = g.base_neg;
-        double nr, ni;
+        const Coef<T> pc = coef<T>(g);
+        T nr, ni;
         switch (g.kind) {
         case OP_SWAP:
             nr = pair_out_par<OP_SWAP>(is_a, false, vr, pr, pi, par, pc);
@@ -588,6 +629,8 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
             g.base_imag = d.pc.base_imag;
             g.base_neg = d.pc.base_neg;
         }
+        g.cf = (float)g.c;
+        g.sf = (float)g.s;
         // bits outside the tile that every gate fixes to the same value
         if (i == 0) vcom = g.vo;
         mcom &= g.mo;
@@ -645,15 +688,25 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     }
     if (n > g0) close(n, false);
     const uint64_t g8 = ((a.ntiles + 7) / 8) * 8;
-    const size_t lds = sizeof(double) * 2 * ((size_t)1 << t);
+    const size_t es = c->f32 ? sizeof(float) : sizeof(double);
+    const size_t lds = es * 2 * ((size_t)1 << t);
     ++c->passes;
     c->pass_gates += n;
-    timing_begin(c, "tile_pass", 32.0 * (double)((uint64_t)1 << t) * (double)a.ntiles);
-    if (tiny)
-        hipLaunchKernelGGL(k_tile_tiny, dim3(1), dim3(1u << t), 2 * lds + sizeof(TileGate) * kMaxPass, c->stream,
-                           c->d_re, c->d_im, a);
-    else
-        hipLaunchKernelGGL(k_tile, dim3((uint32_t)g8), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a);
+    // algorithmic bytes: re and im of every tile amplitude read and written
+    timing_begin(c, c->f32 ? "tile_pass32" : "tile_pass", 4.0 * (double)es * (double)((uint64_t)1 << t) * (double)a.ntiles);
+    if (c->f32) {
+        if (tiny)
+            hipLaunchKernelGGL(k_tile_tiny<float>, dim3(1), dim3(1u << t), 2 * lds + sizeof(TileGate) * kMaxPass,
+                               c->stream, c->f32_re, c->f32_im, a);
+        else
+            hipLaunchKernelGGL(k_tile<float>, dim3((uint32_t)g8), dim3(kBlock), lds, c->stream, c->f32_re, c->f32_im, a);
+    } else {
+        if (tiny)
+            hipLaunchKernelGGL(k_tile_tiny<double>, dim3(1), dim3(1u << t), 2 * lds + sizeof(TileGate) * kMaxPass,
+                               c->stream, c->d_re, c->d_im, a);
+        else
+            hipLaunchKernelGGL(k_tile<double>, dim3((uint32_t)g8), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a);
+    }
     timing_end(c);
     return hipGetLastError();
 }
@@ -678,7 +731,13 @@ hipError_t flush(suqa_hip_ctx* c) {
             ++j;
         }
         hipError_t e;
-        if (j - i <= 1 || t < kSub) {   // a lone gate, one whose partner bits exceed the tile, or a tiny state
+        if (c->f32) {   // float states have no one-gate sweeps: a lone gate is a pass of one
+            if (j == i) {
+                set_last_error("a gate's partner qubits do not fit the FP32 pass tile");
+                return hipErrorInvalidValue;
+            }
+            e = run_pass(c, q.data() + i, (uint32_t)(j - i), T, t);
+        } else if (j - i <= 1 || t < kSub) {   // a lone gate, one whose partner bits exceed the tile, or a tiny state
             e = run_single(c, q[i]);
             j = i + 1;
         } else {

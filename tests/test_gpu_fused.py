@@ -1,8 +1,9 @@
 """The precision pattern on the GPU: FP32 state, gates fused into 4..6-qubit
-blocks applied by FP32 MFMA, FP64 norms and probabilities.  Compared with the
-FP64 engine (itself bit-exact with the reference) and with the oracle.
+blocks applied by FP32 MFMA (modes 0-2), or every gate applied on its own in
+FP32 VALU tile passes (mode 3), FP64 norms and probabilities.  Compared with
+the FP64 engine (itself bit-exact with the reference) and with the oracle.
 Tolerances: amplitudes within 2e-6 absolute at n <= 14 (FP32 rounding of
-~50 fused blocks of unit-norm columns), fidelity
+~50 fused blocks of unit-norm columns, or of ~350 single gates), fidelity
 |<psi64|psi32>|^2 / (<psi64|psi64><psi32|psi32>) >= 1 - 1e-6,
 norm and probabilities within 1e-5."""
 import sys
@@ -32,10 +33,14 @@ def circuit(n, layers, seed):
     return [wl.G(wl.H, q, [], 0.0) for q in range(n)] + gates
 
 
-@pytest.mark.parametrize("passes", [False, True], ids=["blocks", "passes"])
+MODES = [(0, 12), (1, 12), (3, 12), (3, 13)]
+MODE_IDS = ["blocks", "passes", "gates12", "gates13"]
+
+
+@pytest.mark.parametrize("passes,tile_bits", MODES, ids=MODE_IDS)
 @pytest.mark.parametrize("n,width", [(9, 4), (10, 5), (11, 6), (12, 4), (13, 5), (14, 6)])
-def test_fused_vs_fp64_and_oracle(capi, n, width, passes):
-    log(f"fused n={n} width={width} passes={passes}")
+def test_fused_vs_fp64_and_oracle(capi, n, width, passes, tile_bits):
+    log(f"fused n={n} width={width} passes={passes} tile_bits={tile_bits}")
     allg = circuit(n, 24, 11 + n)
     arr = capi.gates_from(allg)
     e64 = capi.Engine(n)
@@ -44,7 +49,8 @@ def test_fused_vs_fp64_and_oracle(capi, n, width, passes):
     r64, i64 = e64.get()
     f32 = capi.F32Engine(n)
     f32.passes = passes
-    assert f32.passes == passes
+    f32.tile_bits = tile_bits
+    assert f32.passes == passes and f32.tile_bits == tile_bits
     f32.init_state()
     nb = f32.run_gates(arr, width)
     assert 0 < nb < len(allg)
@@ -74,7 +80,7 @@ def test_fused_vs_fp64_and_oracle(capi, n, width, passes):
     e64.close()
 
 
-@pytest.mark.parametrize("passes", [False, True], ids=["blocks", "passes"])
+@pytest.mark.parametrize("passes", [0, 1, 3], ids=["blocks", "passes", "gates"])
 def test_fused_measure_matches_fp64(capi, passes):
     n = 12
     log(f"fused measure n={n} passes={passes}")
@@ -115,7 +121,17 @@ def test_fused_rejects_bad_arguments(capi):
     with pytest.raises(capi.SuqaError):
         f32.measure(10, 0.5)
     with pytest.raises(capi.SuqaError):
-        capi.check32(capi.lib().suqa_hip_f32_set_passes(f32.ctx, 3))   # modes are 0, 1 and 2
+        capi.check32(capi.lib().suqa_hip_f32_set_passes(f32.ctx, 4))   # modes are 0..3
+    for bits in (11, 14):
+        with pytest.raises(capi.SuqaError):
+            f32.tile_bits = bits                                       # gate-pass tiles are 12 or 13 qubits
+    f32.passes = 3
+    f32.run_gates(arr, 0)                                              # mode 3 ignores the width
+    with pytest.raises(capi.SuqaError):                                # a repeated qubit
+        f32.run_paulis(capi.pauli_ops([([2, 2], [1, 1], 0.1, 0)]), 0)
+    with pytest.raises(capi.SuqaError):                                # control on a qubit of the string
+        f32.run_paulis(capi.pauli_ops([([2, 3], [1, 3], 0.1, 1 << 3)]), 0)
+    assert abs(f32.vnorm() - 1.0) < 1e-5                               # nothing of the failed calls applied
     e64 = capi.Engine(11)
     with pytest.raises(capi.SuqaError):
         f32.fidelity(e64)
@@ -141,7 +157,7 @@ def test_fused_full_size_fidelity(capi):
     f32 = capi.F32Engine(n)
     f32.init_state()
     had = capi.gates_from([wl.G(wl.H, q, [], 0.0) for q in range(n)])
-    for passes in (False, True):
+    for passes in (0, 1, 3):
         f32.passes = passes
         f32.init_state()
         f32.run_gates(had, 6)
@@ -151,9 +167,53 @@ def test_fused_full_size_fidelity(capi):
         assert F > 1 - 1e-5
         assert abs(f32.vnorm() - 1.0) < 1e-5
     best, t = f32.autotune(arr)
-    log(f"autotune {best} passes={f32.passes} {t}")
-    assert best in (4, 5, 6) and len(t) == 6 and all(v > 0 for v in t.values())
-    mode = "passes" if f32.passes else "blocks"
-    assert t[f"{mode}{best}"] == min(t.values())
+    log(f"autotune {best} passes={f32.passes} tile_bits={f32.tile_bits} {t}")
+    assert best in (4, 5, 6) and len(t) == 8 and all(v > 0 for v in t.values())
+    blocks = {k: v for k, v in t.items() if not k.startswith("gates")}
+    assert min(blocks, key=blocks.get).endswith(str(best))
+    key = f"gates{f32.tile_bits}" if f32.passes == 3 else f"{['blocks', 'passes'][f32.passes]}{best}"
+    assert t[key] == min(t.values())
+    f32.close()
+    e64.close()
+
+
+@pytest.mark.parametrize("tile_bits", [12, 13])
+@pytest.mark.parametrize("n", [10, 14])
+def test_gate_passes_paulis_and_qft(capi, n, tile_bits):
+    """Mode 3 on the ops of the QPE workload: controlled Pauli rotations of
+    weight 1..8 (controls anywhere, qubits 0 and 1 included) and the
+    reference's inverse QFT, against the FP64 engine's run_paulis and
+    apply_qft_inverse on the same list (amplitudes within 2e-6)."""
+    log(f"gate passes paulis + qft n={n} tile_bits={tile_bits}")
+    rng = np.random.default_rng(5 + n)
+    ops = []
+    for k in range(60):
+        w = int(rng.integers(1, 9))
+        qs = sorted(int(q) for q in rng.choice(n - 2, size=w, replace=False))
+        free = [q for q in range(n) if q not in qs]
+        gc = 0
+        for q in rng.choice(free, size=int(rng.integers(0, 3)), replace=False):
+            gc |= 1 << int(q)
+        ops.append((qs, [int(t) for t in rng.integers(1, 4, size=w)], float(rng.uniform(-1, 1)), gc))
+    had = capi.gates_from([wl.G(wl.H, q, [], 0.0) for q in range(n)])
+    reg = [n - 3, n - 2, n - 1]
+    e64 = capi.Engine(n)
+    e64.init_state()
+    e64.run_gates(had)
+    e64.run_paulis(capi.pauli_ops(ops))
+    e64.qft_inverse(reg)
+    r64, i64 = e64.get()
+    f32 = capi.F32Engine(n)
+    f32.passes = 3
+    f32.tile_bits = tile_bits
+    f32.init_state()
+    f32.run_gates(had, 0)
+    npass = f32.run_paulis(capi.pauli_ops(ops), 0)
+    assert 0 < npass < len(ops)
+    f32.qft_inverse(reg, 0)
+    r32, i32 = f32.get()
+    np.testing.assert_allclose(r32, r64, atol=2e-6, rtol=0)
+    np.testing.assert_allclose(i32, i64, atol=2e-6, rtol=0)
+    assert f32.fidelity(e64) > 1 - 1e-6
     f32.close()
     e64.close()

@@ -226,23 +226,45 @@ def test_every_gate_kind_in_passes_matches_one_sweep_per_gate(capi, n, bits, see
     tiled.close()
 
 
+def qpe_10_4(capi, bits, reorder):
+    nsys, nreg = 10, 4
+    terms = wl.trotter_terms(nsys, 96, 11)
+    os.environ["SUQA_HIP_PAULI_ORDER"] = "1" if reorder else "0"
+    try:
+        e = engine(capi, nsys + nreg, bits)
+    finally:
+        del os.environ["SUQA_HIP_PAULI_ORDER"]
+    e.init_state()
+    h = wl.qpe_trotter(e, nsys, nreg, terms, 0.05, 2)
+    out = (h, e.get())
+    e.close()
+    return out
+
+
 @pytest.mark.parametrize("bits", [8, 12])
 def test_qpe_in_passes_matches_one_sweep_per_gate(capi, bits):
     """Controlled Pauli rotations whose control lies outside the tile (8 bits)
-    or inside it (12 bits); the FP64 QPE histogram and state are identical."""
-    nsys, nreg = 10, 4
-    log(f"qpe 10+4 tile bits {bits}")
-    terms = wl.trotter_terms(nsys, 96, 11)
-    out = []
-    for b in (0, bits):
-        e = engine(capi, nsys + nreg, b)
-        e.init_state()
-        h = wl.qpe_trotter(e, nsys, nreg, terms, 0.05, 2)
-        out.append((h, e.get()))
-        e.close()
-    assert out[0][0] == out[1][0]
-    assert_bitwise(out[1][1][0], out[0][1][0], "re")
-    assert_bitwise(out[1][1][1], out[0][1][1], "im")
+    or inside it (12 bits), in list order (SUQA_HIP_PAULI_ORDER=0): the FP64
+    QPE histogram and state are identical."""
+    log(f"qpe 10+4 tile bits {bits}, list order")
+    ref = qpe_10_4(capi, 0, False)
+    got = qpe_10_4(capi, bits, False)
+    assert got[0] == ref[0]
+    assert_bitwise(got[1][0], ref[1][0], "re")
+    assert_bitwise(got[1][1], ref[1][1], "im")
+
+
+@pytest.mark.parametrize("bits", [8, 12])
+def test_qpe_in_reordered_passes_within_fp64_rounding(capi, bits):
+    """run_paulis' pass packing (the default) moves commuting rotations ahead:
+    the product is the same operator, the rounding differs, so the state and
+    histogram agree with list order to 1e-12 (not bit for bit)."""
+    log(f"qpe 10+4 tile bits {bits}, reordered")
+    ref = qpe_10_4(capi, 0, False)
+    got = qpe_10_4(capi, bits, True)
+    np.testing.assert_allclose(got[0], ref[0], atol=1e-12, rtol=0)
+    for k in (0, 1):
+        np.testing.assert_allclose(got[1][k], ref[1][k], atol=1e-12, rtol=0)
 
 
 @pytest.mark.parametrize("fx", QMS_FIXTURES[:2])
