@@ -171,92 +171,227 @@ struct RM {
 // Register steps compute every element and pick results with selects, never
 // with branches on data: a branch around an update makes the compiler copy
 // the whole register set at the join.
+//
+// Every arm updates the 8 amplitudes in place: each output is defined after
+// the last use of the input it replaces (products and sums of all inputs
+// first, then the final operation of each output, or a select whose other
+// operand is that input).  The dispatch's join then needs no copy: the
+// results are already in the registers the next gate reads.  One arm that
+// computes an output while its input is still live makes the compiler keep
+// two register sets and copy all 16 amplitudes after every gate.
 
-// Pair gate with local partner mask XL: pair (J, J ^ XL) for each J without
-// the lowest bit of XL, expanded at compile time.  DENSE: every pair of the
-// group is selected (no test at all).  Otherwise the roles come from the
-// uniform table lsel, gpass (this group meets the selection's other bits)
-// and gpar (parity of the other bits) are per thread.
-template <int XL, int OP, bool DENSE, int J, typename T>
-__device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, const Coef<T>& pc, bool gpass,
-                                              uint32_t gpar) {
+// A copy of v in a new register (passed-through values in the select arms).
+__device__ __forceinline__ float fresh(float v) {
+    float r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ double fresh(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32), rl, rh;
+    asm("v_mov_b32 %0, %1" : "=v"(rl) : "v"(lo));
+    asm("v_mov_b32 %0, %1" : "=v"(rh) : "v"(hi));
+    return __longlong_as_double((long long)(((uint64_t)rh << 32) | rl));
+}
+// Makes the values exist before anything that uses them is computed (no
+// instruction: an empty asm reading and writing them).  The arms compute
+// every partial result from the inputs first and pass them through here, so
+// the outputs, computed from the barrier's results, are defined after the
+// inputs' last uses whatever order the scheduler picks.
+template <typename T>
+__device__ __forceinline__ void settle(T& a, T& b, T& c, T& d) {
+    asm volatile("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
+}
+
+template <typename T>
+__device__ __forceinline__ void settle6(T& a, T& b, T& c, T& d, T& e, T& f) {
+    asm volatile("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d), "+v"(e), "+v"(f));
+}
+
+// Exchanges a and b in their registers.
+__device__ __forceinline__ void vswap(float& a, float& b) { asm("v_swap_b32 %0, %1" : "+v"(a), "+v"(b)); }
+__device__ __forceinline__ void vswap(double& a, double& b) {
+    const uint64_t ua = (uint64_t)__double_as_longlong(a), ub = (uint64_t)__double_as_longlong(b);
+    uint32_t al = (uint32_t)ua, ah = (uint32_t)(ua >> 32), bl = (uint32_t)ub, bh = (uint32_t)(ub >> 32);
+    asm("v_swap_b32 %0, %1" : "+v"(al), "+v"(bl));
+    asm("v_swap_b32 %0, %1" : "+v"(ah), "+v"(bh));
+    a = __longlong_as_double((long long)(((uint64_t)ah << 32) | al));
+    b = __longlong_as_double((long long)(((uint64_t)bh << 32) | bl));
+}
+
+// Pair operations of the register arms (OPV): the four pair kinds, with the
+// Pauli rotation split by its base (real: psi' = c psi + t psi_partner,
+// imaginary: the partner's other component, re with -t, im with +t).
+constexpr int kSwap = 0, kY = 1, kH = 2, kPauliR = 3, kPauliI = 4;
+// Selection modes: every pair (kDense), the pairs whose element u has tile
+// bit CB equal to CV (kHalf + 2 CB + CV, no other selection), or per element
+// from lsel and gpass (kGeneral).
+constexpr int kDense = 0, kGeneral = 1, kHalf = 2;
+
+// Pauli sign: t = (base_neg ^ parity) ? -s : s; tg, ntg are the signed s and
+// its negation for a parity-0 element of this group, flipped where the
+// element's own bits on F have odd parity (lpar).
+template <typename T>
+__device__ __forceinline__ T pauli_t(const TileGate& g, int elem, T tg, T ntg) {
+    return ((g.lpar >> elem) & 1u) ? ntg : tg;
+}
+
+// Pair (u, u ^ XL) of a register step, u = J when J lacks the lowest bit of
+// XL.  Dense pairs are updated in place; general pairs compute their outputs
+// first and select them per element.  Outputs follow pair_out_par exactly
+// (same operations, same order), so FP64 results stay bit-identical.
+template <int OPV, int XL, int SEL, int J, typename T>
+__device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, T c, T tg, T ntg, bool act) {
     constexpr int PIV = XL & -XL;
     if constexpr ((J & PIV) == 0) {
         constexpr int u = J, w = J ^ XL;
-        const T ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
-        if constexpr (XL == PIV) {
-            // one flipped bit: gates of this kind select their pivot bit at 0 (the
-            // host checks it), so the lower element u is the reference's i
-            const uint32_t par_u = OP == OP_PAULI ? (((g.lpar >> w) & 1u) ^ gpar) : 0u;
-            const uint32_t par_w = OP == OP_PAULI ? (((g.lpar >> u) & 1u) ^ gpar) : 0u;
-            const T nru = pair_out_par<OP>(true, false, ru, rw, mw, par_u, pc);
-            const T nmu = pair_out_par<OP>(true, true, mu, mw, rw, par_u, pc);
-            const T nrw = pair_out_par<OP>(false, false, rw, ru, mu, par_w, pc);
-            const T nmw = pair_out_par<OP>(false, true, mw, mu, ru, par_w, pc);
-            if constexpr (DENSE) {
-                x.r[u] = nru;
-                x.m[u] = nmu;
-                x.r[w] = nrw;
-                x.m[w] = nmw;
+        constexpr bool dense = SEL == kDense || (SEL >= kHalf && (((u >> ((SEL - kHalf) >> 1)) & 1) == ((SEL - kHalf) & 1)));
+        if constexpr (SEL >= kHalf && !dense) return x;   // this pair is outside the half selection
+        T ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
+        constexpr T kh = (T)kTwoSqInv;
+        if constexpr (dense) {
+            if constexpr (OPV == kSwap) {
+                vswap(ru, rw);
+                vswap(mu, mw);
+            } else if constexpr (OPV == kY) {   // re_u' = im_w, im_w' = re_u, im_u' = -re_w, re_w' = -im_u
+                vswap(ru, mw);
+                vswap(mu, rw);
+                mu = mu * (T)-1.0;
+                rw = rw * (T)-1.0;
+            } else if constexpr (OPV == kH) {   // u: k (u + w); w: k (u - w)
+                T sr = ru + rw, sm = mu + mw, dr = ru - rw, dm = mu - mw;
+                settle(sr, sm, dr, dm);
+                ru = kh * sr;
+                mu = kh * sm;
+                rw = kh * dr;
+                mw = kh * dm;
             } else {
-                const bool on = ((g.lsel >> u) & 1u) && (g.gtriv || gpass);
-                x.r[u] = on ? nru : ru;
-                x.m[u] = on ? nmu : mu;
-                x.r[w] = on ? nrw : rw;
-                x.m[w] = on ? nmw : mw;
+                const T tu = pauli_t(g, w, tg, ntg), tw = pauli_t(g, u, tg, ntg);
+                T pru = ru * c, pmu = mu * c, prw = rw * c, pmw = mw * c;
+                T qru, qmu, qrw, qmw;
+                if constexpr (OPV == kPauliR) {
+                    qru = rw * tu;
+                    qmu = mw * tu;
+                    qrw = ru * tw;
+                    qmw = mu * tw;
+                } else {
+                    qru = mw * (-tu);
+                    qmu = rw * tu;
+                    qrw = mu * (-tw);
+                    qmw = ru * tw;
+                }
+                settle(pru, pmu, prw, pmw);
+                settle(qru, qmu, qrw, qmw);
+                ru = pru + qru;
+                mu = pmu + qmu;
+                rw = prw + qrw;
+                mw = pmw + qmw;
             }
-        } else {
-            // several flipped bits (SWAP, Pauli strings): either element may be i
-            const bool su = (g.lsel >> u) & 1u;
-            const bool sw = !su && ((g.lsel >> w) & 1u);
-            const bool on = (su || sw) && (g.gtriv || gpass);
-            const uint32_t par_u = OP == OP_PAULI ? (((g.lpar >> w) & 1u) ^ gpar) : 0u;
-            const uint32_t par_w = OP == OP_PAULI ? (((g.lpar >> u) & 1u) ^ gpar) : 0u;
-            const T nru = pair_out_par<OP>(su, false, ru, rw, mw, par_u, pc);
-            const T nmu = pair_out_par<OP>(su, true, mu, mw, rw, par_u, pc);
-            const T nrw = pair_out_par<OP>(sw, false, rw, ru, mu, par_w, pc);
-            const T nmw = pair_out_par<OP>(sw, true, mw, mu, ru, par_w, pc);
-            x.r[u] = on ? nru : ru;
-            x.m[u] = on ? nmu : mu;
-            x.r[w] = on ? nrw : rw;
-            x.m[w] = on ? nmw : mw;
+        } else {   // kGeneral
+            // no short-circuit: a branch here would split the arm into blocks
+            bool on;
+            if constexpr (XL == PIV) {
+                on = (((g.lsel >> u) & 1u) != 0u) & act;
+            } else {   // several flipped bits (SWAP, Pauli strings): either element may be selected
+                on = ((((g.lsel >> u) | (g.lsel >> w)) & 1u) != 0u) & act;
+            }
+            T cru, cmu, crw, cmw;
+            if constexpr (OPV == kSwap) {
+                const T tr = fresh(ru), tm = fresh(mu);
+                ru = on ? rw : tr;
+                rw = on ? tr : rw;
+                mu = on ? mw : tm;
+                mw = on ? tm : mw;
+            } else {
+                if constexpr (OPV == kY) {
+                    cru = fresh(mw);
+                    cmw = fresh(ru);
+                    cmu = rw * (T)-1.0;
+                    crw = mu * (T)-1.0;
+                } else if constexpr (OPV == kH) {
+                    cru = kh * (ru + rw);
+                    cmu = kh * (mu + mw);
+                    crw = kh * (ru - rw);
+                    cmw = kh * (mu - mw);
+                } else {
+                    const T tu = pauli_t(g, w, tg, ntg), tw = pauli_t(g, u, tg, ntg);
+                    if constexpr (OPV == kPauliR) {
+                        cru = ru * c + rw * tu;
+                        cmu = mu * c + mw * tu;
+                        crw = rw * c + ru * tw;
+                        cmw = mw * c + mu * tw;
+                    } else {
+                        cru = ru * c + mw * (-tu);
+                        cmu = mu * c + rw * tu;
+                        crw = rw * c + mu * (-tw);
+                        cmw = mw * c + ru * tw;
+                    }
+                }
+                settle(cru, cmu, crw, cmw);
+                ru = on ? cru : ru;
+                mu = on ? cmu : mu;
+                rw = on ? crw : rw;
+                mw = on ? cmw : mw;
+            }
         }
+        x.r[u] = ru;
+        x.m[u] = mu;
+        x.r[w] = rw;
+        x.m[w] = mw;
     }
     return x;
 }
 
-template <int XL, int OP, bool DENSE, typename T, int... J>
+template <int OPV, int XL, int SEL, typename T, int... J>
 __device__ __forceinline__ RM<T> reg_pairs_seq(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar,
                                                std::integer_sequence<int, J...>) {
-    const Coef<T> pc = coef<T>(g);
-    ((x = reg_pair_one<XL, OP, DENSE, J>(x, g, pc, gpass, gpar)), ...);
+    T c = (T)0, tg = (T)0, ntg = (T)0;
+    if constexpr (OPV == kPauliR || OPV == kPauliI) {
+        c = gate_c<T>(g);
+        const T s = gate_s<T>(g);
+        tg = (g.base_neg ^ gpar) ? -s : s;
+        ntg = -tg;
+    }
+    const bool act = (g.gtriv != 0u) | gpass;   // this group meets the selection's bits off F
+    ((x = reg_pair_one<OPV, XL, SEL, J>(x, g, c, tg, ntg, act)), ...);
     return x;
 }
 
-// Phase rotation of the selected register amplitudes.
-template <bool DENSE, int J, typename T>
-__device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar) {
-    const uint32_t par = ((g.lpar >> J) & 1u) ^ gpar;
-    const T s = gate_s<T>(g);
-    const T sn = par ? -s : s;
+// Phase rotation of the register amplitudes (util_rotate2 on each selected
+// one): dense (all 8), half (the elements whose bit B equals V), general.
+template <int SEL, int J, typename T>
+__device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const TileGate& g, T c, T sg, T nsg, bool act) {
+    constexpr bool half_on = SEL >= kHalf && (((J >> ((SEL - kHalf) >> 1)) & 1) == ((SEL - kHalf) & 1));
+    if constexpr (SEL >= kHalf && !half_on) return x;
+    const T sn = ((g.lpar >> J) & 1u) ? nsg : sg;
     const T rr = x.r[J], mm = x.m[J];
-    const T nr = phase_out(false, rr, mm, gate_c<T>(g), sn);
-    const T nm = phase_out(true, mm, rr, gate_c<T>(g), sn);
-    if constexpr (DENSE) {
-        x.r[J] = nr;
-        x.m[J] = nm;
-    } else {
-        const bool on = ((g.lsel >> J) & 1u) && (g.gtriv || gpass);
+    T a = rr * c, b = mm * sn, cc = mm * c, d = rr * sn;
+    // the next element's inputs pass the same barrier: its products start
+    // after this element's, which bounds the registers in flight
+    constexpr int N = (J + 1) & 7;
+    T nr0 = x.r[N], nm0 = x.m[N];
+    settle6(a, b, cc, d, nr0, nm0);
+    x.r[N] = nr0;
+    x.m[N] = nm0;
+    if constexpr (SEL == kGeneral) {
+        const bool on = (((g.lsel >> J) & 1u) != 0u) & act;
+        const T nr = a - b, nm = cc + d;
         x.r[J] = on ? nr : rr;
         x.m[J] = on ? nm : mm;
+    } else {
+        x.r[J] = a - b;
+        x.m[J] = cc + d;
     }
     return x;
 }
 
-template <bool DENSE, typename T, int... J>
+template <int SEL, typename T, int... J>
 __device__ __forceinline__ RM<T> reg_phase_seq(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar,
                                                std::integer_sequence<int, J...>) {
-    ((x = reg_phase_one<DENSE, J>(x, g, gpass, gpar)), ...);
+    const T c = gate_c<T>(g), s = gate_s<T>(g);
+    const T sg = gpar ? -s : s, nsg = -sg;
+    const bool act = (g.gtriv != 0u) | gpass;
+    ((x = reg_phase_one<SEL, J>(x, g, c, sg, nsg, act)), ...);
     return x;
 }
 
@@ -264,37 +399,69 @@ __device__ __forceinline__ RM<T> reg_phase_seq(RM<T> x, const TileGate& g, bool 
 // host computes (local_tables): nested switches with a fall-through from the
 // dense cases made the compiler copy the 16 amplitudes into the result
 // registers before dispatching, on top of the copy back at the join.
-// Arms: 0/1 phase dense/general; 2 + 10 op + {0, 1, 2}: pair op with a
-// one-bit partner mask 1, 2, 4, dense; 2 + 10 op + 3 + (xl - 1): general.
 // Every arm is its own case: an earlier dense dispatch written as `case 1`,
 // `case 2`, `default` (= 4) was miscompiled by hipcc (ROCm 7.2), which merged
 // the OP_Y cases and left half of the xl = 4 outputs unassigned.
-#define SQE_PAIR_ARMS(OPV)                                                                                    \
-    case 2 + 10 * OPV + 0: return reg_pairs_seq<1, OPV, true>(x, g, gpass, gpar, seq);                        \
-    case 2 + 10 * OPV + 1: return reg_pairs_seq<2, OPV, true>(x, g, gpass, gpar, seq);                        \
-    case 2 + 10 * OPV + 2: return reg_pairs_seq<4, OPV, true>(x, g, gpass, gpar, seq);                        \
-    case 2 + 10 * OPV + 3: return reg_pairs_seq<1, OPV, false>(x, g, gpass, gpar, seq);                       \
-    case 2 + 10 * OPV + 4: return reg_pairs_seq<2, OPV, false>(x, g, gpass, gpar, seq);                       \
-    case 2 + 10 * OPV + 5: return reg_pairs_seq<3, OPV, false>(x, g, gpass, gpar, seq);                       \
-    case 2 + 10 * OPV + 6: return reg_pairs_seq<4, OPV, false>(x, g, gpass, gpar, seq);                       \
-    case 2 + 10 * OPV + 7: return reg_pairs_seq<5, OPV, false>(x, g, gpass, gpar, seq);                       \
-    case 2 + 10 * OPV + 8: return reg_pairs_seq<6, OPV, false>(x, g, gpass, gpar, seq);                       \
-    case 2 + 10 * OPV + 9: return reg_pairs_seq<7, OPV, false>(x, g, gpass, gpar, seq);
+// Arms (kArm* below): phase dense / general / half (bit b, value v); per pair
+// kind: dense one-bit partner masks 1, 2, 4; general one-bit masks; general
+// multi-bit masks 3, 5, 6, 7 (SWAP and Pauli only); for SWAP also half arms
+// (one-bit mask, control bit cb of F at value cv).
+constexpr uint32_t kArmPhaseDense = 0, kArmPhaseGeneral = 1, kArmPhaseHalf = 2;   // + 2 b + v
+constexpr uint32_t kArmPair = 8;              // + 16 OPV + ...
+constexpr uint32_t kArmPairDense = 0;         // + log2 xl
+constexpr uint32_t kArmPairGeneral1 = 3;      // + log2 xl
+constexpr uint32_t kArmPairMulti = 6;         // + {3: 0, 5: 1, 6: 2, 7: 3}
+constexpr uint32_t kArmSwapHalf = kArmPair + 16 * 5;   // + 4 log2 xl + 2 (cb index among the other two bits) + cv
+
+#define SQE_PAIR_ARMS(OPV)                                                                                   \
+    case kArmPair + 16 * OPV + 0: return reg_pairs_seq<OPV, 1, kDense>(x, g, gpass, gpar, seq);              \
+    case kArmPair + 16 * OPV + 1: return reg_pairs_seq<OPV, 2, kDense>(x, g, gpass, gpar, seq);              \
+    case kArmPair + 16 * OPV + 2: return reg_pairs_seq<OPV, 4, kDense>(x, g, gpass, gpar, seq);              \
+    case kArmPair + 16 * OPV + 3: return reg_pairs_seq<OPV, 1, kGeneral>(x, g, gpass, gpar, seq);            \
+    case kArmPair + 16 * OPV + 4: return reg_pairs_seq<OPV, 2, kGeneral>(x, g, gpass, gpar, seq);            \
+    case kArmPair + 16 * OPV + 5: return reg_pairs_seq<OPV, 4, kGeneral>(x, g, gpass, gpar, seq);
+#define SQE_MULTI_ARMS(OPV)                                                                                  \
+    case kArmPair + 16 * OPV + 6: return reg_pairs_seq<OPV, 3, kGeneral>(x, g, gpass, gpar, seq);            \
+    case kArmPair + 16 * OPV + 7: return reg_pairs_seq<OPV, 5, kGeneral>(x, g, gpass, gpar, seq);            \
+    case kArmPair + 16 * OPV + 8: return reg_pairs_seq<OPV, 6, kGeneral>(x, g, gpass, gpar, seq);            \
+    case kArmPair + 16 * OPV + 9: return reg_pairs_seq<OPV, 7, kGeneral>(x, g, gpass, gpar, seq);
+// SWAP half arms: partner mask XL, control bit CB (the other two F bits) at CV
+#define SQE_SWAP_HALF(LXL, XL, K, CB)                                                                        \
+    case kArmSwapHalf + 4 * LXL + 2 * K + 0: return reg_pairs_seq<kSwap, XL, kHalf + 2 * CB + 0>(x, g, gpass, gpar, seq); \
+    case kArmSwapHalf + 4 * LXL + 2 * K + 1: return reg_pairs_seq<kSwap, XL, kHalf + 2 * CB + 1>(x, g, gpass, gpar, seq);
 
 template <typename T>
 __device__ __forceinline__ RM<T> reg_gate(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar) {
     constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
     switch (g.arm) {
-    case 0: return reg_phase_seq<true>(x, g, gpass, gpar, seq);
-    case 1: return reg_phase_seq<false>(x, g, gpass, gpar, seq);
-    SQE_PAIR_ARMS(OP_SWAP)
-    SQE_PAIR_ARMS(OP_Y)
-    SQE_PAIR_ARMS(OP_H)
-    SQE_PAIR_ARMS(OP_PAULI)
+    case kArmPhaseDense: return reg_phase_seq<kDense>(x, g, gpass, gpar, seq);
+    case kArmPhaseGeneral: return reg_phase_seq<kGeneral>(x, g, gpass, gpar, seq);
+    case kArmPhaseHalf + 0: return reg_phase_seq<kHalf + 0>(x, g, gpass, gpar, seq);
+    case kArmPhaseHalf + 1: return reg_phase_seq<kHalf + 1>(x, g, gpass, gpar, seq);
+    case kArmPhaseHalf + 2: return reg_phase_seq<kHalf + 2>(x, g, gpass, gpar, seq);
+    case kArmPhaseHalf + 3: return reg_phase_seq<kHalf + 3>(x, g, gpass, gpar, seq);
+    case kArmPhaseHalf + 4: return reg_phase_seq<kHalf + 4>(x, g, gpass, gpar, seq);
+    case kArmPhaseHalf + 5: return reg_phase_seq<kHalf + 5>(x, g, gpass, gpar, seq);
+    SQE_PAIR_ARMS(kSwap)
+    SQE_MULTI_ARMS(kSwap)
+    SQE_PAIR_ARMS(kY)
+    SQE_PAIR_ARMS(kH)
+    SQE_PAIR_ARMS(kPauliR)
+    SQE_MULTI_ARMS(kPauliR)
+    SQE_PAIR_ARMS(kPauliI)
+    SQE_MULTI_ARMS(kPauliI)
+    SQE_SWAP_HALF(0, 1, 0, 1)
+    SQE_SWAP_HALF(0, 1, 1, 2)
+    SQE_SWAP_HALF(1, 2, 0, 0)
+    SQE_SWAP_HALF(1, 2, 1, 2)
+    SQE_SWAP_HALF(2, 4, 0, 0)
+    SQE_SWAP_HALF(2, 4, 1, 1)
     default: return x;   // never set by the host
     }
 }
 #undef SQE_PAIR_ARMS
+#undef SQE_MULTI_ARMS
+#undef SQE_SWAP_HALF
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restrict__ im, const TileArgs a) {
@@ -585,12 +752,28 @@ void local_tables(TileGate& g, const uint32_t* f) {
     }
     g.dense = g.gtriv && need != 0 && (g.lsel & need) == need;
     // the code path of reg_gate
+    const bool one_local_sel = g.gtriv && __builtin_popcount(mcl) == 1;   // selection: one F bit, nothing else
     if (g.kind == (uint32_t)kPhaseOp) {
-        g.arm = g.dense ? 0u : 1u;
-    } else {
-        const uint32_t base = 2u + 10u * g.kind;
-        if (g.dense && (xl == 1u || xl == 2u || xl == 4u)) g.arm = base + (xl == 1u ? 0u : xl == 2u ? 1u : 2u);
-        else g.arm = base + 3u + (xl - 1u);
+        if (g.dense) g.arm = kArmPhaseDense;
+        else if (one_local_sel) {
+            const uint32_t b = (uint32_t)__builtin_ctz(mcl);
+            g.arm = kArmPhaseHalf + 2u * b + ((vcl >> b) & 1u);
+        } else g.arm = kArmPhaseGeneral;
+        return;
+    }
+    const uint32_t opv = g.kind == (uint32_t)OP_PAULI ? (g.base_imag ? (uint32_t)kPauliI : (uint32_t)kPauliR) : g.kind;
+    const uint32_t base = kArmPair + 16u * opv;
+    if (__builtin_popcount(xl) == 1) {
+        const uint32_t lx = (uint32_t)__builtin_ctz(xl);
+        const uint32_t ctl = mcl & ~xl;   // selection bits on F besides the pivot (selected at 0)
+        if (g.dense) g.arm = base + kArmPairDense + lx;
+        else if (g.kind == (uint32_t)OP_SWAP && g.gtriv && __builtin_popcount(ctl) == 1) {
+            const uint32_t cb = (uint32_t)__builtin_ctz(ctl);
+            const uint32_t k = cb > lx ? cb - 1u : cb;   // index of cb among the two other bits
+            g.arm = kArmSwapHalf + 4u * lx + 2u * (k == 0u ? 0u : 1u) + ((vcl >> cb) & 1u);
+        } else g.arm = base + kArmPairGeneral1 + lx;
+    } else {   // SWAP and Pauli only (run_pass sends other multi-bit gates to LDS steps)
+        g.arm = base + kArmPairMulti + (xl == 3u ? 0u : xl == 5u ? 1u : xl == 6u ? 2u : 3u);
     }
 }
 
@@ -671,7 +854,10 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
         const uint32_t xc = a.g[i].xc;
         // register steps assume that a one-bit partner mask is selected at 0
         const bool pivot_zero = __builtin_popcount(xc) != 1 || (((a.g[i].mc & xc) != 0) && ((a.g[i].vc & xc) == 0));
-        if ((uint32_t)__builtin_popcount(xc) > kSub || !pivot_zero) {   // a step of its own, in LDS
+        // register arms for several flipped bits exist for SWAP and Pauli strings only
+        const bool multi_ok = __builtin_popcount(xc) <= 1 || a.g[i].kind == (uint32_t)OP_SWAP ||
+                              a.g[i].kind == (uint32_t)OP_PAULI;
+        if ((uint32_t)__builtin_popcount(xc) > kSub || !pivot_zero || !multi_ok) {   // a step of its own, in LDS
             if (i > g0) close(i, false);
             g0 = i;
             close(i + 1, true);
