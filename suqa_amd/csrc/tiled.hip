@@ -66,7 +66,10 @@ struct AmpT<float> {
     typedef float chunk __attribute__((ext_vector_type(4)));
     static constexpr uint32_t kVec = 4, kMaxBits = kMaxTileBits32;
 };
-typedef uint32_t uv8 __attribute__((ext_vector_type(8)));  // tile offsets of the 8 register amplitudes
+// A tile in LDS: element e is the amplitude (re, im) of tile index e, one
+// 8-byte (FP32) or 16-byte (FP64) access.
+template <typename T>
+using Cplx = T __attribute__((ext_vector_type(2)));
 
 struct TileGate {
     uint32_t kind;        // OP_SWAP / OP_Y / OP_H / OP_PAULI or kPhaseOp
@@ -132,7 +135,7 @@ __device__ __forceinline__ Coef<T> coef(const TileGate& g) {
 
 // LDS step: pair gate over the whole tile, pair (u, u ^ X) per thread.
 template <int OP, typename T>
-__device__ __forceinline__ void tile_pairs(T* lre, T* lim, uint32_t half, const TileGate& g, uint32_t opar) {
+__device__ __forceinline__ void tile_pairs(Cplx<T>* lc, uint32_t half, const TileGate& g, uint32_t opar) {
     const Coef<T> pc = coef<T>(g);
     const uint32_t lo = (1u << g.pivot) - 1u;
     for (uint32_t p = threadIdx.x; p < half; p += kBlock) {
@@ -148,14 +151,18 @@ __device__ __forceinline__ void tile_pairs(T* lre, T* lim, uint32_t half, const 
         } else {
             continue;
         }
-        const T ra = lre[ia], ma = lim[ia], rb = lre[ib], mb = lim[ib];
+        const Cplx<T> va = lc[ia], vb = lc[ib];
+        const T ra = va.x, ma = va.y, rb = vb.x, mb = vb.y;
         // the partner of a is b and vice versa
         const uint32_t par_a = OP == OP_PAULI ? (((uint32_t)__popc(ib & g.yzc) & 1u) ^ opar) : 0u;
         const uint32_t par_b = OP == OP_PAULI ? (((uint32_t)__popc(ia & g.yzc) & 1u) ^ opar) : 0u;
-        lre[ia] = pair_out_par<OP>(true, false, ra, rb, mb, par_a, pc);
-        lim[ia] = pair_out_par<OP>(true, true, ma, mb, rb, par_a, pc);
-        lre[ib] = pair_out_par<OP>(false, false, rb, ra, ma, par_b, pc);
-        lim[ib] = pair_out_par<OP>(false, true, mb, ma, ra, par_b, pc);
+        Cplx<T> oa, ob;
+        oa.x = pair_out_par<OP>(true, false, ra, rb, mb, par_a, pc);
+        oa.y = pair_out_par<OP>(true, true, ma, mb, rb, par_a, pc);
+        ob.x = pair_out_par<OP>(false, false, rb, ra, ma, par_b, pc);
+        ob.y = pair_out_par<OP>(false, true, mb, ma, ra, par_b, pc);
+        lc[ia] = oa;
+        lc[ib] = ob;
     }
 }
 
@@ -165,7 +172,7 @@ __device__ __forceinline__ void tile_pairs(T* lre, T* lim, uint32_t half, const 
 // scratch memory.
 template <typename T>
 struct RM {
-    typename AmpT<T>::v8 r, m;
+    Cplx<T> e[kLoc];   // (re, im) pairs: one register pair per amplitude, as LDS holds them
 };
 
 // Register steps compute every element and pick results with selects, never
@@ -247,7 +254,7 @@ __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, T c, T
         constexpr int u = J, w = J ^ XL;
         constexpr bool dense = SEL == kDense || (SEL >= kHalf && (((u >> ((SEL - kHalf) >> 1)) & 1) == ((SEL - kHalf) & 1)));
         if constexpr (SEL >= kHalf && !dense) return x;   // this pair is outside the half selection
-        T ru = x.r[u], mu = x.m[u], rw = x.r[w], mw = x.m[w];
+        T ru = x.e[u].x, mu = x.e[u].y, rw = x.e[w].x, mw = x.e[w].y;
         constexpr T kh = (T)kTwoSqInv;
         if constexpr (dense) {
             if constexpr (OPV == kSwap) {
@@ -334,10 +341,10 @@ __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, T c, T
                 mw = on ? cmw : mw;
             }
         }
-        x.r[u] = ru;
-        x.m[u] = mu;
-        x.r[w] = rw;
-        x.m[w] = mw;
+        x.e[u].x = ru;
+        x.e[u].y = mu;
+        x.e[w].x = rw;
+        x.e[w].y = mw;
     }
     return x;
 }
@@ -364,23 +371,23 @@ __device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const TileGate& g, T c, 
     constexpr bool half_on = SEL >= kHalf && (((J >> ((SEL - kHalf) >> 1)) & 1) == ((SEL - kHalf) & 1));
     if constexpr (SEL >= kHalf && !half_on) return x;
     const T sn = ((g.lpar >> J) & 1u) ? nsg : sg;
-    const T rr = x.r[J], mm = x.m[J];
+    const T rr = x.e[J].x, mm = x.e[J].y;
     T a = rr * c, b = mm * sn, cc = mm * c, d = rr * sn;
     // the next element's inputs pass the same barrier: its products start
     // after this element's, which bounds the registers in flight
     constexpr int N = (J + 1) & 7;
-    T nr0 = x.r[N], nm0 = x.m[N];
+    T nr0 = x.e[N].x, nm0 = x.e[N].y;
     settle6(a, b, cc, d, nr0, nm0);
-    x.r[N] = nr0;
-    x.m[N] = nm0;
+    x.e[N].x = nr0;
+    x.e[N].y = nm0;
     if constexpr (SEL == kGeneral) {
         const bool on = (((g.lsel >> J) & 1u) != 0u) & act;
         const T nr = a - b, nm = cc + d;
-        x.r[J] = on ? nr : rr;
-        x.m[J] = on ? nm : mm;
+        x.e[J].x = on ? nr : rr;
+        x.e[J].y = on ? nm : mm;
     } else {
-        x.r[J] = a - b;
-        x.m[J] = cc + d;
+        x.e[J].x = a - b;
+        x.e[J].y = cc + d;
     }
     return x;
 }
@@ -413,35 +420,55 @@ constexpr uint32_t kArmPairGeneral1 = 3;      // + log2 xl
 constexpr uint32_t kArmPairMulti = 6;         // + {3: 0, 5: 1, 6: 2, 7: 3}
 constexpr uint32_t kArmSwapHalf = kArmPair + 16 * 5;   // + 4 log2 xl + 2 (cb index among the other two bits) + cv
 
+// G groups of a thread go through one dispatch: the gate's uniform work (its
+// descriptor loads, the dispatch, the per-element selection and parity masks)
+// is done once for all of them.
+template <typename T, int G>
+struct RMG {
+    RM<T> q[G];
+};
+template <int G>
+struct GroupSel {
+    bool pass[G];      // the group meets the selection's bits off F
+    uint32_t par[G];   // parity of the group's bits off F (and outside the tile)
+};
+#define SQE_EACH(CALL)                                                                                       \
+    {                                                                                                        \
+        _Pragma("unroll") for (int q_ = 0; q_ < G; ++q_) x.q[q_] = CALL;                                     \
+        return x;                                                                                            \
+    }
 #define SQE_PAIR_ARMS(OPV)                                                                                   \
-    case kArmPair + 16 * OPV + 0: return reg_pairs_seq<OPV, 1, kDense>(x, g, gpass, gpar, seq);              \
-    case kArmPair + 16 * OPV + 1: return reg_pairs_seq<OPV, 2, kDense>(x, g, gpass, gpar, seq);              \
-    case kArmPair + 16 * OPV + 2: return reg_pairs_seq<OPV, 4, kDense>(x, g, gpass, gpar, seq);              \
-    case kArmPair + 16 * OPV + 3: return reg_pairs_seq<OPV, 1, kGeneral>(x, g, gpass, gpar, seq);            \
-    case kArmPair + 16 * OPV + 4: return reg_pairs_seq<OPV, 2, kGeneral>(x, g, gpass, gpar, seq);            \
-    case kArmPair + 16 * OPV + 5: return reg_pairs_seq<OPV, 4, kGeneral>(x, g, gpass, gpar, seq);
+    case kArmPair + 16 * OPV + 0: SQE_EACH((reg_pairs_seq<OPV, 1, kDense>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))   \
+    case kArmPair + 16 * OPV + 1: SQE_EACH((reg_pairs_seq<OPV, 2, kDense>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))   \
+    case kArmPair + 16 * OPV + 2: SQE_EACH((reg_pairs_seq<OPV, 4, kDense>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))   \
+    case kArmPair + 16 * OPV + 3: SQE_EACH((reg_pairs_seq<OPV, 1, kGeneral>(x.q[q_], g, p.pass[q_], p.par[q_], seq))) \
+    case kArmPair + 16 * OPV + 4: SQE_EACH((reg_pairs_seq<OPV, 2, kGeneral>(x.q[q_], g, p.pass[q_], p.par[q_], seq))) \
+    case kArmPair + 16 * OPV + 5: SQE_EACH((reg_pairs_seq<OPV, 4, kGeneral>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
 #define SQE_MULTI_ARMS(OPV)                                                                                  \
-    case kArmPair + 16 * OPV + 6: return reg_pairs_seq<OPV, 3, kGeneral>(x, g, gpass, gpar, seq);            \
-    case kArmPair + 16 * OPV + 7: return reg_pairs_seq<OPV, 5, kGeneral>(x, g, gpass, gpar, seq);            \
-    case kArmPair + 16 * OPV + 8: return reg_pairs_seq<OPV, 6, kGeneral>(x, g, gpass, gpar, seq);            \
-    case kArmPair + 16 * OPV + 9: return reg_pairs_seq<OPV, 7, kGeneral>(x, g, gpass, gpar, seq);
+    case kArmPair + 16 * OPV + 6: SQE_EACH((reg_pairs_seq<OPV, 3, kGeneral>(x.q[q_], g, p.pass[q_], p.par[q_], seq))) \
+    case kArmPair + 16 * OPV + 7: SQE_EACH((reg_pairs_seq<OPV, 5, kGeneral>(x.q[q_], g, p.pass[q_], p.par[q_], seq))) \
+    case kArmPair + 16 * OPV + 8: SQE_EACH((reg_pairs_seq<OPV, 6, kGeneral>(x.q[q_], g, p.pass[q_], p.par[q_], seq))) \
+    case kArmPair + 16 * OPV + 9: SQE_EACH((reg_pairs_seq<OPV, 7, kGeneral>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
 // SWAP half arms: partner mask XL, control bit CB (the other two F bits) at CV
 #define SQE_SWAP_HALF(LXL, XL, K, CB)                                                                        \
-    case kArmSwapHalf + 4 * LXL + 2 * K + 0: return reg_pairs_seq<kSwap, XL, kHalf + 2 * CB + 0>(x, g, gpass, gpar, seq); \
-    case kArmSwapHalf + 4 * LXL + 2 * K + 1: return reg_pairs_seq<kSwap, XL, kHalf + 2 * CB + 1>(x, g, gpass, gpar, seq);
+    case kArmSwapHalf + 4 * LXL + 2 * K + 0:                                                                 \
+        SQE_EACH((reg_pairs_seq<kSwap, XL, kHalf + 2 * CB + 0>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))     \
+    case kArmSwapHalf + 4 * LXL + 2 * K + 1:                                                                 \
+        SQE_EACH((reg_pairs_seq<kSwap, XL, kHalf + 2 * CB + 1>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
+#define SQE_PHASE_ARM(CASE, SEL) case CASE: SQE_EACH((reg_phase_seq<SEL>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
 
-template <typename T>
-__device__ __forceinline__ RM<T> reg_gate(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar) {
+template <typename T, int G>
+__device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const TileGate& g, const GroupSel<G>& p) {
     constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
     switch (g.arm) {
-    case kArmPhaseDense: return reg_phase_seq<kDense>(x, g, gpass, gpar, seq);
-    case kArmPhaseGeneral: return reg_phase_seq<kGeneral>(x, g, gpass, gpar, seq);
-    case kArmPhaseHalf + 0: return reg_phase_seq<kHalf + 0>(x, g, gpass, gpar, seq);
-    case kArmPhaseHalf + 1: return reg_phase_seq<kHalf + 1>(x, g, gpass, gpar, seq);
-    case kArmPhaseHalf + 2: return reg_phase_seq<kHalf + 2>(x, g, gpass, gpar, seq);
-    case kArmPhaseHalf + 3: return reg_phase_seq<kHalf + 3>(x, g, gpass, gpar, seq);
-    case kArmPhaseHalf + 4: return reg_phase_seq<kHalf + 4>(x, g, gpass, gpar, seq);
-    case kArmPhaseHalf + 5: return reg_phase_seq<kHalf + 5>(x, g, gpass, gpar, seq);
+    SQE_PHASE_ARM(kArmPhaseDense, kDense)
+    SQE_PHASE_ARM(kArmPhaseGeneral, kGeneral)
+    SQE_PHASE_ARM(kArmPhaseHalf + 0, kHalf + 0)
+    SQE_PHASE_ARM(kArmPhaseHalf + 1, kHalf + 1)
+    SQE_PHASE_ARM(kArmPhaseHalf + 2, kHalf + 2)
+    SQE_PHASE_ARM(kArmPhaseHalf + 3, kHalf + 3)
+    SQE_PHASE_ARM(kArmPhaseHalf + 4, kHalf + 4)
+    SQE_PHASE_ARM(kArmPhaseHalf + 5, kHalf + 5)
     SQE_PAIR_ARMS(kSwap)
     SQE_MULTI_ARMS(kSwap)
     SQE_PAIR_ARMS(kY)
@@ -459,21 +486,22 @@ __device__ __forceinline__ RM<T> reg_gate(RM<T> x, const TileGate& g, bool gpass
     default: return x;   // never set by the host
     }
 }
+#undef SQE_EACH
 #undef SQE_PAIR_ARMS
 #undef SQE_MULTI_ARMS
 #undef SQE_SWAP_HALF
+#undef SQE_PHASE_ARM
 
-template <typename T>
+template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restrict__ im, const TileArgs a) {
     typedef typename AmpT<T>::chunk CH;
     constexpr uint32_t V = AmpT<T>::kVec;
     constexpr uint32_t kMaxChunks = (1u << AmpT<T>::kMaxBits) / V / kBlock;   // chunks per array per thread
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t tsz = 1u << a.t;
-    T* lre = reinterpret_cast<T*>(lds_raw);
-    T* lim = lre + tsz;
-    const uint64_t G = gridDim.x, b = blockIdx.x;
-    const uint64_t tile = (b & 7ull) * (G >> 3) + (b >> 3);   // XCD-contiguous (grid % 8 == 0)
+    Cplx<T>* lc = reinterpret_cast<Cplx<T>*>(lds_raw);
+    const uint64_t NB = gridDim.x, b = blockIdx.x;
+    const uint64_t tile = (b & 7ull) * (NB >> 3) + (b >> 3);   // XCD-contiguous (grid % 8 == 0)
     if (tile >= a.ntiles) return;                             // whole workgroup leaves together
     const uint64_t base = deposit(tile, a.freemask) | a.fixval;
 
@@ -494,8 +522,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restri
         if (e0 < tsz) {
             const CH vr = __builtin_nontemporal_load(reinterpret_cast<const CH*>(re + toff));
             const CH vi = __builtin_nontemporal_load(reinterpret_cast<const CH*>(im + toff));
-            *reinterpret_cast<CH*>(lre + e0) = vr;
-            *reinterpret_cast<CH*>(lim + e0) = vi;
+#pragma unroll
+            for (uint32_t i = 0; i < V; ++i) lc[e0 + i] = Cplx<T>{vr[i], vi[i]};
         }
     } else {
         CH vr[kMaxChunks], vi[kMaxChunks];
@@ -510,8 +538,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restri
 #pragma unroll
         for (uint32_t k = 0; k < kMaxChunks; ++k) {
             if (k < nch) {
-                *reinterpret_cast<CH*>(lre + e0 + k * V * kBlock) = vr[k];
-                *reinterpret_cast<CH*>(lim + e0 + k * V * kBlock) = vi[k];
+#pragma unroll
+                for (uint32_t i = 0; i < V; ++i) lc[e0 + k * V * kBlock + i] = Cplx<T>{vr[k][i], vi[k][i]};
             }
         }
     }
@@ -525,43 +553,53 @@ __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restri
             if ((base & g.mo) == g.vo) {
                 const uint32_t opar = (uint32_t)__popcll(base & g.yzo) & 1u;
                 switch (g.kind) {
-                case OP_SWAP: tile_pairs<OP_SWAP>(lre, lim, tsz / 2, g, opar); break;
-                case OP_Y: tile_pairs<OP_Y>(lre, lim, tsz / 2, g, opar); break;
-                case OP_H: tile_pairs<OP_H>(lre, lim, tsz / 2, g, opar); break;
-                default: tile_pairs<OP_PAULI>(lre, lim, tsz / 2, g, opar); break;
+                case OP_SWAP: tile_pairs<OP_SWAP>(lc, tsz / 2, g, opar); break;
+                case OP_Y: tile_pairs<OP_Y>(lc, tsz / 2, g, opar); break;
+                case OP_H: tile_pairs<OP_H>(lc, tsz / 2, g, opar); break;
+                default: tile_pairs<OP_PAULI>(lc, tsz / 2, g, opar); break;
                 }
             }
             __syncthreads();
             continue;
         }
-        uv8 off;
+        uint32_t off[kLoc];   // uniform: element j of a group sits at gb + off[j]
 #pragma unroll
         for (uint32_t j = 0; j < kLoc; ++j)
             off[j] = ((j & 1u) << S.f[0]) | (((j >> 1) & 1u) << S.f[1]) | (((j >> 2) & 1u) << S.f[2]);
-        for (uint32_t gi = threadIdx.x; gi < tsz / kLoc; gi += kBlock) {
-            uint32_t gb = gi;   // insert zeros at the step's bits (ascending)
+        // G groups per thread at a time (the host picks G with tsz / kLoc a multiple of G kBlock)
+        for (uint32_t gi0 = threadIdx.x; gi0 < tsz / kLoc; gi0 += G * kBlock) {
+            uint32_t gb[G];
+            RMG<T, G> x;
 #pragma unroll
-            for (uint32_t q = 0; q < kSub; ++q) {
-                const uint32_t lo = (1u << S.f[q]) - 1u;
-                gb = (gb & lo) | ((gb & ~lo) << 1);
-            }
-            RM<T> x;
+            for (int q = 0; q < G; ++q) {
+                uint32_t b2 = gi0 + (uint32_t)q * kBlock;   // insert zeros at the step's bits (ascending)
 #pragma unroll
-            for (uint32_t j = 0; j < kLoc; ++j) {
-                x.r[j] = lre[gb | off[j]];
-                x.m[j] = lim[gb | off[j]];
+                for (uint32_t z = 0; z < kSub; ++z) {
+                    const uint32_t lo = (1u << S.f[z]) - 1u;
+                    b2 = (b2 & lo) | ((b2 & ~lo) << 1);
+                }
+                gb[q] = b2;
+                const Cplx<T>* pg = lc + b2;   // gb has zeros at the step's bits: gb | off = gb + off
+#pragma unroll
+                for (uint32_t j = 0; j < kLoc; ++j) x.q[q].e[j] = pg[off[j]];
             }
             for (uint32_t k = S.g0; k < S.g1; ++k) {
                 const TileGate& g = a.g[k];
                 if ((base & g.mo) != g.vo) continue;   // uniform: tile outside the gate's selection
-                const bool gpass = (gb & g.mcg) == g.vcg;
-                const uint32_t gpar = ((uint32_t)__popcll(base & g.yzo) + (uint32_t)__popc(gb & g.yzg)) & 1u;
-                x = reg_gate(x, g, gpass, gpar);
+                const uint32_t opar = (uint32_t)__popcll(base & g.yzo);
+                GroupSel<G> p;
+#pragma unroll
+                for (int q = 0; q < G; ++q) {
+                    p.pass[q] = (gb[q] & g.mcg) == g.vcg;
+                    p.par[q] = (opar + (uint32_t)__popc(gb[q] & g.yzg)) & 1u;
+                }
+                x = reg_gate<T, G>(x, g, p);
             }
 #pragma unroll
-            for (uint32_t j = 0; j < kLoc; ++j) {
-                lre[gb | off[j]] = x.r[j];
-                lim[gb | off[j]] = x.m[j];
+            for (int q = 0; q < G; ++q) {
+                Cplx<T>* pg = lc + gb[q];
+#pragma unroll
+                for (uint32_t j = 0; j < kLoc; ++j) pg[off[j]] = x.q[q].e[j];
             }
         }
         __syncthreads();
@@ -570,18 +608,30 @@ __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restri
     // 3. LDS -> tile
     if (nch == 0) {
         if (e0 < tsz) {
-            __builtin_nontemporal_store(*reinterpret_cast<const CH*>(lre + e0), reinterpret_cast<CH*>(re + toff));
-            __builtin_nontemporal_store(*reinterpret_cast<const CH*>(lim + e0), reinterpret_cast<CH*>(im + toff));
+            CH vr, vi;
+#pragma unroll
+            for (uint32_t i = 0; i < V; ++i) {
+                const Cplx<T> v = lc[e0 + i];
+                vr[i] = v.x;
+                vi[i] = v.y;
+            }
+            __builtin_nontemporal_store(vr, reinterpret_cast<CH*>(re + toff));
+            __builtin_nontemporal_store(vi, reinterpret_cast<CH*>(im + toff));
         }
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < kMaxChunks; ++k) {
             if (k < nch) {
                 const uint64_t g = toff | chunk_off(k);
-                __builtin_nontemporal_store(*reinterpret_cast<const CH*>(lre + e0 + k * V * kBlock),
-                                            reinterpret_cast<CH*>(re + g));
-                __builtin_nontemporal_store(*reinterpret_cast<const CH*>(lim + e0 + k * V * kBlock),
-                                            reinterpret_cast<CH*>(im + g));
+                CH vr, vi;
+#pragma unroll
+                for (uint32_t i = 0; i < V; ++i) {
+                    const Cplx<T> v = lc[e0 + k * V * kBlock + i];
+                    vr[i] = v.x;
+                    vi[i] = v.y;
+                }
+                __builtin_nontemporal_store(vr, reinterpret_cast<CH*>(re + g));
+                __builtin_nontemporal_store(vi, reinterpret_cast<CH*>(im + g));
             }
         }
     }
@@ -880,18 +930,18 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     c->pass_gates += n;
     // algorithmic bytes: re and im of every tile amplitude read and written
     timing_begin(c, c->f32 ? "tile_pass32" : "tile_pass", 4.0 * (double)es * (double)((uint64_t)1 << t) * (double)a.ntiles);
+    // groups per thread and dispatch: 2 when the tile has 2 kBlock groups or more
+    const bool g2 = ((1u << t) / kLoc) >= 2 * kBlock;
+    const dim3 grid((uint32_t)g8), blk(kBlock);
+    const size_t tiny_lds = 2 * lds + sizeof(TileGate) * kMaxPass;
     if (c->f32) {
-        if (tiny)
-            hipLaunchKernelGGL(k_tile_tiny<float>, dim3(1), dim3(1u << t), 2 * lds + sizeof(TileGate) * kMaxPass,
-                               c->stream, c->f32_re, c->f32_im, a);
-        else
-            hipLaunchKernelGGL(k_tile<float>, dim3((uint32_t)g8), dim3(kBlock), lds, c->stream, c->f32_re, c->f32_im, a);
+        if (tiny) hipLaunchKernelGGL(k_tile_tiny<float>, dim3(1), dim3(1u << t), tiny_lds, c->stream, c->f32_re, c->f32_im, a);
+        else if (g2) hipLaunchKernelGGL((k_tile<float, 2>), grid, blk, lds, c->stream, c->f32_re, c->f32_im, a);
+        else hipLaunchKernelGGL((k_tile<float, 1>), grid, blk, lds, c->stream, c->f32_re, c->f32_im, a);
     } else {
-        if (tiny)
-            hipLaunchKernelGGL(k_tile_tiny<double>, dim3(1), dim3(1u << t), 2 * lds + sizeof(TileGate) * kMaxPass,
-                               c->stream, c->d_re, c->d_im, a);
-        else
-            hipLaunchKernelGGL(k_tile<double>, dim3((uint32_t)g8), dim3(kBlock), lds, c->stream, c->d_re, c->d_im, a);
+        if (tiny) hipLaunchKernelGGL(k_tile_tiny<double>, dim3(1), dim3(1u << t), tiny_lds, c->stream, c->d_re, c->d_im, a);
+        else if (g2) hipLaunchKernelGGL((k_tile<double, 2>), grid, blk, lds, c->stream, c->d_re, c->d_im, a);
+        else hipLaunchKernelGGL((k_tile<double, 1>), grid, blk, lds, c->stream, c->d_re, c->d_im, a);
     }
     timing_end(c);
     return hipGetLastError();
