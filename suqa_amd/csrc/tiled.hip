@@ -215,6 +215,14 @@ __device__ __forceinline__ void settle6(T& a, T& b, T& c, T& d, T& e, T& f) {
     asm volatile("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d), "+v"(e), "+v"(f));
 }
 
+// Keeps the addends of the FP32 FMAs alive past them: an FMA whose addend
+// dies is emitted in the accumulating form (result in the addend's register),
+// which moves the amplitude out of its own register.
+template <typename T>
+__device__ __forceinline__ void after(T a, T b, T c, T d) {
+    asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d));
+}
+
 // Exchanges a and b in their registers.
 __device__ __forceinline__ void vswap(float& a, float& b) { asm("v_swap_b32 %0, %1" : "+v"(a), "+v"(b)); }
 __device__ __forceinline__ void vswap(double& a, double& b) {
@@ -225,6 +233,12 @@ __device__ __forceinline__ void vswap(double& a, double& b) {
     a = __longlong_as_double((long long)(((uint64_t)ah << 32) | al));
     b = __longlong_as_double((long long)(((uint64_t)bh << 32) | bl));
 }
+
+// FP32 arms fuse each product and sum into one FMA (more accurate, fewer
+// instructions); FP64 arms keep the reference's separate rounding of each
+// operation, so FP64 results stay bit-identical.
+template <typename T>
+constexpr bool kFma = std::is_same<T, float>::value;
 
 // Pair operations of the register arms (OPV): the four pair kinds, with the
 // Pauli rotation split by its base (real: psi' = c psi + t psi_partner,
@@ -274,7 +288,13 @@ __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, T c, T
                 mw = kh * dm;
             } else {
                 const T tu = pauli_t(g, w, tg, ntg), tw = pauli_t(g, u, tg, ntg);
-                T pru = ru * c, pmu = mu * c, prw = rw * c, pmw = mw * c;
+                T pru = (T)0, pmu = (T)0, prw = (T)0, pmw = (T)0;
+                if constexpr (!kFma<T>) {
+                    pru = ru * c;
+                    pmu = mu * c;
+                    prw = rw * c;
+                    pmw = mw * c;
+                }
                 T qru, qmu, qrw, qmw;
                 if constexpr (OPV == kPauliR) {
                     qru = rw * tu;
@@ -287,12 +307,21 @@ __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, T c, T
                     qrw = mu * (-tw);
                     qmw = ru * tw;
                 }
-                settle(pru, pmu, prw, pmw);
-                settle(qru, qmu, qrw, qmw);
-                ru = pru + qru;
-                mu = pmu + qmu;
-                rw = prw + qrw;
-                mw = pmw + qmw;
+                if constexpr (kFma<T>) {   // FP32: c psi + q in one rounding
+                    settle(qru, qmu, qrw, qmw);
+                    ru = __builtin_fmaf(ru, c, qru);
+                    mu = __builtin_fmaf(mu, c, qmu);
+                    rw = __builtin_fmaf(rw, c, qrw);
+                    mw = __builtin_fmaf(mw, c, qmw);
+                    after(qru, qmu, qrw, qmw);
+                } else {
+                    settle(pru, pmu, prw, pmw);
+                    settle(qru, qmu, qrw, qmw);
+                    ru = pru + qru;
+                    mu = pmu + qmu;
+                    rw = prw + qrw;
+                    mw = pmw + qmw;
+                }
             }
         } else {   // kGeneral
             // no short-circuit: a branch here would split the arm into blocks
@@ -322,7 +351,19 @@ __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, T c, T
                     cmw = kh * (mu - mw);
                 } else {
                     const T tu = pauli_t(g, w, tg, ntg), tw = pauli_t(g, u, tg, ntg);
-                    if constexpr (OPV == kPauliR) {
+                    if constexpr (kFma<T>) {
+                        if constexpr (OPV == kPauliR) {
+                            cru = __builtin_fmaf(ru, c, rw * tu);
+                            cmu = __builtin_fmaf(mu, c, mw * tu);
+                            crw = __builtin_fmaf(rw, c, ru * tw);
+                            cmw = __builtin_fmaf(mw, c, mu * tw);
+                        } else {
+                            cru = __builtin_fmaf(ru, c, mw * (-tu));
+                            cmu = __builtin_fmaf(mu, c, rw * tu);
+                            crw = __builtin_fmaf(rw, c, mu * (-tw));
+                            cmw = __builtin_fmaf(mw, c, ru * tw);
+                        }
+                    } else if constexpr (OPV == kPauliR) {
                         cru = ru * c + rw * tu;
                         cmu = mu * c + mw * tu;
                         crw = rw * c + ru * tw;
@@ -372,22 +413,32 @@ __device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const TileGate& g, T c, 
     if constexpr (SEL >= kHalf && !half_on) return x;
     const T sn = ((g.lpar >> J) & 1u) ? nsg : sg;
     const T rr = x.e[J].x, mm = x.e[J].y;
-    T a = rr * c, b = mm * sn, cc = mm * c, d = rr * sn;
     // the next element's inputs pass the same barrier: its products start
     // after this element's, which bounds the registers in flight
     constexpr int N = (J + 1) & 7;
     T nr0 = x.e[N].x, nm0 = x.e[N].y;
-    settle6(a, b, cc, d, nr0, nm0);
+    T nr, nm;
+    if constexpr (kFma<T>) {   // FP32: re c - im s and im c + re s, one rounding each
+        T b = mm * sn, d = rr * sn;
+        settle(b, d, nr0, nm0);
+        nr = __builtin_fmaf(rr, c, -b);
+        nm = __builtin_fmaf(mm, c, d);
+        after(b, d, b, d);
+    } else {
+        T a = rr * c, b = mm * sn, cc = mm * c, d = rr * sn;
+        settle6(a, b, cc, d, nr0, nm0);
+        nr = a - b;
+        nm = cc + d;
+    }
     x.e[N].x = nr0;
     x.e[N].y = nm0;
     if constexpr (SEL == kGeneral) {
         const bool on = (((g.lsel >> J) & 1u) != 0u) & act;
-        const T nr = a - b, nm = cc + d;
         x.e[J].x = on ? nr : rr;
         x.e[J].y = on ? nm : mm;
     } else {
-        x.e[J].x = a - b;
-        x.e[J].y = cc + d;
+        x.e[J].x = nr;
+        x.e[J].y = nm;
     }
     return x;
 }
