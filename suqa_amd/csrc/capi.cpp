@@ -205,6 +205,8 @@ suqa_hip_ctx* f32_gate_queue(int device, hipStream_t stream, uint32_t nq, float*
     q->tile_bits = kMaxTileBits32 - 1;
     const char* e = std::getenv("SUQA_HIP_PAULI_ORDER");
     q->pauli_reorder = !(e && std::strcmp(e, "0") == 0);
+    const char* e2 = std::getenv("SUQA_HIP_GATE_ORDER");
+    q->gate_reorder = !(e2 && std::strcmp(e2, "0") == 0);
     return q;
 }
 

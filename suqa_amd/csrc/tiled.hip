@@ -1000,6 +1000,71 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
 
 }  // namespace
 
+namespace {
+
+// Qubits a gate reads or writes: flipped, selected, and parity qubits.
+uint64_t gate_support(const GateDesc& d) {
+    const uint64_t par = d.kind == kPhaseOp ? d.zmask : d.pc.yz;
+    const uint64_t flip = d.kind == kPhaseOp ? 0ull : d.X;
+    return flip | d.M | par;
+}
+uint64_t gate_flips(const GateDesc& d) { return d.kind == kPhaseOp ? 0ull : d.X; }
+
+// Gates a, b commute when neither flips a qubit the other reads or writes:
+// on their shared qubits both are then diagonal, so each is a sum over the
+// shared qubits' values of projectors times operators on disjoint qubits.
+bool gates_commute(const GateDesc& a, const GateDesc& b) {
+    return (gate_flips(a) & gate_support(b)) == 0 && (gate_flips(b) & gate_support(a)) == 0;
+}
+
+// The order in which a float gate queue is applied: passes packed greedily
+// as flush() cuts them (qubits 0..f-1 plus flipped qubits, at most t; at most
+// kMaxPass gates), a gate moving ahead only past gates it commutes with,
+// looking `window` gates ahead.  The product of the gates is unchanged; the
+// rounding is not, which FP32 states accept and FP64 states (bit-identical to
+// the reference's gate order) do not.
+void pack_gates(std::vector<GateDesc>& q, uint32_t t, uint64_t low, size_t window) {
+    std::vector<GateDesc> rest(q.begin(), q.end()), out;
+    out.reserve(q.size());
+    size_t head = 0;   // rest[head..] not yet placed (erase from a vector front is linear; keep an index)
+    std::vector<char> used(rest.size(), 0);
+    while (head < rest.size()) {
+        uint64_t T = low;
+        uint32_t count = 0;
+        bool took = true;
+        while (took && count < kMaxPass && head < rest.size()) {
+            took = false;
+            std::vector<size_t> skipped;
+            for (size_t j = head, seen = 0; j < rest.size() && seen < window; ++j) {
+                if (used[j]) continue;
+                ++seen;
+                const uint64_t need = gate_flips(rest[j]);
+                bool ok = (uint32_t)__builtin_popcountll(T | need) <= t;
+                for (size_t i = 0; ok && i < skipped.size(); ++i) ok = gates_commute(rest[skipped[i]], rest[j]);
+                if (!ok) {
+                    skipped.push_back(j);
+                    continue;
+                }
+                T |= need;
+                ++count;
+                out.push_back(rest[j]);
+                used[j] = 1;
+                took = true;
+                break;
+            }
+            while (head < rest.size() && used[head]) ++head;
+        }
+        if (count == 0) {   // the first gate alone
+            out.push_back(rest[head]);
+            used[head] = 1;
+            while (head < rest.size() && used[head]) ++head;
+        }
+    }
+    q.swap(out);
+}
+
+}  // namespace
+
 hipError_t flush(suqa_hip_ctx* c) {
     if (c->queue.empty()) return hipSuccess;
     std::vector<GateDesc> q;
@@ -1007,6 +1072,7 @@ hipError_t flush(suqa_hip_ctx* c) {
     const uint32_t t = std::min(c->tile_bits, c->nq);
     const uint32_t f = std::min(kLowFixed, t);
     const uint64_t low = (1ull << f) - 1ull;
+    if (c->f32 && c->gate_reorder) pack_gates(q, t, low, 64);
     size_t i = 0;
     while (i < q.size()) {
         uint64_t T = low;
