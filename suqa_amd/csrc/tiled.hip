@@ -248,6 +248,9 @@ constexpr int kSwap = 0, kY = 1, kH = 2, kPauliR = 3, kPauliI = 4;
 // bit CB equal to CV (kHalf + 2 CB + CV, no other selection), or per element
 // from lsel and gpass (kGeneral).
 constexpr int kDense = 0, kGeneral = 1, kHalf = 2;
+// kGroup: every element of the groups that meet the selection's bits off F
+// (and none on F): the dense update on the lanes of those groups only.
+constexpr int kGroup = 8;
 
 // Pauli sign: t = (base_neg ^ parity) ? -s : s; tg, ntg are the signed s and
 // its negation for a parity-0 element of this group, flipped where the
@@ -400,9 +403,16 @@ __device__ __forceinline__ RM<T> reg_pairs_seq(RM<T> x, const TileGate& g, bool 
         tg = (g.base_neg ^ gpar) ? -s : s;
         ntg = -tg;
     }
-    const bool act = (g.gtriv != 0u) | gpass;   // this group meets the selection's bits off F
-    ((x = reg_pair_one<OPV, XL, SEL, J>(x, g, c, tg, ntg, act)), ...);
-    return x;
+    if constexpr (SEL == kGroup) {
+        if (gpass) {   // per lane: the update runs under the lanes' mask, in place
+            ((x = reg_pair_one<OPV, XL, kDense, J>(x, g, c, tg, ntg, true)), ...);
+        }
+        return x;
+    } else {
+        const bool act = (g.gtriv != 0u) | gpass;   // this group meets the selection's bits off F
+        ((x = reg_pair_one<OPV, XL, SEL, J>(x, g, c, tg, ntg, act)), ...);
+        return x;
+    }
 }
 
 // Phase rotation of the register amplitudes (util_rotate2 on each selected
@@ -448,9 +458,16 @@ __device__ __forceinline__ RM<T> reg_phase_seq(RM<T> x, const TileGate& g, bool 
                                                std::integer_sequence<int, J...>) {
     const T c = gate_c<T>(g), s = gate_s<T>(g);
     const T sg = gpar ? -s : s, nsg = -sg;
-    const bool act = (g.gtriv != 0u) | gpass;
-    ((x = reg_phase_one<SEL, J>(x, g, c, sg, nsg, act)), ...);
-    return x;
+    if constexpr (SEL == kGroup) {
+        if (gpass) {
+            ((x = reg_phase_one<kDense, J>(x, g, c, sg, nsg, true)), ...);
+        }
+        return x;
+    } else {
+        const bool act = (g.gtriv != 0u) | gpass;
+        ((x = reg_phase_one<SEL, J>(x, g, c, sg, nsg, act)), ...);
+        return x;
+    }
 }
 
 // One flat dispatch over every register-step code path, on the arm index the
@@ -470,6 +487,8 @@ constexpr uint32_t kArmPairDense = 0;         // + log2 xl
 constexpr uint32_t kArmPairGeneral1 = 3;      // + log2 xl
 constexpr uint32_t kArmPairMulti = 6;         // + {3: 0, 5: 1, 6: 2, 7: 3}
 constexpr uint32_t kArmSwapHalf = kArmPair + 16 * 5;   // + 4 log2 xl + 2 (cb index among the other two bits) + cv
+constexpr uint32_t kArmPhaseGroup = kArmSwapHalf + 12;
+constexpr uint32_t kArmPairGroup = kArmPhaseGroup + 1;   // + 3 OPV + log2 xl
 
 // G groups of a thread go through one dispatch: the gate's uniform work (its
 // descriptor loads, the dispatch, the per-element selection and parity masks)
@@ -507,6 +526,10 @@ struct GroupSel {
     case kArmSwapHalf + 4 * LXL + 2 * K + 1:                                                                 \
         SQE_EACH((reg_pairs_seq<kSwap, XL, kHalf + 2 * CB + 1>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
 #define SQE_PHASE_ARM(CASE, SEL) case CASE: SQE_EACH((reg_phase_seq<SEL>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
+#define SQE_GROUP_ARMS(OPV)                                                                                  \
+    case kArmPairGroup + 3 * OPV + 0: SQE_EACH((reg_pairs_seq<OPV, 1, kGroup>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))   \
+    case kArmPairGroup + 3 * OPV + 1: SQE_EACH((reg_pairs_seq<OPV, 2, kGroup>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))   \
+    case kArmPairGroup + 3 * OPV + 2: SQE_EACH((reg_pairs_seq<OPV, 4, kGroup>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
 
 template <typename T, int G>
 __device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const TileGate& g, const GroupSel<G>& p) {
@@ -534,6 +557,12 @@ __device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const TileGate& g, co
     SQE_SWAP_HALF(1, 2, 1, 2)
     SQE_SWAP_HALF(2, 4, 0, 0)
     SQE_SWAP_HALF(2, 4, 1, 1)
+    SQE_PHASE_ARM(kArmPhaseGroup, kGroup)
+    SQE_GROUP_ARMS(kSwap)
+    SQE_GROUP_ARMS(kY)
+    SQE_GROUP_ARMS(kH)
+    SQE_GROUP_ARMS(kPauliR)
+    SQE_GROUP_ARMS(kPauliI)
     default: return x;   // never set by the host
     }
 }
@@ -542,6 +571,7 @@ __device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const TileGate& g, co
 #undef SQE_MULTI_ARMS
 #undef SQE_SWAP_HALF
 #undef SQE_PHASE_ARM
+#undef SQE_GROUP_ARMS
 
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restrict__ im, const TileArgs a) {
@@ -856,6 +886,7 @@ void local_tables(TileGate& g, const uint32_t* f) {
     const bool one_local_sel = g.gtriv && __builtin_popcount(mcl) == 1;   // selection: one F bit, nothing else
     if (g.kind == (uint32_t)kPhaseOp) {
         if (g.dense) g.arm = kArmPhaseDense;
+        else if (!g.gtriv && mcl == 0u) g.arm = kArmPhaseGroup;
         else if (one_local_sel) {
             const uint32_t b = (uint32_t)__builtin_ctz(mcl);
             g.arm = kArmPhaseHalf + 2u * b + ((vcl >> b) & 1u);
@@ -868,6 +899,7 @@ void local_tables(TileGate& g, const uint32_t* f) {
         const uint32_t lx = (uint32_t)__builtin_ctz(xl);
         const uint32_t ctl = mcl & ~xl;   // selection bits on F besides the pivot (selected at 0)
         if (g.dense) g.arm = base + kArmPairDense + lx;
+        else if (!g.gtriv && ctl == 0u) g.arm = kArmPairGroup + 3u * opv + lx;
         else if (g.kind == (uint32_t)OP_SWAP && g.gtriv && __builtin_popcount(ctl) == 1) {
             const uint32_t cb = (uint32_t)__builtin_ctz(ctl);
             const uint32_t k = cb > lx ? cb - 1u : cb;   // index of cb among the two other bits
