@@ -292,7 +292,7 @@ std::vector<uint32_t> pass_order(const suqa_hip_pauli_op* ops, uint32_t n, uint3
 
 namespace sqe {
 std::vector<uint32_t> pauli_pass_order(const suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t n) {
-    return pass_order(ops, n, std::min(c->tile_bits, c->nq), kPassLowFixed, kPassMaxGates, 128);
+    return pass_order(ops, n, std::min(c->tile_bits, c->nq), c->f32 ? kPassLowFixed32 : kPassLowFixed, kPassMaxGates, 128);
 }
 }  // namespace sqe
 

@@ -201,6 +201,7 @@ hipError_t flush(suqa_hip_ctx* c);
 constexpr uint32_t kMinTileBits = 1, kMaxTileBits = 12;
 constexpr uint32_t kMaxTileBits32 = 13;   // float tiles (64 KiB of LDS)
 constexpr uint32_t kPassLowFixed = 4;    // qubits 0..3 are in every pass tile (one 128-byte line)
+constexpr uint32_t kPassLowFixed32 = 5;  // the same for float states: qubits 0..4
 constexpr uint32_t kPassMaxGates = 24;   // gates per pass
 // run_paulis' order of an op list with gate passes on (capi.cpp)
 std::vector<uint32_t> pauli_pass_order(const suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t n);

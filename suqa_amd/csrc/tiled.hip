@@ -1102,7 +1102,7 @@ hipError_t flush(suqa_hip_ctx* c) {
     std::vector<GateDesc> q;
     q.swap(c->queue);
     const uint32_t t = std::min(c->tile_bits, c->nq);
-    const uint32_t f = std::min(kLowFixed, t);
+    const uint32_t f = std::min(c->f32 ? kPassLowFixed32 : kLowFixed, t);   // whole 128-byte lines
     const uint64_t low = (1ull << f) - 1ull;
     if (c->f32 && c->gate_reorder) pack_gates(q, t, low, 64);
     size_t i = 0;
