@@ -48,6 +48,7 @@ constexpr uint32_t kMaxPass = kPassMaxGates;
 constexpr uint32_t kLowFixed = kPassLowFixed;     // qubits 0..3: 16 amplitudes, one 128-byte line in FP64
 constexpr uint32_t kSub = 3, kLoc = 1u << kSub;   // register step: 3 bits, 8 amplitudes per thread
 constexpr uint32_t kTinyBits = 10;                // one-tile states up to 2^10: k_tile_tiny
+constexpr uint32_t kMaxRuns = 20;                 // runs of consecutive set bits of a mask of <= 39 bits
 
 // Amplitude types: double (the FP64 engine) or float (the FP32 engine's gate
 // passes).  A tile moves between HBM and LDS in 16-byte chunks of kVec
@@ -101,6 +102,13 @@ struct TileArgs {
     uint64_t tmask;       // the tile's state bits (tile element e -> deposit(e, tmask))
     uint64_t freemask;    // state bits enumerated by the tile number (not in the tile,
                           // not fixed by a selection common to every gate)
+    // The same two maps as runs of consecutive mask bits, for deposit_runs (the
+    // bit-by-bit deposit costs ~10 scalar instructions per mask bit, a few
+    // hundred per tile): run i = lo | len << 8 | src << 16.
+    uint32_t nfree_runs, ntile_runs;
+    uint32_t free_runs[kMaxRuns];
+    uint32_t tile_runs[kMaxRuns];   // only the runs fed by the bits of a thread's first element
+    uint64_t kbit[3];     // state offsets of the tile bits that chunk index bits 0, 1, 2 stand for
     struct Step {
         uint32_t f[kSub]; // tile positions of the step's bits, ascending; f[0] = kLdsStep: LDS step
         uint32_t g0, g1;  // gates [g0, g1)
@@ -108,6 +116,7 @@ struct TileArgs {
     TileGate g[kMaxPass];
 };
 constexpr uint32_t kLdsStep = 0xffu;
+static_assert(sizeof(TileArgs) <= 4096, "kernel arguments are limited to 4 KiB");
 
 // A gate's coefficients in the pass's amplitude type (FP64: the exact values).
 template <typename T>
@@ -573,6 +582,21 @@ __device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const TileGate& g, co
 #undef SQE_PHASE_ARM
 #undef SQE_GROUP_ARMS
 
+// deposit(x, mask) for a mask given as runs of consecutive bits (host-built):
+// a few instructions per run instead of per mask bit.
+__device__ __forceinline__ uint64_t deposit_runs(uint64_t x, const uint32_t* runs, uint32_t n) {
+    uint64_t r = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxRuns; ++i) {
+        if (i < n) {
+            const uint32_t w = runs[i];
+            const uint32_t lo = w & 63u, len = (w >> 8) & 63u, src = w >> 16;
+            r |= ((x >> src) & ((1ull << len) - 1ull)) << lo;
+        }
+    }
+    return r;
+}
+
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restrict__ im, const TileArgs a) {
     typedef typename AmpT<T>::chunk CH;
@@ -584,18 +608,16 @@ __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restri
     const uint64_t NB = gridDim.x, b = blockIdx.x;
     const uint64_t tile = (b & 7ull) * (NB >> 3) + (b >> 3);   // XCD-contiguous (grid % 8 == 0)
     if (tile >= a.ntiles) return;                             // whole workgroup leaves together
-    const uint64_t base = deposit(tile, a.freemask) | a.fixval;
+    const uint64_t base = deposit_runs(tile, a.free_runs, a.nfree_runs) | a.fixval;
 
     // 1. tile -> LDS: chunk k of a thread is tile elements e = V tid + k V kBlock
     //    onwards (tile bits 0..log2 V - 1 are state qubits 0.. of one line); every
     //    load of the thread is issued before the LDS writes
     const uint32_t e0 = V * threadIdx.x;
     const uint32_t nch = tsz / (V * kBlock);   // 0 when the tile is smaller than V kBlock
-    const uint64_t toff = e0 < tsz ? (base | deposit(e0, a.tmask)) : 0ull;
+    const uint64_t toff = e0 < tsz ? (base | deposit_runs(e0, a.tile_runs, a.ntile_runs)) : 0ull;
     // state offsets of the tile bits that chunk index bits 0, 1, 2 stand for
-    uint64_t kbit[3];
-#pragma unroll
-    for (uint32_t j = 0; j < 3; ++j) kbit[j] = deposit(V * kBlock << j, a.tmask);
+    const uint64_t* kbit = a.kbit;
     auto chunk_off = [&](uint32_t k) {
         return ((k & 1u) ? kbit[0] : 0ull) | ((k & 2u) ? kbit[1] : 0ull) | ((k & 4u) ? kbit[2] : 0ull);
     };
@@ -836,6 +858,32 @@ __global__ __launch_bounds__(1024) void k_tile_tiny(T* __restrict__ re, T* __res
     im[off] = vi;
 }
 
+// Runs of consecutive set bits of m, lowest first, as lo | len << 8 | src << 16
+// (src: the index of the run's first bit among m's set bits); only the runs
+// whose src is below src_limit.  At most kMaxRuns (a 64-bit mask of t tile
+// bits has at most t + 1 runs of clear bits and t of set bits, t <= 13).
+uint32_t mask_runs(uint64_t m, uint32_t* runs, uint32_t src_limit) {
+    uint32_t n = 0, src = 0;
+    while (m && src < src_limit && n < kMaxRuns) {
+        const uint32_t lo = (uint32_t)__builtin_ctzll(m);
+        const uint64_t shifted = m >> lo;
+        const uint32_t len = shifted == ~0ull ? 64u - lo : (uint32_t)__builtin_ctzll(~shifted);
+        runs[n++] = lo | (len << 8) | (src << 16);
+        src += len;
+        m &= len >= 64 ? 0ull : ~(((1ull << len) - 1ull) << lo);
+    }
+    return n;
+}
+
+uint64_t deposit_host(uint64_t x, uint64_t mask) {
+    uint64_t r = 0;
+    for (uint64_t mm = mask; mm; mm &= mm - 1) {
+        if (x & 1ull) r |= mm & (0ull - mm);
+        x >>= 1;
+    }
+    return r;
+}
+
 uint32_t compact(uint64_t m, const uint8_t* tpos, uint32_t t) {
     uint32_t o = 0;
     for (uint32_t j = 0; j < t; ++j)
@@ -957,6 +1005,17 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
     a.freemask = ((nq == 64 ? 0ull : (1ull << nq)) - 1ull) & ~fix;
     a.fixval = vcom;
     a.ntiles = 1ull << __builtin_popcountll(a.freemask);
+    // the masks as runs; for the tile mask only the runs that the bits of a
+    // thread's first element (below V kBlock) reach
+    const uint32_t vec = c->f32 ? 4u : 2u;
+    a.nfree_runs = mask_runs(a.freemask, a.free_runs, 64);
+    if (a.nfree_runs && (a.free_runs[a.nfree_runs - 1] >> 16) + ((a.free_runs[a.nfree_runs - 1] >> 8) & 63u) !=
+                            (uint32_t)__builtin_popcountll(a.freemask)) {
+        set_last_error("tile enumeration mask has more runs than the pass kernel takes");
+        return hipErrorInvalidValue;
+    }
+    a.ntile_runs = mask_runs(T, a.tile_runs, (uint32_t)__builtin_ctz(vec * (uint32_t)kBlock));
+    for (uint32_t j = 0; j < 3; ++j) a.kbit[j] = deposit_host((uint64_t)(vec * kBlock) << j, T);
     // A state that is one small tile (t <= kTinyBits) is one workgroup and the
     // pass is latency-bound: k_tile_tiny, one amplitude per thread (the steps
     // below are not used).
