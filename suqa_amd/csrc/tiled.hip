@@ -72,25 +72,33 @@ struct AmpT<float> {
 template <typename T>
 using Cplx = T __attribute__((ext_vector_type(2)));
 
-struct TileGate {
+// What a register step reads of a gate, first in TileGate so that the gate
+// loop fetches it with one 64-byte scalar load (plus 16 bytes for FP64
+// coefficients) and one wait, instead of field by field.
+struct GateR {
+    uint64_t mo, vo;      // selection bits outside the tile (global positions)
+    uint64_t yzo;         // parity bits outside the tile
+    // the part on the tile bits off the step's bits F (tile positions, zero on F) ...
+    uint32_t mcg, vcg, yzg;
+    uint32_t arm;         // register-step code path (reg_gate)
+    // ... and the part on F
+    uint32_t lsel;        // bit j: group element j meets the selection's part on F
+    uint32_t lpar;        // bit j: parity of element j's bits on F in the parity mask
+    uint32_t gtriv;       // mcg == 0: every group meets the selection
+    uint32_t base_neg;
+    float cf, sf;         // c, s rounded to FP32 (float passes)
+    double c, s;
+};
+static_assert(sizeof(GateR) == 80, "GateR is read as 64 + 16 bytes");
+
+struct TileGate : GateR {
     uint32_t kind;        // OP_SWAP / OP_Y / OP_H / OP_PAULI or kPhaseOp
     uint32_t pivot;       // tile position of the lowest partner bit (pair operations)
     uint32_t xc, mc, vc;  // partner mask and selection in tile positions (LDS steps)
     uint32_t yzc;         // parity mask in tile positions (Y|Z positions or the phase's zmask)
-    // register steps: the local part over the step's bits F ...
     uint32_t xl;          // partner mask over F
-    uint32_t lsel;        // bit j: group element j meets the selection's part on F
-    uint32_t lpar;        // bit j: parity of element j's bits on F in the parity mask
-    // ... and the part on the other tile bits (tile positions, zero on F)
-    uint32_t mcg, vcg, yzg;
-    uint32_t gtriv;       // mcg == 0: every group meets the selection
     uint32_t dense;       // gtriv, and every element (pair) of a group is selected
-    uint32_t base_imag, base_neg;
-    uint32_t arm;         // register-step code path (reg_gate): kind x partner mask x dense
-    float cf, sf;         // c, s rounded to FP32 (float passes)
-    uint64_t mo, vo;      // selection bits outside the tile (global positions)
-    uint64_t yzo;         // parity bits outside the tile
-    double c, s;
+    uint32_t base_imag;
 };
 
 struct TileArgs {
@@ -120,17 +128,17 @@ static_assert(sizeof(TileArgs) <= 4096, "kernel arguments are limited to 4 KiB")
 
 // A gate's coefficients in the pass's amplitude type (FP64: the exact values).
 template <typename T>
-__device__ __forceinline__ T gate_c(const TileGate& g);
+__device__ __forceinline__ T gate_c(const GateR& g);
 template <>
-__device__ __forceinline__ double gate_c<double>(const TileGate& g) { return g.c; }
+__device__ __forceinline__ double gate_c<double>(const GateR& g) { return g.c; }
 template <>
-__device__ __forceinline__ float gate_c<float>(const TileGate& g) { return g.cf; }
+__device__ __forceinline__ float gate_c<float>(const GateR& g) { return g.cf; }
 template <typename T>
-__device__ __forceinline__ T gate_s(const TileGate& g);
+__device__ __forceinline__ T gate_s(const GateR& g);
 template <>
-__device__ __forceinline__ double gate_s<double>(const TileGate& g) { return g.s; }
+__device__ __forceinline__ double gate_s<double>(const GateR& g) { return g.s; }
 template <>
-__device__ __forceinline__ float gate_s<float>(const TileGate& g) { return g.sf; }
+__device__ __forceinline__ float gate_s<float>(const GateR& g) { return g.sf; }
 
 template <typename T>
 struct Coef {   // what pair_out_par reads of a pair gate
@@ -265,7 +273,7 @@ constexpr int kGroup = 8;
 // its negation for a parity-0 element of this group, flipped where the
 // element's own bits on F have odd parity (lpar).
 template <typename T>
-__device__ __forceinline__ T pauli_t(const TileGate& g, int elem, T tg, T ntg) {
+__device__ __forceinline__ T pauli_t(const GateR& g, int elem, T tg, T ntg) {
     return ((g.lpar >> elem) & 1u) ? ntg : tg;
 }
 
@@ -274,7 +282,7 @@ __device__ __forceinline__ T pauli_t(const TileGate& g, int elem, T tg, T ntg) {
 // first and select them per element.  Outputs follow pair_out_par exactly
 // (same operations, same order), so FP64 results stay bit-identical.
 template <int OPV, int XL, int SEL, int J, typename T>
-__device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, T c, T tg, T ntg, bool act) {
+__device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const GateR& g, T c, T tg, T ntg, bool act) {
     constexpr int PIV = XL & -XL;
     if constexpr ((J & PIV) == 0) {
         constexpr int u = J, w = J ^ XL;
@@ -403,7 +411,7 @@ __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const TileGate& g, T c, T
 }
 
 template <int OPV, int XL, int SEL, typename T, int... J>
-__device__ __forceinline__ RM<T> reg_pairs_seq(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar,
+__device__ __forceinline__ RM<T> reg_pairs_seq(RM<T> x, const GateR& g, bool gpass, uint32_t gpar,
                                                std::integer_sequence<int, J...>) {
     T c = (T)0, tg = (T)0, ntg = (T)0;
     if constexpr (OPV == kPauliR || OPV == kPauliI) {
@@ -427,7 +435,7 @@ __device__ __forceinline__ RM<T> reg_pairs_seq(RM<T> x, const TileGate& g, bool 
 // Phase rotation of the register amplitudes (util_rotate2 on each selected
 // one): dense (all 8), half (the elements whose bit B equals V), general.
 template <int SEL, int J, typename T>
-__device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const TileGate& g, T c, T sg, T nsg, bool act) {
+__device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const GateR& g, T c, T sg, T nsg, bool act) {
     constexpr bool half_on = SEL >= kHalf && (((J >> ((SEL - kHalf) >> 1)) & 1) == ((SEL - kHalf) & 1));
     if constexpr (SEL >= kHalf && !half_on) return x;
     const T sn = ((g.lpar >> J) & 1u) ? nsg : sg;
@@ -463,7 +471,7 @@ __device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const TileGate& g, T c, 
 }
 
 template <int SEL, typename T, int... J>
-__device__ __forceinline__ RM<T> reg_phase_seq(RM<T> x, const TileGate& g, bool gpass, uint32_t gpar,
+__device__ __forceinline__ RM<T> reg_phase_seq(RM<T> x, const GateR& g, bool gpass, uint32_t gpar,
                                                std::integer_sequence<int, J...>) {
     const T c = gate_c<T>(g), s = gate_s<T>(g);
     const T sg = gpar ? -s : s, nsg = -sg;
@@ -541,7 +549,7 @@ struct GroupSel {
     case kArmPairGroup + 3 * OPV + 2: SQE_EACH((reg_pairs_seq<OPV, 4, kGroup>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
 
 template <typename T, int G>
-__device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const TileGate& g, const GroupSel<G>& p) {
+__device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const GateR& g, const GroupSel<G>& p) {
     constexpr auto seq = std::make_integer_sequence<int, (int)kLoc>{};
     switch (g.arm) {
     SQE_PHASE_ARM(kArmPhaseDense, kDense)
@@ -581,6 +589,24 @@ __device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const TileGate& g, co
 #undef SQE_SWAP_HALF
 #undef SQE_PHASE_ARM
 #undef SQE_GROUP_ARMS
+
+// The register-step part of a gate descriptor, loaded whole (FP32 passes skip
+// the FP64 coefficients).
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ GateR load_gate(const TileGate& tg) {
+    GateR r;
+    const u32x16 h = *reinterpret_cast<const u32x16*>(&tg);
+    __builtin_memcpy(&r, &h, 64);
+    if constexpr (std::is_same<T, double>::value) {
+        const u32x4 h2 = *reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(&tg) + 64);
+        __builtin_memcpy(reinterpret_cast<char*>(&r) + 64, &h2, 16);
+    } else {
+        r.c = r.s = 0.0;
+    }
+    return r;
+}
 
 // deposit(x, mask) for a mask given as runs of consecutive bits (host-built):
 // a few instructions per run instead of per mask bit.
@@ -687,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restri
                 for (uint32_t j = 0; j < kLoc; ++j) x.q[q].e[j] = pg[off[j]];
             }
             for (uint32_t k = S.g0; k < S.g1; ++k) {
-                const TileGate& g = a.g[k];
+                const GateR g = load_gate<T>(a.g[k]);
                 if ((base & g.mo) != g.vo) continue;   // uniform: tile outside the gate's selection
                 const uint32_t opar = (uint32_t)__popcll(base & g.yzo);
                 GroupSel<G> p;
