@@ -268,6 +268,9 @@ constexpr int kDense = 0, kGeneral = 1, kHalf = 2;
 // kGroup: every element of the groups that meet the selection's bits off F
 // (and none on F): the dense update on the lanes of those groups only.
 constexpr int kGroup = 8;
+// Added to a Pauli or phase mode: the gate has no parity qubits at all (no Y or
+// Z in its string, no zmask): its sign is uniform and no element tests lpar.
+constexpr int kNoPar = 16;
 
 // Pauli sign: t = (base_neg ^ parity) ? -s : s; tg, ntg are the signed s and
 // its negation for a parity-0 element of this group, flipped where the
@@ -284,10 +287,12 @@ __device__ __forceinline__ T pauli_t(const GateR& g, int elem, T tg, T ntg) {
 template <int OPV, int XL, int SEL, int J, typename T>
 __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const GateR& g, T c, T tg, T ntg, bool act) {
     constexpr int PIV = XL & -XL;
+    constexpr int SM = SEL & (kNoPar - 1);
+    constexpr bool NP = (SEL & kNoPar) != 0;
     if constexpr ((J & PIV) == 0) {
         constexpr int u = J, w = J ^ XL;
-        constexpr bool dense = SEL == kDense || (SEL >= kHalf && (((u >> ((SEL - kHalf) >> 1)) & 1) == ((SEL - kHalf) & 1)));
-        if constexpr (SEL >= kHalf && !dense) return x;   // this pair is outside the half selection
+        constexpr bool dense = SM == kDense || (SM >= kHalf && (((u >> ((SM - kHalf) >> 1)) & 1) == ((SM - kHalf) & 1)));
+        if constexpr (SM >= kHalf && !dense) return x;   // this pair is outside the half selection
         T ru = x.e[u].x, mu = x.e[u].y, rw = x.e[w].x, mw = x.e[w].y;
         constexpr T kh = (T)kTwoSqInv;
         if constexpr (dense) {
@@ -307,7 +312,7 @@ __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const GateR& g, T c, T tg
                 rw = kh * dr;
                 mw = kh * dm;
             } else {
-                const T tu = pauli_t(g, w, tg, ntg), tw = pauli_t(g, u, tg, ntg);
+                const T tu = NP ? tg : pauli_t(g, w, tg, ntg), tw = NP ? tg : pauli_t(g, u, tg, ntg);
                 T pru = (T)0, pmu = (T)0, prw = (T)0, pmw = (T)0;
                 if constexpr (!kFma<T>) {
                     pru = ru * c;
@@ -370,7 +375,7 @@ __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const GateR& g, T c, T tg
                     crw = kh * (ru - rw);
                     cmw = kh * (mu - mw);
                 } else {
-                    const T tu = pauli_t(g, w, tg, ntg), tw = pauli_t(g, u, tg, ntg);
+                    const T tu = NP ? tg : pauli_t(g, w, tg, ntg), tw = NP ? tg : pauli_t(g, u, tg, ntg);
                     if constexpr (kFma<T>) {
                         if constexpr (OPV == kPauliR) {
                             cru = __builtin_fmaf(ru, c, rw * tu);
@@ -413,16 +418,17 @@ __device__ __forceinline__ RM<T> reg_pair_one(RM<T> x, const GateR& g, T c, T tg
 template <int OPV, int XL, int SEL, typename T, int... J>
 __device__ __forceinline__ RM<T> reg_pairs_seq(RM<T> x, const GateR& g, bool gpass, uint32_t gpar,
                                                std::integer_sequence<int, J...>) {
+    constexpr bool NP = (SEL & kNoPar) != 0;
     T c = (T)0, tg = (T)0, ntg = (T)0;
     if constexpr (OPV == kPauliR || OPV == kPauliI) {
         c = gate_c<T>(g);
         const T s = gate_s<T>(g);
-        tg = (g.base_neg ^ gpar) ? -s : s;
+        tg = ((NP ? 0u : gpar) ^ g.base_neg) ? -s : s;
         ntg = -tg;
     }
-    if constexpr (SEL == kGroup) {
+    if constexpr ((SEL & (kNoPar - 1)) == kGroup) {
         if (gpass) {   // per lane: the update runs under the lanes' mask, in place
-            ((x = reg_pair_one<OPV, XL, kDense, J>(x, g, c, tg, ntg, true)), ...);
+            ((x = reg_pair_one<OPV, XL, kDense | (SEL & kNoPar), J>(x, g, c, tg, ntg, true)), ...);
         }
         return x;
     } else {
@@ -436,9 +442,11 @@ __device__ __forceinline__ RM<T> reg_pairs_seq(RM<T> x, const GateR& g, bool gpa
 // one): dense (all 8), half (the elements whose bit B equals V), general.
 template <int SEL, int J, typename T>
 __device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const GateR& g, T c, T sg, T nsg, bool act) {
-    constexpr bool half_on = SEL >= kHalf && (((J >> ((SEL - kHalf) >> 1)) & 1) == ((SEL - kHalf) & 1));
-    if constexpr (SEL >= kHalf && !half_on) return x;
-    const T sn = ((g.lpar >> J) & 1u) ? nsg : sg;
+    constexpr int SM = SEL & (kNoPar - 1);
+    constexpr bool NP = (SEL & kNoPar) != 0;
+    constexpr bool half_on = SM >= kHalf && (((J >> ((SM - kHalf) >> 1)) & 1) == ((SM - kHalf) & 1));
+    if constexpr (SM >= kHalf && !half_on) return x;
+    const T sn = NP ? sg : (((g.lpar >> J) & 1u) ? nsg : sg);
     const T rr = x.e[J].x, mm = x.e[J].y;
     // the next element's inputs pass the same barrier: its products start
     // after this element's, which bounds the registers in flight
@@ -459,7 +467,7 @@ __device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const GateR& g, T c, T s
     }
     x.e[N].x = nr0;
     x.e[N].y = nm0;
-    if constexpr (SEL == kGeneral) {
+    if constexpr (SM == kGeneral) {
         const bool on = (((g.lsel >> J) & 1u) != 0u) & act;
         x.e[J].x = on ? nr : rr;
         x.e[J].y = on ? nm : mm;
@@ -473,11 +481,12 @@ __device__ __forceinline__ RM<T> reg_phase_one(RM<T> x, const GateR& g, T c, T s
 template <int SEL, typename T, int... J>
 __device__ __forceinline__ RM<T> reg_phase_seq(RM<T> x, const GateR& g, bool gpass, uint32_t gpar,
                                                std::integer_sequence<int, J...>) {
+    constexpr bool NP = (SEL & kNoPar) != 0;
     const T c = gate_c<T>(g), s = gate_s<T>(g);
-    const T sg = gpar ? -s : s, nsg = -sg;
-    if constexpr (SEL == kGroup) {
+    const T sg = (NP ? 0u : gpar) ? -s : s, nsg = -sg;
+    if constexpr ((SEL & (kNoPar - 1)) == kGroup) {
         if (gpass) {
-            ((x = reg_phase_one<kDense, J>(x, g, c, sg, nsg, true)), ...);
+            ((x = reg_phase_one<kDense | (SEL & kNoPar), J>(x, g, c, sg, nsg, true)), ...);
         }
         return x;
     } else {
@@ -506,6 +515,11 @@ constexpr uint32_t kArmPairMulti = 6;         // + {3: 0, 5: 1, 6: 2, 7: 3}
 constexpr uint32_t kArmSwapHalf = kArmPair + 16 * 5;   // + 4 log2 xl + 2 (cb index among the other two bits) + cv
 constexpr uint32_t kArmPhaseGroup = kArmSwapHalf + 12;
 constexpr uint32_t kArmPairGroup = kArmPhaseGroup + 1;   // + 3 OPV + log2 xl
+// gates with no parity qubits: phase dense, group, half (+ 2 b + v); Pauli
+// dense and group (+ 3 (imaginary base) + log2 xl)
+constexpr uint32_t kArmPhaseNp = kArmPairGroup + 15;      // dense; + 1 group; + 2 + 2 b + v half
+constexpr uint32_t kArmPauliNpDense = kArmPhaseNp + 8;
+constexpr uint32_t kArmPauliNpGroup = kArmPauliNpDense + 6;
 
 // G groups of a thread go through one dispatch: the gate's uniform work (its
 // descriptor loads, the dispatch, the per-element selection and parity masks)
@@ -543,6 +557,13 @@ struct GroupSel {
     case kArmSwapHalf + 4 * LXL + 2 * K + 1:                                                                 \
         SQE_EACH((reg_pairs_seq<kSwap, XL, kHalf + 2 * CB + 1>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
 #define SQE_PHASE_ARM(CASE, SEL) case CASE: SQE_EACH((reg_phase_seq<SEL>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
+#define SQE_NP_ARMS(OPV, I)                                                                                  \
+    case kArmPauliNpDense + 3 * I + 0: SQE_EACH((reg_pairs_seq<OPV, 1, kDense | kNoPar>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))  \
+    case kArmPauliNpDense + 3 * I + 1: SQE_EACH((reg_pairs_seq<OPV, 2, kDense | kNoPar>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))  \
+    case kArmPauliNpDense + 3 * I + 2: SQE_EACH((reg_pairs_seq<OPV, 4, kDense | kNoPar>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))  \
+    case kArmPauliNpGroup + 3 * I + 0: SQE_EACH((reg_pairs_seq<OPV, 1, kGroup | kNoPar>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))  \
+    case kArmPauliNpGroup + 3 * I + 1: SQE_EACH((reg_pairs_seq<OPV, 2, kGroup | kNoPar>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))  \
+    case kArmPauliNpGroup + 3 * I + 2: SQE_EACH((reg_pairs_seq<OPV, 4, kGroup | kNoPar>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))
 #define SQE_GROUP_ARMS(OPV)                                                                                  \
     case kArmPairGroup + 3 * OPV + 0: SQE_EACH((reg_pairs_seq<OPV, 1, kGroup>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))   \
     case kArmPairGroup + 3 * OPV + 1: SQE_EACH((reg_pairs_seq<OPV, 2, kGroup>(x.q[q_], g, p.pass[q_], p.par[q_], seq)))   \
@@ -580,6 +601,16 @@ __device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const GateR& g, const
     SQE_GROUP_ARMS(kH)
     SQE_GROUP_ARMS(kPauliR)
     SQE_GROUP_ARMS(kPauliI)
+    SQE_PHASE_ARM(kArmPhaseNp + 0, kDense | kNoPar)
+    SQE_PHASE_ARM(kArmPhaseNp + 1, kGroup | kNoPar)
+    SQE_PHASE_ARM(kArmPhaseNp + 2, kHalf + 0 + kNoPar)
+    SQE_PHASE_ARM(kArmPhaseNp + 3, kHalf + 1 + kNoPar)
+    SQE_PHASE_ARM(kArmPhaseNp + 4, kHalf + 2 + kNoPar)
+    SQE_PHASE_ARM(kArmPhaseNp + 5, kHalf + 3 + kNoPar)
+    SQE_PHASE_ARM(kArmPhaseNp + 6, kHalf + 4 + kNoPar)
+    SQE_PHASE_ARM(kArmPhaseNp + 7, kHalf + 5 + kNoPar)
+    SQE_NP_ARMS(kPauliR, 0)
+    SQE_NP_ARMS(kPauliI, 1)
     default: return x;   // never set by the host
     }
 }
@@ -589,6 +620,7 @@ __device__ __forceinline__ RMG<T, G> reg_gate(RMG<T, G> x, const GateR& g, const
 #undef SQE_SWAP_HALF
 #undef SQE_PHASE_ARM
 #undef SQE_GROUP_ARMS
+#undef SQE_NP_ARMS
 
 // The register-step part of a gate descriptor, loaded whole (FP32 passes skip
 // the FP64 coefficients).
@@ -958,12 +990,13 @@ void local_tables(TileGate& g, const uint32_t* f) {
     g.dense = g.gtriv && need != 0 && (g.lsel & need) == need;
     // the code path of reg_gate
     const bool one_local_sel = g.gtriv && __builtin_popcount(mcl) == 1;   // selection: one F bit, nothing else
+    const bool nopar = g.yzc == 0u && g.yzo == 0ull;   // no parity qubit anywhere: a uniform sign
     if (g.kind == (uint32_t)kPhaseOp) {
-        if (g.dense) g.arm = kArmPhaseDense;
-        else if (!g.gtriv && mcl == 0u) g.arm = kArmPhaseGroup;
+        if (g.dense) g.arm = nopar ? kArmPhaseNp : kArmPhaseDense;
+        else if (!g.gtriv && mcl == 0u) g.arm = nopar ? kArmPhaseNp + 1u : kArmPhaseGroup;
         else if (one_local_sel) {
             const uint32_t b = (uint32_t)__builtin_ctz(mcl);
-            g.arm = kArmPhaseHalf + 2u * b + ((vcl >> b) & 1u);
+            g.arm = (nopar ? kArmPhaseNp + 2u : kArmPhaseHalf) + 2u * b + ((vcl >> b) & 1u);
         } else g.arm = kArmPhaseGeneral;
         return;
     }
@@ -972,8 +1005,10 @@ void local_tables(TileGate& g, const uint32_t* f) {
     if (__builtin_popcount(xl) == 1) {
         const uint32_t lx = (uint32_t)__builtin_ctz(xl);
         const uint32_t ctl = mcl & ~xl;   // selection bits on F besides the pivot (selected at 0)
-        if (g.dense) g.arm = base + kArmPairDense + lx;
-        else if (!g.gtriv && ctl == 0u) g.arm = kArmPairGroup + 3u * opv + lx;
+        const bool pauli = g.kind == (uint32_t)OP_PAULI;
+        if (g.dense) g.arm = pauli && nopar ? kArmPauliNpDense + 3u * (opv - kPauliR) + lx : base + kArmPairDense + lx;
+        else if (!g.gtriv && ctl == 0u)
+            g.arm = pauli && nopar ? kArmPauliNpGroup + 3u * (opv - kPauliR) + lx : kArmPairGroup + 3u * opv + lx;
         else if (g.kind == (uint32_t)OP_SWAP && g.gtriv && __builtin_popcount(ctl) == 1) {
             const uint32_t cb = (uint32_t)__builtin_ctz(ctl);
             const uint32_t k = cb > lx ? cb - 1u : cb;   // index of cb among the two other bits
