@@ -192,6 +192,12 @@ void timing_end(suqa_hip_ctx* c) {
     (void)hipEventRecord(c->recs.back().e1, c->stream);
 }
 
+// Span tiles for new contexts and gate queues unless SUQA_HIP_SPAN_TILES=0.
+bool span_tiles_env() {
+    const char* e = std::getenv("SUQA_HIP_SPAN_TILES");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
 suqa_hip_ctx* f32_gate_queue(int device, hipStream_t stream, uint32_t nq, float* re, float* im) {
     if (nq < 1 || nq > 33 || !re || !im) return nullptr;
     auto* q = new suqa_hip_ctx();
@@ -207,6 +213,7 @@ suqa_hip_ctx* f32_gate_queue(int device, hipStream_t stream, uint32_t nq, float*
     q->pauli_reorder = !(e && std::strcmp(e, "0") == 0);
     const char* e2 = std::getenv("SUQA_HIP_GATE_ORDER");
     q->gate_reorder = !(e2 && std::strcmp(e2, "0") == 0);
+    q->span_tiles = span_tiles_env();
     return q;
 }
 
@@ -249,30 +256,32 @@ bool commute(const PauliSym& a, const PauliSym& b) {
 // The order in which run_paulis queues a list with gate passes on: passes
 // packed greedily, an op moving ahead only past ops it commutes with, so the
 // product is the list's.  A pass is a run of at most `max_pass` ops whose
-// flipped qubits fit, with qubits 0..low-1, in `t` qubits (the tile the
-// flush forms, tiled.hip), so the flush cuts the reordered queue into exactly
-// these passes.  Lookahead over `window` ops.
-std::vector<uint32_t> pass_order(const suqa_hip_pauli_op* ops, uint32_t n, uint32_t t, uint32_t low, uint32_t max_pass,
-                                 uint32_t window) {
+// flip masks fit, with qubits 0..low-1, in a tile of `t` coordinates
+// (PassSpace: the span of the flip masks, or the flipped qubits), as the
+// flush forms it (tiled.hip), so the flush cuts the reordered queue into
+// exactly these passes.  Lookahead over `window` ops.
+std::vector<uint32_t> pass_order(const suqa_hip_pauli_op* ops, uint32_t n, uint32_t t, uint32_t low, bool span,
+                                 uint32_t max_pass, uint32_t window) {
     std::vector<PauliSym> sym(n);
     for (uint32_t k = 0; k < n; ++k) sym[k] = pauli_sym(ops[k]);
     std::vector<uint32_t> rest(n), order;
     for (uint32_t k = 0; k < n; ++k) rest[k] = k;
     order.reserve(n);
-    const uint32_t lowmask = (uint32_t)((1ull << std::min(low, t)) - 1ull);
+    const uint32_t f = std::min(low, t);
     while (!rest.empty()) {
-        uint32_t T = lowmask, count = 0;
+        sqe::PassSpace ps;
+        ps.low = (1ull << f) - 1ull;
+        ps.span = span;
+        uint32_t count = 0;
         bool took = true;
         while (took && count < max_pass && !rest.empty()) {
             took = false;
             const size_t lim = std::min<size_t>(rest.size(), window);
             for (size_t j = 0; j < lim; ++j) {
                 const PauliSym& s = sym[rest[j]];
-                if ((uint32_t)__builtin_popcount(T | s.x) > t) continue;
                 bool ok = true;   // commutes with every op it would move ahead of
                 for (size_t i = 0; i < j && ok; ++i) ok = commute(sym[rest[i]], s);
-                if (!ok) continue;
-                T |= s.x;
+                if (!ok || !ps.add(s.x, s.g, t - f)) continue;
                 ++count;
                 order.push_back(rest[j]);
                 rest.erase(rest.begin() + (ptrdiff_t)j);
@@ -292,7 +301,8 @@ std::vector<uint32_t> pass_order(const suqa_hip_pauli_op* ops, uint32_t n, uint3
 
 namespace sqe {
 std::vector<uint32_t> pauli_pass_order(const suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t n) {
-    return pass_order(ops, n, std::min(c->tile_bits, c->nq), c->f32 ? kPassLowFixed32 : kPassLowFixed, kPassMaxGates, 128);
+    return pass_order(ops, n, std::min(c->tile_bits, c->nq), c->f32 ? kPassLowFixed32 : kPassLowFixed, c->span_tiles,
+                      kPassMaxGates, 128);
 }
 }  // namespace sqe
 
@@ -353,6 +363,7 @@ int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out) {
         const char* e = std::getenv("SUQA_HIP_PAULI_ORDER");
         c->pauli_reorder = !(e && std::strcmp(e, "0") == 0);
     }
+    c->span_tiles = sqe::span_tiles_env();
     void* p = nullptr;
     SETUP_HIP(hipMalloc(&p, sizeof(double) * 4));
     c->d_result = static_cast<double*>(p);

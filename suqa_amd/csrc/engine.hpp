@@ -101,6 +101,10 @@ struct suqa_hip_ctx {
     // amplitudes held in LDS (tiled.hip); every other operation flushes first
     uint32_t tile_bits = 0;
     bool pauli_reorder = true;   // run_paulis packs passes with commuting ops moved ahead
+    // span tiles: a pass's tile coordinates above the low qubits are the GF(2)
+    // span of its gates' flip masks (a weight-w Pauli string takes one
+    // coordinate, not w); off: one coordinate per flipped qubit
+    bool span_tiles = true;
     // the FP32 engine's gate queue (sqe::f32_gate_queue): no FP64 state of its
     // own; gates queue as above and its passes run on the float state f32_re,
     // f32_im (one-gate sweeps and every other operation are refused)
@@ -203,6 +207,74 @@ constexpr uint32_t kMaxTileBits32 = 13;   // float tiles (64 KiB of LDS)
 constexpr uint32_t kPassLowFixed = 4;    // qubits 0..3 are in every pass tile (one 128-byte line)
 constexpr uint32_t kPassLowFixed32 = 5;  // the same for float states: qubits 0..4
 constexpr uint32_t kPassMaxGates = 24;   // gates per pass
+// The selection a pass must honour for gate d.  A Pauli string's sweep
+// enumerates each pair (i, i ^ X) once, from the element whose lowest flipped
+// qubit is 0 (that qubit in M at value 0); its arithmetic treats both elements
+// of a pair alike (pair_out_par), so when no other flipped qubit is selected
+// the pass may pick each pair's first element by a tile coordinate instead,
+// and the qubit needs no coordinate of its own.
+inline uint64_t pass_sel(const GateDesc& d) {
+    if (d.kind != OP_PAULI) return d.M;
+    const uint64_t p = d.X & (0ull - d.X);
+    if ((d.M & d.X) == p && (d.V & p) == 0ull) return d.M & ~p;
+    return d.M;
+}
+
+// The tile coordinates that a pass of gates needs above its low qubits.  Span
+// tiles: the GF(2) span of the gates' flip masks (the part above the low
+// qubits), where every selection qubit inside the span's support gets a
+// coordinate of its own (a unit vector: a selection must read one tile
+// coordinate), so dims = |sel & supp| + rank{flip & ~(sel & supp)}.  Qubit
+// tiles (span off): one coordinate per flipped qubit, dims = |supp|.  For
+// gates that flip single qubits both counts are the same.
+struct PassSpace {
+    uint64_t low = 0;          // the low qubits (always whole tile coordinates)
+    bool span = true;
+    uint64_t sel = 0, supp = 0;
+    uint32_t nflips = 0;
+    uint64_t flips[kPassMaxGates] = {};
+    static uint32_t rank_of(const uint64_t* v, uint32_t n, uint64_t clear) {
+        uint64_t basis[64] = {};
+        uint32_t r = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint64_t x = v[i] & ~clear;
+            while (x) {
+                const uint32_t hb = 63u - (uint32_t)__builtin_clzll(x);
+                if (!basis[hb]) {
+                    basis[hb] = x;
+                    ++r;
+                    break;
+                }
+                x ^= basis[hb];
+            }
+        }
+        return r;
+    }
+    uint32_t dims() const {
+        if (!span) return (uint32_t)__builtin_popcountll(supp);
+        const uint64_t q = sel & supp;
+        return (uint32_t)__builtin_popcountll(q) + rank_of(flips, nflips, q);
+    }
+    // Adds a gate with flip mask X and selection mask M if the pass then needs
+    // at most `room` coordinates above the low qubits (and holds fewer than
+    // kPassMaxGates flipping gates); false leaves the space unchanged.
+    bool add(uint64_t X, uint64_t M, uint32_t room) {
+        const uint64_t h = X & ~low;
+        if (h && nflips >= kPassMaxGates) return false;
+        const uint64_t sel0 = sel, supp0 = supp;
+        sel |= M & ~low;
+        supp |= h;
+        if (h) flips[nflips++] = h;
+        if (dims() <= room) return true;
+        sel = sel0;
+        supp = supp0;
+        if (h) --nflips;
+        return false;
+    }
+};
+
+// SUQA_HIP_SPAN_TILES=0 in the environment turns span tiles off (capi.cpp)
+bool span_tiles_env();
 // run_paulis' order of an op list with gate passes on (capi.cpp)
 std::vector<uint32_t> pauli_pass_order(const suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t n);
 constexpr size_t kAutoFlush = 256;   // queued gates that trigger a flush (keeps the GPU busy)

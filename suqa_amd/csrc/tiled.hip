@@ -117,6 +117,11 @@ struct TileArgs {
     uint32_t free_runs[kMaxRuns];
     uint32_t tile_runs[kMaxRuns];   // only the runs fed by the bits of a thread's first element
     uint64_t kbit[3];     // state offsets of the tile bits that chunk index bits 0, 1, 2 stand for
+    // span tiles: tile element e sits at base ^ (XOR of lvec[m] over e's set
+    // bits m); skew = 0 when every lvec is one qubit (then tmask and the runs
+    // give the same offsets)
+    uint32_t skew;
+    uint64_t lvec[kMaxTileBits32];
     struct Step {
         uint32_t f[kSub]; // tile positions of the step's bits, ascending; f[0] = kLdsStep: LDS step
         uint32_t g0, g1;  // gates [g0, g1)
@@ -673,11 +678,22 @@ __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restri
     //    load of the thread is issued before the LDS writes
     const uint32_t e0 = V * threadIdx.x;
     const uint32_t nch = tsz / (V * kBlock);   // 0 when the tile is smaller than V kBlock
-    const uint64_t toff = e0 < tsz ? (base | deposit_runs(e0, a.tile_runs, a.ntile_runs)) : 0ull;
+    uint64_t toff = 0;
+    if (e0 < tsz) {
+        if (a.skew) {   // base is zero at the vectors' pivots: XOR, not OR
+            constexpr uint32_t lv = V == 4 ? 2u : 1u;   // tile bit of thread index bit 0
+            uint64_t l = 0;
+#pragma unroll
+            for (uint32_t m = 0; m < 8; ++m) l ^= ((threadIdx.x >> m) & 1u) ? a.lvec[m + lv] : 0ull;
+            toff = base ^ l;
+        } else {
+            toff = base | deposit_runs(e0, a.tile_runs, a.ntile_runs);
+        }
+    }
     // state offsets of the tile bits that chunk index bits 0, 1, 2 stand for
     const uint64_t* kbit = a.kbit;
     auto chunk_off = [&](uint32_t k) {
-        return ((k & 1u) ? kbit[0] : 0ull) | ((k & 2u) ? kbit[1] : 0ull) | ((k & 4u) ? kbit[2] : 0ull);
+        return ((k & 1u) ? kbit[0] : 0ull) ^ ((k & 2u) ? kbit[1] : 0ull) ^ ((k & 4u) ? kbit[2] : 0ull);
     };
     if (nch == 0) {
         if (e0 < tsz) {
@@ -691,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restri
 #pragma unroll
         for (uint32_t k = 0; k < kMaxChunks; ++k) {
             if (k < nch) {
-                const uint64_t g = toff | chunk_off(k);
+                const uint64_t g = toff ^ chunk_off(k);
                 vr[k] = __builtin_nontemporal_load(reinterpret_cast<const CH*>(re + g));
                 vi[k] = __builtin_nontemporal_load(reinterpret_cast<const CH*>(im + g));
             }
@@ -783,7 +799,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(T* __restrict__ re, T* __restri
 #pragma unroll
         for (uint32_t k = 0; k < kMaxChunks; ++k) {
             if (k < nch) {
-                const uint64_t g = toff | chunk_off(k);
+                const uint64_t g = toff ^ chunk_off(k);
                 CH vr, vi;
 #pragma unroll
                 for (uint32_t i = 0; i < V; ++i) {
@@ -933,22 +949,6 @@ uint32_t mask_runs(uint64_t m, uint32_t* runs, uint32_t src_limit) {
     return n;
 }
 
-uint64_t deposit_host(uint64_t x, uint64_t mask) {
-    uint64_t r = 0;
-    for (uint64_t mm = mask; mm; mm &= mm - 1) {
-        if (x & 1ull) r |= mm & (0ull - mm);
-        x >>= 1;
-    }
-    return r;
-}
-
-uint32_t compact(uint64_t m, const uint8_t* tpos, uint32_t t) {
-    uint32_t o = 0;
-    for (uint32_t j = 0; j < t; ++j)
-        if ((m >> tpos[j]) & 1ull) o |= 1u << j;
-    return o;
-}
-
 hipError_t run_single(suqa_hip_ctx* c, const GateDesc& d) {
     Sel sel;
     if (!make_sel(c->nq, d.M, d.V, &sel)) return hipSuccess;
@@ -1019,35 +1019,181 @@ void local_tables(TileGate& g, const uint32_t* f) {
     }
 }
 
-hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, uint32_t t) {
+uint64_t gate_flips(const GateDesc& d) { return d.kind == kPhaseOp ? 0ull : d.X; }
+
+// The tile of a pass.  Tile coordinate m stands for the state vector vec[m]:
+// element e of the tile with base b is b ^ (XOR of vec[m] over e's set bits m).
+// Coordinates 0..f-1 are the low qubits; the others are, ordered by pivot, a
+// unit vector for every selection qubit inside the flip masks' support, the
+// linearly independent flip masks (above the low qubits, those selection
+// qubits cleared; span tiles only), and unit vectors that pad to t
+// coordinates.  With span tiles off every flipped qubit is a unit vector.
+// Gaussian elimination in coordinate order gives every vector a pivot bit
+// (its lowest bit once reduced by the earlier rows at their pivots); tile
+// bases are zero at the pivots, and the vectors restricted to the pivots form
+// an invertible matrix, so the tiles partition the state.  A selection qubit
+// in some vector's support is a coordinate only when its unit vector is the one
+// vector holding it, which the construction guarantees.
+struct TileBasis {
+    uint32_t t = 0;
+    uint64_t vec[kMaxTileBits32] = {};
+    uint64_t P = 0;        // pivot bits
+    uint64_t TB = 0;       // union of the vectors' supports
+    bool skew = false;     // some vector holds more than one qubit
+    uint64_t row[kMaxTileBits32] = {};   // elimination rows: row[i] = XOR of vec[m] over comb[i]'s bits
+    uint32_t comb[kMaxTileBits32] = {};
+    uint32_t rp[kMaxTileBits32] = {};    // row pivots
+    // coordinates of a state vector in the tile, false when it is outside the span
+    bool coords(uint64_t v, uint32_t* out) const {
+        uint32_t cm = 0;
+        for (uint32_t i = 0; i < t; ++i)
+            if ((v >> rp[i]) & 1ull) {
+                v ^= row[i];
+                cm ^= comb[i];
+            }
+        *out = cm;
+        return v == 0ull;
+    }
+    // the coordinate that reads qubit b alone, -1 when none does
+    int unit_coord(uint32_t b) const {
+        int m0 = -1;
+        for (uint32_t m = 0; m < t; ++m)
+            if ((vec[m] >> b) & 1ull) {
+                if (m0 >= 0 || vec[m] != (1ull << b)) return -1;
+                m0 = (int)m;
+            }
+        return m0;
+    }
+};
+
+bool make_basis(const GateDesc* q, uint32_t n, uint32_t t, uint32_t f, uint32_t nq, bool span, TileBasis& B) {
+    const uint64_t low = (1ull << f) - 1ull;
+    uint64_t S = 0, supp = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        S |= pass_sel(q[i]) & ~low;
+        supp |= gate_flips(q[i]) & ~low;
+    }
+    const uint64_t Q = span ? (S & supp) : supp;
+    struct HV {
+        uint64_t v;
+        uint32_t pivot;
+    };
+    std::vector<HV> hv;
+    for (uint64_t m = Q; m; m &= m - 1) hv.push_back(HV{m & (0ull - m), (uint32_t)__builtin_ctzll(m)});
+    uint64_t hpiv = 0;
+    if (span) {   // independent flip masks, kept whole; pivots by elimination
+        std::vector<uint64_t> red;
+        std::vector<uint32_t> rpv;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t h = gate_flips(q[i]) & ~low & ~Q;
+            if (!h) continue;
+            uint64_t x = h;
+            for (size_t j = 0; j < red.size(); ++j)
+                if ((x >> rpv[j]) & 1ull) x ^= red[j];
+            if (!x) continue;   // in the span already
+            const uint32_t p = (uint32_t)__builtin_ctzll(x);
+            red.push_back(x);
+            rpv.push_back(p);
+            hpiv |= 1ull << p;
+            hv.push_back(HV{h, p});
+        }
+    }
+    if (hv.size() + f > t) return false;
+    uint32_t need = t - f - (uint32_t)hv.size();
+    for (uint32_t b = f; b < nq && need; ++b)   // padding: qubits outside the support first
+        if (!((supp >> b) & 1ull)) {
+            hv.push_back(HV{1ull << b, b});
+            --need;
+        }
+    for (uint32_t b = f; b < nq && need; ++b)   // then support qubits that are neither pivots nor selected
+        if (((supp >> b) & 1ull) && !(((Q | hpiv) >> b) & 1ull)) {
+            hv.push_back(HV{1ull << b, b});
+            --need;
+        }
+    if (need) return false;
+    std::stable_sort(hv.begin(), hv.end(), [](const HV& x, const HV& y) { return x.pivot < y.pivot; });
+    B.t = t;
+    for (uint32_t m = 0; m < f; ++m) B.vec[m] = 1ull << m;
+    for (size_t k = 0; k < hv.size(); ++k) B.vec[f + k] = hv[k].v;
+    B.P = B.TB = 0;
+    B.skew = false;
+    for (uint32_t i = 0; i < t; ++i) {
+        uint64_t x = B.vec[i];
+        uint32_t cm = 1u << i;
+        for (uint32_t j = 0; j < i; ++j)
+            if ((x >> B.rp[j]) & 1ull) {
+                x ^= B.row[j];
+                cm ^= B.comb[j];
+            }
+        if (!x) return false;   // dependent vectors: a planning error
+        B.row[i] = x;
+        B.comb[i] = cm;
+        B.rp[i] = (uint32_t)__builtin_ctzll(x);
+        B.P |= 1ull << B.rp[i];
+        B.TB |= B.vec[i];
+        if (B.vec[i] & (B.vec[i] - 1ull)) B.skew = true;
+    }
+    return true;
+}
+
+hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint32_t t, uint32_t f, bool span) {
     const uint32_t nq = c->nq;
-    for (uint32_t b = 0; b < nq && (uint32_t)__builtin_popcountll(T) < t; ++b) T |= 1ull << b;   // pad
+    TileBasis B;
+    if (!make_basis(q, n, t, f, nq, span, B)) {
+        set_last_error("pass tile: the gates' flip masks do not fit the tile");
+        return hipErrorInvalidValue;
+    }
     TileArgs a;
     std::memset(&a, 0, sizeof(a));
     a.t = t;
-    a.tmask = T;
-    uint8_t tpos[64];   // tile bit positions, ascending (tile index order)
-    uint32_t k = 0;
-    for (uint32_t b = 0; b < nq; ++b)
-        if ((T >> b) & 1ull) tpos[k++] = (uint8_t)b;
+    a.tmask = B.P;
+    a.skew = B.skew ? 1u : 0u;
+    for (uint32_t m = 0; m < t; ++m) a.lvec[m] = B.vec[m];
     a.ngates = n;
     uint64_t mcom = ~0ull, vcom = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const GateDesc& d = q[i];
         TileGate& g = a.g[i];
         g.kind = (uint32_t)d.kind;
-        g.mc = compact(d.M, tpos, t);
-        g.vc = compact(d.V, tpos, t);
-        g.mo = d.M & ~T;
-        g.vo = d.V & ~T;
+        if (d.kind != kPhaseOp && (!B.coords(d.X, &g.xc) || g.xc == 0u)) {
+            set_last_error("pass tile: a flip mask outside the tile");
+            return hipErrorInvalidValue;
+        }
+        // selection: coordinates for qubits the tile reads, tile-level bits for
+        // the rest; a Pauli string's pair enumeration qubit (pass_sel) becomes
+        // its lowest flipped coordinate at 0
+        const uint64_t msel = pass_sel(d);
+        uint32_t mc = msel != d.M ? (g.xc & (0u - g.xc)) : 0u, vc = 0;
+        uint64_t mo = 0, vo = 0;
+        for (uint64_t mm = msel; mm; mm &= mm - 1) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(mm);
+            const uint32_t vb = (uint32_t)((d.V >> b) & 1ull);
+            if ((B.TB >> b) & 1ull) {
+                const int m = B.unit_coord(b);
+                if (m < 0) {
+                    set_last_error("pass tile: a selection qubit is not a tile coordinate");
+                    return hipErrorInvalidValue;
+                }
+                mc |= 1u << m;
+                vc |= vb << m;
+            } else {
+                mo |= 1ull << b;
+                vo |= (uint64_t)vb << b;
+            }
+        }
+        g.mc = mc;
+        g.vc = vc;
+        g.mo = mo;
+        g.vo = vo;
         const uint64_t par = d.kind == kPhaseOp ? d.zmask : d.pc.yz;
-        g.yzc = compact(par, tpos, t);
-        g.yzo = par & ~T;
+        uint32_t yzc = 0;
+        for (uint32_t m = 0; m < t; ++m) yzc |= (uint32_t)(__builtin_popcountll(B.vec[m] & par) & 1) << m;
+        g.yzc = yzc;
+        g.yzo = par & ~B.P;   // bases are zero at the pivots
         if (d.kind == kPhaseOp) {
             g.c = d.c;
             g.s = d.s;
         } else {
-            g.xc = compact(d.X, tpos, t);
             g.pivot = (uint32_t)__builtin_ctz(g.xc);
             g.c = d.pc.c;
             g.s = d.pc.s;
@@ -1062,7 +1208,7 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
         mcom &= ~((g.vo ^ vcom) & mcom);
     }
     vcom &= mcom;
-    const uint64_t fix = T | mcom;
+    const uint64_t fix = B.P | mcom;
     a.freemask = ((nq == 64 ? 0ull : (1ull << nq)) - 1ull) & ~fix;
     a.fixval = vcom;
     a.ntiles = 1ull << __builtin_popcountll(a.freemask);
@@ -1075,12 +1221,13 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint64_t T, 
         set_last_error("tile enumeration mask has more runs than the pass kernel takes");
         return hipErrorInvalidValue;
     }
-    a.ntile_runs = mask_runs(T, a.tile_runs, (uint32_t)__builtin_ctz(vec * (uint32_t)kBlock));
-    for (uint32_t j = 0; j < 3; ++j) a.kbit[j] = deposit_host((uint64_t)(vec * kBlock) << j, T);
+    a.ntile_runs = mask_runs(B.P, a.tile_runs, (uint32_t)__builtin_ctz(vec * (uint32_t)kBlock));
+    const uint32_t cb = (uint32_t)__builtin_ctz(vec * (uint32_t)kBlock);   // tile bit of chunk index bit 0
+    for (uint32_t j = 0; j < 3; ++j) a.kbit[j] = cb + j < t ? B.vec[cb + j] : 0ull;
     // A state that is one small tile (t <= kTinyBits) is one workgroup and the
     // pass is latency-bound: k_tile_tiny, one amplitude per thread (the steps
-    // below are not used).
-    const bool tiny = t <= kTinyBits && a.ntiles == 1;
+    // below are not used; its offsets are deposits of tmask: qubit tiles only).
+    const bool tiny = t <= kTinyBits && a.ntiles == 1 && !B.skew;
     // register steps: runs of gates whose flipped tile bits fit in kSub bits
     a.nsteps = 0;
     uint32_t F = 0, g0 = 0;
@@ -1160,8 +1307,6 @@ uint64_t gate_support(const GateDesc& d) {
     const uint64_t flip = d.kind == kPhaseOp ? 0ull : d.X;
     return flip | d.M | par;
 }
-uint64_t gate_flips(const GateDesc& d) { return d.kind == kPhaseOp ? 0ull : d.X; }
-
 // Gates a, b commute when neither flips a qubit the other reads or writes:
 // on their shared qubits both are then diagonal, so each is a sum over the
 // shared qubits' values of projectors times operators on disjoint qubits.
@@ -1175,13 +1320,15 @@ bool gates_commute(const GateDesc& a, const GateDesc& b) {
 // looking `window` gates ahead.  The product of the gates is unchanged; the
 // rounding is not, which FP32 states accept and FP64 states (bit-identical to
 // the reference's gate order) do not.
-void pack_gates(std::vector<GateDesc>& q, uint32_t t, uint64_t low, size_t window) {
+void pack_gates(std::vector<GateDesc>& q, uint32_t t, uint32_t f, bool span, size_t window) {
     std::vector<GateDesc> rest(q.begin(), q.end()), out;
     out.reserve(q.size());
     size_t head = 0;   // rest[head..] not yet placed (erase from a vector front is linear; keep an index)
     std::vector<char> used(rest.size(), 0);
     while (head < rest.size()) {
-        uint64_t T = low;
+        PassSpace ps;
+        ps.low = (1ull << f) - 1ull;
+        ps.span = span;
         uint32_t count = 0;
         bool took = true;
         while (took && count < kMaxPass && head < rest.size()) {
@@ -1190,14 +1337,12 @@ void pack_gates(std::vector<GateDesc>& q, uint32_t t, uint64_t low, size_t windo
             for (size_t j = head, seen = 0; j < rest.size() && seen < window; ++j) {
                 if (used[j]) continue;
                 ++seen;
-                const uint64_t need = gate_flips(rest[j]);
-                bool ok = (uint32_t)__builtin_popcountll(T | need) <= t;
+                bool ok = true;
                 for (size_t i = 0; ok && i < skipped.size(); ++i) ok = gates_commute(rest[skipped[i]], rest[j]);
-                if (!ok) {
+                if (!ok || !ps.add(gate_flips(rest[j]), pass_sel(rest[j]), t - f)) {
                     skipped.push_back(j);
                     continue;
                 }
-                T |= need;
                 ++count;
                 out.push_back(rest[j]);
                 used[j] = 1;
@@ -1223,30 +1368,27 @@ hipError_t flush(suqa_hip_ctx* c) {
     q.swap(c->queue);
     const uint32_t t = std::min(c->tile_bits, c->nq);
     const uint32_t f = std::min(c->f32 ? kPassLowFixed32 : kLowFixed, t);   // whole 128-byte lines
-    const uint64_t low = (1ull << f) - 1ull;
-    if (c->f32 && c->gate_reorder) pack_gates(q, t, low, 64);
+    const bool span = c->span_tiles && c->nq > t;   // a state of one tile needs no span
+    if (c->f32 && c->gate_reorder) pack_gates(q, t, f, span, 64);
     size_t i = 0;
     while (i < q.size()) {
-        uint64_t T = low;
+        PassSpace ps;
+        ps.low = (1ull << f) - 1ull;
+        ps.span = span;
         size_t j = i;
-        while (j < q.size() && j - i < kMaxPass) {
-            const uint64_t need = q[j].kind == kPhaseOp ? 0ull : q[j].X;
-            if ((uint32_t)__builtin_popcountll(T | need) > t) break;
-            T |= need;
-            ++j;
-        }
+        while (j < q.size() && j - i < kMaxPass && ps.add(gate_flips(q[j]), pass_sel(q[j]), t - f)) ++j;
         hipError_t e;
         if (c->f32) {   // float states have no one-gate sweeps: a lone gate is a pass of one
             if (j == i) {
                 set_last_error("a gate's partner qubits do not fit the FP32 pass tile");
                 return hipErrorInvalidValue;
             }
-            e = run_pass(c, q.data() + i, (uint32_t)(j - i), T, t);
+            e = run_pass(c, q.data() + i, (uint32_t)(j - i), t, f, span);
         } else if (j - i <= 1 || t < kSub) {   // a lone gate, one whose partner bits exceed the tile, or a tiny state
             e = run_single(c, q[i]);
             j = i + 1;
         } else {
-            e = run_pass(c, q.data() + i, (uint32_t)(j - i), T, t);
+            e = run_pass(c, q.data() + i, (uint32_t)(j - i), t, f, span);
         }
         if (e != hipSuccess) return e;
         i = j;

@@ -226,6 +226,72 @@ def test_every_gate_kind_in_passes_matches_one_sweep_per_gate(capi, n, bits, see
     tiled.close()
 
 
+def string_program(n, count, seed):
+    """Pauli strings of weight 2 to 6 (their flip masks span the pass tiles),
+    mixed with controlled gates whose controls and targets fall on qubits the
+    strings flip, and gc_mask controls."""
+    rng = np.random.default_rng(seed)
+    prog = []
+    for _ in range(count):
+        k = int(rng.integers(0, 10))
+        qs = [int(q) for q in rng.permutation(n)]
+        ang = float(rng.uniform(-math.pi, math.pi))
+        if k < 6:
+            w = int(rng.integers(2, 7))
+            prog.append(("pauli_general", qs[:w], [int(t) for t in rng.integers(0, 4, w)], ang))
+        elif k == 6:
+            prog.append(("cx", qs[0], qs[1], int(rng.integers(0, 2))))
+        elif k == 7:
+            nc = int(rng.integers(1, 3))
+            prog.append(("mcu1", qs[:nc], qs[nc], ang, [int(v) for v in rng.integers(0, 2, nc)]))
+        elif k == 8:
+            prog.append(("h", qs[0]))
+        else:
+            prog.append(("gc", qs[:int(rng.integers(1, 3))]))
+    return prog
+
+
+def span_engine(capi, n, bits, span):
+    os.environ["SUQA_HIP_SPAN_TILES"] = "1" if span else "0"
+    try:
+        return engine(capi, n, bits)
+    finally:
+        del os.environ["SUQA_HIP_SPAN_TILES"]
+
+
+@pytest.mark.parametrize("n,bits,seed", [(16, 8, 21), (16, 11, 22), (18, 10, 23), (14, 6, 24), (20, 12, 25)])
+def test_span_tiles_match_one_sweep_per_gate(capi, n, bits, seed):
+    """Pass tiles spanned by the gates' flip masks (coordinates that are sums
+    of qubits, selections on the qubits they mix given coordinates of their
+    own): bit-identical to one sweep per gate and to tiles of whole qubits,
+    with fewer passes than those."""
+    log(f"span tiles n={n} tile bits {bits} seed {seed}")
+    prog = string_program(n, 300, seed)
+    ref = engine(capi, n, 0)
+    span = span_engine(capi, n, bits, True)
+    qubit = span_engine(capi, n, bits, False)
+    for e in (ref, span, qubit):
+        for q in range(n):
+            e.h(q)
+    for chunk in range(3):
+        part = prog[chunk * 100:(chunk + 1) * 100]
+        for e in (ref, span, qubit):
+            play(e, part)
+        p = ref.prob_filter([1, n - 2], [0, 1])
+        assert span.prob_filter([1, n - 2], [0, 1]) == p
+        assert qubit.prob_filter([1, n - 2], [0, 1]) == p
+    r0, i0 = ref.get()
+    for e in (span, qubit):
+        r1, i1 = e.get()
+        assert_bitwise(r1, r0, "re")
+        assert_bitwise(i1, i0, "im")
+    ss, sq = span.pass_stats(), qubit.pass_stats()
+    log(f"  passes: span {ss['passes']} ({ss['single_sweeps']} single), qubit tiles {sq['passes']} ({sq['single_sweeps']} single)")
+    assert ss["passes"] + ss["single_sweeps"] < sq["passes"] + sq["single_sweeps"]
+    for e in (ref, span, qubit):
+        e.close()
+
+
 def qpe_10_4(capi, bits, reorder):
     nsys, nreg = 10, 4
     terms = wl.trotter_terms(nsys, 96, 11)
