@@ -5,9 +5,12 @@
 Headline (c2): 30 qubits, complex FP64 (16 GiB in HBM), |0> then an H layer,
 then 200 layers of 30 random gates from {H, X, RX, RZ, U1, CX, MCX(3)} drawn
 with pcg32 seed 7 (suqa_amd/workloads.py); one step = one layer.  The FP64
-engine applies the gates in passes (runs of consecutive gates whose flipped
-qubits fit one tile, applied to LDS-resident tiles in one HBM round trip,
-bit-identical to one sweep per gate); the tile size is autotuned first on 5
+engine applies the gates in passes (runs of gates whose flipped qubits fit
+one tile, applied to LDS-resident tiles in one HBM round trip); by default a
+pass may take a later gate ahead of earlier ones it commutes with (the same
+product, amplitudes within 1e-12 of call order, extra.gate_order_sample
+compares both orders on 40 layers); --gate-order call keeps call order,
+bit-identical to one sweep per gate.  The tile size is autotuned first on 5
 layers, then the state is re-initialised.  W warmup layers run untimed,
 exactly K layers are timed between stream synchronizations, and the rest of
 the 200 layers run untimed after them, so the checksum and the 8 final
@@ -119,6 +122,30 @@ def cpu_baseline(n=27):
     return dict(value=round(nbytes / dt / 1e9, 2), unit="GB/s (gate-equivalent)", cores=1, kind="port",
                 sample=f"first layer (30 gates) of the seed-7 circuit at {n} qubits, "
                        f"{dt:.1f} s on 1 host core (oracle/suqa_oracle.c, gcc -O3 -ffp-contract=off)")
+
+
+def gate_order_sample(eng, arr, n, order, layers=40):
+    """The first layers of the circuit in both gate orders of the FP64 passes: ms per
+    layer, passes, and the largest checksum difference between the two states."""
+    out = {}
+    for name in ("call", "packed"):
+        eng.set_gate_order(name == "packed")
+        eng.init_state()
+        for q in range(n):
+            eng.h(q)
+        eng.synchronize()
+        p0 = eng.pass_stats()["passes"]
+        t0 = time.perf_counter()
+        eng.run_gates(arr, layers * n, 0)
+        eng.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"ms_per_layer": round(dt / layers * 1e3, 3), "passes": eng.pass_stats()["passes"] - p0,
+                     "checksum": eng.checksum()}
+    eng.set_gate_order(order == "packed")
+    diff = max(abs(x - y) for x, y in zip(out["call"]["checksum"], out["packed"]["checksum"]))
+    return {"layers": layers, "call_order": {k: v for k, v in out["call"].items() if k != "checksum"},
+            "packed": {k: v for k, v in out["packed"].items() if k != "checksum"},
+            "checksum_max_abs_diff": diff}
 
 
 def cpu_fp64_state_c3(n=24, layers=2):
@@ -544,6 +571,8 @@ def main():
                     help="only the 4096-chain QMS batches (profiling runs)")
     ap.add_argument("--short-qpe", action="store_true",
                     help="the whole QPE with 2 Trotter steps per power instead of 20 (profiling runs)")
+    ap.add_argument("--gate-order", choices=("packed", "call"), default="packed",
+                    help="FP64 passes: commuting gates packed (amplitudes within 1e-12 of call order) or call order")
     ap.add_argument("--tile-bits", type=int, default=-1,
                     help="FP64 tile size (0: one sweep per gate); default: autotuned")
     a = ap.parse_args()
@@ -564,6 +593,7 @@ def main():
     peak_copy = hbm_copy_calibration(capi, eng)
     log(f"copy-kernel bandwidth {peak_copy:.0f} GB/s")
     tune = None
+    eng.set_gate_order(a.gate_order == "packed")
     if a.tile_bits < 0:
         best, tms = eng.autotune_tile_bits(capi.gates_from(gates[:5 * n]), (0, 8, 9, 10, 11, 12))
         tune = {"tile_bits": best, "ms_per_gate": {str(k): round(v, 4) for k, v in tms.items()}}
@@ -616,6 +646,7 @@ def main():
     tm = time.perf_counter() - tm
     norm = eng.vnorm()
     checksum_final = eng.checksum()
+    order_cmp = gate_order_sample(eng, arr, n, a.gate_order)
     eng.close()
     if f32 is not None:
         fres["measure_outcomes"] = [f32.measure(q, r) for q, r in zip(mq, mr)]
@@ -642,7 +673,7 @@ def main():
         "data": "synthetic: pcg32 seed-7 random circuit",
         "config": {"workload": "FP64 gate sweeps, random circuit {H,X,RX,RZ,U1,CX,MCX3}",
                    "qubits": n, "gates_per_step": per, "layers_total": layers, "seed": 7,
-                   "tile_bits": eng_tile_bits},
+                   "tile_bits": eng_tile_bits, "gate_order": a.gate_order},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic), "kernel": fam,
@@ -653,6 +684,7 @@ def main():
                                   "what": "HBM bytes all kernels of the timed layers must move / wall time"},
                        "checksum": {"layers": layers, "sum_re": checksum[0], "sum_im": checksum[1],
                                     "sum_index_prob": checksum[2], "after_measurements": list(checksum_final)},
+                       "gate_order_sample": order_cmp,
                        "copy_kernel_gbs": round(peak_copy, 1), "kernel_families": fams,
                        "tile_autotune": tune, "passes": pass_info,
                        "measure_outcomes": outcomes, "measure_seconds": round(tm, 4), "final_norm": norm, "fused": fres,

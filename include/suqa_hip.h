@@ -212,6 +212,16 @@ int suqa_hip_run_gates(suqa_hip_ctx* ctx, const suqa_hip_gate* gates, uint32_t n
  * context takes SUQA_HIP_TILE_BITS from the environment, else 11. */
 int suqa_hip_set_tile_bits(suqa_hip_ctx* ctx, uint32_t bits);
 int suqa_hip_get_tile_bits(suqa_hip_ctx* ctx, uint32_t* bits);
+
+/* Gate order of the passes (extension; the reference applies gates in call
+ * order).  packed = 0 (the default): queued gates run in the order they were
+ * queued, bit-identical to one sweep per gate.  packed = 1: each flush packs
+ * its passes greedily, a gate moving ahead of earlier queued gates only when
+ * it commutes with each of them (neither flips a qubit the other reads): the
+ * product is unchanged, the rounding is not (amplitudes within 1e-12 of call
+ * order at 30 qubits and 200 layers), and there are fewer passes.  Gates
+ * queued before the call keep their order.  The suqa:: C++ API never sets it. */
+int suqa_hip_set_gate_order(suqa_hip_ctx* ctx, int packed);
 /* Autotuner: applies the gate sample once per candidate tile size (so the
  * state changes: re-initialise it afterwards), times it, keeps the fastest
  * in *best and sets it; ms_per_gate[k] (may be NULL) gets each candidate's

@@ -87,6 +87,7 @@ SIGNATURES = {
     "suqa_hip_run_paulis": (C.c_int, [CTX, C.POINTER(PauliOp), U32]),
     "suqa_hip_set_tile_bits": (C.c_int, [CTX, U32]),
     "suqa_hip_get_tile_bits": (C.c_int, [CTX, PU32]),
+    "suqa_hip_set_gate_order": (C.c_int, [CTX, C.c_int]),
     "suqa_hip_autotune_tile_bits": (C.c_int, [CTX, C.POINTER(Gate), U32, PU32, U32, PU32, PD]),
     "suqa_hip_pass_stats": (C.c_int, [CTX, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     # FP32 fused-block engine (extension)
@@ -342,6 +343,10 @@ class Engine:
 
     def set_tile_bits(self, bits):
         check(lib().suqa_hip_set_tile_bits(self.ctx, bits))
+
+    def set_gate_order(self, packed):
+        """0: call order (bit-identical to one sweep per gate); 1: commuting gates packed."""
+        check(lib().suqa_hip_set_gate_order(self.ctx, 1 if packed else 0))
 
     def tile_bits(self):
         v = U32()

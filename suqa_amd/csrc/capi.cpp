@@ -909,6 +909,14 @@ int suqa_hip_set_tile_bits(suqa_hip_ctx* c, uint32_t bits) {
     return SUQA_OK;
 }
 
+int suqa_hip_set_gate_order(suqa_hip_ctx* c, int packed) {
+    SQE_CTX(c);
+    if (packed != 0 && packed != 1) return fail(SUQA_ERR_ARG, "gate order must be 0 (program order) or 1 (packed)");
+    SQE_HIP(sqe::flush(c));   // gates queued so far keep the order they were queued in
+    c->gate_reorder = packed == 1;
+    return SUQA_OK;
+}
+
 int suqa_hip_autotune_tile_bits(suqa_hip_ctx* c, const suqa_hip_gate* gates, uint32_t ngates,
                                 const uint32_t* candidates, uint32_t ncand, uint32_t* best, double* ms_per_gate) {
     SQE_CTX(c);

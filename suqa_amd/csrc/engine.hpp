@@ -111,7 +111,10 @@ struct suqa_hip_ctx {
     bool f32 = false;
     float* f32_re = nullptr;
     float* f32_im = nullptr;
-    bool gate_reorder = false;   // f32 queue: flush packs passes with commuting gates moved ahead
+    // flush packs passes with commuting gates moved ahead (the same product,
+    // another rounding): FP32 queues unless SUQA_HIP_GATE_ORDER=0, FP64
+    // contexts when suqa_hip_set_gate_order asks for it (the suqa:: API never does)
+    bool gate_reorder = false;
     std::vector<sqe::GateDesc> queue;
     uint64_t passes = 0, pass_gates = 0, single_sweeps = 0;
     // captured launch sequences (suqa_hip_graph_*): replayable graphs on the
@@ -278,7 +281,7 @@ bool span_tiles_env();
 // run_paulis' order of an op list with gate passes on (capi.cpp)
 std::vector<uint32_t> pauli_pass_order(const suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t n);
 constexpr size_t kAutoFlush = 256;   // queued gates that trigger a flush (keeps the GPU busy)
-constexpr size_t kAutoFlush32 = 1024;   // the same for the FP32 gate queue (a longer reorder span)
+constexpr size_t kAutoFlush32 = 1024;   // the same for queues that pack commuting gates (a longer reorder span)
 
 // the message suqa_hip_last_error() returns on this thread (capi.cpp)
 void set_last_error(const std::string& msg);

@@ -621,7 +621,7 @@ hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, 
     d.tag = tag;
     d.alg_bytes = alg_bytes;
     c->queue.push_back(d);
-    return c->queue.size() >= (c->f32 ? kAutoFlush32 : kAutoFlush) ? flush(c) : hipSuccess;
+    return c->queue.size() >= ((c->f32 || c->gate_reorder) ? kAutoFlush32 : kAutoFlush) ? flush(c) : hipSuccess;
 }
 
 hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
@@ -637,7 +637,7 @@ hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, u
     d.tag = tag;
     d.alg_bytes = alg_bytes;
     c->queue.push_back(d);
-    return c->queue.size() >= (c->f32 ? kAutoFlush32 : kAutoFlush) ? flush(c) : hipSuccess;
+    return c->queue.size() >= ((c->f32 || c->gate_reorder) ? kAutoFlush32 : kAutoFlush) ? flush(c) : hipSuccess;
 }
 
 hipError_t launch_pair_now(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,

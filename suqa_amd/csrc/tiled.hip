@@ -1314,12 +1314,14 @@ bool gates_commute(const GateDesc& a, const GateDesc& b) {
     return (gate_flips(a) & gate_support(b)) == 0 && (gate_flips(b) & gate_support(a)) == 0;
 }
 
-// The order in which a float gate queue is applied: passes packed greedily
+// The order in which a packing gate queue is applied: passes packed greedily
 // as flush() cuts them (qubits 0..f-1 plus flipped qubits, at most t; at most
 // kMaxPass gates), a gate moving ahead only past gates it commutes with,
 // looking `window` gates ahead.  The product of the gates is unchanged; the
-// rounding is not, which FP32 states accept and FP64 states (bit-identical to
-// the reference's gate order) do not.
+// rounding is not: FP32 queues pack by default, FP64 contexts (whose default
+// is the reference's gate order, bit for bit) only when asked to, for gate
+// lists whose tolerance is a bound (amplitudes within 1e-12) rather than
+// bit-identity.
 void pack_gates(std::vector<GateDesc>& q, uint32_t t, uint32_t f, bool span, size_t window) {
     std::vector<GateDesc> rest(q.begin(), q.end()), out;
     out.reserve(q.size());
@@ -1369,7 +1371,7 @@ hipError_t flush(suqa_hip_ctx* c) {
     const uint32_t t = std::min(c->tile_bits, c->nq);
     const uint32_t f = std::min(c->f32 ? kPassLowFixed32 : kLowFixed, t);   // whole 128-byte lines
     const bool span = c->span_tiles && c->nq > t;   // a state of one tile needs no span
-    if (c->f32 && c->gate_reorder) pack_gates(q, t, f, span, 64);
+    if (c->gate_reorder) pack_gates(q, t, f, span, 64);
     size_t i = 0;
     while (i < q.size()) {
         PassSpace ps;
