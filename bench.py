@@ -55,6 +55,7 @@ extra, per workload (each with its roofline and its CPU leg):
                       FP64 and FP32, chain-steps/s, the LDS-bandwidth fraction
                       (algorithmic LDS bytes / kernel time against 256 CUs x
                       128 B/clk x 2.4 GHz), per-beta <E> of FP32 against FP64,
+                      the FP32 chains' final-state infidelity against FP64,
                       sample chains against the oracle, and the oracle's
                       two-register chains over the host cores beside it; the
                       same batch in the master one-register layout too.
@@ -263,6 +264,19 @@ def per_beta(chains, b64, b32):
     return bins
 
 
+def state_infidelity(o64, o32, bound=1e-5):
+    """Each chain's state after its last step, FP32 against FP64, over the chains
+    whose decisions agree: 1 - |<psi64|psi32>|^2 / (|psi64|^2 |psi32|^2)."""
+    inf = []
+    for c in range(len(o64["final"])):
+        if np.array_equal(o64["rets"][c], o32["rets"][c]):
+            a, b = o64["final"][c], o32["final"][c]
+            inf.append(max(0.0, 1.0 - abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)))
+    inf = np.array(inf)
+    return {"chains": int(len(inf)), "max": float(inf.max()), "mean": float(inf.mean()),
+            "median": float(np.median(inf)), "above_1e-5": int((inf > bound).sum())}
+
+
 def qms_legs(no_cpu):
     """Plain FP64 GPU QMS (c1), batched chains (c5), their oracle checks and CPU rates."""
     from suqa_amd import qms
@@ -293,12 +307,14 @@ def qms_legs(no_cpu):
     log(f"qms plain, two registers: {steps / two['seconds']:.1f} steps/s")
     nch, bsteps = 4096, 500
     chains = batch_chains(nch)
-    runs = {}
-    # each batch is run once untimed first: the GPU's clocks after the previous
-    # leg (a different kernel, another load) would otherwise set the figure
+    runs, finals = {}, {}
+    # each batch is run once untimed first (with the chains' final states, for
+    # the FP32 infidelity): the GPU's clocks after the previous leg (a different
+    # kernel, another load) would otherwise set the figure
     for two_reg in (True, False):
         for fp32 in (False, True):
-            qms.batch_run(chains, bsteps, 1, 100, emin, emax, 10, 100, fp32=fp32, two_registers=two_reg)
+            finals[(two_reg, fp32)] = qms.batch_run(chains, bsteps, 1, 100, emin, emax, 10, 100, fp32=fp32,
+                                                    two_registers=two_reg, final_state=True)
             runs[(two_reg, fp32)] = qms.batch_run(chains, bsteps, 1, 100, emin, emax, 10, 100, fp32=fp32,
                                                   two_registers=two_reg, lds_bytes=True)
     for two_reg, key in ((True, "qms_batch"), (False, "qms_batch_one_register")):
@@ -311,9 +327,12 @@ def qms_legs(no_cpu):
         same = int(sum(1 for c in range(nch) if np.array_equal(b32["rets"][c], b["rets"][c])))
         res[key + "_fp32"] = dict(batch_figures(b32, nch, bsteps, plain_rate),
                                   dtype="f32 amplitudes, f64 probabilities", chains_identical_to_fp64=same,
+                                  state_infidelity=state_infidelity(finals[(two_reg, False)],
+                                                                    finals[(two_reg, True)]),
                                   per_beta_E=per_beta(chains, b, b32))
         log(f"{key}: {res[key]['chain_steps_per_s']:.0f} chain-steps/s, fp32 "
-            f"{res[key + '_fp32']['chain_steps_per_s']:.0f} ({same}/{nch} chains identical to FP64), "
+            f"{res[key + '_fp32']['chain_steps_per_s']:.0f} ({same}/{nch} chains identical to FP64, state "
+            f"infidelity max {res[key + '_fp32']['state_infidelity']['max']:.2g}), "
             f"LDS {res[key]['lds_roofline']['frac']:.3f} / {res[key + '_fp32']['lds_roofline']['frac']:.3f} of peak")
     if no_cpu:
         return res

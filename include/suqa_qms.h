@@ -91,6 +91,10 @@ typedef struct {
     uint64_t* lds_accesses;   /* [nchains] or NULL: algorithmic LDS bytes of each chain (the
                                  amplitude reads and writes its gates and measurements must
                                  make); 0 for chains of the one-amplitude-per-thread layout */
+    double*   final_state;    /* [2 * sum over chains of 2^(ene_qbits + 4)] or NULL: each chain's
+                                 stored amplitudes after its last step as FP64 (re, im) pairs,
+                                 chains in input order (two registers: the compact state, with
+                                 E_old held by the value its register reads) */
 } suqa_qms_batch_result;
 
 int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains, const suqa_qms_batch_params* p,

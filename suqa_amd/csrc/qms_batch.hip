@@ -295,6 +295,14 @@ struct LdsOps : ChainBase {
         return c;
     }
 
+    // the stored amplitudes as FP64 (re, im) pairs (after the last step)
+    __device__ __forceinline__ void dump(double* dst) const {
+        for (uint32_t i = lane; i < N; i += 64 * TW) {
+            dst[2 * i] = (double)st[i].r;
+            dst[2 * i + 1] = (double)st[i].i;
+        }
+    }
+
     // qms::apply_W, dense form (qms.cuh:286-307); acc is the top qubit, E_old
     // the classical value (two registers: the value the E_old register holds).
     __device__ __forceinline__ void W(uint32_t Eold) {
@@ -456,6 +464,11 @@ struct RegOps : ChainBase {
         vr = T((double)vr * f);
         vi = T((double)vi * f);
         return c;
+    }
+
+    __device__ __forceinline__ void dump(double* dst) const {
+        dst[2 * idx] = (double)vr;
+        dst[2 * idx + 1] = (double)vi;
     }
 
     __device__ __forceinline__ void W(uint32_t Eold) {   // qms.cuh:286-307: pairs (j, j + N/2)
@@ -716,6 +729,7 @@ struct Chain : Ops {
             b.accepted[chain] = acc_count;
             if (b.lds_amps) b.lds_amps[chain] = this->acc_amps;
         }
+        if (b.final_state) this->dump(b.final_state + P->state_off);
     }
 };
 
@@ -861,10 +875,12 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     int rc = SUQA_OK;
     std::vector<ChainDev> hc(nchains);
     std::vector<std::vector<double>> wt(nchains);
-    size_t wtotal = 0;
+    size_t wtotal = 0, stotal = 0;
     for (uint32_t c = 0; c < nchains; ++c) {
         fill_chain(chains[c], *p, hc[c], wt[c]);
         wtotal += wt[c].size();
+        hc[c].state_off = stotal;
+        stotal += 2 * ((size_t)1 << (chains[c].ene_qbits + 4));
     }
     ChainDev* d_chains = nullptr;
     double *d_w = nullptr, *d_E = nullptr, *d_X = nullptr;
@@ -872,6 +888,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     uint64_t* d_rng = nullptr;
     uint32_t *d_nm = nullptr, *d_acc = nullptr, *d_next = nullptr, *d_order = nullptr;
     uint64_t* d_lds = nullptr;
+    double* d_final = nullptr;
     std::vector<size_t> need(nchains);
     std::vector<uint32_t> order(nchains);
     hipStream_t st = nullptr;
@@ -906,6 +923,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     QB_HIP(hipMalloc(&d_acc, sizeof(uint32_t) * nchains));
     QB_HIP(hipMalloc(&d_next, sizeof(uint32_t)));
     if (out->lds_accesses) QB_HIP(hipMalloc(&d_lds, sizeof(uint64_t) * nchains));
+    if (out->final_state) QB_HIP(hipMalloc(&d_final, sizeof(double) * stotal));
     // chains grouped by the LDS their state needs, widest first: when LDS
     // limits the resident waves, one launch per group, so narrow chains are not
     // held to the widest chain's occupancy
@@ -934,6 +952,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     b.accepted = d_acc;
     b.two_registers = p->two_registers ? 1u : 0u;
     b.lds_amps = d_lds;
+    b.final_state = d_final;
     {
         int dev = 0;
         QB_HIP(hipGetDevice(&dev));
@@ -1025,6 +1044,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
         QB_HIP(hipMemcpy(out->lds_accesses, d_lds, sizeof(uint64_t) * nchains, hipMemcpyDeviceToHost));
         for (uint32_t c = 0; c < nchains; ++c) out->lds_accesses[c] *= 2 * elt;   // accesses -> bytes
     }
+    if (d_final) QB_HIP(hipMemcpy(out->final_state, d_final, sizeof(double) * stotal, hipMemcpyDeviceToHost));
 done:
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1040,6 +1060,7 @@ done:
     (void)hipFree(d_next);
     (void)hipFree(d_order);
     (void)hipFree(d_lds);
+    (void)hipFree(d_final);
     if (st) (void)hipStreamDestroy(st);
     return rc;
 }

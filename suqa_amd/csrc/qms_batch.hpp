@@ -17,6 +17,7 @@ struct ChainDev {
     uint64_t seed;
     uint32_t r;                             // energy qubits
     uint32_t pad_;
+    uint64_t state_off;                     // first double of the chain's final state in BatchDev::final_state
     double xx_c[2][kMaxEne], xx_s[2][kMaxEne];      // [Phi, Phi^dag][trg]
     double u1_c[2][kMaxEne], u1_s[2][kMaxEne];      // energy shift u1 on register qubit trg
     double qft_c[2][kMaxEne + 2], qft_s[2][kMaxEne + 2];  // [+, -][m]: crm phase +-2 pi / 2^m
@@ -42,6 +43,8 @@ struct BatchDev {
     uint32_t* accepted;                     // [nchains]
     uint32_t two_registers;                 // E_old in its own register: 2r + 4 qubits per chain
     uint64_t* lds_amps;                     // [nchains] LDS amplitude accesses per chain, or null
+    double* final_state;                    // the chains' stored amplitudes after the last step,
+                                            // (re, im) in FP64, at ChainDev::state_off; or null
 };
 
 }  // namespace qb

@@ -41,7 +41,7 @@ class BatchResult(C.Structure):
     _fields_ = [("rets", C.POINTER(C.c_int32)), ("rng_after", C.POINTER(C.c_uint64)),
                 ("E", C.POINTER(C.c_double)), ("X", C.POINTER(C.c_double)), ("cap_meas", C.c_uint32),
                 ("n_meas", C.POINTER(C.c_uint32)), ("accepted", C.POINTER(C.c_uint32)), ("seconds", C.c_double),
-                ("lds_accesses", C.POINTER(C.c_uint64))]
+                ("lds_accesses", C.POINTER(C.c_uint64)), ("final_state", C.POINTER(C.c_double))]
 
 
 SIGNATURES = {
@@ -102,12 +102,14 @@ def run(beta, steps, reset_each, r, seed, ene_min=0.0, ene_max=1.0, pe_steps=10,
 
 
 def batch_run(chains, steps, reset_each=1, max_reverse=100, ene_min=0.0, ene_max=1.0, pe_steps=10,
-              thermalization=100, cap_meas=None, rng_after=False, fp32=False, two_registers=False, lds_bytes=False):
+              thermalization=100, cap_meas=None, rng_after=False, fp32=False, two_registers=False, lds_bytes=False,
+              final_state=False):
     """chains: list of (beta, r, seed).  Returns per-chain arrays.  fp32: FP32
     amplitudes with FP64 probabilities (the precision pattern).  two_registers:
     E_old in its own register (2r + 4 qubits per chain, r in 2..6, stored in
     2^(r+4) amplitudes: E_old is definite outside its own phase estimation).
-    lds_bytes: also return each chain's algorithmic LDS bytes."""
+    lds_bytes: also return each chain's algorithmic LDS bytes.  final_state:
+    also return each chain's stored amplitudes after its last step (complex128)."""
     n = len(chains)
     specs = (ChainSpec * n)(*[ChainSpec(b, r, s) for (b, r, s) in chains])
     bp = BatchParams(steps, reset_each, max_reverse, ene_min, ene_max, pe_steps, thermalization, 1 if fp32 else 0,
@@ -126,7 +128,16 @@ def batch_run(chains, steps, reset_each=1, max_reverse=100, ene_min=0.0, ene_max
     res.n_meas, res.accepted = _ptr(nm, C.c_uint32), _ptr(acc, C.c_uint32)
     lds = np.zeros(n, dtype=np.uint64) if lds_bytes else None
     res.lds_accesses = _ptr(lds, C.c_uint64) if lds_bytes else C.POINTER(C.c_uint64)()
+    sizes = [1 << (r + 4) for (_, r, _) in chains]
+    fin = np.zeros(2 * sum(sizes)) if final_state else None
+    res.final_state = _ptr(fin, C.c_double) if final_state else C.POINTER(C.c_double)()
     rc = lib().suqa_qms_batch_run(specs, n, C.byref(bp), C.byref(res))
     if rc != 0:
         raise capi.SuqaError(rc, "suqa_qms_batch_run failed (see stderr)")
-    return dict(rets=rets, rng_after=rng, E=E, X=X, n_meas=nm, accepted=acc, seconds=res.seconds, lds_bytes=lds)
+    states = None
+    if final_state:
+        z = fin.view(np.complex128)
+        offs = np.concatenate(([0], np.cumsum(sizes)))
+        states = [z[offs[c]:offs[c + 1]] for c in range(n)]
+    return dict(rets=rets, rng_after=rng, E=E, X=X, n_meas=nm, accepted=acc, seconds=res.seconds, lds_bytes=lds,
+                final=states)

@@ -49,6 +49,12 @@ def test_plain_qms_vs_golden(qms, fx, graphs, monkeypatch):
     np.testing.assert_allclose(out["final"][1], im, atol=1e-12, rtol=0)
 
 
+
+def infidelity(a, b):
+    """1 - |<a|b>|^2 / (<a|a> <b|b>)."""
+    return 1.0 - abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
 def test_driver_output_file_matches_reference(qms):
     log("suqa_qms driver file")
     with tempfile.TemporaryDirectory() as td:
@@ -140,8 +146,9 @@ def test_batch_fp32_chains_follow_fp64(qms, widths):
     chains = [(float(0.1 + 4.9 * k / 47), int(2 + k % widths), int(1000 + k)) for k in range(48)]
     steps, reset_each, therm, maxrev, emin, emax = 120, 2, 10, 30, -1.0, 3.0
     log("batch fp32 chains")
-    out = qms.batch_run(chains, steps, reset_each, maxrev, emin, emax, 10, therm, fp32=True)
-    same = 0
+    out = qms.batch_run(chains, steps, reset_each, maxrev, emin, emax, 10, therm, fp32=True, final_state=True)
+    o64 = qms.batch_run(chains, steps, reset_each, maxrev, emin, emax, 10, therm, final_state=True)
+    same, worst = 0, 0.0
     for c, (beta, r, seed) in enumerate(chains):
         rets, E, X = oracle_rets(beta, r, seed, steps, reset_each, therm, maxrev, emin, emax)
         if list(out["rets"][c]) == rets:
@@ -150,5 +157,36 @@ def test_batch_fp32_chains_follow_fp64(qms, widths):
             assert nm == len(E)
             assert_bitwise(out["E"][c][:nm], np.array(E), f"E chain {c}")
             assert_bitwise(out["X"][c][:nm], np.array(X), f"X chain {c}")
-    log(f"{same} of {len(chains)} FP32 chains identical to the FP64 oracle")
+            assert list(o64["rets"][c]) == rets
+            worst = max(worst, infidelity(o64["final"][c], out["final"][c]))
+    log(f"{same} of {len(chains)} FP32 chains identical to the FP64 oracle, largest state infidelity {worst:.3g}")
     assert same >= 44
+    assert worst <= 1e-5   # the precision pattern's bound on the state
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_batch_final_state_vs_plain_path(qms, fp32):
+    """The batch's final-state output against the plain FP64 path's state
+    after the same chain (same amplitude order): within 1e-12 in FP64 (the
+    measurement sums are summed in another order), infidelity <= 1e-5 in FP32;
+    both the LDS layout (a batch larger than the CU count) and, in FP64, the
+    one-amplitude-per-thread layout (a single chain)."""
+    steps, reset_each, therm, maxrev, emin, emax = 60, 1, 5, 30, -1.0, 3.0
+    picks = [(0.7, 3, 77), (2.5, 5, 91)]
+    filler = [(1.0, 2, 5000 + k) for k in range(300)]   # more chains than CUs: the LDS layout
+    for beta, r, seed in picks:
+        plain = qms.run(beta, steps, reset_each, r, seed, emin, emax, 10, therm, maxrev, final_state=True)
+        ref = plain["final"][0] + 1j * plain["final"][1]
+        for chains in ([(beta, r, seed)] + filler, [(beta, r, seed)]):
+            if fp32 and len(chains) == 1:
+                continue
+            log(f"final state beta {beta} r {r} seed {seed}, {len(chains)} chains, fp32 {fp32}")
+            out = qms.batch_run(chains, steps, reset_each, maxrev, emin, emax, 10, therm, fp32=fp32,
+                                final_state=True)
+            assert list(out["rets"][0]) == list(plain["rets"])
+            got = out["final"][0]
+            assert got.shape == ref.shape
+            if fp32:
+                assert infidelity(ref, got) <= 1e-5
+            else:
+                np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
