@@ -373,6 +373,11 @@ int suqa_hip_setup(uint32_t nq, suqa_hip_ctx** out) {
     c->d_flags = static_cast<uint32_t*>(p);
     SETUP_HIP(hipHostMalloc(&p, sizeof(uint32_t) * sqe::kMeasMax, hipHostMallocDefault));
     c->h_flags = static_cast<uint32_t*>(p);
+    SETUP_HIP(hipMalloc(&p, sizeof(uint32_t) * sqe::kSlots));
+    c->d_slots = static_cast<uint32_t*>(p);
+    SETUP_HIP(hipMemsetAsync(c->d_slots, 0, sizeof(uint32_t) * sqe::kSlots, c->stream));
+    SETUP_HIP(hipHostMalloc(&p, sizeof(uint32_t) * sqe::kSlots, hipHostMallocDefault));
+    c->h_slots = static_cast<uint32_t*>(p);
 #undef SETUP_HIP
     int rc = suqa_hip_allocate_state(c, nq);
     if (rc) {
@@ -402,6 +407,8 @@ int suqa_hip_clear(suqa_hip_ctx* c) {
     if (c->h_result) (void)hipHostFree(c->h_result);
     if (c->d_flags) (void)hipFree(c->d_flags);
     if (c->h_flags) (void)hipHostFree(c->h_flags);
+    if (c->d_slots) (void)hipFree(c->d_slots);
+    if (c->h_slots) (void)hipHostFree(c->h_slots);
     if (c->d_wtab) (void)hipFree(c->d_wtab);
     if (c->d_wrow) (void)hipFree(c->d_wrow);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -685,6 +692,20 @@ int suqa_hip_apply_qft_inverse(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, 
 static int measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, bool reset,
                          uint32_t gc, uint32_t* cs) {
     if (cs && c->capturing) return fail(SUQA_ERR_ARG, "a measurement cannot be read back while a graph is being captured");
+    if (sqe::tiny_prog(c)) {   // queued beside the gates: one launch for the queue and the measurements
+        if (n > sqe::kSlots) return fail(SUQA_ERR_ARG, "too many measurements in one call");
+        const uint32_t slot = sqe::alloc_slots(c, n);
+        sqe::queue_measure(c, qs, rds, n, reset, gc, slot);
+        if (!cs) {
+            if (c->queue.size() >= sqe::kAutoFlush) SQE_HIP(sqe::flush(c));
+            return SUQA_OK;
+        }
+        SQE_HIP(sqe::flush(c));
+        SQE_HIP(hipMemcpyAsync(c->h_slots, c->d_slots + slot, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+        SQE_HIP(hipStreamSynchronize(c->stream));
+        for (uint32_t k = 0; k < n; ++k) cs[k] = c->h_slots[k];
+        return SUQA_OK;
+    }
     for (uint32_t k0 = 0; k0 < n; k0 += sqe::kMeasMax) {
         const uint32_t m = std::min<uint32_t>(sqe::kMeasMax, n - k0);
         SQE_HIP(sqe::launch_measure_small(c, qs + k0, rds + k0, m, reset ? ~0u : 0u, gc));
@@ -790,6 +811,7 @@ void w_coef(uint64_t En, uint64_t Eo, double cf, double* out) {
 // (L <= 256), rebuilt when c_factor, the mask or the offset change.
 int w_table(suqa_hip_ctx* c, uint32_t wmask, uint32_t b0, double cf, uint64_t L) {
     if (c->wtab_valid && c->wtab_c == cf && c->wtab_mask == wmask && c->wtab_b0 == b0) return SUQA_OK;
+    SQE_HIP(sqe::flush(c));   // a queued apply_W reads the table it was queued with
     std::vector<double> tab(2 * L * L);
     for (uint64_t eo = 0; eo < L; ++eo)
         for (uint64_t en = 0; en < L; ++en) w_coef(en, eo, cf, &tab[2 * (eo * L + en)]);
@@ -833,6 +855,7 @@ int suqa_hip_qms_apply_W(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmask, uint32
     } else {
         std::vector<double> r(2 * L);
         for (uint64_t en = 0; en < L; ++en) w_coef(en, Eold, cf, &r[2 * en]);
+        SQE_HIP(sqe::flush(c));   // a queued apply_W reads the row it was queued with
         if (c->wrow_cap < L) {
             if (c->d_wrow) SQE_HIP(hipFree(c->d_wrow));
             c->d_wrow = nullptr;

@@ -51,14 +51,29 @@ struct PauliCoef {
 // pairs (i, i ^ X) with (i & M) == V, or a phase rotation (kPhaseOp) of the
 // amplitudes with (i & M) == V, s negated where par(i & zmask) == 1.
 constexpr int kPhaseOp = 4;
+// Queued operations other than gates, for states of one small tile only
+// (tiny_prog below): a measurement of qubit q with draw c, its outcome into
+// outcome slot `slot`, and for a reset X on q over the gc bits (M) when the
+// outcome is 1; qms::apply_W on the acceptance pairs.
+constexpr int kMeasOp = 5, kWOp = 6;
+constexpr uint32_t kNoRef = 0xffffffffu;
 struct GateDesc {
     int kind;
     uint64_t M, V, X;
     PauliCoef pc;        // pair operations
-    double c, s;         // phase rotations
+    double c, s;         // phase rotations; kMeasOp: c = the draw
     uint64_t zmask;
     const char* tag;
     double alg_bytes;
+    // tiny_prog operations: a gate with cond_ref != kNoRef applies only when
+    // outcome slot cond_ref reads 1; kMeasOp: q, slot, reset; kWOp: the
+    // (fs1, fs2) table, the register masks and E_old (a value, the outcome
+    // slots eold_ref .. + eold_bits - 1 read LSB first, or omask bits)
+    uint32_t cond_ref = kNoRef;
+    uint32_t q = 0, slot = 0, reset = 0;
+    const double* tab = nullptr;
+    uint32_t wmask = 0, b0 = 0, omask = 0, ob0 = 0, L = 0;
+    uint32_t eold = 0, eold_ref = kNoRef, eold_bits = 0;
 };
 
 struct KStat {
@@ -85,6 +100,11 @@ struct suqa_hip_ctx {
     double* h_result = nullptr;     // pinned, 4 doubles
     uint32_t* d_flags = nullptr;    // outcomes of launch_measure_small (sqe::kMeasMax)
     uint32_t* h_flags = nullptr;    // pinned copy
+    // outcome slots of queued measurements (tiny_prog states): a ring of
+    // sqe::kSlots outcomes on the device, a pinned copy, the next free slot
+    uint32_t* d_slots = nullptr;
+    uint32_t* h_slots = nullptr;
+    uint32_t slot_next = 0;
     // qms::apply_W coefficient table (fs1, fs2) per (E_old, E_new)
     double* d_wtab = nullptr;
     uint64_t wtab_L = 0;
@@ -205,6 +225,29 @@ hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double s
                             const char* tag, double alg_bytes);
 // Applies the queued gates in tile passes (tiled.hip); a no-op when empty.
 hipError_t flush(suqa_hip_ctx* c);
+// States of at most 2^kTinyProgBits amplitudes that are one pass tile
+// (tile_bits >= nq) are latency-bound: measurements, resets and apply_W queue
+// beside the gates, and a flush runs the queue as programs of up to kProgMax
+// operations, one single-workgroup launch each (k_tiny_prog, tiled.hip), with
+// the one-gate sweeps' arithmetic and launch_measure_small's reduction order
+// (bit-identical results).  Outcomes go to a ring of kSlots device slots.
+constexpr uint32_t kTinyProgBits = 10, kProgMax = 96, kSlots = 1024;
+inline bool tiny_prog(const suqa_hip_ctx* c) {
+    return !c->f32 && c->tile_bits >= c->nq && c->nq >= 1 && c->nq <= kTinyProgBits && c->d_slots;
+}
+// n consecutive outcome slots (n <= kSlots); the ring wraps to 0 when fewer
+// are left before its end.  A slot keeps its outcome until kSlots later
+// slots have been handed out.
+inline uint32_t alloc_slots(suqa_hip_ctx* c, uint32_t n) {
+    if (c->slot_next + n > kSlots) c->slot_next = 0;
+    const uint32_t b = c->slot_next;
+    c->slot_next += n;
+    return b;
+}
+// Queues the measurements of qs[0..n) (draws rds, consecutive slots from
+// `slot`), resets when `reset` (X over the gc bits when the outcome is 1).
+void queue_measure(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, bool reset, uint32_t gc,
+                   uint32_t slot);
 constexpr uint32_t kMinTileBits = 1, kMaxTileBits = 12;
 constexpr uint32_t kMaxTileBits32 = 13;   // float tiles (64 KiB of LDS)
 constexpr uint32_t kPassLowFixed = 4;    // qubits 0..3 are in every pass tile (one 128-byte line)

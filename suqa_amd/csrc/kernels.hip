@@ -892,6 +892,19 @@ hipError_t launch_init(suqa_hip_ctx* c) {
 
 hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_tab, uint32_t wmask, uint32_t b0, uint32_t omask,
                         uint32_t ob0, uint32_t L) {
+    if (tiny_prog(c)) {   // queued beside the gates
+        GateDesc d{};
+        d.kind = kWOp;
+        d.tab = d_tab;
+        d.wmask = wmask;
+        d.b0 = b0;
+        d.omask = omask;
+        d.ob0 = ob0;
+        d.L = L;
+        d.tag = "qms_W";
+        c->queue.push_back(d);
+        return c->queue.size() >= kAutoFlush ? flush(c) : hipSuccess;
+    }
     if (hipError_t e = flush(c)) return e;
     const uint64_t half = c->N / 2;
     timing_begin(c, "qms_W", 32.0 * (double)c->N);

@@ -932,6 +932,230 @@ __global__ __launch_bounds__(1024) void k_tile_tiny(T* __restrict__ re, T* __res
     im[off] = vi;
 }
 
+// ------------------------------------------------------------ tiny programs
+// A queued operation of a state of at most 2^kTinyProgBits amplitudes, as
+// k_tiny_prog reads it (40 bytes).  The state is one tile: every mask is over
+// the state's own bits.
+struct ProgOp {
+    uint32_t kf;    // bits 0-3 kind; 4 base_imag; 5 base_neg; 6 reset; 7 the refs are op indices of
+                    // this program (else outcome slots); 8 conditional gate; 9-13 pivot; 16-23 E_old bits
+    uint32_t x;     // pair: partner mask; measurement: the qubit's bit; W: E_new mask
+    uint32_t m, v;  // selection (gates); measurement: gc bits in m; W: E_old mask in m, b0 | ob0 << 8 in v
+    uint32_t yz;    // parity mask (gates); measurement: its slot; W: L
+    uint32_t ref;   // conditional gate: the outcome it reads; W: E_old's first outcome, or E_old itself
+    double c, s;    // gate coefficients; measurement: the draw in c; W: the table's address in c
+};
+static_assert(sizeof(ProgOp) == 40, "ProgOp layout");
+constexpr uint32_t kPfImag = 1u << 4, kPfNeg = 1u << 5, kPfReset = 1u << 6, kPfLocal = 1u << 7, kPfCond = 1u << 8;
+struct ProgArgs {
+    double* re;
+    double* im;
+    uint32_t* slots;   // the outcome ring
+    uint32_t nops, nq;
+    ProgOp op[kProgMax];
+};
+static_assert(sizeof(ProgArgs) <= 4096, "kernel arguments are limited to 4 KiB");
+
+__device__ __forceinline__ double wave_sum64(double v) {   // the butterfly of kernels.hip's wave_sum
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One workgroup of max(N, kBlock) threads, amplitude y in thread y < N, the
+// program's operations in order: gates as k_tile_tiny (bit-identical to one
+// sweep per gate), measurements as k_measure_small (the same partial sums in
+// the same order: thread t < kBlock sums amplitudes t, t + kBlock, ...; the
+// wave butterfly; wave 0..3 in order), apply_W as k_qms_W.
+__global__ __launch_bounds__(1024) void k_tiny_prog(const ProgArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ __align__(16) ProgOp sop[kProgMax];
+    __shared__ uint32_t lout[kProgMax];          // outcome of each measurement op of this program
+    __shared__ double red[2][2][kBlock / 64];    // alternating wave partials: one barrier per measurement
+    const uint32_t N = 1u << a.nq;
+    const uint32_t y = threadIdx.x;
+    const bool act = y < N;
+    double* xbuf = reinterpret_cast<double*>(lds_raw);   // 2 exchange buffers of (re[N], im[N])
+    double* pbuf = xbuf + 4 * (size_t)N;                 // |psi|^2 for the measurements (N > kBlock)
+    double vr = act ? a.re[y] : 0.0, vi = act ? a.im[y] : 0.0;
+    {   // the ops into LDS, 16 bytes per thread (kProgMax * 40 / 16 <= kBlock <= blockDim.x)
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        const uint32_t n16 = (a.nops * (uint32_t)sizeof(ProgOp) + 15u) / 16u;
+        if (y < n16) reinterpret_cast<u4*>(sop)[y] = reinterpret_cast<const u4*>(&a.op[0])[y];
+    }
+    __syncthreads();
+    uint32_t xb = 0;   // alternating exchange buffers: one barrier per cross-wave exchange
+    uint32_t rb = 0;   // alternating reduction partials
+    // partner amplitude y ^ x (every thread calls it: barriers inside)
+    auto exchange = [&](uint32_t x, double& pr, double& pi) {
+        if (x < 64u) {
+            pr = __shfl_xor(vr, (int)x, 64);
+            pi = __shfl_xor(vi, (int)x, 64);
+        } else {   // the previous use of this buffer ended before the last exchange's barrier
+            double* bx = xbuf + 2 * (size_t)N * xb;
+            if (act) {
+                bx[y] = vr;
+                bx[N + y] = vi;
+            }
+            __syncthreads();
+            if (act) {
+                pr = bx[y ^ x];
+                pi = bx[N + (y ^ x)];
+            } else {
+                pr = pi = 0.0;
+            }
+            xb ^= 1u;
+        }
+    };
+    // outcome k of an op's references (uniform)
+    auto outcome = [&](uint32_t kf, uint32_t ref, uint32_t k) -> uint32_t {
+        if (kf & kPfLocal) return uni(lout[ref + k]);
+        return uni(__hip_atomic_load(a.slots + ((ref + k) % kSlots), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    };
+    // the next op's words are read from LDS while the current one runs
+    ProgOp nx = sop[0];
+    for (uint32_t k = 0; k < a.nops; ++k) {
+        const uint32_t kf = uni(nx.kf);
+        const uint32_t kind = kf & 15u;
+        const uint32_t x = uni(nx.x), m = uni(nx.m), v = uni(nx.v), yz = uni(nx.yz);
+        const uint32_t ref = uni(nx.ref);
+        const double c = unid(nx.c), s = unid(nx.s);
+        if (k + 1 < a.nops) nx = sop[k + 1];
+        if (kind == (uint32_t)kMeasOp) {   // k_measure_small's sums for one qubit
+            double s0 = 0.0, s1 = 0.0;
+            if (N <= (uint32_t)kBlock) {   // one term per thread
+                const double p = vr * vr + vi * vi;
+                if (act) {
+                    if (y & x) s1 += p;
+                    else s0 += p;
+                }
+            } else {   // thread t < kBlock sums amplitudes t, t + kBlock, ...
+                if (act) pbuf[y] = vr * vr + vi * vi;
+                __syncthreads();
+                if (y < (uint32_t)kBlock)
+                    for (uint32_t i = y; i < N; i += kBlock) {
+                        const double p = pbuf[i];
+                        if (i & x) s1 += p;
+                        else s0 += p;
+                    }
+            }
+            if (y < (uint32_t)kBlock) {
+                s0 = wave_sum64(s0);
+                s1 = wave_sum64(s1);
+                if ((y & 63u) == 0u) {
+                    red[rb][0][y >> 6] = s0;
+                    red[rb][1][y >> 6] = s1;
+                }
+            }
+            __syncthreads();
+            double p0 = 0.0, p1 = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) {
+                p0 += red[rb][0][w];
+                p1 += red[rb][1][w];
+            }
+            rb ^= 1u;
+            const uint32_t cbit = uni(c < p1 ? 1u : 0u);        // suqa.cu:690
+            const double scale = 1. / sqrt(cbit ? p1 : p0);    // set_ampl_to_zero + vnormalize (suqa.cu:694-695)
+            const uint32_t keep = cbit ? x : 0u;
+            if ((y & 63u) == 0u) lout[k] = cbit;   // each wave's own write precedes its reads
+            if (y == 0) a.slots[yz] = cbit;
+            if (act) {
+                const bool kp = (y & x) == keep;
+                vr = kp ? vr * scale : 0.0;
+                vi = kp ? vi * scale : 0.0;
+            }
+            if ((kf & kPfReset) && cbit) {   // apply_reset: X on q over the gc bits (suqa.cu:735-737)
+                double pr, pi;
+                exchange(x, pr, pi);
+                const uint32_t g = m & ~x;
+                if (act && (y & g) == g) {
+                    vr = pr;
+                    vi = pi;
+                }
+            }
+            continue;
+        }
+        if (kind == (uint32_t)kWOp) {   // k_qms_W: pairs (j, i = j + N/2)
+            const uint32_t half = N >> 1;
+            double pr, pi;
+            exchange(half, pr, pi);
+            if (act) {
+                const uint32_t i = y | half;
+                const uint32_t En = (i & x) >> (v & 255u);
+                uint32_t Eo;
+                const uint32_t eb = (kf >> 16) & 255u;
+                if (m) {
+                    Eo = (i & m) >> ((v >> 8) & 255u);
+                } else if (eb) {
+                    Eo = 0;
+                    for (uint32_t b = 0; b < eb; ++b) Eo |= outcome(kf, ref, b) << b;
+                } else {
+                    Eo = ref;
+                }
+                const double* row = reinterpret_cast<const double*>(__double_as_longlong(c)) + 2 * (uint64_t)Eo * yz;
+                const double fs1 = row[2 * En], fs2 = row[2 * En + 1];
+                if (y < half) {   // j: t = own, ri = the partner
+                    vr = fs2 * vr + fs1 * pr;
+                    vi = fs2 * vi + fs1 * pi;
+                } else {          // i: t = the partner, ri = own
+                    vr = fs1 * pr - fs2 * vr;
+                    vi = fs1 * pi - fs2 * vi;
+                }
+            }
+            continue;
+        }
+        if ((kf & kPfCond) && !outcome(kf, ref, 0)) continue;   // uniform: a conditional gate not taken
+        if (kind == (uint32_t)kPhaseOp) {
+            if (act && (y & m) == v) {
+                const uint32_t par = (uint32_t)__popc(y & yz) & 1u;
+                const double sn = par ? -s : s;
+                const double rr = vr, mm = vi;
+                vr = phase_out(false, rr, mm, c, sn);
+                vi = phase_out(true, mm, rr, c, sn);
+            }
+            continue;
+        }
+        double pr, pi;
+        exchange(x, pr, pi);
+        if (!act) continue;
+        // roles of tile_pairs: u has the pivot bit 0; the selected one of (u, u ^ X) is the a side
+        const uint32_t w = y ^ x;
+        const uint32_t pivot = (kf >> 9) & 31u;
+        const uint32_t u = ((y >> pivot) & 1u) ? w : y;
+        const uint32_t uw = u ^ x;
+        const bool su = (u & m) == v;
+        if (!su && (uw & m) != v) continue;
+        const bool is_a = y == (su ? u : uw);
+        const uint32_t par = (uint32_t)__popc(w & yz) & 1u;   // parity of the partner
+        const Coef<double> pc{c, s, (kf & kPfImag) ? 1u : 0u, (kf & kPfNeg) ? 1u : 0u};
+        double nr, ni;
+        switch (kind) {
+        case OP_SWAP:
+            nr = pair_out_par<OP_SWAP>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_SWAP>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        case OP_Y:
+            nr = pair_out_par<OP_Y>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_Y>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        case OP_H:
+            nr = pair_out_par<OP_H>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_H>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        default:
+            nr = pair_out_par<OP_PAULI>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_PAULI>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        }
+        vr = nr;
+        vi = ni;
+    }
+    if (act) {
+        a.re[y] = vr;
+        a.im[y] = vi;
+    }
+}
+
 // Runs of consecutive set bits of m, lowest first, as lo | len << 8 | src << 16
 // (src: the index of the run's first bit among m's set bits); only the runs
 // whose src is below src_limit.  At most kMaxRuns (a 64-bit mask of t tile
@@ -1364,10 +1588,137 @@ void pack_gates(std::vector<GateDesc>& q, uint32_t t, uint32_t f, bool span, siz
 
 }  // namespace
 
+namespace {
+
+// The queue of a tiny_prog state as programs of up to kProgMax operations, in
+// queue order.  A reference to an outcome that a measurement of the same
+// program writes becomes that op's index (read from LDS); other references
+// read the outcome slot (written by an earlier launch).
+hipError_t run_tiny_programs(suqa_hip_ctx* c, const std::vector<GateDesc>& q) {
+    const uint32_t N = 1u << c->nq;
+    const uint32_t nthr = std::max<uint32_t>(N, (uint32_t)kBlock);
+    const size_t lds = sizeof(double) * 5 * (size_t)N;
+    size_t i = 0;
+    while (i < q.size()) {
+        ProgArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.re = c->d_re;
+        a.im = c->d_im;
+        a.slots = c->d_slots;
+        a.nq = c->nq;
+        uint32_t meas_op[kSlots];   // slot -> op index of this program's measurement writing it
+        uint32_t n = 0;
+        // the op index of this program's measurement writing slots ref .. ref + nb - 1, kNoRef when
+        // none of them is written here; -2 (error) when only some are, or not by consecutive ops
+        auto localize = [&](uint32_t ref, uint32_t nb, uint32_t* op0) -> int {
+            uint32_t hits = 0;
+            for (uint32_t b = 0; b < nb; ++b) {
+                const uint32_t s = (ref + b) % kSlots;
+                if (meas_op[s] != kNoRef) ++hits;
+            }
+            if (hits == 0) return 0;
+            const uint32_t o = meas_op[ref % kSlots];
+            if (hits != nb || o == kNoRef) return -2;
+            for (uint32_t b = 0; b < nb; ++b)
+                if (meas_op[(ref + b) % kSlots] != o + b) return -2;
+            *op0 = o;
+            return 1;
+        };
+        for (uint32_t s = 0; s < kSlots; ++s) meas_op[s] = kNoRef;
+        uint64_t alg = 0;
+        for (; i < q.size() && n < kProgMax; ++i) {
+            const GateDesc& d = q[i];
+            ProgOp& o = a.op[n];
+            o.kf = (uint32_t)d.kind;
+            if (d.kind == kMeasOp) {
+                o.x = 1u << d.q;
+                o.m = (uint32_t)d.M;
+                o.yz = d.slot;
+                o.c = d.c;
+                if (d.reset) o.kf |= kPfReset;
+                meas_op[d.slot % kSlots] = n;
+                alg += 40ull * N;
+            } else if (d.kind == kWOp) {
+                o.x = d.wmask;
+                o.m = d.omask;
+                o.v = d.b0 | (d.ob0 << 8);
+                o.yz = d.L;
+                uint64_t tb = 0;
+                std::memcpy(&tb, &d.tab, sizeof(d.tab));
+                std::memcpy(&o.c, &tb, sizeof(o.c));
+                if (d.eold_bits) {
+                    uint32_t op0 = 0;
+                    const int lz = localize(d.eold_ref, d.eold_bits, &op0);
+                    if (lz < 0) {
+                        set_last_error("apply_W reads outcomes that this program writes out of order");
+                        return hipErrorInvalidValue;
+                    }
+                    o.ref = lz ? op0 : d.eold_ref;
+                    if (lz) o.kf |= kPfLocal;
+                    o.kf |= d.eold_bits << 16;
+                } else {
+                    o.ref = d.eold;
+                }
+                alg += 32ull * N;
+            } else {
+                o.m = (uint32_t)d.M;
+                o.v = (uint32_t)d.V;
+                if (d.kind == kPhaseOp) {
+                    o.yz = (uint32_t)d.zmask;
+                    o.c = d.c;
+                    o.s = d.s;
+                } else {
+                    o.x = (uint32_t)d.X;
+                    o.yz = (uint32_t)d.pc.yz;
+                    o.c = d.pc.c;
+                    o.s = d.pc.s;
+                    o.kf |= (uint32_t)__builtin_ctz(o.x) << 9;
+                    if (d.pc.base_imag) o.kf |= kPfImag;
+                    if (d.pc.base_neg) o.kf |= kPfNeg;
+                }
+                if (d.cond_ref != kNoRef) {
+                    uint32_t op0 = 0;
+                    const int lz = localize(d.cond_ref, 1, &op0);
+                    o.ref = lz ? op0 : d.cond_ref;
+                    o.kf |= kPfCond | (lz ? kPfLocal : 0u);
+                }
+                alg += (uint64_t)d.alg_bytes;
+            }
+            ++n;
+        }
+        a.nops = n;
+        ++c->passes;
+        c->pass_gates += n;
+        timing_begin(c, "tiny_prog", (double)alg);
+        hipLaunchKernelGGL(k_tiny_prog, dim3(1), dim3(nthr), lds, c->stream, a);
+        timing_end(c);
+        if (hipError_t e = hipGetLastError()) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+void queue_measure(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, bool reset, uint32_t gc,
+                   uint32_t slot) {
+    for (uint32_t k = 0; k < n; ++k) {
+        GateDesc d{};
+        d.kind = kMeasOp;
+        d.q = qs[k];
+        d.c = rds[k];
+        d.M = gc;
+        d.reset = reset ? 1u : 0u;
+        d.slot = (slot + k) % kSlots;
+        d.tag = "measure";
+        c->queue.push_back(d);
+    }
+}
+
 hipError_t flush(suqa_hip_ctx* c) {
     if (c->queue.empty()) return hipSuccess;
     std::vector<GateDesc> q;
     q.swap(c->queue);
+    if (tiny_prog(c)) return run_tiny_programs(c, q);
     const uint32_t t = std::min(c->tile_bits, c->nq);
     const uint32_t f = std::min(c->f32 ? kPassLowFixed32 : kLowFixed, t);   // whole 128-byte lines
     const bool span = c->span_tiles && c->nq > t;   // a state of one tile needs no span
