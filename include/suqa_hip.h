@@ -116,12 +116,31 @@ int suqa_hip_apply_reset(suqa_hip_ctx* ctx, uint32_t q, double rdoub, uint32_t g
 /* suqa::measure_qbits (suqa.cu:699-702): measure_qbit on qs[k] with rdoubs[k],
  * in order, outcomes into cs[0..n).  On a state of at most 2^14 amplitudes
  * the n measurements are one launch and one wait; with cs == NULL (outcomes
- * not wanted) they are one launch and nothing waits. */
+ * not wanted) they are one launch and nothing waits.  On a state of at most
+ * 2^10 amplitudes that is one pass tile they join the queued gates instead:
+ * the queue and the measurements are one launch. */
 int suqa_hip_measure_qbits(suqa_hip_ctx* ctx, const uint32_t* qs, const double* rdoubs, uint32_t n, uint32_t* cs);
 /* suqa::apply_reset(bmReg, vector<double>) (suqa.cu:752-757): apply_reset on
  * qs[k] with rdoubs[k], in order; one launch and no wait on a state of at
  * most 2^14 amplitudes. */
 int suqa_hip_apply_resets(suqa_hip_ctx* ctx, const uint32_t* qs, const double* rdoubs, uint32_t n, uint32_t gc_mask);
+/* Extension (no counterpart in the reference): deferred measurements, for
+ * callers that need an outcome on the device before they need it on the host
+ * (qms::'s energy readout, which the X flips and apply_W read).  On a state of
+ * at most 2^10 amplitudes that is one pass tile, measure_qbits' measurements
+ * (draws rdoubs[k]) are queued beside the gates and their outcomes land in
+ * outcome slots *slot .. *slot + n - 1 of a ring of SUQA_HIP_OUTCOME_SLOTS per
+ * context; a slot keeps its outcome until that many later slots have been
+ * handed out.  SUQA_ERR_UNSUPPORTED on any other state. */
+#define SUQA_HIP_OUTCOME_SLOTS 1024u
+int suqa_hip_measure_qbits_deferred(suqa_hip_ctx* ctx, const uint32_t* qs, const double* rdoubs, uint32_t n,
+                                    uint32_t* slot);
+/* Outcomes of slots slot, slot + 1, ... (n of them, ring order) into
+ * out[0..n): applies what is queued and waits for it. */
+int suqa_hip_read_outcomes(suqa_hip_ctx* ctx, uint32_t slot, uint32_t n, uint32_t* out);
+/* X on qs[k] over the indices with the gc bits set, where outcome slot + k
+ * reads 1 (queued; the flips of a measured register, qms.cuh:404-406). */
+int suqa_hip_apply_x_if(suqa_hip_ctx* ctx, const uint32_t* qs, uint32_t n, uint32_t slot, uint32_t gc_mask);
 /* suqa::prob_filter (suqa.cu:705-727): sum |psi_i|^2 over i whose bits qs[k]
  * read q_mask[k]. */
 int suqa_hip_prob_filter(suqa_hip_ctx* ctx, const uint32_t* qs, const uint32_t* q_mask,
@@ -148,6 +167,11 @@ int suqa_hip_prob_filter_noalloc(const double* d_re, const double* d_im, uint32_
  * evaluated on the host with libm (as the reference CPU path does). */
 int suqa_hip_qms_apply_W(suqa_hip_ctx* ctx, uint32_t q_acc, uint32_t W_mask_Enew,
                          uint32_t bm_enes0, double c_factor, uint32_t curr_E_old);
+/* Extension: the same with E_old read on the device from the outcome slots
+ * eold_slot .. eold_slot + r - 1 (LSB first; r = the register's width) of
+ * deferred measurements (suqa_hip_measure_qbits_deferred). */
+int suqa_hip_qms_apply_W_deferred(suqa_hip_ctx* ctx, uint32_t q_acc, uint32_t W_mask_Enew, uint32_t bm_enes0,
+                                  double c_factor, uint32_t eold_slot);
 /* The same rotation in the two-register layout (the commented bm_enes_old
  * code of qms.cuh:72-85, 134-136, 201-202; extension): E_old is read from
  * the register (W_mask_Eold, offset bm_enes_old0) of each basis state instead

@@ -65,6 +65,10 @@ SIGNATURES = {
     "suqa_hip_apply_reset": (C.c_int, [CTX, U32, C.c_double, U32]),
     "suqa_hip_measure_qbits": (C.c_int, [CTX, PU32, C.POINTER(C.c_double), U32, PU32]),
     "suqa_hip_apply_resets": (C.c_int, [CTX, PU32, C.POINTER(C.c_double), U32, U32]),
+    "suqa_hip_measure_qbits_deferred": (C.c_int, [CTX, PU32, C.POINTER(C.c_double), U32, PU32]),
+    "suqa_hip_read_outcomes": (C.c_int, [CTX, U32, U32, PU32]),
+    "suqa_hip_apply_x_if": (C.c_int, [CTX, PU32, U32, U32, U32]),
+    "suqa_hip_qms_apply_W_deferred": (C.c_int, [CTX, U32, U32, U32, C.c_double, U32]),
     "suqa_hip_prob_filter": (C.c_int, [CTX, PU32, PU32, U32, PD]),
     "suqa_hip_qms_apply_W": (C.c_int, [CTX, U32, U32, U32, C.c_double, U32]),
     "suqa_hip_qms_apply_W2": (C.c_int, [CTX, U32, U32, U32, U32, U32, C.c_double]),
@@ -285,6 +289,24 @@ class Engine:
         rd = (C.c_double * max(1, len(rdoubs)))(*rdoubs)
         check(lib().suqa_hip_apply_resets(self.ctx, _u32(qs), rd, len(qs), self.gc_mask))
 
+    def measure_qbits_deferred(self, qs, rdoubs):
+        """Queued measurements; returns the first outcome slot (outcomes: read_outcomes)."""
+        rd = (C.c_double * max(1, len(rdoubs)))(*rdoubs)
+        slot = U32()
+        check(lib().suqa_hip_measure_qbits_deferred(self.ctx, _u32(qs), rd, len(qs), C.byref(slot)))
+        return slot.value
+
+    def read_outcomes(self, slot, n):
+        out = (U32 * max(1, n))()
+        check(lib().suqa_hip_read_outcomes(self.ctx, slot, n, out))
+        return [out[k] for k in range(n)]
+
+    def x_if(self, qs, slot):
+        check(lib().suqa_hip_apply_x_if(self.ctx, _u32(qs), len(qs), slot, self.gc_mask))
+
+    def qms_apply_W_deferred(self, q_acc, w_mask_enew, bm_enes0, c_factor, eold_slot):
+        check(lib().suqa_hip_qms_apply_W_deferred(self.ctx, q_acc, w_mask_enew, bm_enes0, c_factor, eold_slot))
+
     def prob_filter(self, qs, q_mask):
         p = C.c_double()
         check(lib().suqa_hip_prob_filter(self.ctx, _u32(qs), _u32(q_mask), len(qs), C.byref(p)))
@@ -467,6 +489,24 @@ class F32Engine:
         v = C.c_double()
         check32(lib().suqa_hip_f32_vnorm(self.ctx, C.byref(v)))
         return v.value
+
+    def measure_qbits_deferred(self, qs, rdoubs):
+        """Queued measurements; returns the first outcome slot (outcomes: read_outcomes)."""
+        rd = (C.c_double * max(1, len(rdoubs)))(*rdoubs)
+        slot = U32()
+        check(lib().suqa_hip_measure_qbits_deferred(self.ctx, _u32(qs), rd, len(qs), C.byref(slot)))
+        return slot.value
+
+    def read_outcomes(self, slot, n):
+        out = (U32 * max(1, n))()
+        check(lib().suqa_hip_read_outcomes(self.ctx, slot, n, out))
+        return [out[k] for k in range(n)]
+
+    def x_if(self, qs, slot):
+        check(lib().suqa_hip_apply_x_if(self.ctx, _u32(qs), len(qs), slot, self.gc_mask))
+
+    def qms_apply_W_deferred(self, q_acc, w_mask_enew, bm_enes0, c_factor, eold_slot):
+        check(lib().suqa_hip_qms_apply_W_deferred(self.ctx, q_acc, w_mask_enew, bm_enes0, c_factor, eold_slot))
 
     def prob_filter(self, qs, q_mask):
         p = C.c_double()

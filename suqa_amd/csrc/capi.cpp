@@ -776,6 +776,63 @@ int suqa_hip_apply_resets(suqa_hip_ctx* c, const uint32_t* qs, const double* rdo
     return SUQA_OK;
 }
 
+static_assert(SUQA_HIP_OUTCOME_SLOTS == sqe::kSlots, "outcome ring size");
+
+int suqa_hip_measure_qbits_deferred(suqa_hip_ctx* c, const uint32_t* qs, const double* rdoubs, uint32_t n,
+                                    uint32_t* slot) {
+    SQE_CTX(c);
+    if (!slot || (n && (!qs || !rdoubs))) return fail(SUQA_ERR_ARG, "null argument");
+    if (!sqe::tiny_prog(c)) return fail(SUQA_ERR_UNSUPPORTED, "deferred measurements need a state of one small pass tile");
+    if (n > sqe::kSlots / 2) return fail(SUQA_ERR_ARG, "too many measurements in one call");
+    for (uint32_t k = 0; k < n; ++k)
+        if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+    *slot = sqe::alloc_slots(c, n);
+    sqe::queue_measure(c, qs, rdoubs, n, false, 0u, *slot);
+    if (c->queue.size() >= sqe::kAutoFlush) SQE_HIP(sqe::flush(c));
+    return SUQA_OK;
+}
+
+int suqa_hip_read_outcomes(suqa_hip_ctx* c, uint32_t slot, uint32_t n, uint32_t* out) {
+    SQE_CTX(c);
+    if (n && !out) return fail(SUQA_ERR_ARG, "null argument");
+    if (!c->d_slots) return fail(SUQA_ERR_UNSUPPORTED, "no outcome slots on this context");
+    if (slot >= sqe::kSlots || n > sqe::kSlots) return fail(SUQA_ERR_ARG, "outcome slot range out of the ring");
+    if (c->capturing) return fail(SUQA_ERR_ARG, "outcomes cannot be read back while a graph is being captured");
+    SQE_HIP(sqe::flush(c));
+    const uint32_t n1 = std::min(n, sqe::kSlots - slot);   // up to the ring's end, then from 0
+    if (n1) SQE_HIP(hipMemcpyAsync(c->h_slots, c->d_slots + slot, sizeof(uint32_t) * n1, hipMemcpyDeviceToHost, c->stream));
+    if (n > n1) SQE_HIP(hipMemcpyAsync(c->h_slots + n1, c->d_slots, sizeof(uint32_t) * (n - n1), hipMemcpyDeviceToHost, c->stream));
+    SQE_HIP(hipStreamSynchronize(c->stream));
+    for (uint32_t k = 0; k < n; ++k) out[k] = c->h_slots[k];
+    return SUQA_OK;
+}
+
+int suqa_hip_apply_x_if(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_t slot, uint32_t gc) {
+    SQE_CTX(c);
+    if (n && !qs) return fail(SUQA_ERR_ARG, "null argument");
+    if (!sqe::tiny_prog(c)) return fail(SUQA_ERR_UNSUPPORTED, "conditional gates need a state of one small pass tile");
+    if (slot >= sqe::kSlots) return fail(SUQA_ERR_ARG, "outcome slot out of the ring");
+    if ((uint64_t)gc >> c->nq) return fail(SUQA_ERR_ARG, "gc_mask has qubits beyond the state");
+    for (uint32_t k = 0; k < n; ++k) {
+        if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+        if ((gc >> qs[k]) & 1u) return fail(SUQA_ERR_ARG, "a conditional X on a qubit of the gc_mask");
+    }
+    for (uint32_t k = 0; k < n; ++k) {   // pair_1q's selection (i & (gc|T)) == gc|T on i = j|T
+        const uint64_t T = 1ull << qs[k];
+        sqe::GateDesc d{};
+        d.kind = sqe::OP_SWAP;
+        d.M = (uint64_t)gc | T;
+        d.V = (uint64_t)gc;
+        d.X = T;
+        d.tag = "swap_pair_if";
+        d.alg_bytes = 64.0 * (double)(c->N >> __builtin_popcount(gc | (uint32_t)T));
+        d.cond_ref = (slot + k) % sqe::kSlots;
+        c->queue.push_back(d);
+    }
+    if (c->queue.size() >= sqe::kAutoFlush) SQE_HIP(sqe::flush(c));
+    return SUQA_OK;
+}
+
 int suqa_hip_prob_filter(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* q_mask, uint32_t n, double* prob) {
     SQE_CTX(c);
     if (n && (!qs || !q_mask)) return fail(SUQA_ERR_ARG, "null argument");
@@ -869,6 +926,29 @@ int suqa_hip_qms_apply_W(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmask, uint32
         row = c->d_wrow;
     }
     SQE_HIP(sqe::launch_qms_W(c, row, wmask, b0, 0u, 0u, (uint32_t)L));
+    return SUQA_OK;
+}
+
+int suqa_hip_qms_apply_W_deferred(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmask, uint32_t b0, double cf,
+                                  uint32_t eold_slot) {
+    SQE_CTX(c);
+    if (!sqe::tiny_prog(c)) return fail(SUQA_ERR_UNSUPPORTED, "deferred apply_W needs a state of one small pass tile");
+    if (q_acc != c->nq - 1) return fail(SUQA_ERR_ARG, "apply_W needs the acceptance qubit to be the most significant");
+    if (!reg_mask_ok(wmask, b0) || (wmask >> q_acc) != 0) return fail(SUQA_ERR_ARG, "bad energy register");
+    if (eold_slot >= sqe::kSlots) return fail(SUQA_ERR_ARG, "outcome slot out of the ring");
+    const uint64_t L = (uint64_t)(wmask >> b0) + 1ull;
+    if (int rc = w_table(c, wmask, b0, cf, L)) return rc;
+    sqe::GateDesc d{};
+    d.kind = sqe::kWOp;
+    d.tab = c->d_wtab;
+    d.wmask = wmask;
+    d.b0 = b0;
+    d.L = (uint32_t)L;
+    d.eold_ref = eold_slot;
+    d.eold_bits = (uint32_t)__builtin_popcount(wmask);
+    d.tag = "qms_W";
+    c->queue.push_back(d);
+    if (c->queue.size() >= sqe::kAutoFlush) SQE_HIP(sqe::flush(c));
     return SUQA_OK;
 }
 

@@ -33,6 +33,7 @@ bool two_registers = false;
 std::vector<uint> bm_enes_old;
 uint W_mask_Eold = 0U;
 bool use_graphs = false;
+bool deferred = false;
 
 namespace {
 
@@ -84,7 +85,80 @@ void replay(Graph& g, F apply) {
     chk_graph(suqa_hip_graph_launch(suqa::ctx, g.id));
 }
 
+// Deferred readouts: the outcome slots handed out since the last read (ring
+// order), the host copy of what was read, and the E / X values that wait for
+// their outcomes.
+struct Deferred {
+    bool pending = false;
+    uint32_t begin = 0, end = 0;
+    std::vector<uint32_t> ring = std::vector<uint32_t>(SUQA_HIP_OUTCOME_SLOTS, 0u);
+    struct Rec {
+        bool is_x;
+        uint32_t slot;
+        size_t idx;
+    };
+    std::vector<Rec> recs;
+} D;
+
+void chk(int rc, const char* where) {
+    if (rc != SUQA_OK) throw std::runtime_error(std::string("ERROR: in ") + where + ": " + suqa_hip_last_error());
+}
+
+void note_slots(uint32_t slot, uint32_t n) {
+    if (!D.pending) {
+        D.begin = slot;
+        D.pending = true;
+    }
+    D.end = slot + n;
+}
+
+uint32_t measure_deferred(const bmReg& qs, const std::vector<double>& rdoubs) {
+    uint32_t slot = 0;
+    chk(suqa_hip_measure_qbits_deferred(suqa::ctx, qs.data(), rdoubs.data(), (uint32_t)qs.size(), &slot),
+        "suqa::measure_qbits()");
+    note_slots(slot, (uint32_t)qs.size());
+    return slot;
+}
+
+// The register value of n outcomes from slot (LSB first, creg_to_uint).
+uint slot_value(uint32_t slot, uint32_t n) {
+    uint v = 0;
+    for (uint32_t k = 0; k < n; ++k) v += D.ring[(slot + k) % SUQA_HIP_OUTCOME_SLOTS] << k;
+    return v;
+}
+
+double ene_value(uint e) { return t_PE_shift + e / (double)(t_PE_factor * ene_levels); }
+
+// One wait: runs what is queued and reads every slot handed out since the
+// last read, then fills in the values that waited for them.
+void read_slots() {
+    if (!D.pending) return;
+    const uint32_t S = SUQA_HIP_OUTCOME_SLOTS;
+    const uint32_t n = (D.end + S - D.begin) % S;
+    if (n == 0 || n > S / 2) throw std::runtime_error("ERROR: deferred readouts: outcome ring overrun");
+    std::vector<uint32_t> out(n);
+    chk(suqa_hip_read_outcomes(suqa::ctx, D.begin, n, out.data()), "qms::metro_step()");
+    for (uint32_t k = 0; k < n; ++k) D.ring[(D.begin + k) % S] = out[k];
+    D.pending = false;
+    for (const auto& r : D.recs) {
+        if (r.is_x) X_measures[r.idx] = X_value(slot_value(r.slot, 3));
+        else E_measures[r.idx] = ene_value(slot_value(r.slot, ene_qbits));
+    }
+    D.recs.clear();
+}
+
+void push_x_deferred() {
+    const uint32_t sx = measure_X_deferred(rangen);
+    note_slots(sx, 3);
+    D.recs.push_back(Deferred::Rec{true, sx, X_measures.size()});
+    X_measures.push_back(0.0);
+}
+
 }  // namespace
+
+void sync_measures() {
+    if (deferred) read_slots();
+}
 
 void fill_bitmap() {
     bm_syst.resize(syst_qbits);
@@ -228,9 +302,87 @@ std::vector<double> extract_rands(uint n) {
     return ret;
 }
 
+namespace {
+
+// metro_step with deferred readouts (one register, no counters, no graphs):
+// the draws, gates and measurements of metro_step below in the same order.
+int metro_step_deferred(bool take_measure) {
+    int ret = 0;
+    reset_non_syst_qbits();
+    apply_Phi();
+    const uint32_t s_eold = measure_deferred(bm_enes_new, extract_rands(ene_qbits));
+    chk(suqa_hip_apply_x_if(suqa::ctx, bm_enes_new.data(), ene_qbits, s_eold, 0u), "suqa::apply_x()");
+    gCi = draw_C();
+    apply_C(gCi);   // apply_U: C_i, Phi, W with E_old read on the device
+    apply_Phi();
+    chk(suqa_hip_qms_apply_W_deferred(suqa::ctx, bm_acc, W_mask_Enew, bm_enes_new[0], c_factor, s_eold),
+        "qms::apply_W()");
+    const uint32_t s_acc = measure_deferred({bm_acc}, {rangen.doub()});
+    read_slots();   // the step's one wait
+    curr_E_old = slot_value(s_eold, ene_qbits);
+    curr_E_old_d = ene_value(curr_E_old);
+    c_acc = D.ring[s_acc];
+
+    if (c_acc == 1U) {
+        uint32_t s_enew = 0;
+        if (take_measure) s_enew = measure_deferred(bm_enes_new, extract_rands(ene_qbits));
+        else measure_discarded(bm_enes_new, extract_rands(ene_qbits));
+        apply_Phi_inverse();
+        if (take_measure) {
+            D.recs.push_back(Deferred::Rec{false, s_enew, E_measures.size()});
+            E_measures.push_back(0.0);
+            chk(suqa_hip_apply_x_if(suqa::ctx, bm_enes_new.data(), ene_qbits, s_enew, 0u), "suqa::apply_x()");
+            push_x_deferred();
+            suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));
+            apply_Phi();
+            measure_discarded(bm_enes_new, extract_rands(ene_qbits));   // E re-collapse
+            apply_Phi_inverse();
+            ret = 2;
+        } else {
+            ret = 1;
+        }
+        return ret;
+    }
+
+    apply_U_inverse();   // E_old is known on the host now
+    uint iters = 0;
+    while (iters < max_reverse_attempts) {
+        apply_Phi();
+        const uint32_t s_en = measure_deferred(bm_enes_new, extract_rands(ene_qbits));
+        read_slots();
+        const uint Enew_meas = slot_value(s_en, ene_qbits);
+        apply_Phi_inverse();
+        if (curr_E_old == Enew_meas) {
+            if (take_measure) {
+                E_measures.push_back(curr_E_old_d);
+                suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));
+                push_x_deferred();
+                suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));
+                apply_Phi();
+                measure_discarded(bm_enes_new, extract_rands(ene_qbits));   // E re-collapse
+                apply_Phi_inverse();
+                ret = 4;
+            } else {
+                ret = 3;
+            }
+            break;
+        }
+        apply_U();
+        measure_discarded({bm_acc}, {rangen.doub()});   // the outcome is not used
+        apply_U_inverse();
+        iters++;
+    }
+    if (record_reverse) reverse_counters.push_back(iters);
+    if (iters == max_reverse_attempts) return -1;
+    return ret;
+}
+
+}  // namespace
+
 int metro_step(bool take_measure) {
     // 1/2: accepted (not measured/measured), 3/4: rejected and restored,
     // -1: rejected and not restored within max_reverse_attempts.
+    if (deferred && suqa::gatecounters.counters.empty()) return metro_step_deferred(take_measure);
     const bool counting = !suqa::gatecounters.counters.empty();
     if (counting) gctr_metrostep.new_record();
     int ret = 0;
@@ -331,8 +483,17 @@ void setup(double beta) {
     drop_graphs();   // a new engine context holds none of them
     const char* e = std::getenv("SUQA_QMS_GRAPHS");
     use_graphs = e && std::strcmp(e, "1") == 0;
+    const char* d = std::getenv("SUQA_QMS_DEFERRED");
+    uint32_t probe = 0;
+    deferred = !(d && std::strcmp(d, "0") == 0) && !use_graphs && !two_registers &&
+               suqa::gatecounters.counters.empty() &&
+               suqa_hip_measure_qbits_deferred(suqa::ctx, nullptr, nullptr, 0, &probe) == SUQA_OK;
+    D = Deferred{};
 }
 
-void clear() { drop_graphs(); }
+void clear() {
+    sync_measures();
+    drop_graphs();
+}
 
 }  // namespace qms

@@ -28,6 +28,7 @@
 #include "system.hpp"
 
 static void save_measures(const std::string& outfilename) {   // qms.cu:41-49
+    qms::sync_measures();   // values of deferred readouts
     FILE* fil = fopen(outfilename.c_str(), "a");
     if (!fil) throw std::runtime_error("ERROR: cannot open " + outfilename);
     for (uint ei = 0; ei < qms::E_measures.size(); ++ei)

@@ -77,6 +77,7 @@ extern "C" int suqa_qms_run(const suqa_qms_params* p, suqa_qms_result* out) {
             if (out->rets) out->rets[s] = ret;
             if (out->rng_after) out->rng_after[s] = qms::rangen.get_state().state;
         }
+        qms::sync_measures();   // the E and X values of the last steps
         suqa_hip_synchronize(suqa::ctx);
         auto t1 = std::chrono::steady_clock::now();
         out->seconds = std::chrono::duration<double>(t1 - t0).count();

@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <stdexcept>
+#include <string>
 
 void init_state() {   // an E = -1 eigenstate: CX(1->0) H(1) X(1) |000>
     suqa::init_state();
@@ -45,6 +46,19 @@ double measure_X(pcg& rgen) {
     for (uint i = 0; i < op_bits; ++i) meas |= (classics[i] << i);
     return op_vals[meas];
 }
+
+uint32_t measure_X_deferred(pcg& rgen) {
+    apply_measure_rotation();
+    std::vector<double> rdoubs(op_bits);
+    for (auto& el : rdoubs) el = rgen.doub();
+    uint32_t slot = 0;
+    if (suqa_hip_measure_qbits_deferred(suqa::ctx, bm_spin.data(), rdoubs.data(), op_bits, &slot) != SUQA_OK)
+        throw std::runtime_error(std::string("ERROR: in measure_X(): ") + suqa_hip_last_error());
+    apply_measure_rotation();
+    return slot;
+}
+
+double X_value(uint meas) { return op_vals[meas]; }
 
 static const std::vector<double> C_weigthsums = {1. / 3, 2. / 3, 1.0};
 

@@ -13,6 +13,12 @@ const bmReg bm_spin = {0, 1, 2};
 void init_state();                               // system.cu:6-11
 void evolution(const double& t, const int& n);   // system.cu:21-26, exp(-iHt)
 double measure_X(pcg& rgen);                     // system.cu:99-119
+// measure_X with the spins' outcomes left on the device (extension): the same
+// gates and draws; returns the outcome slot of spin 0 (spins 1, 2 follow,
+// suqa_hip_measure_qbits_deferred); X_value maps the outcomes (bit i = spin i)
+// to the observable as measure_X does.
+uint32_t measure_X_deferred(pcg& rgen);
+double X_value(uint meas);
 void apply_C(const uint& Ci);                    // system.cu:125-129
 void apply_C_inverse(const uint& Ci);            // system.cu:131-133
 std::vector<double> get_C_weigthsums();          // system.cu:151

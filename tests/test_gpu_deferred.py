@@ -1,0 +1,144 @@
+"""Small states run their queued gates, measurements, resets and apply_W as
+one-workgroup programs.  Checked through the C-ABI: measurements in a program
+give the outcomes and the state, bit for bit, of the one-measurement kernel
+(tile passes off); deferred measurements (outcomes left on the device) match
+the ones read back; the conditional X and the apply_W that read outcome slots
+on the device match their host-driven forms, also when the measurement ran in
+an earlier program of the same queue; larger states refuse them."""
+import numpy as np
+import pytest
+
+from tests.common import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from suqa_amd import capi
+    return capi
+
+
+def log(m):
+    print(m, flush=True)
+
+
+def prepared(capi, n, seed, tile_bits=None):
+    """Every amplitude nonzero: an H layer, then X rotations, phases and CX."""
+    eng = capi.Engine(n, tile_bits=tile_bits)
+    rng = np.random.default_rng(seed)
+    for q in range(n):
+        eng.h(q)
+    for _ in range(3 * n):
+        eng.pauli([int(rng.integers(n))], [1], float(rng.uniform(-3, 3)))
+        eng.u1(int(rng.integers(n)), float(rng.uniform(-3, 3)))
+        a, b = rng.choice(n, 2, replace=False)
+        eng.cx(int(a), int(b))
+    return eng
+
+
+def same_state(a, b, tag):
+    ra, ia = a.get()
+    rb, ib = b.get()
+    assert_bitwise(ra, rb, tag + " re")
+    assert_bitwise(ia, ib, tag + " im")
+
+
+@pytest.mark.parametrize("n", [3, 5, 8, 9, 10])
+def test_program_measurements_match_the_measurement_kernel(capi, n):
+    log(f"program measurements n={n}")
+    rng = np.random.default_rng(100 + n)
+    qs = [int(q) for q in rng.permutation(n)[:min(n, 4)]]
+    for rep in range(3):
+        rd = [float(x) for x in rng.uniform(0, 1, len(qs))]
+        prog = prepared(capi, n, rep)          # one pass tile: measurements join the queue
+        plain = prepared(capi, n, rep, 0)      # tile passes off: k_measure_small
+        defer = prepared(capi, n, rep)
+        c_prog = prog.measure_qbits(qs, rd)
+        c_plain = plain.measure_qbits(qs, rd)
+        slot = defer.measure_qbits_deferred(qs, rd)
+        c_defer = defer.read_outcomes(slot, len(qs))
+        assert c_prog == c_plain == c_defer
+        same_state(prog, plain, "program vs kernel")
+        same_state(prog, defer, "program vs deferred")
+        # resets (X over the gc bits when the outcome is 1) likewise
+        prog.resets(qs, rd[::-1])
+        plain.resets(qs, rd[::-1])
+        same_state(prog, plain, "resets")
+        for e in (prog, plain, defer):
+            e.close()
+
+
+@pytest.mark.parametrize("n", [6, 8])
+def test_conditional_x_matches_host_flips(capi, n):
+    log(f"x_if n={n}")
+    rng = np.random.default_rng(7 + n)
+    qs = [1, 3, n - 1]
+    rd = [float(x) for x in rng.uniform(0, 1, len(qs))]
+    host = prepared(capi, n, 3)
+    dev = prepared(capi, n, 3)
+    cs = host.measure_qbits(qs, rd)
+    for q, c in zip(qs, cs):
+        if c:
+            host.x(q)
+    host.h(0)
+    slot = dev.measure_qbits_deferred(qs, rd)
+    dev.x_if(qs, slot)
+    dev.h(0)
+    same_state(host, dev, "x_if")
+    # the measurement in one program, the flips in a later one (more queued
+    # operations than a program holds between them: the flips read the slots)
+    rd2 = [float(x) for x in rng.uniform(0, 1, len(qs))]
+    cs = host.measure_qbits(qs, rd2)
+    slot = dev.measure_qbits_deferred(qs, rd2)
+    for k in range(150):
+        host.u1(k % n, 0.01 * k)
+        dev.u1(k % n, 0.01 * k)
+    for q, c in zip(qs, cs):
+        if c:
+            host.x(q)
+    dev.x_if(qs, slot)
+    same_state(host, dev, "x_if across programs")
+    assert dev.read_outcomes(slot, len(qs)) == cs
+    host.close()
+    dev.close()
+
+
+def test_deferred_apply_W_matches_host_E_old(capi):
+    r = 4
+    n = 3 + r + 1
+    wmask = ((1 << r) - 1) << 3
+    log("deferred apply_W, r=4")
+    for rep, beta in enumerate((0.3, 1.0, 4.0)):
+        cf = 0.25 * beta
+        rng = np.random.default_rng(rep)
+        rd = [float(x) for x in rng.uniform(0, 1, r)]
+        host = prepared(capi, n, 10 + rep)
+        plain = prepared(capi, n, 10 + rep, 0)
+        dev = prepared(capi, n, 10 + rep)
+        ene = list(range(3, 3 + r))
+        cs = host.measure_qbits(ene, rd)
+        assert plain.measure_qbits(ene, rd) == cs
+        e_old = sum(c << k for k, c in enumerate(cs))
+        slot = dev.measure_qbits_deferred(ene, rd)
+        for e in (host, plain, dev):
+            e.h(0)
+            e.h(5)
+        host.qms_apply_W(n - 1, wmask, 3, cf, e_old)
+        plain.qms_apply_W(n - 1, wmask, 3, cf, e_old)
+        dev.qms_apply_W_deferred(n - 1, wmask, 3, cf, slot)
+        same_state(host, plain, "W program vs kernel")
+        same_state(host, dev, "W deferred")
+        for e in (host, plain, dev):
+            e.close()
+
+
+def test_larger_states_refuse_deferred_operations(capi):
+    log("deferred on 12 qubits")
+    eng = capi.Engine(12)
+    with pytest.raises(capi.SuqaError) as ei:
+        eng.measure_qbits_deferred([0, 1], [0.5, 0.5])
+    assert ei.value.code == -2
+    with pytest.raises(capi.SuqaError):
+        eng.x_if([0], 0)
+    eng.close()
