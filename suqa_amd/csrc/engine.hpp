@@ -225,18 +225,15 @@ hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double s
                             const char* tag, double alg_bytes);
 // Applies the queued gates in tile passes (tiled.hip); a no-op when empty.
 hipError_t flush(suqa_hip_ctx* c);
-// States of at most 2^kTinyProgBits amplitudes with gate passes on (tile bits
-// at least min(nq, kTinyOneTileBits): for the smaller ones, one pass tile) are
-// latency-bound: measurements, resets and apply_W queue
+// States of at most 2^kTinyProgBits amplitudes that are one pass tile
+// (tile_bits >= nq) are latency-bound: measurements, resets and apply_W queue
 // beside the gates, and a flush runs the queue as programs of up to kProgMax
 // operations, one single-workgroup launch each (k_tiny_prog, tiled.hip), with
 // the one-gate sweeps' arithmetic and launch_measure_small's reduction order
 // (bit-identical results).  Outcomes go to a ring of kSlots device slots.
-constexpr uint32_t kTinyProgBits = 12, kTinyOneTileBits = 10, kProgMax = 96, kSlots = 1024;
+constexpr uint32_t kTinyProgBits = 10, kProgMax = 96, kSlots = 1024;
 inline bool tiny_prog(const suqa_hip_ctx* c) {
-    const uint32_t need = c->nq < kTinyOneTileBits ? c->nq : kTinyOneTileBits;
-    return !c->f32 && c->tile_bits > 0 && c->tile_bits >= need && c->nq >= 1 && c->nq <= kTinyProgBits &&
-           c->d_slots;
+    return !c->f32 && c->tile_bits >= c->nq && c->nq >= 1 && c->nq <= kTinyProgBits && c->d_slots;
 }
 // n consecutive outcome slots (n <= kSlots); the ring wraps to 0 when fewer
 // are left before its end.  A slot keeps its outcome until kSlots later

@@ -962,98 +962,49 @@ __device__ __forceinline__ double wave_sum64(double v) {   // the butterfly of k
     return v;
 }
 
-// v[idx] for a uniform idx < AMP (selects, no indexed register access)
-template <int AMP>
-__device__ __forceinline__ double pick(const double (&v)[AMP], uint32_t idx) {
-    double r = v[0];
-#pragma unroll
-    for (int j = 1; j < AMP; ++j) r = idx == (uint32_t)j ? v[j] : r;
-    return r;
-}
-
-// One workgroup, the program's operations in order: gates as k_tile_tiny
-// (bit-identical to one sweep per gate), measurements as k_measure_small (the
-// same partial sums in the same order: thread t < kBlock sums amplitudes t,
-// t + kBlock, ...; the wave butterfly; waves 0..3 in order), apply_W as
-// k_qms_W.  AMP = 1: max(N, kBlock) threads, amplitude y in thread y < N;
-// AMP = 2, 4: N / AMP = 1024 threads, amplitude y + 1024 j in register j of
-// thread y (the top log2 AMP qubits select the register: their gates need no
-// exchange).
-template <int AMP>
+// One workgroup of max(N, kBlock) threads, amplitude y in thread y < N, the
+// program's operations in order: gates as k_tile_tiny (bit-identical to one
+// sweep per gate), measurements as k_measure_small (the same partial sums in
+// the same order: thread t < kBlock sums amplitudes t, t + kBlock, ...; the
+// wave butterfly; wave 0..3 in order), apply_W as k_qms_W.
 __global__ __launch_bounds__(1024) void k_tiny_prog(const ProgArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ __align__(16) ProgOp sop[kProgMax];
     __shared__ uint32_t lout[kProgMax];          // outcome of each measurement op of this program
     __shared__ double red[2][2][kBlock / 64];    // alternating wave partials: one barrier per measurement
     const uint32_t N = 1u << a.nq;
-    const uint32_t NT = blockDim.x;
     const uint32_t y = threadIdx.x;
-    const uint32_t tbits = AMP == 1 ? 31u : (uint32_t)__builtin_ctz(NT);   // register-select qubits from here
-    // LDS: AMP = 1: two alternating exchange buffers of (re[N], im[N]), then |psi|^2 of the
-    // measurements (N > kBlock); AMP > 1: one buffer of N doubles, one component per exchange
-    // round, also holding |psi|^2 of the measurements
-    double* xbuf = reinterpret_cast<double*>(lds_raw);
-    double* pbuf = AMP == 1 ? xbuf + 4 * (size_t)N : xbuf;
-    double vr[AMP], vi[AMP];
-    uint32_t ix[AMP];   // the amplitudes' indices
-    bool act[AMP];
-#pragma unroll
-    for (int j = 0; j < AMP; ++j) {
-        ix[j] = y + NT * (uint32_t)j;
-        act[j] = AMP > 1 || y < N;
-        vr[j] = act[j] ? a.re[ix[j]] : 0.0;
-        vi[j] = act[j] ? a.im[ix[j]] : 0.0;
-    }
+    const bool act = y < N;
+    double* xbuf = reinterpret_cast<double*>(lds_raw);   // 2 exchange buffers of (re[N], im[N])
+    double* pbuf = xbuf + 4 * (size_t)N;                 // |psi|^2 for the measurements (N > kBlock)
+    double vr = act ? a.re[y] : 0.0, vi = act ? a.im[y] : 0.0;
     {   // the ops into LDS, 16 bytes per thread (kProgMax * 40 / 16 <= kBlock <= blockDim.x)
         typedef uint32_t u4 __attribute__((ext_vector_type(4)));
         const uint32_t n16 = (a.nops * (uint32_t)sizeof(ProgOp) + 15u) / 16u;
         if (y < n16) reinterpret_cast<u4*>(sop)[y] = reinterpret_cast<const u4*>(&a.op[0])[y];
     }
     __syncthreads();
-    uint32_t xb = 0;   // AMP = 1: alternating exchange buffers, one barrier per cross-wave exchange
+    uint32_t xb = 0;   // alternating exchange buffers: one barrier per cross-wave exchange
     uint32_t rb = 0;   // alternating reduction partials
-    // partner amplitudes index ^ x (every thread calls it: barriers inside)
-    auto exchange = [&](uint32_t x, double (&pr)[AMP], double (&pi)[AMP]) {
-        const uint32_t xl = AMP == 1 ? x : (x & (NT - 1u)), xh = AMP == 1 ? 0u : (x >> tbits);
-        if (AMP > 1 && xl == 0u) {   // inside the thread
-#pragma unroll
-            for (int j = 0; j < AMP; ++j) {
-                pr[j] = pick<AMP>(vr, (uint32_t)j ^ xh);
-                pi[j] = pick<AMP>(vi, (uint32_t)j ^ xh);
-            }
-        } else if (xl < 64u) {
-#pragma unroll
-            for (int j = 0; j < AMP; ++j) {
-                pr[j] = __shfl_xor(pick<AMP>(vr, (uint32_t)j ^ xh), (int)xl, 64);
-                pi[j] = __shfl_xor(pick<AMP>(vi, (uint32_t)j ^ xh), (int)xl, 64);
-            }
-        } else if (AMP == 1) {   // the previous use of this buffer ended before the last exchange's barrier
+    // partner amplitude y ^ x (every thread calls it: barriers inside)
+    auto exchange = [&](uint32_t x, double& pr, double& pi) {
+        if (x < 64u) {
+            pr = __shfl_xor(vr, (int)x, 64);
+            pi = __shfl_xor(vi, (int)x, 64);
+        } else {   // the previous use of this buffer ended before the last exchange's barrier
             double* bx = xbuf + 2 * (size_t)N * xb;
-            if (act[0]) {
-                bx[y] = vr[0];
-                bx[N + y] = vi[0];
+            if (act) {
+                bx[y] = vr;
+                bx[N + y] = vi;
             }
             __syncthreads();
-            if (act[0]) {
-                pr[0] = bx[y ^ x];
-                pi[0] = bx[N + (y ^ x)];
+            if (act) {
+                pr = bx[y ^ x];
+                pi = bx[N + (y ^ x)];
             } else {
-                pr[0] = pi[0] = 0.0;
+                pr = pi = 0.0;
             }
             xb ^= 1u;
-        } else {   // one buffer: a barrier before each write (its readers are done) and after it
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < AMP; ++j) xbuf[ix[j]] = vr[j];
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < AMP; ++j) pr[j] = xbuf[ix[j] ^ x];
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < AMP; ++j) xbuf[ix[j]] = vi[j];
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < AMP; ++j) pi[j] = xbuf[ix[j] ^ x];
         }
     };
     // outcome k of an op's references (uniform)
@@ -1072,17 +1023,14 @@ __global__ __launch_bounds__(1024) void k_tiny_prog(const ProgArgs a) {
         if (k + 1 < a.nops) nx = sop[k + 1];
         if (kind == (uint32_t)kMeasOp) {   // k_measure_small's sums for one qubit
             double s0 = 0.0, s1 = 0.0;
-            if (AMP == 1 && N <= (uint32_t)kBlock) {   // one term per thread
-                const double p = vr[0] * vr[0] + vi[0] * vi[0];
-                if (act[0]) {
+            if (N <= (uint32_t)kBlock) {   // one term per thread
+                const double p = vr * vr + vi * vi;
+                if (act) {
                     if (y & x) s1 += p;
                     else s0 += p;
                 }
             } else {   // thread t < kBlock sums amplitudes t, t + kBlock, ...
-                if (AMP > 1) __syncthreads();   // pbuf is the exchange buffer: its readers are done
-#pragma unroll
-                for (int j = 0; j < AMP; ++j)
-                    if (act[j]) pbuf[ix[j]] = vr[j] * vr[j] + vi[j] * vi[j];
+                if (act) pbuf[y] = vr * vr + vi * vi;
                 __syncthreads();
                 if (y < (uint32_t)kBlock)
                     for (uint32_t i = y; i < N; i += kBlock) {
@@ -1111,113 +1059,101 @@ __global__ __launch_bounds__(1024) void k_tiny_prog(const ProgArgs a) {
             const uint32_t keep = cbit ? x : 0u;
             if ((y & 63u) == 0u) lout[k] = cbit;   // each wave's own write precedes its reads
             if (y == 0) a.slots[yz] = cbit;
-#pragma unroll
-            for (int j = 0; j < AMP; ++j)
-                if (act[j]) {
-                    const bool kp = (ix[j] & x) == keep;
-                    vr[j] = kp ? vr[j] * scale : 0.0;
-                    vi[j] = kp ? vi[j] * scale : 0.0;
-                }
+            if (act) {
+                const bool kp = (y & x) == keep;
+                vr = kp ? vr * scale : 0.0;
+                vi = kp ? vi * scale : 0.0;
+            }
             if ((kf & kPfReset) && cbit) {   // apply_reset: X on q over the gc bits (suqa.cu:735-737)
-                double pr[AMP], pi[AMP];
+                double pr, pi;
                 exchange(x, pr, pi);
                 const uint32_t g = m & ~x;
-#pragma unroll
-                for (int j = 0; j < AMP; ++j)
-                    if (act[j] && (ix[j] & g) == g) {
-                        vr[j] = pr[j];
-                        vi[j] = pi[j];
-                    }
+                if (act && (y & g) == g) {
+                    vr = pr;
+                    vi = pi;
+                }
             }
             continue;
         }
         if (kind == (uint32_t)kWOp) {   // k_qms_W: pairs (j, i = j + N/2)
             const uint32_t half = N >> 1;
-            double pr[AMP], pi[AMP];
+            double pr, pi;
             exchange(half, pr, pi);
-            const uint32_t eb = (kf >> 16) & 255u;
-            uint32_t Eref = ref;
-            if (!m && eb) {
-                Eref = 0;
-                for (uint32_t b = 0; b < eb; ++b) Eref |= outcome(kf, ref, b) << b;
-            }
-            const double* tab = reinterpret_cast<const double*>(__double_as_longlong(c));
-#pragma unroll
-            for (int j = 0; j < AMP; ++j) {
-                if (!act[j]) continue;
-                const uint32_t i = ix[j] | half;
+            if (act) {
+                const uint32_t i = y | half;
                 const uint32_t En = (i & x) >> (v & 255u);
-                const uint32_t Eo = m ? (i & m) >> ((v >> 8) & 255u) : Eref;
-                const double* row = tab + 2 * (uint64_t)Eo * yz;
+                uint32_t Eo;
+                const uint32_t eb = (kf >> 16) & 255u;
+                if (m) {
+                    Eo = (i & m) >> ((v >> 8) & 255u);
+                } else if (eb) {
+                    Eo = 0;
+                    for (uint32_t b = 0; b < eb; ++b) Eo |= outcome(kf, ref, b) << b;
+                } else {
+                    Eo = ref;
+                }
+                const double* row = reinterpret_cast<const double*>(__double_as_longlong(c)) + 2 * (uint64_t)Eo * yz;
                 const double fs1 = row[2 * En], fs2 = row[2 * En + 1];
-                if (ix[j] < half) {   // j: t = own, ri = the partner
-                    vr[j] = fs2 * vr[j] + fs1 * pr[j];
-                    vi[j] = fs2 * vi[j] + fs1 * pi[j];
-                } else {              // i: t = the partner, ri = own
-                    vr[j] = fs1 * pr[j] - fs2 * vr[j];
-                    vi[j] = fs1 * pi[j] - fs2 * vi[j];
+                if (y < half) {   // j: t = own, ri = the partner
+                    vr = fs2 * vr + fs1 * pr;
+                    vi = fs2 * vi + fs1 * pi;
+                } else {          // i: t = the partner, ri = own
+                    vr = fs1 * pr - fs2 * vr;
+                    vi = fs1 * pi - fs2 * vi;
                 }
             }
             continue;
         }
         if ((kf & kPfCond) && !outcome(kf, ref, 0)) continue;   // uniform: a conditional gate not taken
         if (kind == (uint32_t)kPhaseOp) {
-#pragma unroll
-            for (int j = 0; j < AMP; ++j)
-                if (act[j] && (ix[j] & m) == v) {
-                    const uint32_t par = (uint32_t)__popc(ix[j] & yz) & 1u;
-                    const double sn = par ? -s : s;
-                    const double rr = vr[j], mm = vi[j];
-                    vr[j] = phase_out(false, rr, mm, c, sn);
-                    vi[j] = phase_out(true, mm, rr, c, sn);
-                }
+            if (act && (y & m) == v) {
+                const uint32_t par = (uint32_t)__popc(y & yz) & 1u;
+                const double sn = par ? -s : s;
+                const double rr = vr, mm = vi;
+                vr = phase_out(false, rr, mm, c, sn);
+                vi = phase_out(true, mm, rr, c, sn);
+            }
             continue;
         }
-        double pr[AMP], pi[AMP];
+        double pr, pi;
         exchange(x, pr, pi);
+        if (!act) continue;
+        // roles of tile_pairs: u has the pivot bit 0; the selected one of (u, u ^ X) is the a side
+        const uint32_t w = y ^ x;
         const uint32_t pivot = (kf >> 9) & 31u;
+        const uint32_t u = ((y >> pivot) & 1u) ? w : y;
+        const uint32_t uw = u ^ x;
+        const bool su = (u & m) == v;
+        if (!su && (uw & m) != v) continue;
+        const bool is_a = y == (su ? u : uw);
+        const uint32_t par = (uint32_t)__popc(w & yz) & 1u;   // parity of the partner
         const Coef<double> pc{c, s, (kf & kPfImag) ? 1u : 0u, (kf & kPfNeg) ? 1u : 0u};
-#pragma unroll
-        for (int j = 0; j < AMP; ++j) {
-            if (!act[j]) continue;
-            // roles of tile_pairs: u has the pivot bit 0; the selected one of (u, u ^ X) is the a side
-            const uint32_t yy = ix[j];
-            const uint32_t w = yy ^ x;
-            const uint32_t u = ((yy >> pivot) & 1u) ? w : yy;
-            const uint32_t uw = u ^ x;
-            const bool su = (u & m) == v;
-            if (!su && (uw & m) != v) continue;
-            const bool is_a = yy == (su ? u : uw);
-            const uint32_t par = (uint32_t)__popc(w & yz) & 1u;   // parity of the partner
-            double nr, ni;
-            switch (kind) {
-            case OP_SWAP:
-                nr = pair_out_par<OP_SWAP>(is_a, false, vr[j], pr[j], pi[j], par, pc);
-                ni = pair_out_par<OP_SWAP>(is_a, true, vi[j], pi[j], pr[j], par, pc);
-                break;
-            case OP_Y:
-                nr = pair_out_par<OP_Y>(is_a, false, vr[j], pr[j], pi[j], par, pc);
-                ni = pair_out_par<OP_Y>(is_a, true, vi[j], pi[j], pr[j], par, pc);
-                break;
-            case OP_H:
-                nr = pair_out_par<OP_H>(is_a, false, vr[j], pr[j], pi[j], par, pc);
-                ni = pair_out_par<OP_H>(is_a, true, vi[j], pi[j], pr[j], par, pc);
-                break;
-            default:
-                nr = pair_out_par<OP_PAULI>(is_a, false, vr[j], pr[j], pi[j], par, pc);
-                ni = pair_out_par<OP_PAULI>(is_a, true, vi[j], pi[j], pr[j], par, pc);
-                break;
-            }
-            vr[j] = nr;
-            vi[j] = ni;
+        double nr, ni;
+        switch (kind) {
+        case OP_SWAP:
+            nr = pair_out_par<OP_SWAP>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_SWAP>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        case OP_Y:
+            nr = pair_out_par<OP_Y>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_Y>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        case OP_H:
+            nr = pair_out_par<OP_H>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_H>(is_a, true, vi, pi, pr, par, pc);
+            break;
+        default:
+            nr = pair_out_par<OP_PAULI>(is_a, false, vr, pr, pi, par, pc);
+            ni = pair_out_par<OP_PAULI>(is_a, true, vi, pi, pr, par, pc);
+            break;
         }
+        vr = nr;
+        vi = ni;
     }
-#pragma unroll
-    for (int j = 0; j < AMP; ++j)
-        if (act[j]) {
-            a.re[ix[j]] = vr[j];
-            a.im[ix[j]] = vi[j];
-        }
+    if (act) {
+        a.re[y] = vr;
+        a.im[y] = vi;
+    }
 }
 
 // Runs of consecutive set bits of m, lowest first, as lo | len << 8 | src << 16
@@ -1660,10 +1596,8 @@ namespace {
 // read the outcome slot (written by an earlier launch).
 hipError_t run_tiny_programs(suqa_hip_ctx* c, const std::vector<GateDesc>& q) {
     const uint32_t N = 1u << c->nq;
-    // one amplitude per thread up to 1024 amplitudes, then 2 or 4 per thread of 1024
-    const uint32_t amp = N <= 1024u ? 1u : N / 1024u;
-    const uint32_t nthr = amp == 1 ? std::max<uint32_t>(N, (uint32_t)kBlock) : 1024u;
-    const size_t lds = sizeof(double) * (amp == 1 ? 5 : 1) * (size_t)N;
+    const uint32_t nthr = std::max<uint32_t>(N, (uint32_t)kBlock);
+    const size_t lds = sizeof(double) * 5 * (size_t)N;
     size_t i = 0;
     while (i < q.size()) {
         ProgArgs a;
@@ -1756,9 +1690,7 @@ hipError_t run_tiny_programs(suqa_hip_ctx* c, const std::vector<GateDesc>& q) {
         ++c->passes;
         c->pass_gates += n;
         timing_begin(c, "tiny_prog", (double)alg);
-        if (amp == 1) hipLaunchKernelGGL(k_tiny_prog<1>, dim3(1), dim3(nthr), lds, c->stream, a);
-        else if (amp == 2) hipLaunchKernelGGL(k_tiny_prog<2>, dim3(1), dim3(nthr), lds, c->stream, a);
-        else hipLaunchKernelGGL(k_tiny_prog<4>, dim3(1), dim3(nthr), lds, c->stream, a);
+        hipLaunchKernelGGL(k_tiny_prog, dim3(1), dim3(nthr), lds, c->stream, a);
         timing_end(c);
         if (hipError_t e = hipGetLastError()) return e;
     }
