@@ -459,9 +459,14 @@ __global__ __launch_bounds__(kBlock) void k_collapse(double* __restrict__ re, do
     v.store(base + i);
 }
 
-// launch_measure_small (engine.hpp): one workgroup, the whole state.  The
-// probabilities are a fixed-order block reduction; every thread reads the
-// same two sums from LDS, so the outcome and the scale are uniform.
+// launch_measure_small (engine.hpp): one workgroup, the whole state held in
+// registers, APT amplitudes per thread (amplitude y + T j in register j of
+// thread y; T = blockDim.x = 256 for N <= 256, else min(N, 1024)).  The
+// probabilities are a fixed-order block reduction: thread t < kBlock sums
+// |psi_i|^2 for i = t, t + kBlock, t + 2 kBlock, ... in increasing i (the
+// amplitudes of T consecutive indices pass through LDS at a time), then the
+// wave butterfly and the four waves' sums in order; every thread reads the same
+// partial sums, so the outcome and the scale are uniform.
 struct MeasArgs {
     uint64_t N;
     uint32_t n, resets, gc, pad_;
@@ -469,64 +474,113 @@ struct MeasArgs {
     double rd[kMeasMax];
 };
 
-__global__ __launch_bounds__(kBlock) void k_measure_small(double* __restrict__ re, double* __restrict__ im,
-                                                          const MeasArgs a, uint32_t* __restrict__ out) {
-    __shared__ double red[2][kBlock / 64];
-    __shared__ double fin[2];
-    const uint32_t tid = threadIdx.x;
+template <int APT>
+__global__ __launch_bounds__(1024) void k_measure_small(double* __restrict__ re, double* __restrict__ im,
+                                                        const MeasArgs a, uint32_t* __restrict__ out) {
+    __shared__ double pbuf[1024];        // |psi|^2 of T consecutive indices
+    __shared__ double xr[1024], xi[1024];   // the reset's exchange
+    __shared__ double red[2][2][kBlock / 64];
+    const uint32_t T = blockDim.x, y = threadIdx.x;
+    const uint32_t N = (uint32_t)a.N;
+    const uint32_t tb = (uint32_t)__builtin_ctz(T);
+    double vr[APT], vi[APT];
+#pragma unroll
+    for (int j = 0; j < APT; ++j) {
+        const uint32_t i = y + T * (uint32_t)j;
+        vr[j] = i < N ? re[i] : 0.0;
+        vi[j] = i < N ? im[i] : 0.0;
+    }
+    uint32_t rb = 0;
     for (uint32_t k = 0; k < a.n; ++k) {
         const uint32_t q = a.q[k];
-        const uint64_t qb = 1ull << q;
+        const uint32_t qb = 1u << q;
         double s0 = 0.0, s1 = 0.0;
-        for (uint64_t i = tid; i < a.N; i += kBlock) {
-            const double x = re[i], y = im[i];
-            const double p = x * x + y * y;
-            if (i & qb) s1 += p;
-            else s0 += p;
-        }
-        s0 = wave_sum(s0);
-        s1 = wave_sum(s1);
-        if ((tid & 63u) == 0u) {
-            red[0][tid >> 6] = s0;
-            red[1][tid >> 6] = s1;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double t0 = 0.0, t1 = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) {
-                t0 += red[0][w];
-                t1 += red[1][w];
+        if (N <= (uint32_t)kBlock) {   // one term per thread: thread y holds amplitude y
+            const double p = vr[0] * vr[0] + vi[0] * vi[0];
+            if (y < N) {
+                if (y & qb) s1 += p;
+                else s0 += p;
             }
-            fin[0] = t0;
-            fin[1] = t1;
+        } else {
+#pragma unroll
+            for (int j = 0; j < APT; ++j) {
+                pbuf[y] = vr[j] * vr[j] + vi[j] * vi[j];
+                __syncthreads();
+                if (y < (uint32_t)kBlock)
+                    for (uint32_t u = y; u < T; u += kBlock) {
+                        const double p = pbuf[u];
+                        if ((u + T * (uint32_t)j) & qb) s1 += p;
+                        else s0 += p;
+                    }
+                __syncthreads();
+            }
+        }
+        if (y < (uint32_t)kBlock) {
+            s0 = wave_sum(s0);
+            s1 = wave_sum(s1);
+            if ((y & 63u) == 0u) {
+                red[rb][0][y >> 6] = s0;
+                red[rb][1][y >> 6] = s1;
+            }
         }
         __syncthreads();
-        const double p0 = fin[0], p1 = fin[1];
-        const uint32_t cbit = a.rd[k] < p1 ? 1u : 0u;   // suqa.cu:690
+        double p0 = 0.0, p1 = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            p0 += red[rb][0][w];
+            p1 += red[rb][1][w];
+        }
+        rb ^= 1u;
+        const uint32_t cbit = __builtin_amdgcn_readfirstlane(a.rd[k] < p1 ? 1u : 0u);   // suqa.cu:690
         const double scale = 1. / sqrt(cbit ? p1 : p0);  // set_ampl_to_zero + vnormalize (suqa.cu:694-695)
-        const uint64_t keep = cbit ? qb : 0ull;
-        for (uint64_t i = tid; i < a.N; i += kBlock) {
-            const bool kp = (i & qb) == keep;
-            re[i] = kp ? re[i] * scale : 0.0;
-            im[i] = kp ? im[i] * scale : 0.0;
+        const uint32_t keep = cbit ? qb : 0u;
+#pragma unroll
+        for (int j = 0; j < APT; ++j) {
+            const bool kp = ((y + T * (uint32_t)j) & qb) == keep;
+            vr[j] = kp ? vr[j] * scale : 0.0;
+            vi[j] = kp ? vi[j] * scale : 0.0;
         }
-        if (tid == 0) out[k] = cbit;
-        __syncthreads();
-        if (((a.resets >> k) & 1u) && cbit) {   // apply_reset: X on q (suqa.cu:735-737)
-            const uint64_t M = (uint64_t)a.gc | qb, V = M & ~qb;
-            for (uint64_t pr = tid; pr < a.N / 2; pr += kBlock) {
-                const uint64_t i = ((pr >> q) << (q + 1)) | (pr & (qb - 1ull));
-                if ((i & M) != V) continue;
-                const uint64_t j = i | qb;
-                const double tr = re[i], ti = im[i];
-                re[i] = re[j];
-                im[i] = im[j];
-                re[j] = tr;
-                im[j] = ti;
+        if (y == 0) out[k] = cbit;
+        if (((a.resets >> k) & 1u) && cbit) {   // apply_reset: X on q over the other gc bits (suqa.cu:735-737)
+            const uint32_t g = a.gc & ~qb;   // both ends of a pair meet it or neither does
+            if (qb >= T) {   // the partner in this thread's register j ^ (qb / T)
+                const uint32_t h = qb >> tb;
+#pragma unroll
+                for (int hh = 1; hh < APT; hh <<= 1) {
+                    if (h != (uint32_t)hh) continue;
+#pragma unroll
+                    for (int j = 0; j < APT; ++j) {
+                        if (j & hh) continue;
+                        if (((y + T * (uint32_t)j) & g) != g) continue;
+                        const double tr = vr[j], ti = vi[j];
+                        vr[j] = vr[j | hh];
+                        vi[j] = vi[j | hh];
+                        vr[j | hh] = tr;
+                        vi[j | hh] = ti;
+                    }
+                }
+            } else {         // the partner in thread y ^ qb, same register
+#pragma unroll
+                for (int j = 0; j < APT; ++j) {
+                    xr[y] = vr[j];
+                    xi[y] = vi[j];
+                    __syncthreads();
+                    const double pr = xr[y ^ qb], pi = xi[y ^ qb];
+                    __syncthreads();
+                    if (((y + T * (uint32_t)j) & g) == g) {
+                        vr[j] = pr;
+                        vi[j] = pi;
+                    }
+                }
             }
-            __syncthreads();
         }
-        __syncthreads();   // fin[] is rewritten by the next measurement
+    }
+#pragma unroll
+    for (int j = 0; j < APT; ++j) {
+        const uint32_t i = y + T * (uint32_t)j;
+        if (i < N) {
+            re[i] = vr[j];
+            im[i] = vi[j];
+        }
     }
 }
 
@@ -865,7 +919,16 @@ hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const doubl
         a.rd[k] = rds[k];
     }
     timing_begin(c, "measure_small", 40.0 * (double)c->N * n);
-    hipLaunchKernelGGL(k_measure_small, dim3(1), dim3(kBlock), 0, c->stream, c->d_re, c->d_im, a, c->d_flags);
+    // the whole state in registers: 256 threads up to 256 amplitudes, then up to 1024 threads
+    const uint32_t T = c->N <= (uint64_t)kBlock ? (uint32_t)kBlock : (uint32_t)std::min<uint64_t>(c->N, 1024);
+    const uint32_t apt = (uint32_t)std::max<uint64_t>(1, c->N / T);
+    switch (apt) {
+    case 1: hipLaunchKernelGGL(k_measure_small<1>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
+    case 2: hipLaunchKernelGGL(k_measure_small<2>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
+    case 4: hipLaunchKernelGGL(k_measure_small<4>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
+    case 8: hipLaunchKernelGGL(k_measure_small<8>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
+    default: hipLaunchKernelGGL(k_measure_small<16>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
+    }
     timing_end(c);
     return hipGetLastError();
 }
