@@ -34,6 +34,7 @@ std::vector<uint> bm_enes_old;
 uint W_mask_Eold = 0U;
 bool use_graphs = false;
 bool deferred = false;
+bool defer_values = false;
 
 namespace {
 
@@ -337,6 +338,7 @@ int metro_step_deferred(bool take_measure) {
             apply_Phi();
             measure_discarded(bm_enes_new, extract_rands(ene_qbits));   // E re-collapse
             apply_Phi_inverse();
+            if (!defer_values) read_slots();
             ret = 2;
         } else {
             ret = 1;
@@ -361,6 +363,7 @@ int metro_step_deferred(bool take_measure) {
                 apply_Phi();
                 measure_discarded(bm_enes_new, extract_rands(ene_qbits));   // E re-collapse
                 apply_Phi_inverse();
+                if (!defer_values) read_slots();
                 ret = 4;
             } else {
                 ret = 3;
@@ -488,6 +491,7 @@ void setup(double beta) {
     deferred = !(d && std::strcmp(d, "0") == 0) && !use_graphs && !two_registers &&
                suqa::gatecounters.counters.empty() &&
                suqa_hip_measure_qbits_deferred(suqa::ctx, nullptr, nullptr, 0, &probe) == SUQA_OK;
+    defer_values = false;   // a driver that syncs before reading turns it on after setup
     D = Deferred{};
 }
 

@@ -47,8 +47,14 @@ extern bool use_graphs;
 // same accept/reject sequence and values.  setup() turns it on unless
 // SUQA_QMS_DEFERRED=0; never with gate counters, graphs or two registers.
 extern bool deferred;
-// Resolves the E and X values of deferred steps (E_measures, X_measures):
-// drivers call it before they read those vectors.
+// With deferred readouts a step's E and X values (E_measures, X_measures)
+// exist on the host only after a wait.  By default metro_step waits for them
+// before it returns (a measured step then waits twice), so the vectors are
+// complete after every step, as with the reference.  A driver that reads them
+// only through sync_measures() (the bundled suqa_qms and suqa_qms_run) sets
+// defer_values after setup() and lets them wait for the next step's wait.
+extern bool defer_values;
+// Resolves the E and X values of deferred steps (E_measures, X_measures).
 void sync_measures();
 
 // Gate-count records of the Metropolis step (qms.cuh:21-27, defined in

@@ -101,6 +101,7 @@ int main(int argc, char** argv) {
         auto t_start = std::chrono::high_resolution_clock::now();
         suqa::setup(qms::nqubits);
         qms::setup(p.beta);
+        qms::defer_values = true;   // save_measures reads E/X after sync_measures()
 #ifdef GATECOUNT
         // the gate-count build of the driver (qms.cu:97-105)
         suqa::gatecounters.add_counter(&qms::gctr_global);

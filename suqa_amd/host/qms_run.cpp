@@ -3,6 +3,7 @@
 // setup and per-step code.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -60,6 +61,10 @@ extern "C" int suqa_qms_run(const suqa_qms_params* p, suqa_qms_result* out) {
         qmsdrv::configure(*p);
         suqa::setup(qms::nqubits);
         qms::setup(p->beta);
+        {   // E/X are read after sync_measures() below; SUQA_QMS_DEFER_VALUES=0 resolves them every step
+            const char* e = std::getenv("SUQA_QMS_DEFER_VALUES");
+            qms::defer_values = !(e && std::strcmp(e, "0") == 0);
+        }
         init_state();
         out->accepted = 0;
         bool take_measure;

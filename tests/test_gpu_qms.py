@@ -26,15 +26,19 @@ def log(m):
     print(m, flush=True)
 
 
-@pytest.mark.parametrize("graphs", ["1", "0"], ids=["graphs", "no-graphs"])
+@pytest.mark.parametrize("mode", ["graphs", "deferred", "deferred-values-per-step", "synchronous"])
 @pytest.mark.parametrize("fx", QMS_FIXTURES)
-def test_plain_qms_vs_golden(qms, fx, graphs, monkeypatch):
-    """With the step's fixed launch sequences captured once and replayed
-    (default) and with every gate call launched."""
-    monkeypatch.setenv("SUQA_QMS_GRAPHS", graphs)
+def test_plain_qms_vs_golden(qms, fx, mode, monkeypatch):
+    """With the step's fixed launch sequences captured once and replayed; with
+    deferred readouts (the default), their E/X values left for the next wait
+    (the library driver's default) or resolved before every step returns (the
+    default for other callers of qms::metro_step); with every readout waited for."""
+    monkeypatch.setenv("SUQA_QMS_GRAPHS", "1" if mode == "graphs" else "0")
+    monkeypatch.setenv("SUQA_QMS_DEFERRED", "0" if mode == "synchronous" else "1")
+    monkeypatch.setenv("SUQA_QMS_DEFER_VALUES", "0" if mode == "deferred-values-per-step" else "1")
     d = load(fx)
     p = qms_params(d)
-    log(f"plain QMS {fx} graphs {graphs}")
+    log(f"plain QMS {fx} {mode}")
     out = qms.run(p["beta"], p["steps"], p["reset_each"], p["r"], p["seed"], p["ene_min"], p["ene_max"],
                   p["pe_steps"], p["thermalization"], p["max_reverse"], p["record_reverse"], final_state=True)
     assert list(out["rets"]) == d["rets"]
