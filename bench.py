@@ -406,13 +406,17 @@ def kernel_roofline(rep, fam, peak_gbs=HBM_PEAK_GBS, mfma_tflops=None):
             "frac": round(max(t_hbm, t_mfma) / t, 4)}
 
 
-def fused_leg(capi, wl, eng, gates, n, layers, no_cpu):
+def fused_leg(capi, wl, eng, gates, n, layers, no_cpu, use_cache=True):
     """FP32 fused-block run of the whole circuit; eng holds the FP64 state after all layers."""
+    from suqa_amd import tune_cache
     f32 = capi.F32Engine(n)
     try:
         sample = capi.gates_from(gates[:10 * n])
-        best, tms = f32.autotune(sample)
-        log(f"fused: autotuned width {best}, passes {f32.passes} ({tms})")
+        if use_cache:
+            best, tms, how = tune_cache.f32_config(f32, sample, "random-circuit-seed7")
+        else:
+            (best, tms), how = f32.autotune(sample), "off"
+        log(f"fused: autotuned width {best}, passes {f32.passes} (cache {how}; {tms})")
         arr = capi.gates_from(gates[:layers * n])
         t0 = time.perf_counter()
         capi.fuse_plan(arr, n, best)
@@ -437,7 +441,8 @@ def fused_leg(capi, wl, eng, gates, n, layers, no_cpu):
         out = {
             "width": None if f32.passes == 3 else best, "mode": ["blocks", "passes", "sparse", "gates"][f32.passes],
             "tile_bits": f32.tile_bits if f32.passes == 3 else None,
-            "ms_per_gate_by_mode_width": {k: (None if v < 0 else round(v, 4)) for k, v in tms.items()},
+            "ms_per_gate_by_mode_width": {k: (None if v is None or v < 0 else round(v, 4)) for k, v in (tms or {}).items()},
+            "tune_cache": how,
             "layers": layers, "gates": layers * n, "blocks": nb, "gates_per_block": round(layers * n / nb, 2),
             "seconds": round(dt, 4), "layers_per_s": round(layers / dt, 3),
             "ms_per_layer": round(dt / layers * 1e3, 3), "host_fusion_seconds": round(t_plan, 4),
@@ -594,6 +599,8 @@ def main():
                     help="FP64 passes: commuting gates packed (amplitudes within 1e-12 of call order) or call order")
     ap.add_argument("--tile-bits", type=int, default=-1,
                     help="FP64 tile size (0: one sweep per gate); default: autotuned")
+    ap.add_argument("--no-tune-cache", action="store_true",
+                    help="autotune every run instead of reusing the JSON cache's choices (suqa_amd/tune_cache.py)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); the engine has no multi-GPU "
@@ -602,7 +609,7 @@ def main():
     if a.qms_batch_only:
         print(json.dumps({"mode": "qms_batch_only", **qms_batch_only()}), flush=True)
         return
-    from suqa_amd import capi, workloads as wl
+    from suqa_amd import capi, tune_cache, workloads as wl
     n = a.n
     layers = max(CIRCUIT_LAYERS, a.warmup + a.steps)
     gates, mq, mr = wl.random_circuit(n, layers, 7)
@@ -614,9 +621,14 @@ def main():
     tune = None
     eng.set_gate_order(a.gate_order == "packed")
     if a.tile_bits < 0:
-        best, tms = eng.autotune_tile_bits(capi.gates_from(gates[:5 * n]), (0, 8, 9, 10, 11, 12))
-        tune = {"tile_bits": best, "ms_per_gate": {str(k): round(v, 4) for k, v in tms.items()}}
-        log(f"autotuned tile bits {best}: {tms}")
+        if a.no_tune_cache:
+            best, tms = eng.autotune_tile_bits(capi.gates_from(gates[:5 * n]), (0, 8, 9, 10, 11, 12))
+            tms, how = {str(k): round(v, 4) for k, v in tms.items()}, "off"
+        else:
+            best, tms, how = tune_cache.tile_bits(eng, capi.gates_from(gates[:5 * n]), (0, 8, 9, 10, 11, 12),
+                                                  f"random-circuit-seed7-{a.gate_order}")
+        tune = {"tile_bits": best, "ms_per_gate": tms, "cache": how}
+        log(f"autotuned tile bits {best} (cache {how}): {tms}")
     else:
         eng.set_tile_bits(a.tile_bits)
     eng.init_state()
@@ -658,7 +670,8 @@ def main():
     pass_info = {"passes": npass, "gates_in_passes": ps1["pass_gates"] - ps0["pass_gates"],
                  "single_sweeps": ps1["single_sweeps"] - ps0["single_sweeps"],
                  "gates_per_pass": round((ps1["pass_gates"] - ps0["pass_gates"]) / npass, 2) if npass else None}
-    f32, fres = (None, None) if a.no_fused else fused_leg(capi, wl, eng, gates, n, layers, a.no_cpu)
+    f32, fres = (None, None) if a.no_fused else fused_leg(capi, wl, eng, gates, n, layers, a.no_cpu,
+                                                          not a.no_tune_cache)
     # the final measurements of the workload (not in the timed region)
     tm = time.perf_counter()
     outcomes = [eng.measure(q, r) for q, r in zip(mq, mr)]
