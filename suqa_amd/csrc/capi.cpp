@@ -697,7 +697,7 @@ static int measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds,
         const uint32_t slot = sqe::alloc_slots(c, n);
         sqe::queue_measure(c, qs, rds, n, reset, gc, slot);
         if (!cs) {
-            if (c->queue.size() >= sqe::kAutoFlush) SQE_HIP(sqe::flush(c));
+            if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
             return SUQA_OK;
         }
         SQE_HIP(sqe::flush(c));
@@ -788,7 +788,7 @@ int suqa_hip_measure_qbits_deferred(suqa_hip_ctx* c, const uint32_t* qs, const d
         if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
     *slot = sqe::alloc_slots(c, n);
     sqe::queue_measure(c, qs, rdoubs, n, false, 0u, *slot);
-    if (c->queue.size() >= sqe::kAutoFlush) SQE_HIP(sqe::flush(c));
+    if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
     return SUQA_OK;
 }
 
@@ -829,7 +829,7 @@ int suqa_hip_apply_x_if(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_
         d.cond_ref = (slot + k) % sqe::kSlots;
         c->queue.push_back(d);
     }
-    if (c->queue.size() >= sqe::kAutoFlush) SQE_HIP(sqe::flush(c));
+    if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
     return SUQA_OK;
 }
 
@@ -948,7 +948,7 @@ int suqa_hip_qms_apply_W_deferred(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmas
     d.eold_bits = (uint32_t)__builtin_popcount(wmask);
     d.tag = "qms_W";
     c->queue.push_back(d);
-    if (c->queue.size() >= sqe::kAutoFlush) SQE_HIP(sqe::flush(c));
+    if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
     return SUQA_OK;
 }
 

@@ -325,6 +325,12 @@ bool span_tiles_env();
 std::vector<uint32_t> pauli_pass_order(const suqa_hip_ctx* c, const suqa_hip_pauli_op* ops, uint32_t n);
 constexpr size_t kAutoFlush = 256;   // queued gates that trigger a flush (keeps the GPU busy)
 constexpr size_t kAutoFlush32 = 1024;   // the same for queues that pack commuting gates (a longer reorder span)
+// Queue length that triggers a flush: a full program for small states (the GPU
+// runs it while the host queues the next operations), else the above.
+inline size_t auto_flush_at(const suqa_hip_ctx* c) {
+    if (tiny_prog(c)) return kProgMax;
+    return (c->f32 || c->gate_reorder) ? kAutoFlush32 : kAutoFlush;
+}
 
 // the message suqa_hip_last_error() returns on this thread (capi.cpp)
 void set_last_error(const std::string& msg);

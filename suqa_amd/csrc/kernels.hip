@@ -675,7 +675,7 @@ hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, 
     d.tag = tag;
     d.alg_bytes = alg_bytes;
     c->queue.push_back(d);
-    return c->queue.size() >= ((c->f32 || c->gate_reorder) ? kAutoFlush32 : kAutoFlush) ? flush(c) : hipSuccess;
+    return c->queue.size() >= auto_flush_at(c) ? flush(c) : hipSuccess;
 }
 
 hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
@@ -691,7 +691,7 @@ hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, u
     d.tag = tag;
     d.alg_bytes = alg_bytes;
     c->queue.push_back(d);
-    return c->queue.size() >= ((c->f32 || c->gate_reorder) ? kAutoFlush32 : kAutoFlush) ? flush(c) : hipSuccess;
+    return c->queue.size() >= auto_flush_at(c) ? flush(c) : hipSuccess;
 }
 
 hipError_t launch_pair_now(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,
@@ -966,7 +966,7 @@ hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_tab, uint32_t wmask, ui
         d.L = L;
         d.tag = "qms_W";
         c->queue.push_back(d);
-        return c->queue.size() >= kAutoFlush ? flush(c) : hipSuccess;
+        return c->queue.size() >= auto_flush_at(c) ? flush(c) : hipSuccess;
     }
     if (hipError_t e = flush(c)) return e;
     const uint64_t half = c->N / 2;
