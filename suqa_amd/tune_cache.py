@@ -26,16 +26,23 @@ def cache_path():
 
 
 def device_name():
-    """The HIP device 0's name, or "unknown" where no GPU is visible."""
+    """HIP device 0: its name, else its compute capability and memory size
+    (some boxes report an empty name); "unknown" where no GPU is visible."""
     for name in ("libamdhip64.so", "libamdhip64.so.7", "/opt/rocm/lib/libamdhip64.so"):
         try:
             hip = C.CDLL(name)
         except OSError:
             continue
-        buf = C.create_string_buffer(256)
         dev = C.c_int(0)
-        if hip.hipDeviceGet(C.byref(dev), 0) == 0 and hip.hipDeviceGetName(buf, 256, dev) == 0 and buf.value:
+        if hip.hipDeviceGet(C.byref(dev), 0) != 0:
+            break
+        buf = C.create_string_buffer(256)
+        if hip.hipDeviceGetName(buf, 256, dev) == 0 and buf.value:
             return buf.value.decode(errors="replace")
+        major, minor, mem = C.c_int(0), C.c_int(0), C.c_size_t(0)
+        if hip.hipDeviceComputeCapability(C.byref(major), C.byref(minor), dev) == 0 and \
+                hip.hipDeviceTotalMem(C.byref(mem), dev) == 0:
+            return f"hip-cc{major.value}.{minor.value}-{mem.value >> 30}GiB"
         break
     return "unknown"
 
