@@ -252,7 +252,7 @@ constexpr uint32_t kMinTileBits = 1, kMaxTileBits = 12;
 constexpr uint32_t kMaxTileBits32 = 13;   // float tiles (64 KiB of LDS)
 constexpr uint32_t kPassLowFixed = 4;    // qubits 0..3 are in every pass tile (one 128-byte line)
 constexpr uint32_t kPassLowFixed32 = 5;  // the same for float states: qubits 0..4
-constexpr uint32_t kPassMaxGates = 24;   // gates per pass
+constexpr uint32_t kPassMaxGates = 32;   // gates per pass
 // The selection a pass must honour for gate d.  A Pauli string's sweep
 // enumerates each pair (i, i ^ X) once, from the element whose lowest flipped
 // qubit is 0 (that qubit in M at value 0); its arithmetic treats both elements

@@ -91,15 +91,18 @@ struct GateR {
 };
 static_assert(sizeof(GateR) == 80, "GateR is read as 64 + 16 bytes");
 
+// The LDS-step fields in 16 and 8 bits (tile positions are below 2^13), so
+// that kMaxPass descriptors fit the 4 KiB of kernel arguments.
 struct TileGate : GateR {
-    uint32_t kind;        // OP_SWAP / OP_Y / OP_H / OP_PAULI or kPhaseOp
-    uint32_t pivot;       // tile position of the lowest partner bit (pair operations)
-    uint32_t xc, mc, vc;  // partner mask and selection in tile positions (LDS steps)
-    uint32_t yzc;         // parity mask in tile positions (Y|Z positions or the phase's zmask)
-    uint32_t xl;          // partner mask over F
-    uint32_t dense;       // gtriv, and every element (pair) of a group is selected
-    uint32_t base_imag;
+    uint16_t xc, mc, vc;  // partner mask and selection in tile positions (LDS steps)
+    uint16_t yzc;         // parity mask in tile positions (Y|Z positions or the phase's zmask)
+    uint8_t kind;         // OP_SWAP / OP_Y / OP_H / OP_PAULI or kPhaseOp
+    uint8_t pivot;        // tile position of the lowest partner bit (pair operations)
+    uint8_t xl;           // partner mask over F
+    uint8_t dense;        // gtriv, and every element (pair) of a group is selected
+    uint8_t base_imag;
 };
+static_assert(sizeof(TileGate) == 96, "TileGate layout");
 
 struct TileArgs {
     uint64_t ntiles;
@@ -1379,10 +1382,12 @@ hipError_t run_pass(suqa_hip_ctx* c, const GateDesc* q, uint32_t n, uint32_t t, 
         const GateDesc& d = q[i];
         TileGate& g = a.g[i];
         g.kind = (uint32_t)d.kind;
-        if (d.kind != kPhaseOp && (!B.coords(d.X, &g.xc) || g.xc == 0u)) {
+        uint32_t xcoord = 0;
+        if (d.kind != kPhaseOp && (!B.coords(d.X, &xcoord) || xcoord == 0u)) {
             set_last_error("pass tile: a flip mask outside the tile");
             return hipErrorInvalidValue;
         }
+        g.xc = (uint16_t)xcoord;
         // selection: coordinates for qubits the tile reads, tile-level bits for
         // the rest; a Pauli string's pair enumeration qubit (pass_sel) becomes
         // its lowest flipped coordinate at 0
