@@ -126,12 +126,13 @@ int suqa_hip_measure_qbits(suqa_hip_ctx* ctx, const uint32_t* qs, const double* 
 int suqa_hip_apply_resets(suqa_hip_ctx* ctx, const uint32_t* qs, const double* rdoubs, uint32_t n, uint32_t gc_mask);
 /* Extension (no counterpart in the reference): deferred measurements, for
  * callers that need an outcome on the device before they need it on the host
- * (qms::'s energy readout, which the X flips and apply_W read).  On a state of
- * at most 2^10 amplitudes that is one pass tile, measure_qbits' measurements
- * (draws rdoubs[k]) are queued beside the gates and their outcomes land in
- * outcome slots *slot .. *slot + n - 1 of a ring of SUQA_HIP_OUTCOME_SLOTS per
- * context; a slot keeps its outcome until that many later slots have been
- * handed out.  SUQA_ERR_UNSUPPORTED on any other state. */
+ * (qms::'s energy readout, which the X flips and apply_W read).  measure_qbits'
+ * measurements (draws rdoubs[k]) with their outcomes in outcome slots
+ * *slot .. *slot + n - 1 of a ring of SUQA_HIP_OUTCOME_SLOTS per context, and
+ * nothing waits: on a state of at most 2^10 amplitudes that is one pass tile
+ * they are queued beside the gates, on other states of at most 2^14 amplitudes
+ * they are launched at once.  A slot keeps its outcome until that many later
+ * slots have been handed out.  SUQA_ERR_UNSUPPORTED on larger states. */
 #define SUQA_HIP_OUTCOME_SLOTS 1024u
 int suqa_hip_measure_qbits_deferred(suqa_hip_ctx* ctx, const uint32_t* qs, const double* rdoubs, uint32_t n,
                                     uint32_t* slot);
@@ -139,7 +140,9 @@ int suqa_hip_measure_qbits_deferred(suqa_hip_ctx* ctx, const uint32_t* qs, const
  * out[0..n): applies what is queued and waits for it. */
 int suqa_hip_read_outcomes(suqa_hip_ctx* ctx, uint32_t slot, uint32_t n, uint32_t* out);
 /* X on qs[k] over the indices with the gc bits set, where outcome slot + k
- * reads 1 (queued; the flips of a measured register, qms.cuh:404-406). */
+ * reads 1 (the flips of a measured register, qms.cuh:404-406): queued on a
+ * state of one small pass tile, else one launch (states of at most 2^14
+ * amplitudes). */
 int suqa_hip_apply_x_if(suqa_hip_ctx* ctx, const uint32_t* qs, uint32_t n, uint32_t slot, uint32_t gc_mask);
 /* suqa::prob_filter (suqa.cu:705-727): sum |psi_i|^2 over i whose bits qs[k]
  * read q_mask[k]. */

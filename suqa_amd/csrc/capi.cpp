@@ -782,13 +782,23 @@ int suqa_hip_measure_qbits_deferred(suqa_hip_ctx* c, const uint32_t* qs, const d
                                     uint32_t* slot) {
     SQE_CTX(c);
     if (!slot || (n && (!qs || !rdoubs))) return fail(SUQA_ERR_ARG, "null argument");
-    if (!sqe::tiny_prog(c)) return fail(SUQA_ERR_UNSUPPORTED, "deferred measurements need a state of one small pass tile");
+    const bool prog = sqe::tiny_prog(c);
+    if (!prog && (c->f32 || !c->d_slots || c->nq > sqe::kMeasSmallBits))
+        return fail(SUQA_ERR_UNSUPPORTED, "deferred measurements need a state of at most 2^14 amplitudes");
     if (n > sqe::kSlots / 2) return fail(SUQA_ERR_ARG, "too many measurements in one call");
     for (uint32_t k = 0; k < n; ++k)
         if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
     *slot = sqe::alloc_slots(c, n);
-    sqe::queue_measure(c, qs, rdoubs, n, false, 0u, *slot);
-    if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
+    if (prog) {   // beside the queued gates
+        sqe::queue_measure(c, qs, rdoubs, n, false, 0u, *slot);
+        if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
+        return SUQA_OK;
+    }
+    if (c->capturing) return fail(SUQA_ERR_ARG, "deferred measurements are not captured in graphs on this state");
+    for (uint32_t k0 = 0; k0 < n; k0 += sqe::kMeasMax) {   // at once, the outcomes into the ring
+        const uint32_t m = std::min<uint32_t>(sqe::kMeasMax, n - k0);
+        SQE_HIP(sqe::launch_measure_small(c, qs + k0, rdoubs + k0, m, 0u, 0u, c->d_slots + *slot + k0));
+    }
     return SUQA_OK;
 }
 
@@ -810,12 +820,20 @@ int suqa_hip_read_outcomes(suqa_hip_ctx* c, uint32_t slot, uint32_t n, uint32_t*
 int suqa_hip_apply_x_if(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_t slot, uint32_t gc) {
     SQE_CTX(c);
     if (n && !qs) return fail(SUQA_ERR_ARG, "null argument");
-    if (!sqe::tiny_prog(c)) return fail(SUQA_ERR_UNSUPPORTED, "conditional gates need a state of one small pass tile");
+    const bool prog = sqe::tiny_prog(c);
+    if (!prog && (c->f32 || !c->d_slots || c->nq > sqe::kMeasSmallBits))
+        return fail(SUQA_ERR_UNSUPPORTED, "conditional gates need a state of at most 2^14 amplitudes");
     if (slot >= sqe::kSlots) return fail(SUQA_ERR_ARG, "outcome slot out of the ring");
     if ((uint64_t)gc >> c->nq) return fail(SUQA_ERR_ARG, "gc_mask has qubits beyond the state");
     for (uint32_t k = 0; k < n; ++k) {
         if (qs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
         if ((gc >> qs[k]) & 1u) return fail(SUQA_ERR_ARG, "a conditional X on a qubit of the gc_mask");
+    }
+    if (!prog) {   // one launch after the queued gates, reading the slots on the device
+        if (slot + n > sqe::kSlots) return fail(SUQA_ERR_ARG, "outcome slots run past the ring's end");
+        for (uint32_t k0 = 0; k0 < n; k0 += sqe::kMeasMax)
+            SQE_HIP(sqe::launch_x_if(c, qs + k0, c->d_slots + slot + k0, std::min<uint32_t>(sqe::kMeasMax, n - k0), gc));
+        return SUQA_OK;
     }
     for (uint32_t k = 0; k < n; ++k) {   // pair_1q's selection (i & (gc|T)) == gc|T on i = j|T
         const uint64_t T = 1ull << qs[k];

@@ -207,7 +207,10 @@ hipError_t launch_checksum(suqa_hip_ctx* c);
 // with the gc bits set.  n <= kMeasMax.
 constexpr uint32_t kMeasSmallBits = 14, kMeasMax = 16;
 hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, uint32_t resets,
-                                uint32_t gc);
+                                uint32_t gc, uint32_t* out = nullptr);   // out: device outcomes (d_flags when null)
+// X on qubit qs[k] over the indices with the gc bits set where the device
+// outcome flags[k] is 1, k < n <= kMeasMax, in order (one launch).
+hipError_t launch_x_if(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* flags, uint32_t n, uint32_t gc);
 // W of qms.cuh:220-242 over the acceptance pairs (i, i without the top bit):
 // coefficients (fs1, fs2) at d_tab[2 (Eo L + En)], En = (i & wmask) >> b0 and
 // Eo = (i & omask) >> ob0 (omask = 0: Eo = 0, d_tab is the row of a classical

@@ -584,6 +584,37 @@ __global__ __launch_bounds__(1024) void k_measure_small(double* __restrict__ re,
     }
 }
 
+// launch_x_if: X on qubit q[k] over the indices with the gc bits set where
+// the outcome flags[k] reads 1 (the flips of a measured register), pairs
+// (i, i | 2^q) with one thread each; the qubits in order, a barrier between
+// them (one workgroup: small states only).
+struct XifArgs {
+    uint32_t n, gc;
+    uint32_t q[kMeasMax];
+    const uint32_t* flags;
+};
+__global__ __launch_bounds__(1024) void k_x_if(double* __restrict__ re, double* __restrict__ im, uint64_t N,
+                                               const XifArgs a) {
+    for (uint32_t k = 0; k < a.n; ++k) {
+        const uint32_t f = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(a.flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (!f) continue;   // uniform
+        const uint64_t qb = 1ull << a.q[k];
+        const uint64_t g = (uint64_t)a.gc & ~qb;
+        for (uint64_t p = threadIdx.x; p < N / 2; p += blockDim.x) {
+            const uint64_t i = ((p >> a.q[k]) << (a.q[k] + 1)) | (p & (qb - 1ull));
+            if ((i & g) != g) continue;
+            const uint64_t j = i | qb;
+            const double tr = re[i], ti = im[i];
+            re[i] = re[j];
+            im[i] = im[j];
+            re[j] = tr;
+            im[j] = ti;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scale(double* __restrict__ data, uint64_t nitems, double f) {
     const uint64_t it = item_id();
     if (it >= nitems) return;
@@ -904,7 +935,8 @@ hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double
 }
 
 hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, uint32_t resets,
-                                uint32_t gc) {
+                                uint32_t gc, uint32_t* out) {
+    if (!out) out = c->d_flags;
     if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
     if (n == 0 || n > kMeasMax || c->nq > kMeasSmallBits) return hipErrorInvalidValue;
@@ -923,12 +955,32 @@ hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const doubl
     const uint32_t T = c->N <= (uint64_t)kBlock ? (uint32_t)kBlock : (uint32_t)std::min<uint64_t>(c->N, 1024);
     const uint32_t apt = (uint32_t)std::max<uint64_t>(1, c->N / T);
     switch (apt) {
-    case 1: hipLaunchKernelGGL(k_measure_small<1>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
-    case 2: hipLaunchKernelGGL(k_measure_small<2>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
-    case 4: hipLaunchKernelGGL(k_measure_small<4>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
-    case 8: hipLaunchKernelGGL(k_measure_small<8>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
-    default: hipLaunchKernelGGL(k_measure_small<16>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, c->d_flags); break;
+    case 1: hipLaunchKernelGGL(k_measure_small<1>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, out); break;
+    case 2: hipLaunchKernelGGL(k_measure_small<2>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, out); break;
+    case 4: hipLaunchKernelGGL(k_measure_small<4>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, out); break;
+    case 8: hipLaunchKernelGGL(k_measure_small<8>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, out); break;
+    default: hipLaunchKernelGGL(k_measure_small<16>, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, a, out); break;
     }
+    timing_end(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_x_if(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* flags, uint32_t n, uint32_t gc) {
+    if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
+    if (hipError_t e = flush(c)) return e;
+    if (n == 0) return hipSuccess;
+    if (n > kMeasMax || c->nq > kMeasSmallBits) return hipErrorInvalidValue;
+    XifArgs a{};
+    a.n = n;
+    a.gc = gc;
+    a.flags = flags;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (qs[k] >= c->nq) return hipErrorInvalidValue;
+        a.q[k] = qs[k];
+    }
+    const uint32_t T = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(c->N / 2, (uint64_t)kBlock), 1024);
+    timing_begin(c, "x_if", 32.0 * (double)c->N * n);
+    hipLaunchKernelGGL(k_x_if, dim3(1), dim3(T), 0, c->stream, c->d_re, c->d_im, c->N, a);
     timing_end(c);
     return hipGetLastError();
 }

@@ -305,19 +305,27 @@ std::vector<double> extract_rands(uint n) {
 
 namespace {
 
-// metro_step with deferred readouts (one register, no counters, no graphs):
-// the draws, gates and measurements of metro_step below in the same order.
+// metro_step with deferred readouts (no counters, no graphs): the draws,
+// gates and measurements of metro_step below in the same order.
 int metro_step_deferred(bool take_measure) {
     int ret = 0;
     reset_non_syst_qbits();
-    apply_Phi();
-    const uint32_t s_eold = measure_deferred(bm_enes_new, extract_rands(ene_qbits));
-    chk(suqa_hip_apply_x_if(suqa::ctx, bm_enes_new.data(), ene_qbits, s_eold, 0u), "suqa::apply_x()");
-    gCi = draw_C();
-    apply_C(gCi);   // apply_U: C_i, Phi, W with E_old read on the device
-    apply_Phi();
-    chk(suqa_hip_qms_apply_W_deferred(suqa::ctx, bm_acc, W_mask_Enew, bm_enes_new[0], c_factor, s_eold),
-        "qms::apply_W()");
+    uint32_t s_eold;
+    if (two_registers) {   // E_old into its own register, read by W there
+        apply_Phi_old();
+        s_eold = measure_deferred(bm_enes_old, extract_rands(ene_qbits));
+        gCi = draw_C();
+        apply_U();
+    } else {
+        apply_Phi();
+        s_eold = measure_deferred(bm_enes_new, extract_rands(ene_qbits));
+        chk(suqa_hip_apply_x_if(suqa::ctx, bm_enes_new.data(), ene_qbits, s_eold, 0u), "suqa::apply_x()");
+        gCi = draw_C();
+        apply_C(gCi);   // apply_U: C_i, Phi, W with E_old read on the device
+        apply_Phi();
+        chk(suqa_hip_qms_apply_W_deferred(suqa::ctx, bm_acc, W_mask_Enew, bm_enes_new[0], c_factor, s_eold),
+            "qms::apply_W()");
+    }
     const uint32_t s_acc = measure_deferred({bm_acc}, {rangen.doub()});
     read_slots();   // the step's one wait
     curr_E_old = slot_value(s_eold, ene_qbits);
@@ -332,7 +340,14 @@ int metro_step_deferred(bool take_measure) {
         if (take_measure) {
             D.recs.push_back(Deferred::Rec{false, s_enew, E_measures.size()});
             E_measures.push_back(0.0);
-            chk(suqa_hip_apply_x_if(suqa::ctx, bm_enes_new.data(), ene_qbits, s_enew, 0u), "suqa::apply_x()");
+            if (two_registers) {   // each E_old qubit reset with its draw before its flip (qms.cuh:431-434)
+                for (uint ei = 0U; ei < ene_qbits; ++ei) {
+                    suqa::apply_reset(bm_enes_old[ei], rangen.doub());
+                    chk(suqa_hip_apply_x_if(suqa::ctx, &bm_enes_new[ei], 1, s_enew + ei, 0u), "suqa::apply_x()");
+                }
+            } else {
+                chk(suqa_hip_apply_x_if(suqa::ctx, bm_enes_new.data(), ene_qbits, s_enew, 0u), "suqa::apply_x()");
+            }
             push_x_deferred();
             suqa::apply_reset(bm_enes_new, extract_rands(ene_qbits));
             apply_Phi();
@@ -350,6 +365,7 @@ int metro_step_deferred(bool take_measure) {
     uint iters = 0;
     while (iters < max_reverse_attempts) {
         apply_Phi();
+        if (two_registers) (void)extract_rands(ene_qbits);   // the E_old readout's draws (qms.cuh:478-479)
         const uint32_t s_en = measure_deferred(bm_enes_new, extract_rands(ene_qbits));
         read_slots();
         const uint Enew_meas = slot_value(s_en, ene_qbits);
@@ -488,7 +504,7 @@ void setup(double beta) {
     use_graphs = e && std::strcmp(e, "1") == 0;
     const char* d = std::getenv("SUQA_QMS_DEFERRED");
     uint32_t probe = 0;
-    deferred = !(d && std::strcmp(d, "0") == 0) && !use_graphs && !two_registers &&
+    deferred = !(d && std::strcmp(d, "0") == 0) && !use_graphs &&
                suqa::gatecounters.counters.empty() &&
                suqa_hip_measure_qbits_deferred(suqa::ctx, nullptr, nullptr, 0, &probe) == SUQA_OK;
     defer_values = false;   // a driver that syncs before reading turns it on after setup

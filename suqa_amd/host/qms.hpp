@@ -38,14 +38,15 @@ extern uint W_mask_Eold;
 // two kernel launches, and a graph launch costs more host time than they do.
 // Never with gate counters.
 extern bool use_graphs;
-// Extension: deferred readouts.  On a state of at most 2^10 amplitudes (the
-// engine's queued small-state programs) the step keeps the outcomes that only
-// the device needs on the device: the E_old readout feeds the X flips and
-// apply_W there, and the E and X values of a measured step are read at the
-// next wait, so a step waits once (the acceptance) plus once per reverse
-// attempt.  The same draws, gates and measurements in the same order, so the
-// same accept/reject sequence and values.  setup() turns it on unless
-// SUQA_QMS_DEFERRED=0; never with gate counters, graphs or two registers.
+// Extension: deferred readouts.  On a state of at most 2^14 amplitudes the
+// step keeps the outcomes that only the device needs on the device: the E_old
+// readout feeds the X flips and apply_W there (two registers: W reads the
+// register), the flips of E_new read their outcomes there, and the E and X
+// values of a measured step are read at a later wait, so a step waits once
+// (the acceptance) plus once per reverse attempt.  The same draws, gates and
+// measurements in the same order, so the same accept/reject sequence and
+// values.  setup() turns it on unless SUQA_QMS_DEFERRED=0; never with gate
+// counters or graphs.
 extern bool deferred;
 // With deferred readouts a step's E and X values (E_measures, X_measures)
 // exist on the host only after a wait.  By default metro_step waits for them

@@ -4,7 +4,8 @@ give the outcomes and the state, bit for bit, of the one-measurement kernel
 (tile passes off); deferred measurements (outcomes left on the device) match
 the ones read back; the conditional X and the apply_W that read outcome slots
 on the device match their host-driven forms, also when the measurement ran in
-an earlier program of the same queue; larger states refuse them."""
+an earlier program of the same queue; states of 11-14 qubits run them as
+launches of their own; larger states refuse them."""
 import numpy as np
 import pytest
 
@@ -133,9 +134,33 @@ def test_deferred_apply_W_matches_host_E_old(capi):
             e.close()
 
 
+@pytest.mark.parametrize("n", [12, 13])
+def test_deferred_on_states_beyond_one_program(capi, n):
+    """11-14 qubits: the measurements launch at once with their outcomes in the
+    ring, the conditional X is its own launch reading them."""
+    log(f"deferred launches n={n}")
+    rng = np.random.default_rng(40 + n)
+    qs = [2, 7, n - 1]
+    rd = [float(x) for x in rng.uniform(0, 1, len(qs))]
+    host = prepared(capi, n, 5)
+    dev = prepared(capi, n, 5)
+    cs = host.measure_qbits(qs, rd)
+    slot = dev.measure_qbits_deferred(qs, rd)
+    for q, c in zip(qs, cs):
+        if c:
+            host.x(q)
+    dev.x_if(qs, slot)
+    host.h(3)
+    dev.h(3)
+    same_state(host, dev, "deferred launches")
+    assert dev.read_outcomes(slot, len(qs)) == cs
+    host.close()
+    dev.close()
+
+
 def test_larger_states_refuse_deferred_operations(capi):
-    log("deferred on 12 qubits")
-    eng = capi.Engine(12)
+    log("deferred on 15 qubits")
+    eng = capi.Engine(15)
     with pytest.raises(capi.SuqaError) as ei:
         eng.measure_qbits_deferred([0, 1], [0.5, 0.5])
     assert ei.value.code == -2
