@@ -71,6 +71,7 @@ struct GateDesc {
     // slots eold_ref .. + eold_bits - 1 read LSB first, or omask bits)
     uint32_t cond_ref = kNoRef;
     uint32_t q = 0, slot = 0, reset = 0;
+    uint32_t meas_group = 0;   // kMeasOp: on the first of a call's measurements, their number
     const double* tab = nullptr;
     uint32_t wmask = 0, b0 = 0, omask = 0, ob0 = 0, L = 0;
     uint32_t eold = 0, eold_ref = kNoRef, eold_bits = 0;

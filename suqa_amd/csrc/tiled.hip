@@ -1633,6 +1633,9 @@ hipError_t run_tiny_programs(suqa_hip_ctx* c, const std::vector<GateDesc>& q) {
         uint64_t alg = 0;
         for (; i < q.size() && n < kProgMax; ++i) {
             const GateDesc& d = q[i];
+            // a call's measurements stay in one program, so that an operation reading
+            // their outcomes reads them all from this program or all from the slots
+            if (d.kind == kMeasOp && n > 0 && d.meas_group <= kProgMax && n + d.meas_group > kProgMax) break;
             ProgOp& o = a.op[n];
             o.kf = (uint32_t)d.kind;
             if (d.kind == kMeasOp) {
@@ -1714,6 +1717,7 @@ void queue_measure(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint3
         d.M = gc;
         d.reset = reset ? 1u : 0u;
         d.slot = (slot + k) % kSlots;
+        d.meas_group = k == 0 ? n : 0u;
         d.tag = "measure";
         c->queue.push_back(d);
     }

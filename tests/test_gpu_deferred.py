@@ -134,6 +134,37 @@ def test_deferred_apply_W_matches_host_E_old(capi):
             e.close()
 
 
+@pytest.mark.parametrize("lead", [88, 91, 92, 93, 95])
+def test_measurements_at_program_boundaries(capi, lead):
+    """A register's deferred measurements queued where a 96-operation program
+    ends (after `lead` gates), then apply_W reading them: the same state and
+    outcomes as the one-operation kernels."""
+    r, n = 4, 8
+    wmask = ((1 << r) - 1) << 3
+    log(f"program boundary, {lead} gates first")
+    rng = np.random.default_rng(lead)
+    rd = [float(x) for x in rng.uniform(0, 1, r)]
+    dev = prepared(capi, n, 30)
+    plain = prepared(capi, n, 30, 0)
+    dev.synchronize()
+    for k in range(lead):
+        dev.u1(k % n, 0.01 * (k + 1))
+        plain.u1(k % n, 0.01 * (k + 1))
+    ene = list(range(3, 3 + r))
+    slot = dev.measure_qbits_deferred(ene, rd)
+    cs = plain.measure_qbits(ene, rd)
+    dev.x_if(ene, slot)
+    for q, c in zip(ene, cs):
+        if c:
+            plain.x(q)
+    dev.qms_apply_W_deferred(n - 1, wmask, 3, 0.7, slot)
+    plain.qms_apply_W(n - 1, wmask, 3, 0.7, sum(c << k for k, c in enumerate(cs)))
+    same_state(dev, plain, "boundary")
+    assert dev.read_outcomes(slot, r) == cs
+    dev.close()
+    plain.close()
+
+
 @pytest.mark.parametrize("n", [12, 13])
 def test_deferred_on_states_beyond_one_program(capi, n):
     """11-14 qubits: the measurements launch at once with their outcomes in the
