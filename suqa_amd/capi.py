@@ -23,6 +23,7 @@ CTX = C.c_void_p
 SUQA_OK = 0
 SUQA_ERR_ARG = -1
 SUQA_ERR_UNSUPPORTED = -2
+OUTCOME_SLOTS = 1024   # SUQA_HIP_OUTCOME_SLOTS: the device ring of deferred outcomes
 
 GATE_H, GATE_X, GATE_RX, GATE_RZ, GATE_U1, GATE_CX, GATE_MCX = range(7)
 

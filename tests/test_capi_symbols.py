@@ -58,6 +58,13 @@ def test_binding_covers_header(built):
     assert set(capi.SIGNATURES) == declared("suqa_hip.h", "suqa_hip_")
 
 
+def test_binding_constants_match_header():
+    from suqa_amd import capi
+    with open(os.path.join(ROOT, "include", "suqa_hip.h")) as f:
+        src = f.read()
+    assert int(re.search(r"#define SUQA_HIP_OUTCOME_SLOTS (\d+)u", src).group(1)) == capi.OUTCOME_SLOTS
+
+
 def test_library_loads_without_gpu(built):
     lib = ctypes.CDLL(built)
     for name in declared("suqa_hip.h", "suqa_hip_"):

@@ -198,3 +198,65 @@ def test_larger_states_refuse_deferred_operations(capi):
     with pytest.raises(capi.SuqaError):
         eng.x_if([0], 0)
     eng.close()
+
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_outcome_ring_wraps(capi, n):
+    """More deferred measurements than the ring has slots: a group of slots never
+    straddles the ring's end, the conditional X reads each group after the wrap
+    as before it, and the last groups still read back their outcomes."""
+    log(f"outcome ring wrap n={n}")
+    rng = np.random.default_rng(70 + n)
+    host = prepared(capi, n, 9)
+    dev = prepared(capi, n, 9)
+    groups = []
+    for k in range(450):   # 1350 slots: the ring (1024) wraps once
+        qs = [k % n, (k + 3) % n, (k + 5) % n]
+        rd = [float(x) for x in rng.uniform(0, 1, 3)]
+        cs = host.measure_qbits(qs, rd)
+        slot = dev.measure_qbits_deferred(qs, rd)
+        assert 0 <= slot and slot + 3 <= capi.OUTCOME_SLOTS
+        for q, c in zip(qs, cs):
+            if c:
+                host.x(q)
+        dev.x_if(qs, slot)
+        for e in (host, dev):
+            for q in qs:
+                e.h(q)
+            e.u1(k % n, 0.1 + 0.01 * k)
+        groups.append((slot, cs))
+    same_state(host, dev, "after the wrap")
+    starts = [s for s, _ in groups]
+    assert min(starts[1:]) == 0 and starts.index(0, 1) > 300   # wrapped once, after ~341 groups
+    for slot, cs in groups[-300:]:   # not yet overwritten
+        assert dev.read_outcomes(slot, 3) == cs
+    host.close()
+    dev.close()
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 8])
+def test_tiny_states_and_calls_longer_than_a_program(capi, n):
+    """States of 1-8 qubits, and calls of more measurements than one program
+    holds (split across launches): outcomes and amplitudes as the
+    one-measurement kernel's."""
+    log(f"tiny state n={n}, 150-measurement calls")
+    rng = np.random.default_rng(90 + n)
+    prog, plain = capi.Engine(n), capi.Engine(n, tile_bits=0)
+    for rep in range(3):
+        for e in (prog, plain):
+            for q in range(n):
+                e.h(q)
+                e.pauli([q], [1], 0.3 + 0.2 * q + rep)
+                e.u1(q, 0.7 - 0.1 * q)
+        qs = [int(q) for q in rng.integers(0, n, 150)]
+        rd = [float(x) for x in rng.uniform(0, 1, 150)]
+        assert prog.measure_qbits(qs, rd) == plain.measure_qbits(qs, rd)
+        same_state(prog, plain, "measure_qbits")
+        for e in (prog, plain):
+            for q in range(n):
+                e.h(q)
+        prog.resets(qs, rd[::-1])
+        plain.resets(qs, rd[::-1])
+        same_state(prog, plain, "resets")
+    prog.close()
+    plain.close()
