@@ -12,6 +12,7 @@
 // is compiled with -ffp-contract=off, so amplitudes match the reference CPU
 // path bit for bit (suqa_cpu.hpp:49-66 forms x*c - y*s, y*c + x*s).
 #include "engine.hpp"
+#include "lanes.hpp"
 #include "sweep_math.hpp"
 
 namespace sqe {
@@ -238,11 +239,9 @@ __global__ __launch_bounds__(kBlock) void k_phase_list(double* __restrict__ re, 
     im[i] = m * c + r * s;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+// the lane-xor butterfly over o = 32, 16, ..., 1 (every lane ends with the
+// same sum), by DPP moves and permlane swaps instead of LDS permutes
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum64(v); }
 
 struct NormArgs {
     Fixed fb;

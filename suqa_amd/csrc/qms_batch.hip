@@ -76,8 +76,9 @@ __device__ __forceinline__ double dpp_xor(double v) {
 }
 
 // Butterfly sum over the 64 lanes (every lane ends with a sum of all; lane
-// 0's is the one used): DPP moves for xor 1..8, a swizzle for 16, a permute
-// for 32 (12 two-dword permutes before: 24 LDS-path round trips).
+// 0's is the one used): DPP moves for xor 1..8, the permlane swaps for 16 and
+// 32 (the swap of rows, or of halves, hands each lane its partner's value
+// beside its own; 12 two-dword LDS permutes before).
 __device__ __forceinline__ double wave_sum(double v) {
     v += dpp_xor<0xB1, -1>(v);
     v += dpp_xor<0x4E, -1>(v);
@@ -85,11 +86,18 @@ __device__ __forceinline__ double wave_sum(double v) {
     v += dpp_xor<0x140, 0x141>(v);
     {
         const uint64_t u = (uint64_t)__double_as_longlong(v);
-        const int lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)u, 0x401F);   // xor 16
-        const int hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), 0x401F);
-        v += __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+        const auto l = __builtin_amdgcn_permlane16_swap((uint32_t)u, (uint32_t)u, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32), false, false);
+        v = __longlong_as_double((long long)(((uint64_t)h[0] << 32) | l[0])) +
+            __longlong_as_double((long long)(((uint64_t)h[1] << 32) | l[1]));
     }
-    v += __shfl_xor(v, 32, 64);
+    {
+        const uint64_t u = (uint64_t)__double_as_longlong(v);
+        const auto l = __builtin_amdgcn_permlane32_swap((uint32_t)u, (uint32_t)u, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32), false, false);
+        v = __longlong_as_double((long long)(((uint64_t)h[0] << 32) | l[0])) +
+            __longlong_as_double((long long)(((uint64_t)h[1] << 32) | l[1]));
+    }
     return v;
 }
 

@@ -38,6 +38,7 @@
 
 #include "deposit.hpp"
 #include "engine.hpp"
+#include "lanes.hpp"
 #include "sweep_math.hpp"
 
 namespace sqe {
@@ -958,12 +959,6 @@ struct ProgArgs {
     ProgOp op[kProgMax];
 };
 static_assert(sizeof(ProgArgs) <= 4096, "kernel arguments are limited to 4 KiB");
-
-__device__ __forceinline__ double wave_sum64(double v) {   // the butterfly of kernels.hip's wave_sum
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // One workgroup of max(N, kBlock) threads, amplitude y in thread y < N, the
 // program's operations in order: gates as k_tile_tiny (bit-identical to one
