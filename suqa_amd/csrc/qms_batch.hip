@@ -901,6 +901,9 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     std::vector<uint32_t> order(nchains);
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<hipStream_t> gst;   // a stream per group of equal LDS need: the groups run side by side
+    std::vector<hipEvent_t> gev;    // each group's end, waited for on st
+    uint32_t ngroups = 0;
     const size_t nsteps = (size_t)nchains * p->metro_steps;
     const size_t nmeas = (size_t)nchains * out->cap_meas;
     BatchDev b{};
@@ -929,12 +932,12 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     QB_HIP(hipMalloc(&d_X, sizeof(double) * (nmeas ? nmeas : 1)));
     QB_HIP(hipMalloc(&d_nm, sizeof(uint32_t) * nchains));
     QB_HIP(hipMalloc(&d_acc, sizeof(uint32_t) * nchains));
-    QB_HIP(hipMalloc(&d_next, sizeof(uint32_t)));
+    QB_HIP(hipMalloc(&d_next, sizeof(uint32_t) * nchains));   // a work-queue counter per group
     if (out->lds_accesses) QB_HIP(hipMalloc(&d_lds, sizeof(uint64_t) * nchains));
     if (out->final_state) QB_HIP(hipMalloc(&d_final, sizeof(double) * stotal));
     // chains grouped by the LDS their state needs, widest first: when LDS
-    // limits the resident waves, one launch per group, so narrow chains are not
-    // held to the widest chain's occupancy
+    // limits the resident waves, one launch per group (the launches run side by
+    // side), so narrow chains are not held to the widest chain's occupancy
     for (uint32_t c = 0; c < nchains; ++c) {
         const uint32_t n = chains[c].ene_qbits + 4;   // both layouts (two registers: see Chain)
         need[c] = (p->fp32 ? sizeof(float) : sizeof(double)) * 2 * ((size_t)1 << n);
@@ -951,7 +954,6 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     b.max_reverse = p->max_reverse;
     b.thermalization = p->thermalization;
     b.cap_meas = out->cap_meas;
-    b.next_chain = d_next;
     b.rets = d_rets;
     b.rng_after = d_rng;
     b.E = d_E;
@@ -994,17 +996,29 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     QB_HIP(hipEventCreate(&e0));
     QB_HIP(hipEventCreate(&e1));
     QB_HIP(hipEventRecord(e0, st));
+    // The groups run concurrently, each on its own stream with its own queue
+    // (their chains fill the CUs together: every group alone left most wave
+    // slots empty while its longest chains finished); st waits for all.
     for (uint32_t lo = 0; lo < nchains;) {   // one wave per chain, LDS sized for the group
         uint32_t hi = lo;
         while (hi < nchains && need[order[hi]] == need[order[lo]]) ++hi;
         b.first = lo;
         b.nchains = hi;
+        b.next_chain = d_next + ngroups;
         grid = hi - lo < (uint32_t)ncu * 16u ? hi - lo : (uint32_t)ncu * 16u;
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
+        hipStream_t gs = nullptr;
+        hipEvent_t ge = nullptr;
+        QB_HIP(hipStreamCreate(&gs));
+        gst.push_back(gs);
+        QB_HIP(hipEventCreateWithFlags(&ge, hipEventDisableTiming));
+        gev.push_back(ge);
+        ++ngroups;
+        QB_HIP(hipStreamWaitEvent(gs, e0, 0));
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gs, b.next_chain);
         if (reg && need[order[lo]] <= 2 * elt * 1024) {
             const uint32_t Ng = (uint32_t)(need[order[lo]] / (2 * elt));   // amplitudes per chain
             const size_t sh = 64 * sizeof(double) + 4 * (size_t)Ng * elt;
-            hipLaunchKernelGGL(k_qms_chain_reg<double>, dim3(hi - lo), dim3(Ng), sh, st, b);
+            hipLaunchKernelGGL(k_qms_chain_reg<double>, dim3(hi - lo), dim3(Ng), sh, gs, b);
         } else {
             // waves per chain: the fewer chains LDS holds per CU, the more waves
             // each (up to 8), so the SIMDs stay busy; the measurement's partial
@@ -1019,7 +1033,7 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
                 goto done;
             }
 #define QB_LAUNCH_LDS(TYPE, TWV) \
-    hipLaunchKernelGGL((k_qms_batch<TYPE, TWV>), dim3(grid), dim3(64 * TWV), sh, st, b)
+    hipLaunchKernelGGL((k_qms_batch<TYPE, TWV>), dim3(grid), dim3(64 * TWV), sh, gs, b)
             if (p->fp32) {
                 if (tw == 1) QB_LAUNCH_LDS(float, 1);
                 else if (tw == 2) QB_LAUNCH_LDS(float, 2);
@@ -1034,6 +1048,8 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
 #undef QB_LAUNCH_LDS
         }
         QB_HIP(hipGetLastError());
+        QB_HIP(hipEventRecord(ge, gs));
+        QB_HIP(hipStreamWaitEvent(st, ge, 0));
         lo = hi;
     }
     QB_HIP(hipEventRecord(e1, st));
@@ -1054,6 +1070,9 @@ extern "C" int suqa_qms_batch_run(const suqa_qms_chain* chains, uint32_t nchains
     }
     if (d_final) QB_HIP(hipMemcpy(out->final_state, d_final, sizeof(double) * stotal, hipMemcpyDeviceToHost));
 done:
+    for (hipStream_t gs : gst) (void)hipStreamSynchronize(gs);
+    for (hipStream_t gs : gst) (void)hipStreamDestroy(gs);
+    for (hipEvent_t ge : gev) (void)hipEventDestroy(ge);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamSynchronize(st);
