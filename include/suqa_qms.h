@@ -9,8 +9,10 @@
  *
  * suqa_qms_batch_run: many independent chains (frustrated triangle, uneven
  * energy-register widths, betas and seeds) in a few GPU launches (one per
- * group of equal register width, chains taken from a device work queue), each
+ * group of equal register width, chains taken from a device work queue; the
+ * launches run side by side on streams the call creates and destroys), each
  * chain following the same metro_step control flow and pcg32 draw order.
+ * The call returns when every chain has finished.
  * No counterpart in the reference, which runs one chain per process.
  */
 #ifndef SUQA_QMS_H
