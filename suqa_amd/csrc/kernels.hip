@@ -462,10 +462,10 @@ __global__ __launch_bounds__(kBlock) void k_collapse(double* __restrict__ re, do
 // registers, APT amplitudes per thread (amplitude y + T j in register j of
 // thread y; T = blockDim.x = 256 for N <= 256, else min(N, 1024)).  The
 // probabilities are a fixed-order block reduction: thread t < kBlock sums
-// |psi_i|^2 for i = t, t + kBlock, t + 2 kBlock, ... in increasing i (the
-// amplitudes of T consecutive indices pass through LDS at a time), then the
-// wave butterfly and the four waves' sums in order; every thread reads the same
-// partial sums, so the outcome and the scale are uniform.
+// |psi_i|^2 for i = t, t + kBlock, t + 2 kBlock, ... in increasing i (the terms
+// of up to 4 T consecutive indices pass through LDS at a time, one barrier
+// each), then the wave butterfly and the four waves' sums in order; every
+// thread reads the same partial sums, so the outcome and the scale are uniform.
 struct MeasArgs {
     uint64_t N;
     uint32_t n, resets, gc, pad_;
@@ -476,7 +476,8 @@ struct MeasArgs {
 template <int APT>
 __global__ __launch_bounds__(1024) void k_measure_small(double* __restrict__ re, double* __restrict__ im,
                                                         const MeasArgs a, uint32_t* __restrict__ out) {
-    __shared__ double pbuf[1024];        // |psi|^2 of T consecutive indices
+    constexpr int kPB = APT < 4 ? APT : 4;   // registers whose terms share one pass through LDS
+    __shared__ double pbuf[1024 * kPB];      // |psi|^2 of kPB T consecutive indices
     __shared__ double xr[1024], xi[1024];   // the reset's exchange
     __shared__ double red[2][2][kBlock / 64];
     const uint32_t T = blockDim.x, y = threadIdx.x;
@@ -502,16 +503,17 @@ __global__ __launch_bounds__(1024) void k_measure_small(double* __restrict__ re,
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < APT; ++j) {
-                pbuf[y] = vr[j] * vr[j] + vi[j] * vi[j];
+            for (int j0 = 0; j0 < APT; j0 += kPB) {
+                if (j0) __syncthreads();   // the previous block's terms are summed
+#pragma unroll
+                for (int jj = 0; jj < kPB; ++jj) pbuf[y + T * (uint32_t)jj] = vr[j0 + jj] * vr[j0 + jj] + vi[j0 + jj] * vi[j0 + jj];
                 __syncthreads();
                 if (y < (uint32_t)kBlock)
-                    for (uint32_t u = y; u < T; u += kBlock) {
+                    for (uint32_t u = y; u < T * (uint32_t)kPB; u += kBlock) {   // index u + T j0, increasing
                         const double p = pbuf[u];
-                        if ((u + T * (uint32_t)j) & qb) s1 += p;
+                        if ((u + T * (uint32_t)j0) & qb) s1 += p;
                         else s0 += p;
                     }
-                __syncthreads();
             }
         }
         if (y < (uint32_t)kBlock) {
