@@ -697,7 +697,7 @@ static int measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds,
         const uint32_t slot = sqe::alloc_slots(c, n);
         sqe::queue_measure(c, qs, rds, n, reset, gc, slot);
         if (!cs) {
-            if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
+            if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush_auto(c));
             return SUQA_OK;
         }
         SQE_HIP(sqe::flush(c));
@@ -791,7 +791,7 @@ int suqa_hip_measure_qbits_deferred(suqa_hip_ctx* c, const uint32_t* qs, const d
     *slot = sqe::alloc_slots(c, n);
     if (prog) {   // beside the queued gates
         sqe::queue_measure(c, qs, rdoubs, n, false, 0u, *slot);
-        if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
+        if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush_auto(c));
         return SUQA_OK;
     }
     if (c->capturing) return fail(SUQA_ERR_ARG, "deferred measurements are not captured in graphs on this state");
@@ -847,7 +847,7 @@ int suqa_hip_apply_x_if(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_
         d.cond_ref = (slot + k) % sqe::kSlots;
         c->queue.push_back(d);
     }
-    if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
+    if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush_auto(c));
     return SUQA_OK;
 }
 
@@ -966,7 +966,7 @@ int suqa_hip_qms_apply_W_deferred(suqa_hip_ctx* c, uint32_t q_acc, uint32_t wmas
     d.eold_bits = (uint32_t)__builtin_popcount(wmask);
     d.tag = "qms_W";
     c->queue.push_back(d);
-    if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush(c));
+    if (c->queue.size() >= sqe::auto_flush_at(c)) SQE_HIP(sqe::flush_auto(c));
     return SUQA_OK;
 }
 

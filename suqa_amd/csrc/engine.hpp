@@ -229,6 +229,10 @@ hipError_t launch_phase_now(suqa_hip_ctx* c, const Sel& sel, double cs, double s
                             const char* tag, double alg_bytes);
 // Applies the queued gates in tile passes (tiled.hip); a no-op when empty.
 hipError_t flush(suqa_hip_ctx* c);
+// The flush a queue of auto_flush_at(c) operations triggers: the passes of the
+// queue but its last ones, whose gates stay queued to be packed with the next
+// operations (tiled.hip).
+hipError_t flush_auto(suqa_hip_ctx* c);
 // States of at most 2^kTinyProgBits amplitudes that are one pass tile
 // (tile_bits >= nq) are latency-bound: measurements, resets and apply_W queue
 // beside the gates, and a flush runs the queue as programs of up to kProgMax

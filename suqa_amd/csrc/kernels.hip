@@ -707,7 +707,7 @@ hipError_t launch_pair(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, 
     d.tag = tag;
     d.alg_bytes = alg_bytes;
     c->queue.push_back(d);
-    return c->queue.size() >= auto_flush_at(c) ? flush(c) : hipSuccess;
+    return c->queue.size() >= auto_flush_at(c) ? flush_auto(c) : hipSuccess;
 }
 
 hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, uint64_t zmask,
@@ -723,7 +723,7 @@ hipError_t launch_phase(suqa_hip_ctx* c, const Sel& sel, double cs, double sn, u
     d.tag = tag;
     d.alg_bytes = alg_bytes;
     c->queue.push_back(d);
-    return c->queue.size() >= auto_flush_at(c) ? flush(c) : hipSuccess;
+    return c->queue.size() >= auto_flush_at(c) ? flush_auto(c) : hipSuccess;
 }
 
 hipError_t launch_pair_now(suqa_hip_ctx* c, int op, const Sel& sel, uint64_t xmask, const PauliCoef& pc,
@@ -1019,7 +1019,7 @@ hipError_t launch_qms_W(suqa_hip_ctx* c, const double* d_tab, uint32_t wmask, ui
         d.L = L;
         d.tag = "qms_W";
         c->queue.push_back(d);
-        return c->queue.size() >= auto_flush_at(c) ? flush(c) : hipSuccess;
+        return c->queue.size() >= auto_flush_at(c) ? flush_auto(c) : hipSuccess;
     }
     if (hipError_t e = flush(c)) return e;
     const uint64_t half = c->N / 2;

@@ -1718,7 +1718,14 @@ void queue_measure(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint3
     }
 }
 
-hipError_t flush(suqa_hip_ctx* c) {
+// The queue in passes (programs for a tiny_prog state).  keep_tail, for the
+// flush a long queue triggers: the gates of the queue's last passes stay
+// queued, in their applied order, to join the operations queued next (packed:
+// the passes formed from the last 2 kPackWindow gates, which had few gates to
+// choose from; in list order: the last pass, which later gates may fill).
+// Either way the gates apply in an order flush() could have produced.
+constexpr size_t kPackWindow = 64;
+hipError_t run_queue(suqa_hip_ctx* c, bool keep_tail) {
     if (c->queue.empty()) return hipSuccess;
     std::vector<GateDesc> q;
     q.swap(c->queue);
@@ -1726,14 +1733,16 @@ hipError_t flush(suqa_hip_ctx* c) {
     const uint32_t t = std::min(c->tile_bits, c->nq);
     const uint32_t f = std::min(c->f32 ? kPassLowFixed32 : kLowFixed, t);   // whole 128-byte lines
     const bool span = c->span_tiles && c->nq > t;   // a state of one tile needs no span
-    if (c->gate_reorder) pack_gates(q, t, f, span, 64);
+    if (c->gate_reorder) pack_gates(q, t, f, span, kPackWindow);
     size_t i = 0;
     while (i < q.size()) {
+        if (keep_tail && c->gate_reorder && q.size() - i < 2 * kPackWindow) break;
         PassSpace ps;
         ps.low = (1ull << f) - 1ull;
         ps.span = span;
         size_t j = i;
         while (j < q.size() && j - i < kMaxPass && ps.add(gate_flips(q[j]), pass_sel(q[j]), t - f)) ++j;
+        if (keep_tail && j == q.size()) break;
         hipError_t e;
         if (c->f32) {   // float states have no one-gate sweeps: a lone gate is a pass of one
             if (j == i) {
@@ -1750,7 +1759,12 @@ hipError_t flush(suqa_hip_ctx* c) {
         if (e != hipSuccess) return e;
         i = j;
     }
+    if (i < q.size()) c->queue.assign(q.begin() + (std::ptrdiff_t)i, q.end());
     return hipSuccess;
 }
+
+hipError_t flush(suqa_hip_ctx* c) { return run_queue(c, false); }
+
+hipError_t flush_auto(suqa_hip_ctx* c) { return run_queue(c, true); }
 
 }  // namespace sqe
