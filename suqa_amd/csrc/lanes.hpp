@@ -3,7 +3,8 @@
 // across rows.  In lane_xor<O>, lane l receives the value of lane l ^ O.  The moves'
 // lane layouts (quad_perm, row_shl/row_shr with a bank mask, row_ror:8,
 // permlane16/32_swap) were checked on gfx950 by writing the lane index
-// through each of them.
+// through each of them; tests/test_gpu_reduction_order.py pins the sums
+// wave_sum64 forms, bit for bit, to a host restatement of the butterfly.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
