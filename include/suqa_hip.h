@@ -144,6 +144,15 @@ int suqa_hip_read_outcomes(suqa_hip_ctx* ctx, uint32_t slot, uint32_t n, uint32_
  * state of one small pass tile, else one launch (states of at most 2^14
  * amplitudes). */
 int suqa_hip_apply_x_if(suqa_hip_ctx* ctx, const uint32_t* qs, uint32_t n, uint32_t slot, uint32_t gc_mask);
+/* For k = 0 .. n-1 in order: apply_reset(qs[k], rdoubs[k]) over the gc bits,
+ * then X on xqs[k] over the gc bits where outcome slot + k reads 1 -- the
+ * reference's interleaving of the E_old register's resets with the flips of
+ * the measured E_new register (qms.cuh:431-434, the two-register layout).
+ * The same operations as those calls in that order (bit-identical); on a
+ * state of at most 2^14 amplitudes that is not one pass tile, one launch
+ * instead of 2n. */
+int suqa_hip_resets_then_x_if(suqa_hip_ctx* ctx, const uint32_t* qs, const double* rdoubs, const uint32_t* xqs,
+                              uint32_t n, uint32_t slot, uint32_t gc_mask);
 /* suqa::prob_filter (suqa.cu:705-727): sum |psi_i|^2 over i whose bits qs[k]
  * read q_mask[k]. */
 int suqa_hip_prob_filter(suqa_hip_ctx* ctx, const uint32_t* qs, const uint32_t* q_mask,

@@ -69,6 +69,7 @@ SIGNATURES = {
     "suqa_hip_measure_qbits_deferred": (C.c_int, [CTX, PU32, C.POINTER(C.c_double), U32, PU32]),
     "suqa_hip_read_outcomes": (C.c_int, [CTX, U32, U32, PU32]),
     "suqa_hip_apply_x_if": (C.c_int, [CTX, PU32, U32, U32, U32]),
+    "suqa_hip_resets_then_x_if": (C.c_int, [CTX, PU32, PD, PU32, U32, U32, U32]),
     "suqa_hip_qms_apply_W_deferred": (C.c_int, [CTX, U32, U32, U32, C.c_double, U32]),
     "suqa_hip_prob_filter": (C.c_int, [CTX, PU32, PU32, U32, PD]),
     "suqa_hip_qms_apply_W": (C.c_int, [CTX, U32, U32, U32, C.c_double, U32]),
@@ -304,6 +305,12 @@ class Engine:
 
     def x_if(self, qs, slot):
         check(lib().suqa_hip_apply_x_if(self.ctx, _u32(qs), len(qs), slot, self.gc_mask))
+
+    def resets_then_x_if(self, qs, rdoubs, xqs, slot):
+        """Each reset of qs[k], then X on xqs[k] where outcome slot + k reads 1."""
+        assert len(qs) == len(rdoubs) == len(xqs)
+        rd = (C.c_double * max(1, len(rdoubs)))(*rdoubs)
+        check(lib().suqa_hip_resets_then_x_if(self.ctx, _u32(qs), rd, _u32(xqs), len(qs), slot, self.gc_mask))
 
     def qms_apply_W_deferred(self, q_acc, w_mask_enew, bm_enes0, c_factor, eold_slot):
         check(lib().suqa_hip_qms_apply_W_deferred(self.ctx, q_acc, w_mask_enew, bm_enes0, c_factor, eold_slot))

@@ -851,6 +851,31 @@ int suqa_hip_apply_x_if(suqa_hip_ctx* c, const uint32_t* qs, uint32_t n, uint32_
     return SUQA_OK;
 }
 
+int suqa_hip_resets_then_x_if(suqa_hip_ctx* c, const uint32_t* qs, const double* rdoubs, const uint32_t* xqs,
+                              uint32_t n, uint32_t slot, uint32_t gc) {
+    SQE_CTX(c);
+    if (n && (!qs || !rdoubs || !xqs)) return fail(SUQA_ERR_ARG, "null argument");
+    if (sqe::tiny_prog(c) || c->f32 || !c->d_slots || c->nq > sqe::kMeasSmallBits) {
+        for (uint32_t k = 0; k < n; ++k) {   // queued beside the gates, or refused as the calls it stands for
+            if (int rc = suqa_hip_apply_reset(c, qs[k], rdoubs[k], gc)) return rc;
+            if (int rc = suqa_hip_apply_x_if(c, xqs + k, 1, (slot + k) % sqe::kSlots, gc)) return rc;
+        }
+        return SUQA_OK;
+    }
+    if (slot >= sqe::kSlots || slot + n > sqe::kSlots) return fail(SUQA_ERR_ARG, "outcome slots run past the ring's end");
+    if ((uint64_t)gc >> c->nq) return fail(SUQA_ERR_ARG, "gc_mask has qubits beyond the state");
+    for (uint32_t k = 0; k < n; ++k) {
+        if (qs[k] >= c->nq || xqs[k] >= c->nq) return fail(SUQA_ERR_ARG, "qubit index out of range");
+        if ((gc >> xqs[k]) & 1u) return fail(SUQA_ERR_ARG, "a conditional X on a qubit of the gc_mask");
+    }
+    for (uint32_t k0 = 0; k0 < n; k0 += sqe::kMeasMax) {   // one launch: each reset, then its flip
+        const uint32_t m = std::min<uint32_t>(sqe::kMeasMax, n - k0);
+        SQE_HIP(sqe::launch_measure_small(c, qs + k0, rdoubs + k0, m, ~0u, gc, nullptr, xqs + k0,
+                                          c->d_slots + slot + k0));
+    }
+    return SUQA_OK;
+}
+
 int suqa_hip_prob_filter(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* q_mask, uint32_t n, double* prob) {
     SQE_CTX(c);
     if (n && (!qs || !q_mask)) return fail(SUQA_ERR_ARG, "null argument");

@@ -205,10 +205,12 @@ hipError_t launch_checksum(suqa_hip_ctx* c);
 // workgroup: for k < n in order, p0 and p1 of qubit qs[k], the outcome
 // c = (rds[k] < p1) into ctx->d_flags[k], the collapse scaled by 1/sqrt(p_c),
 // and when bit k of `resets` is set and c = 1, X on qs[k] over the indices
-// with the gc bits set.  n <= kMeasMax.
+// with the gc bits set; with xqs, then X on qubit xqs[k] over the same
+// indices where the device flag xflags[k] reads 1.  n <= kMeasMax.
 constexpr uint32_t kMeasSmallBits = 14, kMeasMax = 16;
 hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, uint32_t resets,
-                                uint32_t gc, uint32_t* out = nullptr);   // out: device outcomes (d_flags when null)
+                                uint32_t gc, uint32_t* out = nullptr,   // out: device outcomes (d_flags when null)
+                                const uint32_t* xqs = nullptr, const uint32_t* xflags = nullptr);
 // X on qubit qs[k] over the indices with the gc bits set where the device
 // outcome flags[k] is 1, k < n <= kMeasMax, in order (one launch).
 hipError_t launch_x_if(suqa_hip_ctx* c, const uint32_t* qs, const uint32_t* flags, uint32_t n, uint32_t gc);

@@ -471,6 +471,8 @@ struct MeasArgs {
     uint32_t n, resets, gc, pad_;
     uint32_t q[kMeasMax];
     double rd[kMeasMax];
+    uint32_t xq[kMeasMax];     // after op k, X on qubit xq[k] (over the gc bits) where xflags[k] reads 1;
+    const uint32_t* xflags;    // xq[k] >= 32: none (the flips of another register, interleaved)
 };
 
 template <int APT>
@@ -490,6 +492,41 @@ __global__ __launch_bounds__(1024) void k_measure_small(double* __restrict__ re,
         vr[j] = i < N ? re[i] : 0.0;
         vi[j] = i < N ? im[i] : 0.0;
     }
+    // X on the qubit of bit qb over the indices with the other gc bits set
+    // (uniform qb; every thread calls it: barriers inside)
+    auto flip = [&](uint32_t qb) {
+        const uint32_t g = a.gc & ~qb;   // both ends of a pair meet it or neither does
+        if (qb >= T) {   // the partner in this thread's register j ^ (qb / T)
+            const uint32_t h = qb >> tb;
+#pragma unroll
+            for (int hh = 1; hh < APT; hh <<= 1) {
+                if (h != (uint32_t)hh) continue;
+#pragma unroll
+                for (int j = 0; j < APT; ++j) {
+                    if (j & hh) continue;
+                    if (((y + T * (uint32_t)j) & g) != g) continue;
+                    const double tr = vr[j], ti = vi[j];
+                    vr[j] = vr[j | hh];
+                    vi[j] = vi[j | hh];
+                    vr[j | hh] = tr;
+                    vi[j | hh] = ti;
+                }
+            }
+        } else {         // the partner in thread y ^ qb, same register
+#pragma unroll
+            for (int j = 0; j < APT; ++j) {
+                xr[y] = vr[j];
+                xi[y] = vi[j];
+                __syncthreads();
+                const double pr = xr[y ^ qb], pi = xi[y ^ qb];
+                __syncthreads();
+                if (((y + T * (uint32_t)j) & g) == g) {
+                    vr[j] = pr;
+                    vi[j] = pi;
+                }
+            }
+        }
+    };
     uint32_t rb = 0;
     for (uint32_t k = 0; k < a.n; ++k) {
         const uint32_t q = a.q[k];
@@ -541,38 +578,11 @@ __global__ __launch_bounds__(1024) void k_measure_small(double* __restrict__ re,
             vi[j] = kp ? vi[j] * scale : 0.0;
         }
         if (y == 0) out[k] = cbit;
-        if (((a.resets >> k) & 1u) && cbit) {   // apply_reset: X on q over the other gc bits (suqa.cu:735-737)
-            const uint32_t g = a.gc & ~qb;   // both ends of a pair meet it or neither does
-            if (qb >= T) {   // the partner in this thread's register j ^ (qb / T)
-                const uint32_t h = qb >> tb;
-#pragma unroll
-                for (int hh = 1; hh < APT; hh <<= 1) {
-                    if (h != (uint32_t)hh) continue;
-#pragma unroll
-                    for (int j = 0; j < APT; ++j) {
-                        if (j & hh) continue;
-                        if (((y + T * (uint32_t)j) & g) != g) continue;
-                        const double tr = vr[j], ti = vi[j];
-                        vr[j] = vr[j | hh];
-                        vi[j] = vi[j | hh];
-                        vr[j | hh] = tr;
-                        vi[j | hh] = ti;
-                    }
-                }
-            } else {         // the partner in thread y ^ qb, same register
-#pragma unroll
-                for (int j = 0; j < APT; ++j) {
-                    xr[y] = vr[j];
-                    xi[y] = vi[j];
-                    __syncthreads();
-                    const double pr = xr[y ^ qb], pi = xi[y ^ qb];
-                    __syncthreads();
-                    if (((y + T * (uint32_t)j) & g) == g) {
-                        vr[j] = pr;
-                        vi[j] = pi;
-                    }
-                }
-            }
+        if (((a.resets >> k) & 1u) && cbit) flip(qb);   // apply_reset: X on q over the other gc bits (suqa.cu:735-737)
+        if (a.xq[k] < 32u) {   // then the flip of another register's qubit by its outcome
+            const uint32_t f = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(a.xflags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (f) flip(1u << a.xq[k]);
         }
     }
 #pragma unroll
@@ -936,7 +946,7 @@ hipError_t launch_collapse(suqa_hip_ctx* c, uint64_t qbit, uint32_t keep, double
 }
 
 hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const double* rds, uint32_t n, uint32_t resets,
-                                uint32_t gc, uint32_t* out) {
+                                uint32_t gc, uint32_t* out, const uint32_t* xqs, const uint32_t* xflags) {
     if (!out) out = c->d_flags;
     if (c->f32) return hipErrorInvalidValue;   // a gate queue over a float state: gates only
     if (hipError_t e = flush(c)) return e;
@@ -946,10 +956,12 @@ hipError_t launch_measure_small(suqa_hip_ctx* c, const uint32_t* qs, const doubl
     a.n = n;
     a.resets = resets;
     a.gc = gc;
+    a.xflags = xflags;
     for (uint32_t k = 0; k < n; ++k) {
-        if (qs[k] >= c->nq) return hipErrorInvalidValue;
+        if (qs[k] >= c->nq || (xqs && xqs[k] >= c->nq)) return hipErrorInvalidValue;
         a.q[k] = qs[k];
         a.rd[k] = rds[k];
+        a.xq[k] = xqs ? xqs[k] : ~0u;
     }
     timing_begin(c, "measure_small", 40.0 * (double)c->N * n);
     // the whole state in registers: 256 threads up to 256 amplitudes, then up to 1024 threads

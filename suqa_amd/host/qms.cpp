@@ -340,7 +340,12 @@ int metro_step_deferred(bool take_measure) {
         if (take_measure) {
             D.recs.push_back(Deferred::Rec{false, s_enew, E_measures.size()});
             E_measures.push_back(0.0);
-            if (two_registers) {   // each E_old qubit reset with its draw before its flip (qms.cuh:431-434)
+            if (two_registers && suqa::gc_mask == 0u) {   // each E_old qubit reset with its draw before its flip
+                const std::vector<double> rds = extract_rands(ene_qbits);   // (qms.cuh:431-434), in one call
+                chk(suqa_hip_resets_then_x_if(suqa::ctx, bm_enes_old.data(), rds.data(), bm_enes_new.data(), ene_qbits,
+                                              s_enew, 0u),
+                    "suqa::apply_reset()");
+            } else if (two_registers) {
                 for (uint ei = 0U; ei < ene_qbits; ++ei) {
                     suqa::apply_reset(bm_enes_old[ei], rangen.doub());
                     chk(suqa_hip_apply_x_if(suqa::ctx, &bm_enes_new[ei], 1, s_enew + ei, 0u), "suqa::apply_x()");

@@ -260,3 +260,34 @@ def test_tiny_states_and_calls_longer_than_a_program(capi, n):
         same_state(prog, plain, "resets")
     prog.close()
     plain.close()
+
+
+@pytest.mark.parametrize("n,gc", [(8, 0), (12, 0), (13, 0), (12, 1 << 11)])
+def test_resets_then_x_if_match_the_calls_it_stands_for(capi, n, gc):
+    """One call for a register's resets interleaved with the flips of another
+    register's measured qubits: the state of the separate calls, bit for bit."""
+    log(f"resets_then_x_if n={n} gc={gc}")
+    rng = np.random.default_rng(60 + n)
+    r = 2 if n < 12 else 4
+    old = [3 + r + k for k in range(r)]   # a two-register layout: E_new at 3.., E_old after it
+    new = [3 + k for k in range(r)]
+    for rep in range(3):
+        loop = prepared(capi, n, 20 + rep)
+        merged = prepared(capi, n, 20 + rep)
+        rd_m = [float(x) for x in rng.uniform(0, 1, r)]
+        rd_r = [float(x) for x in rng.uniform(0, 1, r)]
+        s_loop = loop.measure_qbits_deferred(new, rd_m)
+        s_merged = merged.measure_qbits_deferred(new, rd_m)
+        loop.gc_mask = gc
+        merged.gc_mask = gc
+        for k in range(r):
+            loop.reset(old[k], rd_r[k])
+            loop.x_if([new[k]], s_loop + k)
+        merged.resets_then_x_if(old, rd_r, new, s_merged)
+        loop.gc_mask = 0
+        merged.gc_mask = 0
+        loop.h(0)
+        merged.h(0)
+        same_state(loop, merged, "resets then x_if")
+        loop.close()
+        merged.close()
