@@ -512,6 +512,26 @@ __global__ __launch_bounds__(1024) void k_measure_small(double* __restrict__ re,
                     vi[j | hh] = ti;
                 }
             }
+        } else if constexpr (APT <= 4) {   // the partner in thread y ^ qb, same register: every
+            double pp[APT];                  // register of one component through pbuf at a time
+#pragma unroll
+            for (int j = 0; j < APT; ++j) pbuf[y + T * (uint32_t)j] = vr[j];
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < APT; ++j) pp[j] = pbuf[(y ^ qb) + T * (uint32_t)j];
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < APT; ++j)
+                if (((y + T * (uint32_t)j) & g) == g) vr[j] = pp[j];
+#pragma unroll
+            for (int j = 0; j < APT; ++j) pbuf[y + T * (uint32_t)j] = vi[j];
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < APT; ++j) pp[j] = pbuf[(y ^ qb) + T * (uint32_t)j];
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < APT; ++j)
+                if (((y + T * (uint32_t)j) & g) == g) vi[j] = pp[j];
         } else {         // the partner in thread y ^ qb, same register
 #pragma unroll
             for (int j = 0; j < APT; ++j) {
